@@ -1,0 +1,193 @@
+/*
+ * tsgpu -- C ABI of the MI355X tree-sequence statistics library
+ * (tskit_amd/libtsgpu.so).
+ *
+ * The statistics entry points take the reference's arguments in the
+ * reference's order (c/tskit/trees.h:1098-1239) with the
+ * `const tsk_treeseq_t *` replaced by an opaque, device-resident
+ * `tsg_treeseq_t *`.  Types: sizes are uint64_t (tsk_size_t), ids int32_t
+ * (tsk_id_t), options uint32_t (tsk_flags_t); option bits and error numbers
+ * have the reference's values (c/tskit/trees.h:47-56, c/tskit/core.h).
+ * Results are written to caller-owned host buffers, C order, W x M doubles
+ * (AFS: W x TW x prod(n_k + 1)).  Calls on one handle are serialised by the
+ * library; the handle is never mutated by a statistics call.
+ */
+#ifndef TSGPU_H
+#define TSGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tsg_treeseq tsg_treeseq_t;
+typedef struct tsg_tables tsg_tables_t;
+
+/* option bits: c/tskit/trees.h:47-56 */
+#define TSG_STAT_SITE (1u << 0)
+#define TSG_STAT_BRANCH (1u << 1)
+#define TSG_STAT_NODE (1u << 2)
+#define TSG_STAT_POLARISED (1u << 10)
+#define TSG_STAT_SPAN_NORMALISE (1u << 11)
+#define TSG_STAT_ALLOW_TIME_UNCALIBRATED (1u << 12)
+#define TSG_STAT_NONCENTRED (1u << 14)
+/* tree-sequence option: time_units == "uncalibrated" (trees.c:537-543) */
+#define TSG_TS_TIME_UNCALIBRATED (1u << 0)
+
+/* Error message for an error number (replaces tsk_strerror, core.c). */
+const char *tsg_strerror(int err);
+
+/* ---- tree sequence lifecycle ------------------------------------------- */
+
+/* Validates the tables, builds the edge indexes if I/O are NULL, finds the
+ * trees and per-tree sites and builds the chunk plan (host side).  The
+ * device copy is made on first use.  Replaces tsk_treeseq_init
+ * (trees.c:455-545) with TSK_TS_INIT_BUILD_INDEXES and
+ * tsk_table_collection_build_index (tables.c:11393-11461).  String columns
+ * are (data, offsets) as in the reference tables; NULL states mean "0"
+ * ancestral / "1" derived. */
+int tsg_treeseq_init(tsg_treeseq_t **ts, double sequence_length, uint64_t num_nodes,
+    const uint32_t *node_flags, const double *node_time, uint64_t num_edges,
+    const double *edge_left, const double *edge_right, const int32_t *edge_parent,
+    const int32_t *edge_child, uint64_t num_sites, const double *site_position,
+    const char *ancestral_state, const uint64_t *ancestral_state_offset,
+    uint64_t num_mutations, const int32_t *mutation_site, const int32_t *mutation_node,
+    const int32_t *mutation_parent, const char *derived_state,
+    const uint64_t *derived_state_offset, const int32_t *edge_insertion_order,
+    const int32_t *edge_removal_order, uint32_t options);
+/* Replaces tsk_treeseq_free (trees.c:107). */
+int tsg_treeseq_free(tsg_treeseq_t *ts);
+/* Copies the tree sequence to the current GPU now (otherwise on first use). */
+int tsg_treeseq_upload(tsg_treeseq_t *ts);
+/* Rebuilds the chunk plan with num_chunks chunks (autotuner hook). */
+int tsg_treeseq_set_num_chunks(tsg_treeseq_t *ts, uint32_t num_chunks);
+
+/* accessors: tsk_treeseq_get_num_trees / _samples / ... (trees.h:560-700) */
+uint64_t tsg_treeseq_get_num_trees(const tsg_treeseq_t *ts);
+uint64_t tsg_treeseq_get_num_samples(const tsg_treeseq_t *ts);
+uint64_t tsg_treeseq_get_num_nodes(const tsg_treeseq_t *ts);
+uint64_t tsg_treeseq_get_num_edges(const tsg_treeseq_t *ts);
+uint64_t tsg_treeseq_get_num_sites(const tsg_treeseq_t *ts);
+uint64_t tsg_treeseq_get_num_mutations(const tsg_treeseq_t *ts);
+double tsg_treeseq_get_sequence_length(const tsg_treeseq_t *ts);
+/* tsk_treeseq_get_breakpoints (trees.h:692): num_trees + 1 values */
+int tsg_treeseq_get_breakpoints(const tsg_treeseq_t *ts, double *breakpoints);
+/* tsk_treeseq_get_samples (trees.h:706) */
+int tsg_treeseq_get_samples(const tsg_treeseq_t *ts, int32_t *samples);
+/* indexes.edge_insertion_order / edge_removal_order (tables.h:560-601) */
+int tsg_treeseq_get_index(const tsg_treeseq_t *ts, int32_t *insertion, int32_t *removal);
+/* device-side facts: chunk count, device bytes held, largest chunk */
+uint64_t tsg_treeseq_get_num_chunks(const tsg_treeseq_t *ts);
+uint64_t tsg_treeseq_get_device_bytes(const tsg_treeseq_t *ts);
+uint64_t tsg_treeseq_get_max_chunk_nodes(const tsg_treeseq_t *ts);
+/* host copy of the chunk plan; sizes = {chunks, local nodes, events, trees,
+ * sites, mutations} */
+int tsg_treeseq_get_plan_sizes(const tsg_treeseq_t *ts, uint64_t *sizes);
+int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t *chunk_node0,
+    int32_t *loc_parent0, float *loc_bl0, double *loc_time, double *loc_lastrem,
+    int64_t *tr_off, int32_t *tr_nrem, int32_t *ev_child, int32_t *ev_parent, float *ev_bl,
+    int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc, int32_t *mut_allele,
+    int32_t *mut_pallele, int32_t *site_nalleles);
+/* HIP-event timing of the sweep kernel on the library's stream */
+int tsg_treeseq_set_profiling(tsg_treeseq_t *ts, int enable);
+int tsg_treeseq_get_profile(const tsg_treeseq_t *ts, double *sweep_ms, uint64_t *launches);
+
+/* ---- one-way statistics ------------------------------------------------ */
+/* tsk_treeseq_diversity (trees.h:1103, trees.c:3950-3958) */
+int tsg_treeseq_diversity(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+/* tsk_treeseq_segregating_sites (trees.h:1106, trees.c:4238-4246) */
+int tsg_treeseq_segregating_sites(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+/* tsk_treeseq_Y1 (trees.h:1109, trees.c:4266-4274) */
+int tsg_treeseq_Y1(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+/* Tajima's D, the Python composite of trees.py:9967-10022 (diversity and
+ * segregating sites, not span normalised, one sweep, FP64 final). */
+int tsg_treeseq_Tajimas_D(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+
+/* ---- k-way statistics -------------------------------------------------- */
+/* tsk_treeseq_divergence (trees.h:1189, trees.c:4711-4727) */
+int tsg_treeseq_divergence(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+/* tsk_treeseq_Y2 (trees.h:1193, trees.c:4920-4936) */
+int tsg_treeseq_Y2(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+/* tsk_treeseq_f2 (trees.h:1197, trees.c:4961-4977) */
+int tsg_treeseq_f2(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+/* tsk_treeseq_Y3 (trees.h:1226, trees.c:5201-5217) */
+int tsg_treeseq_Y3(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+/* tsk_treeseq_f3 (trees.h:1230, trees.c:5244-5260) */
+int tsg_treeseq_f3(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+/* tsk_treeseq_f4 (trees.h:1236, trees.c:5293-5309) */
+int tsg_treeseq_f4(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+/* tsk_treeseq_genetic_relatedness (trees.h:1201, trees.c:4775-4797);
+ * TSG_STAT_NONCENTRED selects the non-centred summary. */
+int tsg_treeseq_genetic_relatedness(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+/* genetic_relatedness(proportion=True), the Python composite of
+ * trees.py:8830-8976: relatedness divided by the unpolarised segregating
+ * sites of the union of the sample sets, one sweep, FP64 final. */
+int tsg_treeseq_genetic_relatedness_proportion(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+/* Fst, the Python composite of trees.py:10024-10136: diversity of every set
+ * and divergence of each pair in one sweep, FP64 final. */
+int tsg_treeseq_Fst(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+
+/* ---- allele frequency spectrum ----------------------------------------- */
+/* tsk_treeseq_allele_frequency_spectrum (trees.h:1112, trees.c:3815-3928);
+ * at most 16 sample sets. */
+int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint64_t num_time_windows, const double *time_windows,
+    uint32_t options, double *result);
+
+/* ---- synthetic inputs (no counterpart in the reference; msprime's role
+ *      in its tests and benchmarks) --------------------------------------- */
+/* Hudson coalescent with recombination, discrete genome, haploid samples. */
+int tsg_sim_ancestry(uint32_t num_samples, double sequence_length, double population_size,
+    double recombination_rate, uint64_t seed, tsg_tables_t **tables);
+/* Infinite-sites 0/1 mutations at rate mu per unit genome per generation. */
+int tsg_sim_mutations(tsg_tables_t *tables, double rate, uint64_t seed);
+/* sizes = {nodes, edges, sites, mutations} */
+int tsg_tables_get_sizes(const tsg_tables_t *tables, uint64_t *sizes);
+int tsg_tables_copy_out(const tsg_tables_t *tables, uint32_t *node_flags, double *node_time,
+    double *edge_left, double *edge_right, int32_t *edge_parent, int32_t *edge_child,
+    double *site_position, int32_t *mutation_site, int32_t *mutation_node,
+    int32_t *mutation_parent, double *mutation_time);
+void tsg_tables_free(tsg_tables_t *tables);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSGPU_H */

@@ -1,0 +1,152 @@
+"""Shared inputs for the tests: generated tree sequences and the reference's
+hand-drawn fixtures (text format of c/tests/testlib.c:25-140)."""
+import functools
+
+import numpy as np
+
+import tskit_amd as ta
+
+# c/tests/testlib.c:55-96 ("paper_ex", the tree sequence drawn in the PLOS
+# paper): 4 samples, 3 trees on [0, 10), one mutation per tree.
+PAPER_EX_NODES = """
+1  0       -1   0
+1  0       -1   0
+1  0       -1   1
+1  0       -1   1
+0  0.071   -1   -1
+0  0.090   -1   -1
+0  0.170   -1   -1
+0  0.202   -1   -1
+0  0.253   -1   -1
+"""
+PAPER_EX_EDGES = """
+2 10 4 2
+2 10 4 3
+0 10 5 1
+0 2  5 3
+2 10 5 4
+0 7  6 0,5
+7 10 7 0,5
+0 2  8 2,6
+"""
+PAPER_EX_SITES = """
+1      0
+4.5    0
+8.5    0
+"""
+PAPER_EX_MUTATIONS = """
+0      2   1
+1      0   1
+2      5   1
+"""
+
+# c/tests/testlib.c:31-54 ("single_tree_ex"): one tree, back and recurrent
+# mutations with explicit parents.
+SINGLE_TREE_EX_NODES = """
+1  0   -1   -1
+1  0   -1   -1
+1  0   -1   -1
+1  0   -1   -1
+0  1   -1   -1
+0  2   -1   -1
+0  3   -1   -1
+"""
+SINGLE_TREE_EX_EDGES = """
+0  1   4   0,1
+0  1   5   2,3
+0  1   6   4,5
+"""
+SINGLE_TREE_EX_SITES = """
+0.125  0
+0.25   0
+0.5    0
+"""
+SINGLE_TREE_EX_MUTATIONS = """
+0    2     1   -1
+1    4     1   -1
+1    0     0   1
+2    0     1   -1
+2    1     1   -1
+2    2     1   -1
+2    3     1   -1
+"""
+
+# c/tests/testlib.c:98-140 ("nonbinary_ex"): polytomies and a unary span.
+NONBINARY_EX_NODES = """
+1  0       0   -1
+1  0       0   -1
+1  0       0   -1
+1  0       0   -1
+1  0       0   -1
+1  0       0   -1
+1  0       0   -1
+1  0       0   -1
+0  0.01    0   -1
+0  0.068   0   -1
+0  0.130   0   -1
+0  0.279   0   -1
+0  0.405   0   -1
+"""
+NONBINARY_EX_EDGES = """
+0	100	8	0,1,2,3
+0	100	9	6,8
+0  100 10  4
+0  17  10  5
+0  100 10  7
+17	100	11	5,9
+0	17	12	9
+0  100 12  10
+17	100	12	11
+"""
+NONBINARY_EX_SITES = """
+1  0
+18 0
+"""
+NONBINARY_EX_MUTATIONS = """
+0    2   1
+1    11  1
+"""
+
+
+def paper_ex():
+    return ta.parse_text(10, PAPER_EX_NODES, PAPER_EX_EDGES, PAPER_EX_SITES, PAPER_EX_MUTATIONS)
+
+
+def single_tree_ex():
+    return ta.parse_text(1, SINGLE_TREE_EX_NODES, SINGLE_TREE_EX_EDGES, SINGLE_TREE_EX_SITES,
+                         SINGLE_TREE_EX_MUTATIONS)
+
+
+def nonbinary_ex():
+    return ta.parse_text(100, NONBINARY_EX_NODES, NONBINARY_EX_EDGES, NONBINARY_EX_SITES,
+                         NONBINARY_EX_MUTATIONS)
+
+
+@functools.lru_cache(maxsize=None)
+def generated(n, L, seed, mu=1e-8):
+    """Hudson coalescent (Ne=1e4, r=1e-8) with infinite-sites mutations."""
+    return ta.simulate_tables(n, L, 1e4, 1e-8, mu, seed)
+
+
+def config1():
+    """The small workload: 1000 samples, L=1e6, seed 1."""
+    return generated(1000, 1e6, 1)
+
+
+def sample_sets(n, k, size, seed):
+    return ta.random_sample_sets(n, k, size, seed)
+
+
+def rel_err(got, exp, scale=None):
+    got = np.asarray(got, dtype=np.float64)
+    exp = np.asarray(exp, dtype=np.float64)
+    assert got.shape == exp.shape, (got.shape, exp.shape)
+    nan_g, nan_e = np.isnan(got), np.isnan(exp)
+    assert np.array_equal(nan_g, nan_e), "NaN pattern differs"
+    if scale is None:
+        scale = np.abs(exp)
+    scale = np.maximum(np.broadcast_to(scale, exp.shape), 1e-300)
+    fin = ~nan_e
+    if not np.any(fin):
+        return 0.0
+    return float(np.max(np.abs(got[fin] - exp[fin]) / scale[fin]))

@@ -1,0 +1,266 @@
+"""GPU parity: every statistic computed by the HIP kernels (through the C ABI)
+against the FP64 oracle on the same inputs.
+
+Tolerances (stated per statistic, the bar of the hot path):
+  * non-negative statistics (diversity, segregating sites, divergence, Y1,
+    Y2, Y3, non-centred relatedness, branch AFS): elementwise relative error
+    <= 1e-6;
+  * statistics that change sign (f2, f3, f4, centred relatedness): error
+    <= 1e-6 of the largest |value| of that output over the windows;
+  * FP64 composites (Fst, Tajima's D, relatedness proportion): error
+    <= 1e-6 * max(1, |value|);
+  * site-mode AFS: bit-exact.
+"""
+import numpy as np
+import pytest
+
+import cases
+from cases import rel_err
+from oracle.oracle import Oracle, fst, tajimas_d
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+POSITIVE = {"diversity", "segregating_sites", "Y1", "divergence", "Y2", "Y3"}
+
+
+def _say(*args):
+    print("[case]", *args, flush=True)
+
+
+def _check(name, got, exp):
+    if name in POSITIVE or name == "genetic_relatedness_nc":
+        err = rel_err(got, exp)
+    else:
+        scale = np.max(np.abs(exp), axis=tuple(range(exp.ndim - 1)), keepdims=True)
+        err = rel_err(got, exp, scale=np.maximum(scale, 1e-300))
+    assert err <= TOL, f"{name}: relative error {err:.3e}"
+    return err
+
+
+@pytest.fixture(scope="module")
+def small(require_gpu):
+    tc = cases.generated(200, 2e5, 7)
+    return tc, tc.tree_sequence(), Oracle(tc)
+
+
+@pytest.fixture(scope="module")
+def cfg1(require_gpu):
+    tc = cases.config1()
+    return tc, tc.tree_sequence(), Oracle(tc)
+
+
+ONE_WAY = ["diversity", "segregating_sites", "Y1"]
+K_WAY = [
+    ("divergence", [(0, 1), (0, 0), (2, 3), (3, 1)]),
+    ("Y2", [(0, 1), (2, 3)]),
+    ("f2", [(0, 1), (1, 2)]),
+    ("Y3", [(0, 1, 2), (3, 2, 1)]),
+    ("f3", [(0, 1, 2), (2, 0, 3)]),
+    ("f4", [(0, 1, 2, 3), (3, 2, 0, 1)]),
+]
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
+@pytest.mark.parametrize("windows", ["none", "ten", "trees", "sites"])
+def test_one_way(small, mode, windows):
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, 4, 40, 1)
+    w = {"none": None, "ten": np.linspace(0, tc.sequence_length, 11),
+         "trees": ts.breakpoints(as_array=True), "sites": ts.parse_windows("sites")}[windows]
+    for name in ONE_WAY:
+        _say(name, mode, windows)
+        got = getattr(ts, name)(sets, windows=w, mode=mode)
+        exp = orc.stat(name, sets, None, ts.parse_windows(w), mode)
+        if w is None:
+            exp = exp[0]
+        _check(name, got, exp)
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
+@pytest.mark.parametrize("span_normalise", [True, False])
+def test_k_way(small, mode, span_normalise):
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, 4, 30, 2)
+    w = np.array([0, 1.5e4, 6e4, 6.00001e4, 1.3e5, 2e5])
+    for name, idx in K_WAY:
+        _say(name, mode, span_normalise)
+        got = getattr(ts, name)(sets, indexes=idx, windows=w, mode=mode,
+                                span_normalise=span_normalise)
+        exp = orc.stat(name, sets, idx, w, mode, span_normalise)
+        _check(name, got, exp)
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
+def test_relatedness(small, mode):
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, 5, 12, 3)
+    idx = [(0, 1), (1, 1), (2, 4), (3, 0)]
+    w = np.linspace(0, tc.sequence_length, 4)
+    for polarised in [True, False]:
+        for centre in [True, False]:
+            _say("relatedness", mode, polarised, centre)
+            got = ts.genetic_relatedness(sets, indexes=idx, windows=w, mode=mode,
+                                         polarised=polarised, centre=centre, proportion=False)
+            exp = orc.stat("genetic_relatedness", sets, idx, w, mode, True, polarised, centre)
+            _check("genetic_relatedness" if centre else "genetic_relatedness_nc", got, exp)
+            # proportion: divide by segregating sites of the union (unpolarised)
+            got = ts.genetic_relatedness(sets, indexes=idx, windows=w, mode=mode,
+                                         polarised=polarised, centre=centre, proportion=True)
+            union = np.array(sorted({int(u) for s in sets for u in s}), dtype=np.int32)
+            seg = orc.stat("segregating_sites", [union], None, w, mode, True, False)
+            exp2 = exp / seg
+            err = np.max(np.abs(got - exp2) / np.maximum(1.0, np.abs(exp2)))
+            assert err <= TOL
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
+def test_composites(small, mode):
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, 3, 50, 4)
+    idx = [(0, 1), (1, 2), (0, 2)]
+    w = np.linspace(0, tc.sequence_length, 8)
+    _say("Fst", mode)
+    got = ts.Fst(sets, indexes=idx, windows=w, mode=mode)
+    div = orc.stat("diversity", sets, None, w, mode)
+    dvg = orc.stat("divergence", sets, idx, w, mode)
+    exp = fst(div, dvg, idx)
+    assert np.max(np.abs(got - exp) / np.maximum(1.0, np.abs(exp))) <= TOL
+    _say("Tajimas_D", mode)
+    got = ts.Tajimas_D(sets, windows=w, mode=mode)
+    T = orc.stat("diversity", sets, None, w, mode, False)
+    S = orc.stat("segregating_sites", sets, None, w, mode, False)
+    exp = tajimas_d(T, S, [len(s) for s in sets])
+    assert np.array_equal(np.isnan(got), np.isnan(exp))
+    fin = ~np.isnan(exp)
+    assert np.max(np.abs(got[fin] - exp[fin]) / np.maximum(1.0, np.abs(exp[fin]))) <= TOL
+
+
+def test_config1_headline(cfg1):
+    """The small workload: diversity, divergence and Fst on 2 disjoint sets of
+    500 over 10 windows, both modes."""
+    tc, ts, orc = cfg1
+    sets = cases.sample_sets(1000, 2, 500, 1)
+    w = np.linspace(0, tc.sequence_length, 11)
+    for mode in ["branch", "site"]:
+        _say("config1", mode)
+        _check("diversity", ts.diversity(sets, windows=w, mode=mode),
+               orc.stat("diversity", sets, None, w, mode))
+        _check("divergence", ts.divergence(sets, windows=w, mode=mode),
+               orc.stat("divergence", sets, [(0, 1)], w, mode)[:, 0])
+        div = orc.stat("diversity", sets, None, w, mode)
+        dvg = orc.stat("divergence", sets, [(0, 1)], w, mode)
+        exp = fst(div, dvg, [(0, 1)])[:, 0]
+        got = ts.Fst(sets, windows=w, mode=mode)
+        assert np.max(np.abs(got - exp) / np.maximum(1.0, np.abs(exp))) <= TOL
+
+
+@pytest.mark.parametrize("num_chunks", [1, 2, 37, 100000])
+def test_chunk_counts(require_gpu, num_chunks):
+    """Results do not depend on how the genome is cut into chunks."""
+    tc = cases.generated(120, 1e5, 11)
+    ts = tc.tree_sequence()
+    ts.ll_tree_sequence.set_num_chunks(num_chunks)
+    orc = Oracle(tc)
+    sets = cases.sample_sets(120, 3, 30, 5)
+    w = np.linspace(0, tc.sequence_length, 7)
+    for mode in ["branch", "site"]:
+        _say("chunks", num_chunks, mode)
+        _check("diversity", ts.diversity(sets, windows=w, mode=mode),
+               orc.stat("diversity", sets, None, w, mode))
+        _check("f4", ts.f4(sets + [sets[0]], windows=w, mode=mode),
+               orc.stat("f4", sets + [sets[0]], [(0, 1, 2, 3)], w, mode)[:, 0])
+
+
+def test_global_memory_path(require_gpu):
+    """Chunks too large for LDS run from per-chunk HBM slabs."""
+    tc = cases.generated(3000, 1e6, 13)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    sets = cases.sample_sets(3000, 8, 100, 6)
+    w = np.linspace(0, tc.sequence_length, 5)
+    idx = [(i, j) for i in range(8) for j in range(i + 1, 8)]
+    for mode in ["branch", "site"]:
+        _say("global", mode)
+        _check("divergence", ts.divergence(sets, indexes=idx, windows=w, mode=mode),
+               orc.stat("divergence", sets, idx, w, mode))
+
+
+@pytest.mark.parametrize("fixture", ["paper_ex", "single_tree_ex", "nonbinary_ex"])
+def test_reference_fixtures(require_gpu, fixture):
+    tc = getattr(cases, fixture)()
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    n = ts.num_samples
+    samples = list(range(n))
+    sets = [samples[: n // 2], samples[n // 2:]]
+    for mode in ["branch", "site"]:
+        for pol in [False, True]:
+            _say(fixture, mode, pol)
+            got = ts.ll_tree_sequence.diversity(
+                [len(s) for s in sets], np.concatenate(sets), [0, tc.sequence_length],
+                mode=mode, span_normalise=False, polarised=pol)
+            exp = orc.stat("diversity", sets, None, None, mode, False, pol)
+            _check("diversity", got, exp)
+            got = ts.ll_tree_sequence.divergence(
+                [len(s) for s in sets], np.concatenate(sets), [(0, 1)], [0, tc.sequence_length],
+                mode=mode, span_normalise=False, polarised=pol)
+            exp = orc.stat("divergence", sets, [(0, 1)], None, mode, False, pol)
+            _check("divergence", got, exp)
+
+
+def test_paper_ex_known_answers(require_gpu):
+    """c/tests/test_stats.c:1718-1742, 1872-1897, 2518-2547."""
+    ts = cases.paper_ex().tree_sequence()
+    ll = ts.ll_tree_sequence
+    pi = ll.diversity([4], [0, 1, 2, 3], [0, 10], mode="site", span_normalise=False)
+    assert abs(pi[0, 0] - 1.5) < 1e-6
+    pi1 = ll.diversity([1], [0], [0, 10], mode="site", span_normalise=False)
+    assert np.isnan(pi1[0, 0])
+    seg = ll.segregating_sites([4], [0, 1, 2, 3], [0, 10], mode="site", span_normalise=False)
+    assert abs(seg[0, 0] - 3.0) < 1e-6
+    seg1 = ll.segregating_sites([1], [0], [0, 10], mode="site", span_normalise=False)
+    assert abs(seg1[0, 0]) < 1e-6
+    afs = ll.allele_frequency_spectrum([4], [0, 1, 2, 3], [0, 10], [0, np.inf], mode="site",
+                                       span_normalise=False)
+    assert list(afs[0, 0]) == [0, 3, 0, 0, 0]
+    afs = ll.allele_frequency_spectrum([4], [0, 1, 2, 3], [0, 10], [0, np.inf], mode="site",
+                                       span_normalise=False, polarised=True)
+    assert list(afs[0, 0]) == [0, 2, 0, 1, 0]
+
+
+@pytest.mark.parametrize("mode", ["site", "branch"])
+@pytest.mark.parametrize("nsets", [1, 2, 3])
+@pytest.mark.parametrize("polarised", [False, True])
+def test_afs(small, mode, nsets, polarised):
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, nsets, 13, 8)
+    w = np.linspace(0, tc.sequence_length, 5)
+    _say("afs", mode, nsets, polarised)
+    got = ts.allele_frequency_spectrum(sets, windows=w, mode=mode, polarised=polarised)
+    exp = orc.afs(sets, w, None, mode, True, polarised)[:, 0]
+    if mode == "site":
+        assert np.array_equal(got, exp)
+    else:
+        scale = np.max(np.abs(exp), axis=tuple(range(1, exp.ndim)), keepdims=True)
+        assert rel_err(got, exp, np.maximum(scale, 1e-300)) <= TOL
+
+
+def test_afs_time_windows(small):
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, 2, 20, 9)
+    w = np.linspace(0, tc.sequence_length, 3)
+    tw = np.array([0, 2000, 10000, np.inf])
+    got = ts.allele_frequency_spectrum(sets, windows=w, time_windows=tw, mode="branch")
+    exp = orc.afs(sets, w, tw, "branch", True, False)
+    scale = np.max(np.abs(exp))
+    assert rel_err(got, exp, scale) <= TOL
+
+
+def test_repeatable(small):
+    """Two launches give bit-identical results (deterministic combine)."""
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, 3, 30, 10)
+    a = ts.diversity(sets, windows="trees", mode="branch")
+    b = ts.diversity(sets, windows="trees", mode="branch")
+    assert np.array_equal(a, b)

@@ -1,0 +1,228 @@
+"""CPU tests of the host side of the product: the C ABI library loads and
+exports every symbol include/tsgpu.h declares, tree-sequence construction
+(validation, indexes, breakpoints) agrees with the oracle, errors carry the
+reference's numbers, and the chunk plan is in-bounds and executes to the
+oracle's answer (plan_emulator) -- all without calling a GPU kernel."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cases
+import plan_emulator
+import tskit_amd as ta
+from oracle.oracle import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "tsgpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tsg_[a-zA-Z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_header():
+    from tskit_amd import _lib
+
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) > 30
+    for name in names:
+        assert hasattr(lib, name), name
+    # and the binding declares a signature for each of them
+    assert set(names) <= set(_lib.SIGNATURES)
+
+
+def test_generator_deterministic():
+    a = ta.simulate_tables(300, 3e5, 1e4, 1e-8, 1e-8, 42)
+    b = ta.simulate_tables(300, 3e5, 1e4, 1e-8, 1e-8, 42)
+    c = ta.simulate_tables(300, 3e5, 1e4, 1e-8, 1e-8, 43)
+    assert a.column_digest() == b.column_digest()
+    assert a.column_digest() != c.column_digest()
+
+
+def test_config1_shape():
+    tc = cases.config1()
+    assert tc.num_nodes == 3895 and tc.num_edges == 11726 and tc.num_sites == 2898
+    ts = tc.tree_sequence()
+    assert ts.num_trees == 2625
+    assert ts.num_samples == 1000
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_host_init_matches_oracle(seed):
+    tc = cases.generated(150, 2e5, seed)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    assert ts.num_trees == orc.num_trees
+    assert np.array_equal(ts.breakpoints(as_array=True), orc.breakpoints())
+    I, O = ts.ll_tree_sequence.get_index()
+    Io, Oo = orc.index()
+    assert np.array_equal(I, Io) and np.array_equal(O, Oo)
+    assert np.array_equal(ts.samples(), np.arange(150))
+
+
+@pytest.mark.parametrize("fixture", ["paper_ex", "single_tree_ex", "nonbinary_ex"])
+def test_fixture_trees(fixture):
+    tc = getattr(cases, fixture)()
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    assert np.array_equal(ts.breakpoints(as_array=True), orc.breakpoints())
+
+
+def test_paper_ex_breakpoints():
+    ts = cases.paper_ex().tree_sequence()
+    assert list(ts.breakpoints(as_array=True)) == [0, 2, 7, 10]
+
+
+# ---- table validation errors (same numbers as c/tskit/core.h) ---------------
+def _bad(mutate):
+    tc = cases.paper_ex()
+    mutate(tc)
+    with pytest.raises(ta.LibraryError) as e:
+        tc.tree_sequence()
+    return e.value.err
+
+
+def test_error_bad_edge_interval():
+    def m(tc):
+        tc.edge_right[0] = tc.edge_left[0]
+    assert _bad(m) == -307
+
+
+def test_error_node_out_of_bounds():
+    def m(tc):
+        tc.edge_child[0] = 100
+    assert _bad(m) == -202
+
+
+def test_error_time_ordering():
+    def m(tc):
+        tc.node_time[4] = 1.0
+    assert _bad(m) in (-306, -302)
+
+
+def test_error_right_greater_than_L():
+    def m(tc):
+        tc.edge_right[-1] = 11
+    assert _bad(m) == -309
+
+
+def test_error_unsorted_sites():
+    def m(tc):
+        tc.site_position[:] = [4.5, 1.0, 8.5]
+    assert _bad(m) == -400
+
+
+def test_error_bad_site_position():
+    def m(tc):
+        tc.site_position[2] = 10.0
+    assert _bad(m) == -402
+
+
+def test_error_contradictory_children():
+    tc = cases.paper_ex()
+    # a second parent for node 0 over [0, 7) (parent 7 is older than 6)
+    tc.set_edges(
+        np.append(tc.edge_left, 0), np.append(tc.edge_right, 7),
+        np.append(tc.edge_parent, 7), np.append(tc.edge_child, 0))
+    tc.sort_edges()
+    with pytest.raises(ta.LibraryError) as e:
+        tc.tree_sequence()
+    assert e.value.err == -311
+
+
+# ---- statistics argument errors: raised before any device work -------------
+@pytest.fixture(scope="module")
+def paper():
+    return cases.paper_ex().tree_sequence()
+
+
+def _err(fn):
+    with pytest.raises(ta.LibraryError) as e:
+        fn()
+    return e.value.err
+
+
+def test_stat_errors_match_reference_numbers(paper):
+    ll = paper.ll_tree_sequence
+    w = [0, 10]
+    assert _err(lambda: ll.diversity([2, 0], [0, 1], w)) == -908          # empty set
+    assert _err(lambda: ll.diversity([2], [0, 100], w)) == -202           # node OOB
+    assert _err(lambda: ll.diversity([2], [0, 5], w)) == -601             # not a sample
+    assert _err(lambda: ll.diversity([2], [0, 0], w)) == -600             # duplicate
+    assert _err(lambda: ll.diversity([2], [0, 1], [0, 5])) == -901        # windows end
+    assert _err(lambda: ll.diversity([2], [0, 1], [1, 10])) == -901       # windows start
+    assert _err(lambda: ll.diversity([2], [0, 1], [0, 5, 5, 10])) == -901  # not increasing
+    assert _err(lambda: ll.diversity([2], [0, 1], w, mode="node")) == -909
+    assert _err(lambda: ll.divergence([2, 2], [0, 1, 2, 3], [(0, 2)], w)) == -907
+    assert _err(lambda: ll.f4([1, 1, 1, 1], [0, 1, 2, 3], [(0, 1, 2, 4)], w)) == -907
+    assert _err(lambda: ll.allele_frequency_spectrum(
+        [2], [0, 1], w, [0, 1, np.inf], mode="site")) == -909
+    assert _err(lambda: ll.allele_frequency_spectrum(
+        [2], [0, 1], w, [1, np.inf], mode="branch")) == -924
+    assert _err(lambda: ll.allele_frequency_spectrum(
+        [2], [0, 1], w, [0, np.inf], mode="node")) == -909
+
+
+def test_stat_value_errors(paper):
+    ll = paper.ll_tree_sequence
+    with pytest.raises(ValueError):
+        ll.diversity([2], [0, 1], [0, 10], mode="bad")
+    with pytest.raises(ValueError):
+        ll.diversity([2], [0, 1], [0])
+    with pytest.raises(ValueError):
+        ll.diversity([3], [0, 1], [0, 10])
+    with pytest.raises(ValueError):
+        ll.divergence([2, 2], [0, 1, 2, 3], [(0, 1, 1)], [0, 10])
+    with pytest.raises(ValueError):
+        paper.divergence([[0, 1], [2, 3], [0]])  # 3 sets, no indexes
+    with pytest.raises(ValueError):
+        paper.diversity([[0, 1], []])
+    with pytest.raises(ValueError):
+        paper.diversity(windows="bad")
+
+
+def test_time_uncalibrated(paper):
+    tc = cases.paper_ex()
+    tc.time_units = "uncalibrated"
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    assert _err(lambda: ll.diversity([2], [0, 1], [0, 10], mode="branch")) == -910
+    assert _err(lambda: ll.allele_frequency_spectrum(
+        [2], [0, 1], [0, 10], [0, np.inf], mode="branch")) == -910
+
+
+# ---- the chunk plan ----------------------------------------------------------
+@pytest.mark.parametrize("n,L,seed,chunks", [
+    (200, 2e5, 3, None), (200, 2e5, 3, 7), (60, 5e4, 5, 1), (300, 3e5, 9, 50),
+])
+def test_plan_executes_to_oracle(n, L, seed, chunks):
+    tc = cases.generated(n, L, seed)
+    ts = tc.tree_sequence()
+    if chunks:
+        ts.ll_tree_sequence.set_num_chunks(chunks)
+    plan = ts.ll_tree_sequence.get_plan()
+    plan["_site_pos"] = tc.site_position
+    bp = ts.breakpoints(as_array=True)
+    plan_emulator.check_plan_bounds(plan, ts.num_trees, ts.num_samples)
+    orc = Oracle(tc)
+    sets = cases.sample_sets(n, 2, n // 3, seed)
+    w = np.linspace(0, L, 6)
+    for mode in ["branch", "site"]:
+        for stat, idx in [("diversity", None), ("divergence", [(0, 1), (1, 1)])]:
+            for pol in [False, True]:
+                got = plan_emulator.run(plan, bp, w, sets, n, stat, mode, idx, pol)
+                exp = orc.stat(stat, sets, idx, w, mode, False, pol)
+                # the plan holds branch lengths in FP32
+                assert np.max(np.abs(got - exp)) / np.max(np.abs(exp)) < 1e-7
+
+
+def test_plan_bounds_config1():
+    tc = cases.config1()
+    ts = tc.tree_sequence()
+    plan = ts.ll_tree_sequence.get_plan()
+    plan_emulator.check_plan_bounds(plan, ts.num_trees, ts.num_samples)

@@ -1,0 +1,303 @@
+"""Pins the oracle (oracle/tsk_oracle.c, the FP64 CPU restatement) so the GPU
+parity tests can trust it:
+
+  * against golden vectors written by the reference C library itself
+    (tests/golden/config1_reference.npz, see make_golden.py);
+  * against the reference C library live, where oracle/_ref was built;
+  * against brute-force definitions on tiny trees (pairwise path lengths,
+    genotype pairs, textbook Tajima's D and Fst from pairwise differences),
+    which pin the mathematics independently of the reference's sweep.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import oracle as om
+from oracle.oracle import Oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                      "config1_reference.npz")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    g = np.load(GOLDEN)
+    tc = cases.config1()
+    digest = bytes(g["digest"]).decode()
+    assert tc.column_digest() == digest, "generator output changed"
+    return g, tc, Oracle(tc)
+
+
+def _sets(flat, k):
+    return list(np.asarray(flat).reshape(k, -1))
+
+
+def test_golden_trees(golden):
+    g, tc, orc = golden
+    assert orc.num_trees == int(g["num_trees"][0])
+    assert np.array_equal(orc.breakpoints(), g["breakpoints"])
+    I, O = orc.index()
+    assert np.array_equal(I, g["index_I"]) and np.array_equal(O, g["index_O"])
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
+def test_golden_stats(golden, mode):
+    g, tc, orc = golden
+    s2, s4 = _sets(g["sets2"], 2), _sets(g["sets4"], 4)
+    w = g["windows"]
+    checks = [
+        ("diversity", s2, None, True, False, True, "diversity"),
+        ("divergence", s2, [(0, 1)], True, False, True, "divergence"),
+        ("segregating_sites", s2, None, True, False, True, "segsites"),
+        ("f4", s4, [(0, 1, 2, 3), (0, 2, 1, 3)], True, False, True, "f4"),
+        ("f2", s4, [(0, 1), (2, 3)], True, False, True, "f2"),
+        ("genetic_relatedness", s4, [(0, 1), (2, 2)], True, True, True, "relatedness"),
+        ("diversity", s2, None, False, False, True, "diversity_raw"),
+        ("segregating_sites", s2, None, False, False, True, "segsites_raw"),
+    ]
+    for name, sets, idx, sn, pol, centre, key in checks:
+        got = orc.stat(name, sets, idx, w, mode, sn, pol, centre)
+        exp = g[f"{key}_{mode}"]
+        assert np.allclose(got, exp, rtol=1e-12, atol=0), key
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
+def test_golden_afs(golden, mode):
+    g, tc, orc = golden
+    sets = _sets(g["afs_sets"], 2)
+    w = np.linspace(0, 1e6, 5)
+    for pol, key in [(False, "afs"), (True, "afs_polarised")]:
+        got = orc.afs(sets, w, None, mode, True, pol)
+        exp = g[f"{key}_{mode}"]
+        if mode == "site":
+            assert np.array_equal(got, exp)
+        else:
+            assert np.allclose(got, exp, rtol=1e-12, atol=1e-300)
+    got = orc.afs(sets, np.linspace(0, 1e6, 3), np.array([0, 5000, 20000, np.inf]), "branch")
+    assert np.allclose(got, g["afs_tw_branch"], rtol=1e-12, atol=1e-300)
+
+
+# ---- the reference, live ----------------------------------------------------
+needs_ref = pytest.mark.skipif(not om.reference_available(), reason="oracle/_ref not built")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [21, 22])
+def test_oracle_vs_reference_generated(seed):
+    tc = cases.generated(80, 1e5, seed)
+    orc, ref = Oracle(tc), om.Reference(tc)
+    sets = cases.sample_sets(80, 4, 15, seed)
+    w = np.linspace(0, 1e5, 4)
+    for mode in ["branch", "site"]:
+        for name in ["diversity", "segregating_sites", "Y1"]:
+            for pol in [False, True]:
+                assert np.allclose(orc.stat(name, sets, None, w, mode, True, pol),
+                                   ref.stat(name, sets, None, w, mode, True, pol),
+                                   rtol=1e-13, atol=0)
+        for name, idx in [("divergence", [(0, 1), (2, 2)]), ("Y2", [(1, 3)]), ("f2", [(0, 3)]),
+                          ("Y3", [(0, 1, 2)]), ("f3", [(2, 1, 0)]), ("f4", [(0, 1, 2, 3)]),
+                          ("genetic_relatedness", [(0, 1), (3, 3)])]:
+            for pol in [False, True]:
+                for centre in ([True, False] if name == "genetic_relatedness" else [True]):
+                    a = orc.stat(name, sets, idx, w, mode, True, pol, centre)
+                    b = ref.stat(name, sets, idx, w, mode, True, pol, centre)
+                    assert np.allclose(a, b, rtol=1e-12, atol=1e-15 * np.max(np.abs(b))), name
+        for pol in [False, True]:
+            assert np.allclose(orc.afs(sets[:3], w, None, mode, True, pol),
+                               ref.afs(sets[:3], w, None, mode, True, pol), rtol=1e-12, atol=0)
+
+
+@needs_ref
+@pytest.mark.parametrize("fixture", ["paper_ex", "single_tree_ex", "nonbinary_ex"])
+def test_oracle_vs_reference_fixtures(fixture):
+    tc = getattr(cases, fixture)()
+    orc, ref = Oracle(tc), om.Reference(tc)
+    n = int(np.sum(tc.node_flags & 1))
+    sets = [list(range(n // 2)), list(range(n // 2, n))]
+    for mode in ["branch", "site"]:
+        for pol in [False, True]:
+            for sn in [False, True]:
+                assert np.allclose(orc.stat("diversity", sets, None, None, mode, sn, pol),
+                                   ref.stat("diversity", sets, None, None, mode, sn, pol))
+                assert np.allclose(orc.stat("divergence", sets, [(0, 1)], None, mode, sn, pol),
+                                   ref.stat("divergence", sets, [(0, 1)], None, mode, sn, pol))
+            assert np.array_equal(orc.afs(sets, None, None, mode, True, pol),
+                                  ref.afs(sets, None, None, mode, True, pol))
+
+
+@needs_ref
+def test_oracle_error_numbers_match_reference():
+    tc = cases.paper_ex()
+    orc, ref = Oracle(tc), om.Reference(tc)
+    bad = [
+        (("diversity", [[0, 1], []], None, None), {}),
+        (("diversity", [[0, 100]], None, None), {}),
+        (("diversity", [[0, 5]], None, None), {}),
+        (("diversity", [[0, 0]], None, None), {}),
+        (("diversity", [[0, 1]], None, np.array([0, 5.0])), {}),
+        (("divergence", [[0, 1], [2, 3]], [(0, 2)], None), {}),
+    ]
+    for (name, sets, idx, w), kw in bad:
+        codes = []
+        for impl in (orc, ref):
+            try:
+                impl.stat(name, sets, idx, w)
+                codes.append(0)
+            except om.OracleError as e:
+                codes.append(e.err)
+        assert codes[0] == codes[1] != 0, (name, codes)
+
+
+# ---- brute force on tiny trees ----------------------------------------------
+def _trees(tc):
+    """[(left, right, parent array)] for every tree."""
+    bps = sorted(set(tc.edge_left) | set(tc.edge_right) | {0.0, tc.sequence_length})
+    out = []
+    for a, b in zip(bps[:-1], bps[1:]):
+        par = np.full(tc.num_nodes, -1)
+        on = (tc.edge_left <= a) & (tc.edge_right >= b)
+        par[tc.edge_child[on]] = tc.edge_parent[on]
+        out.append((a, b, par))
+    return out
+
+
+def _path(par, u):
+    p = [u]
+    while par[p[-1]] != -1:
+        p.append(par[p[-1]])
+    return p
+
+
+def _branch_distance(tc, par, u, v):
+    pu, pv = _path(par, u), _path(par, v)
+    common = set(pu) & set(pv)
+    t = tc.node_time
+    mrca = min(common, key=lambda x: t[x]) if common else None
+    if mrca is None:
+        return np.inf
+    return (t[mrca] - t[u]) + (t[mrca] - t[v])
+
+
+def _genotypes(tc):
+    """Derived-allele indicator per (site, sample) for 0/1 single-mutation sites."""
+    trees = _trees(tc)
+    n = int(np.sum(tc.node_flags & 1))
+    G = np.zeros((tc.num_sites, n), dtype=int)
+    for s, x in enumerate(tc.site_position):
+        par = next(p for a, b, p in trees if a <= x < b)
+        for m in np.where(tc.mutation_site == s)[0]:
+            node = tc.mutation_node[m]
+            for j in range(n):
+                if node in _path(par, j):
+                    G[s, j] = 1
+    return G
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    tc = cases.generated(8, 3e4, 31, mu=3e-8)
+    assert tc.num_sites > 3 and len(_trees(tc)) > 2
+    return tc, Oracle(tc)
+
+
+def test_brute_branch_diversity_divergence(tiny):
+    tc, orc = tiny
+    A, B = [0, 1, 2, 3], [4, 5, 6, 7]
+    L = tc.sequence_length
+    div_a = div_ab = 0.0
+    for a, b, par in _trees(tc):
+        span = b - a
+        div_a += span * np.mean([_branch_distance(tc, par, u, v)
+                                 for u, v in itertools.combinations(A, 2)])
+        div_ab += span * np.mean([_branch_distance(tc, par, u, v) for u in A for v in B])
+    got = orc.stat("diversity", [A], None, None, "branch", True)[0, 0]
+    assert np.isclose(got, div_a / L, rtol=1e-12)
+    got = orc.stat("divergence", [A, B], [(0, 1)], None, "branch", True)[0, 0]
+    assert np.isclose(got, div_ab / L, rtol=1e-12)
+
+
+def test_brute_site_stats(tiny):
+    tc, orc = tiny
+    G = _genotypes(tc)
+    A, B = [0, 1, 2], [3, 4, 5, 6]
+    pi = sum(np.mean([G[s, u] != G[s, v] for u, v in itertools.combinations(A, 2)])
+             for s in range(G.shape[0]))
+    assert np.isclose(orc.stat("diversity", [A], None, None, "site", False)[0, 0], pi)
+    dxy = sum(np.mean([G[s, u] != G[s, v] for u in A for v in B]) for s in range(G.shape[0]))
+    assert np.isclose(orc.stat("divergence", [A, B], [(0, 1)], None, "site", False)[0, 0], dxy)
+    seg = sum(0 < G[s, A].sum() < len(A) for s in range(G.shape[0]))
+    assert np.isclose(orc.stat("segregating_sites", [A], None, None, "site", False)[0, 0], seg)
+    # f4(A,B;C,D) = mean over quartets of (g_a - g_b)(g_c - g_d) for 0/1 sites
+    C, D = [6, 7], [0, 7]
+    f4 = sum(np.mean([(G[s, a] - G[s, b]) * (G[s, c] - G[s, d])
+                      for a in A for b in B for c in C for d in D]) for s in range(G.shape[0]))
+    got = orc.stat("f4", [A, B, C, D], [(0, 1, 2, 3)], None, "site", False)[0, 0]
+    assert np.isclose(got, f4, atol=1e-12)
+
+
+def test_brute_tajimas_d(tiny):
+    """Tajima (1989) eq. 38 written with a1, a2, b1, b2, c1, c2, e1, e2 -- a
+    different algebraic form from trees.py:10008-10018."""
+    tc, orc = tiny
+    G = _genotypes(tc)
+    S_ = list(range(8))
+    n = len(S_)
+    T = orc.stat("diversity", [S_], None, None, "site", False)[0, 0]
+    S = orc.stat("segregating_sites", [S_], None, None, "site", False)[0, 0]
+    assert S == sum(0 < G[s].sum() < n for s in range(G.shape[0]))
+    a1 = sum(1 / i for i in range(1, n))
+    a2 = sum(1 / i**2 for i in range(1, n))
+    b1 = (n + 1) / (3 * (n - 1))
+    b2 = 2 * (n**2 + n + 3) / (9 * n * (n - 1))
+    c1 = b1 - 1 / a1
+    c2 = b2 - (n + 2) / (a1 * n) + a2 / a1**2
+    e1, e2 = c1 / a1, c2 / (a1**2 + a2)
+    D = (T - S / a1) / np.sqrt(e1 * S + e2 * S * (S - 1))
+    assert np.isclose(om.tajimas_d(np.array([[T]]), np.array([[S]]), [n])[0, 0], D, rtol=1e-12)
+
+
+def test_brute_fst(tiny):
+    """Slatkin's Fst = (pi_T - pi_S) / pi_T from mean within- and all-pair
+    differences, against trees.py:10113-10125's form."""
+    tc, orc = tiny
+    A, B = [0, 1, 2, 3], [4, 5, 6, 7]
+    du = orc.stat("diversity", [A], None, None, "site")[0, 0]
+    dv = orc.stat("diversity", [B], None, None, "site")[0, 0]
+    duv = orc.stat("divergence", [A, B], [(0, 1)], None, "site")[0, 0]
+    pi_S = (du + dv) / 2
+    pi_T = (du + dv + 2 * duv) / 4
+    exp = (pi_T - pi_S) / pi_T
+    got = om.fst(np.array([[du, dv]]), np.array([[duv]]), [(0, 1)])[0, 0]
+    assert np.isclose(got, exp, rtol=1e-12)
+
+
+def test_brute_afs_site(tiny):
+    tc, orc = tiny
+    G = _genotypes(tc)
+    A = [0, 1, 2, 3, 4]
+    exp = np.zeros(len(A) + 1)
+    for s in range(G.shape[0]):
+        k = G[s, A].sum()
+        if 0 < G[s].sum() < 8:
+            exp[k] += 1
+    got = orc.afs([A], None, None, "site", False, True)[0, 0]
+    assert np.array_equal(got, exp)
+
+
+def test_identity_invariant():
+    """test_stats.c:1061-1124: branch diversity-style identity -- the branch
+    segregating-sites of all samples with span normalisation off equals the
+    total branch length integrated over the genome (each branch subtends
+    0 < x < n samples, summary (1 - x/n) + x/n = 1 unpolarised)."""
+    tc = cases.generated(10, 2e4, 33)
+    orc = Oracle(tc)
+    total = 0.0
+    for a, b, par in _trees(tc):
+        for u in range(tc.num_nodes):
+            if par[u] != -1:
+                total += (b - a) * (tc.node_time[par[u]] - tc.node_time[u])
+    got = orc.stat("segregating_sites", [list(range(10))], None, None, "branch", False)[0, 0]
+    assert np.isclose(got, total, rtol=1e-12)

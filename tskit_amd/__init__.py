@@ -1,0 +1,23 @@
+"""tskit_amd: the tree-sequence statistics hot path of tskit on MI355X.
+
+A drop-in for the ``general_stat``-backed statistics of ``tskit.TreeSequence``
+(diversity, divergence, Fst, Tajima's D, f2/f3/f4, allele frequency spectrum,
+genetic relatedness, ...), computed by hand-written HIP kernels behind a C ABI
+(``libtsgpu.so``, declared in ``include/tsgpu.h``).
+"""
+from .exceptions import LibraryError, TskitException
+from .sim import random_sample_sets, sim_ancestry, simulate_tables
+from .tables import NODE_IS_SAMPLE, TableCollection, parse_text
+from .trees import TreeSequence
+
+__all__ = [
+    "LibraryError",
+    "TskitException",
+    "TableCollection",
+    "TreeSequence",
+    "NODE_IS_SAMPLE",
+    "parse_text",
+    "sim_ancestry",
+    "simulate_tables",
+    "random_sample_sets",
+]

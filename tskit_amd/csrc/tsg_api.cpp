@@ -1,0 +1,1119 @@
+// C ABI of the tsgpu library: device-resident tree sequences and the
+// general_stat-family statistics computed by the HIP kernels.
+//
+// Every statistics entry point takes the same arguments, in the same order
+// and with the same meaning, as the reference function it replaces
+// (c/tskit/trees.h:1098-1239), with the const tsk_treeseq_t* replaced by an
+// opaque device-resident handle.  Inputs are validated in the reference's
+// order and the same error numbers are returned (trees.c:2035-2220,
+// 3815-3928, 4667-4688).  The caller owns the result buffer (host memory,
+// W x M doubles, C order); the callee zero-fills nothing the caller sees
+// until the result is complete.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <unordered_set>
+#include <vector>
+
+#include "tsg_common.h"
+#include "tsg_launch.h"
+#include "tsg_treeseq.h"
+
+void tsg_build_index(const double *left, const double *right, const int32_t *parent,
+                     const int32_t *child, const double *time, int32_t E, int32_t *I, int32_t *O);
+int tsg_init_trees(tsg_treeseq *ts, const char *ancestral_state,
+                   const uint64_t *ancestral_state_offset, const char *derived_state,
+                   const uint64_t *derived_state_offset);
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Workspace {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf pair_s, pair_c, tot, outs, queries, windows, cq_wfirst, cq_nwin, cq_poff, partial, dR,
+        Rstart, raw, rawbase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
+    int lds_limit = 65536;
+    // profiling of the dominant (sweep) kernel
+    bool profile = false;
+    double sweep_ms = 0;
+    uint64_t sweep_launches = 0;
+};
+
+std::mutex g_mutex;
+
+int hip_check(hipError_t e) { return e == hipSuccess ? 0 : TSG_ERR_DEVICE; }
+
+int ensure(DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (b.bytes >= bytes) return 0;
+    if (b.p != nullptr) {
+        (void) hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        if (hipMalloc(&b.p, bytes) != hipSuccess) {
+            (void) hipGetLastError();
+            return TSG_ERR_DEVICE_MEMORY;
+        }
+        want = bytes;
+    }
+    b.bytes = want;
+    return 0;
+}
+
+template <typename T>
+int upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+    int ret = ensure(b, v.size() * sizeof(T));
+    if (ret != 0) return ret;
+    if (!v.empty()) {
+        ret = hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    return ret;
+}
+
+template <typename T>
+int upload_ptr(T **dst, const std::vector<T> &v, size_t &bytes) {
+    const size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
+    if (hipMalloc((void **) dst, n) != hipSuccess) {
+        (void) hipGetLastError();
+        return TSG_ERR_DEVICE_MEMORY;
+    }
+    bytes += n;
+    if (!v.empty()) return hip_check(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+Workspace *ws_of(tsg_treeseq *ts);
+
+// ---- query description -------------------------------------------------
+struct Block {
+    int stat;
+    int tuple;                  // 0 for one-way (one output per set)
+    std::vector<int32_t> idx;   // M x tuple, indexes into the query's sets
+    bool polarised;
+};
+
+struct Query {
+    uint32_t mode = TSG_STAT_SITE;
+    bool span_normalise = false;
+    std::vector<std::vector<int32_t>> sets;  // sample indexes per column
+    int ncentre = 0;                         // columns used by the relatedness mean
+    std::vector<Block> blocks;
+    std::vector<double> windows;
+    int final_kind = TSG_FINAL_NONE;
+    int P = 0;                               // final outputs per window
+    std::vector<int32_t> fin_ipar;
+    std::vector<double> fin_dpar;
+    double *result = nullptr;
+    int M() const {
+        int m = 0;
+        for (const Block &b : blocks) m += (b.tuple == 0) ? (int) sets.size() : (int) (b.idx.size() / b.tuple);
+        return m;
+    }
+};
+
+int check_windows(const tsg_treeseq *ts, uint64_t nw, const double *w) {  // trees.c:1244-1286
+    if (nw < 1) return TSG_ERR_BAD_NUM_WINDOWS;
+    if (w[0] != 0 || w[nw] != ts->L) return TSG_ERR_BAD_WINDOWS;
+    for (uint64_t j = 0; j < nw; j++) {
+        if (w[j] >= w[j + 1]) return TSG_ERR_BAD_WINDOWS;
+    }
+    return 0;
+}
+
+// trees.c:2114-2149, then the duplicate check of trees.c:2196-2214.
+int parse_sample_sets(const tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, const int32_t *sets,
+                      std::vector<std::vector<int32_t>> &out) {
+    if (K == 0) return TSG_ERR_INSUFFICIENT_SAMPLE_SETS;
+    uint64_t j = 0;
+    for (uint64_t k = 0; k < K; k++) {
+        if (sizes[k] == 0) return TSG_ERR_EMPTY_SAMPLE_SET;
+        for (uint64_t l = 0; l < sizes[k]; l++) {
+            const int32_t u = sets[j++];
+            if (u < 0 || u >= ts->num_nodes) return TSG_ERR_NODE_OUT_OF_BOUNDS;
+            if (ts->sample_index_map[u] == TSG_NULL) return TSG_ERR_BAD_SAMPLES;
+        }
+    }
+    out.assign(K, {});
+    std::vector<uint64_t> seen(ts->num_samples, UINT64_MAX);
+    j = 0;
+    for (uint64_t k = 0; k < K; k++) {
+        out[k].reserve(sizes[k]);
+        for (uint64_t l = 0; l < sizes[k]; l++) {
+            const int32_t si = ts->sample_index_map[sets[j++]];
+            if (seen[si] == k) return TSG_ERR_DUPLICATE_SAMPLE;
+            seen[si] = k;
+            out[k].push_back(si);
+        }
+    }
+    return 0;
+}
+
+// general_stat checks (trees.c:2048-2078) + mode/time checks.
+int parse_mode_windows(const tsg_treeseq *ts, uint32_t options, uint64_t nw, const double *windows,
+                       Query &q) {
+    bool site = options & TSG_STAT_SITE, branch = options & TSG_STAT_BRANCH;
+    const bool node = options & TSG_STAT_NODE;
+    if (!(site || branch || node)) site = true;
+    if ((int) site + (int) branch + (int) node > 1) return TSG_ERR_MULTIPLE_STAT_MODES;
+    if (node) return TSG_ERR_UNSUPPORTED_STAT_MODE;  // node mode is not on this path
+    if (windows == nullptr) {
+        q.windows = {0.0, ts->L};
+    } else {
+        int ret = check_windows(ts, nw, windows);
+        if (ret != 0) return ret;
+        q.windows.assign(windows, windows + nw + 1);
+    }
+    q.mode = branch ? TSG_STAT_BRANCH : TSG_STAT_SITE;
+    if (branch && ts->time_uncalibrated && !(options & TSG_STAT_ALLOW_TIME_UNCALIBRATED)) {
+        return TSG_ERR_TIME_UNCALIBRATED;
+    }
+    q.span_normalise = (options & TSG_STAT_SPAN_NORMALISE) != 0;
+    return 0;
+}
+
+int check_tuples(uint64_t K, int tuple, uint64_t M, const int32_t *idx) {  // trees.c:4667-4688
+    if (K < 1) return TSG_ERR_INSUFFICIENT_SAMPLE_SETS;
+    if (M < 1) return TSG_ERR_INSUFFICIENT_INDEX_TUPLES;
+    for (uint64_t j = 0; j < M * (uint64_t) tuple; j++) {
+        if (idx[j] < 0 || idx[j] >= (int32_t) K) return TSG_ERR_BAD_SAMPLE_SET_INDEX;
+    }
+    return 0;
+}
+
+double summary_inv(int stat, const int32_t *n, const int32_t *c) {
+    auto N = [&](int t) { return (double) n[c[t]]; };
+    switch (stat) {
+    case TSG_STAT_DIVERSITY:
+        return 1.0 / (N(0) * (N(0) - 1));
+    case TSG_STAT_SEGREGATING_SITES:
+        return 1.0 / N(0);
+    case TSG_STAT_Y1:
+        return 1.0 / (N(0) * (N(0) - 1) * (N(0) - 2));
+    case TSG_STAT_DIVERGENCE:
+        return 1.0 / (N(0) * (N(1) - (c[0] == c[1])));
+    case TSG_STAT_RELATEDNESS:
+        return 1.0;
+    case TSG_STAT_RELATEDNESS_NONCENTRED:
+        return 1.0 / (N(0) * N(1));
+    case TSG_STAT_Y2:
+        return 1.0 / (N(0) * N(1) * (N(1) - 1));
+    case TSG_STAT_F2:
+        return 1.0 / (N(0) * (N(0) - 1) * N(1) * (N(1) - 1));
+    case TSG_STAT_Y3:
+        return 1.0 / (N(0) * N(1) * N(2));
+    case TSG_STAT_F3:
+        return 1.0 / (N(0) * (N(0) - 1) * N(1) * N(2));
+    case TSG_STAT_F4:
+        return 1.0 / (N(0) * N(1) * N(2) * N(3));
+    }
+    return 0.0;
+}
+
+int device_upload(tsg_treeseq *ts);
+
+// Runs a batch of queries (same tree sequence, same mode) in one sweep.
+int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
+    int ret = device_upload(ts);
+    if (ret != 0) return ret;
+    Workspace *ws = ws_of(ts);
+    const int C = ts->num_chunks;
+    const int Q = (int) batch.size();
+    const uint32_t mode = batch[0]->mode;
+    // fused columns / outputs
+    std::vector<int32_t> pair_s, pair_c, col_total;
+    std::vector<TsgOut> outs;
+    std::vector<TsgQuery> qs(Q);
+    std::vector<double> wins;
+    bool noskip = false, any_unpolarised = false;
+    int K = 0, M = 0;
+    for (int qi = 0; qi < Q; qi++) {
+        Query &q = *batch[qi];
+        const int cbase = K;
+        for (const auto &s : q.sets) {
+            for (int32_t si : s) {
+                pair_s.push_back(si);
+                pair_c.push_back(K);
+            }
+            col_total.push_back((int32_t) s.size());
+            K++;
+        }
+        TsgQuery &tq = qs[qi];
+        tq.W = (int32_t) q.windows.size() - 1;
+        tq.mbase = M;
+        tq.cbase = cbase;
+        tq.win_off = (int64_t) wins.size();
+        wins.insert(wins.end(), q.windows.begin(), q.windows.end());
+        for (const Block &b : q.blocks) {
+            const int nout = (b.tuple == 0) ? (int) q.sets.size() : (int) (b.idx.size() / b.tuple);
+            for (int m = 0; m < nout; m++) {
+                TsgOut o{};
+                o.stat = b.stat;
+                o.q = qi;
+                int32_t cols[4] = {0, 0, 0, 0};
+                if (b.tuple == 0) {
+                    cols[0] = m;
+                } else {
+                    for (int t = 0; t < b.tuple; t++) cols[t] = b.idx[(size_t) m * b.tuple + t];
+                }
+                int32_t gc[4];
+                for (int t = 0; t < 4; t++) gc[t] = cbase + cols[t];
+                o.i = gc[0];
+                o.j = gc[1];
+                o.k = gc[2];
+                o.l = gc[3];
+                o.ni = col_total[gc[0]];
+                o.nj = col_total[gc[1]];
+                o.nk = col_total[gc[2]];
+                o.nl = col_total[gc[3]];
+                o.cbase = cbase;
+                o.ccount = q.ncentre;
+                o.flags = b.polarised ? 1 : 0;
+                o.inv = summary_inv(b.stat, col_total.data(), gc);
+                if (!std::isfinite(o.inv)) noskip = true;
+                if (!b.polarised) any_unpolarised = true;
+                outs.push_back(o);
+                M++;
+            }
+        }
+        tq.M = M - tq.mbase;
+    }
+    // per (chunk, query) window ranges
+    std::vector<int32_t> cq_wfirst((size_t) C * Q), cq_nwin((size_t) C * Q);
+    std::vector<int64_t> cq_poff((size_t) C * Q);
+    int64_t pcount = 0;
+    for (int qi = 0; qi < Q; qi++) {
+        const double *w = wins.data() + qs[qi].win_off;
+        const int W = qs[qi].W;
+        qs[qi].pbase = pcount;
+        int64_t slots = 0;
+        for (int c = 0; c < C; c++) {
+            const double cs = ts->breakpoints[ts->chunk_tree0[c]];
+            const double ce = ts->breakpoints[ts->chunk_tree0[c + 1]];
+            const int wf = (int) (std::upper_bound(w, w + W + 1, cs) - w) - 1;
+            const int wl = (int) (std::lower_bound(w, w + W + 1, ce) - w) - 1;
+            cq_wfirst[(size_t) c * Q + qi] = wf;
+            cq_nwin[(size_t) c * Q + qi] = wl - wf + 1;
+            cq_poff[(size_t) c * Q + qi] = slots;
+            slots += wl - wf + 1;
+        }
+        pcount += slots * qs[qi].M;
+    }
+    std::vector<int64_t> rawbase(Q);
+    int64_t rawcount = 0;
+    for (int qi = 0; qi < Q; qi++) {
+        rawbase[qi] = rawcount;
+        rawcount += (int64_t) qs[qi].W * qs[qi].M;
+    }
+    hipStream_t s = ws->stream;
+    if ((ret = upload(ws->pair_s, pair_s, s)) || (ret = upload(ws->pair_c, pair_c, s)) ||
+        (ret = upload(ws->tot, col_total, s)) || (ret = upload(ws->outs, outs, s)) ||
+        (ret = upload(ws->queries, qs, s)) || (ret = upload(ws->windows, wins, s)) ||
+        (ret = upload(ws->cq_wfirst, cq_wfirst, s)) || (ret = upload(ws->cq_nwin, cq_nwin, s)) ||
+        (ret = upload(ws->cq_poff, cq_poff, s)) || (ret = upload(ws->rawbase, rawbase, s)) ||
+        (ret = ensure(ws->partial, (size_t) pcount * 8)) ||
+        (ret = ensure(ws->raw, (size_t) rawcount * 8))) {
+        return ret;
+    }
+    if (mode == TSG_STAT_BRANCH) {
+        if ((ret = ensure(ws->dR, (size_t) C * M * 8)) || (ret = ensure(ws->Rstart, (size_t) C * M * 8))) {
+            return ret;
+        }
+    }
+    const int R = 2 + K;
+    const size_t stage = (size_t) 3 * K * 4 + 64;
+    const size_t lds_need = (size_t) ts->max_nloc * R * 4 + stage;
+    const bool lds = lds_need <= (size_t) ws->lds_limit;
+    if (!lds) {
+        const size_t slab = (size_t) C * ts->max_nloc * R * 4;
+        if ((ret = ensure(ws->slab, slab))) return ret;
+    }
+    const TsgDevice &d = ts->dev;
+    TsgSweepArgs a{};
+    a.chunk_tree0 = d.chunk_tree0;
+    a.chunk_node0 = d.chunk_node0;
+    a.loc_parent0 = d.loc_parent0;
+    a.loc_bl0 = d.loc_bl0;
+    a.loc_time = d.loc_time;
+    a.loc_lastrem = d.loc_lastrem;
+    a.tr_off = d.tr_off;
+    a.tr_nrem = d.tr_nrem;
+    a.ev_child = d.ev_child;
+    a.ev_parent = d.ev_parent;
+    a.ev_bl = d.ev_bl;
+    a.bp = d.breakpoints;
+    a.tree_site0 = d.tree_site0;
+    a.site_pos = d.site_pos;
+    a.site_mut0 = d.site_mut0;
+    a.mut_node = d.mut_node;
+    a.mut_allele = d.mut_allele;
+    a.mut_pallele = d.mut_pallele;
+    a.site_nalleles = d.site_nalleles;
+    a.num_trees = ts->num_trees;
+    a.num_chunks = C;
+    a.num_samples = ts->num_samples;
+    a.K = K;
+    a.M = M;
+    a.Q = Q;
+    a.npairs = (int32_t) pair_s.size();
+    a.pair_sample = (const int32_t *) ws->pair_s.p;
+    a.pair_col = (const int32_t *) ws->pair_c.p;
+    a.col_total = (const int32_t *) ws->tot.p;
+    a.outs = (const TsgOut *) ws->outs.p;
+    a.queries = (const TsgQuery *) ws->queries.p;
+    a.windows = (const double *) ws->windows.p;
+    a.cq_wfirst = (const int32_t *) ws->cq_wfirst.p;
+    a.cq_nwin = (const int32_t *) ws->cq_nwin.p;
+    a.cq_poff = (const int64_t *) ws->cq_poff.p;
+    a.partial = (double *) ws->partial.p;
+    a.dR = (double *) ws->dR.p;
+    a.flags = (noskip ? TSG_F_NOSKIP : 0u) | (any_unpolarised ? TSG_F_ANY_UNPOLARISED : 0u);
+    a.g_rec = (int32_t *) ws->slab.p;
+    a.g_stride = (int64_t) ts->max_nloc * R;
+    const int mode_id = (mode == TSG_STAT_BRANCH) ? 0 : 1;
+    const size_t shmem = lds ? lds_need : stage;
+    if (ws->profile) (void) hipEventRecord(ws->ev0, s);
+    for (int m0 = 0; m0 < M; m0 += 512) {
+        a.m_begin = m0;
+        a.m_end = std::min(M, m0 + 512);
+        const int span = a.m_end - a.m_begin;
+        const int mpl = span <= 64 ? 1 : span <= 128 ? 2 : span <= 256 ? 4 : 8;
+        if ((ret = hip_check(tsg_launch_sweep(a, mode_id, lds, mpl, shmem, s)))) return ret;
+    }
+    if (ws->profile) (void) hipEventRecord(ws->ev1, s);
+    TsgCombineArgs ca{};
+    ca.bp = d.breakpoints;
+    ca.chunk_tree0 = d.chunk_tree0;
+    ca.num_chunks = C;
+    ca.Q = Q;
+    ca.M = M;
+    ca.queries = a.queries;
+    ca.windows = a.windows;
+    ca.cq_wfirst = a.cq_wfirst;
+    ca.cq_nwin = a.cq_nwin;
+    ca.cq_poff = a.cq_poff;
+    ca.partial = a.partial;
+    ca.Rstart = (mode == TSG_STAT_BRANCH) ? (const double *) ws->Rstart.p : nullptr;
+    ca.raw = (double *) ws->raw.p;
+    ca.rawbase = (const int64_t *) ws->rawbase.p;
+    ca.flags = batch[0]->span_normalise ? TSG_F_SPAN_NORMALISE : 0u;
+    // span normalisation is per query; the batch builder only fuses queries
+    // that agree on it.
+    if ((ret = hip_check(tsg_launch_combine(ca, (mode == TSG_STAT_BRANCH) ? a.dR : nullptr,
+                                            (double *) ws->Rstart.p, rawcount, s)))) {
+        return ret;
+    }
+    // finals
+    std::vector<TsgFinalHost> fin(Q);
+    std::vector<int32_t> ipar;
+    std::vector<double> dpar;
+    int64_t outcount = 0;
+    for (int qi = 0; qi < Q; qi++) {
+        Query &q = *batch[qi];
+        TsgFinalHost &f = fin[qi];
+        f.kind = q.final_kind;
+        f.Kq = (int32_t) q.sets.size();
+        f.P = q.P;
+        f.W = qs[qi].W;
+        f.rawbase = rawbase[qi];
+        f.outbase = outcount;
+        f.Mraw = qs[qi].M;
+        if (q.final_kind == TSG_FINAL_TAJIMAS_D) {
+            f.pbase = (int64_t) dpar.size();
+            dpar.insert(dpar.end(), q.fin_dpar.begin(), q.fin_dpar.end());
+        } else {
+            f.pbase = (int64_t) ipar.size();
+            ipar.insert(ipar.end(), q.fin_ipar.begin(), q.fin_ipar.end());
+        }
+        outcount += (int64_t) f.W * f.P;
+    }
+    if ((ret = upload(ws->fin, fin, s)) || (ret = upload(ws->ipar, ipar, s)) ||
+        (ret = upload(ws->dpar, dpar, s)) || (ret = ensure(ws->out, (size_t) outcount * 8))) {
+        return ret;
+    }
+    if ((ret = hip_check(tsg_launch_finals(ws->fin.p, Q, (const double *) ws->raw.p,
+                                           (const int32_t *) ws->ipar.p,
+                                           (const double *) ws->dpar.p, (double *) ws->out.p,
+                                           outcount, s)))) {
+        return ret;
+    }
+    for (int qi = 0; qi < Q; qi++) {
+        const TsgFinalHost &f = fin[qi];
+        if ((ret = hip_check(hipMemcpyAsync(batch[qi]->result, (double *) ws->out.p + f.outbase,
+                                            (size_t) f.W * f.P * 8, hipMemcpyDeviceToHost, s)))) {
+            return ret;
+        }
+    }
+    if ((ret = hip_check(hipStreamSynchronize(s)))) return ret;
+    if (ws->profile) {
+        float ms = 0;
+        (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
+        ws->sweep_ms += ms;
+        ws->sweep_launches += 1;
+    }
+    return 0;
+}
+
+int run_one(tsg_treeseq *ts, Query &q) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    std::vector<Query *> b{&q};
+    return run_batch(ts, b);
+}
+
+// ---- device residency -----------------------------------------------------
+Workspace *ws_of(tsg_treeseq *ts) { return reinterpret_cast<Workspace *>(ts->workspace); }
+
+int device_upload(tsg_treeseq *ts) {
+    if (ts->dev.device >= 0) return 0;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+        (void) hipGetLastError();
+        return TSG_ERR_NO_DEVICE;
+    }
+    int dev = 0;
+    if (hip_check(hipGetDevice(&dev))) return TSG_ERR_NO_DEVICE;
+    Workspace *ws = new (std::nothrow) Workspace();
+    if (ws == nullptr) return TSG_ERR_NO_MEMORY;
+    ts->workspace = ws;
+    int ret;
+    if ((ret = hip_check(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking)))) return ret;
+    if ((ret = hip_check(hipEventCreate(&ws->ev0))) || (ret = hip_check(hipEventCreate(&ws->ev1)))) {
+        return ret;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        ws->lds_limit = (int) std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    }
+    TsgDevice &d = ts->dev;
+    size_t &by = d.bytes;
+    std::vector<int32_t> site_nalleles = ts->site_nalleles;
+    if ((ret = upload_ptr(&d.chunk_tree0, ts->chunk_tree0, by)) ||
+        (ret = upload_ptr(&d.chunk_node0, ts->chunk_node0, by)) ||
+        (ret = upload_ptr(&d.loc_parent0, ts->loc_parent0, by)) ||
+        (ret = upload_ptr(&d.loc_bl0, ts->loc_bl0, by)) ||
+        (ret = upload_ptr(&d.loc_time, ts->loc_time, by)) ||
+        (ret = upload_ptr(&d.loc_lastrem, ts->loc_lastrem, by)) ||
+        (ret = upload_ptr(&d.tr_off, ts->tr_off, by)) ||
+        (ret = upload_ptr(&d.tr_nrem, ts->tr_nrem, by)) ||
+        (ret = upload_ptr(&d.ev_child, ts->ev_child, by)) ||
+        (ret = upload_ptr(&d.ev_parent, ts->ev_parent, by)) ||
+        (ret = upload_ptr(&d.ev_bl, ts->ev_bl, by)) ||
+        (ret = upload_ptr(&d.breakpoints, ts->breakpoints, by)) ||
+        (ret = upload_ptr(&d.tree_site0, ts->tree_site0, by)) ||
+        (ret = upload_ptr(&d.site_pos, ts->site_pos, by)) ||
+        (ret = upload_ptr(&d.site_mut0, ts->site_mut0, by)) ||
+        (ret = upload_ptr(&d.mut_node, ts->mut_node_loc, by)) ||
+        (ret = upload_ptr(&d.mut_allele, ts->mut_allele, by)) ||
+        (ret = upload_ptr(&d.mut_pallele, ts->mut_pallele, by)) ||
+        (ret = upload_ptr(&d.site_nalleles, site_nalleles, by))) {
+        return ret;
+    }
+    d.device = dev;
+    return 0;
+}
+
+void free_workspace(tsg_treeseq *ts) {
+    Workspace *ws = ws_of(ts);
+    if (ws == nullptr) return;
+    DevBuf *bufs[] = {&ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
+                      &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
+                      &ws->Rstart, &ws->raw, &ws->rawbase, &ws->out, &ws->fin, &ws->ipar,
+                      &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin};
+    for (DevBuf *b : bufs) {
+        if (b->p != nullptr) (void) hipFree(b->p);
+    }
+    if (ws->ev0) (void) hipEventDestroy(ws->ev0);
+    if (ws->ev1) (void) hipEventDestroy(ws->ev1);
+    if (ws->stream) (void) hipStreamDestroy(ws->stream);
+    delete ws;
+    ts->workspace = nullptr;
+}
+
+}  // namespace
+
+void tsg_device_free(TsgDevice *d) {
+    void *ptrs[] = {d->chunk_tree0, d->chunk_node0, d->loc_parent0, d->loc_bl0, d->loc_time,
+                    d->loc_lastrem, d->tr_off, d->tr_nrem, d->ev_child, d->ev_parent, d->ev_bl,
+                    d->breakpoints, d->tree_site0, d->site_pos, d->site_mut0, d->mut_node,
+                    d->mut_allele, d->mut_pallele, d->site_nalleles};
+    for (void *p : ptrs) {
+        if (p != nullptr) (void) hipFree(p);
+    }
+    *d = TsgDevice();
+}
+
+namespace {
+int default_num_chunks(const tsg_treeseq *ts) {
+    // Enough chunks to fill the device (about 4 waves per CU), each with a
+    // few dozen edge events; the plan's node snapshots cost ~24 bytes per
+    // local node, so cap the total near 4 GiB.
+    const int64_t events = 2 * (int64_t) ts->num_edges;
+    int64_t c = std::max<int64_t>(1, events / 48);
+    c = std::min<int64_t>(c, 4096);
+    const int64_t per_chunk = 2 * (int64_t) ts->num_samples + 64;
+    const int64_t budget = (int64_t) 4 << 30;
+    c = std::min<int64_t>(c, std::max<int64_t>(1, budget / (per_chunk * 24)));
+    c = std::min<int64_t>(c, std::max(1, ts->num_trees));
+    return (int) c;
+}
+}  // namespace
+
+extern "C" {
+
+const char *tsg_strerror(int err) {
+    switch (err) {
+    case 0: return "Normal exit condition. This is not an error!";
+    case TSG_ERR_NO_MEMORY: return "Out of memory";
+    case TSG_ERR_BAD_PARAM_VALUE: return "Bad parameter value provided";
+    case TSG_ERR_NODE_OUT_OF_BOUNDS: return "Node out of bounds";
+    case TSG_ERR_EDGE_OUT_OF_BOUNDS: return "Edge out of bounds";
+    case TSG_ERR_DUPLICATE_SAMPLE: return "Duplicate sample value";
+    case TSG_ERR_BAD_SAMPLES: return "The nodes provided are not samples";
+    case TSG_ERR_BAD_NUM_WINDOWS: return "Must have at least one window, [0, L]";
+    case TSG_ERR_BAD_WINDOWS: return "Windows must be increasing list [0, ..., L]";
+    case TSG_ERR_MULTIPLE_STAT_MODES: return "Cannot specify more than one stats mode";
+    case TSG_ERR_BAD_STATE_DIMS: return "Must have state dimension >= 1";
+    case TSG_ERR_BAD_RESULT_DIMS: return "Must have result dimension >= 1";
+    case TSG_ERR_INSUFFICIENT_SAMPLE_SETS: return "Insufficient sample sets provided";
+    case TSG_ERR_INSUFFICIENT_INDEX_TUPLES: return "Insufficient sample set index tuples provided";
+    case TSG_ERR_BAD_SAMPLE_SET_INDEX: return "Sample set index out of bounds";
+    case TSG_ERR_EMPTY_SAMPLE_SET: return "Samples cannot be empty";
+    case TSG_ERR_UNSUPPORTED_STAT_MODE: return "Requested statistics mode not supported for this method";
+    case TSG_ERR_TIME_UNCALIBRATED: return "Statistics using branch lengths cannot be calculated when time_units is 'uncalibrated'";
+    case TSG_ERR_BAD_TIME_WINDOWS_DIM: return "Must have at least one time window";
+    case TSG_ERR_BAD_TIME_WINDOWS: return "Time windows must be strictly increasing and begin at 0";
+    case TSG_ERR_DEVICE: return "A HIP runtime call failed";
+    case TSG_ERR_NO_DEVICE: return "No usable GPU device";
+    case TSG_ERR_DEVICE_MEMORY: return "Device memory allocation failed";
+    case TSG_ERR_UNSUPPORTED_INPUT: return "Input shape not supported by the device path";
+    case TSG_ERR_NOT_BIALLELIC: return "Every site must carry exactly one mutation";
+    default: return "Error in tree sequence input or statistics arguments";
+    }
+}
+
+int tsg_treeseq_init(tsg_treeseq **out, double sequence_length, uint64_t num_nodes,
+                     const uint32_t *node_flags, const double *node_time, uint64_t num_edges,
+                     const double *edge_left, const double *edge_right, const int32_t *edge_parent,
+                     const int32_t *edge_child, uint64_t num_sites, const double *site_position,
+                     const char *ancestral_state, const uint64_t *ancestral_state_offset,
+                     uint64_t num_mutations, const int32_t *mutation_site,
+                     const int32_t *mutation_node, const int32_t *mutation_parent,
+                     const char *derived_state, const uint64_t *derived_state_offset,
+                     const int32_t *edge_insertion_order, const int32_t *edge_removal_order,
+                     uint32_t options) {
+    *out = nullptr;
+    if (!(sequence_length > 0) || !std::isfinite(sequence_length)) return TSG_ERR_BAD_SEQUENCE_LENGTH;
+    if (num_nodes > INT32_MAX || num_edges > INT32_MAX || num_sites > INT32_MAX ||
+        num_mutations > INT32_MAX) {
+        return TSG_ERR_BAD_PARAM_VALUE;
+    }
+    std::unique_ptr<tsg_treeseq> ts(new (std::nothrow) tsg_treeseq());
+    if (!ts) return TSG_ERR_NO_MEMORY;
+    ts->L = sequence_length;
+    ts->options = options;
+    ts->time_uncalibrated = (options & TSG_TS_TIME_UNCALIBRATED) != 0;
+    ts->num_nodes = (int32_t) num_nodes;
+    ts->num_edges = (int32_t) num_edges;
+    ts->num_sites = (int32_t) num_sites;
+    ts->num_mutations = (int32_t) num_mutations;
+    ts->flags.assign(node_flags, node_flags + num_nodes);
+    ts->time.assign(node_time, node_time + num_nodes);
+    ts->left.assign(edge_left, edge_left + num_edges);
+    ts->right.assign(edge_right, edge_right + num_edges);
+    ts->parent.assign(edge_parent, edge_parent + num_edges);
+    ts->child.assign(edge_child, edge_child + num_edges);
+    ts->site_pos.assign(site_position, site_position + num_sites);
+    ts->mut_site.assign(mutation_site, mutation_site + num_mutations);
+    ts->mut_node.assign(mutation_node, mutation_node + num_mutations);
+    ts->mut_parent.assign(mutation_parent, mutation_parent + num_mutations);
+    ts->I.resize(num_edges);
+    ts->O.resize(num_edges);
+    // validate node references before the index build reads time[parent]
+    for (uint64_t j = 0; j < num_edges; j++) {
+        if (edge_parent[j] < 0 || edge_parent[j] >= (int32_t) num_nodes || edge_child[j] < 0 ||
+            edge_child[j] >= (int32_t) num_nodes) {
+            return (edge_parent[j] == TSG_NULL) ? TSG_ERR_NULL_PARENT
+                   : (edge_child[j] == TSG_NULL) ? TSG_ERR_NULL_CHILD
+                                                 : TSG_ERR_NODE_OUT_OF_BOUNDS;
+        }
+    }
+    if (edge_insertion_order != nullptr && edge_removal_order != nullptr) {
+        std::copy(edge_insertion_order, edge_insertion_order + num_edges, ts->I.begin());
+        std::copy(edge_removal_order, edge_removal_order + num_edges, ts->O.begin());
+    } else {
+        tsg_build_index(edge_left, edge_right, edge_parent, edge_child, node_time,
+                        (int32_t) num_edges, ts->I.data(), ts->O.data());
+    }
+    int ret = tsg_init_trees(ts.get(), ancestral_state, ancestral_state_offset, derived_state,
+                             derived_state_offset);
+    if (ret != 0) return ret;
+    ret = tsg_build_plan(ts.get(), default_num_chunks(ts.get()));
+    if (ret != 0) return ret;
+    *out = ts.release();
+    return 0;
+}
+
+int tsg_treeseq_free(tsg_treeseq *ts) {
+    if (ts == nullptr) return 0;
+    free_workspace(ts);
+    tsg_device_free(&ts->dev);
+    delete ts;
+    return 0;
+}
+
+// Rebuilds the chunk plan with a given chunk count (autotuner hook).
+int tsg_treeseq_set_num_chunks(tsg_treeseq *ts, uint32_t num_chunks) {
+    if (ts == nullptr || num_chunks == 0) return TSG_ERR_BAD_PARAM_VALUE;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    free_workspace(ts);
+    tsg_device_free(&ts->dev);
+    return tsg_build_plan(ts, (int32_t) num_chunks);
+}
+
+int tsg_treeseq_upload(tsg_treeseq *ts) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    return device_upload(ts);
+}
+
+uint64_t tsg_treeseq_get_num_trees(const tsg_treeseq *ts) { return (uint64_t) ts->num_trees; }
+uint64_t tsg_treeseq_get_num_samples(const tsg_treeseq *ts) { return (uint64_t) ts->num_samples; }
+uint64_t tsg_treeseq_get_num_nodes(const tsg_treeseq *ts) { return (uint64_t) ts->num_nodes; }
+uint64_t tsg_treeseq_get_num_edges(const tsg_treeseq *ts) { return (uint64_t) ts->num_edges; }
+uint64_t tsg_treeseq_get_num_sites(const tsg_treeseq *ts) { return (uint64_t) ts->num_sites; }
+uint64_t tsg_treeseq_get_num_mutations(const tsg_treeseq *ts) { return (uint64_t) ts->num_mutations; }
+uint64_t tsg_treeseq_get_num_chunks(const tsg_treeseq *ts) { return (uint64_t) ts->num_chunks; }
+double tsg_treeseq_get_sequence_length(const tsg_treeseq *ts) { return ts->L; }
+uint64_t tsg_treeseq_get_device_bytes(const tsg_treeseq *ts) { return (uint64_t) ts->dev.bytes; }
+uint64_t tsg_treeseq_get_max_chunk_nodes(const tsg_treeseq *ts) { return (uint64_t) ts->max_nloc; }
+
+int tsg_treeseq_get_breakpoints(const tsg_treeseq *ts, double *out) {
+    std::copy(ts->breakpoints.begin(), ts->breakpoints.end(), out);
+    return 0;
+}
+int tsg_treeseq_get_samples(const tsg_treeseq *ts, int32_t *out) {
+    std::copy(ts->samples.begin(), ts->samples.end(), out);
+    return 0;
+}
+int tsg_treeseq_get_index(const tsg_treeseq *ts, int32_t *I, int32_t *O) {
+    std::copy(ts->I.begin(), ts->I.end(), I);
+    std::copy(ts->O.begin(), ts->O.end(), O);
+    return 0;
+}
+
+// Introspection of the chunk plan (host copies), used to check the plan on
+// machines without a GPU.  sizes = {chunks, local nodes, events, trees,
+// sites, mutations}.
+int tsg_treeseq_get_plan_sizes(const tsg_treeseq *ts, uint64_t *sizes) {
+    sizes[0] = (uint64_t) ts->num_chunks;
+    sizes[1] = ts->loc_parent0.size();
+    sizes[2] = ts->ev_child.size();
+    sizes[3] = (uint64_t) ts->num_trees;
+    sizes[4] = (uint64_t) ts->num_sites;
+    sizes[5] = (uint64_t) ts->num_mutations;
+    return 0;
+}
+
+int tsg_treeseq_copy_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *chunk_node0,
+                          int32_t *loc_parent0, float *loc_bl0, double *loc_time,
+                          double *loc_lastrem, int64_t *tr_off, int32_t *tr_nrem,
+                          int32_t *ev_child, int32_t *ev_parent, float *ev_bl,
+                          int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc,
+                          int32_t *mut_allele, int32_t *mut_pallele, int32_t *site_nalleles) {
+    auto cp = [](auto *dst, const auto &v) { std::copy(v.begin(), v.end(), dst); };
+    cp(chunk_tree0, ts->chunk_tree0);
+    cp(chunk_node0, ts->chunk_node0);
+    cp(loc_parent0, ts->loc_parent0);
+    cp(loc_bl0, ts->loc_bl0);
+    cp(loc_time, ts->loc_time);
+    cp(loc_lastrem, ts->loc_lastrem);
+    cp(tr_off, ts->tr_off);
+    cp(tr_nrem, ts->tr_nrem);
+    cp(ev_child, ts->ev_child);
+    cp(ev_parent, ts->ev_parent);
+    cp(ev_bl, ts->ev_bl);
+    cp(tree_site0, ts->tree_site0);
+    cp(site_mut0, ts->site_mut0);
+    cp(mut_node_loc, ts->mut_node_loc);
+    cp(mut_allele, ts->mut_allele);
+    cp(mut_pallele, ts->mut_pallele);
+    cp(site_nalleles, ts->site_nalleles);
+    return 0;
+}
+
+// Profiling of the sweep kernel: HIP events on the library's stream around
+// every sweep launch.
+int tsg_treeseq_set_profiling(tsg_treeseq *ts, int enable) {
+    int ret = tsg_treeseq_upload(ts);
+    if (ret != 0) return ret;
+    Workspace *ws = ws_of(ts);
+    ws->profile = enable != 0;
+    ws->sweep_ms = 0;
+    ws->sweep_launches = 0;
+    return 0;
+}
+int tsg_treeseq_get_profile(const tsg_treeseq *ts, double *sweep_ms, uint64_t *launches) {
+    const Workspace *ws = reinterpret_cast<const Workspace *>(ts->workspace);
+    *sweep_ms = ws ? ws->sweep_ms : 0.0;
+    *launches = ws ? ws->sweep_launches : 0;
+    return 0;
+}
+
+// ---- one-way statistics (trees.h:1098-1110) -----------------------------
+static int one_way(tsg_treeseq *ts, int stat, uint64_t K, const uint64_t *sizes,
+                   const int32_t *sets, uint64_t nw, const double *windows, uint32_t options,
+                   double *result) {
+    Query q;
+    int ret = parse_sample_sets(ts, K, sizes, sets, q.sets);
+    if (ret != 0) return ret;
+    ret = parse_mode_windows(ts, options, nw, windows, q);
+    if (ret != 0) return ret;
+    const bool pol = (options & TSG_STAT_POLARISED) != 0;
+    q.blocks.push_back(Block{stat, 0, {}, pol});
+    q.P = (int) K;
+    q.result = result;
+    return run_one(ts, q);
+}
+
+int tsg_treeseq_diversity(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, const int32_t *sets,
+                          uint64_t nw, const double *windows, uint32_t options, double *result) {
+    return one_way(ts, TSG_STAT_DIVERSITY, K, sizes, sets, nw, windows, options, result);
+}
+int tsg_treeseq_segregating_sites(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes,
+                                  const int32_t *sets, uint64_t nw, const double *windows,
+                                  uint32_t options, double *result) {
+    return one_way(ts, TSG_STAT_SEGREGATING_SITES, K, sizes, sets, nw, windows, options, result);
+}
+int tsg_treeseq_Y1(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, const int32_t *sets,
+                   uint64_t nw, const double *windows, uint32_t options, double *result) {
+    return one_way(ts, TSG_STAT_Y1, K, sizes, sets, nw, windows, options, result);
+}
+
+// ---- k-way statistics (trees.h:1112-1239) -------------------------------
+static int k_way(tsg_treeseq *ts, int stat, int tuple, uint64_t K, const uint64_t *sizes,
+                 const int32_t *sets, uint64_t M, const int32_t *idx, uint64_t nw,
+                 const double *windows, uint32_t options, double *result) {
+    int ret = check_tuples(K, tuple, M, idx);
+    if (ret != 0) return ret;
+    Query q;
+    ret = parse_sample_sets(ts, K, sizes, sets, q.sets);
+    if (ret != 0) return ret;
+    ret = parse_mode_windows(ts, options, nw, windows, q);
+    if (ret != 0) return ret;
+    const bool pol = (options & TSG_STAT_POLARISED) != 0;
+    q.ncentre = (int) K;
+    q.blocks.push_back(Block{stat, tuple, std::vector<int32_t>(idx, idx + M * tuple), pol});
+    q.P = (int) M;
+    q.result = result;
+    return run_one(ts, q);
+}
+
+#define KWAY(name, stat, tuple)                                                                 \
+    int name(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, const int32_t *sets, uint64_t M, \
+             const int32_t *idx, uint64_t nw, const double *windows, uint32_t options,          \
+             double *result) {                                                                  \
+        return k_way(ts, stat, tuple, K, sizes, sets, M, idx, nw, windows, options, result);    \
+    }
+KWAY(tsg_treeseq_divergence, TSG_STAT_DIVERGENCE, 2)
+KWAY(tsg_treeseq_Y2, TSG_STAT_Y2, 2)
+KWAY(tsg_treeseq_f2, TSG_STAT_F2, 2)
+KWAY(tsg_treeseq_Y3, TSG_STAT_Y3, 3)
+KWAY(tsg_treeseq_f3, TSG_STAT_F3, 3)
+KWAY(tsg_treeseq_f4, TSG_STAT_F4, 4)
+#undef KWAY
+
+int tsg_treeseq_genetic_relatedness(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes,
+                                    const int32_t *sets, uint64_t M, const int32_t *idx,
+                                    uint64_t nw, const double *windows, uint32_t options,
+                                    double *result) {
+    const int stat = (options & TSG_STAT_NONCENTRED) ? TSG_STAT_RELATEDNESS_NONCENTRED
+                                                     : TSG_STAT_RELATEDNESS;
+    return k_way(ts, stat, 2, K, sizes, sets, M, idx, nw, windows, options, result);
+}
+
+// ---- composites of the reference's Python layer, fused into one sweep ----
+// Fst (trees.py:10024-10136): diversity of every set and divergence of each
+// pair in one sweep, then 1 - 2(d_u + d_v) / (d_u + d_v + 2 d_uv) in FP64.
+int tsg_treeseq_Fst(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, const int32_t *sets,
+                    uint64_t M, const int32_t *idx, uint64_t nw, const double *windows,
+                    uint32_t options, double *result) {
+    int ret = check_tuples(K, 2, M, idx);
+    if (ret != 0) return ret;
+    Query q;
+    ret = parse_sample_sets(ts, K, sizes, sets, q.sets);
+    if (ret != 0) return ret;
+    ret = parse_mode_windows(ts, options, nw, windows, q);
+    if (ret != 0) return ret;
+    const bool pol = (options & TSG_STAT_POLARISED) != 0;
+    q.blocks.push_back(Block{TSG_STAT_DIVERSITY, 0, {}, pol});
+    q.blocks.push_back(Block{TSG_STAT_DIVERGENCE, 2, std::vector<int32_t>(idx, idx + 2 * M), pol});
+    q.final_kind = TSG_FINAL_FST;
+    q.fin_ipar.assign(idx, idx + 2 * M);
+    q.P = (int) M;
+    q.result = result;
+    return run_one(ts, q);
+}
+
+// Tajima's D (trees.py:9967-10022): diversity and segregating sites, not span
+// normalised, in one sweep, then the closed form in FP64.
+int tsg_treeseq_Tajimas_D(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, const int32_t *sets,
+                          uint64_t nw, const double *windows, uint32_t options, double *result) {
+    Query q;
+    int ret = parse_sample_sets(ts, K, sizes, sets, q.sets);
+    if (ret != 0) return ret;
+    ret = parse_mode_windows(ts, options & ~TSG_STAT_SPAN_NORMALISE, nw, windows, q);
+    if (ret != 0) return ret;
+    q.blocks.push_back(Block{TSG_STAT_DIVERSITY, 0, {}, false});
+    q.blocks.push_back(Block{TSG_STAT_SEGREGATING_SITES, 0, {}, false});
+    q.final_kind = TSG_FINAL_TAJIMAS_D;
+    for (uint64_t k = 0; k < K; k++) {
+        const double n = (double) sizes[k];
+        double h = 0, g = 0;
+        for (uint64_t i = 1; i < sizes[k]; i++) {
+            h += 1.0 / (double) i;
+            g += 1.0 / ((double) i * (double) i);
+        }
+        q.fin_dpar.push_back(n);
+        q.fin_dpar.push_back(h);
+        q.fin_dpar.push_back(g);
+    }
+    q.P = (int) K;
+    q.result = result;
+    return run_one(ts, q);
+}
+
+// genetic_relatedness with proportion=True (trees.py:8830-8976): the
+// relatedness of each pair divided by the (unpolarised) segregating sites of
+// the union of all sets, both from one sweep.
+int tsg_treeseq_genetic_relatedness_proportion(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes,
+                                               const int32_t *sets, uint64_t M,
+                                               const int32_t *idx, uint64_t nw,
+                                               const double *windows, uint32_t options,
+                                               double *result) {
+    int ret = check_tuples(K, 2, M, idx);
+    if (ret != 0) return ret;
+    Query q;
+    ret = parse_sample_sets(ts, K, sizes, sets, q.sets);
+    if (ret != 0) return ret;
+    ret = parse_mode_windows(ts, options, nw, windows, q);
+    if (ret != 0) return ret;
+    const bool pol = (options & TSG_STAT_POLARISED) != 0;
+    const int stat = (options & TSG_STAT_NONCENTRED) ? TSG_STAT_RELATEDNESS_NONCENTRED
+                                                     : TSG_STAT_RELATEDNESS;
+    q.ncentre = (int) K;
+    std::vector<char> inunion(ts->num_samples, 0);
+    std::vector<int32_t> uni;
+    for (const auto &s : q.sets) {
+        for (int32_t si : s) {
+            if (!inunion[si]) {
+                inunion[si] = 1;
+                uni.push_back(si);
+            }
+        }
+    }
+    q.sets.push_back(uni);
+    q.blocks.push_back(Block{stat, 2, std::vector<int32_t>(idx, idx + 2 * M), pol});
+    // segregating sites of the union, a one-output block on column K
+    q.blocks.push_back(Block{TSG_STAT_SEGREGATING_SITES, 1, {(int32_t) K}, false});
+    q.final_kind = TSG_FINAL_PROPORTION;
+    q.P = (int) M;
+    q.result = result;
+    return run_one(ts, q);
+}
+
+// ---- allele frequency spectrum (trees.c:3815-3928) -----------------------
+#define TSG_AFS_MAX_SETS 16
+
+int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes,
+                                          const int32_t *sets, uint64_t nw,
+                                          const double *windows, uint64_t ntw,
+                                          const double *time_windows, uint32_t options,
+                                          double *result) {
+    bool site = options & TSG_STAT_SITE, branch = options & TSG_STAT_BRANCH;
+    if (options & TSG_STAT_NODE) return TSG_ERR_UNSUPPORTED_STAT_MODE;
+    if (!(site || branch)) site = true;
+    if (site && branch) return TSG_ERR_MULTIPLE_STAT_MODES;
+    std::vector<double> win;
+    if (windows == nullptr) {
+        win = {0.0, ts->L};
+    } else {
+        int r = check_windows(ts, nw, windows);
+        if (r != 0) return r;
+        win.assign(windows, windows + nw + 1);
+    }
+    std::vector<double> tw;
+    if (time_windows == nullptr) {
+        tw = {0.0, INFINITY};
+    } else {
+        if (ntw < 1) return TSG_ERR_BAD_TIME_WINDOWS_DIM;
+        if (time_windows[0] != 0.0) return TSG_ERR_BAD_TIME_WINDOWS;
+        for (uint64_t j = 0; j < ntw; j++) {
+            if (time_windows[j] >= time_windows[j + 1]) return TSG_ERR_BAD_TIME_WINDOWS;
+        }
+        if (site && !(time_windows[0] == 0.0 && std::isinf((float) time_windows[1]))) {
+            return TSG_ERR_UNSUPPORTED_STAT_MODE;
+        }
+        tw.assign(time_windows, time_windows + ntw + 1);
+    }
+    std::vector<std::vector<int32_t>> ss;
+    int ret = parse_sample_sets(ts, K, sizes, sets, ss);
+    if (ret != 0) return ret;
+    if (branch && ts->time_uncalibrated && !(options & TSG_STAT_ALLOW_TIME_UNCALIBRATED)) {
+        return TSG_ERR_TIME_UNCALIBRATED;
+    }
+    if (K > TSG_AFS_MAX_SETS) return TSG_ERR_UNSUPPORTED_INPUT;
+    const int W = (int) win.size() - 1, TW = (int) tw.size() - 1;
+    std::vector<int64_t> dims(K);
+    double afs_size_d = 1;
+    for (uint64_t k = 0; k < K; k++) {
+        dims[k] = (int64_t) sizes[k] + 1;
+        afs_size_d *= (double) dims[k];
+    }
+    if (afs_size_d * W * TW > 4e9) return TSG_ERR_UNSUPPORTED_INPUT;
+    const int64_t afs_size = (int64_t) afs_size_d;
+
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if ((ret = device_upload(ts))) return ret;
+    Workspace *ws = ws_of(ts);
+    hipStream_t s = ws->stream;
+    const int C = ts->num_chunks;
+    const int KK = (int) K + 1;
+    std::vector<int32_t> pair_s, pair_c, col_total;
+    for (uint64_t k = 0; k < K; k++) {
+        for (int32_t si : ss[k]) {
+            pair_s.push_back(si);
+            pair_c.push_back((int32_t) k);
+        }
+        col_total.push_back((int32_t) ss[k].size());
+    }
+    for (int32_t si = 0; si < ts->num_samples; si++) {
+        pair_s.push_back(si);
+        pair_c.push_back((int32_t) K);
+    }
+    col_total.push_back(ts->num_samples);
+    std::vector<int32_t> cq_wfirst(C), cq_nwin(C);
+    std::vector<int64_t> cq_poff(C, 0);
+    for (int c = 0; c < C; c++) {
+        const double cs = ts->breakpoints[ts->chunk_tree0[c]];
+        const double ce = ts->breakpoints[ts->chunk_tree0[c + 1]];
+        const int wf = (int) (std::upper_bound(win.begin(), win.end(), cs) - win.begin()) - 1;
+        const int wl = (int) (std::lower_bound(win.begin(), win.end(), ce) - win.begin()) - 1;
+        cq_wfirst[c] = wf;
+        cq_nwin[c] = wl - wf + 1;
+    }
+    const int64_t hist_n = (int64_t) W * TW * afs_size;
+    if ((ret = upload(ws->pair_s, pair_s, s)) || (ret = upload(ws->pair_c, pair_c, s)) ||
+        (ret = upload(ws->tot, col_total, s)) || (ret = upload(ws->windows, win, s)) ||
+        (ret = upload(ws->twin, tw, s)) || (ret = upload(ws->dims, dims, s)) ||
+        (ret = upload(ws->cq_wfirst, cq_wfirst, s)) || (ret = upload(ws->cq_nwin, cq_nwin, s)) ||
+        (ret = upload(ws->cq_poff, cq_poff, s)) || (ret = ensure(ws->hist, (size_t) hist_n * 8)) ||
+        (ret = ensure(ws->out, (size_t) hist_n * 8))) {
+        return ret;
+    }
+    const int R = 2 + KK;
+    const size_t stage = (size_t) 2 * KK * 4 + 64;
+    size_t lds_need = (size_t) ts->max_nloc * R * 4 + stage;
+    if (branch) lds_need += (size_t) ts->max_nloc * 8;
+    const bool lds = lds_need <= (size_t) ws->lds_limit;
+    if (!lds) {
+        if ((ret = ensure(ws->slab, (size_t) C * ts->max_nloc * R * 4))) return ret;
+        if (branch && (ret = ensure(ws->last, (size_t) C * ts->max_nloc * 8))) return ret;
+    }
+    const TsgDevice &d = ts->dev;
+    TsgSweepArgs a{};
+    a.chunk_tree0 = d.chunk_tree0;
+    a.chunk_node0 = d.chunk_node0;
+    a.loc_parent0 = d.loc_parent0;
+    a.loc_bl0 = d.loc_bl0;
+    a.loc_time = d.loc_time;
+    a.loc_lastrem = d.loc_lastrem;
+    a.tr_off = d.tr_off;
+    a.tr_nrem = d.tr_nrem;
+    a.ev_child = d.ev_child;
+    a.ev_parent = d.ev_parent;
+    a.ev_bl = d.ev_bl;
+    a.bp = d.breakpoints;
+    a.tree_site0 = d.tree_site0;
+    a.site_pos = d.site_pos;
+    a.site_mut0 = d.site_mut0;
+    a.mut_node = d.mut_node;
+    a.mut_allele = d.mut_allele;
+    a.mut_pallele = d.mut_pallele;
+    a.site_nalleles = d.site_nalleles;
+    a.num_trees = ts->num_trees;
+    a.num_chunks = C;
+    a.num_samples = ts->num_samples;
+    a.K = KK;
+    a.Q = 1;
+    a.npairs = (int32_t) pair_s.size();
+    a.pair_sample = (const int32_t *) ws->pair_s.p;
+    a.pair_col = (const int32_t *) ws->pair_c.p;
+    a.col_total = (const int32_t *) ws->tot.p;
+    a.windows = (const double *) ws->windows.p;
+    a.cq_wfirst = (const int32_t *) ws->cq_wfirst.p;
+    a.cq_nwin = (const int32_t *) ws->cq_nwin.p;
+    a.cq_poff = (const int64_t *) ws->cq_poff.p;
+    a.flags = (options & TSG_STAT_POLARISED) ? TSG_F_POLARISED : 0u;
+    a.g_rec = (int32_t *) ws->slab.p;
+    a.g_stride = (int64_t) ts->max_nloc * R;
+    a.g_last = (double *) ws->last.p;
+    a.g_last_stride = ts->max_nloc;
+    a.afs_K = (int32_t) K;
+    a.afs_size = afs_size;
+    a.afs_dims = (const int64_t *) ws->dims.p;
+    a.TW = TW;
+    a.time_windows = (const double *) ws->twin.p;
+    a.afs_hist_u = (unsigned long long *) ws->hist.p;
+    a.afs_hist_d = (double *) ws->hist.p;
+    const uint32_t fflags = (options & TSG_STAT_SPAN_NORMALISE) ? TSG_F_SPAN_NORMALISE : 0u;
+    if (branch) {
+        if ((ret = hip_check(tsg_launch_fill_f64((double *) ws->hist.p, hist_n, s)))) return ret;
+    } else {
+        if ((ret = hip_check(tsg_launch_fill_u64((unsigned long long *) ws->hist.p, hist_n, s)))) {
+            return ret;
+        }
+    }
+    if (ws->profile) (void) hipEventRecord(ws->ev0, s);
+    if ((ret = hip_check(tsg_launch_afs(a, branch, lds, lds ? lds_need : stage, s)))) return ret;
+    if (ws->profile) (void) hipEventRecord(ws->ev1, s);
+    double *res_dev;
+    if (branch) {
+        if ((ret = hip_check(tsg_launch_afs_branch_final((double *) ws->hist.p, a.windows, W,
+                                                         (int64_t) TW * afs_size, fflags, s)))) {
+            return ret;
+        }
+        res_dev = (double *) ws->hist.p;
+    } else {
+        if ((ret = hip_check(tsg_launch_afs_site_final((const unsigned long long *) ws->hist.p,
+                                                       (double *) ws->out.p, a.windows, W,
+                                                       afs_size, fflags, s)))) {
+            return ret;
+        }
+        res_dev = (double *) ws->out.p;
+    }
+    if ((ret = hip_check(hipMemcpyAsync(result, res_dev, (size_t) hist_n * 8,
+                                        hipMemcpyDeviceToHost, s)))) {
+        return ret;
+    }
+    if ((ret = hip_check(hipStreamSynchronize(s)))) return ret;
+    if (ws->profile) {
+        float ms = 0;
+        (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
+        ws->sweep_ms += ms;
+        ws->sweep_launches += 1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Device-side argument blocks shared by the HIP kernels and their host
+// launchers.  Plain structs of pointers and sizes; no torch types.
+#pragma once
+#include <cstdint>
+
+// One output of a fused query: a reference summary function applied to the
+// state columns named by i..l (columns of the fused state vector).
+struct TsgOut {
+    int32_t stat;        // tsg_stat_id
+    int32_t q;           // query this output belongs to
+    int32_t i, j, k, l;  // state columns
+    int32_t ni, nj, nk, nl;  // sample-set sizes of those columns
+    int32_t cbase, ccount;   // the query's own columns (relatedness mean)
+    int32_t flags;       // bit 0: polarised
+    int32_t pad;
+    double inv;          // 1 / denominator of the summary function
+};
+
+// One query of a batch: its windows and where its outputs and partial sums go.
+struct TsgQuery {
+    int32_t W;           // number of windows
+    int32_t M;           // number of raw outputs
+    int32_t mbase;       // first output index in the batch
+    int32_t cbase;       // first state column
+    int64_t win_off;     // offset of its windows in the concatenated array
+    int64_t pbase;       // offset (in doubles) of its partial-sum block
+};
+
+#define TSG_F_POLARISED 1u
+#define TSG_F_NOSKIP 2u       // a summary has a zero denominator: never skip walks
+#define TSG_F_SPAN_NORMALISE 4u
+#define TSG_F_ANY_UNPOLARISED 8u  // some output sums over the ancestral allele
+
+struct TsgSweepArgs {
+    // tree sequence program
+    const int32_t *chunk_tree0;
+    const int64_t *chunk_node0;
+    const int32_t *loc_parent0;
+    const float *loc_bl0;
+    const double *loc_time;
+    const double *loc_lastrem;   // last removal position before the chunk start
+    const int64_t *tr_off;
+    const int32_t *tr_nrem;
+    const int32_t *ev_child, *ev_parent;
+    const float *ev_bl;
+    const double *bp;
+    const int32_t *tree_site0;
+    const double *site_pos;
+    const int32_t *site_mut0, *mut_node, *mut_allele, *mut_pallele, *site_nalleles;
+    int32_t num_trees, num_chunks, num_samples;
+    // batch
+    int32_t K, M, Q, npairs;
+    int32_t m_begin, m_end;      // outputs handled by this launch
+    const int32_t *pair_sample, *pair_col;
+    const int32_t *col_total;    // [K] total weight (set size) per column
+    const TsgOut *outs;          // [M]
+    const TsgQuery *queries;     // [Q]
+    const double *windows;       // concatenated
+    const int32_t *cq_wfirst;    // [C*Q] first window touched by chunk c
+    const int32_t *cq_nwin;      // [C*Q] number of windows touched
+    const int64_t *cq_poff;      // [C*Q] window-slot offset within query block
+    double *partial;             // per (query, chunk, window) partial sums
+    double *dR;                  // [C*M] branch running-sum change over chunk
+    uint32_t flags;
+    // global-memory node records (when they do not fit in LDS)
+    int32_t *g_rec;
+    int64_t g_stride;            // ints per chunk slab
+    double *g_last;              // AFS branch: last update position per node
+    int64_t g_last_stride;
+    // AFS
+    int32_t afs_K;               // number of AFS sample sets (state has afs_K+1 cols)
+    int64_t afs_size;            // prod(n_k + 1)
+    const int64_t *afs_dims;     // [afs_K]
+    int32_t TW;
+    const double *time_windows;  // [TW+1]
+    unsigned long long *afs_hist_u;  // site AFS: half-unit counts
+    double *afs_hist_d;          // branch AFS
+};
+
+struct TsgCombineArgs {
+    const double *bp;
+    const int32_t *chunk_tree0;
+    int32_t num_chunks;
+    int32_t Q, M;
+    const TsgQuery *queries;
+    const double *windows;
+    const int32_t *cq_wfirst, *cq_nwin;
+    const int64_t *cq_poff;
+    const double *partial;
+    const double *Rstart;        // [C*M] (branch) or null
+    double *raw;                 // per query [W x M_q], offset rawbase[q]
+    const int64_t *rawbase;
+    uint32_t flags;
+};

@@ -1,0 +1,32 @@
+// Host-callable launchers for the kernels in tsg_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "tsg_kernels.h"
+
+hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, bool lds, int mpl, size_t shmem,
+                            hipStream_t s);
+hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t shmem,
+                          hipStream_t s);
+hipError_t tsg_launch_combine(const TsgCombineArgs &a, const double *dR, double *Rstart,
+                              int64_t total, hipStream_t s);
+hipError_t tsg_launch_finals(const void *fin, int nq, const double *raw, const int32_t *ipar,
+                             const double *dpar, double *out, int64_t total, hipStream_t s);
+hipError_t tsg_launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);
+hipError_t tsg_launch_fill_f64(double *p, int64_t n, hipStream_t s);
+hipError_t tsg_launch_afs_site_final(const unsigned long long *h, double *out, const double *win,
+                                     int W, int64_t afs_size, uint32_t flags, hipStream_t s);
+hipError_t tsg_launch_afs_branch_final(double *h, const double *win, int W, int64_t per_window,
+                                       uint32_t flags, hipStream_t s);
+size_t tsg_final_struct_size();
+
+// Mirrors TsgFinal in tsg_kernels.hip.
+struct TsgFinalHost {
+    int32_t kind;
+    int32_t Kq;
+    int32_t P;
+    int32_t W;
+    int64_t rawbase, outbase;
+    int64_t pbase;
+    int32_t Mraw;
+};

@@ -1,0 +1,369 @@
+// Host side of the device-resident tree sequence: validation, edge indexes,
+// tree breakpoints and per-tree site ranges, allele encoding, and the chunk
+// plan (per-chunk "sweep programs") that the HIP kernels execute.
+//
+// Reference behaviour followed (restated, not copied):
+//   integrity checks     c/tskit/tables.c:10397-10498 (edges), 10508-10545
+//                        (sites), 10548-10640 (mutations), 10790-10850 (trees)
+//   index build          c/tskit/tables.c:10286-10310, 11393-11461
+//   samples              c/tskit/trees.c:404-453
+//   breakpoints / sites  c/tskit/trees.c:243-359
+//   allele table         c/tskit/trees.c:1525-1612 (allele order = first seen)
+#include "tsg_treeseq.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string_view>
+
+namespace {
+
+int check_tables(const tsg_treeseq *ts) {
+    const int32_t N = ts->num_nodes;
+    for (int32_t u = 0; u < N; u++) {
+        if (!std::isfinite(ts->time[u])) return TSG_ERR_TIME_NONFINITE;
+    }
+    std::vector<char> parent_seen(N, 0);
+    for (int32_t j = 0; j < ts->num_edges; j++) {
+        const int32_t p = ts->parent[j], c = ts->child[j];
+        const double l = ts->left[j], r = ts->right[j];
+        if (p == TSG_NULL) return TSG_ERR_NULL_PARENT;
+        if (p < 0 || p >= N) return TSG_ERR_NODE_OUT_OF_BOUNDS;
+        if (c == TSG_NULL) return TSG_ERR_NULL_CHILD;
+        if (c < 0 || c >= N) return TSG_ERR_NODE_OUT_OF_BOUNDS;
+        if (!(std::isfinite(l) && std::isfinite(r))) return TSG_ERR_GENOME_COORDS_NONFINITE;
+        if (l < 0) return TSG_ERR_LEFT_LESS_ZERO;
+        if (r > ts->L) return TSG_ERR_RIGHT_GREATER_SEQ_LENGTH;
+        if (l >= r) return TSG_ERR_BAD_EDGE_INTERVAL;
+        if (ts->time[c] >= ts->time[p]) return TSG_ERR_BAD_NODE_TIME_ORDERING;
+        if (parent_seen[p]) return TSG_ERR_EDGES_NONCONTIGUOUS_PARENTS;
+        if (j > 0) {
+            const int32_t lp = ts->parent[j - 1], lc = ts->child[j - 1];
+            if (ts->time[p] < ts->time[lp]) return TSG_ERR_EDGES_NOT_SORTED_PARENT_TIME;
+            if (ts->time[p] == ts->time[lp]) {
+                if (p == lp) {
+                    if (c < lc) return TSG_ERR_EDGES_NOT_SORTED_CHILD;
+                    if (c == lc && l <= ts->left[j - 1]) return TSG_ERR_EDGES_NOT_SORTED_LEFT;
+                } else {
+                    parent_seen[lp] = 1;
+                }
+            }
+        }
+    }
+    for (int32_t j = 0; j < ts->num_sites; j++) {
+        const double x = ts->site_pos[j];
+        if (!std::isfinite(x) || x < 0 || x >= ts->L) return TSG_ERR_BAD_SITE_POSITION;
+        if (j > 0) {
+            if (ts->site_pos[j - 1] == x) return TSG_ERR_DUPLICATE_SITE_POSITION;
+            if (ts->site_pos[j - 1] > x) return TSG_ERR_UNSORTED_SITES;
+        }
+    }
+    for (int32_t j = 0; j < ts->num_mutations; j++) {
+        const int32_t s = ts->mut_site[j], u = ts->mut_node[j], pm = ts->mut_parent[j];
+        if (s < 0 || s >= ts->num_sites) return TSG_ERR_SITE_OUT_OF_BOUNDS;
+        if (u < 0 || u >= N) return TSG_ERR_NODE_OUT_OF_BOUNDS;
+        if (pm < TSG_NULL || pm >= ts->num_mutations) return TSG_ERR_MUTATION_OUT_OF_BOUNDS;
+        if (pm == j) return TSG_ERR_MUTATION_PARENT_EQUAL;
+        if (pm != TSG_NULL) {
+            if (ts->mut_site[pm] != s) return TSG_ERR_MUTATION_PARENT_DIFFERENT_SITE;
+            if (pm > j) return TSG_ERR_MUTATION_PARENT_AFTER_CHILD;
+        }
+        if (j > 0 && ts->mut_site[j - 1] > s) return TSG_ERR_UNSORTED_MUTATIONS;
+    }
+    return 0;
+}
+
+}  // namespace
+
+// Insertion / removal orders (tables.c:11393-11461).
+void tsg_build_index(const double *left, const double *right, const int32_t *parent,
+                     const int32_t *child, const double *time, int32_t E, int32_t *I,
+                     int32_t *O) {
+    std::vector<int32_t> idx(E);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+        if (left[a] != left[b]) return left[a] < left[b];
+        const double ta = time[parent[a]], tb = time[parent[b]];
+        if (ta != tb) return ta < tb;
+        if (parent[a] != parent[b]) return parent[a] < parent[b];
+        return child[a] < child[b];
+    });
+    std::copy(idx.begin(), idx.end(), I);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+        if (right[a] != right[b]) return right[a] < right[b];
+        const double ta = time[parent[a]], tb = time[parent[b]];
+        if (ta != tb) return ta > tb;
+        if (parent[a] != parent[b]) return parent[a] > parent[b];
+        return child[a] > child[b];
+    });
+    std::copy(idx.begin(), idx.end(), O);
+}
+
+// Fills samples, breakpoints, per-tree site ranges and allele encoding, and
+// checks the index describes a valid sequence of trees.
+int tsg_init_trees(tsg_treeseq *ts, const char *ancestral_state,
+                   const uint64_t *ancestral_state_offset, const char *derived_state,
+                   const uint64_t *derived_state_offset) {
+    int ret = check_tables(ts);
+    if (ret != 0) return ret;
+    const int32_t N = ts->num_nodes, E = ts->num_edges;
+    ts->sample_index_map.assign(N, TSG_NULL);
+    ts->samples.clear();
+    for (int32_t u = 0; u < N; u++) {
+        if (ts->flags[u] & TSG_NODE_IS_SAMPLE) {
+            ts->sample_index_map[u] = (int32_t) ts->samples.size();
+            ts->samples.push_back(u);
+        }
+    }
+    ts->num_samples = (int32_t) ts->samples.size();
+    for (int32_t j = 0; j < E; j++) {
+        if (ts->I[j] < 0 || ts->I[j] >= E || ts->O[j] < 0 || ts->O[j] >= E) {
+            return TSG_ERR_EDGE_OUT_OF_BOUNDS;
+        }
+    }
+    // tree sweep: breakpoints + contradictory-children check
+    std::vector<int32_t> par(N, TSG_NULL);
+    std::vector<uint8_t> used(E, 0);
+    ts->breakpoints.clear();
+    ts->tree_site0.clear();
+    int32_t j = 0, k = 0, site = 0;
+    double tl = 0;
+    while (j < E || tl < ts->L) {
+        ts->breakpoints.push_back(tl);
+        ts->tree_site0.push_back(site);
+        while (k < E && ts->right[ts->O[k]] == tl) {
+            const int32_t e = ts->O[k];
+            if (used[e] != 1) return -707;  // TSK_ERR_TABLES_BAD_INDEXES
+            used[e] = 2;
+            par[ts->child[e]] = TSG_NULL;
+            k++;
+        }
+        while (j < E && ts->left[ts->I[j]] == tl) {
+            const int32_t e = ts->I[j];
+            if (used[e] != 0) return -707;
+            used[e] = 1;
+            if (par[ts->child[e]] != TSG_NULL) return TSG_ERR_BAD_EDGES_CONTRADICTORY_CHILDREN;
+            par[ts->child[e]] = ts->parent[e];
+            j++;
+        }
+        double tr = ts->L;
+        if (j < E) tr = std::min(tr, ts->left[ts->I[j]]);
+        if (k < E) tr = std::min(tr, ts->right[ts->O[k]]);
+        while (site < ts->num_sites && ts->site_pos[site] < tr) site++;
+        tl = tr;
+    }
+    ts->num_trees = (int32_t) ts->breakpoints.size();
+    ts->breakpoints.push_back(ts->L);
+    ts->tree_site0.push_back(site);
+
+    // per-site mutation ranges and allele indexes
+    const int32_t S = ts->num_sites, Mu = ts->num_mutations;
+    ts->site_mut0.assign(S + 1, 0);
+    for (int32_t m = 0; m < Mu; m++) ts->site_mut0[ts->mut_site[m] + 1]++;
+    for (int32_t s = 0; s < S; s++) ts->site_mut0[s + 1] += ts->site_mut0[s];
+    ts->mut_allele.assign(Mu, 0);
+    ts->mut_pallele.assign(Mu, 0);
+    ts->site_nalleles.assign(S, 1);
+    ts->max_alleles = 1;
+    std::vector<std::string_view> alleles;
+    auto dstate = [&](int32_t m) {
+        if (derived_state == nullptr) return std::string_view("1", 1);
+        return std::string_view(derived_state + derived_state_offset[m],
+                                derived_state_offset[m + 1] - derived_state_offset[m]);
+    };
+    for (int32_t s = 0; s < S; s++) {
+        alleles.clear();
+        if (ancestral_state == nullptr) {
+            alleles.emplace_back("0", 1);
+        } else {
+            alleles.emplace_back(ancestral_state + ancestral_state_offset[s],
+                                 ancestral_state_offset[s + 1] - ancestral_state_offset[s]);
+        }
+        auto find = [&](std::string_view v) -> int32_t {
+            for (size_t a = 0; a < alleles.size(); a++) {
+                if (alleles[a] == v) return (int32_t) a;
+            }
+            return -1;
+        };
+        for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) {
+            std::string_view d = dstate(m);
+            int32_t a = find(d);
+            if (a < 0) {
+                a = (int32_t) alleles.size();
+                alleles.push_back(d);
+            }
+            ts->mut_allele[m] = a;
+            int32_t pa = 0;
+            if (ts->mut_parent[m] != TSG_NULL) pa = find(dstate(ts->mut_parent[m]));
+            ts->mut_pallele[m] = pa;
+        }
+        ts->site_nalleles[s] = (int32_t) alleles.size();
+        ts->max_alleles = std::max(ts->max_alleles, (int32_t) alleles.size());
+    }
+    return 0;
+}
+
+// Builds the per-chunk sweep programs.  Chunk c covers trees
+// [chunk_tree0[c], chunk_tree0[c+1]).  Its program is
+//   * a local numbering of every node it touches (samples first, so local id
+//     j is sample index j in every chunk),
+//   * the parent / branch length of each local node in its start tree
+//     (chunk 0 starts from the empty forest and owns the opening transition),
+//   * the transitions it applies: into each of its trees after the first,
+//     and the closing transition into the next chunk's first tree (so the
+//     running branch sum at the end of chunk c equals the one at the start
+//     of chunk c+1).
+int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
+    const int32_t T = ts->num_trees, E = ts->num_edges, N = ts->num_nodes;
+    const int32_t n = ts->num_samples;
+    // events per transition
+    std::vector<int64_t> tr_events(T + 1, 0);
+    {
+        int32_t j = 0, k = 0;
+        for (int32_t t = 0; t < T; t++) {
+            const double x = ts->breakpoints[t];
+            while (k < E && ts->right[ts->O[k]] == x) {
+                tr_events[t]++;
+                k++;
+            }
+            while (j < E && ts->left[ts->I[j]] == x) {
+                tr_events[t]++;
+                j++;
+            }
+        }
+    }
+    int64_t total_events = 0;
+    for (int32_t t = 0; t < T; t++) total_events += tr_events[t];
+    int32_t C = std::max(1, std::min(num_chunks, T));
+    ts->chunk_tree0.clear();
+    ts->chunk_tree0.push_back(0);
+    {
+        const double per = (double) total_events / C;
+        int64_t acc = 0;
+        int32_t c = 1;
+        for (int32_t t = 0; t < T && c < C; t++) {
+            acc += tr_events[t];
+            if ((double) acc >= per * c && t + 1 < T) {
+                if (ts->chunk_tree0.back() < t + 1) {
+                    ts->chunk_tree0.push_back(t + 1);
+                }
+                c++;
+            }
+        }
+    }
+    ts->chunk_tree0.push_back(T);
+    C = (int32_t) ts->chunk_tree0.size() - 1;
+    ts->num_chunks = C;
+
+    ts->tr_off.assign(T + 1, 0);
+    ts->tr_nrem.assign(T, 0);
+    ts->ev_child.clear();
+    ts->ev_parent.clear();
+    ts->ev_bl.clear();
+    ts->loc_parent0.clear();
+    ts->loc_bl0.clear();
+    ts->loc_time.clear();
+    ts->loc_lastrem.clear();
+    ts->chunk_node0.assign(C + 1, 0);
+    ts->mut_node_loc.assign(ts->num_mutations, TSG_NULL);
+    ts->max_nloc = 0;
+    ts->max_chunk_events = 0;
+
+    std::vector<int32_t> gpar(N, TSG_NULL);
+    std::vector<double> glastrem(N, 0.0);  // last removal position per node
+    std::vector<double> loc_lastrem;
+    std::vector<int32_t> alive;          // children with a parent
+    std::vector<int32_t> alive_pos(N, -1);
+    std::vector<int32_t> stamp(N, -1), locid(N, -1);
+    std::vector<int32_t> loc2glob;
+    std::vector<int32_t> snap_parent;
+    std::vector<float> snap_bl;
+    int32_t j = 0, k = 0;  // index cursors (insertion / removal)
+
+    for (int32_t c = 0; c < C; c++) {
+        loc2glob.clear();
+        snap_parent.clear();
+        snap_bl.clear();
+        loc_lastrem.clear();
+        auto ensure = [&](int32_t u) -> int32_t {
+            if (stamp[u] != c) {
+                stamp[u] = c;
+                locid[u] = (int32_t) loc2glob.size();
+                loc2glob.push_back(u);
+                loc_lastrem.push_back(glastrem[u]);
+                snap_parent.push_back(TSG_NULL);
+                snap_bl.push_back(0.0f);
+            }
+            return locid[u];
+        };
+        for (int32_t s = 0; s < n; s++) ensure(ts->samples[s]);
+        if (c > 0) {
+            for (int32_t u : alive) {
+                const int32_t lu = ensure(u);
+                const int32_t lp = ensure(gpar[u]);
+                snap_parent[lu] = lp;
+                snap_bl[lu] = (float) (ts->time[gpar[u]] - ts->time[u]);
+            }
+        }
+        const int32_t t0 = ts->chunk_tree0[c], t1 = ts->chunk_tree0[c + 1];
+        // transitions owned by this chunk
+        int32_t tfirst = (c == 0) ? 0 : t0 + 1;
+        int32_t tlast = std::min(t1, T - 1);  // inclusive; t1 is the closing one
+        int64_t chunk_events = 0;
+        for (int32_t t = tfirst; t <= tlast; t++) {
+            ts->tr_off[t] = (int64_t) ts->ev_child.size();
+            const double x = ts->breakpoints[t];
+            int32_t nrem = 0;
+            while (k < E && ts->right[ts->O[k]] == x) {
+                const int32_t e = ts->O[k];
+                const int32_t u = ts->child[e];
+                ts->ev_child.push_back(ensure(u));
+                ts->ev_parent.push_back(ensure(ts->parent[e]));
+                ts->ev_bl.push_back(0.0f);
+                gpar[u] = TSG_NULL;
+                glastrem[u] = x;
+                // swap-remove from alive
+                const int32_t pos = alive_pos[u];
+                alive[pos] = alive.back();
+                alive_pos[alive[pos]] = pos;
+                alive.pop_back();
+                alive_pos[u] = -1;
+                nrem++;
+                k++;
+            }
+            while (j < E && ts->left[ts->I[j]] == x) {
+                const int32_t e = ts->I[j];
+                const int32_t u = ts->child[e], p = ts->parent[e];
+                ts->ev_child.push_back(ensure(u));
+                ts->ev_parent.push_back(ensure(p));
+                ts->ev_bl.push_back((float) (ts->time[p] - ts->time[u]));
+                gpar[u] = p;
+                alive_pos[u] = (int32_t) alive.size();
+                alive.push_back(u);
+                j++;
+            }
+            ts->tr_nrem[t] = nrem;
+            chunk_events += (int64_t) ts->ev_child.size() - ts->tr_off[t];
+        }
+        ts->max_chunk_events = (int32_t) std::max<int64_t>(ts->max_chunk_events, chunk_events);
+        // mutation nodes of this chunk's sites
+        for (int32_t s = ts->tree_site0[t0]; s < ts->tree_site0[t1]; s++) {
+            for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) {
+                ts->mut_node_loc[m] = ensure(ts->mut_node[m]);
+            }
+        }
+        const int32_t nloc = (int32_t) loc2glob.size();
+        ts->chunk_node0[c + 1] = ts->chunk_node0[c] + nloc;
+        ts->max_nloc = std::max(ts->max_nloc, nloc);
+        ts->loc_parent0.insert(ts->loc_parent0.end(), snap_parent.begin(), snap_parent.end());
+        ts->loc_bl0.insert(ts->loc_bl0.end(), snap_bl.begin(), snap_bl.end());
+        for (int32_t i = 0; i < nloc; i++) ts->loc_time.push_back(ts->time[loc2glob[i]]);
+        ts->loc_lastrem.insert(ts->loc_lastrem.end(), loc_lastrem.begin(), loc_lastrem.end());
+    }
+    // Every transition 0..T-1 is owned by exactly one chunk and they were
+    // emitted in order, so the offsets are already monotone.
+    ts->tr_off[T] = (int64_t) ts->ev_child.size();
+    return 0;
+}

@@ -1,0 +1,83 @@
+// Host-side tree sequence and the per-chunk sweep programs uploaded to the GPU.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tsg_common.h"
+
+// Query-independent data the sweep kernels read, resident in HBM.
+struct TsgDevice {
+    int device = -1;
+    // per chunk [C+1]
+    int32_t *chunk_tree0 = nullptr;   // first tree of each chunk (last entry = T)
+    int64_t *chunk_node0 = nullptr;   // offset of each chunk's local-node block
+    // per local node [sum nloc]
+    int32_t *loc_parent0 = nullptr;   // parent (local id) in the chunk's start tree
+    float *loc_bl0 = nullptr;         // branch length in the start tree (0 if root)
+    double *loc_time = nullptr;       // node time
+    double *loc_lastrem = nullptr;    // position of the node's last edge removal
+    // per transition [T+1]: events of transition t are [tr_off[t], tr_off[t+1]),
+    // the first tr_nrem[t] of them removals (in removal-index order), then
+    // insertions (in insertion-index order)
+    int64_t *tr_off = nullptr;
+    int32_t *tr_nrem = nullptr;
+    int32_t *ev_child = nullptr;      // local ids in the owning chunk
+    int32_t *ev_parent = nullptr;
+    float *ev_bl = nullptr;           // branch length of inserted edge
+    double *breakpoints = nullptr;    // [T+1]
+    int32_t *tree_site0 = nullptr;    // [T+1]
+    double *site_pos = nullptr;       // [S]
+    int32_t *site_mut0 = nullptr;     // [S+1]
+    int32_t *mut_node = nullptr;      // [Mu] local id in owning chunk
+    int32_t *mut_allele = nullptr;    // [Mu] allele index of the derived state
+    int32_t *mut_pallele = nullptr;   // [Mu] allele index it replaces
+    int32_t *site_nalleles = nullptr; // [S]
+    size_t bytes = 0;
+};
+
+struct tsg_treeseq {
+    // tables (host copies)
+    double L = 0;
+    uint32_t options = 0;
+    int32_t num_nodes = 0, num_edges = 0, num_sites = 0, num_mutations = 0;
+    int32_t num_samples = 0, num_trees = 0;
+    std::vector<uint32_t> flags;
+    std::vector<double> time;
+    std::vector<double> left, right;
+    std::vector<int32_t> parent, child;
+    std::vector<int32_t> I, O;
+    std::vector<double> site_pos;
+    std::vector<int32_t> mut_site, mut_node, mut_parent;
+    std::vector<int32_t> samples, sample_index_map;
+    std::vector<double> breakpoints;      // [T+1]
+    std::vector<int32_t> tree_site0;      // [T+1]
+    std::vector<int32_t> site_mut0;       // [S+1]
+    std::vector<int32_t> mut_allele, mut_pallele, site_nalleles;
+    int32_t max_alleles = 1;
+    bool time_uncalibrated = false;
+
+    // chunk plan
+    int32_t num_chunks = 0;
+    std::vector<int32_t> chunk_tree0;     // [C+1]
+    std::vector<int64_t> chunk_node0;     // [C+1]
+    int32_t max_nloc = 0;
+    int32_t max_chunk_events = 0;
+    std::vector<int32_t> loc_parent0;
+    std::vector<float> loc_bl0;
+    std::vector<double> loc_time;
+    std::vector<double> loc_lastrem;
+    std::vector<int64_t> tr_off;          // [T+1]
+    std::vector<int32_t> tr_nrem;         // [T]
+    std::vector<int32_t> ev_child, ev_parent;
+    std::vector<float> ev_bl;
+    std::vector<int32_t> mut_node_loc;
+    int64_t total_path_hops = 0;          // topology-only measure: sum of path lengths
+
+    TsgDevice dev;
+    void *workspace = nullptr;            // per-handle device scratch (tsg_api.cpp)
+};
+
+int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks);
+int tsg_upload(tsg_treeseq *ts);
+void tsg_device_free(TsgDevice *d);

@@ -1,0 +1,608 @@
+"""Device-resident tree sequences and the general_stat-family statistics.
+
+Two layers, as in the reference:
+
+* ``LowLevelTreeSequence`` mirrors the statistics methods of the reference's
+  CPython extension (python/_tskitmodule.c:6588-7194): same keyword lists,
+  same argument parsing and the same ValueError messages, forwarding to the
+  C ABI of ``libtsgpu.so`` (include/tsgpu.h).  C errors raise
+  ``LibraryError`` with the reference's error number.
+* ``TreeSequence`` restates the Python wrappers (python/tskit/trees.py
+  :7917-10428): sample-set and index parsing, ``"trees"``/``"sites"`` window
+  shortcuts, dimension dropping, and the composites Fst, Tajima's D and
+  relatedness ``proportion`` (which run fused in one device sweep with their
+  FP64 finals on the device).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import as_c, ptr
+from .exceptions import LibraryError, check
+
+STAT_SITE = 1 << 0
+STAT_BRANCH = 1 << 1
+STAT_NODE = 1 << 2
+STAT_POLARISED = 1 << 10
+STAT_SPAN_NORMALISE = 1 << 11
+STAT_ALLOW_TIME_UNCALIBRATED = 1 << 12
+STAT_NONCENTRED = 1 << 14
+TS_TIME_UNCALIBRATED = 1 << 0
+
+
+def _parse_stats_mode(mode):
+    """_tskitmodule.c:6588-6607"""
+    if mode is None or mode == "site":
+        return STAT_SITE
+    if mode == "branch":
+        return STAT_BRANCH
+    if mode == "node":
+        return STAT_NODE
+    raise ValueError("Unrecognised stats mode")
+
+
+def _parse_sample_sets(sample_set_sizes, sample_sets):
+    """_tskitmodule.c:799-848"""
+    sizes = np.asarray(sample_set_sizes)
+    if sizes.ndim != 1:
+        raise ValueError("sample_set_sizes must be one-dimensional")
+    sizes = as_c(sizes, np.uint64)
+    sets = np.asarray(sample_sets)
+    if sets.ndim != 1:
+        raise ValueError("sample_sets must be one-dimensional")
+    sets = as_c(sets, np.int32)
+    total = 0
+    for s in sizes.tolist():
+        total += s
+    if total != len(sets):
+        raise ValueError("Sum of sample_set_sizes must equal length of sample_sets array")
+    return sizes, sets
+
+
+def _parse_windows(windows):
+    """_tskitmodule.c:6609-6636"""
+    w = np.asarray(windows, dtype=np.float64)
+    if w.ndim != 1:
+        raise ValueError("Windows must be a one-dimensional array")
+    if w.shape[0] < 2:
+        raise ValueError("Windows arrays must have at least 2 elements")
+    return as_c(w, np.float64)
+
+
+def _parse_indexes(indexes, k):
+    idx = np.asarray(indexes)
+    if idx.ndim != 2:
+        raise ValueError("indexes must be two-dimensional")
+    idx = as_c(idx, np.int32)
+    if idx.shape[0] < 1 or idx.shape[1] != k:
+        raise ValueError(f"indexes must be a k x {k} array.")
+    return idx
+
+
+def safe_np_int_cast(int_array, dtype):
+    """Casts with bounds checks (restates python/tskit/util.py:98-130)."""
+    if not isinstance(int_array, np.ndarray):
+        int_array = np.array(int_array)
+    if int_array.size == 0:
+        return int_array.astype(dtype)
+    try:
+        return int_array.astype(dtype, casting="safe")
+    except TypeError:
+        if int_array.dtype == np.dtype("O"):
+            raise TypeError("Cannot convert to a rectangular array.")
+        bounds = np.iinfo(dtype)
+        if np.any(int_array < bounds.min) or np.any(int_array > bounds.max):
+            raise OverflowError(f"Cannot convert safely to {dtype} type")
+        return int_array.astype(dtype, casting="unsafe")
+
+
+class LowLevelTreeSequence:
+    """The statistics surface of the reference's ``_tskit.TreeSequence``."""
+
+    def __init__(self, tables):
+        L = _lib.lib()
+        t = tables
+        handle = ctypes.c_void_p()
+        opts = TS_TIME_UNCALIBRATED if t.time_units == "uncalibrated" else 0
+        self._keep = [
+            as_c(t.node_flags, np.uint32), as_c(t.node_time, np.float64),
+            as_c(t.edge_left, np.float64), as_c(t.edge_right, np.float64),
+            as_c(t.edge_parent, np.int32), as_c(t.edge_child, np.int32),
+            as_c(t.site_position, np.float64), as_c(t.ancestral_state, np.int8),
+            as_c(t.ancestral_state_offset, np.uint64), as_c(t.mutation_site, np.int32),
+            as_c(t.mutation_node, np.int32), as_c(t.mutation_parent, np.int32),
+            as_c(t.derived_state, np.int8), as_c(t.derived_state_offset, np.uint64),
+        ]
+        k = self._keep
+        err = L.tsg_treeseq_init(
+            ctypes.byref(handle), t.sequence_length,
+            len(k[1]), ptr(k[0], ctypes.c_uint32), ptr(k[1], ctypes.c_double),
+            len(k[2]), ptr(k[2], ctypes.c_double), ptr(k[3], ctypes.c_double),
+            ptr(k[4], ctypes.c_int32), ptr(k[5], ctypes.c_int32),
+            len(k[6]), ptr(k[6], ctypes.c_double),
+            k[7].ctypes.data_as(ctypes.c_char_p), ptr(k[8], ctypes.c_uint64),
+            len(k[9]), ptr(k[9], ctypes.c_int32), ptr(k[10], ctypes.c_int32),
+            ptr(k[11], ctypes.c_int32), k[12].ctypes.data_as(ctypes.c_char_p),
+            ptr(k[13], ctypes.c_uint64), None, None, opts,
+        )
+        check(err)
+        self._keep = None
+        self._h = handle
+        self._lib = L
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.tsg_treeseq_free(h)
+            self._h = None
+
+    # -- accessors -------------------------------------------------------------
+    def get_num_trees(self):
+        return int(self._lib.tsg_treeseq_get_num_trees(self._h))
+
+    def get_num_samples(self):
+        return int(self._lib.tsg_treeseq_get_num_samples(self._h))
+
+    def get_num_nodes(self):
+        return int(self._lib.tsg_treeseq_get_num_nodes(self._h))
+
+    def get_num_edges(self):
+        return int(self._lib.tsg_treeseq_get_num_edges(self._h))
+
+    def get_num_sites(self):
+        return int(self._lib.tsg_treeseq_get_num_sites(self._h))
+
+    def get_num_mutations(self):
+        return int(self._lib.tsg_treeseq_get_num_mutations(self._h))
+
+    def get_num_chunks(self):
+        return int(self._lib.tsg_treeseq_get_num_chunks(self._h))
+
+    def get_max_chunk_nodes(self):
+        return int(self._lib.tsg_treeseq_get_max_chunk_nodes(self._h))
+
+    def get_device_bytes(self):
+        return int(self._lib.tsg_treeseq_get_device_bytes(self._h))
+
+    def get_sequence_length(self):
+        return float(self._lib.tsg_treeseq_get_sequence_length(self._h))
+
+    def get_breakpoints(self):
+        out = np.zeros(self.get_num_trees() + 1, dtype=np.float64)
+        self._lib.tsg_treeseq_get_breakpoints(self._h, ptr(out, ctypes.c_double))
+        return out
+
+    def get_samples(self):
+        out = np.zeros(self.get_num_samples(), dtype=np.int32)
+        self._lib.tsg_treeseq_get_samples(self._h, ptr(out, ctypes.c_int32))
+        return out
+
+    def get_index(self):
+        E = self.get_num_edges()
+        I = np.zeros(E, dtype=np.int32)
+        O = np.zeros(E, dtype=np.int32)
+        self._lib.tsg_treeseq_get_index(self._h, ptr(I, ctypes.c_int32), ptr(O, ctypes.c_int32))
+        return I, O
+
+    def get_plan(self):
+        """Host copy of the chunk plan (the per-chunk sweep programs)."""
+        sizes = (ctypes.c_uint64 * 6)()
+        self._lib.tsg_treeseq_get_plan_sizes(self._h, sizes)
+        C, nl, ne, T, S, Mu = (int(x) for x in sizes)
+        p = {
+            "chunk_tree0": np.zeros(C + 1, np.int32), "chunk_node0": np.zeros(C + 1, np.int64),
+            "loc_parent0": np.zeros(nl, np.int32), "loc_bl0": np.zeros(nl, np.float32),
+            "loc_time": np.zeros(nl, np.float64), "loc_lastrem": np.zeros(nl, np.float64),
+            "tr_off": np.zeros(T + 1, np.int64), "tr_nrem": np.zeros(T, np.int32),
+            "ev_child": np.zeros(ne, np.int32), "ev_parent": np.zeros(ne, np.int32),
+            "ev_bl": np.zeros(ne, np.float32), "tree_site0": np.zeros(T + 1, np.int32),
+            "site_mut0": np.zeros(S + 1, np.int32), "mut_node": np.zeros(Mu, np.int32),
+            "mut_allele": np.zeros(Mu, np.int32), "mut_pallele": np.zeros(Mu, np.int32),
+            "site_nalleles": np.zeros(S, np.int32),
+        }
+        self._lib.tsg_treeseq_copy_plan(self._h, *[a.ctypes.data_as(ctypes.c_void_p)
+                                                   for a in p.values()])
+        return p
+
+    def set_num_chunks(self, num_chunks):
+        check(self._lib.tsg_treeseq_set_num_chunks(self._h, int(num_chunks)))
+
+    def upload(self):
+        check(self._lib.tsg_treeseq_upload(self._h))
+
+    def set_profiling(self, enable):
+        check(self._lib.tsg_treeseq_set_profiling(self._h, int(enable)))
+
+    def get_profile(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        self._lib.tsg_treeseq_get_profile(self._h, ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
+    # -- statistics ------------------------------------------------------------
+    def _options(self, mode, span_normalise, polarised, centre=True):
+        opts = _parse_stats_mode(mode)
+        if span_normalise:
+            opts |= STAT_SPAN_NORMALISE
+        if polarised:
+            opts |= STAT_POLARISED
+        if not centre:
+            opts |= STAT_NONCENTRED
+        return opts
+
+    def _one_way(self, fn, sample_set_sizes, sample_sets, windows, mode, span_normalise,
+                 polarised):
+        opts = self._options(mode, span_normalise, polarised)
+        sizes, sets = _parse_sample_sets(sample_set_sizes, sample_sets)
+        w = _parse_windows(windows)
+        out = np.zeros((len(w) - 1, len(sizes)), dtype=np.float64)
+        err = fn(self._h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
+                 len(w) - 1, ptr(w, ctypes.c_double), opts, ptr(out, ctypes.c_double))
+        check(err)
+        return out
+
+    def _k_way(self, fn, k, sample_set_sizes, sample_sets, indexes, windows, mode,
+               span_normalise, polarised, centre):
+        opts = self._options(mode, span_normalise, polarised, centre)
+        sizes, sets = _parse_sample_sets(sample_set_sizes, sample_sets)
+        w = _parse_windows(windows)
+        idx = _parse_indexes(indexes, k)
+        out = np.zeros((len(w) - 1, idx.shape[0]), dtype=np.float64)
+        err = fn(self._h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
+                 idx.shape[0], ptr(idx, ctypes.c_int32), len(w) - 1, ptr(w, ctypes.c_double),
+                 opts, ptr(out, ctypes.c_double))
+        check(err)
+        return out
+
+    def diversity(self, sample_set_sizes, sample_sets, windows, mode=None, span_normalise=1,
+                  polarised=0):
+        return self._one_way(self._lib.tsg_treeseq_diversity, sample_set_sizes, sample_sets,
+                             windows, mode, span_normalise, polarised)
+
+    def segregating_sites(self, sample_set_sizes, sample_sets, windows, mode=None,
+                          span_normalise=1, polarised=0):
+        return self._one_way(self._lib.tsg_treeseq_segregating_sites, sample_set_sizes,
+                             sample_sets, windows, mode, span_normalise, polarised)
+
+    def Y1(self, sample_set_sizes, sample_sets, windows, mode=None, span_normalise=1,
+           polarised=0):
+        return self._one_way(self._lib.tsg_treeseq_Y1, sample_set_sizes, sample_sets, windows,
+                             mode, span_normalise, polarised)
+
+    def Tajimas_D(self, sample_set_sizes, sample_sets, windows, mode=None, span_normalise=0,
+                  polarised=0):
+        return self._one_way(self._lib.tsg_treeseq_Tajimas_D, sample_set_sizes, sample_sets,
+                             windows, mode, 0, 0)
+
+    def divergence(self, sample_set_sizes, sample_sets, indexes, windows, mode=None,
+                   span_normalise=1, polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_divergence, 2, sample_set_sizes, sample_sets,
+                           indexes, windows, mode, span_normalise, polarised, centre)
+
+    def Y2(self, sample_set_sizes, sample_sets, indexes, windows, mode=None, span_normalise=1,
+           polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_Y2, 2, sample_set_sizes, sample_sets, indexes,
+                           windows, mode, span_normalise, polarised, centre)
+
+    def f2(self, sample_set_sizes, sample_sets, indexes, windows, mode=None, span_normalise=1,
+           polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_f2, 2, sample_set_sizes, sample_sets, indexes,
+                           windows, mode, span_normalise, polarised, centre)
+
+    def Y3(self, sample_set_sizes, sample_sets, indexes, windows, mode=None, span_normalise=1,
+           polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_Y3, 3, sample_set_sizes, sample_sets, indexes,
+                           windows, mode, span_normalise, polarised, centre)
+
+    def f3(self, sample_set_sizes, sample_sets, indexes, windows, mode=None, span_normalise=1,
+           polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_f3, 3, sample_set_sizes, sample_sets, indexes,
+                           windows, mode, span_normalise, polarised, centre)
+
+    def f4(self, sample_set_sizes, sample_sets, indexes, windows, mode=None, span_normalise=1,
+           polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_f4, 4, sample_set_sizes, sample_sets, indexes,
+                           windows, mode, span_normalise, polarised, centre)
+
+    def genetic_relatedness(self, sample_set_sizes, sample_sets, indexes, windows, mode=None,
+                            span_normalise=1, polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_genetic_relatedness, 2, sample_set_sizes,
+                           sample_sets, indexes, windows, mode, span_normalise, polarised, centre)
+
+    def genetic_relatedness_proportion(self, sample_set_sizes, sample_sets, indexes, windows,
+                                       mode=None, span_normalise=1, polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_genetic_relatedness_proportion, 2,
+                           sample_set_sizes, sample_sets, indexes, windows, mode,
+                           span_normalise, polarised, centre)
+
+    def Fst(self, sample_set_sizes, sample_sets, indexes, windows, mode=None, span_normalise=1,
+            polarised=0, centre=1):
+        return self._k_way(self._lib.tsg_treeseq_Fst, 2, sample_set_sizes, sample_sets, indexes,
+                           windows, mode, span_normalise, polarised, centre)
+
+    def allele_frequency_spectrum(self, sample_set_sizes, sample_sets, windows, time_windows,
+                                  mode=None, span_normalise=1, polarised=0):
+        """_tskitmodule.c:6977-7062"""
+        opts = self._options(mode, span_normalise, polarised)
+        sizes, sets = _parse_sample_sets(sample_set_sizes, sample_sets)
+        w = _parse_windows(windows)
+        tw = _parse_windows(time_windows)
+        shape = [len(w) - 1, len(tw) - 1] + [int(s) + 1 for s in sizes]
+        out = np.zeros(shape, dtype=np.float64)
+        err = self._lib.tsg_treeseq_allele_frequency_spectrum(
+            self._h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
+            len(w) - 1, ptr(w, ctypes.c_double), len(tw) - 1, ptr(tw, ctypes.c_double), opts,
+            ptr(out, ctypes.c_double))
+        check(err)
+        return out
+
+
+class TreeSequence:
+    """A tree sequence resident on the GPU, with the reference's statistics API."""
+
+    def __init__(self, ll_tree_sequence, tables):
+        self._ll_tree_sequence = ll_tree_sequence
+        self._tables = tables
+
+    @classmethod
+    def load_tables(cls, tables):
+        return cls(LowLevelTreeSequence(tables), tables)
+
+    @property
+    def ll_tree_sequence(self):
+        return self._ll_tree_sequence
+
+    @property
+    def tables(self):
+        return self._tables
+
+    @property
+    def sequence_length(self):
+        return self._tables.sequence_length
+
+    @property
+    def num_trees(self):
+        return self._ll_tree_sequence.get_num_trees()
+
+    @property
+    def num_samples(self):
+        return self._ll_tree_sequence.get_num_samples()
+
+    @property
+    def num_nodes(self):
+        return self._ll_tree_sequence.get_num_nodes()
+
+    @property
+    def num_edges(self):
+        return self._ll_tree_sequence.get_num_edges()
+
+    @property
+    def num_sites(self):
+        return self._ll_tree_sequence.get_num_sites()
+
+    @property
+    def num_mutations(self):
+        return self._ll_tree_sequence.get_num_mutations()
+
+    @property
+    def sites_position(self):
+        return self._tables.site_position
+
+    def samples(self):
+        return self._ll_tree_sequence.get_samples()
+
+    def breakpoints(self, as_array=False):
+        b = self._ll_tree_sequence.get_breakpoints()
+        return b if as_array else iter(b)
+
+    # -- window handling (trees.py:8108-8159) ----------------------------------
+    def parse_windows(self, windows):
+        if windows is None:
+            windows = [0.0, self.sequence_length]
+        elif isinstance(windows, str):
+            if windows == "trees":
+                windows = self.breakpoints(as_array=True)
+            elif windows == "sites":
+                windows = np.concatenate(
+                    [
+                        [] if self.num_sites > 0 else [0.0],
+                        self.sites_position,
+                        [self.sequence_length],
+                    ]
+                )
+                windows[0] = 0.0
+            else:
+                raise ValueError(
+                    f"Unrecognized window specification {windows}:",
+                    "the only allowed strings are 'sites' or 'trees'",
+                )
+        return np.array(windows)
+
+    def parse_time_windows(self, time_windows):
+        if time_windows is None:
+            time_windows = [0.0, np.inf]
+        return np.array(time_windows)
+
+    def _run_windowed_stat(self, windows, method, *args, **kwargs):
+        strip_win = windows is None
+        windows = self.parse_windows(windows)
+        stat = method(*args, **kwargs, windows=windows)
+        if strip_win:
+            stat = stat[0]
+        return stat
+
+    def _run_windowed_stat_tw(self, windows, time_windows, method, *args, **kwargs):
+        strip_win = windows is None
+        strip_timewin = time_windows is None
+        windows = self.parse_windows(windows)
+        time_windows = self.parse_time_windows(time_windows)
+        stat = method(*args, **kwargs, windows=windows, time_windows=time_windows)
+        if strip_win and strip_timewin:
+            stat = stat[0, 0, :]
+        elif strip_win:
+            stat = stat[0, :, :]
+        elif strip_timewin:
+            stat = stat[:, 0, :]
+        return stat
+
+    # -- sample-set handling (trees.py:8161-8221, 8359-8412) -------------------
+    def _one_way_sample_set_stat(self, ll_method, sample_sets, windows=None, time_windows=None,
+                                 mode=None, span_normalise=True, polarised=False, use_tw=False):
+        if sample_sets is None:
+            sample_sets = self.samples()
+        drop_dimension = False
+        try:
+            sample_sets = np.array(sample_sets, dtype=np.uint64)
+        except ValueError:
+            pass
+        else:
+            if len(sample_sets.shape) == 1:
+                sample_sets = [sample_sets]
+                drop_dimension = True
+        sample_set_sizes = np.array([len(s) for s in sample_sets], dtype=np.uint32)
+        if np.any(sample_set_sizes == 0):
+            raise ValueError("Sample sets must contain at least one element")
+        flattened = safe_np_int_cast(np.hstack(sample_sets), np.int32)
+        if use_tw:
+            stat = self._run_windowed_stat_tw(
+                windows, time_windows, ll_method, sample_set_sizes, flattened, mode=mode,
+                span_normalise=span_normalise, polarised=polarised)
+        else:
+            stat = self._run_windowed_stat(
+                windows, ll_method, sample_set_sizes, flattened, mode=mode,
+                span_normalise=span_normalise, polarised=polarised)
+        if drop_dimension:
+            stat = stat.reshape(stat.shape[:-1])
+            if stat.shape == () and windows is None and time_windows is None:
+                stat = stat[()]
+        return stat
+
+    def _k_way_sample_set_stat(self, ll_method, k, sample_sets, indexes=None, windows=None,
+                               mode=None, span_normalise=True, polarised=False, centre=True):
+        sample_set_sizes = np.array([len(s) for s in sample_sets], dtype=np.uint32)
+        if np.any(sample_set_sizes == 0):
+            raise ValueError("Sample sets must contain at least one element")
+        flattened = safe_np_int_cast(np.hstack(sample_sets), np.int32)
+        drop_based_on_index = False
+        if indexes is None:
+            drop_based_on_index = True
+            if len(sample_sets) != k:
+                raise ValueError(
+                    "Must specify indexes if there are not exactly {} sample sets.".format(k))
+            indexes = np.arange(k, dtype=np.int32)
+        drop_dimension = False
+        indexes = safe_np_int_cast(indexes, np.int32)
+        if len(indexes.shape) == 1:
+            indexes = indexes.reshape((1, indexes.shape[0]))
+            drop_dimension = True
+        if len(indexes.shape) != 2 or indexes.shape[1] != k:
+            raise ValueError(
+                "Indexes must be convertable to a 2D numpy array with {} columns".format(k))
+        stat = self._run_windowed_stat(
+            windows, ll_method, sample_set_sizes, flattened, indexes, mode=mode,
+            span_normalise=span_normalise, polarised=polarised, centre=centre)
+        if drop_dimension:
+            stat = stat.reshape(stat.shape[:-1])
+            if stat.shape == () and windows is None and drop_based_on_index:
+                stat = stat[()]
+        return stat
+
+    # -- one-way statistics ------------------------------------------------------
+    def diversity(self, sample_sets=None, windows=None, mode="site", span_normalise=True):
+        """Mean pairwise diversity (trees.py:8496-8556)."""
+        return self._one_way_sample_set_stat(
+            self._ll_tree_sequence.diversity, sample_sets, windows=windows, mode=mode,
+            span_normalise=span_normalise)
+
+    def segregating_sites(self, sample_sets=None, windows=None, mode="site",
+                          span_normalise=True):
+        """Segregating sites (trees.py:9790-9842)."""
+        return self._one_way_sample_set_stat(
+            self._ll_tree_sequence.segregating_sites, sample_sets, windows=windows, mode=mode,
+            span_normalise=span_normalise)
+
+    def Y1(self, sample_sets, windows=None, mode="site", span_normalise=True):
+        """Y1 (trees.py:10226-10270)."""
+        return self._one_way_sample_set_stat(
+            self._ll_tree_sequence.Y1, sample_sets, windows=windows, mode=mode,
+            span_normalise=span_normalise)
+
+    def Tajimas_D(self, sample_sets=None, windows=None, mode="site"):
+        """Tajima's D (trees.py:9967-10022): diversity and segregating sites
+        from one fused sweep, composed in FP64 on the device."""
+        return self._one_way_sample_set_stat(
+            self._ll_tree_sequence.Tajimas_D, sample_sets, windows=windows, mode=mode,
+            span_normalise=False)
+
+    def allele_frequency_spectrum(self, sample_sets=None, windows=None, time_windows=None,
+                                  mode="site", span_normalise=True, polarised=False):
+        """Allele frequency spectrum (trees.py:9844-9965)."""
+        if sample_sets is None:
+            sample_sets = [self.samples()]
+        try:
+            sample_sets = np.array(sample_sets, dtype=np.uint64)
+        except ValueError:
+            pass
+        else:
+            if len(sample_sets.shape) == 1:
+                sample_sets = [sample_sets]
+        return self._one_way_sample_set_stat(
+            self._ll_tree_sequence.allele_frequency_spectrum, sample_sets, windows=windows,
+            time_windows=time_windows, mode=mode, span_normalise=span_normalise,
+            polarised=polarised, use_tw=True)
+
+    # -- multi-way statistics ----------------------------------------------------
+    def divergence(self, sample_sets, indexes=None, windows=None, mode="site",
+                   span_normalise=True):
+        """Mean pairwise divergence (trees.py:8558-8627)."""
+        return self._k_way_sample_set_stat(
+            self._ll_tree_sequence.divergence, 2, sample_sets, indexes=indexes, windows=windows,
+            mode=mode, span_normalise=span_normalise)
+
+    def Y2(self, sample_sets, indexes=None, windows=None, mode="site", span_normalise=True):
+        return self._k_way_sample_set_stat(
+            self._ll_tree_sequence.Y2, 2, sample_sets, indexes=indexes, windows=windows,
+            mode=mode, span_normalise=span_normalise)
+
+    def f2(self, sample_sets, indexes=None, windows=None, mode="site", span_normalise=True):
+        """Patterson's f2 (trees.py:10384-10428)."""
+        return self._k_way_sample_set_stat(
+            self._ll_tree_sequence.f2, 2, sample_sets, indexes=indexes, windows=windows,
+            mode=mode, span_normalise=span_normalise)
+
+    def Y3(self, sample_sets, indexes=None, windows=None, mode="site", span_normalise=True):
+        return self._k_way_sample_set_stat(
+            self._ll_tree_sequence.Y3, 3, sample_sets, indexes=indexes, windows=windows,
+            mode=mode, span_normalise=span_normalise)
+
+    def f3(self, sample_sets, indexes=None, windows=None, mode="site", span_normalise=True):
+        """Patterson's f3 (trees.py:10330-10382)."""
+        return self._k_way_sample_set_stat(
+            self._ll_tree_sequence.f3, 3, sample_sets, indexes=indexes, windows=windows,
+            mode=mode, span_normalise=span_normalise)
+
+    def f4(self, sample_sets, indexes=None, windows=None, mode="site", span_normalise=True):
+        """Patterson's f4 (trees.py:10272-10328)."""
+        return self._k_way_sample_set_stat(
+            self._ll_tree_sequence.f4, 4, sample_sets, indexes=indexes, windows=windows,
+            mode=mode, span_normalise=span_normalise)
+
+    def Fst(self, sample_sets, indexes=None, windows=None, mode="site", span_normalise=True):
+        """Fst (trees.py:10024-10136): one fused device sweep of diversity and
+        divergence, composed in FP64 on the device."""
+        return self._k_way_sample_set_stat(
+            self._ll_tree_sequence.Fst, 2, sample_sets, indexes=indexes, windows=windows,
+            mode=mode, span_normalise=span_normalise)
+
+    def genetic_relatedness(self, sample_sets, indexes=None, windows=None, mode="site",
+                            span_normalise=True, polarised=True, proportion=True, centre=True):
+        """Genetic relatedness (trees.py:8830-8976).  With ``proportion`` the
+        division by the segregating sites of the union of the sample sets
+        happens on the device, from the same sweep."""
+        ll = self._ll_tree_sequence
+        method = ll.genetic_relatedness_proportion if proportion else ll.genetic_relatedness
+        out = self._k_way_sample_set_stat(
+            method, 2, sample_sets, indexes=indexes, windows=windows, mode=mode,
+            span_normalise=span_normalise, polarised=polarised, centre=centre)
+        return out
