@@ -1,0 +1,225 @@
+"""Benchmark of the tree-sequence statistics hot path on one MI355X.
+
+Workload (the scale configuration): a Hudson-coalescent tree sequence with
+1e5 haploid samples, L = 1e8, Ne = 1e4, r = mu = 1e-8 (seed 2), K = 8
+disjoint random sample sets of 1000 samples and 1000 equal windows.  One
+step computes, for branch and site mode, Fst over all 28 pairs, Tajima's D
+of every set and f4 over sets (0,1;2,3), as one fused batch per mode.
+
+Metric: edge x sample-set updates per second.  Every statistic query of a
+step counts 2 * E * K_q updates (each edge inserted and removed once, for
+each of the K_q sample sets the query takes), whatever the implementation
+fuses or repeats internally; the CPU baseline counts the same.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def workload(small):
+    if small:
+        return dict(name="1000 samples, L=1e6 (seed 1), K=2x500, W=10", n=1000, L=1e6, seed=1,
+                    K=2, set_size=500, W=10)
+    return dict(name="1e5 samples, L=1e8, Ne=1e4, r=mu=1e-8 (seed 2), K=8x1000, W=1000",
+                n=100000, L=1e8, seed=2, K=8, set_size=1000, W=1000)
+
+
+def step_queries(cfg):
+    K = cfg["K"]
+    pairs = [(i, j) for i in range(K) for j in range(i + 1, K)]
+    quads = [(0, 1, 2, 3)] if K >= 4 else [(0, 1, 0, 1)]
+    return pairs, quads
+
+
+def algorithmic_bytes(E, N, S, n, queries, W):
+    """Input bytes one fused sweep must read plus output it writes:
+    24 B/edge (left, right, parent, child) + 8 B/edge (insertion/removal
+    index) + 8 B/node (time) + 12 B/site (position + mutation node), and per
+    query n * ceil(K/8) (sample-set bitmask) + 8 (W+1) + 8 W M."""
+    b = 24 * E + 8 * E + 8 * N + 12 * S
+    for K, M in queries:
+        b += n * ((K + 7) // 8) + 8 * (W + 1) + 8 * W * M
+    return b
+
+
+def run_gpu_step(ts, sets, windows, pairs, quads):
+    """One step: Fst (all pairs), Tajima's D and f4, fused per mode."""
+    out = {}
+    for mode in ("branch", "site"):
+        b = ts.batch()
+        p_fst = b.Fst(sets, indexes=pairs, windows=windows, mode=mode)
+        p_tjd = b.Tajimas_D(sets, windows=windows, mode=mode)
+        p_f4 = b.f4(sets, indexes=quads, windows=windows, mode=mode)
+        b.run()
+        out[mode] = (p_fst.result(), p_tjd.result(), p_f4.result())
+    return out
+
+
+def cpu_step(impl, sets, windows, pairs, quads, fst, tajimas_d):
+    """The same step with the CPU implementation (the reference C library
+    when built, else the oracle), composites as the reference's Python
+    layer does them (two sweeps each)."""
+    out = {}
+    for mode in ("branch", "site"):
+        div = impl.stat("diversity", sets, None, windows, mode)
+        dvg = impl.stat("divergence", sets, pairs, windows, mode)
+        T = impl.stat("diversity", sets, None, windows, mode, False)
+        S = impl.stat("segregating_sites", sets, None, windows, mode, False)
+        f4 = impl.stat("f4", sets, quads, windows, mode)
+        out[mode] = (fst(div, dvg, pairs), tajimas_d(T, S, [len(s) for s in sets]), f4)
+    return out
+
+
+def max_rel_err(gpu, cpu):
+    worst = {}
+    for mode in gpu:
+        for name, g, c in zip(("Fst", "Tajimas_D", "f4"), gpu[mode], cpu[mode]):
+            g = np.asarray(g, dtype=np.float64)
+            c = np.asarray(c, dtype=np.float64)
+            fin = np.isfinite(c)
+            if name == "f4":
+                scale = np.max(np.abs(c[fin])) if np.any(fin) else 1.0
+            else:
+                scale = np.maximum(1.0, np.abs(c[fin]))
+            e = float(np.max(np.abs(g[fin] - c[fin]) / scale)) if np.any(fin) else 0.0
+            worst[f"{name}_{mode}"] = e
+    return worst
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--small", action="store_true", help="the 1000-sample workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--chunks", type=int, default=0, help="override the chunk count")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    import tskit_amd as ta
+    from oracle import oracle as om
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    torch.cuda.set_device(0)
+    cfg = workload(args.small)
+    t0 = time.perf_counter()
+    tc = ta.simulate_tables(cfg["n"], cfg["L"], 1e4, 1e-8, 1e-8, cfg["seed"])
+    t_gen = time.perf_counter() - t0
+    log(f"generated {tc.num_nodes} nodes {tc.num_edges} edges {tc.num_sites} sites "
+        f"in {t_gen:.1f}s digest {tc.column_digest()[:16]}")
+    t0 = time.perf_counter()
+    ts = tc.tree_sequence()
+    if args.chunks:
+        ts.ll_tree_sequence.set_num_chunks(args.chunks)
+    t_init = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ts.ll_tree_sequence.upload()
+    torch.cuda.synchronize()
+    t_upload = time.perf_counter() - t0
+    ll = ts.ll_tree_sequence
+    log(f"host init {t_init:.1f}s upload {t_upload:.2f}s trees {ts.num_trees} "
+        f"chunks {ll.get_num_chunks()} max chunk nodes {ll.get_max_chunk_nodes()} "
+        f"device bytes {ll.get_device_bytes() / 2**20:.0f} MiB")
+
+    sets = ta.random_sample_sets(cfg["n"], cfg["K"], cfg["set_size"], cfg["seed"])
+    windows = np.linspace(0, cfg["L"], cfg["W"] + 1)
+    pairs, quads = step_queries(cfg)
+    K = cfg["K"]
+    E, N, S = tc.num_edges, tc.num_nodes, tc.num_sites
+    # queries per step (per mode): Fst, Tajima's D and f4, each over the K sets
+    kq = [K, K, K]
+    updates_per_step = 2 * sum(2 * E * k for k in kq)
+    queries_per_step = 2 * len(kq)
+
+    for i in range(args.warmup):
+        run_gpu_step(ts, sets, windows, pairs, quads)
+        log(f"warmup {i + 1}/{args.warmup} done")
+    ll.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        gpu_out = run_gpu_step(ts, sets, windows, pairs, quads)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sweep_ms, launches = ll.get_profile()
+    ll.set_profiling(False)
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = updates_per_step * args.steps / elapsed
+    log(f"{args.steps} steps in {elapsed:.3f}s ({ms_per_step:.2f} ms/step), "
+        f"sweep kernel {sweep_ms / max(launches, 1):.3f} ms x {launches}")
+
+    # roofline of the dominant kernel (the fused sweep): algorithmic bytes per
+    # launch / average launch time from HIP events on the library's stream
+    per_launch_queries = [(K, K + len(pairs)), (K, 2 * K), (K, len(quads))]
+    alg_bytes = algorithmic_bytes(E, N, S, cfg["n"], per_launch_queries, cfg["W"])
+    avg_launch_s = (sweep_ms / 1e3) / max(launches, 1)
+    achieved = alg_bytes / avg_launch_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_sweep (fused batch)", "bytes_per_launch": alg_bytes,
+                "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+
+    cpu = None
+    errs = {}
+    if not args.no_cpu:
+        kind = "reference" if om.reference_available() else "port"
+        impl = om.Reference(tc) if kind == "reference" else om.Oracle(tc)
+        t0 = time.perf_counter()
+        cpu_out = cpu_step(impl, sets, windows, pairs, quads, om.fst, om.tajimas_d)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": updates_per_step / t_cpu, "unit": "edge×sample-set updates/s",
+               "cores": 1, "kind": kind, "seconds": round(t_cpu, 3),
+               "sample": "one full step (Fst over 28 pairs, Tajima's D, f4; branch and site) "
+                         "on the same tree sequence, single thread"}
+        errs = max_rel_err(gpu_out, cpu_out)
+        log(f"cpu step {t_cpu:.2f}s ({kind}); max rel err {max(errs.values()):.2e}")
+
+    line = {
+        "metric": "edge×sample-set updates/s; served stat queries/s + p99 at 1e-6 rel err",
+        "value": value,
+        "unit": "edge×sample-set updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 counts; fp32 TwoSum sums; fp64 combine",
+        "data": "synthetic (own Hudson coalescent + infinite-sites generator)",
+        "config": {"workload": cfg["name"], "edges": E, "trees": ts.num_trees, "sites": S,
+                   "nodes": N, "queries_per_step": queries_per_step,
+                   "chunks": ll.get_num_chunks()},
+        "queries_per_s": queries_per_step * args.steps / elapsed,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "max_rel_err_vs_cpu": errs,
+        "setup_s": {"generate": round(t_gen, 2), "host_init": round(t_init, 2),
+                    "upload": round(t_upload, 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -171,6 +171,67 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq_t *ts, uint64_t num_sample
     const double *windows, uint64_t num_time_windows, const double *time_windows,
     uint32_t options, double *result);
 
+/* ---- batched form (no counterpart in the reference) ---------------------
+ * A batch collects queries on one tree sequence; tsg_batch_run executes them
+ * as one fused sweep per mode (branch / site), sharing the path walks.  Each
+ * tsg_batch_<stat> takes the arguments of tsg_treeseq_<stat>, validates them
+ * the same way (same error numbers) and records the query; `result` must
+ * stay valid until tsg_batch_run returns. */
+typedef struct tsg_batch tsg_batch_t;
+int tsg_batch_init(tsg_batch_t **batch, tsg_treeseq_t *ts);
+int tsg_batch_free(tsg_batch_t *batch);
+int tsg_batch_clear(tsg_batch_t *batch);
+uint64_t tsg_batch_get_num_queries(const tsg_batch_t *batch);
+int tsg_batch_run(tsg_batch_t *batch);
+int tsg_batch_diversity(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+int tsg_batch_segregating_sites(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+int tsg_batch_Y1(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+int tsg_batch_Tajimas_D(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+int tsg_batch_divergence(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+int tsg_batch_Y2(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+int tsg_batch_f2(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+int tsg_batch_Y3(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+int tsg_batch_f3(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+int tsg_batch_f4(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+int tsg_batch_genetic_relatedness(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+int tsg_batch_genetic_relatedness_proportion(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+int tsg_batch_Fst(tsg_batch_t *batch, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_index_tuples,
+    const int32_t *index_tuples, uint64_t num_windows, const double *windows,
+    uint32_t options, double *result);
+
 /* ---- synthetic inputs (no counterpart in the reference; msprime's role
  *      in its tests and benchmarks) --------------------------------------- */
 /* Hudson coalescent with recombination, discrete genome, haploid samples. */

@@ -78,6 +78,24 @@ SIGNATURES = {
             ctypes.c_uint64, c_f64p, ctypes.c_uint32, c_f64p,
         ],
     ),
+    "tsg_batch_init": (ctypes.c_int, [ctypes.POINTER(c_void), c_void]),
+    "tsg_batch_free": (ctypes.c_int, [c_void]),
+    "tsg_batch_clear": (ctypes.c_int, [c_void]),
+    "tsg_batch_get_num_queries": (ctypes.c_uint64, [c_void]),
+    "tsg_batch_run": (ctypes.c_int, [c_void]),
+    "tsg_batch_diversity": (ctypes.c_int, ONE_WAY),
+    "tsg_batch_segregating_sites": (ctypes.c_int, ONE_WAY),
+    "tsg_batch_Y1": (ctypes.c_int, ONE_WAY),
+    "tsg_batch_Tajimas_D": (ctypes.c_int, ONE_WAY),
+    "tsg_batch_divergence": (ctypes.c_int, K_WAY),
+    "tsg_batch_Y2": (ctypes.c_int, K_WAY),
+    "tsg_batch_f2": (ctypes.c_int, K_WAY),
+    "tsg_batch_Y3": (ctypes.c_int, K_WAY),
+    "tsg_batch_f3": (ctypes.c_int, K_WAY),
+    "tsg_batch_f4": (ctypes.c_int, K_WAY),
+    "tsg_batch_genetic_relatedness": (ctypes.c_int, K_WAY),
+    "tsg_batch_genetic_relatedness_proportion": (ctypes.c_int, K_WAY),
+    "tsg_batch_Fst": (ctypes.c_int, K_WAY),
     "tsg_sim_ancestry": (
         ctypes.c_int,
         [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,

@@ -254,6 +254,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         tq.W = (int32_t) q.windows.size() - 1;
         tq.mbase = M;
         tq.cbase = cbase;
+        tq.flags = q.span_normalise ? TSG_F_SPAN_NORMALISE : 0u;
         tq.win_off = (int64_t) wins.size();
         wins.insert(wins.end(), q.windows.begin(), q.windows.end());
         for (const Block &b : q.blocks) {
@@ -408,9 +409,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     ca.Rstart = (mode == TSG_STAT_BRANCH) ? (const double *) ws->Rstart.p : nullptr;
     ca.raw = (double *) ws->raw.p;
     ca.rawbase = (const int64_t *) ws->rawbase.p;
-    ca.flags = batch[0]->span_normalise ? TSG_F_SPAN_NORMALISE : 0u;
-    // span normalisation is per query; the batch builder only fuses queries
-    // that agree on it.
+    ca.flags = 0;
     if ((ret = hip_check(tsg_launch_combine(ca, (mode == TSG_STAT_BRANCH) ? a.dR : nullptr,
                                             (double *) ws->Rstart.p, rawcount, s)))) {
         return ret;
@@ -466,7 +465,15 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     return 0;
 }
 
+// While a batch is being filled, the validated query is handed to it
+// instead of being run (see tsg_batch_*).
+thread_local std::vector<std::unique_ptr<Query>> *g_capture = nullptr;
+
 int run_one(tsg_treeseq *ts, Query &q) {
+    if (g_capture != nullptr) {
+        g_capture->push_back(std::make_unique<Query>(std::move(q)));
+        return 0;
+    }
     std::lock_guard<std::mutex> lock(g_mutex);
     std::vector<Query *> b{&q};
     return run_batch(ts, b);
@@ -931,6 +938,91 @@ int tsg_treeseq_genetic_relatedness_proportion(tsg_treeseq *ts, uint64_t K, cons
     q.result = result;
     return run_one(ts, q);
 }
+
+// ---- batched form (no counterpart in the reference) ----------------------
+// A batch collects validated queries on one tree sequence and runs them as
+// one fused sweep per mode: the path walks are shared, the state vector is
+// the concatenation of every query's sample sets and the outputs the
+// concatenation of their summary blocks, each with its own windows.
+struct tsg_batch {
+    tsg_treeseq *ts;
+    std::vector<std::unique_ptr<Query>> queries;
+};
+
+namespace {
+struct Capture {
+    explicit Capture(tsg_batch *b) { g_capture = &b->queries; }
+    ~Capture() { g_capture = nullptr; }
+};
+}  // namespace
+
+int tsg_batch_init(tsg_batch **out, tsg_treeseq *ts) {
+    *out = nullptr;
+    if (ts == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+    tsg_batch *b = new (std::nothrow) tsg_batch();
+    if (b == nullptr) return TSG_ERR_NO_MEMORY;
+    b->ts = ts;
+    *out = b;
+    return 0;
+}
+
+int tsg_batch_free(tsg_batch *b) {
+    delete b;
+    return 0;
+}
+
+int tsg_batch_clear(tsg_batch *b) {
+    b->queries.clear();
+    return 0;
+}
+
+uint64_t tsg_batch_get_num_queries(const tsg_batch *b) { return b->queries.size(); }
+
+// Runs every query of the batch: one fused sweep for the branch-mode
+// queries and one for the site-mode queries.  Results land in the buffers
+// given when the queries were added.
+int tsg_batch_run(tsg_batch *b) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    for (uint32_t mode : {TSG_STAT_BRANCH, TSG_STAT_SITE}) {
+        std::vector<Query *> group;
+        for (auto &q : b->queries) {
+            if (q->mode == mode) group.push_back(q.get());
+        }
+        if (group.empty()) continue;
+        int ret = run_batch(b->ts, group);
+        if (ret != 0) return ret;
+    }
+    return 0;
+}
+
+#define BATCH_ONE_WAY(name, fn)                                                               \
+    int name(tsg_batch *b, uint64_t K, const uint64_t *sizes, const int32_t *sets,            \
+             uint64_t nw, const double *windows, uint32_t options, double *result) {           \
+        Capture cap(b);                                                                       \
+        return fn(b->ts, K, sizes, sets, nw, windows, options, result);                       \
+    }
+#define BATCH_K_WAY(name, fn)                                                                 \
+    int name(tsg_batch *b, uint64_t K, const uint64_t *sizes, const int32_t *sets,            \
+             uint64_t M, const int32_t *idx, uint64_t nw, const double *windows,               \
+             uint32_t options, double *result) {                                              \
+        Capture cap(b);                                                                       \
+        return fn(b->ts, K, sizes, sets, M, idx, nw, windows, options, result);                \
+    }
+BATCH_ONE_WAY(tsg_batch_diversity, tsg_treeseq_diversity)
+BATCH_ONE_WAY(tsg_batch_segregating_sites, tsg_treeseq_segregating_sites)
+BATCH_ONE_WAY(tsg_batch_Y1, tsg_treeseq_Y1)
+BATCH_ONE_WAY(tsg_batch_Tajimas_D, tsg_treeseq_Tajimas_D)
+BATCH_K_WAY(tsg_batch_divergence, tsg_treeseq_divergence)
+BATCH_K_WAY(tsg_batch_Y2, tsg_treeseq_Y2)
+BATCH_K_WAY(tsg_batch_f2, tsg_treeseq_f2)
+BATCH_K_WAY(tsg_batch_Y3, tsg_treeseq_Y3)
+BATCH_K_WAY(tsg_batch_f3, tsg_treeseq_f3)
+BATCH_K_WAY(tsg_batch_f4, tsg_treeseq_f4)
+BATCH_K_WAY(tsg_batch_genetic_relatedness, tsg_treeseq_genetic_relatedness)
+BATCH_K_WAY(tsg_batch_genetic_relatedness_proportion, tsg_treeseq_genetic_relatedness_proportion)
+BATCH_K_WAY(tsg_batch_Fst, tsg_treeseq_Fst)
+#undef BATCH_ONE_WAY
+#undef BATCH_K_WAY
 
 // ---- allele frequency spectrum (trees.c:3815-3928) -----------------------
 #define TSG_AFS_MAX_SETS 16

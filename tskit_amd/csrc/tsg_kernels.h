@@ -22,6 +22,8 @@ struct TsgQuery {
     int32_t M;           // number of raw outputs
     int32_t mbase;       // first output index in the batch
     int32_t cbase;       // first state column
+    uint32_t flags;      // TSG_F_SPAN_NORMALISE
+    int32_t pad;
     int64_t win_off;     // offset of its windows in the concatenated array
     int64_t pbase;       // offset (in doubles) of its partial-sum block
 };

@@ -626,15 +626,18 @@ __global__ void k_scan_dR(const double *dR, double *Rstart, int C, int M) {
 __global__ void k_combine(const TsgCombineArgs a, int64_t total) {
     const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= total) return;
-    // locate (q, w, ml)
-    int q = 0;
-    int64_t rem = gid;
-    while (q < a.Q) {
-        const int64_t n = (int64_t) a.queries[q].W * a.queries[q].M;
-        if (rem < n) break;
-        rem -= n;
-        q++;
+    // locate (q, w, ml): rawbase is the prefix sum of W_q * M_q
+    int qlo = 0, qhi = a.Q - 1;
+    while (qlo < qhi) {
+        const int mid = (qlo + qhi + 1) / 2;
+        if (a.rawbase[mid] <= gid) {
+            qlo = mid;
+        } else {
+            qhi = mid - 1;
+        }
     }
+    const int q = qlo;
+    const int64_t rem = gid - a.rawbase[q];
     const TsgQuery &Qr = a.queries[q];
     const int w = (int) (rem / Qr.M);
     const int ml = (int) (rem % Qr.M);
@@ -664,7 +667,7 @@ __global__ void k_combine(const TsgCombineArgs a, int64_t total) {
             if (ov > 0) acc += a.Rstart[(int64_t) c * a.M + Qr.mbase + ml] * ov;
         }
     }
-    if (a.flags & TSG_F_SPAN_NORMALISE) acc /= (wr - wl);
+    if (Qr.flags & TSG_F_SPAN_NORMALISE) acc /= (wr - wl);
     a.raw[a.rawbase[q] + (int64_t) w * Qr.M + ml] = acc;
 }
 

@@ -221,6 +221,10 @@ class LowLevelTreeSequence:
         return ms.value, n.value
 
     # -- statistics ------------------------------------------------------------
+    def _route(self, fn):
+        """The C function and handle a statistics call goes to."""
+        return fn, self._h
+
     def _options(self, mode, span_normalise, polarised, centre=True):
         opts = _parse_stats_mode(mode)
         if span_normalise:
@@ -237,7 +241,8 @@ class LowLevelTreeSequence:
         sizes, sets = _parse_sample_sets(sample_set_sizes, sample_sets)
         w = _parse_windows(windows)
         out = np.zeros((len(w) - 1, len(sizes)), dtype=np.float64)
-        err = fn(self._h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
+        fn, h = self._route(fn)
+        err = fn(h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
                  len(w) - 1, ptr(w, ctypes.c_double), opts, ptr(out, ctypes.c_double))
         check(err)
         return out
@@ -249,7 +254,8 @@ class LowLevelTreeSequence:
         w = _parse_windows(windows)
         idx = _parse_indexes(indexes, k)
         out = np.zeros((len(w) - 1, idx.shape[0]), dtype=np.float64)
-        err = fn(self._h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
+        fn, h = self._route(fn)
+        err = fn(h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
                  idx.shape[0], ptr(idx, ctypes.c_int32), len(w) - 1, ptr(w, ctypes.c_double),
                  opts, ptr(out, ctypes.c_double))
         check(err)
@@ -338,6 +344,103 @@ class LowLevelTreeSequence:
         return out
 
 
+class _BatchLowLevel(LowLevelTreeSequence):
+    """Routes the statistics calls of a LowLevelTreeSequence into a batch:
+    arguments are parsed and validated exactly as for a direct call, the
+    result buffer is allocated, and the query is recorded for one fused
+    sweep."""
+
+    def __init__(self, ll):  # noqa: D107 -- shares the tree sequence handle
+        self._lib = ll._lib
+        self._h = ll._h
+        self._parent = ll
+        b = ctypes.c_void_p()
+        check(self._lib.tsg_batch_init(ctypes.byref(b), ll._h))
+        self._b = b
+        self.last_result = None
+
+    def __del__(self):
+        b = getattr(self, "_b", None)
+        if b is not None and b.value:
+            self._lib.tsg_batch_free(b)
+            self._b = None
+
+    def _route(self, fn):
+        name = fn.__name__.replace("tsg_treeseq_", "tsg_batch_")
+        return getattr(self._lib, name), self._b
+
+    def _one_way(self, *args, **kwargs):
+        out = super()._one_way(*args, **kwargs)
+        self.last_result = out
+        return out
+
+    def _k_way(self, *args, **kwargs):
+        out = super()._k_way(*args, **kwargs)
+        self.last_result = out
+        return out
+
+    def allele_frequency_spectrum(self, *args, **kwargs):
+        raise ValueError("the allele frequency spectrum is not batched; call it directly")
+
+    def run(self):
+        check(self._lib.tsg_batch_run(self._b))
+        check(self._lib.tsg_batch_clear(self._b))
+
+
+class PendingStat:
+    """The result of a batched query; available after StatBatch.run()."""
+
+    def __init__(self, value, buffer, batch):
+        self._value = value
+        self._buffer = buffer
+        self._batch = batch
+
+    def result(self):
+        if not self._batch.done:
+            raise RuntimeError("StatBatch.run() has not been called")
+        if isinstance(self._value, np.ndarray):
+            return self._value
+        return self._buffer.reshape(-1)[0]
+
+
+class StatBatch:
+    """Statistics queries on one tree sequence, executed together.
+
+    Every statistics method of TreeSequence (except the allele frequency
+    spectrum) is available with the same arguments; it returns a PendingStat.
+    ``run()`` executes all recorded queries as one fused device sweep per
+    mode, sharing the edge walks among them.
+    """
+
+    def __init__(self, ts):
+        self._ll = _BatchLowLevel(ts.ll_tree_sequence)
+        self._facade = TreeSequence(self._ll, ts.tables)
+        self._pending = []
+        self.done = False
+
+    def __getattr__(self, name):
+        method = getattr(self._facade, name)
+
+        def call(*args, **kwargs):
+            if self.done:
+                raise RuntimeError("this batch has already run")
+            self._ll.last_result = None
+            value = method(*args, **kwargs)
+            p = PendingStat(value, self._ll.last_result, self)
+            self._pending.append(p)
+            return p
+
+        return call
+
+    def __len__(self):
+        return len(self._pending)
+
+    def run(self):
+        self._ll.run()
+        self.done = True
+        return [p.result() for p in self._pending]
+
+
 class TreeSequence:
     """A tree sequence resident on the GPU, with the reference's statistics API."""
 
@@ -352,6 +455,10 @@ class TreeSequence:
     @property
     def ll_tree_sequence(self):
         return self._ll_tree_sequence
+
+    def batch(self):
+        """A StatBatch of queries on this tree sequence (one fused sweep)."""
+        return StatBatch(self)
 
     @property
     def tables(self):
