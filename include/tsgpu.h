@@ -171,6 +171,17 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq_t *ts, uint64_t num_sample
     const double *windows, uint64_t num_time_windows, const double *time_windows,
     uint32_t options, double *result);
 
+/* ---- relatedness matrix ---------------------------------------------------
+ * genetic_relatedness_matrix(mode="site") of python/tskit/trees.py:8978-9102
+ * (the divergence matrix of c/tskit/trees.c:8900-9001 double-centred) for
+ * singleton sample sets (sample_sets NULL = every sample) and sites with at
+ * most one mutation, computed as K = J (G^T G) J with the cross product on
+ * int8 matrix cores (exact int32) and FP64 centring.  Windows need not span
+ * the genome (as the reference's divergence matrix).  result: W x n x n. */
+int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+
 /* ---- batched form (no counterpart in the reference) ---------------------
  * A batch collects queries on one tree sequence; tsg_batch_run executes them
  * as one fused sweep per mode (branch / site), sharing the path walks.  Each

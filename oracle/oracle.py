@@ -164,6 +164,34 @@ class Oracle(_Base):
         self._lib.oracle_ts_index(self.h, _p(I), _p(O))
         return I, O
 
+    def genotypes(self):
+        """0/1 derived-allele matrix [S, n] (sites with <= 1 mutation)."""
+        G = np.zeros((len(self.tc.site_position), int(np.sum(self.tc.node_flags & 1))),
+                     dtype=np.int8)
+        err = self._lib.oracle_genotypes(self.h, _p(G))
+        if err != 0:
+            raise OracleError(err)
+        return G
+
+    def relatedness_matrix(self, windows=None, span_normalise=True):
+        """Site-mode relatedness of every sample, J (G^T G) J per window (exact
+        integer cross product, FP64 centring)."""
+        G = self.genotypes().astype(np.int64)
+        pos = self.tc.site_position
+        if windows is None:
+            windows = np.array([0.0, self.tc.sequence_length])
+        n = G.shape[1]
+        out = np.zeros((len(windows) - 1, n, n))
+        for w in range(len(windows) - 1):
+            sel = (pos >= windows[w]) & (pos < windows[w + 1])
+            C = (G[sel].T @ G[sel]).astype(np.float64)
+            r = C.sum(axis=1)
+            K = C - r[:, None] / n - r[None, :] / n + r.sum() / n**2
+            if span_normalise:
+                K /= windows[w + 1] - windows[w]
+            out[w] = K
+        return out
+
     def _stat(self, sid, sizes, flat, t, M, idx, windows, opts, out):
         return self._lib.oracle_sample_count_stat(
             self.h, sid, len(sizes), _p(sizes), _p(flat), t, M, _p(idx), len(windows) - 1,

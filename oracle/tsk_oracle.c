@@ -633,6 +633,52 @@ out:
     return ret;
 }
 
+/* ---- genotypes: 0/1 derived-allele matrix [S x n] for sites with at most one
+ * mutation (a sample carries the derived allele iff the mutation's node is on
+ * its root path in the tree covering the site); the role tsk_variant_decode
+ * plays for such sites (genotypes.c:473-595). ---------------------------- */
+int oracle_genotypes(const ots_t *ts, int8_t *G) {
+    const int32_t N = ts->N, E = ts->E;
+    int32_t *par = malloc((size_t) (N + 1) * sizeof(int32_t));
+    int32_t j, h, tj = 0, tk = 0, t = 0, s, u;
+    double tl = 0, tr;
+    for (j = 0; j < N; j++) par[j] = NUL;
+    memset(G, 0, (size_t) ts->S * ts->n);
+    while (tj < E || tl < ts->L) {
+        while (tk < E && ts->right[ts->O[tk]] == tl) {
+            h = ts->O[tk++];
+            par[ts->child[h]] = NUL;
+        }
+        while (tj < E && ts->left[ts->I[tj]] == tl) {
+            h = ts->I[tj++];
+            par[ts->child[h]] = ts->parent[h];
+        }
+        tr = ts->L;
+        if (tj < E && ts->left[ts->I[tj]] < tr) tr = ts->left[ts->I[tj]];
+        if (tk < E && ts->right[ts->O[tk]] < tr) tr = ts->right[ts->O[tk]];
+        for (s = ts->tree_site0[t]; s < ts->tree_site0[t + 1]; s++) {
+            const int32_t m0 = ts->site_mut0[s], m1 = ts->site_mut0[s + 1];
+            if (m1 - m0 > 1) {
+                free(par);
+                return -30005;
+            }
+            if (m1 == m0) continue;
+            for (j = 0; j < ts->n; j++) {
+                for (u = ts->samples[j]; u != NUL; u = par[u]) {
+                    if (u == ts->mut_node[m0]) {
+                        G[(size_t) s * ts->n + j] = 1;
+                        break;
+                    }
+                }
+            }
+        }
+        t++;
+        tl = tr;
+    }
+    free(par);
+    return 0;
+}
+
 /* ---- AFS: trees.c:3469-3928 --------------------------------------------- */
 static void fold(int64_t *coord, const int64_t *dims, int32_t nd) { /* trees.c:3469-3496 */
     int32_t k;

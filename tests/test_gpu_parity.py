@@ -257,6 +257,37 @@ def test_afs_time_windows(small):
     assert rel_err(got, exp, scale) <= TOL
 
 
+@pytest.mark.parametrize("windows", [None, "three"])
+def test_relatedness_matrix(require_gpu, windows):
+    """int8 MFMA cross products are exact, so the GPU matrix equals the FP64
+    closed form to rounding of the FP64 centring."""
+    tc = cases.generated(300, 5e5, 17)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    w = None if windows is None else np.array([0, 1e5, 3.5e5, 5e5])
+    _say("relatedness matrix", windows)
+    got = ts.genetic_relatedness_matrix(windows=w)
+    exp = orc.relatedness_matrix(w, True)
+    if w is None:
+        exp = exp[0]
+    assert got.shape == exp.shape
+    assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
+
+
+def test_relatedness_matrix_subset(require_gpu):
+    tc = cases.generated(200, 3e5, 19)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    sel = np.array([5, 17, 3, 150, 99, 42, 0, 7], dtype=np.int32)
+    got = ts.genetic_relatedness_matrix(sample_sets=[[s] for s in sel])
+    G = orc.genotypes().astype(np.int64)[:, sel]
+    C = (G.T @ G).astype(np.float64)
+    r = C.sum(axis=1)
+    n = len(sel)
+    exp = (C - r[:, None] / n - r[None, :] / n + r.sum() / n**2) / tc.sequence_length
+    assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
+
+
 def test_repeatable(small):
     """Two launches give bit-identical results (deterministic combine)."""
     tc, ts, orc = small

@@ -287,6 +287,36 @@ def test_brute_afs_site(tiny):
     assert np.array_equal(got, exp)
 
 
+def test_oracle_genotypes_brute_force(tiny):
+    tc, orc = tiny
+    assert np.array_equal(orc.genotypes(), _genotypes(tc))
+
+
+def _normalise_divergence(D):
+    """The double centring of python/tskit/trees.py:9086-9095 for singleton
+    sets (n = 1 each, so the diagonal correction zeroes the diagonal)."""
+    B = D.copy()
+    np.fill_diagonal(B, 0.0)
+    K = B + np.mean(B)
+    y = np.mean(B, axis=0)
+    K -= y[:, None] + y[None, :]
+    return K / -2
+
+
+@needs_ref
+def test_relatedness_matrix_closed_form_vs_reference():
+    """K = J (G^T G) J equals the reference's genetic_relatedness_matrix route
+    (its C divergence matrix, double-centred) for single-mutation sites."""
+    tc = cases.generated(60, 2e5, 41)
+    orc, ref = Oracle(tc), om.Reference(tc)
+    samples = np.arange(60, dtype=np.int32)
+    w = np.array([0, 5e4, 1.2e5, 2e5])
+    D = ref.divergence_matrix(samples, w, "site", True)
+    exp = np.array([_normalise_divergence(D[j]) for j in range(D.shape[0])])
+    got = orc.relatedness_matrix(w, True)
+    assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
+
+
 def test_identity_invariant():
     """test_stats.c:1061-1124: branch diversity-style identity -- the branch
     segregating-sites of all samples with span normalisation off equals the

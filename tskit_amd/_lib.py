@@ -78,6 +78,11 @@ SIGNATURES = {
             ctypes.c_uint64, c_f64p, ctypes.c_uint32, c_f64p,
         ],
     ),
+    "tsg_treeseq_genetic_relatedness_matrix": (
+        ctypes.c_int,
+        [c_void, ctypes.c_uint64, c_u64p, c_i32p, ctypes.c_uint64, c_f64p, ctypes.c_uint32,
+         c_f64p],
+    ),
     "tsg_batch_init": (ctypes.c_int, [ctypes.POINTER(c_void), c_void]),
     "tsg_batch_free": (ctypes.c_int, [c_void]),
     "tsg_batch_clear": (ctypes.c_int, [c_void]),

@@ -16,6 +16,7 @@ ARCH = os.environ.get("TSG_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = [
     "tsg_kernels.hip",
+    "tsg_relate.hip",
     "tsg_api.cpp",
     "tsg_treeseq.cpp",
     "tsg_sim.cpp",

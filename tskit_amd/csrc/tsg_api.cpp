@@ -42,6 +42,8 @@ struct Workspace {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf pair_s, pair_c, tot, outs, queries, windows, cq_wfirst, cq_nwin, cq_poff, partial, dR,
         Rstart, raw, rawbase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
+    // relatedness-matrix route
+    DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab;
     int lds_limit = 65536;
     // profiling of the dominant (sweep) kernel
     bool profile = false;
@@ -537,7 +539,9 @@ void free_workspace(tsg_treeseq *ts) {
     DevBuf *bufs[] = {&ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
                       &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
                       &ws->Rstart, &ws->raw, &ws->rawbase, &ws->out, &ws->fin, &ws->ipar,
-                      &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin};
+                      &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
+                      &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
+                      &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab};
     for (DevBuf *b : bufs) {
         if (b->p != nullptr) (void) hipFree(b->p);
     }
@@ -937,6 +941,150 @@ int tsg_treeseq_genetic_relatedness_proportion(tsg_treeseq *ts, uint64_t K, cons
     q.P = (int) M;
     q.result = result;
     return run_one(ts, q);
+}
+
+// ---- relatedness matrix on the matrix cores --------------------------------
+// genetic_relatedness_matrix(mode="site") of trees.py:8978-9102 (divergence
+// matrix trees.c:8900-9001, then the double centring of trees.py:9086-9095)
+// for singleton sample sets and sites with at most one mutation: K = J C J,
+// C = G^T G on int8 MFMA.  sample_sets may be NULL (all samples); result is
+// W x n x n doubles.
+int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
+                                           const uint64_t *sample_set_sizes,
+                                           const int32_t *sample_sets, uint64_t nw,
+                                           const double *windows, uint32_t options,
+                                           double *result) {
+    bool site = options & TSG_STAT_SITE, branch = options & TSG_STAT_BRANCH;
+    if (options & TSG_STAT_NODE) return TSG_ERR_UNSUPPORTED_STAT_MODE;
+    if (!(site || branch)) site = true;
+    if (site && branch) return TSG_ERR_MULTIPLE_STAT_MODES;
+    if (options & TSG_STAT_POLARISED) return -911;  // TSK_ERR_STAT_POLARISED_UNSUPPORTED
+    std::vector<double> win;
+    if (windows == nullptr) {
+        win = {0.0, ts->L};
+    } else {  // tsk_treeseq_check_windows without the full-span requirement
+        if (nw < 1) return TSG_ERR_BAD_NUM_WINDOWS;
+        if (windows[0] < 0 || windows[nw] > ts->L) return TSG_ERR_BAD_WINDOWS;
+        for (uint64_t j = 0; j < nw; j++) {
+            if (windows[j] >= windows[j + 1]) return TSG_ERR_BAD_WINDOWS;
+        }
+        win.assign(windows, windows + nw + 1);
+    }
+    // sample sets (trees.c:8833-8873): singletons only on this route
+    std::vector<int32_t> sel;
+    if (sample_sets == nullptr) {
+        for (int32_t j = 0; j < ts->num_samples; j++) sel.push_back(j);
+    } else {
+        std::vector<char> seen(ts->num_samples, 0);
+        uint64_t i = 0;
+        for (uint64_t k = 0; k < num_sample_sets; k++) {
+            const uint64_t sz = sample_set_sizes ? sample_set_sizes[k] : 1;
+            for (uint64_t l = 0; l < sz; l++) {
+                const int32_t u = sample_sets[i++];
+                if (u < 0 || u >= ts->num_nodes) return TSG_ERR_NODE_OUT_OF_BOUNDS;
+                const int32_t si = ts->sample_index_map[u];
+                if (si == TSG_NULL) return TSG_ERR_BAD_SAMPLES;
+                if (seen[si]) return TSG_ERR_DUPLICATE_SAMPLE;
+                seen[si] = 1;
+                if (sz == 1) sel.push_back(si);
+            }
+            if (sz != 1) return TSG_ERR_UNSUPPORTED_INPUT;
+        }
+    }
+    if (branch) return TSG_ERR_UNSUPPORTED_STAT_MODE;  // the branch matrix is not on this route
+    for (int32_t s = 0; s < ts->num_sites; s++) {
+        if (ts->site_mut0[s + 1] - ts->site_mut0[s] > 1) return TSG_ERR_NOT_BIALLELIC;
+    }
+    const int W = (int) win.size() - 1;
+    const int n = (int) sel.size();
+    if (n == 0) return 0;
+    const int ns = ts->num_samples;
+    const int n_pad = ((ns + 127) / 128) * 128;
+    const int NW = (ns + 31) / 32;
+    // per window: its sites, padded to a multiple of 64 columns of Gt
+    std::vector<int64_t> wk0(W), wk1(W);
+    std::vector<int32_t> sitecol;
+    for (int w = 0; w < W; w++) {
+        const int32_t s0 = (int32_t) (std::lower_bound(ts->site_pos.begin(), ts->site_pos.end(), win[w]) -
+                                      ts->site_pos.begin());
+        const int32_t s1 = (int32_t) (std::lower_bound(ts->site_pos.begin(), ts->site_pos.end(), win[w + 1]) -
+                                      ts->site_pos.begin());
+        wk0[w] = (int64_t) sitecol.size();
+        for (int32_t s = s0; s < s1; s++) sitecol.push_back(s);
+        while (sitecol.size() % 64 != 0) sitecol.push_back(-1);
+        wk1[w] = (int64_t) sitecol.size();
+    }
+    if (sitecol.empty()) sitecol.assign(64, -1);
+    const int64_t Kp = (int64_t) sitecol.size();
+    std::vector<double> scale(W, 1.0);
+    if (options & TSG_STAT_SPAN_NORMALISE) {
+        for (int w = 0; w < W; w++) scale[w] = 1.0 / (win[w + 1] - win[w]);
+    }
+
+    std::lock_guard<std::mutex> lock(g_mutex);
+    int ret = device_upload(ts);
+    if (ret != 0) return ret;
+    Workspace *ws = ws_of(ts);
+    hipStream_t s = ws->stream;
+    const int slots = std::min(ts->num_chunks, 512);
+    const int64_t gstride = (int64_t) ts->max_nloc * (1 + NW);
+    if ((ret = ensure(ws->gbits, (size_t) std::max(ts->num_sites, 1) * NW * 4)) ||
+        (ret = ensure(ws->gslab, (size_t) slots * gstride * 4)) ||
+        (ret = upload(ws->sitecol, sitecol, s)) || (ret = upload(ws->wk0, wk0, s)) ||
+        (ret = upload(ws->wk1, wk1, s)) || (ret = upload(ws->scale, scale, s)) ||
+        (ret = upload(ws->sel, sel, s)) || (ret = ensure(ws->gt, (size_t) n_pad * Kp)) ||
+        (ret = ensure(ws->cmat, (size_t) W * n_pad * n_pad * 4)) ||
+        (ret = ensure(ws->rowsum, (size_t) (W * (int64_t) n + W) * 8)) ||
+        (ret = ensure(ws->kout, (size_t) W * n * (size_t) n * 8))) {
+        return ret;
+    }
+    const TsgDevice &d = ts->dev;
+    TsgGenoArgs g{};
+    g.chunk_tree0 = d.chunk_tree0;
+    g.chunk_node0 = d.chunk_node0;
+    g.loc_parent0 = d.loc_parent0;
+    g.tr_off = d.tr_off;
+    g.tr_nrem = d.tr_nrem;
+    g.ev_child = d.ev_child;
+    g.ev_parent = d.ev_parent;
+    g.tree_site0 = d.tree_site0;
+    g.site_mut0 = d.site_mut0;
+    g.mut_node = d.mut_node;
+    g.num_chunks = ts->num_chunks;
+    g.num_samples = ns;
+    g.NW = NW;
+    g.slots = slots;
+    g.g_rec = (uint32_t *) ws->gslab.p;
+    g.g_stride = gstride;
+    g.gbits = (uint32_t *) ws->gbits.p;
+    if (ws->profile) (void) hipEventRecord(ws->ev0, s);
+    if ((ret = hip_check(tsg_launch_geno_bits(g, s))) ||
+        (ret = hip_check(tsg_launch_bits_to_gt((const uint32_t *) ws->gbits.p, NW,
+                                               (const int32_t *) ws->sitecol.p, Kp, ns, n_pad,
+                                               (int8_t *) ws->gt.p, s))) ||
+        (ret = hip_check(tsg_launch_syrk_i8((const int8_t *) ws->gt.p, Kp, n_pad,
+                                            (const int64_t *) ws->wk0.p,
+                                            (const int64_t *) ws->wk1.p, W,
+                                            (int32_t *) ws->cmat.p, s))) ||
+        (ret = hip_check(tsg_launch_centre((const int32_t *) ws->cmat.p,
+                                           (const int32_t *) ws->sel.p, (int64_t *) ws->rowsum.p,
+                                           n, n_pad, W, (const double *) ws->scale.p,
+                                           (double *) ws->kout.p, s)))) {
+        return ret;
+    }
+    if (ws->profile) (void) hipEventRecord(ws->ev1, s);
+    if ((ret = hip_check(hipMemcpyAsync(result, ws->kout.p, (size_t) W * n * (size_t) n * 8,
+                                        hipMemcpyDeviceToHost, s)))) {
+        return ret;
+    }
+    if ((ret = hip_check(hipStreamSynchronize(s)))) return ret;
+    if (ws->profile) {
+        float ms = 0;
+        (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
+        ws->sweep_ms += ms;
+        ws->sweep_launches += 1;
+    }
+    return 0;
 }
 
 // ---- batched form (no counterpart in the reference) ----------------------

@@ -79,6 +79,24 @@ struct TsgSweepArgs {
     double *afs_hist_d;          // branch AFS
 };
 
+// Genotype bitset sweep (relatedness matrix route).
+struct TsgGenoArgs {
+    const int32_t *chunk_tree0;
+    const int64_t *chunk_node0;
+    const int32_t *loc_parent0;
+    const int64_t *tr_off;
+    const int32_t *tr_nrem;
+    const int32_t *ev_child, *ev_parent;
+    const int32_t *tree_site0;
+    const int32_t *site_mut0, *mut_node;
+    int32_t num_chunks, num_samples;
+    int32_t NW;                  // 32-bit words per bitset
+    int32_t slots;               // resident workgroups (slabs); they loop over chunks
+    uint32_t *g_rec;             // per slot: max_nloc x (1 + NW) words
+    int64_t g_stride;
+    uint32_t *gbits;             // [S x NW] derived-allele bitsets per site
+};
+
 struct TsgCombineArgs {
     const double *bp;
     const int32_t *chunk_tree0;

@@ -19,6 +19,13 @@ hipError_t tsg_launch_afs_site_final(const unsigned long long *h, double *out, c
 hipError_t tsg_launch_afs_branch_final(double *h, const double *win, int W, int64_t per_window,
                                        uint32_t flags, hipStream_t s);
 size_t tsg_final_struct_size();
+hipError_t tsg_launch_geno_bits(const TsgGenoArgs &a, hipStream_t s);
+hipError_t tsg_launch_bits_to_gt(const uint32_t *gbits, int NW, const int32_t *site_of_col,
+                                 int64_t Kp, int n, int n_pad, int8_t *gt, hipStream_t s);
+hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int64_t *win_k0,
+                              const int64_t *win_k1, int W, int32_t *C, hipStream_t s);
+hipError_t tsg_launch_centre(const int32_t *C, const int32_t *sel, int64_t *rowsum, int n,
+                             int n_pad, int W, const double *scale, double *K, hipStream_t s);
 
 // Mirrors TsgFinal in tsg_kernels.hip.
 struct TsgFinalHost {

@@ -1,0 +1,286 @@
+// Site-mode genetic relatedness matrix on the matrix cores (gfx950).
+//
+// For sites carrying at most one mutation the reference's
+// genetic_relatedness_matrix(mode="site") (python/tskit/trees.py:8978-9102,
+// from the divergence matrix of c/tskit/trees.c:8753-8826) reduces to
+//     K = J C J / span,   C = G^T G,   J = I - 11^T / n,
+// where G is the site x sample 0/1 matrix of derived alleles: the divergence
+// matrix is D = a 1^T + 1 a^T - 2C (a = derived counts per sample) and the
+// double centring -1/2 J D J removes the a terms exactly.  C is an exact
+// integer cross product (entries <= number of sites < 2^31), computed here
+// with int8 MFMA and int32 accumulation; the centring is FP64.
+//
+// Kernels:
+//   k_geno_bits  one wavefront per chunk: the chunk sweep with a per-node
+//                bitset of the samples below it (insert: OR, remove: AND-NOT);
+//                each site's row is the bitset of its mutation's node.
+//   k_bits_to_gt bit rows (site-major) -> int8 Gt (sample-major, each
+//                window's sites zero-padded to a multiple of 64).
+//   k_syrk_i8    C = Gt Gt^T on v_mfma_i32_32x32x32_i8, upper-triangular
+//                128x128 tiles, mirrored on store.
+//   k_centre     row sums, then K = (C - r_i/n - r_j/n + t/n^2) / span (FP64).
+#include <hip/hip_runtime.h>
+
+#include "tsg_common.h"
+#include "tsg_kernels.h"
+
+#define WAVE 64
+
+// --- genotype bitsets ------------------------------------------------------
+__device__ void geno_bits_chunk(const TsgGenoArgs &a, int c, uint32_t *rec);
+
+// Each workgroup owns one slab and loops over chunks blockIdx.x,
+// blockIdx.x + gridDim.x, ... so the slab memory is bounded by the grid.
+__global__ void __launch_bounds__(WAVE) k_geno_bits(const TsgGenoArgs a) {
+    for (int c = blockIdx.x; c < a.num_chunks; c += gridDim.x) {
+        __syncthreads();
+        geno_bits_chunk(a, c, a.g_rec + (int64_t) blockIdx.x * a.g_stride);
+    }
+}
+
+__device__ void geno_bits_chunk(const TsgGenoArgs &a, int c, uint32_t *rec) {
+    const int lane = threadIdx.x;
+    const int NW = a.NW;
+    const int R = 1 + NW;  // parent, then NW bitset words
+    const int64_t n0 = a.chunk_node0[c];
+    const int nloc = (int) (a.chunk_node0[c + 1] - n0);
+    auto par = [&](int u) -> int32_t & { return reinterpret_cast<int32_t &>(rec[(int64_t) u * R]); };
+    auto word = [&](int u, int w) -> uint32_t & { return rec[(int64_t) u * R + 1 + w]; };
+
+    for (int u = lane; u < nloc; u += WAVE) {
+        par(u) = a.loc_parent0[n0 + u];
+        for (int w = 0; w < NW; w++) word(u, w) = 0u;
+    }
+    __syncthreads();
+    // every sample's bit, up its root path in the start tree
+    for (int j = lane; j < a.num_samples; j += WAVE) {
+        const uint32_t bit = 1u << (j & 31);
+        const int w = j >> 5;
+        for (int u = j; u >= 0; u = par(u)) atomicOr(&word(u, w), bit);
+    }
+    __syncthreads();
+    const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
+    for (int t = t0; t < t1; t++) {
+        if (t > t0 || c == 0) {
+            const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
+            const int64_t nrem = a.tr_nrem[t];
+            for (int64_t e = e0; e < e1; e++) {
+                const bool ins = (e - e0) >= nrem;
+                const int u = a.ev_child[e], v = a.ev_parent[e];
+                // lanes own words w = lane, lane + 64, ...; a chunk's child
+                // bitset is re-read per ancestor from the same lane's word
+                if (lane == 0) par(u) = ins ? v : TSG_NULL;
+                __syncthreads();
+                for (int w = lane; w < NW; w += WAVE) {
+                    const uint32_t xc = word(u, w);
+                    if (xc == 0u) continue;
+                    for (int p = v; p >= 0; p = par(p)) {
+                        word(p, w) = ins ? (word(p, w) | xc) : (word(p, w) & ~xc);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const int s0 = a.tree_site0[t], s1 = a.tree_site0[t + 1];
+        for (int s = s0; s < s1; s++) {
+            uint32_t *row = a.gbits + (int64_t) s * NW;
+            const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
+            if (m1 == m0) {
+                for (int w = lane; w < NW; w += WAVE) row[w] = 0u;
+            } else {
+                const int node = a.mut_node[m0];
+                for (int w = lane; w < NW; w += WAVE) row[w] = word(node, w);
+            }
+        }
+    }
+}
+
+// --- bits (site-major) -> int8 Gt (sample-major, K padded per window) -------
+// grid: (ceil(S_pad_total / 64), n_pad / 64); block 256.  Tile: 64 sites x
+// 64 samples.  site_of_col[k] maps a padded column to its site (-1 = pad).
+__global__ void __launch_bounds__(256) k_bits_to_gt(const uint32_t *gbits, int NW,
+                                                    const int32_t *site_of_col, int64_t Kp,
+                                                    int n, int8_t *gt) {
+    __shared__ uint32_t tile[64][2];  // 64 sites x 64 sample bits
+    const int64_t k0 = (int64_t) blockIdx.x * 64;
+    const int j0 = blockIdx.y * 64;  // first sample of the tile
+    const int tid = threadIdx.x;
+    if (tid < 128) {
+        const int r = tid >> 1, h = tid & 1;
+        const int64_t k = k0 + r;
+        uint32_t v = 0u;
+        const int w = (j0 >> 5) + h;
+        if (k < Kp) {
+            const int s = site_of_col[k];
+            if (s >= 0 && w < NW) v = gbits[(int64_t) s * NW + w];
+        }
+        tile[r][h] = v;
+    }
+    __syncthreads();
+    // each thread writes 16 consecutive columns (sites) of one sample row
+    const int jj = tid >> 2;          // sample within tile, 0..63
+    const int kk = (tid & 3) * 16;    // 16-column group
+    const int j = j0 + jj;
+    if (k0 + kk >= Kp) return;
+    int8_t out[16];
+    for (int q = 0; q < 16; q++) {
+        const uint32_t wv = tile[kk + q][jj >> 5];
+        out[q] = (j < n) ? (int8_t) ((wv >> (jj & 31)) & 1u) : (int8_t) 0;
+    }
+    int4 v;
+    __builtin_memcpy(&v, out, 16);
+    *reinterpret_cast<int4 *>(gt + (int64_t) j * Kp + k0 + kk) = v;
+}
+
+// --- SYRK on int8 MFMA --------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+#define BT 128        // output tile
+#define BK 64         // k per LDS stage
+#define LDSP 80       // padded LDS row (bytes): 64 + 16
+
+// One workgroup (4 waves) per upper-triangular 128x128 tile of one window.
+// Wave (wy, wx) computes the 64x64 quadrant as 2x2 tiles of 32x32.
+__global__ void __launch_bounds__(256) k_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad,
+                                                 const int64_t *win_k0, const int64_t *win_k1,
+                                                 int tiles_per_dim, int32_t *C) {
+    __shared__ __attribute__((aligned(16))) int8_t As[BT * LDSP];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[BT * LDSP];
+    const int w = blockIdx.y;  // window
+    // invert the upper-triangular tile index
+    int t = blockIdx.x, ti = 0;
+    int rowlen = tiles_per_dim;
+    while (t >= rowlen) {
+        t -= rowlen;
+        ti++;
+        rowlen--;
+    }
+    const int tj = ti + t;
+    const int i0 = ti * BT, j0 = tj * BT;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wy = wave >> 1, wx = wave & 1;
+    v16i acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int r = 0; r < 16; r++) acc[a][b][r] = 0;
+    const int64_t kbeg = win_k0[w], kend = win_k1[w];
+    for (int64_t k = kbeg; k < kend; k += BK) {
+        // stage 128 rows x 64 bytes of A (rows i0..) and B (rows j0..)
+        for (int q = tid; q < BT * 4; q += 256) {
+            const int r = q >> 2, seg = (q & 3) * 16;
+            *reinterpret_cast<int4 *>(As + r * LDSP + seg) =
+                *reinterpret_cast<const int4 *>(gt + (int64_t) (i0 + r) * Kp + k + seg);
+            *reinterpret_cast<int4 *>(Bs + r * LDSP + seg) =
+                *reinterpret_cast<const int4 *>(gt + (int64_t) (j0 + r) * Kp + k + seg);
+        }
+        __syncthreads();
+        for (int kk = 0; kk < BK; kk += 32) {
+            const int ko = kk + 16 * (lane >> 5);
+            v4i af[2], bf[2];
+            for (int a = 0; a < 2; a++) {
+                const int r = wy * 64 + a * 32 + (lane & 31);
+                af[a] = *reinterpret_cast<const v4i *>(As + r * LDSP + ko);
+            }
+            for (int b = 0; b < 2; b++) {
+                const int r = wx * 64 + b * 32 + (lane & 31);
+                bf[b] = *reinterpret_cast<const v4i *>(Bs + r * LDSP + ko);
+            }
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    int32_t *Cw = C + (int64_t) w * n_pad * n_pad;
+    for (int a = 0; a < 2; a++) {
+        for (int b = 0; b < 2; b++) {
+            const int col = j0 + wx * 64 + b * 32 + (lane & 31);
+            for (int r = 0; r < 16; r++) {
+                const int row = i0 + wy * 64 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const int32_t v = acc[a][b][r];
+                Cw[(int64_t) row * n_pad + col] = v;
+                Cw[(int64_t) col * n_pad + row] = v;
+            }
+        }
+    }
+}
+
+// --- centring (FP64) ----------------------------------------------------------
+// The n selected samples (sample indexes sel[0..n)) form the matrix.
+// rowsum[w][a] = sum_b C[w][sel_a][sel_b] (exact in int64); block per (a, w).
+__global__ void __launch_bounds__(256) k_rowsum(const int32_t *C, const int32_t *sel, int n,
+                                                int n_pad, int64_t *rowsum) {
+    __shared__ long long part[256];
+    const int i = blockIdx.x, w = blockIdx.y;
+    const int32_t *row = C + ((int64_t) w * n_pad + sel[i]) * n_pad;
+    long long s = 0;
+    for (int j = threadIdx.x; j < n; j += 256) s += row[sel[j]];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int) threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rowsum[(int64_t) w * n + i] = part[0];
+}
+
+// K[w][i][j] = (C_ij - r_i/n - r_j/n + t/n^2) * scale[w], t = sum_i r_i.
+__global__ void __launch_bounds__(256) k_centre(const int32_t *C, const int32_t *sel,
+                                                const int64_t *rowsum, int n, int n_pad,
+                                                const double *scale, double *K) {
+    const int64_t gid = (int64_t) blockIdx.x * 256 + threadIdx.x;
+    const int w = blockIdx.y;
+    if (gid >= (int64_t) n * n) return;
+    const int i = (int) (gid / n), j = (int) (gid % n);
+    const int64_t *rs = rowsum + (int64_t) w * n;
+    // grand total, recomputed per thread from the row sums would be O(n);
+    // it is passed through rowsum[n * W + w] by the host-side launcher
+    const double t = (double) rowsum[(int64_t) gridDim.y * n + w];
+    const double dn = (double) n;
+    const double cij = (double) C[((int64_t) w * n_pad + sel[i]) * n_pad + sel[j]];
+    const double v = cij - (double) rs[i] / dn - (double) rs[j] / dn + t / (dn * dn);
+    K[(int64_t) w * n * n + gid] = v * scale[w];
+}
+
+// grand totals: one thread per window (n row sums each)
+__global__ void k_total(int64_t *rowsum, int n, int W) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    long long s = 0;
+    for (int i = 0; i < n; i++) s += rowsum[(int64_t) w * n + i];
+    rowsum[(int64_t) W * n + w] = s;
+}
+
+// --- launchers ------------------------------------------------------------------
+hipError_t tsg_launch_geno_bits(const TsgGenoArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_geno_bits, dim3(a.slots), dim3(WAVE), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_bits_to_gt(const uint32_t *gbits, int NW, const int32_t *site_of_col,
+                                 int64_t Kp, int n, int n_pad, int8_t *gt, hipStream_t s) {
+    dim3 grid((unsigned) ((Kp + 63) / 64), (unsigned) (n_pad / 64));
+    hipLaunchKernelGGL(k_bits_to_gt, grid, dim3(256), 0, s, gbits, NW, site_of_col, Kp, n, gt);
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int64_t *win_k0,
+                              const int64_t *win_k1, int W, int32_t *C, hipStream_t s) {
+    const int tpd = n_pad / BT;
+    dim3 grid((unsigned) (tpd * (tpd + 1) / 2), (unsigned) W);
+    hipLaunchKernelGGL(k_syrk_i8, grid, dim3(256), 0, s, gt, Kp, n_pad, win_k0, win_k1, tpd, C);
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_centre(const int32_t *C, const int32_t *sel, int64_t *rowsum, int n,
+                             int n_pad, int W, const double *scale, double *K, hipStream_t s) {
+    hipLaunchKernelGGL(k_rowsum, dim3((unsigned) n, (unsigned) W), dim3(256), 0, s, C, sel, n,
+                       n_pad, rowsum);
+    hipLaunchKernelGGL(k_total, dim3((unsigned) ((W + 63) / 64)), dim3(64), 0, s, rowsum, n, W);
+    const int64_t nn = (int64_t) n * n;
+    hipLaunchKernelGGL(k_centre, dim3((unsigned) ((nn + 255) / 256), (unsigned) W), dim3(256), 0, s,
+                       C, sel, rowsum, n, n_pad, scale, K);
+    return hipGetLastError();
+}

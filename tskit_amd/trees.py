@@ -327,6 +327,26 @@ class LowLevelTreeSequence:
         return self._k_way(self._lib.tsg_treeseq_Fst, 2, sample_set_sizes, sample_sets, indexes,
                            windows, mode, span_normalise, polarised, centre)
 
+    def genetic_relatedness_matrix(self, sample_sets, windows, mode=None, span_normalise=1):
+        """Relatedness of singleton sample sets on the int8 matrix-core route.
+        ``sample_sets`` is None (every sample) or a list of node ids, each its
+        own set.  Returns [W, n, n]."""
+        opts = self._options(mode, span_normalise, False)
+        w = _parse_windows(windows)
+        if sample_sets is None:
+            n = self.get_num_samples()
+            sets_p = None
+        else:
+            sets = as_c(np.asarray(sample_sets), np.int32)
+            n = len(sets)
+            sets_p = ptr(sets, ctypes.c_int32)
+        out = np.zeros((len(w) - 1, n, n), dtype=np.float64)
+        err = self._lib.tsg_treeseq_genetic_relatedness_matrix(
+            self._h, n, None, sets_p, len(w) - 1, ptr(w, ctypes.c_double), opts,
+            ptr(out, ctypes.c_double))
+        check(err)
+        return out
+
     def allele_frequency_spectrum(self, sample_set_sizes, sample_sets, windows, time_windows,
                                   mode=None, span_normalise=1, polarised=0):
         """_tskitmodule.c:6977-7062"""
@@ -701,6 +721,25 @@ class TreeSequence:
         return self._k_way_sample_set_stat(
             self._ll_tree_sequence.Fst, 2, sample_sets, indexes=indexes, windows=windows,
             mode=mode, span_normalise=span_normalise)
+
+    def genetic_relatedness_matrix(self, sample_sets=None, windows=None, mode="site",
+                                   span_normalise=True):
+        """Site-mode relatedness matrix (trees.py:8978-9102) for singleton sample
+        sets: G^T G on the int8 matrix cores, FP64 double centring.  Returns
+        [n, n], or [W, n, n] with windows."""
+        if sample_sets is not None:
+            flat = []
+            for s in sample_sets:
+                s = np.atleast_1d(np.asarray(s))
+                if len(s) != 1:
+                    raise ValueError("only single-sample sets are supported on this path")
+                flat.append(int(s[0]))
+            sample_sets = flat
+        drop = windows is None
+        w = self.parse_windows(windows)
+        out = self._ll_tree_sequence.genetic_relatedness_matrix(
+            sample_sets, w, mode=mode, span_normalise=span_normalise)
+        return out[0] if drop else out
 
     def genetic_relatedness(self, sample_sets, indexes=None, windows=None, mode="site",
                             span_normalise=True, polarised=True, proportion=True, centre=True):
