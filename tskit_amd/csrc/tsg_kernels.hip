@@ -236,7 +236,10 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             nz = __any(nz);
             const bool walk = nz || noskip;
             __syncthreads();
-            if (MODE == 0 && walk) {
+            // the edge's own term bl * F(x_child): needed even for a zero-weight
+            // subtree, because unpolarised summaries need not vanish at zero
+            // (non-centred relatedness: F(0) = f(0) + f(total) = 1)
+            if (MODE == 0) {
                 const float b = ins ? a.ev_bl[e] : nd.bl(u);
                 const float sg = ins ? 1.0f : -1.0f;
 #pragma unroll
