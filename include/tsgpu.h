@@ -89,6 +89,10 @@ int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t
     int64_t *tr_off, int32_t *tr_nrem, int32_t *ev_child, int32_t *ev_parent, float *ev_bl,
     int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc, int32_t *mut_allele,
     int32_t *mut_pallele, int32_t *site_nalleles);
+/* host copy of each event's root path (local ids, the edge's parent first);
+ * ev_path0 has (events + 1) entries */
+uint64_t tsg_treeseq_get_path_size(const tsg_treeseq_t *ts);
+int tsg_treeseq_copy_paths(const tsg_treeseq_t *ts, int64_t *ev_path0, int32_t *path);
 /* HIP-event timing of the sweep kernel on the library's stream */
 int tsg_treeseq_set_profiling(tsg_treeseq_t *ts, int enable);
 int tsg_treeseq_get_profile(const tsg_treeseq_t *ts, double *sweep_ms, uint64_t *launches);

@@ -15,6 +15,10 @@ def check_plan_bounds(plan, num_trees, num_samples):
     tr_off, tr_nrem = plan["tr_off"], plan["tr_nrem"]
     assert np.all(np.diff(tr_off) >= 0)
     assert tr_off[-1] == len(plan["ev_child"])
+    ep = plan["ev_path0"]
+    assert len(ep) == len(plan["ev_child"]) + 1 and ep[0] == 0 and ep[-1] == len(plan["path"])
+    assert np.all(np.diff(ep) >= 1)  # every path starts at the edge's parent
+    assert np.array_equal(plan["path"][ep[:-1]], plan["ev_parent"])
     assert np.all(tr_nrem >= 0) and np.all(tr_nrem <= np.diff(tr_off))
     for c in range(C):
         nloc = cn0[c + 1] - cn0[c]
@@ -31,6 +35,8 @@ def check_plan_bounds(plan, num_trees, num_samples):
             for a in ("ev_child", "ev_parent"):
                 ids = plan[a][e0:e1]
                 assert np.all((ids >= 0) & (ids < nloc)), (c, a)
+            pids = plan["path"][ep[e0]:ep[e1]]
+            assert np.all((pids >= 0) & (pids < nloc)), (c, "path")
         s0, s1 = plan["tree_site0"][t0], plan["tree_site0"][t1]
         m0, m1 = plan["site_mut0"][s0], plan["site_mut0"][s1]
         mn = plan["mut_node"][m0:m1]
@@ -101,17 +107,22 @@ def run(plan, bp, windows, sets, num_samples, stat, mode, idx=None, polarised=Fa
                     par[u], bl[u] = v, plan["ev_bl"][e]
                 else:
                     par[u], bl[u] = -1, 0.0
+                # the recorded root path must be the live parent walk from v
+                p0, p1 = plan["ev_path0"][e], plan["ev_path0"][e + 1]
+                path = plan["path"][p0:p1]
+                walk = []
                 w = v
-                hops = 0
                 while w >= 0:
+                    walk.append(w)
+                    w = par[w]
+                    assert len(walk) <= nloc
+                assert list(path) == walk, (e, list(path), walk)
+                for w in path:
                     if mode == "branch":
                         R -= bl[w] * F(X[w])
                     X[w] = X[w] + xc if ins else X[w] - xc
                     if mode == "branch":
                         R += bl[w] * F(X[w])
-                    w = par[w]
-                    hops += 1
-                    assert hops <= nloc
 
         t0, t1 = ct0[c], ct0[c + 1]
         for t in range(t0, t1):

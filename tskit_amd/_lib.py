@@ -56,6 +56,8 @@ SIGNATURES = {
     "tsg_treeseq_get_index": (ctypes.c_int, [c_void, c_i32p, c_i32p]),
     "tsg_treeseq_get_plan_sizes": (ctypes.c_int, [c_void, c_u64p]),
     "tsg_treeseq_copy_plan": (ctypes.c_int, [c_void] + [c_void] * 17),
+    "tsg_treeseq_get_path_size": (ctypes.c_uint64, [c_void]),
+    "tsg_treeseq_copy_paths": (ctypes.c_int, [c_void, c_void, c_void]),
     "tsg_treeseq_set_profiling": (ctypes.c_int, [c_void, ctypes.c_int]),
     "tsg_treeseq_get_profile": (ctypes.c_int, [c_void, c_f64p, c_u64p]),
     "tsg_treeseq_diversity": (ctypes.c_int, ONE_WAY),

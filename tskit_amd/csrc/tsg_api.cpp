@@ -40,6 +40,7 @@ struct DevBuf {
 struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf seed_off;
     DevBuf pair_s, pair_c, tot, outs, queries, windows, cq_wfirst, cq_nwin, cq_poff, partial, dR,
         Rstart, raw, rawbase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
     // relatedness-matrix route
@@ -98,6 +99,37 @@ int upload_ptr(T **dst, const std::vector<T> &v, size_t &bytes) {
 }
 
 Workspace *ws_of(tsg_treeseq *ts);
+
+// Groups (sample, column) pairs by sample so each weighted sample walks its
+// root path once: seed_sample[i] has columns seed_col[seed_off[i]..seed_off[i+1]).
+int upload_seeds(Workspace *ws, const std::vector<int32_t> &pair_s,
+                 const std::vector<int32_t> &pair_c, int32_t num_samples, hipStream_t s,
+                 TsgSweepArgs &a) {
+    std::vector<int32_t> count(num_samples + 1, 0);
+    for (int32_t si : pair_s) count[si + 1]++;
+    std::vector<int32_t> first(num_samples + 1, 0);
+    for (int32_t j = 0; j < num_samples; j++) first[j + 1] = first[j] + count[j + 1];
+    std::vector<int32_t> col(pair_s.size());
+    std::vector<int32_t> fill(first.begin(), first.end() - 1);
+    for (size_t i = 0; i < pair_s.size(); i++) col[fill[pair_s[i]]++] = pair_c[i];
+    std::vector<int32_t> seed_sample, seed_off{0};
+    for (int32_t j = 0; j < num_samples; j++) {
+        if (count[j + 1] > 0) {
+            seed_sample.push_back(j);
+            seed_off.push_back(first[j + 1]);
+        }
+    }
+    int ret;
+    if ((ret = upload(ws->pair_s, seed_sample, s)) || (ret = upload(ws->pair_c, col, s)) ||
+        (ret = upload(ws->seed_off, seed_off, s))) {
+        return ret;
+    }
+    a.nseed = (int32_t) seed_sample.size();
+    a.seed_sample = (const int32_t *) ws->pair_s.p;
+    a.seed_col = (const int32_t *) ws->pair_c.p;
+    a.seed_off = (const int32_t *) ws->seed_off.p;
+    return 0;
+}
 
 // ---- query description -------------------------------------------------
 struct Block {
@@ -321,7 +353,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         rawcount += (int64_t) qs[qi].W * qs[qi].M;
     }
     hipStream_t s = ws->stream;
-    if ((ret = upload(ws->pair_s, pair_s, s)) || (ret = upload(ws->pair_c, pair_c, s)) ||
+    TsgSweepArgs a{};
+    if ((ret = upload_seeds(ws, pair_s, pair_c, ts->num_samples, s, a)) ||
         (ret = upload(ws->tot, col_total, s)) || (ret = upload(ws->outs, outs, s)) ||
         (ret = upload(ws->queries, qs, s)) || (ret = upload(ws->windows, wins, s)) ||
         (ret = upload(ws->cq_wfirst, cq_wfirst, s)) || (ret = upload(ws->cq_nwin, cq_nwin, s)) ||
@@ -336,7 +369,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         }
     }
     const int R = 2 + K;
-    const size_t stage = (size_t) 3 * K * 4 + 64;
+    // staging: xc[K], counts before/after for a path segment, its branch lengths
+    const size_t stage = (size_t) (K + 2 * TSG_PSEG * K + TSG_PSEG) * 4 + 64;
     const size_t lds_need = (size_t) ts->max_nloc * R * 4 + stage;
     const bool lds = lds_need <= (size_t) ws->lds_limit;
     if (!lds) {
@@ -344,7 +378,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         if ((ret = ensure(ws->slab, slab))) return ret;
     }
     const TsgDevice &d = ts->dev;
-    TsgSweepArgs a{};
+    a.ev_path0 = d.ev_path0;
+    a.path = d.path;
     a.chunk_tree0 = d.chunk_tree0;
     a.chunk_node0 = d.chunk_node0;
     a.loc_parent0 = d.loc_parent0;
@@ -370,9 +405,6 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     a.K = K;
     a.M = M;
     a.Q = Q;
-    a.npairs = (int32_t) pair_s.size();
-    a.pair_sample = (const int32_t *) ws->pair_s.p;
-    a.pair_col = (const int32_t *) ws->pair_c.p;
     a.col_total = (const int32_t *) ws->tot.p;
     a.outs = (const TsgOut *) ws->outs.p;
     a.queries = (const TsgQuery *) ws->queries.p;
@@ -519,6 +551,8 @@ int device_upload(tsg_treeseq *ts) {
         (ret = upload_ptr(&d.ev_child, ts->ev_child, by)) ||
         (ret = upload_ptr(&d.ev_parent, ts->ev_parent, by)) ||
         (ret = upload_ptr(&d.ev_bl, ts->ev_bl, by)) ||
+        (ret = upload_ptr(&d.ev_path0, ts->ev_path0, by)) ||
+        (ret = upload_ptr(&d.path, ts->path, by)) ||
         (ret = upload_ptr(&d.breakpoints, ts->breakpoints, by)) ||
         (ret = upload_ptr(&d.tree_site0, ts->tree_site0, by)) ||
         (ret = upload_ptr(&d.site_pos, ts->site_pos, by)) ||
@@ -536,7 +570,7 @@ int device_upload(tsg_treeseq *ts) {
 void free_workspace(tsg_treeseq *ts) {
     Workspace *ws = ws_of(ts);
     if (ws == nullptr) return;
-    DevBuf *bufs[] = {&ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
+    DevBuf *bufs[] = {&ws->seed_off, &ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
                       &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
                       &ws->Rstart, &ws->raw, &ws->rawbase, &ws->out, &ws->fin, &ws->ipar,
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
@@ -557,6 +591,7 @@ void free_workspace(tsg_treeseq *ts) {
 void tsg_device_free(TsgDevice *d) {
     void *ptrs[] = {d->chunk_tree0, d->chunk_node0, d->loc_parent0, d->loc_bl0, d->loc_time,
                     d->loc_lastrem, d->tr_off, d->tr_nrem, d->ev_child, d->ev_parent, d->ev_bl,
+                    d->ev_path0, d->path,
                     d->breakpoints, d->tree_site0, d->site_pos, d->site_mut0, d->mut_node,
                     d->mut_allele, d->mut_pallele, d->site_nalleles};
     for (void *p : ptrs) {
@@ -760,6 +795,15 @@ int tsg_treeseq_copy_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *
     cp(mut_allele, ts->mut_allele);
     cp(mut_pallele, ts->mut_pallele);
     cp(site_nalleles, ts->site_nalleles);
+    return 0;
+}
+
+// Root paths of the events (host copies): ev_path0 has events + 1 entries.
+uint64_t tsg_treeseq_get_path_size(const tsg_treeseq *ts) { return ts->path.size(); }
+
+int tsg_treeseq_copy_paths(const tsg_treeseq *ts, int64_t *ev_path0, int32_t *path) {
+    std::copy(ts->ev_path0.begin(), ts->ev_path0.end(), ev_path0);
+    std::copy(ts->path.begin(), ts->path.end(), path);
     return 0;
 }
 
@@ -1253,7 +1297,8 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
         cq_nwin[c] = wl - wf + 1;
     }
     const int64_t hist_n = (int64_t) W * TW * afs_size;
-    if ((ret = upload(ws->pair_s, pair_s, s)) || (ret = upload(ws->pair_c, pair_c, s)) ||
+    TsgSweepArgs a{};
+    if ((ret = upload_seeds(ws, pair_s, pair_c, ts->num_samples, s, a)) ||
         (ret = upload(ws->tot, col_total, s)) || (ret = upload(ws->windows, win, s)) ||
         (ret = upload(ws->twin, tw, s)) || (ret = upload(ws->dims, dims, s)) ||
         (ret = upload(ws->cq_wfirst, cq_wfirst, s)) || (ret = upload(ws->cq_nwin, cq_nwin, s)) ||
@@ -1271,7 +1316,6 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
         if (branch && (ret = ensure(ws->last, (size_t) C * ts->max_nloc * 8))) return ret;
     }
     const TsgDevice &d = ts->dev;
-    TsgSweepArgs a{};
     a.chunk_tree0 = d.chunk_tree0;
     a.chunk_node0 = d.chunk_node0;
     a.loc_parent0 = d.loc_parent0;
@@ -1283,6 +1327,8 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
     a.ev_child = d.ev_child;
     a.ev_parent = d.ev_parent;
     a.ev_bl = d.ev_bl;
+    a.ev_path0 = d.ev_path0;
+    a.path = d.path;
     a.bp = d.breakpoints;
     a.tree_site0 = d.tree_site0;
     a.site_pos = d.site_pos;
@@ -1296,9 +1342,6 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
     a.num_samples = ts->num_samples;
     a.K = KK;
     a.Q = 1;
-    a.npairs = (int32_t) pair_s.size();
-    a.pair_sample = (const int32_t *) ws->pair_s.p;
-    a.pair_col = (const int32_t *) ws->pair_c.p;
     a.col_total = (const int32_t *) ws->tot.p;
     a.windows = (const double *) ws->windows.p;
     a.cq_wfirst = (const int32_t *) ws->cq_wfirst.p;

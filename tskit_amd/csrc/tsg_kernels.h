@@ -28,6 +28,9 @@ struct TsgQuery {
     int64_t pbase;       // offset (in doubles) of its partial-sum block
 };
 
+// Hops of an event's root path staged in LDS at once by the sweep kernel.
+#define TSG_PSEG 16
+
 #define TSG_F_POLARISED 1u
 #define TSG_F_NOSKIP 2u       // a summary has a zero denominator: never skip walks
 #define TSG_F_SPAN_NORMALISE 4u
@@ -51,9 +54,13 @@ struct TsgSweepArgs {
     const int32_t *site_mut0, *mut_node, *mut_allele, *mut_pallele, *site_nalleles;
     int32_t num_trees, num_chunks, num_samples;
     // batch
-    int32_t K, M, Q, npairs;
+    int32_t K, M, Q, nseed;
     int32_t m_begin, m_end;      // outputs handled by this launch
-    const int32_t *pair_sample, *pair_col;
+    // weighted samples: seed_sample[i] belongs to the columns
+    // seed_col[seed_off[i] .. seed_off[i+1])
+    const int32_t *seed_sample, *seed_off, *seed_col;
+    const int64_t *ev_path0;     // per event: offset of its root path
+    const int32_t *path;
     const int32_t *col_total;    // [K] total weight (set size) per column
     const TsgOut *outs;          // [M]
     const TsgQuery *queries;     // [Q]

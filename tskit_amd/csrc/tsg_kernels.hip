@@ -119,21 +119,20 @@ struct Nodes {
     }
 };
 
-// Materialise the chunk's start tree and seed the counts of every
-// (sample, column) pair up its root path.
+// Materialise the chunk's start tree and seed the counts: each weighted
+// sample walks its root path once, adding 1 to every column it belongs to.
 __device__ void init_chunk(const TsgSweepArgs &a, const Nodes &nd, int64_t n0, int nloc,
                            int ncols, int lane) {
-    for (int u = lane; u < nloc; u += WAVE) {
-        nd.set_edge(u, a.loc_parent0[n0 + u], a.loc_bl0[n0 + u]);
-        for (int k = 0; k < ncols; k++) nd.x(u, k) = 0;
-    }
+    (void) ncols;
+    const int64_t words = (int64_t) nloc * nd.R;
+    for (int64_t i = lane; i < words; i += WAVE) nd.rec[i] = 0;  // coalesced
     __syncthreads();
-    for (int i = lane; i < a.npairs; i += WAVE) {
-        int u = a.pair_sample[i];
-        const int k = a.pair_col[i];
-        while (u >= 0) {
-            atomicAdd(&nd.x(u, k), 1);
-            u = nd.parent(u);
+    for (int u = lane; u < nloc; u += WAVE) nd.set_edge(u, a.loc_parent0[n0 + u], a.loc_bl0[n0 + u]);
+    __syncthreads();
+    for (int i = lane; i < a.nseed; i += WAVE) {
+        const int c0 = a.seed_off[i], c1 = a.seed_off[i + 1];
+        for (int u = a.seed_sample[i]; u >= 0; u = nd.parent(u)) {
+            for (int c = c0; c < c1; c++) atomicAdd(&nd.x(u, a.seed_col[c]), 1);
         }
     }
     __syncthreads();
@@ -205,9 +204,10 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
         nd.rec = a.g_rec + (int64_t) c * a.g_stride;
         stg = smem;
     }
-    int32_t *xc = stg;          // [K] counts of the moved subtree
-    int32_t *xo = stg + K;      // [K] counts before the update
-    int32_t *xn = stg + 2 * K;  // [K] counts after the update
+    int32_t *xc = stg;                              // [K] counts of the moved subtree
+    int32_t *xo = stg + K;                          // [PSEG x K] counts before the update
+    int32_t *xn = xo + TSG_PSEG * K;                // [PSEG x K] counts after the update
+    float *sbl = reinterpret_cast<float *>(xn + TSG_PSEG * K);  // [PSEG] branch lengths
     const bool any_unpolarised = (a.flags & TSG_F_ANY_UNPOLARISED) != 0;
     const bool noskip = (MODE == 0) && (a.flags & TSG_F_NOSKIP) != 0;
     const int32_t *tot = a.col_total;
@@ -260,35 +260,44 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             }
             __syncthreads();
             if (!walk) continue;
-            int w = v;
-            while (w >= 0) {
-                const int pw = nd.parent(w);
-                const float bw = nd.bl(w);
-                for (int k = lane; k < K; k += WAVE) {
-                    const int32_t old = nd.x(w, k);
+            // The event's root path is precomputed (v first).  All (hop,
+            // column) counts of a path segment are updated in parallel, then
+            // every output sums -bl F(old) + bl F(new) over the segment's
+            // hops -- the same terms as the reference's hop-by-hop walk
+            // (trees.c:1436-1448, 1461-1472), since a path visits each node once.
+            const int64_t p0 = a.ev_path0[e], p1 = a.ev_path0[e + 1];
+            (void) v;
+            for (int64_t sg0 = p0; sg0 < p1; sg0 += TSG_PSEG) {
+                const int h = (int) min((int64_t) TSG_PSEG, p1 - sg0);
+                for (int idx = lane; idx < h * K; idx += WAVE) {
+                    const int i = idx / K, k = idx - i * K;
+                    const int node = a.path[sg0 + i];
+                    const int32_t old = nd.x(node, k);
                     const int32_t nw = ins ? old + xc[k] : old - xc[k];
-                    nd.x(w, k) = nw;
+                    nd.x(node, k) = nw;
                     if (MODE == 0) {
-                        xo[k] = old;
-                        xn[k] = nw;
+                        xo[idx] = old;
+                        xn[idx] = nw;
                     }
                 }
                 if (MODE == 0) {
+                    for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(a.path[sg0 + i]);
                     __syncthreads();
 #pragma unroll
                     for (int j = 0; j < MPL; j++) {
                         if (lo[j].m < a.m_end) {
-                            const float fo = branch_summary(lo[j].o, xo, tot);
-                            const float fn = branch_summary(lo[j].o, xn, tot);
-                            tsum(lo[j].Rs, lo[j].Rc, -__fmul_rn(bw, fo));
-                            tsum(lo[j].Rs, lo[j].Rc, __fmul_rn(bw, fn));
+                            for (int i = 0; i < h; i++) {
+                                const float bw = sbl[i];
+                                const float fo = branch_summary(lo[j].o, xo + i * K, tot);
+                                const float fn = branch_summary(lo[j].o, xn + i * K, tot);
+                                tsum(lo[j].Rs, lo[j].Rc, -__fmul_rn(bw, fo));
+                                tsum(lo[j].Rs, lo[j].Rc, __fmul_rn(bw, fn));
+                            }
                         }
                     }
-                    __syncthreads();
                 }
-                w = pw;
+                __syncthreads();
             }
-            __syncthreads();
         }
     };
 

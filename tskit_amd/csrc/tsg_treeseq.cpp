@@ -262,6 +262,9 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
     ts->ev_child.clear();
     ts->ev_parent.clear();
     ts->ev_bl.clear();
+    ts->path.clear();
+    ts->ev_path0.assign(1, 0);
+    ts->max_path = 0;
     ts->loc_parent0.clear();
     ts->loc_bl0.clear();
     ts->loc_time.clear();
@@ -298,6 +301,16 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
             }
             return locid[u];
         };
+        // root path from p in the current tree, local ids, p first
+        auto record_path = [&](int32_t p) {
+            int32_t len = 0;
+            for (int32_t w = p; w != TSG_NULL; w = gpar[w]) {
+                ts->path.push_back(ensure(w));
+                len++;
+            }
+            ts->ev_path0.push_back((int64_t) ts->path.size());
+            ts->max_path = std::max(ts->max_path, len);
+        };
         for (int32_t s = 0; s < n; s++) ensure(ts->samples[s]);
         if (c > 0) {
             for (int32_t u : alive) {
@@ -322,6 +335,7 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
                 ts->ev_child.push_back(ensure(u));
                 ts->ev_parent.push_back(ensure(ts->parent[e]));
                 ts->ev_bl.push_back(0.0f);
+                record_path(ts->parent[e]);
                 gpar[u] = TSG_NULL;
                 glastrem[u] = x;
                 // swap-remove from alive
@@ -340,6 +354,7 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
                 ts->ev_parent.push_back(ensure(p));
                 ts->ev_bl.push_back((float) (ts->time[p] - ts->time[u]));
                 gpar[u] = p;
+                record_path(p);
                 alive_pos[u] = (int32_t) alive.size();
                 alive.push_back(u);
                 j++;

@@ -25,6 +25,8 @@ struct TsgDevice {
     int32_t *ev_child = nullptr;      // local ids in the owning chunk
     int32_t *ev_parent = nullptr;
     float *ev_bl = nullptr;           // branch length of inserted edge
+    int64_t *ev_path0 = nullptr;      // [events + 1] offsets into path
+    int32_t *path = nullptr;          // ancestors of each event's parent end
     double *breakpoints = nullptr;    // [T+1]
     int32_t *tree_site0 = nullptr;    // [T+1]
     double *site_pos = nullptr;       // [S]
@@ -71,6 +73,11 @@ struct tsg_treeseq {
     std::vector<int32_t> tr_nrem;         // [T]
     std::vector<int32_t> ev_child, ev_parent;
     std::vector<float> ev_bl;
+    // the root path of each event's parent edge end at the time of the event
+    // (local ids, parent first): events of a sweep touch exactly these nodes
+    std::vector<int64_t> ev_path0;        // [events + 1]
+    std::vector<int32_t> path;
+    int32_t max_path = 0;
     std::vector<int32_t> mut_node_loc;
     int64_t total_path_hops = 0;          // topology-only measure: sum of path lengths
 

@@ -203,6 +203,10 @@ class LowLevelTreeSequence:
         }
         self._lib.tsg_treeseq_copy_plan(self._h, *[a.ctypes.data_as(ctypes.c_void_p)
                                                    for a in p.values()])
+        p["ev_path0"] = np.zeros(ne + 1, np.int64)
+        p["path"] = np.zeros(int(self._lib.tsg_treeseq_get_path_size(self._h)), np.int32)
+        self._lib.tsg_treeseq_copy_paths(self._h, p["ev_path0"].ctypes.data_as(ctypes.c_void_p),
+                                         p["path"].ctypes.data_as(ctypes.c_void_p))
         return p
 
     def set_num_chunks(self, num_chunks):
