@@ -20,15 +20,15 @@ def check_plan_bounds(plan, num_trees, num_samples):
     assert np.all(np.diff(ep) >= 1)  # every path starts at the edge's parent
     assert np.array_equal(plan["path"][ep[:-1]], plan["ev_parent"])
     assert np.all(tr_nrem >= 0) and np.all(tr_nrem <= np.diff(tr_off))
+    # the opening transition is applied by the planner: no stored events
+    assert tr_off[0] == 0 and tr_off[1] == 0
     for c in range(C):
         nloc = cn0[c + 1] - cn0[c]
         assert nloc >= num_samples
         par = plan["loc_parent0"][cn0[c]:cn0[c + 1]]
         assert np.all((par >= -1) & (par < nloc))
-        if c == 0:
-            assert np.all(par == -1)
         t0, t1 = ct0[c], ct0[c + 1]
-        first = 0 if c == 0 else t0 + 1
+        first = t0 + 1
         last = min(t1, num_trees - 1)
         if first <= last:
             e0, e1 = tr_off[first], tr_off[last + 1]
@@ -91,6 +91,13 @@ def run(plan, bp, windows, sets, num_samples, stat, mode, idx=None, polarised=Fa
         def F(x):
             return _summary(stat, x, n, idx, polarised, total)
 
+        if c == 0:
+            # running sum of the first tree, evaluated over its nodes
+            R0 = np.zeros(M)
+            for u in range(nloc):
+                if par[u] >= 0:
+                    R0 += bl[u] * F(X[u])
+
         def transition(t):
             nonlocal R
             e0, e1 = plan["tr_off"][t], plan["tr_off"][t + 1]
@@ -126,7 +133,7 @@ def run(plan, bp, windows, sets, num_samples, stat, mode, idx=None, polarised=Fa
 
         t0, t1 = ct0[c], ct0[c + 1]
         for t in range(t0, t1):
-            if t > t0 or c == 0:
+            if t > t0:
                 transition(t)
             if mode == "branch":
                 tl, tr = bp[t], bp[t + 1]
@@ -164,7 +171,7 @@ def run(plan, bp, windows, sets, num_samples, stat, mode, idx=None, polarised=Fa
         out += A
         Rstart_terms.append((bp[t0], bp[t1]))
     if mode == "branch":
-        Rs = np.vstack([np.zeros(M), np.cumsum(dR, axis=0)[:-1]])
+        Rs = R0 + np.vstack([np.zeros(M), np.cumsum(dR, axis=0)[:-1]])
         for c, (cs, ce) in enumerate(Rstart_terms):
             for w in range(W):
                 ov = min(ce, windows[w + 1]) - max(cs, windows[w])

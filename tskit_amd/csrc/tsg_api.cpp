@@ -43,6 +43,8 @@ struct Workspace {
     DevBuf seed_off;
     DevBuf pair_s, pair_c, tot, outs, queries, windows, cq_wfirst, cq_nwin, cq_poff, partial, dR,
         Rstart, raw, rawbase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
+    // first tree: counts and per-block running-sum partials
+    DevBuf x0, r0part;
     // relatedness-matrix route
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab;
     int lds_limit = 65536;
@@ -363,8 +365,13 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         (ret = ensure(ws->raw, (size_t) rawcount * 8))) {
         return ret;
     }
+    const int nloc0 = (int) (ts->chunk_node0[1] - ts->chunk_node0[0]);
+    const int nb0 = tsg_tree0_blocks(nloc0);
     if (mode == TSG_STAT_BRANCH) {
-        if ((ret = ensure(ws->dR, (size_t) C * M * 8)) || (ret = ensure(ws->Rstart, (size_t) C * M * 8))) {
+        if ((ret = ensure(ws->dR, (size_t) C * M * 8)) ||
+            (ret = ensure(ws->Rstart, (size_t) C * M * 8)) ||
+            (ret = ensure(ws->x0, (size_t) nloc0 * K * 4)) ||
+            (ret = ensure(ws->r0part, (size_t) nb0 * M * 8))) {
             return ret;
         }
     }
@@ -428,6 +435,12 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         if ((ret = hip_check(tsg_launch_sweep(a, mode_id, lds, mpl, shmem, s)))) return ret;
     }
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);
+    if (mode == TSG_STAT_BRANCH) {
+        if ((ret = hip_check(tsg_launch_tree0(a, (int32_t *) ws->x0.p, nloc0,
+                                              (double *) ws->r0part.p, s)))) {
+            return ret;
+        }
+    }
     TsgCombineArgs ca{};
     ca.bp = d.breakpoints;
     ca.chunk_tree0 = d.chunk_tree0;
@@ -445,7 +458,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     ca.rawbase = (const int64_t *) ws->rawbase.p;
     ca.flags = 0;
     if ((ret = hip_check(tsg_launch_combine(ca, (mode == TSG_STAT_BRANCH) ? a.dR : nullptr,
-                                            (double *) ws->Rstart.p, rawcount, s)))) {
+                                            (double *) ws->Rstart.p,
+                                            (const double *) ws->r0part.p, nb0, rawcount, s)))) {
         return ret;
     }
     // finals
@@ -575,7 +589,8 @@ void free_workspace(tsg_treeseq *ts) {
                       &ws->Rstart, &ws->raw, &ws->rawbase, &ws->out, &ws->fin, &ws->ipar,
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
                       &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
-                      &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab};
+                      &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->x0,
+                      &ws->r0part};
     for (DevBuf *b : bufs) {
         if (b->p != nullptr) (void) hipFree(b->p);
     }

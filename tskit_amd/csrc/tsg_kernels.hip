@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
 
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
     for (int t = t0; t < t1; t++) {
-        if (t > t0 || c == 0) apply_transition(t);
+        if (t > t0) apply_transition(t);
         if (MODE == 0) {
             const double tl = a.bp[t], tr = a.bp[t + 1];
 #pragma unroll
@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(WAVE) k_afs_site(const TsgSweepArgs a) {
     int wi = a.cq_wfirst[c];
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
     for (int t = t0; t < t1; t++) {
-        if (t > t0 || c == 0) {
+        if (t > t0) {
             const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
             const int64_t nrem = a.tr_nrem[t];
             for (int64_t e = e0; e < e1; e++) {
@@ -579,7 +579,7 @@ __global__ void __launch_bounds__(WAVE) k_afs_branch(const TsgSweepArgs a) {
     };
 
     for (int t = t0; t < t1; t++) {
-        if (t > t0 || c == 0) {
+        if (t > t0) {
             const double x = a.bp[t];
             const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
             const int64_t nrem = a.tr_nrem[t];
@@ -622,13 +622,56 @@ __global__ void __launch_bounds__(WAVE) k_afs_branch(const TsgSweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Running branch sum of the first tree (chunk 0's start tree), evaluated in
+// parallel instead of by replaying its edge insertions one at a time:
+//   R(tree 0) = sum over nodes u with a parent of bl(u) * F(x(u)),
+// which is what the reference's insertion loop accumulates (trees.c:1426-
+// 1480, update_running_sum 1339-1350).  Each term is the FP32 product the
+// sweep adds for the same edge; the sum is FP64, per block of nodes, in a
+// fixed order (deterministic).
+//
+// Counts: one thread per weighted sample walks its root path in the
+// snapshot, adding 1 to each of its columns.
+__global__ void k_tree0_counts(const TsgSweepArgs a, int32_t *x0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nseed) return;
+    const int c0 = a.seed_off[i], c1 = a.seed_off[i + 1];
+    for (int u = a.seed_sample[i]; u >= 0; u = a.loc_parent0[u]) {
+        for (int c = c0; c < c1; c++) atomicAdd(&x0[(int64_t) u * a.K + a.seed_col[c]], 1);
+    }
+}
+
+#define TSG_T0_NODES 256
+
+// grid (ceil(nloc / TSG_T0_NODES), ceil(M / 64)); lane = output.
+__global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *x0,
+                                                    int nloc, double *part) {
+    const int m = blockIdx.y * WAVE + threadIdx.x;
+    if (m >= a.M) return;
+    const int u0 = blockIdx.x * TSG_T0_NODES;
+    const int u1 = min(nloc, u0 + TSG_T0_NODES);
+    const TsgOut o = a.outs[m];
+    double acc = 0;
+    for (int u = u0; u < u1; u++) {
+        if (a.loc_parent0[u] < 0) continue;
+        const float f = branch_summary(o, x0 + (int64_t) u * a.K, a.col_total);
+        acc += (double) __fmul_rn(a.loc_bl0[u], f);
+    }
+    part[(int64_t) blockIdx.x * a.M + m] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // Combine: per (query, window, output), sum the chunk partials of the chunks
 // that overlap the window, re-base branch sums by the chunk start value, and
 // span-normalise.  FP64, deterministic chunk order.
-__global__ void k_scan_dR(const double *dR, double *Rstart, int C, int M) {
+// Rstart[c] = R(first tree) + sum of the chunk deltas before c; the first
+// tree's running sum arrives as nb per-block partials (k_tree0_sum).
+__global__ void k_scan_dR(const double *dR, double *Rstart, int C, int M, const double *r0part,
+                          int nb) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
     double acc = 0;
+    for (int b = 0; b < nb; b++) acc += r0part[(int64_t) b * M + m];
     for (int c = 0; c < C; c++) {
         Rstart[(int64_t) c * M + m] = acc;
         acc += dR[(int64_t) c * M + m];
@@ -766,6 +809,11 @@ __global__ void k_fill_f64(double *p, int64_t n) {
     if (i < n) p[i] = 0.0;
 }
 
+__global__ void k_fill_i32(int32_t *p, int64_t n) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers (C++ linkage, used by tsg_api.cpp).
 #include "tsg_launch.h"
@@ -821,11 +869,31 @@ hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t s
     return hipGetLastError();
 }
 
+int tsg_tree0_blocks(int nloc) { return (nloc + TSG_T0_NODES - 1) / TSG_T0_NODES; }
+
+hipError_t tsg_launch_tree0(const TsgSweepArgs &a, int32_t *x0, int nloc, double *part,
+                            hipStream_t s) {
+    const int64_t words = (int64_t) nloc * a.K;
+    if (words > 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned) ((words + 255) / 256)), dim3(256), 0, s, x0,
+                           words);
+    }
+    if (a.nseed > 0) {
+        hipLaunchKernelGGL(k_tree0_counts, dim3((a.nseed + 255) / 256), dim3(256), 0, s, a, x0);
+    }
+    const int nb = tsg_tree0_blocks(nloc);
+    if (nb > 0 && a.M > 0) {
+        hipLaunchKernelGGL(k_tree0_sum, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a, x0,
+                           nloc, part);
+    }
+    return hipGetLastError();
+}
+
 hipError_t tsg_launch_combine(const TsgCombineArgs &a, const double *dR, double *Rstart,
-                              int64_t total, hipStream_t s) {
+                              const double *r0part, int nb, int64_t total, hipStream_t s) {
     if (dR != nullptr) {
         hipLaunchKernelGGL(k_scan_dR, dim3((a.M + 63) / 64), dim3(64), 0, s, dR, Rstart,
-                           a.num_chunks, a.M);
+                           a.num_chunks, a.M, r0part, nb);
     }
     const int64_t blocks = (total + 255) / 256;
     hipLaunchKernelGGL(k_combine, dim3((unsigned) blocks), dim3(256), 0, s, a, total);

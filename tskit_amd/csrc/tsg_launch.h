@@ -8,8 +8,14 @@ hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, bool lds, int mpl, 
                             hipStream_t s);
 hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t shmem,
                           hipStream_t s);
+// number of per-block partials tsg_launch_tree0 writes (each a.M doubles)
+int tsg_tree0_blocks(int nloc);
+// counts of chunk 0's start tree into x0 [nloc x K] and its running branch
+// sum as per-block partials part [tsg_tree0_blocks(nloc) x M]
+hipError_t tsg_launch_tree0(const TsgSweepArgs &a, int32_t *x0, int nloc, double *part,
+                            hipStream_t s);
 hipError_t tsg_launch_combine(const TsgCombineArgs &a, const double *dR, double *Rstart,
-                              int64_t total, hipStream_t s);
+                              const double *r0part, int nb, int64_t total, hipStream_t s);
 hipError_t tsg_launch_finals(const void *fin, int nq, const double *raw, const int32_t *ipar,
                              const double *dpar, double *out, int64_t total, hipStream_t s);
 hipError_t tsg_launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);

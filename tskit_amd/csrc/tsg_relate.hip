@@ -61,7 +61,7 @@ __device__ void geno_bits_chunk(const TsgGenoArgs &a, int c, uint32_t *rec) {
     __syncthreads();
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
     for (int t = t0; t < t1; t++) {
-        if (t > t0 || c == 0) {
+        if (t > t0) {
             const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
             const int64_t nrem = a.tr_nrem[t];
             for (int64_t e = e0; e < e1; e++) {

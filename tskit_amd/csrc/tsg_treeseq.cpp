@@ -210,7 +210,11 @@ int tsg_init_trees(tsg_treeseq *ts, const char *ancestral_state,
 //   * a local numbering of every node it touches (samples first, so local id
 //     j is sample index j in every chunk),
 //   * the parent / branch length of each local node in its start tree
-//     (chunk 0 starts from the empty forest and owns the opening transition),
+//     (every chunk, chunk 0 included, starts from a snapshot: the opening
+//     transition -- the insertion of every edge with left == 0 -- is applied
+//     here and stores no events, so no chunk replays a whole tree; the
+//     running branch sum of the first tree is evaluated separately, in
+//     parallel over its nodes),
 //   * the transitions it applies: into each of its trees after the first,
 //     and the closing transition into the next chunk's first tree (so the
 //     running branch sum at the end of chunk c equals the one at the start
@@ -234,6 +238,7 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
             }
         }
     }
+    tr_events[0] = 0;  // the opening transition is not replayed by any chunk
     int64_t total_events = 0;
     for (int32_t t = 0; t < T; t++) total_events += tr_events[t];
     int32_t C = std::max(1, std::min(num_chunks, T));
@@ -284,6 +289,14 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
     std::vector<int32_t> snap_parent;
     std::vector<float> snap_bl;
     int32_t j = 0, k = 0;  // index cursors (insertion / removal)
+    // opening transition: the first tree's edges (no removals at x = 0)
+    while (j < E && ts->left[ts->I[j]] == ts->breakpoints[0]) {
+        const int32_t e = ts->I[j];
+        gpar[ts->child[e]] = ts->parent[e];
+        alive_pos[ts->child[e]] = (int32_t) alive.size();
+        alive.push_back(ts->child[e]);
+        j++;
+    }
 
     for (int32_t c = 0; c < C; c++) {
         loc2glob.clear();
@@ -312,17 +325,15 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
             ts->max_path = std::max(ts->max_path, len);
         };
         for (int32_t s = 0; s < n; s++) ensure(ts->samples[s]);
-        if (c > 0) {
-            for (int32_t u : alive) {
-                const int32_t lu = ensure(u);
-                const int32_t lp = ensure(gpar[u]);
-                snap_parent[lu] = lp;
-                snap_bl[lu] = (float) (ts->time[gpar[u]] - ts->time[u]);
-            }
+        for (int32_t u : alive) {
+            const int32_t lu = ensure(u);
+            const int32_t lp = ensure(gpar[u]);
+            snap_parent[lu] = lp;
+            snap_bl[lu] = (float) (ts->time[gpar[u]] - ts->time[u]);
         }
         const int32_t t0 = ts->chunk_tree0[c], t1 = ts->chunk_tree0[c + 1];
-        // transitions owned by this chunk
-        int32_t tfirst = (c == 0) ? 0 : t0 + 1;
+        // transitions owned by this chunk (transition 0 is the opening one)
+        int32_t tfirst = t0 + 1;
         int32_t tlast = std::min(t1, T - 1);  // inclusive; t1 is the closing one
         int64_t chunk_events = 0;
         for (int32_t t = tfirst; t <= tlast; t++) {
