@@ -138,9 +138,12 @@ def main():
     torch.cuda.synchronize()
     t_upload = time.perf_counter() - t0
     ll = ts.ll_tree_sequence
+    hp = ll.get_hot_plan_sizes()
+    props = torch.cuda.get_device_properties(0)
     log(f"host init {t_init:.1f}s upload {t_upload:.2f}s trees {ts.num_trees} "
-        f"chunks {ll.get_num_chunks()} max chunk nodes {ll.get_max_chunk_nodes()} "
-        f"device bytes {ll.get_device_bytes() / 2**20:.0f} MiB")
+        f"hot chunks {hp['chunks']} max slots {hp['max_slots']} max levels {hp['max_levels']} "
+        f"device bytes {ll.get_device_bytes() / 2**20:.0f} MiB "
+        f"LDS/block {getattr(props, 'shared_memory_per_block', 'n/a')}")
 
     sets = ta.random_sample_sets(cfg["n"], cfg["K"], cfg["set_size"], cfg["seed"])
     windows = np.linspace(0, cfg["L"], cfg["W"] + 1)
@@ -154,7 +157,7 @@ def main():
 
     for i in range(args.warmup):
         run_gpu_step(ts, sets, windows, pairs, quads)
-        log(f"warmup {i + 1}/{args.warmup} done")
+        log(f"warmup {i + 1}/{args.warmup} done (node store {ll.get_last_store()})")
     ll.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -210,7 +213,8 @@ def main():
         "data": "synthetic (own Hudson coalescent + infinite-sites generator)",
         "config": {"workload": cfg["name"], "edges": E, "trees": ts.num_trees, "sites": S,
                    "nodes": N, "queries_per_step": queries_per_step,
-                   "chunks": ll.get_num_chunks()},
+                   "chunks": hp["chunks"], "node_store": ["hbm", "lds", "lds-hot"][
+                       ll.get_last_store()]},
         "queries_per_s": queries_per_step * args.steps / elapsed,
         "roofline": roofline,
         "cpu_baseline": cpu,

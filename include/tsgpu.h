@@ -93,6 +93,21 @@ int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t
  * ev_path0 has (events + 1) entries */
 uint64_t tsg_treeseq_get_path_size(const tsg_treeseq_t *ts);
 int tsg_treeseq_copy_paths(const tsg_treeseq_t *ts, int64_t *ev_path0, int32_t *path);
+/* host copy of the hot plan (chunks keep only the nodes they touch, as LDS
+ * slots); sizes = {usable, chunks, slots, level entries, events, path
+ * entries, first-tree nodes, max slots per chunk, max levels per chunk} */
+int tsg_treeseq_get_hot_plan_sizes(const tsg_treeseq_t *ts, uint64_t *sizes);
+int tsg_treeseq_copy_hot_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0,
+    int64_t *chunk_slot0, int32_t *slot_par, float *slot_bl0, int64_t *chunk_lev0,
+    int32_t *lev_slot0, uint16_t *sample_slot, int64_t *tr_off, int32_t *tr_nrem,
+    int32_t *ev_child, float *ev_bl, int64_t *ev_path0, int32_t *path, int32_t *mut_slot,
+    int32_t *t0_parent, float *t0_bl);
+/* plan = 1: run statistics sweeps on the full plan; 0: the library chooses
+ * (the hot plan whenever its slots fit in LDS) -- a testing hook */
+int tsg_treeseq_set_sweep_plan(tsg_treeseq_t *ts, int plan);
+/* node store of the last statistics sweep: 0 HBM slabs, 1 LDS (full plan),
+ * 2 LDS hot slots, -1 before the first sweep */
+int tsg_treeseq_get_last_store(const tsg_treeseq_t *ts);
 /* HIP-event timing of the sweep kernel on the library's stream */
 int tsg_treeseq_set_profiling(tsg_treeseq_t *ts, int enable);
 int tsg_treeseq_get_profile(const tsg_treeseq_t *ts, double *sweep_ms, uint64_t *launches);

@@ -178,3 +178,174 @@ def run(plan, bp, windows, sets, num_samples, stat, mode, idx=None, polarised=Fa
                 if ov > 0:
                     out[w] += Rs[c] * ov
     return out
+
+
+NO_SLOT = 0xFFFF
+
+
+def check_hot_plan_bounds(hp, num_trees, num_samples, num_mutations):
+    """Every index the hot sweep kernel (k_sweep with the hot store) reads is
+    inside its chunk's slot range, and the level order is valid for the
+    bottom-up seeding."""
+    ct0, cs0, cl0 = hp["chunk_tree0"], hp["chunk_slot0"], hp["chunk_lev0"]
+    C = len(ct0) - 1
+    n = num_samples
+    assert ct0[0] == 0 and ct0[-1] == num_trees and np.all(np.diff(ct0) >= 1)
+    assert cs0[0] == 0 and cs0[-1] == len(hp["slot_par"]) == len(hp["slot_bl0"])
+    assert cl0[0] == 0 and cl0[-1] == len(hp["lev_slot0"])
+    assert len(hp["sample_slot"]) == C * n
+    tr_off, tr_nrem, ep = hp["tr_off"], hp["tr_nrem"], hp["ev_path0"]
+    assert tr_off[0] == 0 and tr_off[1] == 0 and tr_off[-1] == len(hp["ev_child"])
+    assert np.all(np.diff(tr_off) >= 0) and np.all(tr_nrem <= np.diff(tr_off))
+    assert ep[0] == 0 and ep[-1] == len(hp["path"]) and np.all(np.diff(ep) >= 1)
+    assert np.all(hp["mut_slot"] >= 0) or num_mutations == 0
+    t0p = hp["t0_parent"]
+    assert len(t0p) >= n and np.all((t0p >= -1) & (t0p < len(t0p)))
+    for c in range(C):
+        S = cs0[c + 1] - cs0[c]
+        assert S <= hp["max_slots"]
+        par = hp["slot_par"][cs0[c]:cs0[c + 1]]
+        assert np.all((par >= -1) & (par < S))
+        lev = hp["lev_slot0"][cl0[c]:cl0[c + 1]]
+        assert lev[0] == 0 and lev[-1] == S and np.all(np.diff(lev) >= 0)
+        level = np.zeros(S, np.int64)
+        for j in range(len(lev) - 1):
+            level[lev[j]:lev[j + 1]] = j
+        nlev = len(lev) - 1
+        # a parent is exactly one level up; the roots of the hot forest, and
+        # only they, form the last level (which is not propagated)
+        has_par = par >= 0
+        assert np.all(level[par[has_par]] == level[has_par] + 1)
+        assert np.array_equal(~has_par, level == nlev - 1)
+        ss = hp["sample_slot"][c * n:(c + 1) * n].astype(np.int64)
+        assert np.all((ss < S) | (ss == NO_SLOT))
+        t0, t1 = ct0[c], ct0[c + 1]
+        first, last = t0 + 1, min(t1, num_trees - 1)
+        if first <= last:
+            e0, e1 = tr_off[first], tr_off[last + 1]
+            ids = hp["ev_child"][e0:e1]
+            assert np.all((ids >= 0) & (ids < S)), c
+            pids = hp["path"][ep[e0]:ep[e1]]
+            assert np.all((pids >= 0) & (pids < S)), c
+    return True
+
+
+def run_hot(hp, plan_sites, bp, windows, sets, num_samples, stat, mode, idx=None,
+            polarised=False):
+    """The hot sweep program executed in FP64 as the kernel does it: slots
+    seeded from sample_slot and the level order, events through the recorded
+    paths, the first tree's running sum over t0_parent.  plan_sites holds the
+    site arrays (tree_site0, site_mut0, mut_allele, mut_pallele,
+    site_nalleles, _site_pos).  Returns the raw [W, M] statistic."""
+    ct0, cs0, cl0 = hp["chunk_tree0"], hp["chunk_slot0"], hp["chunk_lev0"]
+    C = len(ct0) - 1
+    T = len(bp) - 1
+    K = len(sets)
+    n = np.array([len(s) for s in sets], dtype=np.float64)
+    total = n.copy()
+    if idx is not None:
+        idx = np.asarray(idx)
+    M = K if idx is None else len(idx)
+    W = len(windows) - 1
+    nsamp = num_samples
+    weight = np.zeros((nsamp, K))
+    for k, s in enumerate(sets):
+        weight[np.asarray(s), k] = 1
+
+    def F(x):
+        return _summary(stat, x, n, idx, polarised, total)
+
+    # the first tree's running sum
+    t0p, t0b = hp["t0_parent"], hp["t0_bl"].astype(np.float64)
+    X0 = np.zeros((len(t0p), K))
+    for s in range(nsamp):
+        u = s
+        while u >= 0:
+            X0[u] += weight[s]
+            u = t0p[u]
+    R0 = np.zeros(M)
+    for u in range(len(t0p)):
+        if t0p[u] >= 0:
+            R0 += t0b[u] * F(X0[u])
+
+    out = np.zeros((W, M))
+    dR = np.zeros((C, M))
+    spans = []
+    for c in range(C):
+        S = cs0[c + 1] - cs0[c]
+        par = hp["slot_par"][cs0[c]:cs0[c + 1]]
+        bl = hp["slot_bl0"][cs0[c]:cs0[c + 1]].astype(np.float64).copy()
+        X = np.zeros((S, K))
+        ss = hp["sample_slot"][c * nsamp:(c + 1) * nsamp]
+        for s in range(nsamp):
+            if ss[s] != NO_SLOT:
+                X[ss[s]] += weight[s]
+        lev = hp["lev_slot0"][cl0[c]:cl0[c + 1]]
+        for j in range(len(lev) - 2):
+            for h in range(lev[j], lev[j + 1]):
+                if par[h] >= 0:
+                    X[par[h]] += X[h]
+        R = np.zeros(M)
+        A = np.zeros((W, M))
+
+        def transition(t):
+            nonlocal R
+            e0, e1 = hp["tr_off"][t], hp["tr_off"][t + 1]
+            nrem = hp["tr_nrem"][t]
+            for e in range(e0, e1):
+                ins = (e - e0) >= nrem
+                u = hp["ev_child"][e]
+                xc = X[u].copy()
+                if mode == "branch":
+                    b = hp["ev_bl"][e] if ins else bl[u]
+                    R += (1 if ins else -1) * b * F(xc)
+                bl[u] = hp["ev_bl"][e] if ins else 0.0
+                for w in hp["path"][hp["ev_path0"][e]:hp["ev_path0"][e + 1]]:
+                    if mode == "branch":
+                        R -= bl[w] * F(X[w])
+                    X[w] = X[w] + xc if ins else X[w] - xc
+                    if mode == "branch":
+                        R += bl[w] * F(X[w])
+
+        t0, t1 = ct0[c], ct0[c + 1]
+        for t in range(t0, t1):
+            if t > t0:
+                transition(t)
+            if mode == "branch":
+                tl, tr = bp[t], bp[t + 1]
+                for w in range(W):
+                    lo, hi = max(tl, windows[w]), min(tr, windows[w + 1])
+                    if hi > lo:
+                        A[w] += R * (hi - lo)
+            else:
+                ps = plan_sites
+                for s in range(ps["tree_site0"][t], ps["tree_site0"][t + 1]):
+                    m0, m1 = ps["site_mut0"][s], ps["site_mut0"][s + 1]
+                    val = np.zeros(M)
+                    for al in range(1 if polarised else 0, ps["site_nalleles"][s]):
+                        aw = total.copy() if al == 0 else np.zeros(K)
+                        for mm in range(m0, m1):
+                            h = hp["mut_slot"][mm]
+                            if ps["mut_allele"][mm] == al:
+                                aw += X[h]
+                            if ps["mut_pallele"][mm] == al:
+                                aw -= X[h]
+                        if stat == "diversity":
+                            val += aw * (n - aw) / (n * (n - 1))
+                        else:
+                            i, j = idx[:, 0], idx[:, 1]
+                            val += aw[i] * (n[j] - aw[j]) / (n[i] * (n[j] - (i == j)))
+                    A[np.searchsorted(windows, ps["_site_pos"][s], side="right") - 1] += val
+        if mode == "branch" and t1 < T:
+            transition(t1)
+        dR[c] = R
+        out += A
+        spans.append((bp[t0], bp[t1]))
+    if mode == "branch":
+        Rs = R0 + np.vstack([np.zeros(M), np.cumsum(dR, axis=0)[:-1]])
+        for c, (cs, ce) in enumerate(spans):
+            for w in range(W):
+                ov = min(ce, windows[w + 1]) - max(cs, windows[w])
+                if ov > 0:
+                    out[w] += Rs[c] * ov
+    return out

@@ -172,18 +172,26 @@ def test_chunk_counts(require_gpu, num_chunks):
                orc.stat("f4", sets + [sets[0]], [(0, 1, 2, 3)], w, mode)[:, 0])
 
 
-def test_global_memory_path(require_gpu):
-    """Chunks too large for LDS run from per-chunk HBM slabs."""
+@pytest.mark.parametrize("plan", ["hot", "full"])
+def test_node_stores(require_gpu, plan):
+    """The same statistics from the hot plan (slots in LDS) and from the full
+    plan, whose chunks are too large for LDS here and run from per-chunk HBM
+    slabs."""
     tc = cases.generated(3000, 1e6, 13)
     ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    ll.set_sweep_plan(1 if plan == "full" else 0)
     orc = Oracle(tc)
     sets = cases.sample_sets(3000, 8, 100, 6)
     w = np.linspace(0, tc.sequence_length, 5)
     idx = [(i, j) for i in range(8) for j in range(i + 1, 8)]
     for mode in ["branch", "site"]:
-        _say("global", mode)
+        _say("store", plan, mode)
         _check("divergence", ts.divergence(sets, indexes=idx, windows=w, mode=mode),
                orc.stat("divergence", sets, idx, w, mode))
+        assert ll.get_last_store() == (0 if plan == "full" else 2)
+        _check("f4", ts.f4(sets[:4], windows=w, mode=mode),
+               orc.stat("f4", sets[:4], [(0, 1, 2, 3)], w, mode)[:, 0])
 
 
 @pytest.mark.parametrize("fixture", ["paper_ex", "single_tree_ex", "nonbinary_ex"])
@@ -286,6 +294,48 @@ def test_relatedness_matrix_subset(require_gpu):
     n = len(sel)
     exp = (C - r[:, None] / n - r[None, :] / n + r.sum() / n**2) / tc.sequence_length
     assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
+def test_batch_shared_columns(small, mode):
+    """A fused batch whose queries share sample sets (one state column each)
+    gives the single-query results and the oracle's, including divergence
+    between two indices holding the same samples (its denominator depends on
+    the index, not the content) and a centred relatedness query, which keeps
+    its own columns."""
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, 4, 30, 11)
+    same = [sets[0], sets[0][::-1].copy(), sets[1]]
+    w = np.linspace(0, tc.sequence_length, 6)
+    div_idx = [(0, 1), (0, 0), (1, 2)]
+    b = ts.batch()
+    p = [
+        b.diversity(sets, windows=w, mode=mode),
+        b.divergence(same, indexes=div_idx, windows=w, mode=mode),
+        b.Fst(sets, indexes=[(0, 1), (2, 3)], windows=w, mode=mode),
+        b.genetic_relatedness(sets, indexes=[(0, 1), (2, 2)], windows=w, mode=mode,
+                              polarised=True, proportion=False),
+        b.f4(sets, indexes=[(0, 1, 2, 3)], windows=w, mode=mode),
+    ]
+    res = b.run()
+    _say("batch", mode)
+    singles = [
+        ts.diversity(sets, windows=w, mode=mode),
+        ts.divergence(same, indexes=div_idx, windows=w, mode=mode),
+        ts.Fst(sets, indexes=[(0, 1), (2, 3)], windows=w, mode=mode),
+        ts.genetic_relatedness(sets, indexes=[(0, 1), (2, 2)], windows=w, mode=mode,
+                               polarised=True, proportion=False),
+        ts.f4(sets, indexes=[(0, 1, 2, 3)], windows=w, mode=mode),
+    ]
+    assert len(p) == len(res)
+    for got, one in zip(res, singles):
+        scale = np.maximum(np.max(np.abs(one)), 1e-300)
+        assert np.max(np.abs(got - one)) <= 1e-6 * scale
+    _check("diversity", res[0], orc.stat("diversity", sets, None, w, mode))
+    _check("divergence", res[1], orc.stat("divergence", same, div_idx, w, mode))
+    _check("genetic_relatedness", res[3],
+           orc.stat("genetic_relatedness", sets, [(0, 1), (2, 2)], w, mode, True, True))
+    _check("f4", res[4], orc.stat("f4", sets, [(0, 1, 2, 3)], w, mode))
 
 
 def test_repeatable(small):

@@ -221,6 +221,48 @@ def test_plan_executes_to_oracle(n, L, seed, chunks):
                 assert np.max(np.abs(got - exp)) / np.max(np.abs(exp)) < 1e-7
 
 
+@pytest.mark.parametrize("n,L,seed,chunks", [
+    (200, 2e5, 3, None), (200, 2e5, 3, 7), (60, 5e4, 5, 1), (300, 3e5, 9, 50),
+    (120, 1e5, 11, 100000),
+])
+def test_hot_plan_executes_to_oracle(n, L, seed, chunks):
+    """The hot plan (slots seeded from the samples' nearest hot ancestors and
+    the hot forest's levels) reproduces the oracle, and every index the
+    kernel reads is in range."""
+    tc = cases.generated(n, L, seed)
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    if chunks:
+        ll.set_num_chunks(chunks)
+    hp = ll.get_hot_plan()
+    assert hp is not None
+    plan_emulator.check_hot_plan_bounds(hp, ts.num_trees, ts.num_samples,
+                                        ll.get_num_mutations())
+    full = ll.get_plan()
+    sites = {k: full[k] for k in ("tree_site0", "site_mut0", "mut_allele", "mut_pallele",
+                                  "site_nalleles")}
+    sites["_site_pos"] = tc.site_position
+    bp = ts.breakpoints(as_array=True)
+    orc = Oracle(tc)
+    sets = cases.sample_sets(n, 2, n // 3, seed)
+    w = np.linspace(0, L, 6)
+    for mode in ["branch", "site"]:
+        for stat, idx in [("diversity", None), ("divergence", [(0, 1), (1, 1)])]:
+            for pol in [False, True]:
+                got = plan_emulator.run_hot(hp, sites, bp, w, sets, n, stat, mode, idx, pol)
+                exp = orc.stat(stat, sets, idx, w, mode, False, pol)
+                assert np.max(np.abs(got - exp)) / np.max(np.abs(exp)) < 1e-7
+
+
+def test_hot_plan_bounds_config1():
+    tc = cases.config1()
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    hp = ll.get_hot_plan()
+    plan_emulator.check_hot_plan_bounds(hp, ts.num_trees, ts.num_samples,
+                                        ll.get_num_mutations())
+
+
 def test_plan_bounds_config1():
     tc = cases.config1()
     ts = tc.tree_sequence()

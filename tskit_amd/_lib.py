@@ -58,6 +58,10 @@ SIGNATURES = {
     "tsg_treeseq_copy_plan": (ctypes.c_int, [c_void] + [c_void] * 17),
     "tsg_treeseq_get_path_size": (ctypes.c_uint64, [c_void]),
     "tsg_treeseq_copy_paths": (ctypes.c_int, [c_void, c_void, c_void]),
+    "tsg_treeseq_get_hot_plan_sizes": (ctypes.c_int, [c_void, c_u64p]),
+    "tsg_treeseq_copy_hot_plan": (ctypes.c_int, [c_void] + [c_void] * 16),
+    "tsg_treeseq_set_sweep_plan": (ctypes.c_int, [c_void, ctypes.c_int]),
+    "tsg_treeseq_get_last_store": (ctypes.c_int, [c_void]),
     "tsg_treeseq_set_profiling": (ctypes.c_int, [c_void, ctypes.c_int]),
     "tsg_treeseq_get_profile": (ctypes.c_int, [c_void, c_f64p, c_u64p]),
     "tsg_treeseq_diversity": (ctypes.c_int, ONE_WAY),

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -47,7 +48,9 @@ struct Workspace {
     DevBuf x0, r0part;
     // relatedness-matrix route
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab;
-    int lds_limit = 65536;
+    int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
+    bool force_full = false;        // run sweeps on the full plan (testing hook)
+    int last_store = -1;            // node store of the last sweep (TSG_STORE_*)
     // profiling of the dominant (sweep) kernel
     bool profile = false;
     double sweep_ms = 0;
@@ -229,7 +232,10 @@ int check_tuples(uint64_t K, int tuple, uint64_t M, const int32_t *idx) {  // tr
     return 0;
 }
 
-double summary_inv(int stat, const int32_t *n, const int32_t *c) {
+// c: state columns of the summary's sets; same01: whether the first two
+// sets are the same sample set *index* of the query (divergence's
+// denominator, trees.c:4286-4311, depends on the index, not the content)
+double summary_inv(int stat, const int32_t *n, const int32_t *c, bool same01) {
     auto N = [&](int t) { return (double) n[c[t]]; };
     switch (stat) {
     case TSG_STAT_DIVERSITY:
@@ -239,7 +245,7 @@ double summary_inv(int stat, const int32_t *n, const int32_t *c) {
     case TSG_STAT_Y1:
         return 1.0 / (N(0) * (N(0) - 1) * (N(0) - 2));
     case TSG_STAT_DIVERGENCE:
-        return 1.0 / (N(0) * (N(1) - (c[0] == c[1])));
+        return 1.0 / (N(0) * (N(1) - (same01 ? 1 : 0)));
     case TSG_STAT_RELATEDNESS:
         return 1.0;
     case TSG_STAT_RELATEDNESS_NONCENTRED:
@@ -259,13 +265,13 @@ double summary_inv(int stat, const int32_t *n, const int32_t *c) {
 }
 
 int device_upload(tsg_treeseq *ts);
+int ensure_full(tsg_treeseq *ts);
 
 // Runs a batch of queries (same tree sequence, same mode) in one sweep.
 int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     int ret = device_upload(ts);
     if (ret != 0) return ret;
     Workspace *ws = ws_of(ts);
-    const int C = ts->num_chunks;
     const int Q = (int) batch.size();
     const uint32_t mode = batch[0]->mode;
     // fused columns / outputs
@@ -275,16 +281,34 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     std::vector<double> wins;
     bool noskip = false, any_unpolarised = false;
     int K = 0, M = 0;
+    // Identical sample sets (same samples, any order) share one state column
+    // across the batch.  A query with a centred relatedness block keeps its
+    // own contiguous columns: its summary averages over them.
+    std::map<std::vector<int32_t>, int32_t> shared_col;
     for (int qi = 0; qi < Q; qi++) {
         Query &q = *batch[qi];
         const int cbase = K;
-        for (const auto &s : q.sets) {
+        bool own = false;
+        for (const Block &b : q.blocks) own |= (b.stat == TSG_STAT_RELATEDNESS);
+        std::vector<int32_t> gcol(q.sets.size());
+        for (size_t t = 0; t < q.sets.size(); t++) {
+            const auto &s = q.sets[t];
+            if (!own) {
+                std::vector<int32_t> key(s);
+                std::sort(key.begin(), key.end());
+                auto it = shared_col.find(key);
+                if (it != shared_col.end()) {
+                    gcol[t] = it->second;
+                    continue;
+                }
+                shared_col.emplace(std::move(key), K);
+            }
             for (int32_t si : s) {
                 pair_s.push_back(si);
                 pair_c.push_back(K);
             }
             col_total.push_back((int32_t) s.size());
-            K++;
+            gcol[t] = K++;
         }
         TsgQuery &tq = qs[qi];
         tq.W = (int32_t) q.windows.size() - 1;
@@ -306,7 +330,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
                     for (int t = 0; t < b.tuple; t++) cols[t] = b.idx[(size_t) m * b.tuple + t];
                 }
                 int32_t gc[4];
-                for (int t = 0; t < 4; t++) gc[t] = cbase + cols[t];
+                for (int t = 0; t < 4; t++) gc[t] = gcol[cols[t]];
                 o.i = gc[0];
                 o.j = gc[1];
                 o.k = gc[2];
@@ -318,7 +342,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
                 o.cbase = cbase;
                 o.ccount = q.ncentre;
                 o.flags = b.polarised ? 1 : 0;
-                o.inv = summary_inv(b.stat, col_total.data(), gc);
+                o.inv = summary_inv(b.stat, col_total.data(), gc, cols[0] == cols[1]);
                 if (!std::isfinite(o.inv)) noskip = true;
                 if (!b.polarised) any_unpolarised = true;
                 outs.push_back(o);
@@ -327,6 +351,17 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         }
         tq.M = M - tq.mbase;
     }
+    // Plan and node store: the hot plan with its slots in LDS when the
+    // largest chunk's slots fit; otherwise the full plan, in LDS when its
+    // largest chunk fits, else in per-chunk HBM slabs.
+    const int R = 2 + K;
+    // the sweep's LDS staging block for the widest launch (512 outputs)
+    const size_t stage = (size_t) TSG_STAGE_INTS(K, std::min(M, 512)) * 4;
+    const size_t hot_need = (size_t) ts->hot.max_slots * R * 4 + stage;
+    const bool use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
+    if (!use_hot && (ret = ensure_full(ts))) return ret;
+    const int C = use_hot ? ts->hot.num_chunks : ts->num_chunks;
+    const int32_t *ct0 = use_hot ? ts->hot.chunk_tree0.data() : ts->chunk_tree0.data();
     // per (chunk, query) window ranges
     std::vector<int32_t> cq_wfirst((size_t) C * Q), cq_nwin((size_t) C * Q);
     std::vector<int64_t> cq_poff((size_t) C * Q);
@@ -337,8 +372,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         qs[qi].pbase = pcount;
         int64_t slots = 0;
         for (int c = 0; c < C; c++) {
-            const double cs = ts->breakpoints[ts->chunk_tree0[c]];
-            const double ce = ts->breakpoints[ts->chunk_tree0[c + 1]];
+            const double cs = ts->breakpoints[ct0[c]];
+            const double ce = ts->breakpoints[ct0[c + 1]];
             const int wf = (int) (std::upper_bound(w, w + W + 1, cs) - w) - 1;
             const int wl = (int) (std::lower_bound(w, w + W + 1, ce) - w) - 1;
             cq_wfirst[(size_t) c * Q + qi] = wf;
@@ -365,7 +400,10 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         (ret = ensure(ws->raw, (size_t) rawcount * 8))) {
         return ret;
     }
-    const int nloc0 = (int) (ts->chunk_node0[1] - ts->chunk_node0[0]);
+    // the first tree, samples first (its running branch sum is evaluated
+    // separately, over all of its nodes)
+    const int nloc0 = use_hot ? (int) ts->hot.t0_parent.size()
+                              : (int) (ts->chunk_node0[1] - ts->chunk_node0[0]);
     const int nb0 = tsg_tree0_blocks(nloc0);
     if (mode == TSG_STAT_BRANCH) {
         if ((ret = ensure(ws->dR, (size_t) C * M * 8)) ||
@@ -375,34 +413,60 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
             return ret;
         }
     }
-    const int R = 2 + K;
-    // staging: xc[K], counts before/after for a path segment, its branch lengths
-    const size_t stage = (size_t) (K + 2 * TSG_PSEG * K + TSG_PSEG) * 4 + 64;
-    const size_t lds_need = (size_t) ts->max_nloc * R * 4 + stage;
-    const bool lds = lds_need <= (size_t) ws->lds_limit;
-    if (!lds) {
-        const size_t slab = (size_t) C * ts->max_nloc * R * 4;
-        if ((ret = ensure(ws->slab, slab))) return ret;
-    }
+    int store = TSG_STORE_HOT;
+    size_t shmem = hot_need;
     const TsgDevice &d = ts->dev;
-    a.ev_path0 = d.ev_path0;
-    a.path = d.path;
-    a.chunk_tree0 = d.chunk_tree0;
-    a.chunk_node0 = d.chunk_node0;
-    a.loc_parent0 = d.loc_parent0;
-    a.loc_bl0 = d.loc_bl0;
-    a.loc_time = d.loc_time;
-    a.loc_lastrem = d.loc_lastrem;
-    a.tr_off = d.tr_off;
-    a.tr_nrem = d.tr_nrem;
-    a.ev_child = d.ev_child;
-    a.ev_parent = d.ev_parent;
-    a.ev_bl = d.ev_bl;
+    const TsgHotDevice &h = ts->hot.dev;
+    const int32_t *t0_parent = d.loc_parent0;
+    const float *t0_bl = d.loc_bl0;
+    if (use_hot) {
+        a.chunk_tree0 = h.chunk_tree0;
+        a.chunk_node0 = h.chunk_slot0;
+        a.loc_bl0 = h.slot_bl0;
+        a.tr_off = h.tr_off;
+        a.tr_nrem = h.tr_nrem;
+        a.ev_child = h.ev_child;
+        a.ev_bl = h.ev_bl;
+        a.ev_path0 = h.ev_path0;
+        a.path = h.path;
+        a.mut_node = h.mut_slot;
+        a.slot_par = h.slot_par;
+        a.chunk_lev0 = h.chunk_lev0;
+        a.lev_slot0 = h.lev_slot0;
+        a.sample_slot = h.sample_slot;
+        t0_parent = h.t0_parent;
+        t0_bl = h.t0_bl;
+    } else {
+        const size_t lds_need = (size_t) ts->max_nloc * R * 4 + stage;
+        if (lds_need <= (size_t) ws->lds_limit) {
+            store = TSG_STORE_LDS;
+            shmem = lds_need;
+        } else {
+            store = TSG_STORE_GLOBAL;
+            shmem = stage;
+            const size_t slab = (size_t) C * ts->max_nloc * R * 4;
+            if ((ret = ensure(ws->slab, slab))) return ret;
+        }
+        a.ev_path0 = d.ev_path0;
+        a.path = d.path;
+        a.chunk_tree0 = d.chunk_tree0;
+        a.chunk_node0 = d.chunk_node0;
+        a.loc_parent0 = d.loc_parent0;
+        a.loc_bl0 = d.loc_bl0;
+        a.loc_time = d.loc_time;
+        a.loc_lastrem = d.loc_lastrem;
+        a.tr_off = d.tr_off;
+        a.tr_nrem = d.tr_nrem;
+        a.ev_child = d.ev_child;
+        a.ev_parent = d.ev_parent;
+        a.ev_bl = d.ev_bl;
+        a.mut_node = d.mut_node;
+    }
+    ws->last_store = store;
     a.bp = d.breakpoints;
     a.tree_site0 = d.tree_site0;
     a.site_pos = d.site_pos;
     a.site_mut0 = d.site_mut0;
-    a.mut_node = d.mut_node;
     a.mut_allele = d.mut_allele;
     a.mut_pallele = d.mut_pallele;
     a.site_nalleles = d.site_nalleles;
@@ -425,25 +489,24 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     a.g_rec = (int32_t *) ws->slab.p;
     a.g_stride = (int64_t) ts->max_nloc * R;
     const int mode_id = (mode == TSG_STAT_BRANCH) ? 0 : 1;
-    const size_t shmem = lds ? lds_need : stage;
     if (ws->profile) (void) hipEventRecord(ws->ev0, s);
     for (int m0 = 0; m0 < M; m0 += 512) {
         a.m_begin = m0;
         a.m_end = std::min(M, m0 + 512);
         const int span = a.m_end - a.m_begin;
         const int mpl = span <= 64 ? 1 : span <= 128 ? 2 : span <= 256 ? 4 : 8;
-        if ((ret = hip_check(tsg_launch_sweep(a, mode_id, lds, mpl, shmem, s)))) return ret;
+        if ((ret = hip_check(tsg_launch_sweep(a, mode_id, store, mpl, shmem, s)))) return ret;
     }
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);
     if (mode == TSG_STAT_BRANCH) {
-        if ((ret = hip_check(tsg_launch_tree0(a, (int32_t *) ws->x0.p, nloc0,
+        if ((ret = hip_check(tsg_launch_tree0(a, t0_parent, t0_bl, (int32_t *) ws->x0.p, nloc0,
                                               (double *) ws->r0part.p, s)))) {
             return ret;
         }
     }
     TsgCombineArgs ca{};
     ca.bp = d.breakpoints;
-    ca.chunk_tree0 = d.chunk_tree0;
+    ca.chunk_tree0 = a.chunk_tree0;
     ca.num_chunks = C;
     ca.Q = Q;
     ca.M = M;
@@ -549,11 +612,59 @@ int device_upload(tsg_treeseq *ts) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
-        ws->lds_limit = (int) std::min<size_t>(prop.sharedMemPerBlock, 65536);
+        // gfx950 has 160 KiB of LDS per CU, all of it addressable by one
+        // workgroup
+        ws->lds_limit = (int) std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
     }
     TsgDevice &d = ts->dev;
     size_t &by = d.bytes;
     std::vector<int32_t> site_nalleles = ts->site_nalleles;
+    if ((ret = upload_ptr(&d.breakpoints, ts->breakpoints, by)) ||
+        (ret = upload_ptr(&d.tree_site0, ts->tree_site0, by)) ||
+        (ret = upload_ptr(&d.site_pos, ts->site_pos, by)) ||
+        (ret = upload_ptr(&d.site_mut0, ts->site_mut0, by)) ||
+        (ret = upload_ptr(&d.mut_allele, ts->mut_allele, by)) ||
+        (ret = upload_ptr(&d.mut_pallele, ts->mut_pallele, by)) ||
+        (ret = upload_ptr(&d.site_nalleles, site_nalleles, by))) {
+        return ret;
+    }
+    if (ts->hot.usable) {
+        const tsg_hotplan &hp = ts->hot;
+        TsgHotDevice &h = ts->hot.dev;
+        size_t &hb = h.bytes;
+        if ((ret = upload_ptr(&h.chunk_tree0, hp.chunk_tree0, hb)) ||
+            (ret = upload_ptr(&h.chunk_slot0, hp.chunk_slot0, hb)) ||
+            (ret = upload_ptr(&h.slot_par, hp.slot_par, hb)) ||
+            (ret = upload_ptr(&h.slot_bl0, hp.slot_bl0, hb)) ||
+            (ret = upload_ptr(&h.chunk_lev0, hp.chunk_lev0, hb)) ||
+            (ret = upload_ptr(&h.lev_slot0, hp.lev_slot0, hb)) ||
+            (ret = upload_ptr(&h.sample_slot, hp.sample_slot, hb)) ||
+            (ret = upload_ptr(&h.tr_off, hp.tr_off, hb)) ||
+            (ret = upload_ptr(&h.tr_nrem, hp.tr_nrem, hb)) ||
+            (ret = upload_ptr(&h.ev_child, hp.ev_child, hb)) ||
+            (ret = upload_ptr(&h.ev_bl, hp.ev_bl, hb)) ||
+            (ret = upload_ptr(&h.ev_path0, hp.ev_path0, hb)) ||
+            (ret = upload_ptr(&h.path, hp.path, hb)) ||
+            (ret = upload_ptr(&h.mut_slot, hp.mut_slot, hb)) ||
+            (ret = upload_ptr(&h.t0_parent, hp.t0_parent, hb)) ||
+            (ret = upload_ptr(&h.t0_bl, hp.t0_bl, hb))) {
+            return ret;
+        }
+    }
+    d.device = dev;
+    return 0;
+}
+
+// The full plan (every node of each chunk's start tree) is built and
+// uploaded on first need: the allele frequency spectrum, the genotype
+// route, and sweeps whose hot slots do not fit in LDS.
+int ensure_full(tsg_treeseq *ts) {
+    int ret = device_upload(ts);
+    if (ret != 0) return ret;
+    if (!ts->full_built && (ret = tsg_build_plan(ts, ts->full_chunks))) return ret;
+    TsgDevice &d = ts->dev;
+    if (d.chunk_tree0 != nullptr) return 0;
+    size_t &by = d.bytes;
     if ((ret = upload_ptr(&d.chunk_tree0, ts->chunk_tree0, by)) ||
         (ret = upload_ptr(&d.chunk_node0, ts->chunk_node0, by)) ||
         (ret = upload_ptr(&d.loc_parent0, ts->loc_parent0, by)) ||
@@ -567,17 +678,9 @@ int device_upload(tsg_treeseq *ts) {
         (ret = upload_ptr(&d.ev_bl, ts->ev_bl, by)) ||
         (ret = upload_ptr(&d.ev_path0, ts->ev_path0, by)) ||
         (ret = upload_ptr(&d.path, ts->path, by)) ||
-        (ret = upload_ptr(&d.breakpoints, ts->breakpoints, by)) ||
-        (ret = upload_ptr(&d.tree_site0, ts->tree_site0, by)) ||
-        (ret = upload_ptr(&d.site_pos, ts->site_pos, by)) ||
-        (ret = upload_ptr(&d.site_mut0, ts->site_mut0, by)) ||
-        (ret = upload_ptr(&d.mut_node, ts->mut_node_loc, by)) ||
-        (ret = upload_ptr(&d.mut_allele, ts->mut_allele, by)) ||
-        (ret = upload_ptr(&d.mut_pallele, ts->mut_pallele, by)) ||
-        (ret = upload_ptr(&d.site_nalleles, site_nalleles, by))) {
+        (ret = upload_ptr(&d.mut_node, ts->mut_node_loc, by))) {
         return ret;
     }
-    d.device = dev;
     return 0;
 }
 
@@ -615,6 +718,16 @@ void tsg_device_free(TsgDevice *d) {
     *d = TsgDevice();
 }
 
+void tsg_hot_device_free(TsgHotDevice *h) {
+    void *ptrs[] = {h->chunk_tree0, h->chunk_slot0, h->slot_par, h->slot_bl0, h->chunk_lev0,
+                    h->lev_slot0, h->sample_slot, h->tr_off, h->tr_nrem, h->ev_child, h->ev_bl,
+                    h->ev_path0, h->path, h->mut_slot, h->t0_parent, h->t0_bl};
+    for (void *p : ptrs) {
+        if (p != nullptr) (void) hipFree(p);
+    }
+    *h = TsgHotDevice();
+}
+
 namespace {
 int default_num_chunks(const tsg_treeseq *ts) {
     // Enough chunks to fill the device (about 4 waves per CU), each with a
@@ -628,6 +741,29 @@ int default_num_chunks(const tsg_treeseq *ts) {
     c = std::min<int64_t>(c, std::max<int64_t>(1, budget / (per_chunk * 24)));
     c = std::min<int64_t>(c, std::max(1, ts->num_trees));
     return (int) c;
+}
+
+int default_hot_chunks(const tsg_treeseq *ts) {
+    // About 1024 replayed edge events per chunk (so about as many hot slots:
+    // ~40 KiB of LDS at 8 sample sets, three or four chunks per CU), but at
+    // least 1024 chunks of >= 64 events to fill 256 CUs; each chunk stores
+    // one 16-bit slot per sample, capped near 2 GiB in total.
+    const int64_t events = 2 * (int64_t) ts->num_edges;
+    int64_t c = std::max<int64_t>(events / 1024, std::min<int64_t>(1024, events / 64));
+    c = std::max<int64_t>(c, 1);
+    const int64_t per_chunk = 2 * (int64_t) std::max(1, ts->num_samples);
+    c = std::min<int64_t>(c, std::max<int64_t>(1, ((int64_t) 2 << 30) / per_chunk));
+    c = std::min<int64_t>(c, std::max(1, ts->num_trees));
+    return (int) c;
+}
+
+// The full plan's host arrays, built on first use.
+const tsg_treeseq *full_plan(const tsg_treeseq *ts) {
+    if (!ts->full_built) {
+        tsg_treeseq *m = const_cast<tsg_treeseq *>(ts);
+        (void) tsg_build_plan(m, m->full_chunks);
+    }
+    return ts;
 }
 }  // namespace
 
@@ -720,7 +856,8 @@ int tsg_treeseq_init(tsg_treeseq **out, double sequence_length, uint64_t num_nod
     int ret = tsg_init_trees(ts.get(), ancestral_state, ancestral_state_offset, derived_state,
                              derived_state_offset);
     if (ret != 0) return ret;
-    ret = tsg_build_plan(ts.get(), default_num_chunks(ts.get()));
+    ts->full_chunks = default_num_chunks(ts.get());
+    ret = tsg_build_hot_plan(ts.get(), default_hot_chunks(ts.get()));
     if (ret != 0) return ret;
     *out = ts.release();
     return 0;
@@ -730,17 +867,39 @@ int tsg_treeseq_free(tsg_treeseq *ts) {
     if (ts == nullptr) return 0;
     free_workspace(ts);
     tsg_device_free(&ts->dev);
+    tsg_hot_device_free(&ts->hot.dev);
     delete ts;
     return 0;
 }
 
-// Rebuilds the chunk plan with a given chunk count (autotuner hook).
+// Rebuilds the chunk plans with a given chunk count (autotuner hook); the
+// full plan is rebuilt on its next use.
 int tsg_treeseq_set_num_chunks(tsg_treeseq *ts, uint32_t num_chunks) {
     if (ts == nullptr || num_chunks == 0) return TSG_ERR_BAD_PARAM_VALUE;
     std::lock_guard<std::mutex> lock(g_mutex);
     free_workspace(ts);
     tsg_device_free(&ts->dev);
-    return tsg_build_plan(ts, (int32_t) num_chunks);
+    tsg_hot_device_free(&ts->hot.dev);
+    ts->full_chunks = (int32_t) std::min<uint32_t>(num_chunks, INT32_MAX);
+    ts->full_built = false;
+    return tsg_build_hot_plan(ts, ts->full_chunks);
+}
+
+// Testing hook: plan = 1 runs every sweep on the full plan, 0 lets the
+// library choose (the hot plan whenever its slots fit in LDS).
+int tsg_treeseq_set_sweep_plan(tsg_treeseq *ts, int plan) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    int ret = device_upload(ts);
+    if (ret != 0) return ret;
+    ws_of(ts)->force_full = plan == 1;
+    return 0;
+}
+
+// Node store of the last statistics sweep: 0 HBM slabs (full plan), 1 LDS
+// (full plan), 2 LDS hot slots; -1 before the first sweep.
+int tsg_treeseq_get_last_store(const tsg_treeseq *ts) {
+    const Workspace *ws = ws_of(const_cast<tsg_treeseq *>(ts));
+    return ws == nullptr ? -1 : ws->last_store;
 }
 
 int tsg_treeseq_upload(tsg_treeseq *ts) {
@@ -754,10 +913,16 @@ uint64_t tsg_treeseq_get_num_nodes(const tsg_treeseq *ts) { return (uint64_t) ts
 uint64_t tsg_treeseq_get_num_edges(const tsg_treeseq *ts) { return (uint64_t) ts->num_edges; }
 uint64_t tsg_treeseq_get_num_sites(const tsg_treeseq *ts) { return (uint64_t) ts->num_sites; }
 uint64_t tsg_treeseq_get_num_mutations(const tsg_treeseq *ts) { return (uint64_t) ts->num_mutations; }
-uint64_t tsg_treeseq_get_num_chunks(const tsg_treeseq *ts) { return (uint64_t) ts->num_chunks; }
+uint64_t tsg_treeseq_get_num_chunks(const tsg_treeseq *ts) {
+    return (uint64_t) full_plan(ts)->num_chunks;
+}
 double tsg_treeseq_get_sequence_length(const tsg_treeseq *ts) { return ts->L; }
-uint64_t tsg_treeseq_get_device_bytes(const tsg_treeseq *ts) { return (uint64_t) ts->dev.bytes; }
-uint64_t tsg_treeseq_get_max_chunk_nodes(const tsg_treeseq *ts) { return (uint64_t) ts->max_nloc; }
+uint64_t tsg_treeseq_get_device_bytes(const tsg_treeseq *ts) {
+    return (uint64_t) (ts->dev.bytes + ts->hot.dev.bytes);
+}
+uint64_t tsg_treeseq_get_max_chunk_nodes(const tsg_treeseq *ts) {
+    return (uint64_t) full_plan(ts)->max_nloc;
+}
 
 int tsg_treeseq_get_breakpoints(const tsg_treeseq *ts, double *out) {
     std::copy(ts->breakpoints.begin(), ts->breakpoints.end(), out);
@@ -777,6 +942,7 @@ int tsg_treeseq_get_index(const tsg_treeseq *ts, int32_t *I, int32_t *O) {
 // machines without a GPU.  sizes = {chunks, local nodes, events, trees,
 // sites, mutations}.
 int tsg_treeseq_get_plan_sizes(const tsg_treeseq *ts, uint64_t *sizes) {
+    ts = full_plan(ts);
     sizes[0] = (uint64_t) ts->num_chunks;
     sizes[1] = ts->loc_parent0.size();
     sizes[2] = ts->ev_child.size();
@@ -792,6 +958,7 @@ int tsg_treeseq_copy_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *
                           int32_t *ev_child, int32_t *ev_parent, float *ev_bl,
                           int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc,
                           int32_t *mut_allele, int32_t *mut_pallele, int32_t *site_nalleles) {
+    ts = full_plan(ts);
     auto cp = [](auto *dst, const auto &v) { std::copy(v.begin(), v.end(), dst); };
     cp(chunk_tree0, ts->chunk_tree0);
     cp(chunk_node0, ts->chunk_node0);
@@ -814,11 +981,57 @@ int tsg_treeseq_copy_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *
 }
 
 // Root paths of the events (host copies): ev_path0 has events + 1 entries.
-uint64_t tsg_treeseq_get_path_size(const tsg_treeseq *ts) { return ts->path.size(); }
+uint64_t tsg_treeseq_get_path_size(const tsg_treeseq *ts) { return full_plan(ts)->path.size(); }
 
 int tsg_treeseq_copy_paths(const tsg_treeseq *ts, int64_t *ev_path0, int32_t *path) {
+    ts = full_plan(ts);
     std::copy(ts->ev_path0.begin(), ts->ev_path0.end(), ev_path0);
     std::copy(ts->path.begin(), ts->path.end(), path);
+    return 0;
+}
+
+// Introspection of the hot plan (host copies).  sizes = {usable, chunks,
+// slots, level entries, events, path entries, first-tree nodes, max slots
+// per chunk, max levels per chunk}.
+int tsg_treeseq_get_hot_plan_sizes(const tsg_treeseq *ts, uint64_t *sizes) {
+    const tsg_hotplan &hp = ts->hot;
+    sizes[0] = hp.usable ? 1 : 0;
+    sizes[1] = (uint64_t) hp.num_chunks;
+    sizes[2] = hp.slot_par.size();
+    sizes[3] = hp.lev_slot0.size();
+    sizes[4] = hp.ev_child.size();
+    sizes[5] = hp.path.size();
+    sizes[6] = hp.t0_parent.size();
+    sizes[7] = (uint64_t) hp.max_slots;
+    sizes[8] = (uint64_t) hp.max_levels;
+    return 0;
+}
+
+int tsg_treeseq_copy_hot_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *chunk_slot0,
+                              int32_t *slot_par, float *slot_bl0, int64_t *chunk_lev0,
+                              int32_t *lev_slot0, uint16_t *sample_slot, int64_t *tr_off,
+                              int32_t *tr_nrem, int32_t *ev_child, float *ev_bl,
+                              int64_t *ev_path0, int32_t *path, int32_t *mut_slot,
+                              int32_t *t0_parent, float *t0_bl) {
+    const tsg_hotplan &hp = ts->hot;
+    if (!hp.usable) return TSG_ERR_UNSUPPORTED_INPUT;
+    auto cp = [](auto *dst, const auto &v) { std::copy(v.begin(), v.end(), dst); };
+    cp(chunk_tree0, hp.chunk_tree0);
+    cp(chunk_slot0, hp.chunk_slot0);
+    cp(slot_par, hp.slot_par);
+    cp(slot_bl0, hp.slot_bl0);
+    cp(chunk_lev0, hp.chunk_lev0);
+    cp(lev_slot0, hp.lev_slot0);
+    cp(sample_slot, hp.sample_slot);
+    cp(tr_off, hp.tr_off);
+    cp(tr_nrem, hp.tr_nrem);
+    cp(ev_child, hp.ev_child);
+    cp(ev_bl, hp.ev_bl);
+    cp(ev_path0, hp.ev_path0);
+    cp(path, hp.path);
+    cp(mut_slot, hp.mut_slot);
+    cp(t0_parent, hp.t0_parent);
+    cp(t0_bl, hp.t0_bl);
     return 0;
 }
 
@@ -1081,7 +1294,7 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_
     }
 
     std::lock_guard<std::mutex> lock(g_mutex);
-    int ret = device_upload(ts);
+    int ret = ensure_full(ts);
     if (ret != 0) return ret;
     Workspace *ws = ws_of(ts);
     hipStream_t s = ws->stream;
@@ -1283,7 +1496,7 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
     const int64_t afs_size = (int64_t) afs_size_d;
 
     std::lock_guard<std::mutex> lock(g_mutex);
-    if ((ret = device_upload(ts))) return ret;
+    if ((ret = ensure_full(ts))) return ret;
     Workspace *ws = ws_of(ts);
     hipStream_t s = ws->stream;
     const int C = ts->num_chunks;

@@ -9,6 +9,8 @@
 
 #define TSG_NULL (-1)
 #define TSG_NODE_IS_SAMPLE 1u
+// sample without a hot ancestor in a hot-plan chunk (16-bit slot ids)
+#define TSG_NO_SLOT 0xFFFFu
 
 // --- statistic option bits (trees.h:47-56) ---------------------------------
 #define TSG_STAT_SITE (1u << 0)

@@ -31,6 +31,19 @@ struct TsgQuery {
 // Hops of an event's root path staged in LDS at once by the sweep kernel.
 #define TSG_PSEG 16
 
+// 32-bit words of one TsgOut.
+#define TSG_OUT_INTS ((int) (sizeof(TsgOut) / 4))
+// 32-bit words of the sweep kernel's LDS staging block for K state columns
+// and `span` outputs in the launch (layout in k_sweep); node records follow.
+#define TSG_STAGE_INTS(K, span) \
+    ((int64_t) (span) * (TSG_OUT_INTS + 1) + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) + \
+     TSG_PSEG + 2 * 64 + 2)
+
+// Where a sweep keeps its node records (see k_sweep).
+#define TSG_STORE_GLOBAL 0
+#define TSG_STORE_LDS 1
+#define TSG_STORE_HOT 2
+
 #define TSG_F_POLARISED 1u
 #define TSG_F_NOSKIP 2u       // a summary has a zero denominator: never skip walks
 #define TSG_F_SPAN_NORMALISE 4u
@@ -61,6 +74,12 @@ struct TsgSweepArgs {
     const int32_t *seed_sample, *seed_off, *seed_col;
     const int64_t *ev_path0;     // per event: offset of its root path
     const int32_t *path;
+    // hot plan only (node ids are slots; chunk_node0 holds slot ranges and
+    // loc_bl0 the slots' start-tree branch lengths)
+    const int32_t *slot_par;     // nearest hot proper ancestor, or -1
+    const int64_t *chunk_lev0;   // [C+1] into lev_slot0
+    const int32_t *lev_slot0;    // per chunk: level boundaries, deepest first
+    const uint16_t *sample_slot; // [C * num_samples]
     const int32_t *col_total;    // [K] total weight (set size) per column
     const TsgOut *outs;          // [M]
     const TsgQuery *queries;     // [Q]

@@ -2,14 +2,16 @@
 //
 // Work decomposition: the genome is cut into chunks of consecutive trees and
 // each chunk is one 64-lane wavefront (one workgroup).  A chunk executes its
-// sweep program (see tsg_treeseq.cpp): it materialises its start tree, seeds
-// per-sample-set counts up the start tree, then applies the edge removals and
-// insertions of its transitions in the reference order, walking each changed
-// edge's parent path.  Lanes run over state columns (sample sets) for count
-// updates and over outputs for summary functions and window sums.
+// sweep program (see tsg_treeseq.cpp): it seeds per-sample-set counts in its
+// start tree, then applies the edge removals and insertions of its
+// transitions in the reference order, updating each changed edge's recorded
+// root path.  Lanes run over state columns (sample sets) for count updates
+// and over (output, hop) pairs or outputs for summary functions and window
+// sums.
 //
-// Node records {parent, branch length, x[0..K)} live in LDS when the chunk's
-// local node count fits, otherwise in a per-chunk HBM slab.
+// Node records {parent, branch length, x[0..K)}: with the hot plan only the
+// nodes a chunk touches, as LDS slots; with the full plan every node of the
+// chunk's start tree, in LDS when they fit, otherwise in a per-chunk HBM slab.
 //
 // Reference algorithm restated here:
 //   branch sweep      c/tskit/trees.c:1352-1523 (update_state 1317-1327,
@@ -26,8 +28,10 @@
 //     evaluated exactly in int64, then scaled once in FP64 and rounded to FP32;
 //   * running sums and window sums are FP32 TwoSum (value + error) pairs;
 //   * a walk whose moved subtree carries zero weight changes no count and no
-//     summary, so it is skipped (unless a summary has a zero denominator, in
-//     which case the reference's NaN propagation is kept by not skipping).
+//     summary, so it is skipped; an output none of whose columns moved is not
+//     re-evaluated along the walk (its terms cancel exactly).  A summary with
+//     a zero denominator is NaN at every state, so the edge's own term (always
+//     added) carries the reference's NaN into the sums.
 #include <hip/hip_runtime.h>
 
 #include "tsg_common.h"
@@ -113,6 +117,7 @@ struct Nodes {
     __device__ int32_t parent(int u) const { return rec[(int64_t) u * R]; }
     __device__ float bl(int u) const { return __int_as_float(rec[(int64_t) u * R + 1]); }
     __device__ int32_t &x(int u, int k) const { return rec[(int64_t) u * R + 2 + k]; }
+    __device__ void set_bl(int u, float b) const { rec[(int64_t) u * R + 1] = __float_as_int(b); }
     __device__ void set_edge(int u, int p, float b) const {
         rec[(int64_t) u * R] = p;
         rec[(int64_t) u * R + 1] = __float_as_int(b);
@@ -138,33 +143,108 @@ __device__ void init_chunk(const TsgSweepArgs &a, const Nodes &nd, int64_t n0, i
     __syncthreads();
 }
 
+// Hot plan: zero the slots' counts, load their start-tree branch lengths,
+// add every weighted sample into its nearest hot ancestor-or-self, then add
+// each depth level of the hot forest into its parents, deepest level first
+// (the counts of a level are complete before it is added upward).
+__device__ void init_hot(const TsgSweepArgs &a, const Nodes &nd, int c, int nslot, int lane) {
+    const int64_t s0 = a.chunk_node0[c];
+    const int64_t words = (int64_t) nslot * nd.R;
+    for (int64_t i = lane; i < words; i += WAVE) nd.rec[i] = 0;
+    __syncthreads();
+    for (int h = lane; h < nslot; h += WAVE) nd.set_edge(h, TSG_NULL, a.loc_bl0[s0 + h]);
+    const uint16_t *ss = a.sample_slot + (int64_t) c * a.num_samples;
+    for (int i = lane; i < a.nseed; i += WAVE) {
+        const unsigned h = ss[a.seed_sample[i]];
+        if (h == TSG_NO_SLOT) continue;
+        const int c0 = a.seed_off[i], c1 = a.seed_off[i + 1];
+        for (int cc = c0; cc < c1; cc++) atomicAdd(&nd.x((int) h, a.seed_col[cc]), 1);
+    }
+    __syncthreads();
+    const int32_t *lev = a.lev_slot0 + a.chunk_lev0[c];
+    const int nlev = (int) (a.chunk_lev0[c + 1] - a.chunk_lev0[c]) - 1;
+    const int K = nd.R - 2;
+    // the last level holds the roots of the hot forest: nothing above them
+    for (int l = 0; l + 1 < nlev; l++) {
+        const int h0 = lev[l], h1 = lev[l + 1];
+        for (int idx = lane; idx < (h1 - h0) * K; idx += WAVE) {
+            const int h = h0 + idx / K, k = idx % K;
+            const int p = a.slot_par[s0 + h];
+            const int32_t v = nd.x(h, k);
+            if (p >= 0 && v != 0) atomicAdd(&nd.x(p, k), v);
+        }
+        __syncthreads();
+    }
+}
+
 struct LaneOut {
-    TsgOut o;
-    int m;
+    int lm;         // output index within this launch (-1: none)
     int wi, wend;
     const double *win;
     int64_t slot0;  // partial index of window wfirst
     int Mq;
+    uint64_t mask;  // state columns the summary reads
+    float f0;       // its value on an all-zero state (unpolarised wrapper included)
     float Rs, Rc, As, Ac;
 };
 
+// State columns a summary function reads; all ones when it cannot be told
+// (more than 64 columns) or when the summary has a zero denominator (its NaN
+// must reach every sum, as in the reference).
+__device__ uint64_t summary_columns(const TsgOut &o, int K) {
+    if (K > 64 || !isfinite(o.inv)) return ~0ull;
+    uint64_t m = 1ull << o.i;
+    switch (o.stat) {
+    case TSG_STAT_DIVERSITY:
+    case TSG_STAT_SEGREGATING_SITES:
+    case TSG_STAT_Y1:
+        break;
+    case TSG_STAT_DIVERGENCE:
+    case TSG_STAT_Y2:
+    case TSG_STAT_F2:
+    case TSG_STAT_RELATEDNESS_NONCENTRED:
+        m |= 1ull << o.j;
+        break;
+    case TSG_STAT_Y3:
+    case TSG_STAT_F3:
+        m |= (1ull << o.j) | (1ull << o.k);
+        break;
+    case TSG_STAT_F4:
+        m |= (1ull << o.j) | (1ull << o.k) | (1ull << o.l);
+        break;
+    case TSG_STAT_RELATEDNESS:
+        m |= 1ull << o.j;
+        for (int c = o.cbase; c < o.cbase + o.ccount; c++) m |= 1ull << c;
+        break;
+    default:
+        return ~0ull;
+    }
+    return m;
+}
+
 template <int MPL>
-__device__ void setup_outputs(const TsgSweepArgs &a, int c, int lane, LaneOut (&lo)[MPL]) {
+__device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, int lane,
+                              LaneOut (&lo)[MPL]) {
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
         LaneOut &L = lo[j];
-        L.m = a.m_begin + lane + WAVE * j;
+        const int m = a.m_begin + lane + WAVE * j;
+        L.lm = -1;
         L.Rs = L.Rc = L.As = L.Ac = 0.0f;
         L.wi = L.wend = 0;
-        if (L.m < a.m_end) {
-            L.o = a.outs[L.m];
-            const TsgQuery &q = a.queries[L.o.q];
-            const int64_t cq = (int64_t) c * a.Q + L.o.q;
+        L.mask = 0;
+        L.f0 = 0.0f;
+        if (m < a.m_end) {
+            L.lm = m - a.m_begin;
+            const TsgOut &o = sout[L.lm];
+            const TsgQuery &q = a.queries[o.q];
+            const int64_t cq = (int64_t) c * a.Q + o.q;
             L.win = a.windows + q.win_off;
             L.wi = a.cq_wfirst[cq];
             L.wend = L.wi + a.cq_nwin[cq];
             L.Mq = q.M;
-            L.slot0 = q.pbase + a.cq_poff[cq] * (int64_t) q.M + (L.m - q.mbase);
+            L.slot0 = q.pbase + a.cq_poff[cq] * (int64_t) q.M + (m - q.mbase);
+            L.mask = summary_columns(o, a.K);
         }
     }
 }
@@ -185,40 +265,72 @@ __device__ __forceinline__ float branch_summary(const TsgOut &o, const int32_t *
 }  // namespace
 
 // MODE 0: branch statistics, MODE 1: site statistics.
-template <int MODE, bool LDS, int MPL>
+// STORE: TSG_STORE_GLOBAL (full plan, node records in a per-chunk HBM slab),
+// TSG_STORE_LDS (full plan, every local node in LDS) or TSG_STORE_HOT (hot
+// plan, only the chunk's hot slots, in LDS).
+//
+// LDS: the staging block (TSG_STAGE_INTS) first -- this launch's outputs,
+// the moved subtree's counts, a path segment's counts before / after the
+// update and its branch lengths, column totals, an all-zero state, the
+// active-output list and one round of summary terms -- then, for the LDS
+// stores, the node records.
+//
+// Branch mode, per event: the columns the moved subtree carries (a ballot)
+// decide which outputs can change.  An output none of whose columns moved
+// has F(old) == F(new) at every hop, so its hop terms cancel exactly and are
+// not evaluated, and its child term is its all-zero value f0.  The active
+// outputs' (output, hop) terms of a path segment are spread over the lanes
+// (hop fastest, so lanes of one round mostly share an output), written to
+// LDS, and each output's owner lane adds them in hop order -- the order of
+// the reference's walk (trees.c:1436-1448, 1461-1472).
+template <int MODE, int STORE, int MPL>
 __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     extern __shared__ int32_t smem[];
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int K = a.K;
     const int R = 2 + K;
+    const int span = a.m_end - a.m_begin;
     const int64_t n0 = a.chunk_node0[c];
     const int nloc = (int) (a.chunk_node0[c + 1] - n0);
+    TsgOut *sout = reinterpret_cast<TsgOut *>(smem);                  // [span]
+    int32_t *xc = smem + span * TSG_OUT_INTS;                           // [K]
+    int32_t *xo = xc + K;                                               // [PSEG x K]
+    int32_t *xn = xo + TSG_PSEG * K;                                    // [PSEG x K]
+    float *sbl = reinterpret_cast<float *>(xn + TSG_PSEG * K);          // [PSEG]
+    int32_t *tot = reinterpret_cast<int32_t *>(sbl + TSG_PSEG);         // [K]
+    int32_t *zero = tot + K;                                            // [K]
+    int32_t *act = zero + K;                                            // [span]
+    float *td = reinterpret_cast<float *>(act + span);                  // [2 x WAVE]
     Nodes nd;
     nd.R = R;
-    int32_t *stg;
-    if (LDS) {
-        nd.rec = smem;
-        stg = smem + (int64_t) nloc * R;
+    if (STORE != TSG_STORE_GLOBAL) {
+        nd.rec = smem + TSG_STAGE_INTS(K, span);
     } else {
         nd.rec = a.g_rec + (int64_t) c * a.g_stride;
-        stg = smem;
     }
-    int32_t *xc = stg;                              // [K] counts of the moved subtree
-    int32_t *xo = stg + K;                          // [PSEG x K] counts before the update
-    int32_t *xn = xo + TSG_PSEG * K;                // [PSEG x K] counts after the update
-    float *sbl = reinterpret_cast<float *>(xn + TSG_PSEG * K);  // [PSEG] branch lengths
     const bool any_unpolarised = (a.flags & TSG_F_ANY_UNPOLARISED) != 0;
-    const bool noskip = (MODE == 0) && (a.flags & TSG_F_NOSKIP) != 0;
-    const int32_t *tot = a.col_total;
+    const uint64_t below = (1ull << lane) - 1;
 
-    init_chunk(a, nd, n0, nloc, K, lane);
+    for (int m = lane; m < span; m += WAVE) sout[m] = a.outs[a.m_begin + m];
+    for (int k = lane; k < K; k += WAVE) {
+        tot[k] = a.col_total[k];
+        zero[k] = 0;
+    }
+    if (STORE == TSG_STORE_HOT) {
+        init_hot(a, nd, c, nloc, lane);
+    } else {
+        init_chunk(a, nd, n0, nloc, K, lane);
+    }
 
     LaneOut lo[MPL];
-    setup_outputs<MPL>(a, c, lane, lo);
+    setup_outputs<MPL>(a, sout, c, lane, lo);
     int wfirst[MPL];
 #pragma unroll
-    for (int j = 0; j < MPL; j++) wfirst[j] = lo[j].wi;
+    for (int j = 0; j < MPL; j++) {
+        wfirst[j] = lo[j].wi;
+        if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(sout[lo[j].lm], zero, tot);
+    }
 
     auto apply_transition = [&](int t) {
         const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
@@ -226,15 +338,24 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
         for (int64_t e = e0; e < e1; e++) {
             const bool ins = (e - e0) >= nrem;
             const int u = a.ev_child[e];
-            const int v = a.ev_parent[e];
-            bool nz = false;
-            for (int k = lane; k < K; k += WAVE) {
-                const int32_t xk = nd.x(u, k);
-                xc[k] = xk;
-                nz |= (xk != 0);
+            // columns the moved subtree carries
+            uint64_t cm;
+            if (K <= WAVE) {
+                int32_t xk = 0;
+                if (lane < K) {
+                    xk = nd.x(u, lane);
+                    xc[lane] = xk;
+                }
+                cm = __ballot(xk != 0);
+            } else {
+                bool nz = false;
+                for (int k = lane; k < K; k += WAVE) {
+                    const int32_t xk = nd.x(u, k);
+                    xc[k] = xk;
+                    nz |= (xk != 0);
+                }
+                cm = __any(nz) ? ~0ull : 0ull;
             }
-            nz = __any(nz);
-            const bool walk = nz || noskip;
             __syncthreads();
             // the edge's own term bl * F(x_child): needed even for a zero-weight
             // subtree, because unpolarised summaries need not vanish at zero
@@ -244,29 +365,31 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 const float sg = ins ? 1.0f : -1.0f;
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
-                    if (lo[j].m < a.m_end) {
-                        const float f = branch_summary(lo[j].o, xc, tot);
+                    if (lo[j].lm >= 0) {
+                        const float f = (lo[j].mask & cm) ? branch_summary(sout[lo[j].lm], xc, tot)
+                                                          : lo[j].f0;
                         tsum(lo[j].Rs, lo[j].Rc, sg * __fmul_rn(b, f));
                     }
                 }
             }
             __syncthreads();
-            if (lane == 0) {
-                if (ins) {
-                    nd.set_edge(u, v, a.ev_bl[e]);
-                } else {
-                    nd.set_edge(u, TSG_NULL, 0.0f);
-                }
+            // after seeding nothing reads parent pointers (the root paths
+            // are precomputed), so only the branch length is kept current
+            if (lane == 0) nd.set_bl(u, ins ? a.ev_bl[e] : 0.0f);
+            if (cm == 0) continue;  // no count changes, no summary changes
+            // outputs whose summaries can change, compacted into act[]
+            int pos[MPL];
+            int na = 0;
+#pragma unroll
+            for (int j = 0; j < MPL; j++) {
+                const bool on = (MODE == 0) && lo[j].lm >= 0 && (lo[j].mask & cm) != 0;
+                const uint64_t bal = __ballot(on);
+                pos[j] = on ? na + __popcll(bal & below) : -1;
+                if (on) act[pos[j]] = lo[j].lm;
+                na += __popcll(bal);
             }
             __syncthreads();
-            if (!walk) continue;
-            // The event's root path is precomputed (v first).  All (hop,
-            // column) counts of a path segment are updated in parallel, then
-            // every output sums -bl F(old) + bl F(new) over the segment's
-            // hops -- the same terms as the reference's hop-by-hop walk
-            // (trees.c:1436-1448, 1461-1472), since a path visits each node once.
             const int64_t p0 = a.ev_path0[e], p1 = a.ev_path0[e + 1];
-            (void) v;
             for (int64_t sg0 = p0; sg0 < p1; sg0 += TSG_PSEG) {
                 const int h = (int) min((int64_t) TSG_PSEG, p1 - sg0);
                 for (int idx = lane; idx < h * K; idx += WAVE) {
@@ -282,21 +405,34 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 }
                 if (MODE == 0) {
                     for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(a.path[sg0 + i]);
-                    __syncthreads();
-#pragma unroll
-                    for (int j = 0; j < MPL; j++) {
-                        if (lo[j].m < a.m_end) {
-                            for (int i = 0; i < h; i++) {
-                                const float bw = sbl[i];
-                                const float fo = branch_summary(lo[j].o, xo + i * K, tot);
-                                const float fn = branch_summary(lo[j].o, xn + i * K, tot);
-                                tsum(lo[j].Rs, lo[j].Rc, -__fmul_rn(bw, fo));
-                                tsum(lo[j].Rs, lo[j].Rc, __fmul_rn(bw, fn));
-                            }
-                        }
-                    }
                 }
                 __syncthreads();
+                if (MODE == 0 && na > 0) {
+                    const int items = na * h;
+                    for (int r0 = 0; r0 < items; r0 += WAVE) {
+                        const int idx = r0 + lane;
+                        if (idx < items) {
+                            const int ai = idx / h, i = idx - ai * h;
+                            const TsgOut &o = sout[act[ai]];
+                            const float bw = sbl[i];
+                            const float fo = branch_summary(o, xo + i * K, tot);
+                            const float fn = branch_summary(o, xn + i * K, tot);
+                            td[2 * lane] = -__fmul_rn(bw, fo);
+                            td[2 * lane + 1] = __fmul_rn(bw, fn);
+                        }
+                        __syncthreads();
+#pragma unroll
+                        for (int j = 0; j < MPL; j++) {
+                            if (pos[j] < 0) continue;
+                            const int q0 = max(pos[j] * h, r0), q1 = min(pos[j] * h + h, r0 + WAVE);
+                            for (int q = q0; q < q1; q++) {
+                                tsum(lo[j].Rs, lo[j].Rc, td[2 * (q - r0)]);
+                                tsum(lo[j].Rs, lo[j].Rc, td[2 * (q - r0) + 1]);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
             }
         }
     };
@@ -309,7 +445,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
 #pragma unroll
             for (int j = 0; j < MPL; j++) {
                 LaneOut &L = lo[j];
-                if (L.m >= a.m_end) continue;
+                if (L.lm < 0) continue;
                 while (L.wi < L.wend) {
                     const double wl = L.win[L.wi], wr = L.win[L.wi + 1];
                     if (!(wl < tr)) break;
@@ -331,7 +467,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
                     LaneOut &L = lo[j];
-                    if (L.m >= a.m_end) continue;
+                    if (L.lm < 0) continue;
                     while (L.wi < L.wend && L.win[L.wi + 1] <= pos) {
                         flush_window(a, L, wfirst[j]);
                         L.wi++;
@@ -356,15 +492,15 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                     __syncthreads();
 #pragma unroll
                     for (int j = 0; j < MPL; j++) {
-                        if (lo[j].m < a.m_end && (al > 0 || !(lo[j].o.flags & 1))) {
-                            val[j] += eval_summary(lo[j].o, xo, tot, false);
-                        }
+                        if (lo[j].lm < 0) continue;
+                        const TsgOut &o = sout[lo[j].lm];
+                        if (al > 0 || !(o.flags & 1)) val[j] += eval_summary(o, xo, tot, false);
                     }
                     __syncthreads();
                 }
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
-                    if (lo[j].m < a.m_end) tsum(lo[j].As, lo[j].Ac, val[j]);
+                    if (lo[j].lm >= 0) tsum(lo[j].As, lo[j].Ac, val[j]);
                 }
             }
         }
@@ -373,15 +509,15 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
         if (t1 < a.num_trees) apply_transition(t1);
 #pragma unroll
         for (int j = 0; j < MPL; j++) {
-            if (lo[j].m < a.m_end) {
-                a.dR[(int64_t) c * a.M + lo[j].m] = (double) lo[j].Rs + (double) lo[j].Rc;
+            if (lo[j].lm >= 0) {
+                a.dR[(int64_t) c * a.M + a.m_begin + lo[j].lm] = (double) lo[j].Rs + (double) lo[j].Rc;
             }
         }
     }
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
         LaneOut &L = lo[j];
-        if (L.m >= a.m_end) continue;
+        if (L.lm < 0) continue;
         while (L.wi < L.wend) {
             flush_window(a, L, wfirst[j]);
             L.wi++;
@@ -632,11 +768,12 @@ __global__ void __launch_bounds__(WAVE) k_afs_branch(const TsgSweepArgs a) {
 //
 // Counts: one thread per weighted sample walks its root path in the
 // snapshot, adding 1 to each of its columns.
-__global__ void k_tree0_counts(const TsgSweepArgs a, int32_t *x0) {
+// The tree is given samples first (local id j = sample j): t0_parent, t0_bl.
+__global__ void k_tree0_counts(const TsgSweepArgs a, const int32_t *t0_parent, int32_t *x0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nseed) return;
     const int c0 = a.seed_off[i], c1 = a.seed_off[i + 1];
-    for (int u = a.seed_sample[i]; u >= 0; u = a.loc_parent0[u]) {
+    for (int u = a.seed_sample[i]; u >= 0; u = t0_parent[u]) {
         for (int c = c0; c < c1; c++) atomicAdd(&x0[(int64_t) u * a.K + a.seed_col[c]], 1);
     }
 }
@@ -644,7 +781,8 @@ __global__ void k_tree0_counts(const TsgSweepArgs a, int32_t *x0) {
 #define TSG_T0_NODES 256
 
 // grid (ceil(nloc / TSG_T0_NODES), ceil(M / 64)); lane = output.
-__global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *x0,
+__global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *t0_parent,
+                                                    const float *t0_bl, const int32_t *x0,
                                                     int nloc, double *part) {
     const int m = blockIdx.y * WAVE + threadIdx.x;
     if (m >= a.M) return;
@@ -653,9 +791,9 @@ __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const 
     const TsgOut o = a.outs[m];
     double acc = 0;
     for (int u = u0; u < u1; u++) {
-        if (a.loc_parent0[u] < 0) continue;
+        if (t0_parent[u] < 0) continue;
         const float f = branch_summary(o, x0 + (int64_t) u * a.K, a.col_total);
-        acc += (double) __fmul_rn(a.loc_bl0[u], f);
+        acc += (double) __fmul_rn(t0_bl[u], f);
     }
     part[(int64_t) blockIdx.x * a.M + m] = acc;
 }
@@ -819,48 +957,75 @@ __global__ void k_fill_i32(int32_t *p, int64_t n) {
 #include "tsg_launch.h"
 
 namespace {
-template <int MODE, bool LDS>
+template <int MODE, int STORE, int MPL>
+hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
+    const void *fn = reinterpret_cast<const void *>(&k_sweep<MODE, STORE, MPL>);
+    if (shmem > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int) shmem);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sweep<MODE, STORE, MPL>), dim3(a.num_chunks), dim3(WAVE), shmem, s, a);
+    return hipGetLastError();
+}
+
+template <int MODE, int STORE>
 hipError_t launch_sweep_mpl(const TsgSweepArgs &a, int mpl, size_t shmem, hipStream_t s) {
-    dim3 grid(a.num_chunks), block(WAVE);
     switch (mpl) {
     case 1:
-        hipLaunchKernelGGL((k_sweep<MODE, LDS, 1>), grid, block, shmem, s, a);
-        break;
+        return launch_one<MODE, STORE, 1>(a, shmem, s);
     case 2:
-        hipLaunchKernelGGL((k_sweep<MODE, LDS, 2>), grid, block, shmem, s, a);
-        break;
+        return launch_one<MODE, STORE, 2>(a, shmem, s);
     case 4:
-        hipLaunchKernelGGL((k_sweep<MODE, LDS, 4>), grid, block, shmem, s, a);
-        break;
+        return launch_one<MODE, STORE, 4>(a, shmem, s);
     default:
-        hipLaunchKernelGGL((k_sweep<MODE, LDS, 8>), grid, block, shmem, s, a);
-        break;
+        return launch_one<MODE, STORE, 8>(a, shmem, s);
     }
-    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_sweep_store(const TsgSweepArgs &a, int store, int mpl, size_t shmem,
+                              hipStream_t s) {
+    switch (store) {
+    case TSG_STORE_HOT:
+        return launch_sweep_mpl<MODE, TSG_STORE_HOT>(a, mpl, shmem, s);
+    case TSG_STORE_LDS:
+        return launch_sweep_mpl<MODE, TSG_STORE_LDS>(a, mpl, shmem, s);
+    default:
+        return launch_sweep_mpl<MODE, TSG_STORE_GLOBAL>(a, mpl, shmem, s);
+    }
 }
 }  // namespace
 
-hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, bool lds, int mpl, size_t shmem,
+hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl, size_t shmem,
                             hipStream_t s) {
-    if (mode == 0) {
-        return lds ? launch_sweep_mpl<0, true>(a, mpl, shmem, s)
-                   : launch_sweep_mpl<0, false>(a, mpl, shmem, s);
-    }
-    return lds ? launch_sweep_mpl<1, true>(a, mpl, shmem, s)
-               : launch_sweep_mpl<1, false>(a, mpl, shmem, s);
+    return (mode == 0) ? launch_sweep_store<0>(a, store, mpl, shmem, s)
+                       : launch_sweep_store<1>(a, store, mpl, shmem, s);
 }
+
+namespace {
+template <typename F>
+hipError_t allow_lds(F *fn, size_t shmem) {
+    if (shmem <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) shmem);
+}
+}  // namespace
 
 hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t shmem,
                           hipStream_t s) {
     dim3 grid(a.num_chunks), block(WAVE);
+    hipError_t e;
     if (branch) {
         if (lds) {
+            if ((e = allow_lds(&k_afs_branch<true>, shmem)) != hipSuccess) return e;
             hipLaunchKernelGGL((k_afs_branch<true>), grid, block, shmem, s, a);
         } else {
             hipLaunchKernelGGL((k_afs_branch<false>), grid, block, shmem, s, a);
         }
     } else {
         if (lds) {
+            if ((e = allow_lds(&k_afs_site<true>, shmem)) != hipSuccess) return e;
             hipLaunchKernelGGL((k_afs_site<true>), grid, block, shmem, s, a);
         } else {
             hipLaunchKernelGGL((k_afs_site<false>), grid, block, shmem, s, a);
@@ -871,20 +1036,21 @@ hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t s
 
 int tsg_tree0_blocks(int nloc) { return (nloc + TSG_T0_NODES - 1) / TSG_T0_NODES; }
 
-hipError_t tsg_launch_tree0(const TsgSweepArgs &a, int32_t *x0, int nloc, double *part,
-                            hipStream_t s) {
+hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, const float *t0_bl,
+                            int32_t *x0, int nloc, double *part, hipStream_t s) {
     const int64_t words = (int64_t) nloc * a.K;
     if (words > 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned) ((words + 255) / 256)), dim3(256), 0, s, x0,
                            words);
     }
     if (a.nseed > 0) {
-        hipLaunchKernelGGL(k_tree0_counts, dim3((a.nseed + 255) / 256), dim3(256), 0, s, a, x0);
+        hipLaunchKernelGGL(k_tree0_counts, dim3((a.nseed + 255) / 256), dim3(256), 0, s, a,
+                           t0_parent, x0);
     }
     const int nb = tsg_tree0_blocks(nloc);
     if (nb > 0 && a.M > 0) {
-        hipLaunchKernelGGL(k_tree0_sum, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a, x0,
-                           nloc, part);
+        hipLaunchKernelGGL(k_tree0_sum, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a,
+                           t0_parent, t0_bl, x0, nloc, part);
     }
     return hipGetLastError();
 }

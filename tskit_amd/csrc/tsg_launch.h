@@ -4,16 +4,18 @@
 
 #include "tsg_kernels.h"
 
-hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, bool lds, int mpl, size_t shmem,
+// store: TSG_STORE_GLOBAL / TSG_STORE_LDS / TSG_STORE_HOT
+hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl, size_t shmem,
                             hipStream_t s);
 hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t shmem,
                           hipStream_t s);
 // number of per-block partials tsg_launch_tree0 writes (each a.M doubles)
 int tsg_tree0_blocks(int nloc);
-// counts of chunk 0's start tree into x0 [nloc x K] and its running branch
-// sum as per-block partials part [tsg_tree0_blocks(nloc) x M]
-hipError_t tsg_launch_tree0(const TsgSweepArgs &a, int32_t *x0, int nloc, double *part,
-                            hipStream_t s);
+// counts of the first tree (t0_parent / t0_bl, samples first, nloc nodes)
+// into x0 [nloc x K] and its running branch sum as per-block partials
+// part [tsg_tree0_blocks(nloc) x M]
+hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, const float *t0_bl,
+                            int32_t *x0, int nloc, double *part, hipStream_t s);
 hipError_t tsg_launch_combine(const TsgCombineArgs &a, const double *dR, double *Rstart,
                               const double *r0part, int nb, int64_t total, hipStream_t s);
 hipError_t tsg_launch_finals(const void *fin, int nq, const double *raw, const int32_t *ipar,

@@ -205,6 +205,49 @@ int tsg_init_trees(tsg_treeseq *ts, const char *ancestral_state,
     return 0;
 }
 
+namespace {
+
+// Cuts the trees into at most num_chunks runs of consecutive trees with about
+// the same number of replayed edge events (the opening transition -- every
+// edge with left == 0 -- is applied by the planner and not replayed).
+std::vector<int32_t> chunk_bounds(const tsg_treeseq *ts, int32_t num_chunks) {
+    const int32_t T = ts->num_trees, E = ts->num_edges;
+    std::vector<int64_t> tr_events(T + 1, 0);
+    {
+        int32_t j = 0, k = 0;
+        for (int32_t t = 0; t < T; t++) {
+            const double x = ts->breakpoints[t];
+            while (k < E && ts->right[ts->O[k]] == x) {
+                tr_events[t]++;
+                k++;
+            }
+            while (j < E && ts->left[ts->I[j]] == x) {
+                tr_events[t]++;
+                j++;
+            }
+        }
+    }
+    tr_events[0] = 0;
+    int64_t total_events = 0;
+    for (int32_t t = 0; t < T; t++) total_events += tr_events[t];
+    const int32_t C = std::max(1, std::min(num_chunks, T));
+    std::vector<int32_t> bounds{0};
+    const double per = (double) total_events / C;
+    int64_t acc = 0;
+    int32_t c = 1;
+    for (int32_t t = 0; t < T && c < C; t++) {
+        acc += tr_events[t];
+        if ((double) acc >= per * c && t + 1 < T) {
+            if (bounds.back() < t + 1) bounds.push_back(t + 1);
+            c++;
+        }
+    }
+    bounds.push_back(T);
+    return bounds;
+}
+
+}  // namespace
+
 // Builds the per-chunk sweep programs.  Chunk c covers trees
 // [chunk_tree0[c], chunk_tree0[c+1]).  Its program is
 //   * a local numbering of every node it touches (samples first, so local id
@@ -222,44 +265,8 @@ int tsg_init_trees(tsg_treeseq *ts, const char *ancestral_state,
 int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
     const int32_t T = ts->num_trees, E = ts->num_edges, N = ts->num_nodes;
     const int32_t n = ts->num_samples;
-    // events per transition
-    std::vector<int64_t> tr_events(T + 1, 0);
-    {
-        int32_t j = 0, k = 0;
-        for (int32_t t = 0; t < T; t++) {
-            const double x = ts->breakpoints[t];
-            while (k < E && ts->right[ts->O[k]] == x) {
-                tr_events[t]++;
-                k++;
-            }
-            while (j < E && ts->left[ts->I[j]] == x) {
-                tr_events[t]++;
-                j++;
-            }
-        }
-    }
-    tr_events[0] = 0;  // the opening transition is not replayed by any chunk
-    int64_t total_events = 0;
-    for (int32_t t = 0; t < T; t++) total_events += tr_events[t];
-    int32_t C = std::max(1, std::min(num_chunks, T));
-    ts->chunk_tree0.clear();
-    ts->chunk_tree0.push_back(0);
-    {
-        const double per = (double) total_events / C;
-        int64_t acc = 0;
-        int32_t c = 1;
-        for (int32_t t = 0; t < T && c < C; t++) {
-            acc += tr_events[t];
-            if ((double) acc >= per * c && t + 1 < T) {
-                if (ts->chunk_tree0.back() < t + 1) {
-                    ts->chunk_tree0.push_back(t + 1);
-                }
-                c++;
-            }
-        }
-    }
-    ts->chunk_tree0.push_back(T);
-    C = (int32_t) ts->chunk_tree0.size() - 1;
+    ts->chunk_tree0 = chunk_bounds(ts, num_chunks);
+    const int32_t C = (int32_t) ts->chunk_tree0.size() - 1;
     ts->num_chunks = C;
 
     ts->tr_off.assign(T + 1, 0);
@@ -391,5 +398,211 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
     // Every transition 0..T-1 is owned by exactly one chunk and they were
     // emitted in order, so the offsets are already monotone.
     ts->tr_off[T] = (int64_t) ts->ev_child.size();
+    ts->full_built = true;
+    return 0;
+}
+
+// Builds the hot plan (tsg_hotplan) over the same kind of chunks.  Each chunk
+// is planned in three steps on the live tree: record its events and root
+// paths while applying them, undo them to return to the chunk's start tree
+// and derive the slot structure there, then apply them again.
+int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
+    tsg_hotplan &hp = ts->hot;
+    hp = tsg_hotplan();
+    const int32_t T = ts->num_trees, E = ts->num_edges, N = ts->num_nodes;
+    const int32_t n = ts->num_samples;
+    hp.chunk_tree0 = chunk_bounds(ts, num_chunks);
+    const int32_t C = (int32_t) hp.chunk_tree0.size() - 1;
+    hp.num_chunks = C;
+    hp.tr_off.assign(T + 1, 0);
+    hp.tr_nrem.assign(T, 0);
+    hp.ev_path0.assign(1, 0);
+    hp.chunk_slot0.assign(C + 1, 0);
+    hp.chunk_lev0.assign(C + 1, 0);
+    hp.mut_slot.assign(ts->num_mutations, -1);
+    hp.sample_slot.assign((size_t) C * n, (uint16_t) TSG_NO_SLOT);
+
+    std::vector<int32_t> gpar(N, TSG_NULL);
+    std::vector<int32_t> hot_stamp(N, -1), hot_idx(N, -1);   // hot in chunk c, index in `hot`
+    std::vector<int32_t> memo_stamp(N, -1), memo_nha(N, -1); // nearest hot ancestor-or-self
+    std::vector<int32_t> hot;                                // global ids, first-touch order
+    std::vector<int32_t> ev_u, path_g;                       // events (global ids)
+    std::vector<int64_t> ev_p0;
+    std::vector<std::pair<int32_t, int32_t>> undo, redo;     // (child, parent before / after)
+    std::vector<int32_t> hpar, depth, slot_of, stack;
+    int32_t j = 0, k = 0;
+
+    // opening transition, then the first tree's snapshot (samples first)
+    while (j < E && ts->left[ts->I[j]] == ts->breakpoints[0]) {
+        gpar[ts->child[ts->I[j]]] = ts->parent[ts->I[j]];
+        j++;
+    }
+    {
+        std::vector<int32_t> lid(N, -1), order;
+        for (int32_t s = 0; s < n; s++) {
+            lid[ts->samples[s]] = (int32_t) order.size();
+            order.push_back(ts->samples[s]);
+        }
+        for (int32_t u = 0; u < N; u++) {
+            if (gpar[u] == TSG_NULL) continue;
+            for (int32_t w : {u, gpar[u]}) {
+                if (lid[w] < 0) {
+                    lid[w] = (int32_t) order.size();
+                    order.push_back(w);
+                }
+            }
+        }
+        hp.t0_parent.assign(order.size(), TSG_NULL);
+        hp.t0_bl.assign(order.size(), 0.0f);
+        for (size_t i = 0; i < order.size(); i++) {
+            const int32_t u = order[i];
+            if (gpar[u] != TSG_NULL) {
+                hp.t0_parent[i] = lid[gpar[u]];
+                hp.t0_bl[i] = (float) (ts->time[gpar[u]] - ts->time[u]);
+            }
+        }
+    }
+
+    for (int32_t c = 0; c < C; c++) {
+        hot.clear();
+        ev_u.clear();
+        path_g.clear();
+        ev_p0.assign(1, 0);
+        undo.clear();
+        redo.clear();
+        auto touch = [&](int32_t u) {
+            if (hot_stamp[u] != c) {
+                hot_stamp[u] = c;
+                hot_idx[u] = (int32_t) hot.size();
+                hot.push_back(u);
+            }
+        };
+        auto record_path = [&](int32_t p) {
+            for (int32_t w = p; w != TSG_NULL; w = gpar[w]) {
+                path_g.push_back(w);
+                touch(w);
+            }
+            ev_p0.push_back((int64_t) path_g.size());
+        };
+        const int32_t t0 = hp.chunk_tree0[c], t1 = hp.chunk_tree0[c + 1];
+        const int32_t tlast = std::min(t1, T - 1);  // t1 is the closing transition
+        const int64_t ev_base = (int64_t) hp.ev_child.size();
+        for (int32_t t = t0 + 1; t <= tlast; t++) {
+            hp.tr_off[t] = ev_base + (int64_t) ev_u.size();
+            const double x = ts->breakpoints[t];
+            int32_t nrem = 0;
+            while (k < E && ts->right[ts->O[k]] == x) {
+                const int32_t e = ts->O[k];
+                const int32_t u = ts->child[e];
+                ev_u.push_back(u);
+                hp.ev_bl.push_back(0.0f);
+                touch(u);
+                undo.emplace_back(u, gpar[u]);
+                gpar[u] = TSG_NULL;
+                redo.emplace_back(u, TSG_NULL);
+                record_path(ts->parent[e]);
+                nrem++;
+                k++;
+            }
+            while (j < E && ts->left[ts->I[j]] == x) {
+                const int32_t e = ts->I[j];
+                const int32_t u = ts->child[e], p = ts->parent[e];
+                ev_u.push_back(u);
+                hp.ev_bl.push_back((float) (ts->time[p] - ts->time[u]));
+                touch(u);
+                undo.emplace_back(u, gpar[u]);
+                gpar[u] = p;
+                redo.emplace_back(u, p);
+                record_path(p);
+                j++;
+            }
+            hp.tr_nrem[t] = nrem;
+        }
+        for (int32_t s = ts->tree_site0[t0]; s < ts->tree_site0[t1]; s++) {
+            for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) touch(ts->mut_node[m]);
+        }
+        // back to the start tree
+        for (size_t i = undo.size(); i-- > 0;) gpar[undo[i].first] = undo[i].second;
+
+        const int32_t H = (int32_t) hot.size();
+        if (H >= (int32_t) TSG_NO_SLOT) return 0;  // not representable: hp.usable stays false
+        // hot forest: nearest hot proper ancestor in the start tree
+        hpar.assign(H, -1);
+        for (int32_t i = 0; i < H; i++) {
+            int32_t w = gpar[hot[i]];
+            while (w != TSG_NULL && hot_stamp[w] != c) w = gpar[w];
+            hpar[i] = (w == TSG_NULL) ? -1 : hot_idx[w];
+        }
+        depth.assign(H, -1);
+        int32_t maxd = 0;
+        for (int32_t i = 0; i < H; i++) {
+            stack.clear();
+            int32_t w = i;
+            while (w >= 0 && depth[w] < 0) {
+                stack.push_back(w);
+                w = hpar[w];
+            }
+            int32_t d = (w >= 0) ? depth[w] : -1;
+            for (size_t q = stack.size(); q-- > 0;) depth[stack[q]] = ++d;
+            maxd = std::max(maxd, d);
+        }
+        // slots by decreasing depth (level 0 = deepest), stable within a level
+        const int32_t nlev = (H > 0) ? maxd + 1 : 0;
+        std::vector<int32_t> lev_count(nlev + 1, 0);
+        for (int32_t i = 0; i < H; i++) lev_count[maxd - depth[i] + 1]++;
+        for (int32_t l = 0; l < nlev; l++) lev_count[l + 1] += lev_count[l];
+        hp.chunk_lev0[c] = (int64_t) hp.lev_slot0.size();
+        hp.lev_slot0.insert(hp.lev_slot0.end(), lev_count.begin(), lev_count.end());
+        slot_of.assign(H, -1);
+        {
+            std::vector<int32_t> next(lev_count.begin(), lev_count.end() - 1);
+            for (int32_t i = 0; i < H; i++) slot_of[i] = next[maxd - depth[i]]++;
+        }
+        const int64_t s0 = hp.chunk_slot0[c];
+        hp.chunk_slot0[c + 1] = s0 + H;
+        hp.slot_par.resize((size_t) s0 + H);
+        hp.slot_bl0.resize((size_t) s0 + H);
+        for (int32_t i = 0; i < H; i++) {
+            const int32_t u = hot[i];
+            hp.slot_par[s0 + slot_of[i]] = (hpar[i] < 0) ? -1 : slot_of[hpar[i]];
+            hp.slot_bl0[s0 + slot_of[i]] =
+                (gpar[u] == TSG_NULL) ? 0.0f : (float) (ts->time[gpar[u]] - ts->time[u]);
+        }
+        for (int32_t u : ev_u) hp.ev_child.push_back(slot_of[hot_idx[u]]);
+        const int64_t pbase = (int64_t) hp.path.size();
+        for (size_t e = 1; e < ev_p0.size(); e++) hp.ev_path0.push_back(pbase + ev_p0[e]);
+        for (int32_t w : path_g) hp.path.push_back(slot_of[hot_idx[w]]);
+        for (size_t e = 0; e + 1 < ev_p0.size(); e++) {
+            hp.max_path = std::max(hp.max_path, (int32_t) (ev_p0[e + 1] - ev_p0[e]));
+        }
+        for (int32_t s = ts->tree_site0[t0]; s < ts->tree_site0[t1]; s++) {
+            for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) {
+                hp.mut_slot[m] = slot_of[hot_idx[ts->mut_node[m]]];
+            }
+        }
+        // each sample's nearest hot ancestor-or-self, memoised along the walk
+        uint16_t *ss = hp.sample_slot.data() + (size_t) c * n;
+        for (int32_t s = 0; s < n; s++) {
+            stack.clear();
+            int32_t w = ts->samples[s];
+            while (w != TSG_NULL && hot_stamp[w] != c && memo_stamp[w] != c) {
+                stack.push_back(w);
+                w = gpar[w];
+            }
+            const int32_t res = (w == TSG_NULL) ? TSG_NULL : (hot_stamp[w] == c ? w : memo_nha[w]);
+            for (int32_t v : stack) {
+                memo_stamp[v] = c;
+                memo_nha[v] = res;
+            }
+            ss[s] = (res == TSG_NULL) ? (uint16_t) TSG_NO_SLOT : (uint16_t) slot_of[hot_idx[res]];
+        }
+        hp.max_slots = std::max(hp.max_slots, H);
+        hp.max_levels = std::max(hp.max_levels, nlev);
+        // forward to the next chunk's start tree
+        for (const auto &r : redo) gpar[r.first] = r.second;
+    }
+    hp.tr_off[T] = (int64_t) hp.ev_child.size();
+    hp.chunk_lev0[C] = (int64_t) hp.lev_slot0.size();
+    hp.usable = true;
     return 0;
 }

@@ -38,6 +38,66 @@ struct TsgDevice {
     size_t bytes = 0;
 };
 
+// Device copy of a hot plan (see tsg_hotplan).
+struct TsgHotDevice {
+    int32_t *chunk_tree0 = nullptr;
+    int64_t *chunk_slot0 = nullptr;
+    int32_t *slot_par = nullptr;
+    float *slot_bl0 = nullptr;
+    int64_t *chunk_lev0 = nullptr;
+    int32_t *lev_slot0 = nullptr;
+    uint16_t *sample_slot = nullptr;
+    int64_t *tr_off = nullptr;
+    int32_t *tr_nrem = nullptr;
+    int32_t *ev_child = nullptr;
+    float *ev_bl = nullptr;
+    int64_t *ev_path0 = nullptr;
+    int32_t *path = nullptr;
+    int32_t *mut_slot = nullptr;
+    int32_t *t0_parent = nullptr;
+    float *t0_bl = nullptr;
+    size_t bytes = 0;
+};
+
+// Sweep program whose chunks keep only the nodes they touch.  A node is hot
+// in chunk c when an event of c names it (edge child, or a node on the root
+// path of the edge's parent end) or a mutation of c's sites sits on it; only
+// hot nodes are ever read or written by c's sweep, so the kernel holds their
+// counts and branch lengths in LDS ("slots").  The start tree's counts of the
+// hot nodes are rebuilt from the samples without walking the tree:
+//   * sample_slot[c*n + s]: slot of sample s's nearest hot ancestor-or-self
+//     in c's start tree (TSG_NO_SLOT if none) -- every sample adds its weight
+//     there;
+//   * slot_par: the slot's nearest hot proper ancestor in the start tree
+//     (-1 if none); slots are numbered by decreasing depth in that hot forest,
+//     one contiguous range per depth level (lev_slot0), so adding each level
+//     into its parents, deepest level first, completes the subtree counts.
+// Event, path and mutation node ids are slot ids.
+struct tsg_hotplan {
+    bool usable = false;                  // false: a chunk exceeds the slot id range
+    int32_t num_chunks = 0;
+    std::vector<int32_t> chunk_tree0;     // [C+1]
+    std::vector<int64_t> chunk_slot0;     // [C+1]
+    std::vector<int32_t> slot_par;        // [slots] chunk-local slot or -1
+    std::vector<float> slot_bl0;          // [slots] start-tree branch length (0 if root)
+    std::vector<int64_t> chunk_lev0;      // [C+1] offsets into lev_slot0
+    std::vector<int32_t> lev_slot0;       // per chunk: level boundaries (nlev + 1 entries)
+    std::vector<uint16_t> sample_slot;    // [C * n]
+    std::vector<int64_t> tr_off;          // [T+1]
+    std::vector<int32_t> tr_nrem;         // [T]
+    std::vector<int32_t> ev_child;        // slot ids
+    std::vector<float> ev_bl;
+    std::vector<int64_t> ev_path0;        // [events + 1]
+    std::vector<int32_t> path;            // slot ids, parent end first
+    std::vector<int32_t> mut_slot;        // [Mu]
+    // the first tree, samples first (local id j = sample j): its running
+    // branch sum is evaluated over all of its nodes
+    std::vector<int32_t> t0_parent;
+    std::vector<float> t0_bl;
+    int32_t max_slots = 0, max_path = 0, max_levels = 0;
+    TsgHotDevice dev;
+};
+
 struct tsg_treeseq {
     // tables (host copies)
     double L = 0;
@@ -81,10 +141,15 @@ struct tsg_treeseq {
     std::vector<int32_t> mut_node_loc;
     int64_t total_path_hops = 0;          // topology-only measure: sum of path lengths
 
+    bool full_built = false;              // the full plan above is built on first need
+    int32_t full_chunks = 1;              // chunk count it is built with
+    tsg_hotplan hot;
+
     TsgDevice dev;
     void *workspace = nullptr;            // per-handle device scratch (tsg_api.cpp)
 };
 
 int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks);
+int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks);
 int tsg_upload(tsg_treeseq *ts);
 void tsg_device_free(TsgDevice *d);

@@ -209,6 +209,49 @@ class LowLevelTreeSequence:
                                          p["path"].ctypes.data_as(ctypes.c_void_p))
         return p
 
+    def get_hot_plan_sizes(self):
+        """Sizes of the hot plan (no copies)."""
+        sizes = (ctypes.c_uint64 * 9)()
+        self._lib.tsg_treeseq_get_hot_plan_sizes(self._h, sizes)
+        keys = ["usable", "chunks", "slots", "level_entries", "events", "path_entries",
+                "first_tree_nodes", "max_slots", "max_levels"]
+        return dict(zip(keys, (int(x) for x in sizes)))
+
+    def get_hot_plan(self):
+        """Host copy of the hot plan (chunks keep only the nodes they touch),
+        or None when the tree sequence has a chunk it cannot represent."""
+        sizes = (ctypes.c_uint64 * 9)()
+        self._lib.tsg_treeseq_get_hot_plan_sizes(self._h, sizes)
+        usable, C, ns, nlev, ne, npath, n0, max_slots, max_levels = (int(x) for x in sizes)
+        if not usable:
+            return None
+        T = self.get_num_trees()
+        Mu = self.get_num_mutations()
+        n = self.get_num_samples()
+        p = {
+            "chunk_tree0": np.zeros(C + 1, np.int32), "chunk_slot0": np.zeros(C + 1, np.int64),
+            "slot_par": np.zeros(ns, np.int32), "slot_bl0": np.zeros(ns, np.float32),
+            "chunk_lev0": np.zeros(C + 1, np.int64), "lev_slot0": np.zeros(nlev, np.int32),
+            "sample_slot": np.zeros(C * n, np.uint16), "tr_off": np.zeros(T + 1, np.int64),
+            "tr_nrem": np.zeros(T, np.int32), "ev_child": np.zeros(ne, np.int32),
+            "ev_bl": np.zeros(ne, np.float32), "ev_path0": np.zeros(ne + 1, np.int64),
+            "path": np.zeros(npath, np.int32), "mut_slot": np.zeros(Mu, np.int32),
+            "t0_parent": np.zeros(n0, np.int32), "t0_bl": np.zeros(n0, np.float32),
+        }
+        check(self._lib.tsg_treeseq_copy_hot_plan(
+            self._h, *[a.ctypes.data_as(ctypes.c_void_p) for a in p.values()]))
+        p["max_slots"] = max_slots
+        p["max_levels"] = max_levels
+        return p
+
+    def set_sweep_plan(self, plan):
+        """1: run statistics sweeps on the full plan; 0: the library chooses."""
+        check(self._lib.tsg_treeseq_set_sweep_plan(self._h, int(plan)))
+
+    def get_last_store(self):
+        """Node store of the last sweep: 0 HBM slabs, 1 LDS full plan, 2 LDS hot slots."""
+        return int(self._lib.tsg_treeseq_get_last_store(self._h))
+
     def set_num_chunks(self, num_chunks):
         check(self._lib.tsg_treeseq_set_num_chunks(self._h, int(num_chunks)))
 
