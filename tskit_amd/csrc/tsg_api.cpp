@@ -41,11 +41,11 @@ struct DevBuf {
 struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf seed_off;
+    DevBuf seed_off, seed_ps;
     DevBuf pair_s, pair_c, tot, outs, queries, windows, cq_wfirst, cq_nwin, cq_poff, partial, dR,
         Rstart, raw, rawbase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
     // first tree: counts and per-block running-sum partials
-    DevBuf x0, r0part;
+    DevBuf x0, r0part, bsum;
     // relatedness-matrix route
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab;
     int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
@@ -117,18 +117,22 @@ int upload_seeds(Workspace *ws, const std::vector<int32_t> &pair_s,
     std::vector<int32_t> col(pair_s.size());
     std::vector<int32_t> fill(first.begin(), first.end() - 1);
     for (size_t i = 0; i < pair_s.size(); i++) col[fill[pair_s[i]]++] = pair_c[i];
-    std::vector<int32_t> seed_sample, seed_off{0};
+    std::vector<int32_t> seed_sample, seed_off{0}, flat_s(pair_s.size());
     for (int32_t j = 0; j < num_samples; j++) {
         if (count[j + 1] > 0) {
             seed_sample.push_back(j);
             seed_off.push_back(first[j + 1]);
+            std::fill(flat_s.begin() + first[j], flat_s.begin() + first[j + 1], j);
         }
     }
     int ret;
     if ((ret = upload(ws->pair_s, seed_sample, s)) || (ret = upload(ws->pair_c, col, s)) ||
-        (ret = upload(ws->seed_off, seed_off, s))) {
+        (ret = upload(ws->seed_off, seed_off, s)) || (ret = upload(ws->seed_ps, flat_s, s))) {
         return ret;
     }
+    a.npairs = (int32_t) flat_s.size();
+    a.pair_sample = (const int32_t *) ws->seed_ps.p;
+    a.pair_col = (const int32_t *) ws->pair_c.p;
     a.nseed = (int32_t) seed_sample.size();
     a.seed_sample = (const int32_t *) ws->pair_s.p;
     a.seed_col = (const int32_t *) ws->pair_c.p;
@@ -409,7 +413,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         if ((ret = ensure(ws->dR, (size_t) C * M * 8)) ||
             (ret = ensure(ws->Rstart, (size_t) C * M * 8)) ||
             (ret = ensure(ws->x0, (size_t) nloc0 * K * 4)) ||
-            (ret = ensure(ws->r0part, (size_t) nb0 * M * 8))) {
+            (ret = ensure(ws->r0part, (size_t) nb0 * M * 8)) ||
+            (ret = ensure(ws->bsum, (size_t) tsg_scan_blocks(C) * M * 8))) {
             return ret;
         }
     }
@@ -521,7 +526,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     ca.rawbase = (const int64_t *) ws->rawbase.p;
     ca.flags = 0;
     if ((ret = hip_check(tsg_launch_combine(ca, (mode == TSG_STAT_BRANCH) ? a.dR : nullptr,
-                                            (double *) ws->Rstart.p,
+                                            (double *) ws->Rstart.p, (double *) ws->bsum.p,
                                             (const double *) ws->r0part.p, nb0, rawcount, s)))) {
         return ret;
     }
@@ -687,13 +692,13 @@ int ensure_full(tsg_treeseq *ts) {
 void free_workspace(tsg_treeseq *ts) {
     Workspace *ws = ws_of(ts);
     if (ws == nullptr) return;
-    DevBuf *bufs[] = {&ws->seed_off, &ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
+    DevBuf *bufs[] = {&ws->seed_off, &ws->seed_ps, &ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
                       &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
                       &ws->Rstart, &ws->raw, &ws->rawbase, &ws->out, &ws->fin, &ws->ipar,
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
                       &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
                       &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->x0,
-                      &ws->r0part};
+                      &ws->r0part, &ws->bsum};
     for (DevBuf *b : bufs) {
         if (b->p != nullptr) (void) hipFree(b->p);
     }

@@ -33,10 +33,15 @@ struct TsgQuery {
 
 // 32-bit words of one TsgOut.
 #define TSG_OUT_INTS ((int) (sizeof(TsgOut) / 4))
+// Trees, events and path entries the sweep kernel stages in LDS per batch.
+#define TSG_TB 64
+#define TSG_EB 64
+#define TSG_PB 1024
 // 32-bit words of the sweep kernel's LDS staging block for K state columns
 // and `span` outputs in the launch (layout in k_sweep); node records follow.
-#define TSG_STAGE_INTS(K, span) \
-    ((int64_t) (span) * (TSG_OUT_INTS + 1) + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) + \
+#define TSG_STAGE_INTS(K, span)                                                               \
+    ((int64_t) (span) * (TSG_OUT_INTS + 1) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + \
+     (TSG_TB + 1) + 2 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) +   \
      TSG_PSEG + 2 * 64 + 2)
 
 // Where a sweep keeps its node records (see k_sweep).
@@ -72,6 +77,9 @@ struct TsgSweepArgs {
     // weighted samples: seed_sample[i] belongs to the columns
     // seed_col[seed_off[i] .. seed_off[i+1])
     const int32_t *seed_sample, *seed_off, *seed_col;
+    // the same weights as a flat (sample, column) list, by sample
+    const int32_t *pair_sample, *pair_col;
+    int32_t npairs;
     const int64_t *ev_path0;     // per event: offset of its root path
     const int32_t *path;
     // hot plan only (node ids are slots; chunk_node0 holds slot ranges and

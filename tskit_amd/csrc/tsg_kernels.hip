@@ -49,59 +49,66 @@ __device__ __forceinline__ void tsum(float &s, float &c, float x) {
 }
 
 // Reference summary function o.stat evaluated on counts x (or on the
-// complement tot - x for the unpolarised wrapper).
+// complement tot - x for the unpolarised wrapper).  Counts are converted to
+// FP64 first: every product of up to three counts below 2^17.6 (n <= ~2e5
+// samples per set) is exact in FP64, so the numerators are the exact integer
+// polynomials, scaled once by 1/denominator and rounded to FP32 -- the same
+// values an int64 evaluation gives, at FP64 rather than 64-bit integer
+// multiply cost.  (f2 / f3 / f4 multiply two exact factors, one rounding.)
 __device__ float eval_summary(const TsgOut &o, const int32_t *x, const int32_t *tot, bool comp) {
-#define XV(col) ((int64_t) (comp ? (tot[col] - x[col]) : x[col]))
-    const int64_t xi = XV(o.i);
+#define XV(col) ((double) (comp ? (tot[col] - x[col]) : x[col]))
+    const double xi = XV(o.i);
+    const double ni = (double) o.ni;
     switch (o.stat) {
     case TSG_STAT_DIVERSITY:
-        return (float) ((double) (xi * (o.ni - xi)) * o.inv);
+        return (float) ((xi * (ni - xi)) * o.inv);
     case TSG_STAT_SEGREGATING_SITES:
-        return (xi > 0) ? (float) ((double) (o.ni - xi) * o.inv) : 0.0f;
+        return (xi > 0) ? (float) ((ni - xi) * o.inv) : 0.0f;
     case TSG_STAT_Y1:
-        return (float) ((double) (xi * (o.ni - xi) * (o.ni - xi - 1)) * o.inv);
+        return (float) ((xi * (ni - xi) * (ni - xi - 1)) * o.inv);
     case TSG_STAT_DIVERGENCE: {
-        const int64_t xj = XV(o.j);
-        return (float) ((double) (xi * (o.nj - xj)) * o.inv);
+        const double xj = XV(o.j);
+        return (float) ((xi * ((double) o.nj - xj)) * o.inv);
     }
     case TSG_STAT_RELATEDNESS: {
         double sum = 0;
         for (int c = o.cbase; c < o.cbase + o.ccount; c++) {
-            sum += (double) XV(c) / (double) tot[c];
+            sum += XV(c) / (double) tot[c];
         }
         const double mean = sum / (double) o.ccount;
-        const int64_t xj = XV(o.j);
-        return (float) (((double) xi / o.ni - mean) * ((double) xj / o.nj - mean));
+        const double xj = XV(o.j);
+        return (float) ((xi / ni - mean) * (xj / (double) o.nj - mean));
     }
     case TSG_STAT_RELATEDNESS_NONCENTRED: {
-        const int64_t xj = XV(o.j);
-        return (float) ((double) (xi * xj) * o.inv);
+        const double xj = XV(o.j);
+        return (float) ((xi * xj) * o.inv);
     }
     case TSG_STAT_Y2: {
-        const int64_t xj = XV(o.j);
-        return (float) ((double) (xi * (o.nj - xj) * (o.nj - xj - 1)) * o.inv);
+        const double xj = XV(o.j), nj = (double) o.nj;
+        return (float) ((xi * (nj - xj) * (nj - xj - 1)) * o.inv);
     }
     case TSG_STAT_F2: {
-        const int64_t xj = XV(o.j);
-        const int64_t a = xi * (o.nj - xj);
-        const int64_t b = (xi - 1) * (o.nj - xj - 1) - (o.ni - xi) * xj;
-        return (float) ((double) a * (double) b * o.inv);
+        const double xj = XV(o.j), nj = (double) o.nj;
+        const double a = xi * (nj - xj);
+        const double b = (xi - 1) * (nj - xj - 1) - (ni - xi) * xj;
+        return (float) (a * b * o.inv);
     }
     case TSG_STAT_Y3: {
-        const int64_t xj = XV(o.j), xk = XV(o.k);
-        return (float) ((double) (xi * (o.nj - xj) * (o.nk - xk)) * o.inv);
+        const double xj = XV(o.j), xk = XV(o.k);
+        return (float) ((xi * ((double) o.nj - xj) * ((double) o.nk - xk)) * o.inv);
     }
     case TSG_STAT_F3: {
-        const int64_t xj = XV(o.j), xk = XV(o.k);
-        const int64_t a = xi * (o.nj - xj);
-        const int64_t b = (xi - 1) * (o.nk - xk) - (o.ni - xi) * xk;
-        return (float) ((double) a * (double) b * o.inv);
+        const double xj = XV(o.j), xk = XV(o.k), nk = (double) o.nk;
+        const double a = xi * ((double) o.nj - xj);
+        const double b = (xi - 1) * (nk - xk) - (ni - xi) * xk;
+        return (float) (a * b * o.inv);
     }
     case TSG_STAT_F4: {
-        const int64_t xj = XV(o.j), xk = XV(o.k), xl = XV(o.l);
-        const int64_t a = xi * (o.nj - xj);
-        const int64_t b = xk * (o.nl - xl) - xl * (o.nk - xk);
-        return (float) ((double) a * (double) b * o.inv);
+        const double xj = XV(o.j), xk = XV(o.k), xl = XV(o.l);
+        const double nk = (double) o.nk, nl = (double) o.nl;
+        const double a = xi * ((double) o.nj - xj);
+        const double b = xk * (nl - xl) - xl * (nk - xk);
+        return (float) (a * b * o.inv);
     }
     default:
         return 0.0f;
@@ -153,12 +160,25 @@ __device__ void init_hot(const TsgSweepArgs &a, const Nodes &nd, int c, int nslo
     for (int64_t i = lane; i < words; i += WAVE) nd.rec[i] = 0;
     __syncthreads();
     for (int h = lane; h < nslot; h += WAVE) nd.set_edge(h, TSG_NULL, a.loc_bl0[s0 + h]);
+    // (sample, column) pairs, several per lane in flight: the slot gathers
+    // are independent HBM reads
     const uint16_t *ss = a.sample_slot + (int64_t) c * a.num_samples;
-    for (int i = lane; i < a.nseed; i += WAVE) {
-        const unsigned h = ss[a.seed_sample[i]];
-        if (h == TSG_NO_SLOT) continue;
-        const int c0 = a.seed_off[i], c1 = a.seed_off[i + 1];
-        for (int cc = c0; cc < c1; cc++) atomicAdd(&nd.x((int) h, a.seed_col[cc]), 1);
+    constexpr int U = 8;
+    for (int i0 = lane; i0 < a.npairs; i0 += WAVE * U) {
+        int ps[U], pc[U];
+        unsigned ph[U];
+#pragma unroll
+        for (int r = 0; r < U; r++) {
+            const int i = i0 + r * WAVE;
+            ps[r] = (i < a.npairs) ? a.pair_sample[i] : -1;
+            pc[r] = (i < a.npairs) ? a.pair_col[i] : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < U; r++) ph[r] = (ps[r] >= 0) ? (unsigned) ss[ps[r]] : TSG_NO_SLOT;
+#pragma unroll
+        for (int r = 0; r < U; r++) {
+            if (ph[r] != TSG_NO_SLOT) atomicAdd(&nd.x((int) ph[r], pc[r]), 1);
+        }
     }
     __syncthreads();
     const int32_t *lev = a.lev_slot0 + a.chunk_lev0[c];
@@ -269,11 +289,14 @@ __device__ __forceinline__ float branch_summary(const TsgOut &o, const int32_t *
 // TSG_STORE_LDS (full plan, every local node in LDS) or TSG_STORE_HOT (hot
 // plan, only the chunk's hot slots, in LDS).
 //
-// LDS: the staging block (TSG_STAGE_INTS) first -- this launch's outputs,
-// the moved subtree's counts, a path segment's counts before / after the
-// update and its branch lengths, column totals, an all-zero state, the
-// active-output list and one round of summary terms -- then, for the LDS
-// stores, the node records.
+// LDS: the staging block (TSG_STAGE_INTS) first, then for the LDS stores the
+// node records.  The staging block holds this launch's outputs, a batch of
+// trees (breakpoints, transition offsets, site ranges), a batch of events
+// with their root paths, the moved subtree's counts, a path segment's counts
+// before / after the update and its branch lengths, column totals, an
+// all-zero state, the active-output list and one round of summary terms.
+// Trees and events are read from HBM a batch at a time with coalesced loads,
+// so a chunk pays the memory latency once per batch, not once per event.
 //
 // Branch mode, per event: the columns the moved subtree carries (a ballot)
 // decide which outputs can change.  An output none of whose columns moved
@@ -293,15 +316,24 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     const int span = a.m_end - a.m_begin;
     const int64_t n0 = a.chunk_node0[c];
     const int nloc = (int) (a.chunk_node0[c + 1] - n0);
-    TsgOut *sout = reinterpret_cast<TsgOut *>(smem);                  // [span]
-    int32_t *xc = smem + span * TSG_OUT_INTS;                           // [K]
-    int32_t *xo = xc + K;                                               // [PSEG x K]
-    int32_t *xn = xo + TSG_PSEG * K;                                    // [PSEG x K]
-    float *sbl = reinterpret_cast<float *>(xn + TSG_PSEG * K);          // [PSEG]
-    int32_t *tot = reinterpret_cast<int32_t *>(sbl + TSG_PSEG);         // [K]
-    int32_t *zero = tot + K;                                            // [K]
-    int32_t *act = zero + K;                                            // [span]
-    float *td = reinterpret_cast<float *>(act + span);                  // [2 x WAVE]
+    // 8-byte items first
+    TsgOut *sout = reinterpret_cast<TsgOut *>(smem);                    // [span]
+    double *tbp = reinterpret_cast<double *>(sout + span);              // [TB+1] breakpoints
+    int64_t *toff = reinterpret_cast<int64_t *>(tbp + TSG_TB + 1);       // [TB+1] tr_off
+    int64_t *evp = toff + TSG_TB + 1;                                    // [EB+1] ev_path0
+    int32_t *tnrem = reinterpret_cast<int32_t *>(evp + TSG_EB + 1);      // [TB]
+    int32_t *ts0 = tnrem + TSG_TB;                                       // [TB+1] tree_site0
+    int32_t *evc = ts0 + TSG_TB + 1;                                     // [EB] event child
+    float *evb = reinterpret_cast<float *>(evc + TSG_EB);                // [EB] event branch length
+    int32_t *pbuf = reinterpret_cast<int32_t *>(evb + TSG_EB);           // [PB] path entries
+    int32_t *xc = pbuf + TSG_PB;                                         // [K]
+    int32_t *xo = xc + K;                                                // [PSEG x K]
+    int32_t *xn = xo + TSG_PSEG * K;                                     // [PSEG x K]
+    float *sbl = reinterpret_cast<float *>(xn + TSG_PSEG * K);           // [PSEG]
+    int32_t *tot = reinterpret_cast<int32_t *>(sbl + TSG_PSEG);          // [K]
+    int32_t *zero = tot + K;                                             // [K]
+    int32_t *act = zero + K;                                             // [span]
+    float *td = reinterpret_cast<float *>(act + span);                   // [2 x WAVE]
     Nodes nd;
     nd.R = R;
     if (STORE != TSG_STORE_GLOBAL) {
@@ -332,12 +364,71 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
         if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(sout[lo[j].lm], zero, tot);
     }
 
+    const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
+    // the chunk's events: transitions t0+1 .. t1 (the last is the closing
+    // transition into the next chunk, when there is one)
+    const int tlast = min(t1, a.num_trees - 1);
+    const int64_t ev_end = a.tr_off[tlast + 1];
+
+    // tree batch: entries t = tb .. tb + TB of tr_off / bp / tree_site0 and
+    // tb .. tb + TB - 1 of tr_nrem (never past index num_trees)
+    int tb = -1;
+    auto stage_trees = [&](int t) {
+        __syncthreads();
+        tb = t;
+        for (int l = lane; l <= TSG_TB; l += WAVE) {
+            const int x = t + l;
+            if (x <= a.num_trees) {
+                toff[l] = a.tr_off[x];
+                tbp[l] = a.bp[x];
+                ts0[l] = a.tree_site0[x];
+                if (l < TSG_TB) tnrem[l] = (x < a.num_trees) ? a.tr_nrem[x] : 0;
+            }
+        }
+        __syncthreads();
+    };
+    auto tree = [&](int t) {  // make entries t and t + 1 resident
+        if (tb < 0 || t < tb || t + 1 > tb + TSG_TB) stage_trees(t);
+        return t - tb;
+    };
+
+    // event batch: events eb .. eb + neb - 1 with their paths in pbuf (or
+    // read from HBM when the first event's path alone exceeds pbuf)
+    int64_t eb = 0, pb0 = 0;
+    int neb = 0;
+    bool gpath = false;
+    auto stage_events = [&](int64_t e) {
+        __syncthreads();
+        const int n = (int) min((int64_t) TSG_EB, ev_end - e);
+        if (lane < n) {
+            evc[lane] = a.ev_child[e + lane];
+            evb[lane] = a.ev_bl[e + lane];
+        }
+        for (int l = lane; l <= n; l += WAVE) evp[l] = a.ev_path0[e + l];
+        __syncthreads();
+        const int64_t base = evp[0];
+        // events whose paths end inside pbuf (a prefix, offsets increase)
+        const bool fits = lane < n && evp[lane + 1] - base <= TSG_PB;
+        const int nf = __popcll(__ballot(fits));
+        neb = nf > 0 ? nf : 1;
+        gpath = nf == 0;
+        eb = e;
+        pb0 = base;
+        const int plen = gpath ? 0 : (int) (evp[neb] - base);
+        for (int i = lane; i < plen; i += WAVE) pbuf[i] = a.path[base + i];
+        __syncthreads();
+    };
+
     auto apply_transition = [&](int t) {
-        const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
-        const int64_t nrem = a.tr_nrem[t];
+        const int lt = tree(t);
+        const int64_t e0 = toff[lt], e1 = toff[lt + 1];
+        const int64_t nrem = tnrem[lt];
         for (int64_t e = e0; e < e1; e++) {
+            if (e >= eb + neb) stage_events(e);
+            const int le = (int) (e - eb);
             const bool ins = (e - e0) >= nrem;
-            const int u = a.ev_child[e];
+            const int u = evc[le];
+            const float ebl = evb[le];
             // columns the moved subtree carries
             uint64_t cm;
             if (K <= WAVE) {
@@ -361,7 +452,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             // subtree, because unpolarised summaries need not vanish at zero
             // (non-centred relatedness: F(0) = f(0) + f(total) = 1)
             if (MODE == 0) {
-                const float b = ins ? a.ev_bl[e] : nd.bl(u);
+                const float b = ins ? ebl : nd.bl(u);
                 const float sg = ins ? 1.0f : -1.0f;
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
@@ -375,7 +466,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             __syncthreads();
             // after seeding nothing reads parent pointers (the root paths
             // are precomputed), so only the branch length is kept current
-            if (lane == 0) nd.set_bl(u, ins ? a.ev_bl[e] : 0.0f);
+            if (lane == 0) nd.set_bl(u, ins ? ebl : 0.0f);
             if (cm == 0) continue;  // no count changes, no summary changes
             // outputs whose summaries can change, compacted into act[]
             int pos[MPL];
@@ -388,13 +479,14 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 if (on) act[pos[j]] = lo[j].lm;
                 na += __popcll(bal);
             }
+            const int64_t p0 = evp[le], p1 = evp[le + 1];
+            const int32_t *pth = gpath ? a.path + p0 : pbuf + (p0 - pb0);
             __syncthreads();
-            const int64_t p0 = a.ev_path0[e], p1 = a.ev_path0[e + 1];
-            for (int64_t sg0 = p0; sg0 < p1; sg0 += TSG_PSEG) {
-                const int h = (int) min((int64_t) TSG_PSEG, p1 - sg0);
+            for (int64_t sg0 = 0; sg0 < p1 - p0; sg0 += TSG_PSEG) {
+                const int h = (int) min((int64_t) TSG_PSEG, p1 - p0 - sg0);
                 for (int idx = lane; idx < h * K; idx += WAVE) {
                     const int i = idx / K, k = idx - i * K;
-                    const int node = a.path[sg0 + i];
+                    const int node = pth[sg0 + i];
                     const int32_t old = nd.x(node, k);
                     const int32_t nw = ins ? old + xc[k] : old - xc[k];
                     nd.x(node, k) = nw;
@@ -404,7 +496,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                     }
                 }
                 if (MODE == 0) {
-                    for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(a.path[sg0 + i]);
+                    for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(pth[sg0 + i]);
                 }
                 __syncthreads();
                 if (MODE == 0 && na > 0) {
@@ -437,11 +529,11 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
         }
     };
 
-    const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
     for (int t = t0; t < t1; t++) {
         if (t > t0) apply_transition(t);
+        const int lt = tree(t);
         if (MODE == 0) {
-            const double tl = a.bp[t], tr = a.bp[t + 1];
+            const double tl = tbp[lt], tr = tbp[lt + 1];
 #pragma unroll
             for (int j = 0; j < MPL; j++) {
                 LaneOut &L = lo[j];
@@ -461,7 +553,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 }
             }
         } else {
-            const int s0 = a.tree_site0[t], s1 = a.tree_site0[t + 1];
+            const int s0 = ts0[lt], s1 = ts0[lt + 1];
             for (int s = s0; s < s1; s++) {
                 const double pos = a.site_pos[s];
 #pragma unroll
@@ -804,13 +896,41 @@ __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const 
 // span-normalise.  FP64, deterministic chunk order.
 // Rstart[c] = R(first tree) + sum of the chunk deltas before c; the first
 // tree's running sum arrives as nb per-block partials (k_tree0_sum).
-__global__ void k_scan_dR(const double *dR, double *Rstart, int C, int M, const double *r0part,
-                          int nb) {
+// Rstart[c] = R(first tree) + sum of the chunk deltas before c, as a
+// two-level FP64 scan in a fixed order (deterministic): chunk blocks of
+// TSG_SCAN_B sum their deltas, one pass per output turns the block sums into
+// exclusive prefixes (starting from the first tree's nb per-block partials,
+// k_tree0_sum), then every block writes its chunks' prefixes.  Grid of the
+// block kernels: (blocks, ceil(M / 64)), lane = output.
+#define TSG_SCAN_B 128
+
+__global__ void k_scan_blocks(const double *dR, double *bsum, int C, int M) {
+    const int m = blockIdx.y * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const int c0 = blockIdx.x * TSG_SCAN_B, c1 = min(C, c0 + TSG_SCAN_B);
+    double acc = 0;
+    for (int c = c0; c < c1; c++) acc += dR[(int64_t) c * M + m];
+    bsum[(int64_t) blockIdx.x * M + m] = acc;
+}
+
+__global__ void k_scan_top(double *bsum, int nblk, int M, const double *r0part, int nb) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
     double acc = 0;
     for (int b = 0; b < nb; b++) acc += r0part[(int64_t) b * M + m];
-    for (int c = 0; c < C; c++) {
+    for (int b = 0; b < nblk; b++) {
+        const double t = bsum[(int64_t) b * M + m];
+        bsum[(int64_t) b * M + m] = acc;
+        acc += t;
+    }
+}
+
+__global__ void k_scan_apply(const double *dR, const double *bsum, double *Rstart, int C, int M) {
+    const int m = blockIdx.y * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const int c0 = blockIdx.x * TSG_SCAN_B, c1 = min(C, c0 + TSG_SCAN_B);
+    double acc = bsum[(int64_t) blockIdx.x * M + m];
+    for (int c = c0; c < c1; c++) {
         Rstart[(int64_t) c * M + m] = acc;
         acc += dR[(int64_t) c * M + m];
     }
@@ -1055,11 +1175,19 @@ hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, con
     return hipGetLastError();
 }
 
+int tsg_scan_blocks(int num_chunks) { return (num_chunks + TSG_SCAN_B - 1) / TSG_SCAN_B; }
+
 hipError_t tsg_launch_combine(const TsgCombineArgs &a, const double *dR, double *Rstart,
-                              const double *r0part, int nb, int64_t total, hipStream_t s) {
+                              double *bsum, const double *r0part, int nb, int64_t total,
+                              hipStream_t s) {
     if (dR != nullptr) {
-        hipLaunchKernelGGL(k_scan_dR, dim3((a.M + 63) / 64), dim3(64), 0, s, dR, Rstart,
-                           a.num_chunks, a.M, r0part, nb);
+        const int nblk = tsg_scan_blocks(a.num_chunks);
+        const dim3 grid(nblk, (a.M + 63) / 64);
+        hipLaunchKernelGGL(k_scan_blocks, grid, dim3(64), 0, s, dR, bsum, a.num_chunks, a.M);
+        hipLaunchKernelGGL(k_scan_top, dim3((a.M + 63) / 64), dim3(64), 0, s, bsum, nblk, a.M,
+                           r0part, nb);
+        hipLaunchKernelGGL(k_scan_apply, grid, dim3(64), 0, s, dR, bsum, Rstart, a.num_chunks,
+                           a.M);
     }
     const int64_t blocks = (total + 255) / 256;
     hipLaunchKernelGGL(k_combine, dim3((unsigned) blocks), dim3(256), 0, s, a, total);

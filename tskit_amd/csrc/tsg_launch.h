@@ -16,8 +16,11 @@ int tsg_tree0_blocks(int nloc);
 // part [tsg_tree0_blocks(nloc) x M]
 hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, const float *t0_bl,
                             int32_t *x0, int nloc, double *part, hipStream_t s);
+// chunk blocks of the running-sum scan (bsum holds blocks x M doubles)
+int tsg_scan_blocks(int num_chunks);
 hipError_t tsg_launch_combine(const TsgCombineArgs &a, const double *dR, double *Rstart,
-                              const double *r0part, int nb, int64_t total, hipStream_t s);
+                              double *bsum, const double *r0part, int nb, int64_t total,
+                              hipStream_t s);
 hipError_t tsg_launch_finals(const void *fin, int nq, const double *raw, const int32_t *ipar,
                              const double *dpar, double *out, int64_t total, hipStream_t s);
 hipError_t tsg_launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <numeric>
 #include <string_view>
+#include <thread>
 
 namespace {
 
@@ -402,10 +403,192 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
     return 0;
 }
 
+namespace {
+
+// What one planner thread produces for a contiguous run of chunks; the
+// parts are concatenated in chunk order afterwards.  Transition offsets
+// (tr_off) are written part-relative into the plan and rebased on merge;
+// tr_nrem, mut_slot and sample_slot entries are disjoint per part and are
+// written in place.
+struct HotPart {
+    std::vector<int32_t> ev_child, path, slot_par, lev_slot0;
+    std::vector<float> ev_bl, slot_bl0;
+    std::vector<int64_t> ev_end;       // per event: end of its path (part-relative)
+    std::vector<int64_t> chunk_slots;  // per chunk
+    std::vector<int64_t> chunk_levs;   // per chunk: level-boundary entries
+    int32_t max_slots = 0, max_path = 0, max_levels = 0;
+    bool too_big = false;
+};
+
+void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_t cend,
+                    HotPart &out) {
+    const int32_t T = ts->num_trees, E = ts->num_edges, N = ts->num_nodes;
+    const int32_t n = ts->num_samples;
+    std::vector<int32_t> gpar(N, TSG_NULL);
+    std::vector<int32_t> hot_stamp(N, -1), hot_idx(N, -1);    // hot in chunk c, index in `hot`
+    std::vector<int32_t> memo_stamp(N, -1), memo_nha(N, -1);  // nearest hot ancestor-or-self
+    std::vector<int32_t> hot, ev_u, path_g, hpar, depth, slot_of, stack, lev_count, next;
+    std::vector<int64_t> ev_p0;
+    std::vector<std::pair<int32_t, int32_t>> undo, redo;     // (child, parent before / after)
+    // replay up to the start tree of chunk cbeg
+    int32_t j = 0, k = 0;
+    for (int32_t t = 0; t <= hp.chunk_tree0[cbeg]; t++) {
+        const double x = ts->breakpoints[t];
+        while (k < E && ts->right[ts->O[k]] == x) gpar[ts->child[ts->O[k++]]] = TSG_NULL;
+        while (j < E && ts->left[ts->I[j]] == x) {
+            gpar[ts->child[ts->I[j]]] = ts->parent[ts->I[j]];
+            j++;
+        }
+    }
+    for (int32_t c = cbeg; c < cend; c++) {
+        hot.clear();
+        ev_u.clear();
+        path_g.clear();
+        ev_p0.assign(1, 0);
+        undo.clear();
+        redo.clear();
+        auto touch = [&](int32_t u) {
+            if (hot_stamp[u] != c) {
+                hot_stamp[u] = c;
+                hot_idx[u] = (int32_t) hot.size();
+                hot.push_back(u);
+            }
+        };
+        auto record_path = [&](int32_t p) {
+            for (int32_t w = p; w != TSG_NULL; w = gpar[w]) {
+                path_g.push_back(w);
+                touch(w);
+            }
+            ev_p0.push_back((int64_t) path_g.size());
+        };
+        const int32_t t0 = hp.chunk_tree0[c], t1 = hp.chunk_tree0[c + 1];
+        const int32_t tlast = std::min(t1, T - 1);  // t1 is the closing transition
+        const int64_t ev_base = (int64_t) out.ev_child.size();
+        for (int32_t t = t0 + 1; t <= tlast; t++) {
+            hp.tr_off[t] = ev_base + (int64_t) ev_u.size();
+            const double x = ts->breakpoints[t];
+            int32_t nrem = 0;
+            while (k < E && ts->right[ts->O[k]] == x) {
+                const int32_t e = ts->O[k];
+                const int32_t u = ts->child[e];
+                ev_u.push_back(u);
+                out.ev_bl.push_back(0.0f);
+                touch(u);
+                undo.emplace_back(u, gpar[u]);
+                gpar[u] = TSG_NULL;
+                redo.emplace_back(u, TSG_NULL);
+                record_path(ts->parent[e]);
+                nrem++;
+                k++;
+            }
+            while (j < E && ts->left[ts->I[j]] == x) {
+                const int32_t e = ts->I[j];
+                const int32_t u = ts->child[e], p = ts->parent[e];
+                ev_u.push_back(u);
+                out.ev_bl.push_back((float) (ts->time[p] - ts->time[u]));
+                touch(u);
+                undo.emplace_back(u, gpar[u]);
+                gpar[u] = p;
+                redo.emplace_back(u, p);
+                record_path(p);
+                j++;
+            }
+            hp.tr_nrem[t] = nrem;
+        }
+        for (int32_t s = ts->tree_site0[t0]; s < ts->tree_site0[t1]; s++) {
+            for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) touch(ts->mut_node[m]);
+        }
+        // back to the start tree
+        for (size_t i = undo.size(); i-- > 0;) gpar[undo[i].first] = undo[i].second;
+
+        const int32_t H = (int32_t) hot.size();
+        if (H >= (int32_t) TSG_NO_SLOT) {  // slot ids are 16-bit in sample_slot
+            out.too_big = true;
+            return;
+        }
+        // hot forest: nearest hot proper ancestor in the start tree
+        hpar.assign(H, -1);
+        for (int32_t i = 0; i < H; i++) {
+            int32_t w = gpar[hot[i]];
+            while (w != TSG_NULL && hot_stamp[w] != c) w = gpar[w];
+            hpar[i] = (w == TSG_NULL) ? -1 : hot_idx[w];
+        }
+        depth.assign(H, -1);
+        int32_t maxd = 0;
+        for (int32_t i = 0; i < H; i++) {
+            stack.clear();
+            int32_t w = i;
+            while (w >= 0 && depth[w] < 0) {
+                stack.push_back(w);
+                w = hpar[w];
+            }
+            int32_t d = (w >= 0) ? depth[w] : -1;
+            for (size_t q = stack.size(); q-- > 0;) depth[stack[q]] = ++d;
+            maxd = std::max(maxd, d);
+        }
+        // slots by decreasing depth (level 0 = deepest), stable within a level
+        const int32_t nlev = (H > 0) ? maxd + 1 : 0;
+        lev_count.assign(nlev + 1, 0);
+        for (int32_t i = 0; i < H; i++) lev_count[maxd - depth[i] + 1]++;
+        for (int32_t l = 0; l < nlev; l++) lev_count[l + 1] += lev_count[l];
+        out.lev_slot0.insert(out.lev_slot0.end(), lev_count.begin(), lev_count.end());
+        out.chunk_levs.push_back(nlev + 1);
+        slot_of.assign(H, -1);
+        next.assign(lev_count.begin(), lev_count.end() - 1);
+        for (int32_t i = 0; i < H; i++) slot_of[i] = next[maxd - depth[i]]++;
+        const size_t s0 = out.slot_par.size();
+        out.slot_par.resize(s0 + H);
+        out.slot_bl0.resize(s0 + H);
+        out.chunk_slots.push_back(H);
+        for (int32_t i = 0; i < H; i++) {
+            const int32_t u = hot[i];
+            out.slot_par[s0 + slot_of[i]] = (hpar[i] < 0) ? -1 : slot_of[hpar[i]];
+            out.slot_bl0[s0 + slot_of[i]] =
+                (gpar[u] == TSG_NULL) ? 0.0f : (float) (ts->time[gpar[u]] - ts->time[u]);
+        }
+        for (int32_t u : ev_u) out.ev_child.push_back(slot_of[hot_idx[u]]);
+        const int64_t pbase = (int64_t) out.path.size();
+        for (size_t e = 1; e < ev_p0.size(); e++) {
+            out.ev_end.push_back(pbase + ev_p0[e]);
+            out.max_path = std::max(out.max_path, (int32_t) (ev_p0[e] - ev_p0[e - 1]));
+        }
+        for (int32_t w : path_g) out.path.push_back(slot_of[hot_idx[w]]);
+        for (int32_t s = ts->tree_site0[t0]; s < ts->tree_site0[t1]; s++) {
+            for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) {
+                hp.mut_slot[m] = slot_of[hot_idx[ts->mut_node[m]]];
+            }
+        }
+        // each sample's nearest hot ancestor-or-self, memoised along the walk
+        uint16_t *ss = hp.sample_slot.data() + (size_t) c * n;
+        for (int32_t s = 0; s < n; s++) {
+            stack.clear();
+            int32_t w = ts->samples[s];
+            while (w != TSG_NULL && hot_stamp[w] != c && memo_stamp[w] != c) {
+                stack.push_back(w);
+                w = gpar[w];
+            }
+            const int32_t res = (w == TSG_NULL) ? TSG_NULL : (hot_stamp[w] == c ? w : memo_nha[w]);
+            for (int32_t v : stack) {
+                memo_stamp[v] = c;
+                memo_nha[v] = res;
+            }
+            ss[s] = (res == TSG_NULL) ? (uint16_t) TSG_NO_SLOT : (uint16_t) slot_of[hot_idx[res]];
+        }
+        out.max_slots = std::max(out.max_slots, H);
+        out.max_levels = std::max(out.max_levels, nlev);
+        // forward to the next chunk's start tree
+        for (const auto &r : redo) gpar[r.first] = r.second;
+    }
+}
+
+}  // namespace
+
 // Builds the hot plan (tsg_hotplan) over the same kind of chunks.  Each chunk
 // is planned in three steps on the live tree: record its events and root
 // paths while applying them, undo them to return to the chunk's start tree
-// and derive the slot structure there, then apply them again.
+// and derive the slot structure there, then apply them again.  Runs of
+// chunks are planned by separate threads, each replaying the edges up to its
+// first chunk.
 int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
     tsg_hotplan &hp = ts->hot;
     hp = tsg_hotplan();
@@ -416,28 +599,15 @@ int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
     hp.num_chunks = C;
     hp.tr_off.assign(T + 1, 0);
     hp.tr_nrem.assign(T, 0);
-    hp.ev_path0.assign(1, 0);
-    hp.chunk_slot0.assign(C + 1, 0);
-    hp.chunk_lev0.assign(C + 1, 0);
     hp.mut_slot.assign(ts->num_mutations, -1);
     hp.sample_slot.assign((size_t) C * n, (uint16_t) TSG_NO_SLOT);
 
-    std::vector<int32_t> gpar(N, TSG_NULL);
-    std::vector<int32_t> hot_stamp(N, -1), hot_idx(N, -1);   // hot in chunk c, index in `hot`
-    std::vector<int32_t> memo_stamp(N, -1), memo_nha(N, -1); // nearest hot ancestor-or-self
-    std::vector<int32_t> hot;                                // global ids, first-touch order
-    std::vector<int32_t> ev_u, path_g;                       // events (global ids)
-    std::vector<int64_t> ev_p0;
-    std::vector<std::pair<int32_t, int32_t>> undo, redo;     // (child, parent before / after)
-    std::vector<int32_t> hpar, depth, slot_of, stack;
-    int32_t j = 0, k = 0;
-
-    // opening transition, then the first tree's snapshot (samples first)
-    while (j < E && ts->left[ts->I[j]] == ts->breakpoints[0]) {
-        gpar[ts->child[ts->I[j]]] = ts->parent[ts->I[j]];
-        j++;
-    }
+    // the first tree, samples first
     {
+        std::vector<int32_t> gpar(N, TSG_NULL);
+        for (int32_t j = 0; j < E && ts->left[ts->I[j]] == ts->breakpoints[0]; j++) {
+            gpar[ts->child[ts->I[j]]] = ts->parent[ts->I[j]];
+        }
         std::vector<int32_t> lid(N, -1), order;
         for (int32_t s = 0; s < n; s++) {
             lid[ts->samples[s]] = (int32_t) order.size();
@@ -463,146 +633,52 @@ int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
         }
     }
 
-    for (int32_t c = 0; c < C; c++) {
-        hot.clear();
-        ev_u.clear();
-        path_g.clear();
-        ev_p0.assign(1, 0);
-        undo.clear();
-        redo.clear();
-        auto touch = [&](int32_t u) {
-            if (hot_stamp[u] != c) {
-                hot_stamp[u] = c;
-                hot_idx[u] = (int32_t) hot.size();
-                hot.push_back(u);
-            }
-        };
-        auto record_path = [&](int32_t p) {
-            for (int32_t w = p; w != TSG_NULL; w = gpar[w]) {
-                path_g.push_back(w);
-                touch(w);
-            }
-            ev_p0.push_back((int64_t) path_g.size());
-        };
-        const int32_t t0 = hp.chunk_tree0[c], t1 = hp.chunk_tree0[c + 1];
-        const int32_t tlast = std::min(t1, T - 1);  // t1 is the closing transition
+    // runs of chunks per thread: enough work per thread to amortise its
+    // replay of the edges before its first chunk
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int32_t P = (int32_t) std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t) std::min(hw, 16u), (int64_t) C, (int64_t) C / 16 + 1}));
+    std::vector<int32_t> cb(P + 1);
+    for (int32_t p = 0; p <= P; p++) cb[p] = (int32_t) ((int64_t) C * p / P);
+    std::vector<HotPart> parts(P);
+    {
+        std::vector<std::thread> pool;
+        for (int32_t p = 1; p < P; p++) {
+            pool.emplace_back(plan_hot_range, ts, std::ref(hp), cb[p], cb[p + 1], std::ref(parts[p]));
+        }
+        plan_hot_range(ts, hp, cb[0], cb[1], parts[0]);
+        for (auto &th : pool) th.join();
+    }
+    for (const HotPart &part : parts) {
+        if (part.too_big) return 0;  // not representable: hp.usable stays false
+    }
+    // merge in chunk order
+    hp.chunk_slot0.assign(C + 1, 0);
+    hp.chunk_lev0.assign(C + 1, 0);
+    hp.ev_path0.assign(1, 0);
+    for (int32_t p = 0; p < P; p++) {
+        const HotPart &part = parts[p];
         const int64_t ev_base = (int64_t) hp.ev_child.size();
-        for (int32_t t = t0 + 1; t <= tlast; t++) {
-            hp.tr_off[t] = ev_base + (int64_t) ev_u.size();
-            const double x = ts->breakpoints[t];
-            int32_t nrem = 0;
-            while (k < E && ts->right[ts->O[k]] == x) {
-                const int32_t e = ts->O[k];
-                const int32_t u = ts->child[e];
-                ev_u.push_back(u);
-                hp.ev_bl.push_back(0.0f);
-                touch(u);
-                undo.emplace_back(u, gpar[u]);
-                gpar[u] = TSG_NULL;
-                redo.emplace_back(u, TSG_NULL);
-                record_path(ts->parent[e]);
-                nrem++;
-                k++;
-            }
-            while (j < E && ts->left[ts->I[j]] == x) {
-                const int32_t e = ts->I[j];
-                const int32_t u = ts->child[e], p = ts->parent[e];
-                ev_u.push_back(u);
-                hp.ev_bl.push_back((float) (ts->time[p] - ts->time[u]));
-                touch(u);
-                undo.emplace_back(u, gpar[u]);
-                gpar[u] = p;
-                redo.emplace_back(u, p);
-                record_path(p);
-                j++;
-            }
-            hp.tr_nrem[t] = nrem;
+        const int64_t path_base = (int64_t) hp.path.size();
+        const int32_t tfirst = hp.chunk_tree0[cb[p]] + 1;
+        const int32_t tlast = std::min(hp.chunk_tree0[cb[p + 1]], T - 1);
+        for (int32_t t = tfirst; t <= tlast; t++) hp.tr_off[t] += ev_base;
+        hp.ev_child.insert(hp.ev_child.end(), part.ev_child.begin(), part.ev_child.end());
+        hp.ev_bl.insert(hp.ev_bl.end(), part.ev_bl.begin(), part.ev_bl.end());
+        for (int64_t e : part.ev_end) hp.ev_path0.push_back(path_base + e);
+        hp.path.insert(hp.path.end(), part.path.begin(), part.path.end());
+        hp.slot_par.insert(hp.slot_par.end(), part.slot_par.begin(), part.slot_par.end());
+        hp.slot_bl0.insert(hp.slot_bl0.end(), part.slot_bl0.begin(), part.slot_bl0.end());
+        hp.lev_slot0.insert(hp.lev_slot0.end(), part.lev_slot0.begin(), part.lev_slot0.end());
+        for (int32_t c = cb[p]; c < cb[p + 1]; c++) {
+            hp.chunk_slot0[c + 1] = hp.chunk_slot0[c] + part.chunk_slots[c - cb[p]];
+            hp.chunk_lev0[c + 1] = hp.chunk_lev0[c] + part.chunk_levs[c - cb[p]];
         }
-        for (int32_t s = ts->tree_site0[t0]; s < ts->tree_site0[t1]; s++) {
-            for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) touch(ts->mut_node[m]);
-        }
-        // back to the start tree
-        for (size_t i = undo.size(); i-- > 0;) gpar[undo[i].first] = undo[i].second;
-
-        const int32_t H = (int32_t) hot.size();
-        if (H >= (int32_t) TSG_NO_SLOT) return 0;  // not representable: hp.usable stays false
-        // hot forest: nearest hot proper ancestor in the start tree
-        hpar.assign(H, -1);
-        for (int32_t i = 0; i < H; i++) {
-            int32_t w = gpar[hot[i]];
-            while (w != TSG_NULL && hot_stamp[w] != c) w = gpar[w];
-            hpar[i] = (w == TSG_NULL) ? -1 : hot_idx[w];
-        }
-        depth.assign(H, -1);
-        int32_t maxd = 0;
-        for (int32_t i = 0; i < H; i++) {
-            stack.clear();
-            int32_t w = i;
-            while (w >= 0 && depth[w] < 0) {
-                stack.push_back(w);
-                w = hpar[w];
-            }
-            int32_t d = (w >= 0) ? depth[w] : -1;
-            for (size_t q = stack.size(); q-- > 0;) depth[stack[q]] = ++d;
-            maxd = std::max(maxd, d);
-        }
-        // slots by decreasing depth (level 0 = deepest), stable within a level
-        const int32_t nlev = (H > 0) ? maxd + 1 : 0;
-        std::vector<int32_t> lev_count(nlev + 1, 0);
-        for (int32_t i = 0; i < H; i++) lev_count[maxd - depth[i] + 1]++;
-        for (int32_t l = 0; l < nlev; l++) lev_count[l + 1] += lev_count[l];
-        hp.chunk_lev0[c] = (int64_t) hp.lev_slot0.size();
-        hp.lev_slot0.insert(hp.lev_slot0.end(), lev_count.begin(), lev_count.end());
-        slot_of.assign(H, -1);
-        {
-            std::vector<int32_t> next(lev_count.begin(), lev_count.end() - 1);
-            for (int32_t i = 0; i < H; i++) slot_of[i] = next[maxd - depth[i]]++;
-        }
-        const int64_t s0 = hp.chunk_slot0[c];
-        hp.chunk_slot0[c + 1] = s0 + H;
-        hp.slot_par.resize((size_t) s0 + H);
-        hp.slot_bl0.resize((size_t) s0 + H);
-        for (int32_t i = 0; i < H; i++) {
-            const int32_t u = hot[i];
-            hp.slot_par[s0 + slot_of[i]] = (hpar[i] < 0) ? -1 : slot_of[hpar[i]];
-            hp.slot_bl0[s0 + slot_of[i]] =
-                (gpar[u] == TSG_NULL) ? 0.0f : (float) (ts->time[gpar[u]] - ts->time[u]);
-        }
-        for (int32_t u : ev_u) hp.ev_child.push_back(slot_of[hot_idx[u]]);
-        const int64_t pbase = (int64_t) hp.path.size();
-        for (size_t e = 1; e < ev_p0.size(); e++) hp.ev_path0.push_back(pbase + ev_p0[e]);
-        for (int32_t w : path_g) hp.path.push_back(slot_of[hot_idx[w]]);
-        for (size_t e = 0; e + 1 < ev_p0.size(); e++) {
-            hp.max_path = std::max(hp.max_path, (int32_t) (ev_p0[e + 1] - ev_p0[e]));
-        }
-        for (int32_t s = ts->tree_site0[t0]; s < ts->tree_site0[t1]; s++) {
-            for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) {
-                hp.mut_slot[m] = slot_of[hot_idx[ts->mut_node[m]]];
-            }
-        }
-        // each sample's nearest hot ancestor-or-self, memoised along the walk
-        uint16_t *ss = hp.sample_slot.data() + (size_t) c * n;
-        for (int32_t s = 0; s < n; s++) {
-            stack.clear();
-            int32_t w = ts->samples[s];
-            while (w != TSG_NULL && hot_stamp[w] != c && memo_stamp[w] != c) {
-                stack.push_back(w);
-                w = gpar[w];
-            }
-            const int32_t res = (w == TSG_NULL) ? TSG_NULL : (hot_stamp[w] == c ? w : memo_nha[w]);
-            for (int32_t v : stack) {
-                memo_stamp[v] = c;
-                memo_nha[v] = res;
-            }
-            ss[s] = (res == TSG_NULL) ? (uint16_t) TSG_NO_SLOT : (uint16_t) slot_of[hot_idx[res]];
-        }
-        hp.max_slots = std::max(hp.max_slots, H);
-        hp.max_levels = std::max(hp.max_levels, nlev);
-        // forward to the next chunk's start tree
-        for (const auto &r : redo) gpar[r.first] = r.second;
+        hp.max_slots = std::max(hp.max_slots, part.max_slots);
+        hp.max_path = std::max(hp.max_path, part.max_path);
+        hp.max_levels = std::max(hp.max_levels, part.max_levels);
     }
     hp.tr_off[T] = (int64_t) hp.ev_child.size();
-    hp.chunk_lev0[C] = (int64_t) hp.lev_slot0.size();
     hp.usable = true;
     return 0;
 }
