@@ -749,15 +749,17 @@ int default_num_chunks(const tsg_treeseq *ts) {
 }
 
 int default_hot_chunks(const tsg_treeseq *ts) {
-    // About 1024 replayed edge events per chunk (so about as many hot slots:
-    // ~40 KiB of LDS at 8 sample sets, three or four chunks per CU), but at
-    // least 1024 chunks of >= 64 events to fill 256 CUs; each chunk stores
-    // one 16-bit slot per sample, capped near 2 GiB in total.
+    // About 256 replayed edge events per chunk: the sweep is a serial chain
+    // per chunk, so more, shorter chunks run faster as long as their hot
+    // slots stay small (a few hundred, ~25 KiB of LDS at 8 sample sets, five
+    // or six chunks per CU) -- but at least 1024 chunks of >= 64 events to
+    // fill 256 CUs.  Each chunk stores one 16-bit slot per sample: the total
+    // is capped near 4 GiB.
     const int64_t events = 2 * (int64_t) ts->num_edges;
-    int64_t c = std::max<int64_t>(events / 1024, std::min<int64_t>(1024, events / 64));
+    int64_t c = std::max<int64_t>(events / 256, std::min<int64_t>(1024, events / 64));
     c = std::max<int64_t>(c, 1);
     const int64_t per_chunk = 2 * (int64_t) std::max(1, ts->num_samples);
-    c = std::min<int64_t>(c, std::max<int64_t>(1, ((int64_t) 2 << 30) / per_chunk));
+    c = std::min<int64_t>(c, std::max<int64_t>(1, ((int64_t) 4 << 30) / per_chunk));
     c = std::min<int64_t>(c, std::max(1, ts->num_trees));
     return (int) c;
 }

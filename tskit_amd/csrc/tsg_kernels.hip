@@ -41,34 +41,26 @@
 
 namespace {
 
-__device__ __forceinline__ void tsum(float &s, float &c, float x) {
-    const float t = s + x;
-    const float bp = t - s;
-    c += (s - (t - bp)) + (x - bp);
-    s = t;
-}
-
 // Reference summary function o.stat evaluated on counts x (or on the
-// complement tot - x for the unpolarised wrapper).  Counts are converted to
-// FP64 first: every product of up to three counts below 2^17.6 (n <= ~2e5
-// samples per set) is exact in FP64, so the numerators are the exact integer
-// polynomials, scaled once by 1/denominator and rounded to FP32 -- the same
-// values an int64 evaluation gives, at FP64 rather than 64-bit integer
-// multiply cost.  (f2 / f3 / f4 multiply two exact factors, one rounding.)
-__device__ float eval_summary(const TsgOut &o, const int32_t *x, const int32_t *tot, bool comp) {
+// complement tot - x for the unpolarised wrapper), in FP64 as the reference
+// does.  Counts are converted to FP64 first: every product of up to three
+// counts below 2^17.6 (n <= ~2e5 samples per set) is exact in FP64, so the
+// numerators are the exact integer polynomials, scaled once by
+// 1/denominator.  (f2 / f3 / f4 multiply two exact factors, one rounding.)
+__device__ double eval_summary(const TsgOut &o, const int32_t *x, const int32_t *tot, bool comp) {
 #define XV(col) ((double) (comp ? (tot[col] - x[col]) : x[col]))
     const double xi = XV(o.i);
     const double ni = (double) o.ni;
     switch (o.stat) {
     case TSG_STAT_DIVERSITY:
-        return (float) ((xi * (ni - xi)) * o.inv);
+        return ((xi * (ni - xi)) * o.inv);
     case TSG_STAT_SEGREGATING_SITES:
-        return (xi > 0) ? (float) ((ni - xi) * o.inv) : 0.0f;
+        return (xi > 0) ? (ni - xi) * o.inv : 0.0;
     case TSG_STAT_Y1:
-        return (float) ((xi * (ni - xi) * (ni - xi - 1)) * o.inv);
+        return ((xi * (ni - xi) * (ni - xi - 1)) * o.inv);
     case TSG_STAT_DIVERGENCE: {
         const double xj = XV(o.j);
-        return (float) ((xi * ((double) o.nj - xj)) * o.inv);
+        return ((xi * ((double) o.nj - xj)) * o.inv);
     }
     case TSG_STAT_RELATEDNESS: {
         double sum = 0;
@@ -77,41 +69,41 @@ __device__ float eval_summary(const TsgOut &o, const int32_t *x, const int32_t *
         }
         const double mean = sum / (double) o.ccount;
         const double xj = XV(o.j);
-        return (float) ((xi / ni - mean) * (xj / (double) o.nj - mean));
+        return ((xi / ni - mean) * (xj / (double) o.nj - mean));
     }
     case TSG_STAT_RELATEDNESS_NONCENTRED: {
         const double xj = XV(o.j);
-        return (float) ((xi * xj) * o.inv);
+        return ((xi * xj) * o.inv);
     }
     case TSG_STAT_Y2: {
         const double xj = XV(o.j), nj = (double) o.nj;
-        return (float) ((xi * (nj - xj) * (nj - xj - 1)) * o.inv);
+        return ((xi * (nj - xj) * (nj - xj - 1)) * o.inv);
     }
     case TSG_STAT_F2: {
         const double xj = XV(o.j), nj = (double) o.nj;
         const double a = xi * (nj - xj);
         const double b = (xi - 1) * (nj - xj - 1) - (ni - xi) * xj;
-        return (float) (a * b * o.inv);
+        return (a * b * o.inv);
     }
     case TSG_STAT_Y3: {
         const double xj = XV(o.j), xk = XV(o.k);
-        return (float) ((xi * ((double) o.nj - xj) * ((double) o.nk - xk)) * o.inv);
+        return ((xi * ((double) o.nj - xj) * ((double) o.nk - xk)) * o.inv);
     }
     case TSG_STAT_F3: {
         const double xj = XV(o.j), xk = XV(o.k), nk = (double) o.nk;
         const double a = xi * ((double) o.nj - xj);
         const double b = (xi - 1) * (nk - xk) - (ni - xi) * xk;
-        return (float) (a * b * o.inv);
+        return (a * b * o.inv);
     }
     case TSG_STAT_F4: {
         const double xj = XV(o.j), xk = XV(o.k), xl = XV(o.l);
         const double nk = (double) o.nk, nl = (double) o.nl;
         const double a = xi * ((double) o.nj - xj);
         const double b = xk * (nl - xl) - xl * (nk - xk);
-        return (float) (a * b * o.inv);
+        return (a * b * o.inv);
     }
     default:
-        return 0.0f;
+        return 0.0;
     }
 #undef XV
 }
@@ -204,8 +196,13 @@ struct LaneOut {
     int64_t slot0;  // partial index of window wfirst
     int Mq;
     uint64_t mask;  // state columns the summary reads
-    float f0;       // its value on an all-zero state (unpolarised wrapper included)
-    float Rs, Rc, As, Ac;
+    double f0;      // its value on an all-zero state (unpolarised wrapper included)
+    double R, A;    // running branch sum, window sum (FP64)
+    // degree-2 summaries, F(x) = c0 + c1a x_a + c1b x_b + c2aa x_a^2 +
+    // c2bb x_b^2 + c2ab x_a x_b (unpolarised wrapper included); moment < 0:
+    // the summary is evaluated hop by hop instead
+    int moment, ca, cb;
+    double c1a, c1b, c2aa, c2bb, c2ab;
 };
 
 // State columns a summary function reads; all ones when it cannot be told
@@ -242,6 +239,58 @@ __device__ uint64_t summary_columns(const TsgOut &o, int K) {
     return m;
 }
 
+// Coefficients of the degree-2 summaries (diversity trees.c:3935-3954,
+// divergence 4691-4721, non-centred relatedness 4756-4790), unpolarised
+// wrapper f(x) + f(total - x) folded in.  Returns false for the others.
+__device__ bool moment_form(const TsgOut &o, int K, LaneOut &L) {
+    if (K > 64 || !isfinite(o.inv)) return false;
+    const bool pol = (o.flags & 1) != 0;
+    const double inv = o.inv, ni = (double) o.ni, nj = (double) o.nj;
+    double c1a = 0, c1b = 0, c2aa = 0, c2bb = 0, c2ab = 0;
+    switch (o.stat) {
+    case TSG_STAT_DIVERSITY:  // x (n - x) inv; f(n - x) = f(x)
+        c1a = ni * inv;
+        c2aa = -inv;
+        if (!pol) {
+            c1a *= 2;
+            c2aa *= 2;
+        }
+        break;
+    case TSG_STAT_DIVERGENCE:  // x_i (n_j - x_j) inv [+ (n_i - x_i) x_j inv]
+        c1a = nj * inv;
+        c2ab = -inv;
+        if (!pol) {
+            c1b = ni * inv;
+            c2ab = -2 * inv;
+        }
+        break;
+    case TSG_STAT_RELATEDNESS_NONCENTRED:  // x_i x_j inv [+ (n_i - x_i)(n_j - x_j) inv]
+        c2ab = inv;
+        if (!pol) {
+            c1a = -nj * inv;
+            c1b = -ni * inv;
+            c2ab = 2 * inv;
+        }
+        break;
+    default:
+        return false;
+    }
+    int ca = o.i, cb = (o.stat == TSG_STAT_DIVERSITY) ? o.i : o.j;
+    if (ca == cb) {  // one column: x_a x_b = x_a^2
+        c1a += c1b;
+        c2aa += c2bb + c2ab;
+        c1b = c2bb = c2ab = 0;
+    }
+    L.ca = ca;
+    L.cb = cb;
+    L.c1a = c1a;
+    L.c1b = c1b;
+    L.c2aa = c2aa;
+    L.c2bb = c2bb;
+    L.c2ab = c2ab;
+    return true;
+}
+
 template <int MPL>
 __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, int lane,
                               LaneOut (&lo)[MPL]) {
@@ -250,10 +299,11 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, 
         LaneOut &L = lo[j];
         const int m = a.m_begin + lane + WAVE * j;
         L.lm = -1;
-        L.Rs = L.Rc = L.As = L.Ac = 0.0f;
+        L.R = L.A = 0.0;
         L.wi = L.wend = 0;
         L.mask = 0;
-        L.f0 = 0.0f;
+        L.f0 = 0.0;
+        L.moment = -1;
         if (m < a.m_end) {
             L.lm = m - a.m_begin;
             const TsgOut &o = sout[L.lm];
@@ -265,19 +315,20 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, 
             L.Mq = q.M;
             L.slot0 = q.pbase + a.cq_poff[cq] * (int64_t) q.M + (m - q.mbase);
             L.mask = summary_columns(o, a.K);
+            if (moment_form(o, a.K, L)) L.moment = 1;
         }
     }
 }
 
 __device__ __forceinline__ void flush_window(const TsgSweepArgs &a, LaneOut &L, int wfirst) {
-    a.partial[L.slot0 + (int64_t) (L.wi - wfirst) * L.Mq] = (double) L.As + (double) L.Ac;
-    L.As = L.Ac = 0.0f;
+    a.partial[L.slot0 + (int64_t) (L.wi - wfirst) * L.Mq] = L.A;
+    L.A = 0.0;
 }
 
 // Unpolarised summaries add f(total - x) (trees.c:1944-1972).
-__device__ __forceinline__ float branch_summary(const TsgOut &o, const int32_t *x,
-                                                const int32_t *tot) {
-    float f = eval_summary(o, x, tot, false);
+__device__ __forceinline__ double branch_summary(const TsgOut &o, const int32_t *x,
+                                                 const int32_t *tot) {
+    double f = eval_summary(o, x, tot, false);
     if (!(o.flags & 1)) f = f + eval_summary(o, x, tot, true);
     return f;
 }
@@ -301,11 +352,17 @@ __device__ __forceinline__ float branch_summary(const TsgOut &o, const int32_t *
 // Branch mode, per event: the columns the moved subtree carries (a ballot)
 // decide which outputs can change.  An output none of whose columns moved
 // has F(old) == F(new) at every hop, so its hop terms cancel exactly and are
-// not evaluated, and its child term is its all-zero value f0.  The active
-// outputs' (output, hop) terms of a path segment are spread over the lanes
-// (hop fastest, so lanes of one round mostly share an output), written to
-// LDS, and each output's owner lane adds them in hop order -- the order of
-// the reference's walk (trees.c:1436-1448, 1461-1472).
+// not evaluated, and its child term is its all-zero value f0.  For the
+// others the walk's contribution sum_h bl_h (F(x_h + d) - F(x_h)) (d = the
+// moved counts, signed; trees.c:1436-1448, 1461-1472) is formed two ways:
+//   * degree-2 summaries (diversity, divergence, non-centred relatedness):
+//     F(x + d) - F(x) is linear in x, so the sum is a closed form in the
+//     path moments S0 = sum_h bl_h and S1[k] = sum_h bl_h x_h[k], which the
+//     count-update lanes accumulate;
+//   * the others: (output, hop) terms of a path segment are spread over the
+//     lanes (hop fastest), written to LDS, and each output's owner lane adds
+//     them in hop order.
+// Running sums and window sums are FP64, as in the reference.
 template <int MODE, int STORE, int MPL>
 __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     extern __shared__ int32_t smem[];
@@ -321,7 +378,9 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     double *tbp = reinterpret_cast<double *>(sout + span);              // [TB+1] breakpoints
     int64_t *toff = reinterpret_cast<int64_t *>(tbp + TSG_TB + 1);       // [TB+1] tr_off
     int64_t *evp = toff + TSG_TB + 1;                                    // [EB+1] ev_path0
-    int32_t *tnrem = reinterpret_cast<int32_t *>(evp + TSG_EB + 1);      // [TB]
+    double *s1v = reinterpret_cast<double *>(evp + TSG_EB + 1);          // [64] path moments
+    double *td = s1v + 64;                                               // [WAVE] hop terms
+    int32_t *tnrem = reinterpret_cast<int32_t *>(td + WAVE);             // [TB]
     int32_t *ts0 = tnrem + TSG_TB;                                       // [TB+1] tree_site0
     int32_t *evc = ts0 + TSG_TB + 1;                                     // [EB] event child
     float *evb = reinterpret_cast<float *>(evc + TSG_EB);                // [EB] event branch length
@@ -333,7 +392,6 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     int32_t *tot = reinterpret_cast<int32_t *>(sbl + TSG_PSEG);          // [K]
     int32_t *zero = tot + K;                                             // [K]
     int32_t *act = zero + K;                                             // [span]
-    float *td = reinterpret_cast<float *>(act + span);                   // [2 x WAVE]
     Nodes nd;
     nd.R = R;
     if (STORE != TSG_STORE_GLOBAL) {
@@ -358,11 +416,14 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     LaneOut lo[MPL];
     setup_outputs<MPL>(a, sout, c, lane, lo);
     int wfirst[MPL];
+    bool any_moment = false;
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
         wfirst[j] = lo[j].wi;
         if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(sout[lo[j].lm], zero, tot);
+        any_moment |= lo[j].moment > 0;
     }
+    any_moment = MODE == 0 && __any(any_moment);
 
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
     // the chunk's events: transitions t0+1 .. t1 (the last is the closing
@@ -451,15 +512,15 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             // the edge's own term bl * F(x_child): needed even for a zero-weight
             // subtree, because unpolarised summaries need not vanish at zero
             // (non-centred relatedness: F(0) = f(0) + f(total) = 1)
+            const double sg = ins ? 1.0 : -1.0;
             if (MODE == 0) {
-                const float b = ins ? ebl : nd.bl(u);
-                const float sg = ins ? 1.0f : -1.0f;
+                const double b = sg * (double) (ins ? ebl : nd.bl(u));
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
                     if (lo[j].lm >= 0) {
-                        const float f = (lo[j].mask & cm) ? branch_summary(sout[lo[j].lm], xc, tot)
-                                                          : lo[j].f0;
-                        tsum(lo[j].Rs, lo[j].Rc, sg * __fmul_rn(b, f));
+                        const double f = (lo[j].mask & cm) ? branch_summary(sout[lo[j].lm], xc, tot)
+                                                           : lo[j].f0;
+                        lo[j].R += b * f;
                     }
                 }
             }
@@ -473,7 +534,8 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             int na = 0;
 #pragma unroll
             for (int j = 0; j < MPL; j++) {
-                const bool on = (MODE == 0) && lo[j].lm >= 0 && (lo[j].mask & cm) != 0;
+                const bool on = (MODE == 0) && lo[j].lm >= 0 && lo[j].moment < 0 &&
+                                (lo[j].mask & cm) != 0;
                 const uint64_t bal = __ballot(on);
                 pos[j] = on ? na + __popcll(bal & below) : -1;
                 if (on) act[pos[j]] = lo[j].lm;
@@ -482,6 +544,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             const int64_t p0 = evp[le], p1 = evp[le + 1];
             const int32_t *pth = gpath ? a.path + p0 : pbuf + (p0 - pb0);
             __syncthreads();
+            double s0 = 0, s1 = 0;  // path moments (lane k < K: column k)
             for (int64_t sg0 = 0; sg0 < p1 - p0; sg0 += TSG_PSEG) {
                 const int h = (int) min((int64_t) TSG_PSEG, p1 - p0 - sg0);
                 for (int idx = lane; idx < h * K; idx += WAVE) {
@@ -499,6 +562,13 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                     for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(pth[sg0 + i]);
                 }
                 __syncthreads();
+                if (any_moment) {
+                    for (int i = 0; i < h; i++) {
+                        const double bw = (double) sbl[i];
+                        s0 += bw;
+                        if (lane < K) s1 += bw * (double) xo[i * K + lane];
+                    }
+                }
                 if (MODE == 0 && na > 0) {
                     const int items = na * h;
                     for (int r0 = 0; r0 < items; r0 += WAVE) {
@@ -506,25 +576,38 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                         if (idx < items) {
                             const int ai = idx / h, i = idx - ai * h;
                             const TsgOut &o = sout[act[ai]];
-                            const float bw = sbl[i];
-                            const float fo = branch_summary(o, xo + i * K, tot);
-                            const float fn = branch_summary(o, xn + i * K, tot);
-                            td[2 * lane] = -__fmul_rn(bw, fo);
-                            td[2 * lane + 1] = __fmul_rn(bw, fn);
+                            const double fo = branch_summary(o, xo + i * K, tot);
+                            const double fn = branch_summary(o, xn + i * K, tot);
+                            td[lane] = (double) sbl[i] * (fn - fo);
                         }
                         __syncthreads();
 #pragma unroll
                         for (int j = 0; j < MPL; j++) {
                             if (pos[j] < 0) continue;
                             const int q0 = max(pos[j] * h, r0), q1 = min(pos[j] * h + h, r0 + WAVE);
-                            for (int q = q0; q < q1; q++) {
-                                tsum(lo[j].Rs, lo[j].Rc, td[2 * (q - r0)]);
-                                tsum(lo[j].Rs, lo[j].Rc, td[2 * (q - r0) + 1]);
-                            }
+                            for (int q = q0; q < q1; q++) lo[j].R += td[q - r0];
                         }
                         __syncthreads();
                     }
                 }
+            }
+            if (any_moment) {
+                if (lane < K) s1v[lane] = s1;
+                __syncthreads();
+                // sum_h bl_h (F(x_h + d) - F(x_h)) for F = c1.x + x'C2x:
+                // S0 (c1.d + d'C2d) + sum over columns of S1 * (gradient of d'C2x)
+#pragma unroll
+                for (int j = 0; j < MPL; j++) {
+                    LaneOut &L = lo[j];
+                    if (L.moment < 0 || !(L.mask & cm)) continue;
+                    const double da = sg * (double) xc[L.ca], db = sg * (double) xc[L.cb];
+                    const double t0 = L.c1a * da + L.c1b * db + L.c2aa * da * da +
+                                      L.c2bb * db * db + L.c2ab * da * db;
+                    const double ta = 2 * L.c2aa * da + L.c2ab * db;
+                    const double tb = 2 * L.c2bb * db + L.c2ab * da;
+                    L.R += s0 * t0 + s1v[L.ca] * ta + s1v[L.cb] * tb;
+                }
+                __syncthreads();
             }
         }
     };
@@ -541,9 +624,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 while (L.wi < L.wend) {
                     const double wl = L.win[L.wi], wr = L.win[L.wi + 1];
                     if (!(wl < tr)) break;
-                    const float ov = (float) (fmin(tr, wr) - fmax(tl, wl));
-                    tsum(L.As, L.Ac, __fmul_rn(L.Rs, ov));
-                    tsum(L.As, L.Ac, __fmul_rn(L.Rc, ov));
+                    L.A += L.R * (fmin(tr, wr) - fmax(tl, wl));
                     if (wr <= tr) {
                         flush_window(a, L, wfirst[j]);
                         L.wi++;
@@ -567,9 +648,9 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 }
                 const int na = a.site_nalleles[s];
                 const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
-                float val[MPL];
+                double val[MPL];
 #pragma unroll
-                for (int j = 0; j < MPL; j++) val[j] = 0.0f;
+                for (int j = 0; j < MPL; j++) val[j] = 0.0;
                 // polarised outputs skip the ancestral allele (trees.c:1639)
                 for (int al = any_unpolarised ? 0 : 1; al < na; al++) {
                     for (int k = lane; k < K; k += WAVE) {
@@ -592,7 +673,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 }
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
-                    if (lo[j].lm >= 0) tsum(lo[j].As, lo[j].Ac, val[j]);
+                    if (lo[j].lm >= 0) lo[j].A += val[j];
                 }
             }
         }
@@ -602,7 +683,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
 #pragma unroll
         for (int j = 0; j < MPL; j++) {
             if (lo[j].lm >= 0) {
-                a.dR[(int64_t) c * a.M + a.m_begin + lo[j].lm] = (double) lo[j].Rs + (double) lo[j].Rc;
+                a.dR[(int64_t) c * a.M + a.m_begin + lo[j].lm] = lo[j].R;
             }
         }
     }
@@ -854,13 +935,12 @@ __global__ void __launch_bounds__(WAVE) k_afs_branch(const TsgSweepArgs a) {
 // parallel instead of by replaying its edge insertions one at a time:
 //   R(tree 0) = sum over nodes u with a parent of bl(u) * F(x(u)),
 // which is what the reference's insertion loop accumulates (trees.c:1426-
-// 1480, update_running_sum 1339-1350).  Each term is the FP32 product the
-// sweep adds for the same edge; the sum is FP64, per block of nodes, in a
-// fixed order (deterministic).
+// 1480, update_running_sum 1339-1350), in FP64, per block of nodes, in a
+// fixed order (deterministic).  The tree is given samples first (local id
+// j = sample j): t0_parent, t0_bl.
 //
 // Counts: one thread per weighted sample walks its root path in the
 // snapshot, adding 1 to each of its columns.
-// The tree is given samples first (local id j = sample j): t0_parent, t0_bl.
 __global__ void k_tree0_counts(const TsgSweepArgs a, const int32_t *t0_parent, int32_t *x0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nseed) return;
@@ -884,8 +964,7 @@ __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const 
     double acc = 0;
     for (int u = u0; u < u1; u++) {
         if (t0_parent[u] < 0) continue;
-        const float f = branch_summary(o, x0 + (int64_t) u * a.K, a.col_total);
-        acc += (double) __fmul_rn(t0_bl[u], f);
+        acc += (double) t0_bl[u] * branch_summary(o, x0 + (int64_t) u * a.K, a.col_total);
     }
     part[(int64_t) blockIdx.x * a.M + m] = acc;
 }
@@ -894,8 +973,7 @@ __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const 
 // Combine: per (query, window, output), sum the chunk partials of the chunks
 // that overlap the window, re-base branch sums by the chunk start value, and
 // span-normalise.  FP64, deterministic chunk order.
-// Rstart[c] = R(first tree) + sum of the chunk deltas before c; the first
-// tree's running sum arrives as nb per-block partials (k_tree0_sum).
+//
 // Rstart[c] = R(first tree) + sum of the chunk deltas before c, as a
 // two-level FP64 scan in a fixed order (deterministic): chunk blocks of
 // TSG_SCAN_B sum their deltas, one pass per output turns the block sums into
