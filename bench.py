@@ -166,22 +166,27 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     sweep_ms, launches = ll.get_profile()
+    per_mode = ll.get_profile_modes()
     ll.set_profiling(False)
     ms_per_step = 1e3 * elapsed / args.steps
     value = updates_per_step * args.steps / elapsed
-    log(f"{args.steps} steps in {elapsed:.3f}s ({ms_per_step:.2f} ms/step), "
-        f"sweep kernel {sweep_ms / max(launches, 1):.3f} ms x {launches}")
+    mode_ms = {m: t / max(n, 1) for m, (t, n) in per_mode.items()}
+    log(f"{args.steps} steps in {elapsed:.3f}s ({ms_per_step:.2f} ms/step), sweep kernel "
+        f"branch {mode_ms['branch']:.3f} ms, site {mode_ms['site']:.3f} ms "
+        f"({launches} launches)")
 
-    # roofline of the dominant kernel (the fused sweep): algorithmic bytes per
-    # launch / average launch time from HIP events on the library's stream
+    # roofline of the dominant kernel (the fused branch-mode sweep,
+    # k_sweep<0,...>): algorithmic bytes per launch / its average launch time
+    # from HIP events on the library's stream
     per_launch_queries = [(K, K + len(pairs)), (K, 2 * K), (K, len(quads))]
     alg_bytes = algorithmic_bytes(E, N, S, cfg["n"], per_launch_queries, cfg["W"])
-    avg_launch_s = (sweep_ms / 1e3) / max(launches, 1)
+    avg_launch_s = mode_ms["branch"] / 1e3
     achieved = alg_bytes / avg_launch_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_sweep (fused batch)", "bytes_per_launch": alg_bytes,
-                "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+                "kernel": "k_sweep<0> (branch, fused batch)", "bytes_per_launch": alg_bytes,
+                "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                "site_sweep_ms": round(mode_ms["site"], 4)}
 
     cpu = None
     errs = {}
@@ -213,7 +218,7 @@ def main():
         "data": "synthetic (own Hudson coalescent + infinite-sites generator)",
         "config": {"workload": cfg["name"], "edges": E, "trees": ts.num_trees, "sites": S,
                    "nodes": N, "queries_per_step": queries_per_step,
-                   "chunks": hp["chunks"], "node_store": ["hbm", "lds", "lds-hot"][
+                   "chunks": hp["chunks"], "node_store": ["hbm", "lds", "lds-hot", "lds-hot16"][
                        ll.get_last_store()]},
         "queries_per_s": queries_per_step * args.steps / elapsed,
         "roofline": roofline,

@@ -102,15 +102,20 @@ int tsg_treeseq_copy_hot_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0,
     int32_t *lev_slot0, uint16_t *sample_slot, int64_t *tr_off, int32_t *tr_nrem,
     int32_t *ev_child, float *ev_bl, int64_t *ev_path0, int32_t *path, int32_t *mut_slot,
     int32_t *t0_parent, float *t0_bl);
-/* plan = 1: run statistics sweeps on the full plan; 0: the library chooses
- * (the hot plan whenever its slots fit in LDS) -- a testing hook */
+/* plan = 1: run statistics sweeps on the full plan; 2: on the hot plan with
+ * 32-bit counts; 0: the library chooses (the hot plan whenever its slots fit
+ * in LDS, 16-bit counts when every set has <= 65535 samples) -- a testing
+ * hook */
 int tsg_treeseq_set_sweep_plan(tsg_treeseq_t *ts, int plan);
 /* node store of the last statistics sweep: 0 HBM slabs, 1 LDS (full plan),
- * 2 LDS hot slots, -1 before the first sweep */
+ * 2 LDS hot slots, 3 LDS hot slots with 16-bit counts, -1 before the first
+ * sweep */
 int tsg_treeseq_get_last_store(const tsg_treeseq_t *ts);
 /* HIP-event timing of the sweep kernel on the library's stream */
 int tsg_treeseq_set_profiling(tsg_treeseq_t *ts, int enable);
 int tsg_treeseq_get_profile(const tsg_treeseq_t *ts, double *sweep_ms, uint64_t *launches);
+/* the same split by mode: [0] branch sweeps, [1] site sweeps */
+int tsg_treeseq_get_profile_modes(const tsg_treeseq_t *ts, double *ms, uint64_t *launches);
 
 /* ---- one-way statistics ------------------------------------------------ */
 /* tsk_treeseq_diversity (trees.h:1103, trees.c:3950-3958) */

@@ -172,15 +172,15 @@ def test_chunk_counts(require_gpu, num_chunks):
                orc.stat("f4", sets + [sets[0]], [(0, 1, 2, 3)], w, mode)[:, 0])
 
 
-@pytest.mark.parametrize("plan", ["hot", "full"])
+@pytest.mark.parametrize("plan", ["hot16", "hot32", "full"])
 def test_node_stores(require_gpu, plan):
-    """The same statistics from the hot plan (slots in LDS) and from the full
-    plan, whose chunks are too large for LDS here and run from per-chunk HBM
-    slabs."""
+    """The same statistics from the hot plan (slots in LDS, 16- or 32-bit
+    counts) and from the full plan, whose chunks are too large for LDS here
+    and run from per-chunk HBM slabs."""
     tc = cases.generated(3000, 1e6, 13)
     ts = tc.tree_sequence()
     ll = ts.ll_tree_sequence
-    ll.set_sweep_plan(1 if plan == "full" else 0)
+    ll.set_sweep_plan({"hot16": 0, "full": 1, "hot32": 2}[plan])
     orc = Oracle(tc)
     sets = cases.sample_sets(3000, 8, 100, 6)
     w = np.linspace(0, tc.sequence_length, 5)
@@ -189,7 +189,7 @@ def test_node_stores(require_gpu, plan):
         _say("store", plan, mode)
         _check("divergence", ts.divergence(sets, indexes=idx, windows=w, mode=mode),
                orc.stat("divergence", sets, idx, w, mode))
-        assert ll.get_last_store() == (0 if plan == "full" else 2)
+        assert ll.get_last_store() == {"hot16": 3, "full": 0, "hot32": 2}[plan]
         _check("f4", ts.f4(sets[:4], windows=w, mode=mode),
                orc.stat("f4", sets[:4], [(0, 1, 2, 3)], w, mode)[:, 0])
 

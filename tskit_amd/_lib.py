@@ -64,6 +64,7 @@ SIGNATURES = {
     "tsg_treeseq_get_last_store": (ctypes.c_int, [c_void]),
     "tsg_treeseq_set_profiling": (ctypes.c_int, [c_void, ctypes.c_int]),
     "tsg_treeseq_get_profile": (ctypes.c_int, [c_void, c_f64p, c_u64p]),
+    "tsg_treeseq_get_profile_modes": (ctypes.c_int, [c_void, c_f64p, c_u64p]),
     "tsg_treeseq_diversity": (ctypes.c_int, ONE_WAY),
     "tsg_treeseq_segregating_sites": (ctypes.c_int, ONE_WAY),
     "tsg_treeseq_Y1": (ctypes.c_int, ONE_WAY),

@@ -50,11 +50,14 @@ struct Workspace {
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab;
     int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
     bool force_full = false;        // run sweeps on the full plan (testing hook)
+    bool force_c32 = false;         // hot plan with 32-bit counts (testing hook)
     int last_store = -1;            // node store of the last sweep (TSG_STORE_*)
     // profiling of the dominant (sweep) kernel
     bool profile = false;
     double sweep_ms = 0;
     uint64_t sweep_launches = 0;
+    double mode_ms[2] = {0, 0};     // branch, site sweeps
+    uint64_t mode_launches[2] = {0, 0};
 };
 
 std::mutex g_mutex;
@@ -361,7 +364,10 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     const int R = 2 + K;
     // the sweep's LDS staging block for the widest launch (512 outputs)
     const size_t stage = (size_t) TSG_STAGE_INTS(K, std::min(M, 512)) * 4;
-    const size_t hot_need = (size_t) ts->hot.max_slots * R * 4 + stage;
+    // 16-bit slot counts when no sample set exceeds 65535 samples
+    const bool c16 = !ws->force_c32 &&
+                     *std::max_element(col_total.begin(), col_total.end()) <= 65535;
+    const size_t hot_need = (size_t) ts->hot.max_slots * TSG_SLOT_WORDS(K, c16) * 4 + stage;
     const bool use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
     if (!use_hot && (ret = ensure_full(ts))) return ret;
     const int C = use_hot ? ts->hot.num_chunks : ts->num_chunks;
@@ -418,7 +424,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
             return ret;
         }
     }
-    int store = TSG_STORE_HOT;
+    int store = c16 ? TSG_STORE_HOT16 : TSG_STORE_HOT;
     size_t shmem = hot_need;
     const TsgDevice &d = ts->dev;
     const TsgHotDevice &h = ts->hot.dev;
@@ -577,6 +583,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
         ws->sweep_ms += ms;
         ws->sweep_launches += 1;
+        ws->mode_ms[mode_id] += ms;
+        ws->mode_launches[mode_id] += 1;
     }
     return 0;
 }
@@ -892,18 +900,22 @@ int tsg_treeseq_set_num_chunks(tsg_treeseq *ts, uint32_t num_chunks) {
     return tsg_build_hot_plan(ts, ts->full_chunks);
 }
 
-// Testing hook: plan = 1 runs every sweep on the full plan, 0 lets the
-// library choose (the hot plan whenever its slots fit in LDS).
+// Testing hook: plan = 1 runs every sweep on the full plan, 2 on the hot
+// plan with 32-bit counts, 0 lets the library choose (the hot plan whenever
+// its slots fit in LDS, with 16-bit counts when every set has <= 65535
+// samples).
 int tsg_treeseq_set_sweep_plan(tsg_treeseq *ts, int plan) {
     std::lock_guard<std::mutex> lock(g_mutex);
     int ret = device_upload(ts);
     if (ret != 0) return ret;
     ws_of(ts)->force_full = plan == 1;
+    ws_of(ts)->force_c32 = plan == 2;
     return 0;
 }
 
 // Node store of the last statistics sweep: 0 HBM slabs (full plan), 1 LDS
-// (full plan), 2 LDS hot slots; -1 before the first sweep.
+// (full plan), 2 LDS hot slots, 3 LDS hot slots with 16-bit counts; -1
+// before the first sweep.
 int tsg_treeseq_get_last_store(const tsg_treeseq *ts) {
     const Workspace *ws = ws_of(const_cast<tsg_treeseq *>(ts));
     return ws == nullptr ? -1 : ws->last_store;
@@ -1051,12 +1063,24 @@ int tsg_treeseq_set_profiling(tsg_treeseq *ts, int enable) {
     ws->profile = enable != 0;
     ws->sweep_ms = 0;
     ws->sweep_launches = 0;
+    ws->mode_ms[0] = ws->mode_ms[1] = 0;
+    ws->mode_launches[0] = ws->mode_launches[1] = 0;
     return 0;
 }
 int tsg_treeseq_get_profile(const tsg_treeseq *ts, double *sweep_ms, uint64_t *launches) {
     const Workspace *ws = reinterpret_cast<const Workspace *>(ts->workspace);
     *sweep_ms = ws ? ws->sweep_ms : 0.0;
     *launches = ws ? ws->sweep_launches : 0;
+    return 0;
+}
+
+// The same per statistics mode: ms[0] / launches[0] branch sweeps, [1] site.
+int tsg_treeseq_get_profile_modes(const tsg_treeseq *ts, double *ms, uint64_t *launches) {
+    const Workspace *ws = reinterpret_cast<const Workspace *>(ts->workspace);
+    for (int m = 0; m < 2; m++) {
+        ms[m] = ws ? ws->mode_ms[m] : 0.0;
+        launches[m] = ws ? ws->mode_launches[m] : 0;
+    }
     return 0;
 }
 

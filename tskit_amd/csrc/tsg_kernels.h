@@ -34,9 +34,9 @@ struct TsgQuery {
 // 32-bit words of one TsgOut.
 #define TSG_OUT_INTS ((int) (sizeof(TsgOut) / 4))
 // Trees, events and path entries the sweep kernel stages in LDS per batch.
-#define TSG_TB 64
+#define TSG_TB 32
 #define TSG_EB 64
-#define TSG_PB 1024
+#define TSG_PB 512
 // 32-bit words of the sweep kernel's LDS staging block for K state columns
 // and `span` outputs in the launch (layout in k_sweep); node records follow.
 #define TSG_STAGE_INTS(K, span)                                                               \
@@ -48,6 +48,9 @@ struct TsgQuery {
 #define TSG_STORE_GLOBAL 0
 #define TSG_STORE_LDS 1
 #define TSG_STORE_HOT 2
+#define TSG_STORE_HOT16 3
+// 32-bit words per hot slot: branch length + K counts (16-bit when c16)
+#define TSG_SLOT_WORDS(K, c16) (1 + ((c16) ? ((K) + 1) / 2 : (K)))
 
 #define TSG_F_POLARISED 1u
 #define TSG_F_NOSKIP 2u       // a summary has a zero denominator: never skip walks

@@ -23,16 +23,21 @@
 // Differences that are deliberate (and exact up to rounding):
 //   * running branch sums are kept per chunk relative to the chunk start and
 //     re-based afterwards with an FP64 prefix over chunks (the closing
-//     transition of each chunk makes the deltas chain);
-//   * summary numerators are integer polynomials of the counts and are
-//     evaluated exactly in int64, then scaled once in FP64 and rounded to FP32;
-//   * running sums and window sums are FP32 TwoSum (value + error) pairs;
+//     transition of each chunk makes the deltas chain); the first tree's sum
+//     is evaluated over its nodes in parallel;
+//   * summary numerators are integer polynomials of the counts, exact in
+//     FP64, scaled once by the reciprocal denominator; running sums and
+//     window sums are FP64;
+//   * for degree-2 summaries a walk's contribution is a closed form in two
+//     path moments (see k_sweep);
 //   * a walk whose moved subtree carries zero weight changes no count and no
 //     summary, so it is skipped; an output none of whose columns moved is not
 //     re-evaluated along the walk (its terms cancel exactly).  A summary with
 //     a zero denominator is NaN at every state, so the edge's own term (always
 //     added) carries the reference's NaN into the sums.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "tsg_common.h"
 #include "tsg_kernels.h"
@@ -108,18 +113,54 @@ __device__ double eval_summary(const TsgOut &o, const int32_t *x, const int32_t 
 #undef XV
 }
 
-// Per-chunk node storage.  rec[u*R] = parent, rec[u*R+1] = branch length
-// bits, rec[u*R+2+k] = count of state column k below u.
+// Per-chunk node storage of the full plan.  rec[u*R] = parent,
+// rec[u*R+1] = branch length bits, rec[u*R+2+k] = count of state column k
+// below u (R = 2 + K words).
 struct Nodes {
     int32_t *rec;
     int32_t R;
     __device__ int32_t parent(int u) const { return rec[(int64_t) u * R]; }
     __device__ float bl(int u) const { return __int_as_float(rec[(int64_t) u * R + 1]); }
     __device__ int32_t &x(int u, int k) const { return rec[(int64_t) u * R + 2 + k]; }
+    __device__ int32_t get(int u, int k) const { return x(u, k); }
+    __device__ void put(int u, int k, int32_t v) const { x(u, k) = v; }
     __device__ void set_bl(int u, float b) const { rec[(int64_t) u * R + 1] = __float_as_int(b); }
     __device__ void set_edge(int u, int p, float b) const {
         rec[(int64_t) u * R] = p;
         rec[(int64_t) u * R + 1] = __float_as_int(b);
+    }
+};
+
+// Hot-plan slot storage in LDS: rec[h*R] = branch length bits, then the
+// slot's K counts, as 32-bit words or, when every sample set has at most
+// 65535 samples (C16), as 16-bit halves (R = 1 + K or 1 + ceil(K / 2)
+// words).  No parent pointers: the root paths are precomputed.
+template <bool C16>
+struct Slots {
+    int32_t *rec;
+    int32_t R;
+    __device__ float bl(int h) const { return __int_as_float(rec[h * R]); }
+    __device__ void set_bl(int h, float b) const { rec[h * R] = __float_as_int(b); }
+    __device__ int32_t get(int h, int k) const {
+        if (C16) return reinterpret_cast<const uint16_t *>(rec + h * R + 1)[k];
+        return rec[h * R + 1 + k];
+    }
+    __device__ void put(int h, int k, int32_t v) const {
+        if (C16) {
+            reinterpret_cast<uint16_t *>(rec + h * R + 1)[k] = (uint16_t) v;
+        } else {
+            rec[h * R + 1 + k] = v;
+        }
+    }
+    // atomic += v (v >= 0); with 16-bit halves no carry crosses into the
+    // neighbour, since no count exceeds its set size
+    __device__ void add(int h, int k, int32_t v) const {
+        if (C16) {
+            atomicAdd(reinterpret_cast<unsigned *>(rec + h * R + 1 + (k >> 1)),
+                      (unsigned) v << (16 * (k & 1)));
+        } else {
+            atomicAdd(&rec[h * R + 1 + k], v);
+        }
     }
 };
 
@@ -146,12 +187,14 @@ __device__ void init_chunk(const TsgSweepArgs &a, const Nodes &nd, int64_t n0, i
 // add every weighted sample into its nearest hot ancestor-or-self, then add
 // each depth level of the hot forest into its parents, deepest level first
 // (the counts of a level are complete before it is added upward).
-__device__ void init_hot(const TsgSweepArgs &a, const Nodes &nd, int c, int nslot, int lane) {
+template <bool C16>
+__device__ void init_hot(const TsgSweepArgs &a, const Slots<C16> &nd, int c, int nslot, int K,
+                         int lane) {
     const int64_t s0 = a.chunk_node0[c];
-    const int64_t words = (int64_t) nslot * nd.R;
-    for (int64_t i = lane; i < words; i += WAVE) nd.rec[i] = 0;
+    const int words = nslot * nd.R;
+    for (int i = lane; i < words; i += WAVE) nd.rec[i] = 0;
     __syncthreads();
-    for (int h = lane; h < nslot; h += WAVE) nd.set_edge(h, TSG_NULL, a.loc_bl0[s0 + h]);
+    for (int h = lane; h < nslot; h += WAVE) nd.set_bl(h, a.loc_bl0[s0 + h]);
     // (sample, column) pairs, several per lane in flight: the slot gathers
     // are independent HBM reads
     const uint16_t *ss = a.sample_slot + (int64_t) c * a.num_samples;
@@ -169,21 +212,20 @@ __device__ void init_hot(const TsgSweepArgs &a, const Nodes &nd, int c, int nslo
         for (int r = 0; r < U; r++) ph[r] = (ps[r] >= 0) ? (unsigned) ss[ps[r]] : TSG_NO_SLOT;
 #pragma unroll
         for (int r = 0; r < U; r++) {
-            if (ph[r] != TSG_NO_SLOT) atomicAdd(&nd.x((int) ph[r], pc[r]), 1);
+            if (ph[r] != TSG_NO_SLOT) nd.add((int) ph[r], pc[r], 1);
         }
     }
     __syncthreads();
     const int32_t *lev = a.lev_slot0 + a.chunk_lev0[c];
     const int nlev = (int) (a.chunk_lev0[c + 1] - a.chunk_lev0[c]) - 1;
-    const int K = nd.R - 2;
     // the last level holds the roots of the hot forest: nothing above them
     for (int l = 0; l + 1 < nlev; l++) {
         const int h0 = lev[l], h1 = lev[l + 1];
         for (int idx = lane; idx < (h1 - h0) * K; idx += WAVE) {
             const int h = h0 + idx / K, k = idx % K;
             const int p = a.slot_par[s0 + h];
-            const int32_t v = nd.x(h, k);
-            if (p >= 0 && v != 0) atomicAdd(&nd.x(p, k), v);
+            const int32_t v = nd.get(h, k);
+            if (p >= 0 && v != 0) nd.add(p, k, v);
         }
         __syncthreads();
     }
@@ -337,8 +379,9 @@ __device__ __forceinline__ double branch_summary(const TsgOut &o, const int32_t 
 
 // MODE 0: branch statistics, MODE 1: site statistics.
 // STORE: TSG_STORE_GLOBAL (full plan, node records in a per-chunk HBM slab),
-// TSG_STORE_LDS (full plan, every local node in LDS) or TSG_STORE_HOT (hot
-// plan, only the chunk's hot slots, in LDS).
+// TSG_STORE_LDS (full plan, every local node in LDS), TSG_STORE_HOT (hot
+// plan, only the chunk's hot slots, in LDS) or TSG_STORE_HOT16 (the same
+// with 16-bit counts, for sample sets of at most 65535 samples).
 //
 // LDS: the staging block (TSG_STAGE_INTS) first, then for the LDS stores the
 // node records.  The staging block holds this launch's outputs, a batch of
@@ -392,8 +435,10 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     int32_t *tot = reinterpret_cast<int32_t *>(sbl + TSG_PSEG);          // [K]
     int32_t *zero = tot + K;                                             // [K]
     int32_t *act = zero + K;                                             // [span]
-    Nodes nd;
-    nd.R = R;
+    constexpr bool HOTS = STORE == TSG_STORE_HOT || STORE == TSG_STORE_HOT16;
+    using NS = typename std::conditional<HOTS, Slots<STORE == TSG_STORE_HOT16>, Nodes>::type;
+    NS nd;
+    nd.R = HOTS ? TSG_SLOT_WORDS(K, STORE == TSG_STORE_HOT16) : R;
     if (STORE != TSG_STORE_GLOBAL) {
         nd.rec = smem + TSG_STAGE_INTS(K, span);
     } else {
@@ -407,8 +452,8 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
         tot[k] = a.col_total[k];
         zero[k] = 0;
     }
-    if (STORE == TSG_STORE_HOT) {
-        init_hot(a, nd, c, nloc, lane);
+    if constexpr (HOTS) {
+        init_hot(a, nd, c, nloc, K, lane);
     } else {
         init_chunk(a, nd, n0, nloc, K, lane);
     }
@@ -495,14 +540,14 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             if (K <= WAVE) {
                 int32_t xk = 0;
                 if (lane < K) {
-                    xk = nd.x(u, lane);
+                    xk = nd.get(u, lane);
                     xc[lane] = xk;
                 }
                 cm = __ballot(xk != 0);
             } else {
                 bool nz = false;
                 for (int k = lane; k < K; k += WAVE) {
-                    const int32_t xk = nd.x(u, k);
+                    const int32_t xk = nd.get(u, k);
                     xc[k] = xk;
                     nz |= (xk != 0);
                 }
@@ -517,11 +562,19 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 const double b = sg * (double) (ins ? ebl : nd.bl(u));
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
-                    if (lo[j].lm >= 0) {
-                        const double f = (lo[j].mask & cm) ? branch_summary(sout[lo[j].lm], xc, tot)
-                                                           : lo[j].f0;
-                        lo[j].R += b * f;
+                    const LaneOut &L = lo[j];
+                    if (L.lm < 0) continue;
+                    double f = L.f0;
+                    if (L.mask & cm) {
+                        if (L.moment > 0) {  // F(xc) = F(0) + the degree-1 and 2 terms
+                            const double xa = (double) xc[L.ca], xb = (double) xc[L.cb];
+                            f += L.c1a * xa + L.c1b * xb + L.c2aa * xa * xa + L.c2bb * xb * xb +
+                                 L.c2ab * xa * xb;
+                        } else {
+                            f = branch_summary(sout[L.lm], xc, tot);
+                        }
                     }
+                    lo[j].R += b * f;
                 }
             }
             __syncthreads();
@@ -550,9 +603,9 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                 for (int idx = lane; idx < h * K; idx += WAVE) {
                     const int i = idx / K, k = idx - i * K;
                     const int node = pth[sg0 + i];
-                    const int32_t old = nd.x(node, k);
+                    const int32_t old = nd.get(node, k);
                     const int32_t nw = ins ? old + xc[k] : old - xc[k];
-                    nd.x(node, k) = nw;
+                    nd.put(node, k, nw);
                     if (MODE == 0) {
                         xo[idx] = old;
                         xn[idx] = nw;
@@ -657,8 +710,8 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                         int32_t w = (al == 0) ? tot[k] : 0;
                         for (int mm = m0; mm < m1; mm++) {
                             const int node = a.mut_node[mm];
-                            if (a.mut_allele[mm] == al) w += nd.x(node, k);
-                            if (a.mut_pallele[mm] == al) w -= nd.x(node, k);
+                            if (a.mut_allele[mm] == al) w += nd.get(node, k);
+                            if (a.mut_pallele[mm] == al) w -= nd.get(node, k);
                         }
                         xo[k] = w;
                     }
@@ -1187,6 +1240,8 @@ hipError_t launch_sweep_store(const TsgSweepArgs &a, int store, int mpl, size_t 
     switch (store) {
     case TSG_STORE_HOT:
         return launch_sweep_mpl<MODE, TSG_STORE_HOT>(a, mpl, shmem, s);
+    case TSG_STORE_HOT16:
+        return launch_sweep_mpl<MODE, TSG_STORE_HOT16>(a, mpl, shmem, s);
     case TSG_STORE_LDS:
         return launch_sweep_mpl<MODE, TSG_STORE_LDS>(a, mpl, shmem, s);
     default:

@@ -245,11 +245,13 @@ class LowLevelTreeSequence:
         return p
 
     def set_sweep_plan(self, plan):
-        """1: run statistics sweeps on the full plan; 0: the library chooses."""
+        """1: run statistics sweeps on the full plan; 2: on the hot plan with
+        32-bit counts; 0: the library chooses."""
         check(self._lib.tsg_treeseq_set_sweep_plan(self._h, int(plan)))
 
     def get_last_store(self):
-        """Node store of the last sweep: 0 HBM slabs, 1 LDS full plan, 2 LDS hot slots."""
+        """Node store of the last sweep: 0 HBM slabs, 1 LDS full plan, 2 LDS hot
+        slots, 3 LDS hot slots with 16-bit counts."""
         return int(self._lib.tsg_treeseq_get_last_store(self._h))
 
     def set_num_chunks(self, num_chunks):
@@ -260,6 +262,13 @@ class LowLevelTreeSequence:
 
     def set_profiling(self, enable):
         check(self._lib.tsg_treeseq_set_profiling(self._h, int(enable)))
+
+    def get_profile_modes(self):
+        """{mode: (sweep ms, launches)} for "branch" and "site" sweeps."""
+        ms = (ctypes.c_double * 2)()
+        n = (ctypes.c_uint64 * 2)()
+        self._lib.tsg_treeseq_get_profile_modes(self._h, ms, n)
+        return {"branch": (ms[0], int(n[0])), "site": (ms[1], int(n[1]))}
 
     def get_profile(self):
         ms = ctypes.c_double()
