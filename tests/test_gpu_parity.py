@@ -92,6 +92,28 @@ def test_k_way(small, mode, span_normalise):
 
 
 @pytest.mark.parametrize("mode", ["branch", "site"])
+def test_k_way_polarised(small, mode):
+    """Polarised k-way statistics through the low-level interface: in branch
+    mode these are the summaries evaluated hop by hop (cubic and quartic),
+    unlike their unpolarised forms, which are quadratic."""
+    tc, ts, orc = small
+    ll = ts.ll_tree_sequence
+    sets = cases.sample_sets(200, 4, 30, 12)
+    sizes = np.array([len(s) for s in sets], dtype=np.uint64)
+    flat = np.concatenate(sets).astype(np.int32)
+    w = np.array([0, 3e4, 1.2e5, 2e5])
+    for name, idx in K_WAY + [("Y1", None)]:
+        _say(name, "polarised", mode)
+        if idx is None:
+            got = ll.Y1(sizes, flat, w, mode=mode, span_normalise=1, polarised=1)
+        else:
+            got = getattr(ll, name)(sizes, flat, np.array(idx, dtype=np.int32), w, mode=mode,
+                                    span_normalise=1, polarised=1)
+        exp = orc.stat(name, sets, idx, w, mode, True, True)
+        _check(name, got, exp)
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
 def test_relatedness(small, mode):
     tc, ts, orc = small
     sets = cases.sample_sets(200, 5, 12, 3)

@@ -18,6 +18,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <numeric>
 #include <unordered_set>
 #include <vector>
 
@@ -113,21 +114,25 @@ Workspace *ws_of(tsg_treeseq *ts);
 int upload_seeds(Workspace *ws, const std::vector<int32_t> &pair_s,
                  const std::vector<int32_t> &pair_c, int32_t num_samples, hipStream_t s,
                  TsgSweepArgs &a) {
-    std::vector<int32_t> count(num_samples + 1, 0);
-    for (int32_t si : pair_s) count[si + 1]++;
-    std::vector<int32_t> first(num_samples + 1, 0);
-    for (int32_t j = 0; j < num_samples; j++) first[j + 1] = first[j] + count[j + 1];
-    std::vector<int32_t> col(pair_s.size());
-    std::vector<int32_t> fill(first.begin(), first.end() - 1);
-    for (size_t i = 0; i < pair_s.size(); i++) col[fill[pair_s[i]]++] = pair_c[i];
-    std::vector<int32_t> seed_sample, seed_off{0}, flat_s(pair_s.size());
-    for (int32_t j = 0; j < num_samples; j++) {
-        if (count[j + 1] > 0) {
-            seed_sample.push_back(j);
-            seed_off.push_back(first[j + 1]);
-            std::fill(flat_s.begin() + first[j], flat_s.begin() + first[j + 1], j);
+    (void) num_samples;
+    // pairs by sample (columns of one sample in their original order): the
+    // CSR form (one entry per weighted sample) and the flat pair list
+    std::vector<int32_t> order(pair_s.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return pair_s[x] < pair_s[y]; });
+    std::vector<int32_t> seed_sample, seed_off{0}, col(order.size()), flat_s(order.size());
+    for (size_t i = 0; i < order.size(); i++) {
+        const int32_t si = pair_s[order[i]];
+        col[i] = pair_c[order[i]];
+        flat_s[i] = si;
+        if (seed_sample.empty() || seed_sample.back() != si) {
+            if (!seed_sample.empty()) seed_off.push_back((int32_t) i);
+            seed_sample.push_back(si);
         }
     }
+    seed_off.push_back((int32_t) order.size());
+    if (seed_sample.empty()) seed_off.assign(1, 0);
     int ret;
     if ((ret = upload(ws->pair_s, seed_sample, s)) || (ret = upload(ws->pair_c, col, s)) ||
         (ret = upload(ws->seed_off, seed_off, s)) || (ret = upload(ws->seed_ps, flat_s, s))) {
@@ -193,15 +198,16 @@ int parse_sample_sets(const tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, 
         }
     }
     out.assign(K, {});
-    std::vector<uint64_t> seen(ts->num_samples, UINT64_MAX);
+    std::vector<int32_t> sorted;
     j = 0;
     for (uint64_t k = 0; k < K; k++) {
         out[k].reserve(sizes[k]);
-        for (uint64_t l = 0; l < sizes[k]; l++) {
-            const int32_t si = ts->sample_index_map[sets[j++]];
-            if (seen[si] == k) return TSG_ERR_DUPLICATE_SAMPLE;
-            seen[si] = k;
-            out[k].push_back(si);
+        for (uint64_t l = 0; l < sizes[k]; l++) out[k].push_back(ts->sample_index_map[sets[j++]]);
+        // a sample twice in one set (sort, then compare neighbours)
+        sorted = out[k];
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
+            return TSG_ERR_DUPLICATE_SAMPLE;
         }
     }
     return 0;
@@ -381,11 +387,14 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         const int W = qs[qi].W;
         qs[qi].pbase = pcount;
         int64_t slots = 0;
+        // windows touched by each chunk: both ends move forward with c
+        int wf = 0, wl = 0;
         for (int c = 0; c < C; c++) {
             const double cs = ts->breakpoints[ct0[c]];
             const double ce = ts->breakpoints[ct0[c + 1]];
-            const int wf = (int) (std::upper_bound(w, w + W + 1, cs) - w) - 1;
-            const int wl = (int) (std::lower_bound(w, w + W + 1, ce) - w) - 1;
+            while (wf + 1 < W && w[wf + 1] <= cs) wf++;  // last window starting <= cs
+            if (wl < wf) wl = wf;
+            while (wl + 1 < W && w[wl + 1] < ce) wl++;   // last window starting < ce
             cq_wfirst[(size_t) c * Q + qi] = wf;
             cq_nwin[(size_t) c * Q + qi] = wl - wf + 1;
             cq_poff[(size_t) c * Q + qi] = slots;

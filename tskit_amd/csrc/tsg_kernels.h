@@ -42,7 +42,7 @@ struct TsgQuery {
 #define TSG_STAGE_INTS(K, span)                                                               \
     ((int64_t) (span) * (TSG_OUT_INTS + 1) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + \
      (TSG_TB + 1) + 2 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) +   \
-     TSG_PSEG + 4 * 64 + 2)
+     TSG_PSEG + 8 * (int64_t) (K) + 2 * 64 + 2)
 
 // Where a sweep keeps its node records (see k_sweep).
 #define TSG_STORE_GLOBAL 0

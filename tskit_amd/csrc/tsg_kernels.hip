@@ -240,12 +240,30 @@ struct LaneOut {
     uint64_t mask;  // state columns the summary reads
     double f0;      // its value on an all-zero state (unpolarised wrapper included)
     double R, A;    // running branch sum, window sum (FP64)
-    // degree-2 summaries, F(x) = c0 + c1a x_a + c1b x_b + c2aa x_a^2 +
-    // c2bb x_b^2 + c2ab x_a x_b (unpolarised wrapper included); moment < 0:
-    // the summary is evaluated hop by hop instead
-    int moment, ca, cb;
-    double c1a, c1b, c2aa, c2bb, c2ab;
+    // how a walk's contribution is formed (see k_sweep): TSG_KIND_POLY, the
+    // summary (unpolarised wrapper included) as a polynomial in the counts
+    // of four column slots v_s = x[pc[s]]:
+    //   F = c0 + sum_s l1[s] v_s + sum_{s<=t} q[qidx(s,t)] v_s v_t + c3 v_0^3;
+    // TSG_KIND_SEG, segregating sites from per-column indicator sums;
+    // TSG_KIND_HOP, evaluated hop by hop
+    int kind;
+    int pc[4];
+    double c0, c3;
+    double l1[4];
+    double q[10];
 };
+
+#define TSG_KIND_HOP 0
+#define TSG_KIND_POLY 1
+#define TSG_KIND_SEG 2
+
+// Segregating-sites numerators for set size n (times 1/n = inv): polarised
+// f(x) = [x > 0] (n - x); unpolarised f(x) + f(n - x) = [x > 0] (n - x) +
+// [x < n] x, which is n for 0 < x < n and 0 otherwise (trees.c:4222-4236).
+__device__ __forceinline__ double seg_pol(double x, double n) { return x > 0 ? n - x : 0.0; }
+__device__ __forceinline__ double seg_unpol(double x, double n) {
+    return (x > 0 ? n - x : 0.0) + (x < n ? x : 0.0);
+}
 
 // State columns a summary function reads; all ones when it cannot be told
 // (more than 64 columns) or when the summary has a zero denominator (its NaN
@@ -281,56 +299,153 @@ __device__ uint64_t summary_columns(const TsgOut &o, int K) {
     return m;
 }
 
-// Coefficients of the degree-2 summaries (diversity trees.c:3935-3954,
-// divergence 4691-4721, non-centred relatedness 4756-4790), unpolarised
-// wrapper f(x) + f(total - x) folded in.  Returns false for the others.
-__device__ bool moment_form(const TsgOut &o, int K, LaneOut &L) {
-    if (K > 64 || !isfinite(o.inv)) return false;
+// index of the slot pair (s, t), s <= t, in LaneOut::q
+__device__ __forceinline__ int qidx(int s, int t) {
+    return s == 0 ? t : (s == 1 ? 3 + t : (s == 2 ? 5 + t : 9));
+}
+
+// Polynomial coefficients of the summaries that are polynomials of degree
+// <= 2 in the counts, or <= 3 in one column, once the unpolarised wrapper
+// f(x) + f(total - x) is folded in: diversity (trees.c:3935-3954), Y1
+// (4249-4264) and divergence (4691-4721) and non-centred relatedness
+// (4756-4790) either way; Y2 (4900-4925), f2 (4939-4965), Y3 (5178-5205),
+// f3 (5220-5250) and f4 (5267-5291) unpolarised, where their cubic and
+// quartic terms cancel.  The coefficients
+// were derived by expanding f(x) + f(total - x).  Segregating sites
+// (4222-4236) is TSG_KIND_SEG; the rest are TSG_KIND_HOP.
+__device__ int moment_form(const TsgOut &o, int K, LaneOut &L) {
+    if (K > 64 || !isfinite(o.inv)) return TSG_KIND_HOP;
     const bool pol = (o.flags & 1) != 0;
-    const double inv = o.inv, ni = (double) o.ni, nj = (double) o.nj;
-    double c1a = 0, c1b = 0, c2aa = 0, c2bb = 0, c2ab = 0;
+    const double ni = (double) o.ni, nj = (double) o.nj, nk = (double) o.nk, nl = (double) o.nl;
+    L.pc[0] = o.i;
+    L.pc[1] = o.j;
+    L.pc[2] = o.k;
+    L.pc[3] = o.l;
+    L.c0 = L.c3 = 0;
+    for (int s = 0; s < 4; s++) L.l1[s] = 0;
+    for (int s = 0; s < 10; s++) L.q[s] = 0;
+    double *l1 = L.l1, *q = L.q;
     switch (o.stat) {
-    case TSG_STAT_DIVERSITY:  // x (n - x) inv; f(n - x) = f(x)
-        c1a = ni * inv;
-        c2aa = -inv;
-        if (!pol) {
-            c1a *= 2;
-            c2aa *= 2;
+    case TSG_STAT_SEGREGATING_SITES:
+        return TSG_KIND_SEG;
+    case TSG_STAT_DIVERSITY:  // x (n - x); f(n - x) = f(x)
+        l1[0] = (pol ? 1 : 2) * ni;
+        q[0] = pol ? -1 : -2;
+        break;
+    case TSG_STAT_Y1:
+        // x (n - x)(n - x - 1) = n(n-1) x - (2n-1) x^2 + x^3; unpolarised
+        // + (n - x) x (x - 1): (n - 2)(n x - x^2)
+        if (pol) {
+            l1[0] = ni * (ni - 1);
+            q[0] = -(2 * ni - 1);
+            L.c3 = 1;
+        } else {
+            l1[0] = ni * (ni - 2);
+            q[0] = -(ni - 2);
         }
         break;
-    case TSG_STAT_DIVERGENCE:  // x_i (n_j - x_j) inv [+ (n_i - x_i) x_j inv]
-        c1a = nj * inv;
-        c2ab = -inv;
+    case TSG_STAT_DIVERGENCE:  // x_i (n_j - x_j) [+ (n_i - x_i) x_j]
+        l1[0] = nj;
+        q[qidx(0, 1)] = -1;
         if (!pol) {
-            c1b = ni * inv;
-            c2ab = -2 * inv;
+            l1[1] = ni;
+            q[qidx(0, 1)] = -2;
         }
         break;
-    case TSG_STAT_RELATEDNESS_NONCENTRED:  // x_i x_j inv [+ (n_i - x_i)(n_j - x_j) inv]
-        c2ab = inv;
+    case TSG_STAT_RELATEDNESS_NONCENTRED:  // x_i x_j [+ (n_i - x_i)(n_j - x_j)]
+        q[qidx(0, 1)] = 1;
         if (!pol) {
-            c1a = -nj * inv;
-            c1b = -ni * inv;
-            c2ab = 2 * inv;
+            L.c0 = ni * nj;
+            l1[0] = -nj;
+            l1[1] = -ni;
+            q[qidx(0, 1)] = 2;
         }
+        break;
+    case TSG_STAT_Y2:
+        if (pol) return TSG_KIND_HOP;
+        l1[0] = nj * (nj - 1);
+        l1[1] = -ni;
+        q[qidx(0, 1)] = -2 * (nj - 1);
+        q[qidx(1, 1)] = ni;
+        break;
+    case TSG_STAT_F2:
+        if (pol) return TSG_KIND_HOP;
+        l1[0] = -nj * (nj - 1);
+        l1[1] = -ni * (ni - 1);
+        q[qidx(0, 0)] = nj * (nj - 1);
+        q[qidx(0, 1)] = -2 * (ni - 1) * (nj - 1);
+        q[qidx(1, 1)] = ni * (ni - 1);
+        break;
+    case TSG_STAT_Y3:
+        if (pol) return TSG_KIND_HOP;
+        l1[0] = nj * nk;
+        q[qidx(0, 1)] = -nk;
+        q[qidx(0, 2)] = -nj;
+        q[qidx(1, 2)] = ni;
+        break;
+    case TSG_STAT_F3:
+        if (pol) return TSG_KIND_HOP;
+        l1[0] = -nj * nk;
+        q[qidx(0, 0)] = nj * nk;
+        q[qidx(0, 1)] = -nk * (ni - 1);
+        q[qidx(0, 2)] = -nj * (ni - 1);
+        q[qidx(1, 2)] = ni * (ni - 1);
+        break;
+    case TSG_STAT_F4:  // (n_j x_i - n_i x_j)(n_l x_k - n_k x_l)
+        if (pol) return TSG_KIND_HOP;
+        q[qidx(0, 2)] = nj * nl;
+        q[qidx(0, 3)] = -nj * nk;
+        q[qidx(1, 2)] = -ni * nl;
+        q[qidx(1, 3)] = ni * nk;
         break;
     default:
-        return false;
+        return TSG_KIND_HOP;
     }
-    int ca = o.i, cb = (o.stat == TSG_STAT_DIVERSITY) ? o.i : o.j;
-    if (ca == cb) {  // one column: x_a x_b = x_a^2
-        c1a += c1b;
-        c2aa += c2bb + c2ab;
-        c1b = c2bb = c2ab = 0;
+    const double inv = o.inv;
+    L.c0 *= inv;
+    L.c3 *= inv;
+    for (int s = 0; s < 4; s++) l1[s] *= inv;
+    for (int s = 0; s < 10; s++) q[s] *= inv;
+    return TSG_KIND_POLY;
+}
+
+// F at the counts x (TSG_KIND_POLY)
+__device__ __forceinline__ double poly_value(const LaneOut &L, const int32_t *x) {
+    double v[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) v[s] = (double) x[L.pc[s]];
+    double f = L.c0 + L.c3 * v[0] * v[0] * v[0];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        f += L.l1[s] * v[s];
+#pragma unroll
+        for (int t = s; t < 4; t++) f += L.q[qidx(s, t)] * v[s] * v[t];
     }
-    L.ca = ca;
-    L.cb = cb;
-    L.c1a = c1a;
-    L.c1b = c1b;
-    L.c2aa = c2aa;
-    L.c2bb = c2bb;
-    L.c2ab = c2ab;
-    return true;
+    return f;
+}
+
+// sum_h bl_h (F(x_h + d) - F(x_h)) from the path sums S0, S1[k] = sum bl x[k]
+// and S2[k] = sum bl x[k]^2; d[s] = the signed moved count of slot s:
+//   S0 (l1.d + d'Qd + c3 d0^3) + sum_{s<=t} q_st (d_t S1_s + d_s S1_t)
+//   + c3 (3 d0^2 S1_0 + 3 d0 S2_0)
+__device__ __forceinline__ double poly_walk(const LaneOut &L, const double *d, double s0,
+                                            const double *s1v, const double *s2v) {
+    double sv[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) sv[s] = s1v[L.pc[s]];
+    double lin = L.c3 * d[0] * d[0] * d[0];
+    double grad = L.c3 * (3 * d[0] * d[0] * sv[0] + 3 * d[0] * s2v[L.pc[0]]);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        lin += L.l1[s] * d[s];
+#pragma unroll
+        for (int t = s; t < 4; t++) {
+            const double qst = L.q[qidx(s, t)];
+            lin += qst * d[s] * d[t];
+            grad += qst * (d[t] * sv[s] + d[s] * sv[t]);
+        }
+    }
+    return s0 * lin + grad;
 }
 
 template <int MPL>
@@ -345,7 +460,7 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, 
         L.wi = L.wend = 0;
         L.mask = 0;
         L.f0 = 0.0;
-        L.moment = -1;
+        L.kind = TSG_KIND_HOP;
         if (m < a.m_end) {
             L.lm = m - a.m_begin;
             const TsgOut &o = sout[L.lm];
@@ -357,7 +472,7 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, 
             L.Mq = q.M;
             L.slot0 = q.pbase + a.cq_poff[cq] * (int64_t) q.M + (m - q.mbase);
             L.mask = summary_columns(o, a.K);
-            if (moment_form(o, a.K, L)) L.moment = 1;
+            L.kind = moment_form(o, a.K, L);
         }
     }
 }
@@ -397,14 +512,17 @@ __device__ __forceinline__ double branch_summary(const TsgOut &o, const int32_t 
 // has F(old) == F(new) at every hop, so its hop terms cancel exactly and are
 // not evaluated, and its child term is its all-zero value f0.  For the
 // others the walk's contribution sum_h bl_h (F(x_h + d) - F(x_h)) (d = the
-// moved counts, signed; trees.c:1436-1448, 1461-1472) is formed two ways:
-//   * degree-2 summaries (diversity, divergence, non-centred relatedness):
-//     F(x + d) - F(x) is linear in x, so the sum is a closed form in the
-//     path moments S0 = sum_h bl_h and S1[k] = sum_h bl_h x_h[k], which the
-//     count-update lanes accumulate;
-//   * the others: (output, hop) terms of a path segment are spread over the
-//     lanes (hop fastest), written to LDS, and each output's owner lane adds
-//     them in hop order.
+// moved counts, signed; trees.c:1436-1448, 1461-1472) is formed three ways:
+//   * polynomial summaries of degree <= 2 in two columns or <= 3 in one
+//     (diversity, divergence, non-centred relatedness, Y1): F(x + d) - F(x)
+//     has degree one less in x, so the sum is a closed form in the path
+//     sums S0 = sum_h bl_h, S1[k] = sum_h bl_h x_h[k] and S2[k] =
+//     sum_h bl_h x_h[k]^2, which the count-update lanes accumulate;
+//   * segregating sites: per column, sum_h bl_h (g(x_new) - g(x_old)) of
+//     its indicator-valued numerator g, accumulated the same way;
+//   * the others (f2, f3, f4, Y2, Y3, centred relatedness): (output, hop)
+//     terms of a path segment are spread over the lanes (hop fastest),
+//     written to LDS, and each output's owner lane adds them in hop order.
 // Running sums and window sums are FP64, as in the reference.
 template <int MODE, int STORE, int MPL>
 __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
@@ -421,8 +539,11 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     double *tbp = reinterpret_cast<double *>(sout + span);              // [TB+1] breakpoints
     int64_t *toff = reinterpret_cast<int64_t *>(tbp + TSG_TB + 1);       // [TB+1] tr_off
     int64_t *evp = toff + TSG_TB + 1;                                    // [EB+1] ev_path0
-    double *s1v = reinterpret_cast<double *>(evp + TSG_EB + 1);          // [64] path moments
-    double *td = s1v + 64;                                               // [WAVE] hop terms
+    double *s1v = reinterpret_cast<double *>(evp + TSG_EB + 1);          // [K] path sums
+    double *s2v = s1v + K;                                               // [K]
+    double *gpv = s2v + K;                                               // [K]
+    double *guv = gpv + K;                                               // [K]
+    double *td = guv + K;                                                // [WAVE] hop terms
     int32_t *tnrem = reinterpret_cast<int32_t *>(td + WAVE);             // [TB]
     int32_t *ts0 = tnrem + TSG_TB;                                       // [TB+1] tree_site0
     int32_t *evc = ts0 + TSG_TB + 1;                                     // [EB] event child
@@ -461,14 +582,19 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     LaneOut lo[MPL];
     setup_outputs<MPL>(a, sout, c, lane, lo);
     int wfirst[MPL];
-    bool any_moment = false;
+    bool any_poly = false, any_c3 = false, any_seg = false;
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
         wfirst[j] = lo[j].wi;
         if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(sout[lo[j].lm], zero, tot);
-        any_moment |= lo[j].moment > 0;
+        any_poly |= lo[j].kind == TSG_KIND_POLY;
+        any_c3 |= lo[j].kind == TSG_KIND_POLY && lo[j].c3 != 0;
+        any_seg |= lo[j].kind == TSG_KIND_SEG;
     }
-    any_moment = MODE == 0 && __any(any_moment);
+    any_poly = MODE == 0 && __any(any_poly);
+    any_c3 = MODE == 0 && __any(any_c3);
+    any_seg = MODE == 0 && __any(any_seg);
+    const bool any_agg = any_poly || any_seg;
 
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
     // the chunk's events: transitions t0+1 .. t1 (the last is the closing
@@ -566,10 +692,12 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                     if (L.lm < 0) continue;
                     double f = L.f0;
                     if (L.mask & cm) {
-                        if (L.moment > 0) {  // F(xc) = F(0) + the degree-1 and 2 terms
-                            const double xa = (double) xc[L.ca], xb = (double) xc[L.cb];
-                            f += L.c1a * xa + L.c1b * xb + L.c2aa * xa * xa + L.c2bb * xb * xb +
-                                 L.c2ab * xa * xb;
+                        if (L.kind == TSG_KIND_POLY) {
+                            f = poly_value(L, xc);
+                        } else if (L.kind == TSG_KIND_SEG) {
+                            const TsgOut &o = sout[L.lm];
+                            const double xa = (double) xc[o.i];
+                            f = o.inv * ((o.flags & 1) ? seg_pol(xa, o.ni) : seg_unpol(xa, o.ni));
                         } else {
                             f = branch_summary(sout[L.lm], xc, tot);
                         }
@@ -587,7 +715,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             int na = 0;
 #pragma unroll
             for (int j = 0; j < MPL; j++) {
-                const bool on = (MODE == 0) && lo[j].lm >= 0 && lo[j].moment < 0 &&
+                const bool on = (MODE == 0) && lo[j].lm >= 0 && lo[j].kind == TSG_KIND_HOP &&
                                 (lo[j].mask & cm) != 0;
                 const uint64_t bal = __ballot(on);
                 pos[j] = on ? na + __popcll(bal & below) : -1;
@@ -597,7 +725,9 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             const int64_t p0 = evp[le], p1 = evp[le + 1];
             const int32_t *pth = gpath ? a.path + p0 : pbuf + (p0 - pb0);
             __syncthreads();
-            double s0 = 0, s1 = 0;  // path moments (lane k < K: column k)
+            // path sums (lane k < K: column k): S0 = sum bl, S1 = sum bl x,
+            // S2 = sum bl x^2, and the segregating-sites indicator changes
+            double s0 = 0, s1 = 0, s2 = 0, gp = 0, gu = 0;
             for (int64_t sg0 = 0; sg0 < p1 - p0; sg0 += TSG_PSEG) {
                 const int h = (int) min((int64_t) TSG_PSEG, p1 - p0 - sg0);
                 for (int idx = lane; idx < h * K; idx += WAVE) {
@@ -615,11 +745,23 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                     for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(pth[sg0 + i]);
                 }
                 __syncthreads();
-                if (any_moment) {
+                if (any_agg) {
                     for (int i = 0; i < h; i++) {
                         const double bw = (double) sbl[i];
                         s0 += bw;
-                        if (lane < K) s1 += bw * (double) xo[i * K + lane];
+                        if (lane < K) {
+                            const double xold = (double) xo[i * K + lane];
+                            if (any_poly) {
+                                s1 += bw * xold;
+                                if (any_c3) s2 += bw * xold * xold;
+                            }
+                            if (any_seg) {
+                                const double xnew = (double) xn[i * K + lane];
+                                const double nk = (double) tot[lane];
+                                gp += bw * (seg_pol(xnew, nk) - seg_pol(xold, nk));
+                                gu += bw * (seg_unpol(xnew, nk) - seg_unpol(xold, nk));
+                            }
+                        }
                     }
                 }
                 if (MODE == 0 && na > 0) {
@@ -644,21 +786,30 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                     }
                 }
             }
-            if (any_moment) {
-                if (lane < K) s1v[lane] = s1;
+            if (any_agg) {
+                if (lane < K) {
+                    s1v[lane] = s1;
+                    s2v[lane] = s2;
+                    gpv[lane] = gp;
+                    guv[lane] = gu;
+                }
                 __syncthreads();
-                // sum_h bl_h (F(x_h + d) - F(x_h)) for F = c1.x + x'C2x:
-                // S0 (c1.d + d'C2d) + sum over columns of S1 * (gradient of d'C2x)
+                // sum_h bl_h (F(x_h + d) - F(x_h)) for F = c1.x + x'C2x + c3 x_a^3:
+                // S0 (c1.d + d'C2d + c3 d_a^3) + sum over columns of
+                // S1 * (gradient of d'C2x) + 3 c3 (S1_a d_a^2 + S2_a d_a)
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
                     LaneOut &L = lo[j];
-                    if (L.moment < 0 || !(L.mask & cm)) continue;
-                    const double da = sg * (double) xc[L.ca], db = sg * (double) xc[L.cb];
-                    const double t0 = L.c1a * da + L.c1b * db + L.c2aa * da * da +
-                                      L.c2bb * db * db + L.c2ab * da * db;
-                    const double ta = 2 * L.c2aa * da + L.c2ab * db;
-                    const double tb = 2 * L.c2bb * db + L.c2ab * da;
-                    L.R += s0 * t0 + s1v[L.ca] * ta + s1v[L.cb] * tb;
+                    if (L.kind == TSG_KIND_HOP || !(L.mask & cm)) continue;
+                    if (L.kind == TSG_KIND_SEG) {
+                        const TsgOut &o = sout[L.lm];
+                        L.R += o.inv * ((o.flags & 1) ? gpv[o.i] : guv[o.i]);
+                        continue;
+                    }
+                    double d[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) d[q] = sg * (double) xc[L.pc[q]];
+                    L.R += poly_walk(L, d, s0, s1v, s2v);
                 }
                 __syncthreads();
             }
@@ -1003,7 +1154,7 @@ __global__ void k_tree0_counts(const TsgSweepArgs a, const int32_t *t0_parent, i
     }
 }
 
-#define TSG_T0_NODES 256
+#define TSG_T0_NODES 64
 
 // grid (ceil(nloc / TSG_T0_NODES), ceil(M / 64)); lane = output.
 __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *t0_parent,
@@ -1014,10 +1165,19 @@ __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const 
     const int u0 = blockIdx.x * TSG_T0_NODES;
     const int u1 = min(nloc, u0 + TSG_T0_NODES);
     const TsgOut o = a.outs[m];
+    LaneOut L;
+    const bool mom = moment_form(o, a.K, L) == TSG_KIND_POLY;
     double acc = 0;
     for (int u = u0; u < u1; u++) {
         if (t0_parent[u] < 0) continue;
-        acc += (double) t0_bl[u] * branch_summary(o, x0 + (int64_t) u * a.K, a.col_total);
+        const int32_t *x = x0 + (int64_t) u * a.K;
+        double f;
+        if (mom) {  // the polynomial form, as in the sweep
+            f = poly_value(L, x);
+        } else {
+            f = branch_summary(o, x, a.col_total);
+        }
+        acc += (double) t0_bl[u] * f;
     }
     part[(int64_t) blockIdx.x * a.M + m] = acc;
 }
@@ -1044,15 +1204,29 @@ __global__ void k_scan_blocks(const double *dR, double *bsum, int C, int M) {
     bsum[(int64_t) blockIdx.x * M + m] = acc;
 }
 
-__global__ void k_scan_top(double *bsum, int nblk, int M, const double *r0part, int nb) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
+// one workgroup of TSG_SCAN_TOP threads per output: the first tree's
+// partials are summed by a fixed-shape tree (deterministic), then one lane
+// scans the block sums
+#define TSG_SCAN_TOP 256
+__global__ void __launch_bounds__(TSG_SCAN_TOP)
+k_scan_top(double *bsum, int nblk, int M, const double *r0part, int nb) {
+    __shared__ double red[TSG_SCAN_TOP];
+    const int m = blockIdx.x;
+    const int t = threadIdx.x;
     double acc = 0;
-    for (int b = 0; b < nb; b++) acc += r0part[(int64_t) b * M + m];
+    for (int b = t; b < nb; b += TSG_SCAN_TOP) acc += r0part[(int64_t) b * M + m];
+    red[t] = acc;
+    __syncthreads();
+    for (int w = TSG_SCAN_TOP / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    acc = red[0];
     for (int b = 0; b < nblk; b++) {
-        const double t = bsum[(int64_t) b * M + m];
+        const double x = bsum[(int64_t) b * M + m];
         bsum[(int64_t) b * M + m] = acc;
-        acc += t;
+        acc += x;
     }
 }
 
@@ -1317,7 +1491,7 @@ hipError_t tsg_launch_combine(const TsgCombineArgs &a, const double *dR, double 
         const int nblk = tsg_scan_blocks(a.num_chunks);
         const dim3 grid(nblk, (a.M + 63) / 64);
         hipLaunchKernelGGL(k_scan_blocks, grid, dim3(64), 0, s, dR, bsum, a.num_chunks, a.M);
-        hipLaunchKernelGGL(k_scan_top, dim3((a.M + 63) / 64), dim3(64), 0, s, bsum, nblk, a.M,
+        hipLaunchKernelGGL(k_scan_top, dim3(a.M), dim3(TSG_SCAN_TOP), 0, s, bsum, nblk, a.M,
                            r0part, nb);
         hipLaunchKernelGGL(k_scan_apply, grid, dim3(64), 0, s, dR, bsum, Rstart, a.num_chunks,
                            a.M);
