@@ -85,8 +85,8 @@ uint64_t tsg_treeseq_get_max_chunk_nodes(const tsg_treeseq_t *ts);
  * sites, mutations} */
 int tsg_treeseq_get_plan_sizes(const tsg_treeseq_t *ts, uint64_t *sizes);
 int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t *chunk_node0,
-    int32_t *loc_parent0, float *loc_bl0, double *loc_time, double *loc_lastrem,
-    int64_t *tr_off, int32_t *tr_nrem, int32_t *ev_child, int32_t *ev_parent, float *ev_bl,
+    int32_t *loc_parent0, double *loc_bl0, double *loc_time, double *loc_lastrem,
+    int64_t *tr_off, int32_t *tr_nrem, int32_t *ev_child, int32_t *ev_parent, double *ev_bl,
     int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc, int32_t *mut_allele,
     int32_t *mut_pallele, int32_t *site_nalleles);
 /* host copy of each event's root path (local ids, the edge's parent first);
@@ -98,10 +98,10 @@ int tsg_treeseq_copy_paths(const tsg_treeseq_t *ts, int64_t *ev_path0, int32_t *
  * entries, first-tree nodes, max slots per chunk, max levels per chunk} */
 int tsg_treeseq_get_hot_plan_sizes(const tsg_treeseq_t *ts, uint64_t *sizes);
 int tsg_treeseq_copy_hot_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0,
-    int64_t *chunk_slot0, int32_t *slot_par, float *slot_bl0, int64_t *chunk_lev0,
+    int64_t *chunk_slot0, int32_t *slot_par, double *slot_bl0, int64_t *chunk_lev0,
     int32_t *lev_slot0, uint16_t *sample_slot, int64_t *tr_off, int32_t *tr_nrem,
-    int32_t *ev_child, float *ev_bl, int64_t *ev_path0, int32_t *path, int32_t *mut_slot,
-    int32_t *t0_parent, float *t0_bl);
+    int32_t *ev_child, double *ev_bl, int64_t *ev_path0, int32_t *path, int32_t *mut_slot,
+    int32_t *t0_parent, double *t0_bl);
 /* plan = 1: run statistics sweeps on the full plan; 2: on the hot plan with
  * 32-bit counts; 0: the library chooses (the hot plan whenever its slots fit
  * in LDS, 16-bit counts when every set has <= 65535 samples) -- a testing

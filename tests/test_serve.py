@@ -1,0 +1,75 @@
+"""The serving loop: fused launches give the results of the same requests
+served alone (bit for bit), the oracle's within the parity tolerance, and the
+loop reports latencies."""
+import numpy as np
+import pytest
+
+import cases
+from oracle.oracle import Oracle
+
+from tskit_amd import serve
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.fixture(scope="module")
+def resident(require_gpu):
+    tcs = [cases.generated(n, L, seed) for n, L, seed in [(200, 2e5, 21), (300, 1e5, 22),
+                                                           (120, 3e5, 23)]]
+    return tcs, [tc.tree_sequence() for tc in tcs]
+
+
+def test_mixed_equals_alone(resident):
+    tcs, tss = resident
+    reqs = serve.random_requests(tss, 60, seed=5, max_sets=8, max_set_size=60, max_windows=12)
+    alone = serve.random_requests(tss, 60, seed=5, max_sets=8, max_set_size=60, max_windows=12)
+    mixed = serve.StatServer(tss)
+    for r in reqs:
+        mixed.submit(r)
+    done = mixed.step()
+    print("[case] mixed launches", mixed.launches, flush=True)
+    single = serve.StatServer(tss, one_per_launch=True)
+    for r in alone:
+        single.submit(r)
+    single.step()
+    assert len(done) == len(reqs)
+    assert mixed.launches < single.launches == len(reqs)
+    for a, b in zip(reqs, alone):
+        assert _same(a.result, b.result), (a.stat, a.kwargs.get("mode"))
+
+
+def test_served_results_match_oracle(resident):
+    tcs, tss = resident
+    reqs = serve.random_requests(tss, 24, seed=6, max_sets=6, max_set_size=50, max_windows=8,
+                                 stats=("diversity", "divergence", "f2", "f3", "f4"))
+    srv = serve.StatServer(tss)
+    for r in reqs:
+        srv.submit(r)
+    srv.step()
+    orcs = [Oracle(tc) for tc in tcs]
+    for r in reqs:
+        kw = r.kwargs
+        idx = kw.get("indexes")
+        exp = orcs[r.ts_id].stat(r.stat, kw["sample_sets"], idx, kw["windows"], kw["mode"])
+        got = np.asarray(r.result).reshape(exp.shape)
+        scale = np.maximum(np.max(np.abs(exp), axis=0, keepdims=True), 1e-300)
+        fin = np.isfinite(exp)
+        assert np.array_equal(fin, np.isfinite(got))
+        assert np.max(np.abs(got - exp)[fin] / np.broadcast_to(scale, exp.shape)[fin]) <= 1e-6
+
+
+def test_open_loop_latencies(resident):
+    tcs, tss = resident
+    reqs = serve.random_requests(tss, 40, seed=7, max_sets=4, max_set_size=40, max_windows=6)
+    srv = serve.StatServer(tss)
+    arrivals = np.cumsum(np.random.default_rng(1).exponential(1 / 400.0, len(reqs)))
+    wall = srv.run_open_loop(reqs, arrivals)
+    st = serve.latency_stats(reqs)
+    assert wall > 0 and all(r.result is not None for r in reqs)
+    assert 0 < st["p50_ms"] <= st["p99_ms"]

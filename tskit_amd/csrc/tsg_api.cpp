@@ -367,7 +367,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     // Plan and node store: the hot plan with its slots in LDS when the
     // largest chunk's slots fit; otherwise the full plan, in LDS when its
     // largest chunk fits, else in per-chunk HBM slabs.
-    const int R = 2 + K;
+    const int R = TSG_NODE_WORDS(K);
     // the sweep's LDS staging block for the widest launch (512 outputs)
     const size_t stage = (size_t) TSG_STAGE_INTS(K, std::min(M, 512)) * 4;
     // 16-bit slot counts when no sample set exceeds 65535 samples
@@ -438,7 +438,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     const TsgDevice &d = ts->dev;
     const TsgHotDevice &h = ts->hot.dev;
     const int32_t *t0_parent = d.loc_parent0;
-    const float *t0_bl = d.loc_bl0;
+    const double *t0_bl = d.loc_bl0;
     if (use_hot) {
         a.chunk_tree0 = h.chunk_tree0;
         a.chunk_node0 = h.chunk_slot0;
@@ -981,9 +981,9 @@ int tsg_treeseq_get_plan_sizes(const tsg_treeseq *ts, uint64_t *sizes) {
 }
 
 int tsg_treeseq_copy_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *chunk_node0,
-                          int32_t *loc_parent0, float *loc_bl0, double *loc_time,
+                          int32_t *loc_parent0, double *loc_bl0, double *loc_time,
                           double *loc_lastrem, int64_t *tr_off, int32_t *tr_nrem,
-                          int32_t *ev_child, int32_t *ev_parent, float *ev_bl,
+                          int32_t *ev_child, int32_t *ev_parent, double *ev_bl,
                           int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc,
                           int32_t *mut_allele, int32_t *mut_pallele, int32_t *site_nalleles) {
     ts = full_plan(ts);
@@ -1036,11 +1036,11 @@ int tsg_treeseq_get_hot_plan_sizes(const tsg_treeseq *ts, uint64_t *sizes) {
 }
 
 int tsg_treeseq_copy_hot_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *chunk_slot0,
-                              int32_t *slot_par, float *slot_bl0, int64_t *chunk_lev0,
+                              int32_t *slot_par, double *slot_bl0, int64_t *chunk_lev0,
                               int32_t *lev_slot0, uint16_t *sample_slot, int64_t *tr_off,
-                              int32_t *tr_nrem, int32_t *ev_child, float *ev_bl,
+                              int32_t *tr_nrem, int32_t *ev_child, double *ev_bl,
                               int64_t *ev_path0, int32_t *path, int32_t *mut_slot,
-                              int32_t *t0_parent, float *t0_bl) {
+                              int32_t *t0_parent, double *t0_bl) {
     const tsg_hotplan &hp = ts->hot;
     if (!hp.usable) return TSG_ERR_UNSUPPORTED_INPUT;
     auto cp = [](auto *dst, const auto &v) { std::copy(v.begin(), v.end(), dst); };
@@ -1574,7 +1574,7 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
         (ret = ensure(ws->out, (size_t) hist_n * 8))) {
         return ret;
     }
-    const int R = 2 + KK;
+    const int R = TSG_NODE_WORDS(KK);
     const size_t stage = (size_t) 2 * KK * 4 + 64;
     size_t lds_need = (size_t) ts->max_nloc * R * 4 + stage;
     if (branch) lds_need += (size_t) ts->max_nloc * 8;

@@ -41,16 +41,18 @@ struct TsgQuery {
 // and `span` outputs in the launch (layout in k_sweep); node records follow.
 #define TSG_STAGE_INTS(K, span)                                                               \
     ((int64_t) (span) * (TSG_OUT_INTS + 1) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + \
-     (TSG_TB + 1) + 2 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) +   \
-     TSG_PSEG + 8 * (int64_t) (K) + 2 * 64 + 2)
+     (TSG_TB + 1) + 3 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) +   \
+     2 * TSG_PSEG + 8 * (int64_t) (K) + 2 * 64 + 2)
 
 // Where a sweep keeps its node records (see k_sweep).
 #define TSG_STORE_GLOBAL 0
 #define TSG_STORE_LDS 1
 #define TSG_STORE_HOT 2
 #define TSG_STORE_HOT16 3
-// 32-bit words per hot slot: branch length + K counts (16-bit when c16)
-#define TSG_SLOT_WORDS(K, c16) (1 + ((c16) ? ((K) + 1) / 2 : (K)))
+// 32-bit words per hot slot: FP64 branch length + K counts (16-bit when c16)
+#define TSG_SLOT_WORDS(K, c16) (2 + ((c16) ? ((K) + 1) / 2 : (K)))
+// 32-bit words per full-plan node record: parent, FP64 branch length, K counts
+#define TSG_NODE_WORDS(K) (3 + (K))
 
 #define TSG_F_POLARISED 1u
 #define TSG_F_NOSKIP 2u       // a summary has a zero denominator: never skip walks
@@ -62,13 +64,13 @@ struct TsgSweepArgs {
     const int32_t *chunk_tree0;
     const int64_t *chunk_node0;
     const int32_t *loc_parent0;
-    const float *loc_bl0;
+    const double *loc_bl0;
     const double *loc_time;
     const double *loc_lastrem;   // last removal position before the chunk start
     const int64_t *tr_off;
     const int32_t *tr_nrem;
     const int32_t *ev_child, *ev_parent;
-    const float *ev_bl;
+    const double *ev_bl;
     const double *bp;
     const int32_t *tree_site0;
     const double *site_pos;

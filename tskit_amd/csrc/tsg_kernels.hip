@@ -113,53 +113,60 @@ __device__ double eval_summary(const TsgOut &o, const int32_t *x, const int32_t 
 #undef XV
 }
 
+// FP64 value held in two 32-bit words (no 8-byte alignment needed)
+__device__ __forceinline__ double ld_f64(const int32_t *p) { return __hiloint2double(p[1], p[0]); }
+__device__ __forceinline__ void st_f64(int32_t *p, double v) {
+    p[0] = __double2loint(v);
+    p[1] = __double2hiint(v);
+}
+
 // Per-chunk node storage of the full plan.  rec[u*R] = parent,
-// rec[u*R+1] = branch length bits, rec[u*R+2+k] = count of state column k
-// below u (R = 2 + K words).
+// rec[u*R+1..2] = FP64 branch length, rec[u*R+3+k] = count of state column
+// k below u (R = 3 + K words).
 struct Nodes {
     int32_t *rec;
     int32_t R;
     __device__ int32_t parent(int u) const { return rec[(int64_t) u * R]; }
-    __device__ float bl(int u) const { return __int_as_float(rec[(int64_t) u * R + 1]); }
-    __device__ int32_t &x(int u, int k) const { return rec[(int64_t) u * R + 2 + k]; }
+    __device__ double bl(int u) const { return ld_f64(rec + (int64_t) u * R + 1); }
+    __device__ int32_t &x(int u, int k) const { return rec[(int64_t) u * R + 3 + k]; }
     __device__ int32_t get(int u, int k) const { return x(u, k); }
     __device__ void put(int u, int k, int32_t v) const { x(u, k) = v; }
-    __device__ void set_bl(int u, float b) const { rec[(int64_t) u * R + 1] = __float_as_int(b); }
-    __device__ void set_edge(int u, int p, float b) const {
+    __device__ void set_bl(int u, double b) const { st_f64(rec + (int64_t) u * R + 1, b); }
+    __device__ void set_edge(int u, int p, double b) const {
         rec[(int64_t) u * R] = p;
-        rec[(int64_t) u * R + 1] = __float_as_int(b);
+        st_f64(rec + (int64_t) u * R + 1, b);
     }
 };
 
-// Hot-plan slot storage in LDS: rec[h*R] = branch length bits, then the
+// Hot-plan slot storage in LDS: rec[h*R..+1] = FP64 branch length, then the
 // slot's K counts, as 32-bit words or, when every sample set has at most
-// 65535 samples (C16), as 16-bit halves (R = 1 + K or 1 + ceil(K / 2)
+// 65535 samples (C16), as 16-bit halves (R = 2 + K or 2 + ceil(K / 2)
 // words).  No parent pointers: the root paths are precomputed.
 template <bool C16>
 struct Slots {
     int32_t *rec;
     int32_t R;
-    __device__ float bl(int h) const { return __int_as_float(rec[h * R]); }
-    __device__ void set_bl(int h, float b) const { rec[h * R] = __float_as_int(b); }
+    __device__ double bl(int h) const { return ld_f64(rec + h * R); }
+    __device__ void set_bl(int h, double b) const { st_f64(rec + h * R, b); }
     __device__ int32_t get(int h, int k) const {
-        if (C16) return reinterpret_cast<const uint16_t *>(rec + h * R + 1)[k];
-        return rec[h * R + 1 + k];
+        if (C16) return reinterpret_cast<const uint16_t *>(rec + h * R + 2)[k];
+        return rec[h * R + 2 + k];
     }
     __device__ void put(int h, int k, int32_t v) const {
         if (C16) {
-            reinterpret_cast<uint16_t *>(rec + h * R + 1)[k] = (uint16_t) v;
+            reinterpret_cast<uint16_t *>(rec + h * R + 2)[k] = (uint16_t) v;
         } else {
-            rec[h * R + 1 + k] = v;
+            rec[h * R + 2 + k] = v;
         }
     }
     // atomic += v (v >= 0); with 16-bit halves no carry crosses into the
     // neighbour, since no count exceeds its set size
     __device__ void add(int h, int k, int32_t v) const {
         if (C16) {
-            atomicAdd(reinterpret_cast<unsigned *>(rec + h * R + 1 + (k >> 1)),
+            atomicAdd(reinterpret_cast<unsigned *>(rec + h * R + 2 + (k >> 1)),
                       (unsigned) v << (16 * (k & 1)));
         } else {
-            atomicAdd(&rec[h * R + 1 + k], v);
+            atomicAdd(&rec[h * R + 2 + k], v);
         }
     }
 };
@@ -530,7 +537,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int K = a.K;
-    const int R = 2 + K;
+    const int R = 3 + K;
     const int span = a.m_end - a.m_begin;
     const int64_t n0 = a.chunk_node0[c];
     const int nloc = (int) (a.chunk_node0[c + 1] - n0);
@@ -544,16 +551,16 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     double *gpv = s2v + K;                                               // [K]
     double *guv = gpv + K;                                               // [K]
     double *td = guv + K;                                                // [WAVE] hop terms
-    int32_t *tnrem = reinterpret_cast<int32_t *>(td + WAVE);             // [TB]
+    double *evb = td + WAVE;                                             // [EB] event branch length
+    double *sbl = evb + TSG_EB;                                          // [PSEG] path branch lengths
+    int32_t *tnrem = reinterpret_cast<int32_t *>(sbl + TSG_PSEG);        // [TB]
     int32_t *ts0 = tnrem + TSG_TB;                                       // [TB+1] tree_site0
     int32_t *evc = ts0 + TSG_TB + 1;                                     // [EB] event child
-    float *evb = reinterpret_cast<float *>(evc + TSG_EB);                // [EB] event branch length
-    int32_t *pbuf = reinterpret_cast<int32_t *>(evb + TSG_EB);           // [PB] path entries
+    int32_t *pbuf = evc + TSG_EB;                                        // [PB] path entries
     int32_t *xc = pbuf + TSG_PB;                                         // [K]
     int32_t *xo = xc + K;                                                // [PSEG x K]
     int32_t *xn = xo + TSG_PSEG * K;                                     // [PSEG x K]
-    float *sbl = reinterpret_cast<float *>(xn + TSG_PSEG * K);           // [PSEG]
-    int32_t *tot = reinterpret_cast<int32_t *>(sbl + TSG_PSEG);          // [K]
+    int32_t *tot = xn + TSG_PSEG * K;                                    // [K]
     int32_t *zero = tot + K;                                             // [K]
     int32_t *act = zero + K;                                             // [span]
     constexpr bool HOTS = STORE == TSG_STORE_HOT || STORE == TSG_STORE_HOT16;
@@ -660,7 +667,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             const int le = (int) (e - eb);
             const bool ins = (e - e0) >= nrem;
             const int u = evc[le];
-            const float ebl = evb[le];
+            const double ebl = evb[le];
             // columns the moved subtree carries
             uint64_t cm;
             if (K <= WAVE) {
@@ -708,7 +715,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
             __syncthreads();
             // after seeding nothing reads parent pointers (the root paths
             // are precomputed), so only the branch length is kept current
-            if (lane == 0) nd.set_bl(u, ins ? ebl : 0.0f);
+            if (lane == 0) nd.set_bl(u, ins ? ebl : 0.0);
             if (cm == 0) continue;  // no count changes, no summary changes
             // outputs whose summaries can change, compacted into act[]
             int pos[MPL];
@@ -949,7 +956,7 @@ __global__ void __launch_bounds__(WAVE) k_afs_site(const TsgSweepArgs a) {
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int K = a.afs_K + 1;
-    const int R = 2 + K;
+    const int R = 3 + K;
     const int64_t n0 = a.chunk_node0[c];
     const int nloc = (int) (a.chunk_node0[c + 1] - n0);
     Nodes nd;
@@ -1034,7 +1041,7 @@ __global__ void __launch_bounds__(WAVE) k_afs_branch(const TsgSweepArgs a) {
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int K = a.afs_K + 1;
-    const int R = 2 + K;
+    const int R = 3 + K;
     const int64_t n0 = a.chunk_node0[c];
     const int nloc = (int) (a.chunk_node0[c + 1] - n0);
     Nodes nd;
@@ -1158,7 +1165,7 @@ __global__ void k_tree0_counts(const TsgSweepArgs a, const int32_t *t0_parent, i
 
 // grid (ceil(nloc / TSG_T0_NODES), ceil(M / 64)); lane = output.
 __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *t0_parent,
-                                                    const float *t0_bl, const int32_t *x0,
+                                                    const double *t0_bl, const int32_t *x0,
                                                     int nloc, double *part) {
     const int m = blockIdx.y * WAVE + threadIdx.x;
     if (m >= a.M) return;
@@ -1463,7 +1470,7 @@ hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t s
 
 int tsg_tree0_blocks(int nloc) { return (nloc + TSG_T0_NODES - 1) / TSG_T0_NODES; }
 
-hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, const float *t0_bl,
+hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, const double *t0_bl,
                             int32_t *x0, int nloc, double *part, hipStream_t s) {
     const int64_t words = (int64_t) nloc * a.K;
     if (words > 0) {

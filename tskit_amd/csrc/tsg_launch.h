@@ -14,7 +14,7 @@ int tsg_tree0_blocks(int nloc);
 // counts of the first tree (t0_parent / t0_bl, samples first, nloc nodes)
 // into x0 [nloc x K] and its running branch sum as per-block partials
 // part [tsg_tree0_blocks(nloc) x M]
-hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, const float *t0_bl,
+hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, const double *t0_bl,
                             int32_t *x0, int nloc, double *part, hipStream_t s);
 // chunk blocks of the running-sum scan (bsum holds blocks x M doubles)
 int tsg_scan_blocks(int num_chunks);

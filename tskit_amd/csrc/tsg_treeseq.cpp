@@ -295,7 +295,7 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
     std::vector<int32_t> stamp(N, -1), locid(N, -1);
     std::vector<int32_t> loc2glob;
     std::vector<int32_t> snap_parent;
-    std::vector<float> snap_bl;
+    std::vector<double> snap_bl;
     int32_t j = 0, k = 0;  // index cursors (insertion / removal)
     // opening transition: the first tree's edges (no removals at x = 0)
     while (j < E && ts->left[ts->I[j]] == ts->breakpoints[0]) {
@@ -337,7 +337,7 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
             const int32_t lu = ensure(u);
             const int32_t lp = ensure(gpar[u]);
             snap_parent[lu] = lp;
-            snap_bl[lu] = (float) (ts->time[gpar[u]] - ts->time[u]);
+            snap_bl[lu] = ts->time[gpar[u]] - ts->time[u];
         }
         const int32_t t0 = ts->chunk_tree0[c], t1 = ts->chunk_tree0[c + 1];
         // transitions owned by this chunk (transition 0 is the opening one)
@@ -371,7 +371,7 @@ int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks) {
                 const int32_t u = ts->child[e], p = ts->parent[e];
                 ts->ev_child.push_back(ensure(u));
                 ts->ev_parent.push_back(ensure(p));
-                ts->ev_bl.push_back((float) (ts->time[p] - ts->time[u]));
+                ts->ev_bl.push_back(ts->time[p] - ts->time[u]);
                 gpar[u] = p;
                 record_path(p);
                 alive_pos[u] = (int32_t) alive.size();
@@ -412,7 +412,7 @@ namespace {
 // written in place.
 struct HotPart {
     std::vector<int32_t> ev_child, path, slot_par, lev_slot0;
-    std::vector<float> ev_bl, slot_bl0;
+    std::vector<double> ev_bl, slot_bl0;
     std::vector<int64_t> ev_end;       // per event: end of its path (part-relative)
     std::vector<int64_t> chunk_slots;  // per chunk
     std::vector<int64_t> chunk_levs;   // per chunk: level-boundary entries
@@ -485,7 +485,7 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
                 const int32_t e = ts->I[j];
                 const int32_t u = ts->child[e], p = ts->parent[e];
                 ev_u.push_back(u);
-                out.ev_bl.push_back((float) (ts->time[p] - ts->time[u]));
+                out.ev_bl.push_back(ts->time[p] - ts->time[u]);
                 touch(u);
                 undo.emplace_back(u, gpar[u]);
                 gpar[u] = p;
@@ -544,7 +544,7 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
             const int32_t u = hot[i];
             out.slot_par[s0 + slot_of[i]] = (hpar[i] < 0) ? -1 : slot_of[hpar[i]];
             out.slot_bl0[s0 + slot_of[i]] =
-                (gpar[u] == TSG_NULL) ? 0.0f : (float) (ts->time[gpar[u]] - ts->time[u]);
+                (gpar[u] == TSG_NULL) ? 0.0 : ts->time[gpar[u]] - ts->time[u];
         }
         for (int32_t u : ev_u) out.ev_child.push_back(slot_of[hot_idx[u]]);
         const int64_t pbase = (int64_t) out.path.size();
@@ -623,12 +623,12 @@ int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
             }
         }
         hp.t0_parent.assign(order.size(), TSG_NULL);
-        hp.t0_bl.assign(order.size(), 0.0f);
+        hp.t0_bl.assign(order.size(), 0.0);
         for (size_t i = 0; i < order.size(); i++) {
             const int32_t u = order[i];
             if (gpar[u] != TSG_NULL) {
                 hp.t0_parent[i] = lid[gpar[u]];
-                hp.t0_bl[i] = (float) (ts->time[gpar[u]] - ts->time[u]);
+                hp.t0_bl[i] = ts->time[gpar[u]] - ts->time[u];
             }
         }
     }

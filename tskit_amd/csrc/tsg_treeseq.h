@@ -14,7 +14,7 @@ struct TsgDevice {
     int64_t *chunk_node0 = nullptr;   // offset of each chunk's local-node block
     // per local node [sum nloc]
     int32_t *loc_parent0 = nullptr;   // parent (local id) in the chunk's start tree
-    float *loc_bl0 = nullptr;         // branch length in the start tree (0 if root)
+    double *loc_bl0 = nullptr;         // branch length in the start tree (0 if root)
     double *loc_time = nullptr;       // node time
     double *loc_lastrem = nullptr;    // position of the node's last edge removal
     // per transition [T+1]: events of transition t are [tr_off[t], tr_off[t+1]),
@@ -24,7 +24,7 @@ struct TsgDevice {
     int32_t *tr_nrem = nullptr;
     int32_t *ev_child = nullptr;      // local ids in the owning chunk
     int32_t *ev_parent = nullptr;
-    float *ev_bl = nullptr;           // branch length of inserted edge
+    double *ev_bl = nullptr;          // branch length of inserted edge
     int64_t *ev_path0 = nullptr;      // [events + 1] offsets into path
     int32_t *path = nullptr;          // ancestors of each event's parent end
     double *breakpoints = nullptr;    // [T+1]
@@ -43,19 +43,19 @@ struct TsgHotDevice {
     int32_t *chunk_tree0 = nullptr;
     int64_t *chunk_slot0 = nullptr;
     int32_t *slot_par = nullptr;
-    float *slot_bl0 = nullptr;
+    double *slot_bl0 = nullptr;
     int64_t *chunk_lev0 = nullptr;
     int32_t *lev_slot0 = nullptr;
     uint16_t *sample_slot = nullptr;
     int64_t *tr_off = nullptr;
     int32_t *tr_nrem = nullptr;
     int32_t *ev_child = nullptr;
-    float *ev_bl = nullptr;
+    double *ev_bl = nullptr;
     int64_t *ev_path0 = nullptr;
     int32_t *path = nullptr;
     int32_t *mut_slot = nullptr;
     int32_t *t0_parent = nullptr;
-    float *t0_bl = nullptr;
+    double *t0_bl = nullptr;
     size_t bytes = 0;
 };
 
@@ -79,21 +79,21 @@ struct tsg_hotplan {
     std::vector<int32_t> chunk_tree0;     // [C+1]
     std::vector<int64_t> chunk_slot0;     // [C+1]
     std::vector<int32_t> slot_par;        // [slots] chunk-local slot or -1
-    std::vector<float> slot_bl0;          // [slots] start-tree branch length (0 if root)
+    std::vector<double> slot_bl0;         // [slots] start-tree branch length (0 if root)
     std::vector<int64_t> chunk_lev0;      // [C+1] offsets into lev_slot0
     std::vector<int32_t> lev_slot0;       // per chunk: level boundaries (nlev + 1 entries)
     std::vector<uint16_t> sample_slot;    // [C * n]
     std::vector<int64_t> tr_off;          // [T+1]
     std::vector<int32_t> tr_nrem;         // [T]
     std::vector<int32_t> ev_child;        // slot ids
-    std::vector<float> ev_bl;
+    std::vector<double> ev_bl;
     std::vector<int64_t> ev_path0;        // [events + 1]
     std::vector<int32_t> path;            // slot ids, parent end first
     std::vector<int32_t> mut_slot;        // [Mu]
     // the first tree, samples first (local id j = sample j): its running
     // branch sum is evaluated over all of its nodes
     std::vector<int32_t> t0_parent;
-    std::vector<float> t0_bl;
+    std::vector<double> t0_bl;
     int32_t max_slots = 0, max_path = 0, max_levels = 0;
     TsgHotDevice dev;
 };
@@ -126,13 +126,13 @@ struct tsg_treeseq {
     int32_t max_nloc = 0;
     int32_t max_chunk_events = 0;
     std::vector<int32_t> loc_parent0;
-    std::vector<float> loc_bl0;
+    std::vector<double> loc_bl0;
     std::vector<double> loc_time;
     std::vector<double> loc_lastrem;
     std::vector<int64_t> tr_off;          // [T+1]
     std::vector<int32_t> tr_nrem;         // [T]
     std::vector<int32_t> ev_child, ev_parent;
-    std::vector<float> ev_bl;
+    std::vector<double> ev_bl;
     // the root path of each event's parent edge end at the time of the event
     // (local ids, parent first): events of a sweep touch exactly these nodes
     std::vector<int64_t> ev_path0;        // [events + 1]

@@ -1,0 +1,224 @@
+"""Serving loop: tree sequences resident on the GPU, statistics requests
+batched into fused sweeps.
+
+``StatServer`` keeps a set of tree sequences uploaded and answers requests
+(``Request``: a tree-sequence id, a statistic name and its keyword
+arguments, as for ``TreeSequence``).  Each ``step`` drains the queue with
+continuous batching: waiting requests are grouped by (tree sequence, mode)
+and packed into launches of at most ``max_columns`` distinct sample sets and
+``max_outputs`` outputs, admitted while the launch's estimated device memory
+stays within ``launch_budget`` bytes; a request that does not fit starts the
+next launch.  Allele frequency spectra run one per launch (they are not
+fused).  ``one_per_launch=True`` serves every request alone, the baseline the
+batched loop is measured against.
+
+Results do not depend on the company a request keeps: a fused sweep computes
+every output from its own columns in the same order as a sweep of that
+request alone.
+"""
+import collections
+import time
+
+import numpy as np
+
+FUSABLE = {"diversity", "divergence", "Fst", "Tajimas_D", "f2", "f3", "f4", "Y1", "Y2", "Y3",
+           "segregating_sites", "genetic_relatedness"}
+
+
+class Request:
+    """One statistics request: ``getattr(ts, stat)(**kwargs)`` on tree
+    sequence ``ts_id``."""
+
+    __slots__ = ("rid", "ts_id", "stat", "kwargs", "t_arrival", "t_start", "t_done", "result",
+                 "launch")
+
+    def __init__(self, rid, ts_id, stat, kwargs, t_arrival=0.0):
+        self.rid = rid
+        self.ts_id = ts_id
+        self.stat = stat
+        self.kwargs = kwargs
+        self.t_arrival = t_arrival
+        self.t_start = None
+        self.t_done = None
+        self.result = None
+        self.launch = None
+
+    @property
+    def mode(self):
+        return self.kwargs.get("mode", "site")
+
+    def shape(self):
+        """(sample sets, outputs, windows) of the request."""
+        sets = self.kwargs.get("sample_sets")
+        k = len(sets) if sets is not None else 1
+        idx = self.kwargs.get("indexes")
+        if self.stat in ("diversity", "segregating_sites", "Y1", "Tajimas_D"):
+            m = k
+        elif self.stat == "Fst":
+            m = k + (len(idx) if idx is not None else 1)
+        else:
+            m = len(idx) if idx is not None else 1
+        w = self.kwargs.get("windows")
+        nw = 1 if w is None or isinstance(w, str) else len(w) - 1
+        return k, m, nw
+
+
+def launch_bytes(k, m, nw, chunks):
+    """Device bytes one fused launch needs for a query of m outputs over nw
+    windows on a plan of `chunks` chunks: window partials (chunks + windows)
+    x m, the combined result and finals, and the per-chunk branch sums."""
+    return 8 * m * (3 * chunks + 3 * nw) + 64 * k
+
+
+class StatServer:
+    def __init__(self, tree_sequences, launch_budget=2 << 30, max_columns=64, max_outputs=4096,
+                 one_per_launch=False):
+        self.ts = list(tree_sequences)
+        for ts in self.ts:
+            ts.ll_tree_sequence.upload()
+        self.resident_bytes = sum(ts.ll_tree_sequence.get_device_bytes() for ts in self.ts)
+        self.chunks = [ts.ll_tree_sequence.get_hot_plan_sizes()["chunks"] for ts in self.ts]
+        self.launch_budget = launch_budget
+        self.max_columns = max_columns
+        self.max_outputs = max_outputs
+        self.one_per_launch = one_per_launch
+        self.queue = collections.deque()
+        self.launches = 0
+
+    def submit(self, req):
+        self.queue.append(req)
+
+    def _pack(self, reqs, chunks):
+        """Greedy packing of one group into launches under the column,
+        output and memory limits; requests keep their arrival order."""
+        launches, cur, cols, outs, mem = [], [], set(), 0, 0
+        for r in reqs:
+            k, m, nw = r.shape()
+            keys = {tuple(np.sort(np.asarray(s)).tolist()) for s in r.kwargs.get("sample_sets")}
+            b = launch_bytes(k, m, nw, chunks)
+            full = cur and (len(cols | keys) > self.max_columns or outs + m > self.max_outputs
+                            or mem + b > self.launch_budget)
+            if full or self.one_per_launch and cur:
+                launches.append(cur)
+                cur, cols, outs, mem = [], set(), 0, 0
+            cur.append(r)
+            cols |= keys
+            outs += m
+            mem += b
+        if cur:
+            launches.append(cur)
+        return launches
+
+    def step(self):
+        """Serves every request queued now; returns the requests completed."""
+        if not self.queue:
+            return []
+        pending = list(self.queue)
+        self.queue.clear()
+        groups = collections.OrderedDict()
+        alone = []
+        for r in pending:
+            if r.stat in FUSABLE:
+                groups.setdefault((r.ts_id, r.mode), []).append(r)
+            else:
+                alone.append([r])
+        work = []
+        for (tid, _), reqs in groups.items():
+            work.extend(self._pack(reqs, self.chunks[tid]))
+        work.extend(alone)
+        done = []
+        for launch in work:
+            self.launches += 1
+            t0 = time.perf_counter()
+            ts = self.ts[launch[0].ts_id]
+            if len(launch) == 1 and launch[0].stat not in FUSABLE:
+                r = launch[0]
+                r.t_start = t0
+                r.result = getattr(ts, r.stat)(**r.kwargs)
+            else:
+                b = ts.batch()
+                pend = [getattr(b, r.stat)(**r.kwargs) for r in launch]
+                b.run()
+                for r, p in zip(launch, pend):
+                    r.t_start = t0
+                    r.result = p.result()
+            t1 = time.perf_counter()
+            for r in launch:
+                r.t_done = t1
+                r.launch = self.launches
+            done.extend(launch)
+        return done
+
+    def run_open_loop(self, requests, arrivals):
+        """Replays `requests` arriving at `arrivals` (seconds from now):
+        every step serves what has arrived.  Returns the wall time."""
+        order = np.argsort(arrivals, kind="stable")
+        t_begin = time.perf_counter()
+        i = 0
+        n = len(order)
+        served = 0
+        while served < n:
+            now = time.perf_counter() - t_begin
+            while i < n and arrivals[order[i]] <= now:
+                r = requests[order[i]]
+                r.t_arrival = t_begin + arrivals[order[i]]
+                self.submit(r)
+                i += 1
+            if not self.queue:
+                time.sleep(max(0.0, min(1e-3, arrivals[order[i]] - now)))
+                continue
+            served += len(self.step())
+        return time.perf_counter() - t_begin
+
+
+def latency_stats(requests):
+    """p50 / p99 / mean latency (ms) from arrival to completion."""
+    lat = np.array([(r.t_done - r.t_arrival) * 1e3 for r in requests])
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean())}
+
+
+def random_requests(tree_sequences, count, seed, max_sets=64, max_set_size=10000,
+                    max_windows=10000, stats=("diversity", "divergence", "Fst", "f2", "f3", "f4",
+                                              "allele_frequency_spectrum")):
+    """Requests drawn uniformly: a tree sequence, a statistic, K sample sets
+    (disjoint, 10 .. max_set_size samples, clipped to fit), windows and a
+    mode.  f2/f3/f4 and divergence take random index tuples, Fst all pairs
+    of its first (up to 8) sets; the AFS takes 1 or 2 sets of at most 300
+    samples over at most 10 windows."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for rid in range(count):
+        tid = int(rng.integers(len(tree_sequences)))
+        ts = tree_sequences[tid]
+        n = ts.num_samples
+        stat = stats[int(rng.integers(len(stats)))]
+        need = {"f2": 2, "f3": 3, "f4": 4, "divergence": 2, "Fst": 2}.get(stat, 1)
+        k = int(rng.integers(need, max_sets + 1))
+        if stat == "allele_frequency_spectrum":
+            k = int(rng.integers(1, 3))
+        if stat == "Fst":
+            k = min(k, 8)
+        size = int(rng.integers(10, max_set_size + 1))
+        if stat == "allele_frequency_spectrum":  # prod(n_k + 1) bins per window
+            size = min(size, 300)
+        size = max(2, min(size, n // k))
+        k = min(k, n // size)
+        perm = rng.permutation(ts.samples())
+        sets = [perm[j * size:(j + 1) * size] for j in range(k)]
+        L = ts.sequence_length
+        nw = int(rng.integers(1, max_windows + 1))
+        windows = np.linspace(0, L, nw + 1)
+        mode = "branch" if rng.integers(2) else "site"
+        kw = {"sample_sets": sets, "windows": windows, "mode": mode}
+        if stat in ("divergence", "f2", "f3", "f4"):
+            t = {"divergence": 2, "f2": 2, "f3": 3, "f4": 4}[stat]
+            m = int(rng.integers(1, 9))
+            kw["indexes"] = [tuple(int(v) for v in rng.choice(k, t, replace=k < t))
+                             for _ in range(m)]
+        elif stat == "Fst":
+            kw["indexes"] = [(a, b) for a in range(k) for b in range(a + 1, k)]
+        if stat == "allele_frequency_spectrum":
+            kw["windows"] = np.linspace(0, L, min(nw, 10) + 1)
+        out.append(Request(rid, tid, stat, kw))
+    return out

@@ -192,11 +192,11 @@ class LowLevelTreeSequence:
         C, nl, ne, T, S, Mu = (int(x) for x in sizes)
         p = {
             "chunk_tree0": np.zeros(C + 1, np.int32), "chunk_node0": np.zeros(C + 1, np.int64),
-            "loc_parent0": np.zeros(nl, np.int32), "loc_bl0": np.zeros(nl, np.float32),
+            "loc_parent0": np.zeros(nl, np.int32), "loc_bl0": np.zeros(nl, np.float64),
             "loc_time": np.zeros(nl, np.float64), "loc_lastrem": np.zeros(nl, np.float64),
             "tr_off": np.zeros(T + 1, np.int64), "tr_nrem": np.zeros(T, np.int32),
             "ev_child": np.zeros(ne, np.int32), "ev_parent": np.zeros(ne, np.int32),
-            "ev_bl": np.zeros(ne, np.float32), "tree_site0": np.zeros(T + 1, np.int32),
+            "ev_bl": np.zeros(ne, np.float64), "tree_site0": np.zeros(T + 1, np.int32),
             "site_mut0": np.zeros(S + 1, np.int32), "mut_node": np.zeros(Mu, np.int32),
             "mut_allele": np.zeros(Mu, np.int32), "mut_pallele": np.zeros(Mu, np.int32),
             "site_nalleles": np.zeros(S, np.int32),
@@ -230,13 +230,13 @@ class LowLevelTreeSequence:
         n = self.get_num_samples()
         p = {
             "chunk_tree0": np.zeros(C + 1, np.int32), "chunk_slot0": np.zeros(C + 1, np.int64),
-            "slot_par": np.zeros(ns, np.int32), "slot_bl0": np.zeros(ns, np.float32),
+            "slot_par": np.zeros(ns, np.int32), "slot_bl0": np.zeros(ns, np.float64),
             "chunk_lev0": np.zeros(C + 1, np.int64), "lev_slot0": np.zeros(nlev, np.int32),
             "sample_slot": np.zeros(C * n, np.uint16), "tr_off": np.zeros(T + 1, np.int64),
             "tr_nrem": np.zeros(T, np.int32), "ev_child": np.zeros(ne, np.int32),
-            "ev_bl": np.zeros(ne, np.float32), "ev_path0": np.zeros(ne + 1, np.int64),
+            "ev_bl": np.zeros(ne, np.float64), "ev_path0": np.zeros(ne + 1, np.int64),
             "path": np.zeros(npath, np.int32), "mut_slot": np.zeros(Mu, np.int32),
-            "t0_parent": np.zeros(n0, np.int32), "t0_bl": np.zeros(n0, np.float32),
+            "t0_parent": np.zeros(n0, np.int32), "t0_bl": np.zeros(n0, np.float64),
         }
         check(self._lib.tsg_treeseq_copy_hot_plan(
             self._h, *[a.ctypes.data_as(ctypes.c_void_p) for a in p.values()]))
