@@ -11,7 +11,16 @@ step counts 2 * E * K_q updates (each edge inserted and removed once, for
 each of the K_q sample sets the query takes), whatever the implementation
 fuses or repeats internally; the CPU baseline counts the same.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
+Served metric (after the steps, unless --no-serve): 8 resident tree
+sequences -- the one above, one of the 1000-sample shape (L = 1e6, seed 1)
+and six of 2e3..7e4 samples (L = 2e6) -- and requests drawn uniformly (a
+tree sequence; diversity, divergence, Fst, f2, f3, f4 or the AFS; 1-64 sample
+sets of 10-1e4 samples; 1-1e4 windows; a mode).  Reported: queries/s of the
+batched serving loop with every request waiting (capacity) against one query
+per launch, and p50/p99 latency of both under Poisson arrivals at half the
+batched capacity.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--no-serve]
 """
 import argparse
 import json
@@ -101,6 +110,88 @@ def max_rel_err(gpu, cpu):
     return worst
 
 
+SERVE_SHAPES = [(1000, 1e6, 1), (2000, 2e6, 51), (5000, 2e6, 52), (10000, 2e6, 53),
+                (20000, 2e6, 54), (40000, 2e6, 55), (70000, 2e6, 56)]
+
+
+def served_phase(big_ts, count, seed):
+    """Capacity and latency of the serving loop (see the module docstring)."""
+    import tskit_amd as ta
+    from tskit_amd import serve
+
+    t0 = time.perf_counter()
+    tss = [ta.simulate_tables(n, L, 1e4, 1e-8, 1e-8, sd).tree_sequence()
+           for n, L, sd in SERVE_SHAPES]
+    tss.append(big_ts)
+    log(f"serve: 8 resident tree sequences generated in {time.perf_counter() - t0:.1f}s")
+
+    def capacity(one):
+        srv = serve.StatServer(tss, one_per_launch=one)
+        reqs = serve.random_requests(tss, count, seed)
+        t = time.perf_counter()
+        for r in reqs:
+            r.t_arrival = t
+            srv.submit(r)
+        while srv.queue:
+            srv.step()
+        wall = time.perf_counter() - t
+        return count / wall, srv.launches, srv.resident_bytes
+
+    capacity(False)  # warm every tree sequence and buffer
+    qps_b, launches_b, resident = capacity(False)
+    qps_1, launches_1, _ = capacity(True)
+    log(f"serve: capacity batched {qps_b:.1f} q/s ({launches_b} launches), one per launch "
+        f"{qps_1:.1f} q/s ({launches_1} launches)")
+    offered = 0.5 * qps_b
+    rng = np.random.default_rng(seed + 1)
+    out = {"requests": count, "resident": len(tss), "resident_bytes": resident,
+           "qps_batched": qps_b, "qps_one_per_launch": qps_1, "speedup": qps_b / qps_1,
+           "offered_qps": offered}
+    for one, tag in [(False, ""), (True, "_one_per_launch")]:
+        srv = serve.StatServer(tss, one_per_launch=one)
+        reqs = serve.random_requests(tss, count, seed + 2)
+        arrivals = np.cumsum(rng.exponential(1.0 / offered, count))
+        wall = srv.run_open_loop(reqs, arrivals)
+        st = serve.latency_stats(reqs)
+        out["p50_ms" + tag] = st["p50_ms"]
+        out["p99_ms" + tag] = st["p99_ms"]
+        out["served_qps" + tag] = count / wall
+        log(f"serve: offered {offered:.1f} q/s{tag.replace('_', ' ')}: p50 {st['p50_ms']:.2f} ms "
+            f"p99 {st['p99_ms']:.2f} ms")
+    return out, tss
+
+
+def served_accuracy(tss, om, kind, count=16, seed=9):
+    """Served results (fused launches) against the CPU implementation on the
+    smaller resident tree sequences: max |error| / max |value| per output."""
+    from tskit_amd import serve
+
+    small = [i for i, t in enumerate(tss) if t.num_samples <= 20000]
+    sub = [tss[i] for i in small]
+    reqs = serve.random_requests(sub, count, seed, stats=("diversity", "divergence", "f2",
+                                                          "f3", "f4"))
+    srv = serve.StatServer(sub)
+    for r in reqs:
+        srv.submit(r)
+    srv.step()
+    impls = {}
+    worst = 0.0
+    for r in reqs:
+        if r.ts_id not in impls:
+            tc = sub[r.ts_id].tables
+            impls[r.ts_id] = om.Reference(tc) if kind == "reference" else om.Oracle(tc)
+        kw = r.kwargs
+        exp = impls[r.ts_id].stat(r.stat, kw["sample_sets"], kw.get("indexes"), kw["windows"],
+                                  kw["mode"])
+        got = np.asarray(r.result, dtype=np.float64).reshape(exp.shape)
+        fin = np.isfinite(exp)
+        scale = np.broadcast_to(np.maximum(np.max(np.abs(np.where(fin, exp, 0)), axis=0), 1e-300),
+                                exp.shape)
+        if np.any(fin):
+            worst = max(worst, float(np.max(np.abs(got - exp)[fin] / scale[fin])))
+    return worst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +200,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="the 1000-sample workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--chunks", type=int, default=0, help="override the chunk count")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served phase")
+    ap.add_argument("--serve-requests", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -188,6 +281,10 @@ def main():
                 "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                 "site_sweep_ms": round(mode_ms["site"], 4)}
 
+    served, serve_tss = None, None
+    if not args.no_serve and not args.small:
+        served, serve_tss = served_phase(ts, args.serve_requests, 7)
+
     cpu = None
     errs = {}
     if not args.no_cpu:
@@ -202,6 +299,10 @@ def main():
                          "on the same tree sequence, single thread"}
         errs = max_rel_err(gpu_out, cpu_out)
         log(f"cpu step {t_cpu:.2f}s ({kind}); max rel err {max(errs.values()):.2e}")
+        if served is not None:
+            served["max_rel_err_vs_cpu"] = served_accuracy(serve_tss, om, kind)
+            log(f"serve: max rel err of sampled served results vs CPU "
+                f"{served['max_rel_err_vs_cpu']:.2e}")
 
     line = {
         "metric": "edge×sample-set updates/s; served stat queries/s + p99 at 1e-6 rel err",
@@ -224,6 +325,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "max_rel_err_vs_cpu": errs,
+        "served": served,
         "setup_s": {"generate": round(t_gen, 2), "host_init": round(t_init, 2),
                     "upload": round(t_upload, 3)},
     }

@@ -369,7 +369,10 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     // largest chunk fits, else in per-chunk HBM slabs.
     const int R = TSG_NODE_WORDS(K);
     // the sweep's LDS staging block for the widest launch (512 outputs)
-    const size_t stage = (size_t) TSG_STAGE_INTS(K, std::min(M, 512)) * 4;
+    // outputs per launch: up to 256 (four per lane; wider launches would
+    // spill the per-output polynomial coefficients out of registers)
+    const int slice = 256;
+    const size_t stage = (size_t) TSG_STAGE_INTS(K, std::min(M, slice)) * 4;
     // 16-bit slot counts when no sample set exceeds 65535 samples
     const bool c16 = !ws->force_c32 &&
                      *std::max_element(col_total.begin(), col_total.end()) <= 65535;
@@ -510,11 +513,11 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     a.g_stride = (int64_t) ts->max_nloc * R;
     const int mode_id = (mode == TSG_STAT_BRANCH) ? 0 : 1;
     if (ws->profile) (void) hipEventRecord(ws->ev0, s);
-    for (int m0 = 0; m0 < M; m0 += 512) {
+    for (int m0 = 0; m0 < M; m0 += slice) {
         a.m_begin = m0;
-        a.m_end = std::min(M, m0 + 512);
+        a.m_end = std::min(M, m0 + slice);
         const int span = a.m_end - a.m_begin;
-        const int mpl = span <= 64 ? 1 : span <= 128 ? 2 : span <= 256 ? 4 : 8;
+        const int mpl = span <= 64 ? 1 : span <= 128 ? 2 : 4;
         if ((ret = hip_check(tsg_launch_sweep(a, mode_id, store, mpl, shmem, s)))) return ret;
     }
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);

@@ -1408,10 +1408,8 @@ hipError_t launch_sweep_mpl(const TsgSweepArgs &a, int mpl, size_t shmem, hipStr
         return launch_one<MODE, STORE, 1>(a, shmem, s);
     case 2:
         return launch_one<MODE, STORE, 2>(a, shmem, s);
-    case 4:
-        return launch_one<MODE, STORE, 4>(a, shmem, s);
     default:
-        return launch_one<MODE, STORE, 8>(a, shmem, s);
+        return launch_one<MODE, STORE, 4>(a, shmem, s);
     }
 }
 
