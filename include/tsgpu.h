@@ -95,13 +95,22 @@ uint64_t tsg_treeseq_get_path_size(const tsg_treeseq_t *ts);
 int tsg_treeseq_copy_paths(const tsg_treeseq_t *ts, int64_t *ev_path0, int32_t *path);
 /* host copy of the hot plan (chunks keep only the nodes they touch, as LDS
  * slots); sizes = {usable, chunks, slots, level entries, events, path
- * entries, first-tree nodes, max slots per chunk, max levels per chunk} */
+ * entries, first-tree nodes, max slots per chunk, max levels per chunk,
+ * last-tree nodes} */
 int tsg_treeseq_get_hot_plan_sizes(const tsg_treeseq_t *ts, uint64_t *sizes);
 int tsg_treeseq_copy_hot_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0,
     int64_t *chunk_slot0, int32_t *slot_par, double *slot_bl0, int64_t *chunk_lev0,
     int32_t *lev_slot0, uint16_t *sample_slot, int64_t *tr_off, int32_t *tr_nrem,
     int32_t *ev_child, double *ev_bl, int64_t *ev_path0, int32_t *path, int32_t *mut_slot,
     int32_t *t0_parent, double *t0_bl);
+/* host copy of the hot plan's branch-AFS tables: per slot its node, the
+ * samples it is the nearest hot ancestor-or-self of, the position of its
+ * node's last flush by an event before the chunk and of the insertion of its
+ * current parent edge; the last tree (samples first: parent, node time and
+ * the same two positions per node) */
+int tsg_treeseq_copy_hot_afs_plan(const tsg_treeseq_t *ts, int32_t *slot_node,
+    int32_t *slot_nsd, double *slot_last0, double *slot_ins0, double *slot_tp0,
+    int32_t *tl_parent, double *tl_time, double *tl_last, double *tl_ins);
 /* plan = 1: run statistics sweeps on the full plan; 2: on the hot plan with
  * 32-bit counts; 0: the library chooses (the hot plan whenever its slots fit
  * in LDS, 16-bit counts when every set has <= 65535 samples) -- a testing

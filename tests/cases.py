@@ -128,6 +128,41 @@ def generated(n, L, seed, mu=1e-8):
     return ta.simulate_tables(n, L, 1e4, 1e-8, mu, seed)
 
 
+@functools.lru_cache(maxsize=None)
+def gapped(n, L, seed, count=40):
+    """A generated tree sequence with a gap cut into `count` random edges:
+    over each gap the edge's child (a sample or the root of a subtree) has no
+    parent and later regains the same one, so trees have several roots and
+    samples are missing from them for a while."""
+    src = generated(n, L, seed)
+    tc = ta.TableCollection(src.sequence_length)
+    tc.set_nodes(src.node_flags, src.node_time)
+    rng = np.random.default_rng(seed + 1000)
+    left, right = src.edge_left.copy(), src.edge_right.copy()
+    par, child = src.edge_parent.copy(), src.edge_child.copy()
+    span = right - left
+    pick = rng.choice(np.flatnonzero(span > span.mean()), size=count, replace=False)
+    add = []
+    for e in pick:
+        g0, g1 = np.sort(rng.uniform(left[e], right[e], 2)).round(3)
+        if not left[e] < g0 < g1 < right[e]:
+            continue
+        add.append((g1, right[e], par[e], child[e]))
+        right[e] = g0
+    if add:
+        a = np.array(add)
+        left, right = np.append(left, a[:, 0]), np.append(right, a[:, 1])
+        par, child = np.append(par, a[:, 2].astype(np.int32)), np.append(child, a[:, 3].astype(np.int32))
+    tc.set_edges(left, right, par, child)
+    tc.sort_edges()
+    tc.site_position = src.site_position.copy()
+    tc.ancestral_state, tc.ancestral_state_offset = src.ancestral_state, src.ancestral_state_offset
+    tc.mutation_site, tc.mutation_node = src.mutation_site.copy(), src.mutation_node.copy()
+    tc.mutation_parent = src.mutation_parent.copy()
+    tc.derived_state, tc.derived_state_offset = src.derived_state, src.derived_state_offset
+    return tc
+
+
 def config1():
     """The small workload: 1000 samples, L=1e6, seed 1."""
     return generated(1000, 1e6, 1)

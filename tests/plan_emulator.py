@@ -349,3 +349,154 @@ def run_hot(hp, plan_sites, bp, windows, sets, num_samples, stat, mode, idx=None
                 if ov > 0:
                     out[w] += Rs[c] * ov
     return out
+
+
+def _fold(coord, dims):
+    """Minor-allele fold of an AFS coordinate (trees.c:3469-3496)."""
+    n = sum(d - 1 for d in dims) / 2
+    s = sum(coord)
+    k = len(dims)
+    while s == n and k > 0:
+        k -= 1
+        n -= (dims[k] - 1) / 2
+        s -= coord[k]
+    if s > n:
+        return [d - 1 - c for c, d in zip(coord, dims)]
+    return list(coord)
+
+
+def run_hot_afs(hp, plan_sites, bp, windows, time_windows, sets, num_samples, node_time, mode,
+                polarised=False):
+    """The hot-plan AFS program as the kernels run it, in FP64: slot counts
+    (sample sets, then all samples seeded from slot_nsd), events through the
+    recorded paths; branch mode flushes a node only when an event changes it
+    (as a removed edge's child or a node on an event's path), splitting the
+    interval since its last flush at window boundaries, and a last-tree pass
+    flushes the intervals still open at the end.  Returns the span-normalised
+    [W, TW, dims...] spectrum (TW = 1 for sites)."""
+    ct0, cs0, cl0 = hp["chunk_tree0"], hp["chunk_slot0"], hp["chunk_lev0"]
+    C = len(ct0) - 1
+    T = len(bp) - 1
+    L = bp[-1]
+    Ks = len(sets)
+    K1 = Ks + 1
+    dims = [len(s) + 1 for s in sets]
+    W = len(windows) - 1
+    tw = np.asarray([0, np.inf] if time_windows is None else time_windows, np.float64)
+    TW = len(tw) - 1 if mode == "branch" else 1
+    hist = np.zeros((W, TW) + tuple(dims))
+    n = num_samples
+    weight = np.zeros((n, K1))
+    for k, s in enumerate(sets):
+        weight[np.asarray(s), k] += 1
+    weight[:, Ks] = 1
+    total = weight.sum(axis=0)
+
+    def coord(x):
+        c = [int(v) for v in x[:Ks]]
+        return tuple(c if polarised else _fold(c, dims))
+
+    def flush_pieces(x_counts, tu, tp, a, y, x):
+        """Contribution of a node with counts x_counts, own time tu and
+        parent time tp (-inf: none; its edge inserted at y) over [a, x)."""
+        if not (tp > -np.inf) or not (0 < x_counts[Ks] < n):
+            return
+        off = coord(x_counts)
+        wi = int(np.searchsorted(windows, a, side="right")) - 1
+        p = a
+        pieces = []
+        while wi < W and windows[wi + 1] <= x:
+            q = windows[wi + 1]
+            if y < q:
+                pieces.append((wi, q - p))
+            p = q
+            wi += 1
+        if wi < W and x > p:
+            pieces.append((wi, x - p))
+        for w, span in pieces:
+            j = 0
+            while j < TW and tw[j] < tp:
+                twbl = max(0.0, min(tw[j + 1], tp) - max(tw[j], tu))
+                hist[(w, j) + off] += span * twbl
+                j += 1
+
+    for c in range(C):
+        S = cs0[c + 1] - cs0[c]
+        sl = slice(cs0[c], cs0[c + 1])
+        par = hp["slot_par"][sl]
+        X = np.zeros((S, K1))
+        ss = hp["sample_slot"][c * n:(c + 1) * n]
+        for s in range(n):
+            if ss[s] != NO_SLOT:
+                X[ss[s], :Ks] += weight[s, :Ks]
+        X[:, Ks] = hp["slot_nsd"][sl]
+        lev = hp["lev_slot0"][cl0[c]:cl0[c + 1]]
+        for j in range(len(lev) - 2):
+            for h in range(lev[j], lev[j + 1]):
+                if par[h] >= 0:
+                    X[par[h]] += X[h]
+        tu = node_time[hp["slot_node"][sl]]
+        tp = hp["slot_tp0"][sl].copy()
+        last = hp["slot_last0"][sl].copy()
+        ins = hp["slot_ins0"][sl].copy()
+
+        def flush(h, x):
+            flush_pieces(X[h], tu[h], tp[h], last[h], ins[h], x)
+            last[h] = x
+
+        def transition(t):
+            x = bp[t]
+            e0, e1 = hp["tr_off"][t], hp["tr_off"][t + 1]
+            nrem = hp["tr_nrem"][t]
+            for e in range(e0, e1):
+                is_ins = (e - e0) >= nrem
+                u = hp["ev_child"][e]
+                path = hp["path"][hp["ev_path0"][e]:hp["ev_path0"][e + 1]]
+                xc = X[u].copy()
+                if mode == "branch" and not is_ins:
+                    flush(u, x)
+                for w in path:
+                    if mode == "branch":
+                        flush(w, x)
+                    X[w] = X[w] + xc if is_ins else X[w] - xc
+                if is_ins:
+                    tp[u] = tu[path[0]]
+                    ins[u] = x
+                else:
+                    tp[u] = -np.inf
+
+        t0, t1 = ct0[c], ct0[c + 1]
+        for t in range(t0, t1):
+            if t > t0:
+                transition(t)
+            if mode == "site":
+                ps = plan_sites
+                for s in range(ps["tree_site0"][t], ps["tree_site0"][t + 1]):
+                    wi = np.searchsorted(windows, ps["_site_pos"][s], side="right") - 1
+                    m0, m1 = ps["site_mut0"][s], ps["site_mut0"][s + 1]
+                    for al in range(1 if polarised else 0, ps["site_nalleles"][s]):
+                        aw = total.copy() if al == 0 else np.zeros(K1)
+                        for mm in range(m0, m1):
+                            h = hp["mut_slot"][mm]
+                            if ps["mut_allele"][mm] == al:
+                                aw += X[h]
+                            if ps["mut_pallele"][mm] == al:
+                                aw -= X[h]
+                        if 0 < aw[Ks] < n:
+                            hist[(wi, 0) + coord(aw)] += 1.0 if polarised else 0.5
+        if mode == "branch" and t1 < T:
+            transition(t1)
+    if mode == "branch":
+        tlp = hp["tl_parent"]
+        XL = np.zeros((len(tlp), K1))
+        for s in range(n):
+            u = s
+            while u >= 0:
+                XL[u] += weight[s]
+                u = tlp[u]
+        for q in range(len(tlp)):
+            if tlp[q] >= 0:
+                flush_pieces(XL[q], hp["tl_time"][q], hp["tl_time"][tlp[q]], hp["tl_last"][q],
+                             hp["tl_ins"][q], L)
+    span = np.diff(windows)
+    return hist / span.reshape((W,) + (1,) * (hist.ndim - 1))

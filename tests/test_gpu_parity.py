@@ -259,15 +259,33 @@ def test_paper_ex_known_answers(require_gpu):
     assert list(afs[0, 0]) == [0, 2, 0, 1, 0]
 
 
+def _afs_plan(ll, plan):
+    """1: the full plan (every node of each chunk); 0: the library's choice,
+    the hot plan here."""
+    ll.set_sweep_plan(1 if plan == "full" else 0)
+
+
+def _afs_store_ok(ll, plan):
+    store = ll.get_last_store()
+    assert (store == 2) if plan == "hot" else store in (0, 1)
+
+
+@pytest.mark.parametrize("plan", ["hot", "full"])
 @pytest.mark.parametrize("mode", ["site", "branch"])
 @pytest.mark.parametrize("nsets", [1, 2, 3])
 @pytest.mark.parametrize("polarised", [False, True])
-def test_afs(small, mode, nsets, polarised):
+def test_afs(small, plan, mode, nsets, polarised):
     tc, ts, orc = small
+    ll = ts.ll_tree_sequence
     sets = cases.sample_sets(200, nsets, 13, 8)
     w = np.linspace(0, tc.sequence_length, 5)
-    _say("afs", mode, nsets, polarised)
-    got = ts.allele_frequency_spectrum(sets, windows=w, mode=mode, polarised=polarised)
+    _say("afs", plan, mode, nsets, polarised)
+    _afs_plan(ll, plan)
+    try:
+        got = ts.allele_frequency_spectrum(sets, windows=w, mode=mode, polarised=polarised)
+        _afs_store_ok(ll, plan)
+    finally:
+        ll.set_sweep_plan(0)
     exp = orc.afs(sets, w, None, mode, True, polarised)[:, 0]
     if mode == "site":
         assert np.array_equal(got, exp)
@@ -276,15 +294,53 @@ def test_afs(small, mode, nsets, polarised):
         assert rel_err(got, exp, np.maximum(scale, 1e-300)) <= TOL
 
 
-def test_afs_time_windows(small):
+@pytest.mark.parametrize("plan", ["hot", "full"])
+def test_afs_time_windows(small, plan):
     tc, ts, orc = small
+    ll = ts.ll_tree_sequence
     sets = cases.sample_sets(200, 2, 20, 9)
     w = np.linspace(0, tc.sequence_length, 3)
     tw = np.array([0, 2000, 10000, np.inf])
-    got = ts.allele_frequency_spectrum(sets, windows=w, time_windows=tw, mode="branch")
+    _say("afs time windows", plan)
+    _afs_plan(ll, plan)
+    try:
+        got = ts.allele_frequency_spectrum(sets, windows=w, time_windows=tw, mode="branch")
+        _afs_store_ok(ll, plan)
+    finally:
+        ll.set_sweep_plan(0)
     exp = orc.afs(sets, w, tw, "branch", True, False)
     scale = np.max(np.abs(exp))
     assert rel_err(got, exp, scale) <= TOL
+
+
+@pytest.mark.parametrize("plan", ["hot", "full"])
+@pytest.mark.parametrize("polarised", [False, True])
+def test_afs_gapped(require_gpu, plan, polarised):
+    """Trees with several roots and samples missing over gaps: nodes lose
+    their parent edge and regain it later, the case where the reference's
+    update rule (an inserted edge's child keeps its last update position)
+    shapes the branch spectrum; many windows and chunks."""
+    tc = cases.gapped(200, 2e5, 3)
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    ll.set_num_chunks(7)
+    orc = Oracle(tc)
+    sets = cases.sample_sets(200, 2, 66, 3)
+    w = np.linspace(0, tc.sequence_length, 23)
+    tw = np.array([0, 500, 3000, np.inf])
+    _afs_plan(ll, plan)
+    for mode, t in [("site", None), ("branch", None), ("branch", tw)]:
+        _say("afs gapped", plan, mode, t is not None, polarised)
+        got = ts.allele_frequency_spectrum(sets, windows=w, time_windows=t, mode=mode,
+                                           polarised=polarised)
+        _afs_store_ok(ll, plan)
+        exp = orc.afs(sets, w, t, mode, True, polarised)
+        if t is None:
+            exp = exp[:, 0]
+        if mode == "site":
+            assert np.array_equal(got, exp)
+        else:
+            assert rel_err(got, exp, np.max(np.abs(exp))) <= TOL
 
 
 @pytest.mark.parametrize("windows", [None, "three"])

@@ -254,6 +254,62 @@ def test_hot_plan_executes_to_oracle(n, L, seed, chunks):
                 assert np.max(np.abs(got - exp)) / np.max(np.abs(exp)) < 1e-10
 
 
+@pytest.mark.parametrize("case,n,L,seed,chunks", [
+    ("generated", 60, 5e4, 5, 1), ("generated", 200, 2e5, 3, 7),
+    ("generated", 120, 1e5, 11, 100000), ("gapped", 200, 2e5, 3, 7),
+    ("gapped", 120, 1e5, 11, None),
+])
+def test_hot_plan_afs_executes_to_oracle(case, n, L, seed, chunks):
+    """The hot-plan AFS program (flush on update only, pieces split at window
+    boundaries, last-tree pass) reproduces the oracle; the gapped cases have
+    nodes that lose and regain their parent edge, where the reference's
+    update rule for an inserted edge's child matters."""
+    tc = getattr(cases, case)(n, L, seed)
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    if chunks:
+        ll.set_num_chunks(chunks)
+    hp = ll.get_hot_plan()
+    full = ll.get_plan()
+    sites = {k: full[k] for k in ("tree_site0", "site_mut0", "mut_allele", "mut_pallele",
+                                  "site_nalleles")}
+    sites["_site_pos"] = tc.site_position
+    bp = ts.breakpoints(as_array=True)
+    orc = Oracle(tc)
+    sets = cases.sample_sets(n, 2, n // 3, seed)
+    w = np.linspace(0, L, 23)
+    for mode, tw in [("site", None), ("branch", None), ("branch", [0, 500, 3000, np.inf])]:
+        for pol in [False, True]:
+            got = plan_emulator.run_hot_afs(hp, sites, bp, w, tw, sets, n, tc.node_time, mode,
+                                            pol)
+            exp = orc.afs(sets, w, tw, mode, True, pol)
+            if mode == "site":
+                assert np.array_equal(got, exp)
+            else:
+                assert np.max(np.abs(got - exp)) / np.max(np.abs(exp)) < 1e-12
+
+
+def test_hot_plan_afs_tables():
+    """slot_nsd counts each sample at its nearest hot ancestor-or-self, and
+    the last tree's parent pointers and times match the oracle's last tree."""
+    tc = cases.gapped(120, 1e5, 11)
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    ll.set_num_chunks(9)
+    hp = ll.get_hot_plan()
+    n = ts.num_samples
+    cs0 = hp["chunk_slot0"]
+    for c in range(len(cs0) - 1):
+        ss = hp["sample_slot"][c * n:(c + 1) * n]
+        nsd = np.bincount(ss[ss != plan_emulator.NO_SLOT], minlength=cs0[c + 1] - cs0[c])
+        assert np.array_equal(nsd, hp["slot_nsd"][cs0[c]:cs0[c + 1]])
+    tlp, tlt = hp["tl_parent"], hp["tl_time"]
+    assert np.all(tlt[:n] == tc.node_time[:n])
+    assert np.all((tlp == -1) | (tlt[np.maximum(tlp, 0)] > tlt))
+    assert np.all(hp["tl_last"] >= 0) and np.all(hp["tl_last"] < tc.sequence_length)
+    assert np.all(hp["slot_last0"] >= 0) and np.all(hp["slot_ins0"] >= 0)
+
+
 def test_hot_plan_bounds_config1():
     tc = cases.config1()
     ts = tc.tree_sequence()

@@ -60,6 +60,7 @@ SIGNATURES = {
     "tsg_treeseq_copy_paths": (ctypes.c_int, [c_void, c_void, c_void]),
     "tsg_treeseq_get_hot_plan_sizes": (ctypes.c_int, [c_void, c_u64p]),
     "tsg_treeseq_copy_hot_plan": (ctypes.c_int, [c_void] + [c_void] * 16),
+    "tsg_treeseq_copy_hot_afs_plan": (ctypes.c_int, [c_void] + [c_void] * 9),
     "tsg_treeseq_set_sweep_plan": (ctypes.c_int, [c_void, ctypes.c_int]),
     "tsg_treeseq_get_last_store": (ctypes.c_int, [c_void]),
     "tsg_treeseq_set_profiling": (ctypes.c_int, [c_void, ctypes.c_int]),

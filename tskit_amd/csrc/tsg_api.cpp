@@ -709,6 +709,29 @@ int ensure_full(tsg_treeseq *ts) {
     return 0;
 }
 
+// The hot plan's allele-frequency-spectrum tables, uploaded on first need.
+int ensure_hot_afs(tsg_treeseq *ts) {
+    TsgHotDevice &h = ts->hot.dev;
+    if (h.slot_nsd != nullptr) return 0;
+    const tsg_hotplan &hp = ts->hot;
+    std::vector<double> slot_tu(hp.slot_node.size());
+    for (size_t q = 0; q < slot_tu.size(); q++) slot_tu[q] = ts->time[hp.slot_node[q]];
+    size_t &hb = h.bytes;
+    int ret;
+    if ((ret = upload_ptr(&h.slot_tu, slot_tu, hb)) ||
+        (ret = upload_ptr(&h.slot_last0, hp.slot_last0, hb)) ||
+        (ret = upload_ptr(&h.slot_ins0, hp.slot_ins0, hb)) ||
+        (ret = upload_ptr(&h.slot_tp0, hp.slot_tp0, hb)) ||
+        (ret = upload_ptr(&h.tl_parent, hp.tl_parent, hb)) ||
+        (ret = upload_ptr(&h.tl_time, hp.tl_time, hb)) ||
+        (ret = upload_ptr(&h.tl_last, hp.tl_last, hb)) ||
+        (ret = upload_ptr(&h.tl_ins, hp.tl_ins, hb)) ||
+        (ret = upload_ptr(&h.slot_nsd, hp.slot_nsd, hb))) {
+        return ret;
+    }
+    return 0;
+}
+
 void free_workspace(tsg_treeseq *ts) {
     Workspace *ws = ws_of(ts);
     if (ws == nullptr) return;
@@ -746,7 +769,9 @@ void tsg_device_free(TsgDevice *d) {
 void tsg_hot_device_free(TsgHotDevice *h) {
     void *ptrs[] = {h->chunk_tree0, h->chunk_slot0, h->slot_par, h->slot_bl0, h->chunk_lev0,
                     h->lev_slot0, h->sample_slot, h->tr_off, h->tr_nrem, h->ev_child, h->ev_bl,
-                    h->ev_path0, h->path, h->mut_slot, h->t0_parent, h->t0_bl};
+                    h->ev_path0, h->path, h->mut_slot, h->t0_parent, h->t0_bl,
+                    h->slot_nsd, h->slot_tu, h->slot_last0, h->slot_ins0, h->slot_tp0,
+                    h->tl_parent, h->tl_time, h->tl_last, h->tl_ins};
     for (void *p : ptrs) {
         if (p != nullptr) (void) hipFree(p);
     }
@@ -1035,6 +1060,7 @@ int tsg_treeseq_get_hot_plan_sizes(const tsg_treeseq *ts, uint64_t *sizes) {
     sizes[6] = hp.t0_parent.size();
     sizes[7] = (uint64_t) hp.max_slots;
     sizes[8] = (uint64_t) hp.max_levels;
+    sizes[9] = hp.tl_parent.size();
     return 0;
 }
 
@@ -1063,6 +1089,25 @@ int tsg_treeseq_copy_hot_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64
     cp(mut_slot, hp.mut_slot);
     cp(t0_parent, hp.t0_parent);
     cp(t0_bl, hp.t0_bl);
+    return 0;
+}
+
+int tsg_treeseq_copy_hot_afs_plan(const tsg_treeseq *ts, int32_t *slot_node, int32_t *slot_nsd,
+                                  double *slot_last0, double *slot_ins0, double *slot_tp0,
+                                  int32_t *tl_parent, double *tl_time, double *tl_last,
+                                  double *tl_ins) {
+    const tsg_hotplan &hp = ts->hot;
+    if (!hp.usable) return TSG_ERR_UNSUPPORTED_INPUT;
+    auto cp = [](auto *dst, const auto &v) { std::copy(v.begin(), v.end(), dst); };
+    cp(slot_node, hp.slot_node);
+    cp(slot_nsd, hp.slot_nsd);
+    cp(slot_last0, hp.slot_last0);
+    cp(slot_ins0, hp.slot_ins0);
+    cp(slot_tp0, hp.slot_tp0);
+    cp(tl_parent, hp.tl_parent);
+    cp(tl_time, hp.tl_time);
+    cp(tl_last, hp.tl_last);
+    cp(tl_ins, hp.tl_ins);
     return 0;
 }
 
@@ -1539,11 +1584,18 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
     const int64_t afs_size = (int64_t) afs_size_d;
 
     std::lock_guard<std::mutex> lock(g_mutex);
-    if ((ret = ensure_full(ts))) return ret;
+    if ((ret = device_upload(ts))) return ret;
     Workspace *ws = ws_of(ts);
     hipStream_t s = ws->stream;
-    const int C = ts->num_chunks;
     const int KK = (int) K + 1;
+    // The hot plan when its slots fit in LDS: per slot KK counts and, for
+    // branch lengths, four doubles (node time, parent time, last flush,
+    // insertion); then one staging row.
+    const size_t hot_need = (size_t) ts->hot.max_slots * ((size_t) KK * 4 + (branch ? 32 : 0)) +
+                            (size_t) KK * 4 + 64;
+    const bool use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
+    if ((ret = use_hot ? ensure_hot_afs(ts) : ensure_full(ts))) return ret;
+    const int C = use_hot ? ts->hot.num_chunks : ts->num_chunks;
     std::vector<int32_t> pair_s, pair_c, col_total;
     for (uint64_t k = 0; k < K; k++) {
         for (int32_t si : ss[k]) {
@@ -1552,59 +1604,94 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
         }
         col_total.push_back((int32_t) ss[k].size());
     }
-    for (int32_t si = 0; si < ts->num_samples; si++) {
-        pair_s.push_back(si);
-        pair_c.push_back((int32_t) K);
+    if (!use_hot) {  // the hot plan seeds the all-samples column from slot_nsd
+        for (int32_t si = 0; si < ts->num_samples; si++) {
+            pair_s.push_back(si);
+            pair_c.push_back((int32_t) K);
+        }
     }
     col_total.push_back(ts->num_samples);
-    std::vector<int32_t> cq_wfirst(C), cq_nwin(C);
-    std::vector<int64_t> cq_poff(C, 0);
-    for (int c = 0; c < C; c++) {
-        const double cs = ts->breakpoints[ts->chunk_tree0[c]];
-        const double ce = ts->breakpoints[ts->chunk_tree0[c + 1]];
-        const int wf = (int) (std::upper_bound(win.begin(), win.end(), cs) - win.begin()) - 1;
-        const int wl = (int) (std::lower_bound(win.begin(), win.end(), ce) - win.begin()) - 1;
-        cq_wfirst[c] = wf;
-        cq_nwin[c] = wl - wf + 1;
-    }
     const int64_t hist_n = (int64_t) W * TW * afs_size;
     TsgSweepArgs a{};
     if ((ret = upload_seeds(ws, pair_s, pair_c, ts->num_samples, s, a)) ||
         (ret = upload(ws->tot, col_total, s)) || (ret = upload(ws->windows, win, s)) ||
         (ret = upload(ws->twin, tw, s)) || (ret = upload(ws->dims, dims, s)) ||
-        (ret = upload(ws->cq_wfirst, cq_wfirst, s)) || (ret = upload(ws->cq_nwin, cq_nwin, s)) ||
-        (ret = upload(ws->cq_poff, cq_poff, s)) || (ret = ensure(ws->hist, (size_t) hist_n * 8)) ||
+        (ret = ensure(ws->hist, (size_t) hist_n * 8)) ||
         (ret = ensure(ws->out, (size_t) hist_n * 8))) {
         return ret;
     }
+    const TsgDevice &d = ts->dev;
+    const TsgHotDevice &h = ts->hot.dev;
     const int R = TSG_NODE_WORDS(KK);
     const size_t stage = (size_t) 2 * KK * 4 + 64;
     size_t lds_need = (size_t) ts->max_nloc * R * 4 + stage;
     if (branch) lds_need += (size_t) ts->max_nloc * 8;
-    const bool lds = lds_need <= (size_t) ws->lds_limit;
-    if (!lds) {
-        if ((ret = ensure(ws->slab, (size_t) C * ts->max_nloc * R * 4))) return ret;
-        if (branch && (ret = ensure(ws->last, (size_t) C * ts->max_nloc * 8))) return ret;
+    const bool lds = use_hot || lds_need <= (size_t) ws->lds_limit;
+    if (use_hot) {
+        a.chunk_tree0 = h.chunk_tree0;
+        a.chunk_node0 = h.chunk_slot0;
+        a.tr_off = h.tr_off;
+        a.tr_nrem = h.tr_nrem;
+        a.ev_child = h.ev_child;
+        a.ev_path0 = h.ev_path0;
+        a.path = h.path;
+        a.mut_node = h.mut_slot;
+        a.slot_par = h.slot_par;
+        a.chunk_lev0 = h.chunk_lev0;
+        a.lev_slot0 = h.lev_slot0;
+        a.sample_slot = h.sample_slot;
+        a.slot_nsd = h.slot_nsd;
+        a.slot_tu = h.slot_tu;
+        a.slot_tp0 = h.slot_tp0;
+        a.slot_last0 = h.slot_last0;
+        a.slot_ins0 = h.slot_ins0;
+        ws->last_store = TSG_STORE_HOT;
+    } else {
+        std::vector<int32_t> cq_wfirst(C), cq_nwin(C);
+        std::vector<int64_t> cq_poff(C, 0);
+        for (int c = 0; c < C; c++) {
+            const double cs = ts->breakpoints[ts->chunk_tree0[c]];
+            const double ce = ts->breakpoints[ts->chunk_tree0[c + 1]];
+            const int wf = (int) (std::upper_bound(win.begin(), win.end(), cs) - win.begin()) - 1;
+            const int wl = (int) (std::lower_bound(win.begin(), win.end(), ce) - win.begin()) - 1;
+            cq_wfirst[c] = wf;
+            cq_nwin[c] = wl - wf + 1;
+        }
+        if ((ret = upload(ws->cq_wfirst, cq_wfirst, s)) ||
+            (ret = upload(ws->cq_nwin, cq_nwin, s)) || (ret = upload(ws->cq_poff, cq_poff, s))) {
+            return ret;
+        }
+        if (!lds) {
+            if ((ret = ensure(ws->slab, (size_t) C * ts->max_nloc * R * 4))) return ret;
+            if (branch && (ret = ensure(ws->last, (size_t) C * ts->max_nloc * 8))) return ret;
+        }
+        a.chunk_tree0 = d.chunk_tree0;
+        a.chunk_node0 = d.chunk_node0;
+        a.loc_parent0 = d.loc_parent0;
+        a.loc_bl0 = d.loc_bl0;
+        a.loc_time = d.loc_time;
+        a.loc_lastrem = d.loc_lastrem;
+        a.tr_off = d.tr_off;
+        a.tr_nrem = d.tr_nrem;
+        a.ev_child = d.ev_child;
+        a.ev_parent = d.ev_parent;
+        a.ev_bl = d.ev_bl;
+        a.ev_path0 = d.ev_path0;
+        a.path = d.path;
+        a.mut_node = d.mut_node;
+        a.cq_wfirst = (const int32_t *) ws->cq_wfirst.p;
+        a.cq_nwin = (const int32_t *) ws->cq_nwin.p;
+        a.cq_poff = (const int64_t *) ws->cq_poff.p;
+        a.g_rec = (int32_t *) ws->slab.p;
+        a.g_stride = (int64_t) ts->max_nloc * R;
+        a.g_last = (double *) ws->last.p;
+        a.g_last_stride = ts->max_nloc;
+        ws->last_store = lds ? TSG_STORE_LDS : TSG_STORE_GLOBAL;
     }
-    const TsgDevice &d = ts->dev;
-    a.chunk_tree0 = d.chunk_tree0;
-    a.chunk_node0 = d.chunk_node0;
-    a.loc_parent0 = d.loc_parent0;
-    a.loc_bl0 = d.loc_bl0;
-    a.loc_time = d.loc_time;
-    a.loc_lastrem = d.loc_lastrem;
-    a.tr_off = d.tr_off;
-    a.tr_nrem = d.tr_nrem;
-    a.ev_child = d.ev_child;
-    a.ev_parent = d.ev_parent;
-    a.ev_bl = d.ev_bl;
-    a.ev_path0 = d.ev_path0;
-    a.path = d.path;
     a.bp = d.breakpoints;
     a.tree_site0 = d.tree_site0;
     a.site_pos = d.site_pos;
     a.site_mut0 = d.site_mut0;
-    a.mut_node = d.mut_node;
     a.mut_allele = d.mut_allele;
     a.mut_pallele = d.mut_pallele;
     a.site_nalleles = d.site_nalleles;
@@ -1615,14 +1702,7 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
     a.Q = 1;
     a.col_total = (const int32_t *) ws->tot.p;
     a.windows = (const double *) ws->windows.p;
-    a.cq_wfirst = (const int32_t *) ws->cq_wfirst.p;
-    a.cq_nwin = (const int32_t *) ws->cq_nwin.p;
-    a.cq_poff = (const int64_t *) ws->cq_poff.p;
     a.flags = (options & TSG_STAT_POLARISED) ? TSG_F_POLARISED : 0u;
-    a.g_rec = (int32_t *) ws->slab.p;
-    a.g_stride = (int64_t) ts->max_nloc * R;
-    a.g_last = (double *) ws->last.p;
-    a.g_last_stride = ts->max_nloc;
     a.afs_K = (int32_t) K;
     a.afs_size = afs_size;
     a.afs_dims = (const int64_t *) ws->dims.p;
@@ -1630,6 +1710,7 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
     a.time_windows = (const double *) ws->twin.p;
     a.afs_hist_u = (unsigned long long *) ws->hist.p;
     a.afs_hist_d = (double *) ws->hist.p;
+    a.afs_W = W;
     const uint32_t fflags = (options & TSG_STAT_SPAN_NORMALISE) ? TSG_F_SPAN_NORMALISE : 0u;
     if (branch) {
         if ((ret = hip_check(tsg_launch_fill_f64((double *) ws->hist.p, hist_n, s)))) return ret;
@@ -1639,7 +1720,26 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
         }
     }
     if (ws->profile) (void) hipEventRecord(ws->ev0, s);
-    if ((ret = hip_check(tsg_launch_afs(a, branch, lds, lds ? lds_need : stage, s)))) return ret;
+    if (use_hot) {
+        if ((ret = hip_check(tsg_launch_afs_hot(a, branch, hot_need, s)))) return ret;
+        if (branch) {
+            const tsg_hotplan &hp = ts->hot;
+            const int nl = (int) hp.tl_parent.size();
+            if ((ret = ensure(ws->x0, (size_t) nl * KK * 4 + 4))) return ret;
+            TsgLastTree lt{};
+            lt.parent = h.tl_parent;
+            lt.time = h.tl_time;
+            lt.last = h.tl_last;
+            lt.ins = h.tl_ins;
+            lt.nl = nl;
+            lt.counts = (int32_t *) ws->x0.p;
+            if ((ret = hip_check(tsg_launch_afs_last_tree(a, lt, ts->L, s)))) return ret;
+        }
+    } else {
+        if ((ret = hip_check(tsg_launch_afs(a, branch, lds, lds ? lds_need : stage, s)))) {
+            return ret;
+        }
+    }
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);
     double *res_dev;
     if (branch) {

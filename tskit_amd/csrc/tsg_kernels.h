@@ -116,6 +116,22 @@ struct TsgSweepArgs {
     const double *time_windows;  // [TW+1]
     unsigned long long *afs_hist_u;  // site AFS: half-unit counts
     double *afs_hist_d;          // branch AFS
+    // AFS on the hot plan: windows, and per slot the samples it is the
+    // nearest hot ancestor-or-self of, its node's time, its parent's time at
+    // the chunk start (-inf: none), the position of the node's last flush
+    // before the chunk and of its parent edge's insertion
+    int32_t afs_W;
+    const int32_t *slot_nsd;
+    const double *slot_tu, *slot_tp0, *slot_last0, *slot_ins0;
+};
+
+// The last tree of the sequence, samples first, for the branch AFS's final
+// flush of the intervals still open at its end.
+struct TsgLastTree {
+    const int32_t *parent;
+    const double *time, *last, *ins;
+    int32_t nl;
+    int32_t *counts;             // [nl x (afs_K + 1)] scratch
 };
 
 // Genotype bitset sweep (relatedness matrix route).

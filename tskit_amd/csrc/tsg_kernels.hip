@@ -1142,6 +1142,220 @@ __global__ void __launch_bounds__(WAVE) k_afs_branch(const TsgSweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Allele frequency spectrum on the hot plan: a chunk holds only the nodes
+// its events and mutations touch (slots), with counts for the afs_K sample
+// sets and, in column afs_K, all samples (seeded from slot_nsd).
+//
+// Branch mode follows the reference's flush rule (trees.c:3650-3690,
+// 3738-3770): a node's contribution since its last update is added when it
+// is updated -- as the child of a removed edge, or as a node on the path
+// above an inserted or removed edge -- and at every window boundary.  Here a
+// node is flushed only when an event updates it; the interval since its
+// previous update is split at the window boundaries it crosses, and each
+// piece is valued with the node's state at the piece's end, as the
+// reference's boundary flush would see it: the counts are those of the whole
+// interval (only updates change them), and the node has a parent at a
+// boundary q when it has one now and its edge was inserted before q (an
+// inserted edge's child is not updated at insertion, so the piece that ends
+// at the next update includes the span since the previous one).  Intervals
+// still open at the end are flushed from the last tree (k_afs_tl_*).
+namespace {
+
+// window holding position x: windows[w] <= x < windows[w + 1], 0 <= w < W
+__device__ int afs_window_of(const double *win, int W, double x) {
+    int lo = 0, hi = W - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (win[mid] <= x) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
+// The branch AFS contributions of one node with counts cnt, time tu and
+// parent time tp (-inf: no parent; the edge inserted at y) over [a, x).
+__device__ void afs_flush_pieces(const TsgSweepArgs &a, const int32_t *cnt, double tu, double tp,
+                                 double y, double from, double x) {
+    if (!(tp > -INFINITY)) return;
+    const int32_t all = cnt[a.afs_K];
+    if (all <= 0 || all >= a.num_samples) return;
+    const int64_t off = afs_offset(a, cnt, (a.flags & TSG_F_POLARISED) != 0);
+    const double *win = a.windows;
+    const int W = a.afs_W;
+    auto add = [&](int wi, double span) {
+        for (int tw = 0; tw < a.TW && a.time_windows[tw] < tp; tw++) {
+            const double twbl = fmax(0.0, fmin(a.time_windows[tw + 1], tp) -
+                                              fmax(a.time_windows[tw], tu));
+            atomicAdd(&a.afs_hist_d[((int64_t) wi * a.TW + tw) * a.afs_size + off], span * twbl);
+        }
+    };
+    int wi = afs_window_of(win, W, from);
+    double p = from;
+    while (wi < W && win[wi + 1] <= x) {
+        const double q = win[wi + 1];
+        if (y < q) add(wi, q - p);
+        p = q;
+        wi++;
+    }
+    if (wi < W && x > p) add(wi, x - p);
+}
+
+}  // namespace
+
+// One wave per chunk.  LDS: (branch) per slot tu, tp, last, y as doubles,
+// then the counts [S x (afs_K + 1)] and a staging row.
+template <bool BRANCH>
+__global__ void __launch_bounds__(WAVE) k_afs_hot(const TsgSweepArgs a) {
+    extern __shared__ double dsm[];
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int K1 = a.afs_K + 1;
+    const int64_t s0 = a.chunk_node0[c];
+    const int S = (int) (a.chunk_node0[c + 1] - s0);
+    double *tu = dsm, *tp = dsm + S, *last = dsm + 2 * S, *yins = dsm + 3 * S;
+    int32_t *X = reinterpret_cast<int32_t *>(BRANCH ? dsm + 4 * S : dsm);
+    int32_t *aw = X + (int64_t) S * K1;
+    const bool polarised = (a.flags & TSG_F_POLARISED) != 0;
+
+    for (int i = lane; i < S * K1; i += WAVE) X[i] = 0;
+    __syncthreads();
+    for (int h = lane; h < S; h += WAVE) {
+        X[h * K1 + a.afs_K] = a.slot_nsd[s0 + h];
+        if (BRANCH) {
+            tu[h] = a.slot_tu[s0 + h];
+            tp[h] = a.slot_tp0[s0 + h];
+            last[h] = a.slot_last0[s0 + h];
+            yins[h] = a.slot_ins0[s0 + h];
+        }
+    }
+    const uint16_t *ss = a.sample_slot + (int64_t) c * a.num_samples;
+    for (int i = lane; i < a.npairs; i += WAVE) {
+        const unsigned h = ss[a.pair_sample[i]];
+        if (h != TSG_NO_SLOT) atomicAdd(&X[h * K1 + a.pair_col[i]], 1);
+    }
+    __syncthreads();
+    const int32_t *lev = a.lev_slot0 + a.chunk_lev0[c];
+    const int nlev = (int) (a.chunk_lev0[c + 1] - a.chunk_lev0[c]) - 1;
+    for (int l = 0; l + 1 < nlev; l++) {
+        const int h0 = lev[l], h1 = lev[l + 1];
+        for (int idx = lane; idx < (h1 - h0) * K1; idx += WAVE) {
+            const int h = h0 + idx / K1, k = idx % K1;
+            const int p = a.slot_par[s0 + h];
+            const int32_t v = X[h * K1 + k];
+            if (p >= 0 && v != 0) atomicAdd(&X[p * K1 + k], v);
+        }
+        __syncthreads();
+    }
+
+    const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
+    // branch mode also applies the closing transition (its updates happen
+    // at the chunk end, before the next chunk's start state)
+    const int tend = (BRANCH && t1 < a.num_trees) ? t1 + 1 : t1;
+    int wi = BRANCH ? 0 : afs_window_of(a.windows, a.afs_W, a.bp[t0]);
+    for (int t = t0; t < tend; t++) {
+        if (t > t0) {
+            const double x = a.bp[t];
+            const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
+            const int64_t nrem = a.tr_nrem[t];
+            for (int64_t e = e0; e < e1; e++) {
+                const bool ins = (e - e0) >= nrem;
+                const int u = a.ev_child[e];
+                const int64_t p0 = a.ev_path0[e];
+                const int plen = (int) (a.ev_path0[e + 1] - p0);
+                if (BRANCH) {
+                    // the path nodes and, for a removal, the child: distinct
+                    // nodes, one lane each
+                    const int items = plen + (ins ? 0 : 1);
+                    for (int j = lane; j < items; j += WAVE) {
+                        const int w = (j < plen) ? a.path[p0 + j] : u;
+                        afs_flush_pieces(a, X + w * K1, tu[w], tp[w], yins[w], last[w], x);
+                        last[w] = x;
+                    }
+                    __syncthreads();
+                }
+                for (int idx = lane; idx < plen * K1; idx += WAVE) {
+                    const int j = idx / K1, k = idx % K1;
+                    const int w = a.path[p0 + j];
+                    const int32_t v = X[u * K1 + k];
+                    X[w * K1 + k] += ins ? v : -v;
+                }
+                if (BRANCH && lane == 0) {
+                    if (ins) {
+                        tp[u] = tu[a.path[p0]];
+                        yins[u] = x;
+                    } else {
+                        tp[u] = -INFINITY;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (!BRANCH && t < t1) {
+            const int s0s = a.tree_site0[t], s1s = a.tree_site0[t + 1];
+            for (int s = s0s; s < s1s; s++) {
+                const double pos = a.site_pos[s];
+                while (a.windows[wi + 1] <= pos) wi++;
+                const int na = a.site_nalleles[s];
+                const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
+                for (int al = polarised ? 1 : 0; al < na; al++) {
+                    for (int k = lane; k < K1; k += WAVE) {
+                        int32_t w = (al == 0) ? a.col_total[k] : 0;
+                        for (int mm = m0; mm < m1; mm++) {
+                            const int h = a.mut_node[mm];
+                            if (a.mut_allele[mm] == al) w += X[h * K1 + k];
+                            if (a.mut_pallele[mm] == al) w -= X[h * K1 + k];
+                        }
+                        aw[k] = w;
+                    }
+                    __syncthreads();
+                    if (lane == 0) {
+                        const int32_t all = aw[a.afs_K];
+                        if (all > 0 && all < a.num_samples) {
+                            const int64_t off = afs_offset(a, aw, polarised);
+                            atomicAdd(&a.afs_hist_u[(int64_t) wi * a.afs_size + off],
+                                      polarised ? 2ull : 1ull);
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+    }
+}
+
+// Counts of the last tree: thread i < npairs adds its (sample, set) pair,
+// thread npairs + s adds sample s to the all-samples column, each walking
+// its root path.
+__global__ void k_afs_tl_counts(const TsgSweepArgs a, const TsgLastTree lt) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int K1 = a.afs_K + 1;
+    if (i >= (int64_t) a.npairs + a.num_samples) return;
+    int u, k;
+    if (i < a.npairs) {
+        u = a.pair_sample[i];
+        k = a.pair_col[i];
+    } else {
+        u = (int) (i - a.npairs);
+        k = a.afs_K;
+    }
+    for (; u >= 0; u = lt.parent[u]) atomicAdd(&lt.counts[(int64_t) u * K1 + k], 1);
+}
+
+// The reference's flush at the last window boundary (the sequence end) of
+// every node with a parent in the last tree, one thread per node.
+__global__ void k_afs_tl_flush(const TsgSweepArgs a, const TsgLastTree lt, double L) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= lt.nl) return;
+    const int p = lt.parent[q];
+    if (p < 0) return;
+    afs_flush_pieces(a, lt.counts + (int64_t) q * (a.afs_K + 1), lt.time[q], lt.time[p], lt.ins[q],
+                     lt.last[q], L);
+}
+
+// ---------------------------------------------------------------------------
 // Running branch sum of the first tree (chunk 0's start tree), evaluated in
 // parallel instead of by replaying its edge insertions one at a time:
 //   R(tree 0) = sum over nodes u with a parent of bl(u) * F(x(u)),
@@ -1462,6 +1676,38 @@ hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t s
         } else {
             hipLaunchKernelGGL((k_afs_site<false>), grid, block, shmem, s, a);
         }
+    }
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_afs_hot(const TsgSweepArgs &a, bool branch, size_t shmem, hipStream_t s) {
+    dim3 grid(a.num_chunks), block(WAVE);
+    hipError_t e;
+    if (branch) {
+        if ((e = allow_lds(&k_afs_hot<true>, shmem)) != hipSuccess) return e;
+        hipLaunchKernelGGL((k_afs_hot<true>), grid, block, shmem, s, a);
+    } else {
+        if ((e = allow_lds(&k_afs_hot<false>, shmem)) != hipSuccess) return e;
+        hipLaunchKernelGGL((k_afs_hot<false>), grid, block, shmem, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_afs_last_tree(const TsgSweepArgs &a, const TsgLastTree &lt, double L,
+                                    hipStream_t s) {
+    const int64_t words = (int64_t) lt.nl * (a.afs_K + 1);
+    if (words > 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned) ((words + 255) / 256)), dim3(256), 0, s,
+                           lt.counts, words);
+    }
+    const int64_t walkers = (int64_t) a.npairs + a.num_samples;
+    if (walkers > 0) {
+        hipLaunchKernelGGL(k_afs_tl_counts, dim3((unsigned) ((walkers + 255) / 256)), dim3(256), 0,
+                           s, a, lt);
+    }
+    if (lt.nl > 0) {
+        hipLaunchKernelGGL(k_afs_tl_flush, dim3((unsigned) ((lt.nl + 255) / 256)), dim3(256), 0, s,
+                           a, lt, L);
     }
     return hipGetLastError();
 }

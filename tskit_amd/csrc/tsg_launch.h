@@ -9,6 +9,11 @@ hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl,
                             hipStream_t s);
 hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t shmem,
                           hipStream_t s);
+// the AFS on the hot plan: one wave per chunk, shmem bytes of LDS
+hipError_t tsg_launch_afs_hot(const TsgSweepArgs &a, bool branch, size_t shmem, hipStream_t s);
+// the branch AFS's flush of the intervals open at the sequence end L
+hipError_t tsg_launch_afs_last_tree(const TsgSweepArgs &a, const TsgLastTree &lt, double L,
+                                    hipStream_t s);
 // number of per-block partials tsg_launch_tree0 writes (each a.M doubles)
 int tsg_tree0_blocks(int nloc);
 // counts of the first tree (t0_parent / t0_bl, samples first, nloc nodes)

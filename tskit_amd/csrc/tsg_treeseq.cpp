@@ -411,8 +411,15 @@ namespace {
 // tr_nrem, mut_slot and sample_slot entries are disjoint per part and are
 // written in place.
 struct HotPart {
-    std::vector<int32_t> ev_child, path, slot_par, lev_slot0;
-    std::vector<double> ev_bl, slot_bl0;
+    std::vector<int32_t> ev_child, path, slot_par, lev_slot0, slot_node, slot_nsd;
+    std::vector<double> ev_bl, slot_bl0, slot_tp0;
+    std::vector<double> slot_last0;    // -1: before this part (filled on merge)
+    std::vector<double> slot_ins0;     // -1: before this part (filled on merge)
+    std::vector<int32_t> touched;      // nodes an event of this part flushed
+    std::vector<double> touched_last;  // ... and the position of the last one
+    std::vector<int32_t> inserted;     // nodes an insertion of this part gave a parent
+    std::vector<double> inserted_at;   // ... and the position of the last one
+    std::vector<int32_t> end_par;      // parents after the part's last chunk
     std::vector<int64_t> ev_end;       // per event: end of its path (part-relative)
     std::vector<int64_t> chunk_slots;  // per chunk
     std::vector<int64_t> chunk_levs;   // per chunk: level-boundary entries
@@ -429,7 +436,13 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
     std::vector<int32_t> memo_stamp(N, -1), memo_nha(N, -1);  // nearest hot ancestor-or-self
     std::vector<int32_t> hot, ev_u, path_g, hpar, depth, slot_of, stack, lev_count, next;
     std::vector<int64_t> ev_p0;
+    std::vector<double> ev_x;                                // position of each event
+    std::vector<char> ev_ins;                                // insertion (1) or removal (0)
     std::vector<std::pair<int32_t, int32_t>> undo, redo;     // (child, parent before / after)
+    // position of each node's last flush by an event in this part (a node on
+    // an event's root path, or a removed edge's child; -1: none), and of the
+    // insertion that gave it its current parent (-1: none in this part)
+    std::vector<double> last_touch(N, -1.0), ins_at(N, -1.0);
     // replay up to the start tree of chunk cbeg
     int32_t j = 0, k = 0;
     for (int32_t t = 0; t <= hp.chunk_tree0[cbeg]; t++) {
@@ -445,6 +458,8 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
         ev_u.clear();
         path_g.clear();
         ev_p0.assign(1, 0);
+        ev_x.clear();
+        ev_ins.clear();
         undo.clear();
         redo.clear();
         auto touch = [&](int32_t u) {
@@ -472,6 +487,8 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
                 const int32_t e = ts->O[k];
                 const int32_t u = ts->child[e];
                 ev_u.push_back(u);
+                ev_x.push_back(x);
+                ev_ins.push_back(0);
                 out.ev_bl.push_back(0.0f);
                 touch(u);
                 undo.emplace_back(u, gpar[u]);
@@ -485,6 +502,8 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
                 const int32_t e = ts->I[j];
                 const int32_t u = ts->child[e], p = ts->parent[e];
                 ev_u.push_back(u);
+                ev_x.push_back(x);
+                ev_ins.push_back(1);
                 out.ev_bl.push_back(ts->time[p] - ts->time[u]);
                 touch(u);
                 undo.emplace_back(u, gpar[u]);
@@ -539,12 +558,23 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
         const size_t s0 = out.slot_par.size();
         out.slot_par.resize(s0 + H);
         out.slot_bl0.resize(s0 + H);
+        out.slot_tp0.resize(s0 + H);
+        out.slot_node.resize(s0 + H);
+        out.slot_last0.resize(s0 + H);
+        out.slot_ins0.resize(s0 + H);
+        out.slot_nsd.resize(s0 + H, 0);
         out.chunk_slots.push_back(H);
         for (int32_t i = 0; i < H; i++) {
             const int32_t u = hot[i];
-            out.slot_par[s0 + slot_of[i]] = (hpar[i] < 0) ? -1 : slot_of[hpar[i]];
-            out.slot_bl0[s0 + slot_of[i]] =
-                (gpar[u] == TSG_NULL) ? 0.0 : ts->time[gpar[u]] - ts->time[u];
+            const size_t sl = s0 + slot_of[i];
+            out.slot_par[sl] = (hpar[i] < 0) ? -1 : slot_of[hpar[i]];
+            out.slot_bl0[sl] = (gpar[u] == TSG_NULL) ? 0.0 : ts->time[gpar[u]] - ts->time[u];
+            out.slot_tp0[sl] = (gpar[u] == TSG_NULL) ? -INFINITY : ts->time[gpar[u]];
+            out.slot_node[sl] = u;
+            // the node's state (parent, subtree) has been constant since its
+            // last touch by an event, before this chunk
+            out.slot_last0[sl] = last_touch[u];
+            out.slot_ins0[sl] = ins_at[u];
         }
         for (int32_t u : ev_u) out.ev_child.push_back(slot_of[hot_idx[u]]);
         const int64_t pbase = (int64_t) out.path.size();
@@ -573,12 +603,34 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
                 memo_nha[v] = res;
             }
             ss[s] = (res == TSG_NULL) ? (uint16_t) TSG_NO_SLOT : (uint16_t) slot_of[hot_idx[res]];
+            if (res != TSG_NULL) out.slot_nsd[s0 + ss[s]]++;
+        }
+        // this chunk's flushes and insertions, in event order (the last
+        // one wins); an inserted edge's child is not flushed (trees.c:3765)
+        for (size_t e = 0; e < ev_u.size(); e++) {
+            const double x = ev_x[e];
+            auto note = [&](int32_t w) {
+                if (last_touch[w] < 0) out.touched.push_back(w);
+                last_touch[w] = x;
+            };
+            if (ev_ins[e]) {
+                if (ins_at[ev_u[e]] < 0) out.inserted.push_back(ev_u[e]);
+                ins_at[ev_u[e]] = x;
+            } else {
+                note(ev_u[e]);
+            }
+            for (int64_t q = ev_p0[e]; q < ev_p0[e + 1]; q++) note(path_g[q]);
         }
         out.max_slots = std::max(out.max_slots, H);
         out.max_levels = std::max(out.max_levels, nlev);
         // forward to the next chunk's start tree
         for (const auto &r : redo) gpar[r.first] = r.second;
     }
+    out.touched_last.resize(out.touched.size());
+    for (size_t i = 0; i < out.touched.size(); i++) out.touched_last[i] = last_touch[out.touched[i]];
+    out.inserted_at.resize(out.inserted.size());
+    for (size_t i = 0; i < out.inserted.size(); i++) out.inserted_at[i] = ins_at[out.inserted[i]];
+    if (cend == hp.num_chunks) out.end_par = gpar;
 }
 
 }  // namespace
@@ -653,6 +705,8 @@ int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
         if (part.too_big) return 0;  // not representable: hp.usable stays false
     }
     // merge in chunk order
+    std::vector<double> glast(N, 0.0);  // last event flush of each node so far
+    std::vector<double> gins(N, 0.0);   // position of its current edge's insertion
     hp.chunk_slot0.assign(C + 1, 0);
     hp.chunk_lev0.assign(C + 1, 0);
     hp.ev_path0.assign(1, 0);
@@ -669,6 +723,18 @@ int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
         hp.path.insert(hp.path.end(), part.path.begin(), part.path.end());
         hp.slot_par.insert(hp.slot_par.end(), part.slot_par.begin(), part.slot_par.end());
         hp.slot_bl0.insert(hp.slot_bl0.end(), part.slot_bl0.begin(), part.slot_bl0.end());
+        hp.slot_tp0.insert(hp.slot_tp0.end(), part.slot_tp0.begin(), part.slot_tp0.end());
+        hp.slot_node.insert(hp.slot_node.end(), part.slot_node.begin(), part.slot_node.end());
+        hp.slot_nsd.insert(hp.slot_nsd.end(), part.slot_nsd.begin(), part.slot_nsd.end());
+        // a slot whose node was last touched before this part: its last
+        // touch in the earlier parts (0: never touched, alive since x = 0)
+        for (size_t q = 0; q < part.slot_last0.size(); q++) {
+            const double l = part.slot_last0[q], y = part.slot_ins0[q];
+            hp.slot_last0.push_back(l >= 0 ? l : glast[part.slot_node[q]]);
+            hp.slot_ins0.push_back(y >= 0 ? y : gins[part.slot_node[q]]);
+        }
+        for (size_t q = 0; q < part.touched.size(); q++) glast[part.touched[q]] = part.touched_last[q];
+        for (size_t q = 0; q < part.inserted.size(); q++) gins[part.inserted[q]] = part.inserted_at[q];
         hp.lev_slot0.insert(hp.lev_slot0.end(), part.lev_slot0.begin(), part.lev_slot0.end());
         for (int32_t c = cb[p]; c < cb[p + 1]; c++) {
             hp.chunk_slot0[c + 1] = hp.chunk_slot0[c] + part.chunk_slots[c - cb[p]];
@@ -679,6 +745,37 @@ int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
         hp.max_levels = std::max(hp.max_levels, part.max_levels);
     }
     hp.tr_off[T] = (int64_t) hp.ev_child.size();
+    // the last tree, samples first, with each node's last touch: the state
+    // intervals still open at the end of the sequence
+    {
+        const std::vector<int32_t> &gpar = parts[P - 1].end_par;
+        std::vector<int32_t> lid(N, -1), order;
+        for (int32_t s2 = 0; s2 < n; s2++) {
+            lid[ts->samples[s2]] = (int32_t) order.size();
+            order.push_back(ts->samples[s2]);
+        }
+        for (int32_t u = 0; u < N; u++) {
+            if (gpar[u] == TSG_NULL) continue;
+            for (int32_t w : {u, gpar[u]}) {
+                if (lid[w] < 0) {
+                    lid[w] = (int32_t) order.size();
+                    order.push_back(w);
+                }
+            }
+        }
+        const size_t nl = order.size();
+        hp.tl_parent.assign(nl, TSG_NULL);
+        hp.tl_time.assign(nl, 0.0);
+        hp.tl_last.assign(nl, 0.0);
+        hp.tl_ins.assign(nl, 0.0);
+        for (size_t q = 0; q < nl; q++) {
+            const int32_t u = order[q];
+            hp.tl_time[q] = ts->time[u];
+            hp.tl_last[q] = glast[u];
+            hp.tl_ins[q] = gins[u];
+            if (gpar[u] != TSG_NULL) hp.tl_parent[q] = lid[gpar[u]];
+        }
+    }
     hp.usable = true;
     return 0;
 }

@@ -56,6 +56,16 @@ struct TsgHotDevice {
     int32_t *mut_slot = nullptr;
     int32_t *t0_parent = nullptr;
     double *t0_bl = nullptr;
+    // allele frequency spectrum tables (uploaded on first use)
+    int32_t *slot_nsd = nullptr;
+    double *slot_tu = nullptr;
+    double *slot_last0 = nullptr;
+    double *slot_ins0 = nullptr;
+    double *slot_tp0 = nullptr;
+    int32_t *tl_parent = nullptr;
+    double *tl_time = nullptr;
+    double *tl_last = nullptr;
+    double *tl_ins = nullptr;
     size_t bytes = 0;
 };
 
@@ -94,6 +104,18 @@ struct tsg_hotplan {
     // branch sum is evaluated over all of its nodes
     std::vector<int32_t> t0_parent;
     std::vector<double> t0_bl;
+    // for the branch allele frequency spectrum, per slot: its node, the
+    // number of samples whose nearest hot ancestor-or-self it is, the
+    // position of the node's last flush by an event before the chunk (as a
+    // node on an event's root path or a removed edge's child: its state is
+    // constant since then) and of the insertion of its current parent edge;
+    // and the last tree, samples first, with node times and the same two
+    // positions per node
+    std::vector<int32_t> slot_node, slot_nsd;
+    std::vector<double> slot_last0, slot_ins0;
+    std::vector<double> slot_tp0;  // time of the node's parent, -inf: none
+    std::vector<int32_t> tl_parent;
+    std::vector<double> tl_time, tl_last, tl_ins;
     int32_t max_slots = 0, max_path = 0, max_levels = 0;
     TsgHotDevice dev;
 };

@@ -211,18 +211,18 @@ class LowLevelTreeSequence:
 
     def get_hot_plan_sizes(self):
         """Sizes of the hot plan (no copies)."""
-        sizes = (ctypes.c_uint64 * 9)()
+        sizes = (ctypes.c_uint64 * 10)()
         self._lib.tsg_treeseq_get_hot_plan_sizes(self._h, sizes)
         keys = ["usable", "chunks", "slots", "level_entries", "events", "path_entries",
-                "first_tree_nodes", "max_slots", "max_levels"]
+                "first_tree_nodes", "max_slots", "max_levels", "last_tree_nodes"]
         return dict(zip(keys, (int(x) for x in sizes)))
 
     def get_hot_plan(self):
         """Host copy of the hot plan (chunks keep only the nodes they touch),
         or None when the tree sequence has a chunk it cannot represent."""
-        sizes = (ctypes.c_uint64 * 9)()
+        sizes = (ctypes.c_uint64 * 10)()
         self._lib.tsg_treeseq_get_hot_plan_sizes(self._h, sizes)
-        usable, C, ns, nlev, ne, npath, n0, max_slots, max_levels = (int(x) for x in sizes)
+        usable, C, ns, nlev, ne, npath, n0, max_slots, max_levels, nl = (int(x) for x in sizes)
         if not usable:
             return None
         T = self.get_num_trees()
@@ -240,6 +240,16 @@ class LowLevelTreeSequence:
         }
         check(self._lib.tsg_treeseq_copy_hot_plan(
             self._h, *[a.ctypes.data_as(ctypes.c_void_p) for a in p.values()]))
+        afs = {
+            "slot_node": np.zeros(ns, np.int32), "slot_nsd": np.zeros(ns, np.int32),
+            "slot_last0": np.zeros(ns, np.float64), "slot_ins0": np.zeros(ns, np.float64),
+            "slot_tp0": np.zeros(ns, np.float64),
+            "tl_parent": np.zeros(nl, np.int32), "tl_time": np.zeros(nl, np.float64),
+            "tl_last": np.zeros(nl, np.float64), "tl_ins": np.zeros(nl, np.float64),
+        }
+        check(self._lib.tsg_treeseq_copy_hot_afs_plan(
+            self._h, *[a.ctypes.data_as(ctypes.c_void_p) for a in afs.values()]))
+        p.update(afs)
         p["max_slots"] = max_slots
         p["max_levels"] = max_levels
         return p
