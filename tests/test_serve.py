@@ -1,6 +1,8 @@
 """The serving loop: fused launches give the results of the same requests
-served alone (bit for bit), the oracle's within the parity tolerance, and the
-loop reports latencies."""
+served alone (bit for bit: a fused sweep adds every output's terms in the
+same order; the branch AFS, whose bins take FP64 atomic adds in no fixed
+order, to 1e-12 of its largest bin), the oracle's within the parity
+tolerance, and the loop reports latencies."""
 import numpy as np
 import pytest
 
@@ -41,7 +43,12 @@ def test_mixed_equals_alone(resident):
     assert len(done) == len(reqs)
     assert mixed.launches < single.launches == len(reqs)
     for a, b in zip(reqs, alone):
-        assert _same(a.result, b.result), (a.stat, a.kwargs.get("mode"))
+        if a.stat == "allele_frequency_spectrum" and a.mode == "branch":
+            x, y = np.asarray(a.result), np.asarray(b.result)
+            assert x.shape == y.shape
+            assert np.max(np.abs(x - y)) <= 1e-12 * max(np.max(np.abs(y)), 1e-300)
+        else:
+            assert _same(a.result, b.result), (a.stat, a.kwargs.get("mode"))
 
 
 def test_served_results_match_oracle(resident):

@@ -44,7 +44,7 @@ struct Workspace {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf seed_off, seed_ps;
     DevBuf pair_s, pair_c, tot, outs, queries, windows, cq_wfirst, cq_nwin, cq_poff, partial, dR,
-        Rstart, raw, rawbase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
+        Rstart, raw, rawbase, lanebase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
     // first tree: counts and per-block running-sum partials
     DevBuf x0, r0part, bsum;
     // relatedness-matrix route
@@ -405,11 +405,20 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         }
         pcount += slots * qs[qi].M;
     }
-    std::vector<int64_t> rawbase(Q);
-    int64_t rawcount = 0;
+    // combine: lanes per (window, output) ~ chunks per window, a power of 2
+    // <= 64; each query's lanes start on a wave
+    std::vector<int64_t> rawbase(Q), lanebase(Q);
+    int64_t rawcount = 0, lanecount = 0;
     for (int qi = 0; qi < Q; qi++) {
+        const int64_t items = (int64_t) qs[qi].W * qs[qi].M;
         rawbase[qi] = rawcount;
-        rawcount += (int64_t) qs[qi].W * qs[qi].M;
+        rawcount += items;
+        const int64_t per = ((int64_t) C + qs[qi].W - 1) / qs[qi].W;
+        int g = 1;
+        while (g < per && g < 64) g <<= 1;
+        qs[qi].lanes = g;
+        lanebase[qi] = lanecount;
+        lanecount += (items * g + 63) / 64 * 64;
     }
     hipStream_t s = ws->stream;
     TsgSweepArgs a{};
@@ -418,6 +427,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         (ret = upload(ws->queries, qs, s)) || (ret = upload(ws->windows, wins, s)) ||
         (ret = upload(ws->cq_wfirst, cq_wfirst, s)) || (ret = upload(ws->cq_nwin, cq_nwin, s)) ||
         (ret = upload(ws->cq_poff, cq_poff, s)) || (ret = upload(ws->rawbase, rawbase, s)) ||
+        (ret = upload(ws->lanebase, lanebase, s)) ||
         (ret = ensure(ws->partial, (size_t) pcount * 8)) ||
         (ret = ensure(ws->raw, (size_t) rawcount * 8))) {
         return ret;
@@ -542,10 +552,11 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     ca.Rstart = (mode == TSG_STAT_BRANCH) ? (const double *) ws->Rstart.p : nullptr;
     ca.raw = (double *) ws->raw.p;
     ca.rawbase = (const int64_t *) ws->rawbase.p;
+    ca.lanebase = (const int64_t *) ws->lanebase.p;
     ca.flags = 0;
     if ((ret = hip_check(tsg_launch_combine(ca, (mode == TSG_STAT_BRANCH) ? a.dR : nullptr,
                                             (double *) ws->Rstart.p, (double *) ws->bsum.p,
-                                            (const double *) ws->r0part.p, nb0, rawcount, s)))) {
+                                            (const double *) ws->r0part.p, nb0, lanecount, s)))) {
         return ret;
     }
     // finals
@@ -737,7 +748,7 @@ void free_workspace(tsg_treeseq *ts) {
     if (ws == nullptr) return;
     DevBuf *bufs[] = {&ws->seed_off, &ws->seed_ps, &ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
                       &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
-                      &ws->Rstart, &ws->raw, &ws->rawbase, &ws->out, &ws->fin, &ws->ipar,
+                      &ws->Rstart, &ws->raw, &ws->rawbase, &ws->lanebase, &ws->out, &ws->fin, &ws->ipar,
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
                       &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
                       &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->x0,

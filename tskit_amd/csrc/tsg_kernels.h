@@ -23,7 +23,7 @@ struct TsgQuery {
     int32_t mbase;       // first output index in the batch
     int32_t cbase;       // first state column
     uint32_t flags;      // TSG_F_SPAN_NORMALISE
-    int32_t pad;
+    int32_t lanes;       // combine: lanes per (window, output), a power of 2 <= 64
     int64_t win_off;     // offset of its windows in the concatenated array
     int64_t pbase;       // offset (in doubles) of its partial-sum block
 };
@@ -165,5 +165,6 @@ struct TsgCombineArgs {
     const double *Rstart;        // [C*M] (branch) or null
     double *raw;                 // per query [W x M_q], offset rawbase[q]
     const int64_t *rawbase;
+    const int64_t *lanebase;     // per query: first combine lane (multiples of 64)
     uint32_t flags;
 };

@@ -1462,52 +1462,71 @@ __global__ void k_scan_apply(const double *dR, const double *bsum, double *Rstar
     }
 }
 
+// One group of Qr.lanes lanes per (query, window, output), aligned within
+// the wave: lane j of the group sums the window's chunks j, j + lanes, ...,
+// then the group adds its lanes by a fixed butterfly (deterministic).  A
+// window spanning many chunks (few windows) gets up to a wave.
 __global__ void k_combine(const TsgCombineArgs a, int64_t total) {
     const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total) return;
-    // locate (q, w, ml): rawbase is the prefix sum of W_q * M_q
+    if (gid >= total) return;  // total is a multiple of 64: whole waves leave
     int qlo = 0, qhi = a.Q - 1;
     while (qlo < qhi) {
         const int mid = (qlo + qhi + 1) / 2;
-        if (a.rawbase[mid] <= gid) {
+        if (a.lanebase[mid] <= gid) {
             qlo = mid;
         } else {
             qhi = mid - 1;
         }
     }
     const int q = qlo;
-    const int64_t rem = gid - a.rawbase[q];
     const TsgQuery &Qr = a.queries[q];
-    const int w = (int) (rem / Qr.M);
-    const int ml = (int) (rem % Qr.M);
-    const double *win = a.windows + Qr.win_off;
-    const double wl = win[w], wr = win[w + 1];
-    // chunk c spans [bp[chunk_tree0[c]], bp[chunk_tree0[c+1]])
-    int lo = 0, hi = a.num_chunks - 1;
-    while (lo < hi) {  // first chunk whose end is > wl
-        const int mid = (lo + hi) / 2;
-        if (a.bp[a.chunk_tree0[mid + 1]] > wl) {
-            hi = mid;
-        } else {
-            lo = mid + 1;
-        }
-    }
+    const int G = Qr.lanes;
+    const int64_t rem = gid - a.lanebase[q];
+    const int64_t item = rem / G;
+    const int sub = (int) (rem % G);
+    const bool valid = item < (int64_t) Qr.W * Qr.M;
     double acc = 0;
-    for (int c = lo; c < a.num_chunks; c++) {
-        const double cs = a.bp[a.chunk_tree0[c]], ce = a.bp[a.chunk_tree0[c + 1]];
-        if (!(cs < wr)) break;
-        const int64_t cq = (int64_t) c * a.Q + q;
-        const int slot = w - a.cq_wfirst[cq];
-        if (slot >= 0 && slot < a.cq_nwin[cq]) {
-            acc += a.partial[Qr.pbase + (a.cq_poff[cq] + slot) * (int64_t) Qr.M + ml];
+    int w = 0, ml = 0;
+    double wl = 0, wr = 1;
+    if (valid) {
+        w = (int) (item / Qr.M);
+        ml = (int) (item % Qr.M);
+        const double *win = a.windows + Qr.win_off;
+        wl = win[w];
+        wr = win[w + 1];
+        // chunk c spans [bp[chunk_tree0[c]], bp[chunk_tree0[c+1]])
+        int lo = 0, hi = a.num_chunks - 1;
+        while (lo < hi) {  // first chunk whose end is > wl
+            const int mid = (lo + hi) / 2;
+            if (a.bp[a.chunk_tree0[mid + 1]] > wl) {
+                hi = mid;
+            } else {
+                lo = mid + 1;
+            }
         }
-        if (a.Rstart != nullptr) {
-            const double ov = fmin(ce, wr) - fmax(cs, wl);
-            if (ov > 0) acc += a.Rstart[(int64_t) c * a.M + Qr.mbase + ml] * ov;
+        for (int c = lo + sub; c < a.num_chunks; c += G) {
+            const double cs = a.bp[a.chunk_tree0[c]], ce = a.bp[a.chunk_tree0[c + 1]];
+            if (!(cs < wr)) break;
+            const int64_t cq = (int64_t) c * a.Q + q;
+            const int slot = w - a.cq_wfirst[cq];
+            if (slot >= 0 && slot < a.cq_nwin[cq]) {
+                acc += a.partial[Qr.pbase + (a.cq_poff[cq] + slot) * (int64_t) Qr.M + ml];
+            }
+            if (a.Rstart != nullptr) {
+                const double ov = fmin(ce, wr) - fmax(cs, wl);
+                if (ov > 0) acc += a.Rstart[(int64_t) c * a.M + Qr.mbase + ml] * ov;
+            }
         }
     }
-    if (Qr.flags & TSG_F_SPAN_NORMALISE) acc /= (wr - wl);
-    a.raw[a.rawbase[q] + (int64_t) w * Qr.M + ml] = acc;
+    // every lane of the wave takes part (groups are aligned, G divides 64)
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(acc, off);
+        if (off < G) acc += o;
+    }
+    if (valid && sub == 0) {
+        if (Qr.flags & TSG_F_SPAN_NORMALISE) acc /= (wr - wl);
+        a.raw[a.rawbase[q] + (int64_t) w * Qr.M + ml] = acc;
+    }
 }
 
 // FP64 composites of the reference's Python layer.
