@@ -115,7 +115,10 @@ SERVE_SHAPES = [(1000, 1e6, 1), (2000, 2e6, 51), (5000, 2e6, 52), (10000, 2e6, 5
 
 
 def served_phase(big_ts, count, seed):
-    """Capacity and latency of the serving loop (see the module docstring)."""
+    """Capacity and latency of the serving loop (see the module docstring):
+    the served form (fused launches, up to 4 in flight on their own streams)
+    against the plain path (one request per launch, one launch at a time),
+    on the 8 resident tree sequences and on the 1e5-sample one alone."""
     import tskit_amd as ta
     from tskit_amd import serve
 
@@ -124,40 +127,46 @@ def served_phase(big_ts, count, seed):
            for n, L, sd in SERVE_SHAPES]
     tss.append(big_ts)
     log(f"serve: 8 resident tree sequences generated in {time.perf_counter() - t0:.1f}s")
+    forms = {"served": dict(max_inflight=4), "plain": dict(one_per_launch=True, max_inflight=1)}
 
-    def capacity(one):
-        srv = serve.StatServer(tss, one_per_launch=one)
-        reqs = serve.random_requests(tss, count, seed)
+    def capacity(pool, form, n_req, sd):
+        srv = serve.StatServer(pool, **forms[form])
+        reqs = serve.random_requests(pool, n_req, sd)
         t = time.perf_counter()
         for r in reqs:
             r.t_arrival = t
             srv.submit(r)
-        while srv.queue:
-            srv.step()
+        srv.step()
         wall = time.perf_counter() - t
-        return count / wall, srv.launches, srv.resident_bytes
+        srv.close()
+        return n_req / wall, srv.launches, srv.resident_bytes
 
-    capacity(False)  # warm every tree sequence and buffer
-    qps_b, launches_b, resident = capacity(False)
-    qps_1, launches_1, _ = capacity(True)
-    log(f"serve: capacity batched {qps_b:.1f} q/s ({launches_b} launches), one per launch "
-        f"{qps_1:.1f} q/s ({launches_1} launches)")
-    offered = 0.5 * qps_b
-    rng = np.random.default_rng(seed + 1)
-    out = {"requests": count, "resident": len(tss), "resident_bytes": resident,
-           "qps_batched": qps_b, "qps_one_per_launch": qps_1, "speedup": qps_b / qps_1,
-           "offered_qps": offered}
-    for one, tag in [(False, ""), (True, "_one_per_launch")]:
-        srv = serve.StatServer(tss, one_per_launch=one)
-        reqs = serve.random_requests(tss, count, seed + 2)
-        arrivals = np.cumsum(rng.exponential(1.0 / offered, count))
-        wall = srv.run_open_loop(reqs, arrivals)
-        st = serve.latency_stats(reqs)
-        out["p50_ms" + tag] = st["p50_ms"]
-        out["p99_ms" + tag] = st["p99_ms"]
-        out["served_qps" + tag] = count / wall
-        log(f"serve: offered {offered:.1f} q/s{tag.replace('_', ' ')}: p50 {st['p50_ms']:.2f} ms "
-            f"p99 {st['p99_ms']:.2f} ms")
+    out = {"requests": count, "resident": len(tss)}
+    for name, pool, n_req in [("mix8", tss, count), ("big", [big_ts], max(count // 3, 40))]:
+        capacity(pool, "served", max(n_req // 4, 20), seed + 100)  # warm buffers
+        qs, ls, resident = capacity(pool, "served", n_req, seed)
+        qp, lp, _ = capacity(pool, "plain", n_req, seed)
+        log(f"serve {name}: capacity served {qs:.1f} q/s ({ls} launches), plain {qp:.1f} q/s "
+            f"({lp} launches): {qs / qp:.2f}x")
+        res = {"requests": n_req, "resident_bytes": resident, "qps_served": qs,
+               "qps_plain": qp, "speedup": qs / qp, "launches_served": ls,
+               "launches_plain": lp}
+        offered = 0.5 * qp
+        rng = np.random.default_rng(seed + 1)
+        arrivals = np.cumsum(rng.exponential(1.0 / offered, n_req))
+        res["offered_qps"] = offered
+        for form in ["served", "plain"]:
+            srv = serve.StatServer(pool, **forms[form])
+            reqs = serve.random_requests(pool, n_req, seed + 2)
+            wall = srv.run_open_loop(reqs, arrivals)
+            srv.close()
+            st = serve.latency_stats(reqs)
+            res[f"p50_ms_{form}"] = st["p50_ms"]
+            res[f"p99_ms_{form}"] = st["p99_ms"]
+            res[f"achieved_qps_{form}"] = n_req / wall
+            log(f"serve {name}: offered {offered:.1f} q/s, {form}: p50 {st['p50_ms']:.2f} ms "
+                f"p99 {st['p99_ms']:.2f} ms")
+        out[name] = res
     return out, tss
 
 

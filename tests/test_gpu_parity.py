@@ -216,6 +216,28 @@ def test_node_stores(require_gpu, plan):
                orc.stat("f4", sets[:4], [(0, 1, 2, 3)], w, mode)[:, 0])
 
 
+def test_unused_sets_dropped(small):
+    """Sets that no index tuple names do not become state columns; the
+    remaining columns are renumbered (results as the oracle's)."""
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, 7, 20, 11)
+    w = np.linspace(0, tc.sequence_length, 4)
+    for mode in ["branch", "site"]:
+        _say("unused sets", mode)
+        for stat, idx in [("divergence", [(6, 2), (2, 2)]), ("f3", [(5, 1, 3)]),
+                          ("f4", [(6, 4, 2, 0)]), ("Y2", [(3, 5)])]:
+            got = getattr(ts, stat)(sets, indexes=idx, windows=w, mode=mode)
+            _check(stat, np.asarray(got).reshape(len(w) - 1, -1),
+                   orc.stat(stat, sets, idx, w, mode))
+        b = ts.batch()
+        p1 = b.f2(sets, indexes=[(4, 1)], windows=w, mode=mode)
+        p2 = b.diversity(sets[2:4], windows=w, mode=mode)
+        b.run()
+        _check("f2", np.asarray(p1.result()).reshape(len(w) - 1, -1),
+               orc.stat("f2", sets, [(4, 1)], w, mode))
+        _check("diversity", p2.result(), orc.stat("diversity", sets[2:4], None, w, mode))
+
+
 @pytest.mark.parametrize("fixture", ["paper_ex", "single_tree_ex", "nonbinary_ex"])
 def test_reference_fixtures(require_gpu, fixture):
     tc = getattr(cases, fixture)()

@@ -31,12 +31,13 @@ def test_mixed_equals_alone(resident):
     tcs, tss = resident
     reqs = serve.random_requests(tss, 60, seed=5, max_sets=8, max_set_size=60, max_windows=12)
     alone = serve.random_requests(tss, 60, seed=5, max_sets=8, max_set_size=60, max_windows=12)
-    mixed = serve.StatServer(tss)
+    mixed = serve.StatServer(tss, max_inflight=4)  # fused, concurrent launches
     for r in reqs:
         mixed.submit(r)
     done = mixed.step()
+    mixed.close()
     print("[case] mixed launches", mixed.launches, flush=True)
-    single = serve.StatServer(tss, one_per_launch=True)
+    single = serve.StatServer(tss, one_per_launch=True, max_inflight=1)
     for r in alone:
         single.submit(r)
     single.step()

@@ -61,6 +61,19 @@ struct Workspace {
     uint64_t mode_launches[2] = {0, 0};
 };
 
+// Per resident tree sequence: a pool of workspaces, each with its own
+// stream, so that queries on one tree sequence (or on several) issued from
+// different host threads run concurrently on the device; and the settings
+// every workspace takes when it is leased.
+struct Resident {
+    std::vector<Workspace *> all, idle;
+    int lds_limit = 65536;
+    bool force_full = false, force_c32 = false, profile = false;
+    int last_store = -1;
+};
+
+// Guards device residency (uploads, plan builds), the workspace pools and
+// settings; never held while a query runs.
 std::mutex g_mutex;
 
 int hip_check(hipError_t e) { return e == hipSuccess ? 0 : TSG_ERR_DEVICE; }
@@ -107,7 +120,48 @@ int upload_ptr(T **dst, const std::vector<T> &v, size_t &bytes) {
     return 0;
 }
 
-Workspace *ws_of(tsg_treeseq *ts);
+Resident *res_of(const tsg_treeseq *ts) { return reinterpret_cast<Resident *>(ts->workspace); }
+
+// A workspace leased from the tree sequence's pool for one query or batch
+// (created on demand; returned, with its last node store recorded, when the
+// lease ends).  device_upload must have run.
+struct Lease {
+    Resident *res;
+    Workspace *ws = nullptr;
+    explicit Lease(tsg_treeseq *ts) : res(res_of(ts)) {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if (res == nullptr) return;
+        if (res->idle.empty()) {
+            Workspace *w = new (std::nothrow) Workspace();
+            if (w == nullptr) return;
+            if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreate(&w->ev0) != hipSuccess || hipEventCreate(&w->ev1) != hipSuccess) {
+                (void) hipGetLastError();
+                if (w->ev0) (void) hipEventDestroy(w->ev0);
+                if (w->stream) (void) hipStreamDestroy(w->stream);
+                delete w;
+                return;
+            }
+            w->lds_limit = res->lds_limit;
+            w->profile = res->profile;
+            res->all.push_back(w);
+            res->idle.push_back(w);
+        }
+        ws = res->idle.back();
+        res->idle.pop_back();
+        ws->force_full = res->force_full;
+        ws->force_c32 = res->force_c32;
+        ws->last_store = -1;
+    }
+    ~Lease() {
+        if (ws == nullptr) return;
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if (ws->last_store >= 0) res->last_store = ws->last_store;
+        res->idle.push_back(ws);
+    }
+    Lease(const Lease &) = delete;
+    Lease &operator=(const Lease &) = delete;
+};
 
 // Groups (sample, column) pairs by sample so each weighted sample walks its
 // root path once: seed_sample[i] has columns seed_col[seed_off[i]..seed_off[i+1]).
@@ -282,9 +336,14 @@ int ensure_full(tsg_treeseq *ts);
 
 // Runs a batch of queries (same tree sequence, same mode) in one sweep.
 int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
-    int ret = device_upload(ts);
-    if (ret != 0) return ret;
-    Workspace *ws = ws_of(ts);
+    int ret;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = device_upload(ts))) return ret;
+    }
+    Lease lease(ts);
+    Workspace *ws = lease.ws;
+    if (ws == nullptr) return TSG_ERR_DEVICE;
     const int Q = (int) batch.size();
     const uint32_t mode = batch[0]->mode;
     // fused columns / outputs
@@ -364,6 +423,70 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         }
         tq.M = M - tq.mbase;
     }
+    // Columns no output reads (sets of a k-way query that its indexes never
+    // name) are dropped: the sweep's cost grows with the column count.
+    {
+        std::vector<char> used(K, 0);
+        for (const TsgOut &o : outs) {
+            used[o.i] = 1;
+            switch (o.stat) {
+            case TSG_STAT_DIVERSITY:
+            case TSG_STAT_SEGREGATING_SITES:
+            case TSG_STAT_Y1:
+                break;
+            case TSG_STAT_DIVERGENCE:
+            case TSG_STAT_Y2:
+            case TSG_STAT_F2:
+            case TSG_STAT_RELATEDNESS_NONCENTRED:
+                used[o.j] = 1;
+                break;
+            case TSG_STAT_Y3:
+            case TSG_STAT_F3:
+                used[o.j] = used[o.k] = 1;
+                break;
+            case TSG_STAT_RELATEDNESS:
+                used[o.j] = 1;
+                for (int c = o.cbase; c < o.cbase + o.ccount; c++) used[c] = 1;
+                break;
+            default:
+                used[o.j] = used[o.k] = used[o.l] = 1;
+                break;
+            }
+        }
+        std::vector<int32_t> nid(K, -1);
+        int K2 = 0;
+        for (int c = 0; c < K; c++) {
+            if (used[c]) nid[c] = K2++;
+        }
+        if (K2 < K) {
+            size_t w = 0;
+            for (size_t i = 0; i < pair_c.size(); i++) {
+                if (!used[pair_c[i]]) continue;
+                pair_s[w] = pair_s[i];
+                pair_c[w] = nid[pair_c[i]];
+                w++;
+            }
+            pair_s.resize(w);
+            pair_c.resize(w);
+            std::vector<int32_t> tot2(K2);
+            for (int c = 0; c < K; c++) {
+                if (used[c]) tot2[nid[c]] = col_total[c];
+            }
+            col_total.swap(tot2);
+            // the relatedness mean's columns are all kept, so a query's
+            // remaining columns stay contiguous and in order
+            for (TsgOut &o : outs) {
+                const int32_t i2 = nid[o.i];
+                o.j = used[o.j] ? nid[o.j] : i2;
+                o.k = used[o.k] ? nid[o.k] : i2;
+                o.l = used[o.l] ? nid[o.l] : i2;
+                o.i = i2;
+                o.cbase = (o.stat == TSG_STAT_RELATEDNESS) ? nid[o.cbase] : 0;
+            }
+            for (TsgQuery &tq : qs) tq.cbase = 0;  // (not read by the kernels)
+            K = K2;
+        }
+    }
     // Plan and node store: the hot plan with its slots in LDS when the
     // largest chunk's slots fit; otherwise the full plan, in LDS when its
     // largest chunk fits, else in per-chunk HBM slabs.
@@ -378,7 +501,10 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
                      *std::max_element(col_total.begin(), col_total.end()) <= 65535;
     const size_t hot_need = (size_t) ts->hot.max_slots * TSG_SLOT_WORDS(K, c16) * 4 + stage;
     const bool use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
-    if (!use_hot && (ret = ensure_full(ts))) return ret;
+    if (!use_hot) {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = ensure_full(ts))) return ret;
+    }
     const int C = use_hot ? ts->hot.num_chunks : ts->num_chunks;
     const int32_t *ct0 = use_hot ? ts->hot.chunk_tree0.data() : ts->chunk_tree0.data();
     // per (chunk, query) window ranges
@@ -621,14 +747,12 @@ int run_one(tsg_treeseq *ts, Query &q) {
         g_capture->push_back(std::make_unique<Query>(std::move(q)));
         return 0;
     }
-    std::lock_guard<std::mutex> lock(g_mutex);
     std::vector<Query *> b{&q};
     return run_batch(ts, b);
 }
 
 // ---- device residency -----------------------------------------------------
-Workspace *ws_of(tsg_treeseq *ts) { return reinterpret_cast<Workspace *>(ts->workspace); }
-
+// Called with g_mutex held.
 int device_upload(tsg_treeseq *ts) {
     if (ts->dev.device >= 0) return 0;
     int count = 0;
@@ -638,20 +762,20 @@ int device_upload(tsg_treeseq *ts) {
     }
     int dev = 0;
     if (hip_check(hipGetDevice(&dev))) return TSG_ERR_NO_DEVICE;
-    Workspace *ws = new (std::nothrow) Workspace();
-    if (ws == nullptr) return TSG_ERR_NO_MEMORY;
-    ts->workspace = ws;
+    Resident *res = res_of(ts);
+    if (res == nullptr) {
+        res = new (std::nothrow) Resident();
+        if (res == nullptr) return TSG_ERR_NO_MEMORY;
+        ts->workspace = res;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+            // gfx950 has 160 KiB of LDS per CU, all of it addressable by one
+            // workgroup
+            res->lds_limit = (int) std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
+        }
+        tsg_set_lds_max(res->lds_limit);
+    }
     int ret;
-    if ((ret = hip_check(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking)))) return ret;
-    if ((ret = hip_check(hipEventCreate(&ws->ev0))) || (ret = hip_check(hipEventCreate(&ws->ev1)))) {
-        return ret;
-    }
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
-        // gfx950 has 160 KiB of LDS per CU, all of it addressable by one
-        // workgroup
-        ws->lds_limit = (int) std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
-    }
     TsgDevice &d = ts->dev;
     size_t &by = d.bytes;
     std::vector<int32_t> site_nalleles = ts->site_nalleles;
@@ -743,9 +867,7 @@ int ensure_hot_afs(tsg_treeseq *ts) {
     return 0;
 }
 
-void free_workspace(tsg_treeseq *ts) {
-    Workspace *ws = ws_of(ts);
-    if (ws == nullptr) return;
+void free_one_workspace(Workspace *ws) {
     DevBuf *bufs[] = {&ws->seed_off, &ws->seed_ps, &ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
                       &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
                       &ws->Rstart, &ws->raw, &ws->rawbase, &ws->lanebase, &ws->out, &ws->fin, &ws->ipar,
@@ -760,6 +882,14 @@ void free_workspace(tsg_treeseq *ts) {
     if (ws->ev1) (void) hipEventDestroy(ws->ev1);
     if (ws->stream) (void) hipStreamDestroy(ws->stream);
     delete ws;
+}
+
+// Frees every workspace of the tree sequence; no query may be running.
+void free_workspace(tsg_treeseq *ts) {
+    Resident *res = res_of(ts);
+    if (res == nullptr) return;
+    for (Workspace *ws : res->all) free_one_workspace(ws);
+    delete res;
     ts->workspace = nullptr;
 }
 
@@ -956,8 +1086,8 @@ int tsg_treeseq_set_sweep_plan(tsg_treeseq *ts, int plan) {
     std::lock_guard<std::mutex> lock(g_mutex);
     int ret = device_upload(ts);
     if (ret != 0) return ret;
-    ws_of(ts)->force_full = plan == 1;
-    ws_of(ts)->force_c32 = plan == 2;
+    res_of(ts)->force_full = plan == 1;
+    res_of(ts)->force_c32 = plan == 2;
     return 0;
 }
 
@@ -965,8 +1095,9 @@ int tsg_treeseq_set_sweep_plan(tsg_treeseq *ts, int plan) {
 // (full plan), 2 LDS hot slots, 3 LDS hot slots with 16-bit counts; -1
 // before the first sweep.
 int tsg_treeseq_get_last_store(const tsg_treeseq *ts) {
-    const Workspace *ws = ws_of(const_cast<tsg_treeseq *>(ts));
-    return ws == nullptr ? -1 : ws->last_store;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    const Resident *res = res_of(ts);
+    return res == nullptr ? -1 : res->last_store;
 }
 
 int tsg_treeseq_upload(tsg_treeseq *ts) {
@@ -1127,27 +1258,43 @@ int tsg_treeseq_copy_hot_afs_plan(const tsg_treeseq *ts, int32_t *slot_node, int
 int tsg_treeseq_set_profiling(tsg_treeseq *ts, int enable) {
     int ret = tsg_treeseq_upload(ts);
     if (ret != 0) return ret;
-    Workspace *ws = ws_of(ts);
-    ws->profile = enable != 0;
-    ws->sweep_ms = 0;
-    ws->sweep_launches = 0;
-    ws->mode_ms[0] = ws->mode_ms[1] = 0;
-    ws->mode_launches[0] = ws->mode_launches[1] = 0;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Resident *res = res_of(ts);
+    res->profile = enable != 0;
+    for (Workspace *ws : res->all) {
+        ws->profile = enable != 0;
+        ws->sweep_ms = 0;
+        ws->sweep_launches = 0;
+        ws->mode_ms[0] = ws->mode_ms[1] = 0;
+        ws->mode_launches[0] = ws->mode_launches[1] = 0;
+    }
     return 0;
 }
 int tsg_treeseq_get_profile(const tsg_treeseq *ts, double *sweep_ms, uint64_t *launches) {
-    const Workspace *ws = reinterpret_cast<const Workspace *>(ts->workspace);
-    *sweep_ms = ws ? ws->sweep_ms : 0.0;
-    *launches = ws ? ws->sweep_launches : 0;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    const Resident *res = res_of(ts);
+    *sweep_ms = 0.0;
+    *launches = 0;
+    if (res == nullptr) return 0;
+    for (const Workspace *ws : res->all) {
+        *sweep_ms += ws->sweep_ms;
+        *launches += ws->sweep_launches;
+    }
     return 0;
 }
 
 // The same per statistics mode: ms[0] / launches[0] branch sweeps, [1] site.
 int tsg_treeseq_get_profile_modes(const tsg_treeseq *ts, double *ms, uint64_t *launches) {
-    const Workspace *ws = reinterpret_cast<const Workspace *>(ts->workspace);
+    std::lock_guard<std::mutex> lock(g_mutex);
+    const Resident *res = res_of(ts);
     for (int m = 0; m < 2; m++) {
-        ms[m] = ws ? ws->mode_ms[m] : 0.0;
-        launches[m] = ws ? ws->mode_launches[m] : 0;
+        ms[m] = 0.0;
+        launches[m] = 0;
+        if (res == nullptr) continue;
+        for (const Workspace *ws : res->all) {
+            ms[m] += ws->mode_ms[m];
+            launches[m] += ws->mode_launches[m];
+        }
     }
     return 0;
 }
@@ -1392,10 +1539,14 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_
         for (int w = 0; w < W; w++) scale[w] = 1.0 / (win[w + 1] - win[w]);
     }
 
-    std::lock_guard<std::mutex> lock(g_mutex);
-    int ret = ensure_full(ts);
-    if (ret != 0) return ret;
-    Workspace *ws = ws_of(ts);
+    int ret;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = ensure_full(ts))) return ret;
+    }
+    Lease lease(ts);
+    Workspace *ws = lease.ws;
+    if (ws == nullptr) return TSG_ERR_DEVICE;
     hipStream_t s = ws->stream;
     const int slots = std::min(ts->num_chunks, 512);
     const int64_t gstride = (int64_t) ts->max_nloc * (1 + NW);
@@ -1501,7 +1652,6 @@ uint64_t tsg_batch_get_num_queries(const tsg_batch *b) { return b->queries.size(
 // queries and one for the site-mode queries.  Results land in the buffers
 // given when the queries were added.
 int tsg_batch_run(tsg_batch *b) {
-    std::lock_guard<std::mutex> lock(g_mutex);
     for (uint32_t mode : {TSG_STAT_BRANCH, TSG_STAT_SITE}) {
         std::vector<Query *> group;
         for (auto &q : b->queries) {
@@ -1594,9 +1744,14 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
     if (afs_size_d * W * TW > 4e9) return TSG_ERR_UNSUPPORTED_INPUT;
     const int64_t afs_size = (int64_t) afs_size_d;
 
-    std::lock_guard<std::mutex> lock(g_mutex);
-    if ((ret = device_upload(ts))) return ret;
-    Workspace *ws = ws_of(ts);
+    bool use_hot;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = device_upload(ts))) return ret;
+    }
+    Lease lease(ts);
+    Workspace *ws = lease.ws;
+    if (ws == nullptr) return TSG_ERR_DEVICE;
     hipStream_t s = ws->stream;
     const int KK = (int) K + 1;
     // The hot plan when its slots fit in LDS: per slot KK counts and, for
@@ -1604,8 +1759,11 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
     // insertion); then one staging row.
     const size_t hot_need = (size_t) ts->hot.max_slots * ((size_t) KK * 4 + (branch ? 32 : 0)) +
                             (size_t) KK * 4 + 64;
-    const bool use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
-    if ((ret = use_hot ? ensure_hot_afs(ts) : ensure_full(ts))) return ret;
+    use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = use_hot ? ensure_hot_afs(ts) : ensure_full(ts))) return ret;
+    }
     const int C = use_hot ? ts->hot.num_chunks : ts->num_chunks;
     std::vector<int32_t> pair_s, pair_c, col_total;
     for (uint64_t k = 0; k < K; k++) {

@@ -1622,12 +1622,19 @@ __global__ void k_fill_i32(int32_t *p, int64_t n) {
 #include "tsg_launch.h"
 
 namespace {
+// Dynamic LDS a workgroup may be given (the device's per-workgroup limit).
+// Kernels that take more than 64 KiB get this limit as their attribute --
+// one value for every launch, so host threads launching concurrently never
+// undo each other's setting.
+int g_lds_max = 65536;
+
 template <int MODE, int STORE, int MPL>
 hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
     const void *fn = reinterpret_cast<const void *>(&k_sweep<MODE, STORE, MPL>);
     if (shmem > 65536) {
+        if ((int) shmem > g_lds_max) return hipErrorInvalidValue;
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int) shmem);
+                                           g_lds_max);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_sweep<MODE, STORE, MPL>), dim3(a.num_chunks), dim3(WAVE), shmem, s, a);
@@ -1662,6 +1669,8 @@ hipError_t launch_sweep_store(const TsgSweepArgs &a, int store, int mpl, size_t 
 }
 }  // namespace
 
+void tsg_set_lds_max(int bytes) { g_lds_max = std::max(bytes, 65536); }
+
 hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl, size_t shmem,
                             hipStream_t s) {
     return (mode == 0) ? launch_sweep_store<0>(a, store, mpl, shmem, s)
@@ -1672,8 +1681,9 @@ namespace {
 template <typename F>
 hipError_t allow_lds(F *fn, size_t shmem) {
     if (shmem <= 65536) return hipSuccess;
+    if ((int) shmem > g_lds_max) return hipErrorInvalidValue;
     return hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) shmem);
+                               hipFuncAttributeMaxDynamicSharedMemorySize, g_lds_max);
 }
 }  // namespace
 

@@ -4,6 +4,8 @@
 
 #include "tsg_kernels.h"
 
+// the device's dynamic LDS limit per workgroup (set once, before launches)
+void tsg_set_lds_max(int bytes);
 // store: TSG_STORE_GLOBAL / TSG_STORE_LDS / TSG_STORE_HOT
 hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl, size_t shmem,
                             hipStream_t s);

@@ -3,20 +3,22 @@ batched into fused sweeps.
 
 ``StatServer`` keeps a set of tree sequences uploaded and answers requests
 (``Request``: a tree-sequence id, a statistic name and its keyword
-arguments, as for ``TreeSequence``).  Each ``step`` drains the queue with
-continuous batching: waiting requests are grouped by (tree sequence, mode)
-and packed into launches of at most ``max_columns`` distinct sample sets and
-``max_outputs`` outputs, admitted while the launch's estimated device memory
-stays within ``launch_budget`` bytes; a request that does not fit starts the
-next launch.  Allele frequency spectra run one per launch (they are not
-fused).  ``one_per_launch=True`` serves every request alone, the baseline the
-batched loop is measured against.
+arguments, as for ``TreeSequence``).  Waiting requests are grouped by (tree
+sequence, mode) and packed into launches of at most ``max_columns`` distinct
+sample sets and ``max_outputs`` outputs, admitted while the launch's
+estimated device memory stays within ``launch_budget`` bytes; a request that
+does not fit starts the next launch.  Allele frequency spectra run one per
+launch (they are not fused).  Up to ``max_inflight`` launches run
+concurrently on their own streams.  ``one_per_launch=True, max_inflight=1``
+serves every request alone, one launch at a time: the plain path the served
+loop is measured against.
 
 Results do not depend on the company a request keeps: a fused sweep computes
 every output from its own columns in the same order as a sweep of that
 request alone.
 """
 import collections
+import concurrent.futures
 import time
 
 import numpy as np
@@ -71,8 +73,16 @@ def launch_bytes(k, m, nw, chunks):
 
 
 class StatServer:
+    """Continuous batching over resident tree sequences.
+
+    Up to ``max_inflight`` launches run at once, each from its own host
+    thread on its own device stream (the library leases every call a
+    workspace with its own stream), so launches on different tree sequences
+    -- or on the same one -- overlap on the device.  Requests that arrive
+    while launches are in flight are admitted as soon as a slot frees."""
+
     def __init__(self, tree_sequences, launch_budget=2 << 30, max_columns=64, max_outputs=4096,
-                 one_per_launch=False):
+                 one_per_launch=False, max_inflight=4):
         self.ts = list(tree_sequences)
         for ts in self.ts:
             ts.ll_tree_sequence.upload()
@@ -82,8 +92,16 @@ class StatServer:
         self.max_columns = max_columns
         self.max_outputs = max_outputs
         self.one_per_launch = one_per_launch
+        self.max_inflight = max(1, int(max_inflight))
+        self.pool = (concurrent.futures.ThreadPoolExecutor(max_workers=self.max_inflight)
+                     if self.max_inflight > 1 else None)
         self.queue = collections.deque()
         self.launches = 0
+
+    def close(self):
+        if self.pool is not None:
+            self.pool.shutdown(wait=True)
+            self.pool = None
 
     def submit(self, req):
         self.queue.append(req)
@@ -109,12 +127,9 @@ class StatServer:
             launches.append(cur)
         return launches
 
-    def step(self):
-        """Serves every request queued now; returns the requests completed."""
-        if not self.queue:
-            return []
-        pending = list(self.queue)
-        self.queue.clear()
+    def _plan(self, pending):
+        """Launches for the pending requests: fused groups per (tree
+        sequence, mode), each allele frequency spectrum alone."""
         groups = collections.OrderedDict()
         alone = []
         for r in pending:
@@ -126,37 +141,60 @@ class StatServer:
         for (tid, _), reqs in groups.items():
             work.extend(self._pack(reqs, self.chunks[tid]))
         work.extend(alone)
-        done = []
-        for launch in work:
-            self.launches += 1
-            t0 = time.perf_counter()
-            ts = self.ts[launch[0].ts_id]
-            if len(launch) == 1 and launch[0].stat not in FUSABLE:
-                r = launch[0]
+        return work
+
+    def _run(self, launch):
+        t0 = time.perf_counter()
+        ts = self.ts[launch[0].ts_id]
+        if len(launch) == 1 and launch[0].stat not in FUSABLE:
+            r = launch[0]
+            r.t_start = t0
+            r.result = getattr(ts, r.stat)(**r.kwargs)
+        else:
+            b = ts.batch()
+            pend = [getattr(b, r.stat)(**r.kwargs) for r in launch]
+            b.run()
+            for r, p in zip(launch, pend):
                 r.t_start = t0
-                r.result = getattr(ts, r.stat)(**r.kwargs)
-            else:
-                b = ts.batch()
-                pend = [getattr(b, r.stat)(**r.kwargs) for r in launch]
-                b.run()
-                for r, p in zip(launch, pend):
-                    r.t_start = t0
-                    r.result = p.result()
-            t1 = time.perf_counter()
-            for r in launch:
-                r.t_done = t1
-                r.launch = self.launches
-            done.extend(launch)
+                r.result = p.result()
+        t1 = time.perf_counter()
+        for r in launch:
+            r.t_done = t1
+        return launch
+
+    def _start(self, launch):
+        self.launches += 1
+        for r in launch:
+            r.launch = self.launches
+        if self.pool is None:
+            f = concurrent.futures.Future()
+            f.set_result(self._run(launch))
+            return f
+        return self.pool.submit(self._run, launch)
+
+    def step(self):
+        """Serves every request queued now; returns the requests completed."""
+        if not self.queue:
+            return []
+        pending = list(self.queue)
+        self.queue.clear()
+        futures = [self._start(launch) for launch in self._plan(pending)]
+        done = []
+        for f in futures:
+            done.extend(f.result())
         return done
 
     def run_open_loop(self, requests, arrivals):
-        """Replays `requests` arriving at `arrivals` (seconds from now):
-        every step serves what has arrived.  Returns the wall time."""
+        """Replays `requests` arriving at `arrivals` (seconds from now) with
+        continuous batching: whenever a launch slot is free, the requests
+        that have arrived are packed into launches and started.  Returns the
+        wall time."""
         order = np.argsort(arrivals, kind="stable")
         t_begin = time.perf_counter()
         i = 0
         n = len(order)
         served = 0
+        inflight = set()
         while served < n:
             now = time.perf_counter() - t_begin
             while i < n and arrivals[order[i]] <= now:
@@ -164,10 +202,24 @@ class StatServer:
                 r.t_arrival = t_begin + arrivals[order[i]]
                 self.submit(r)
                 i += 1
-            if not self.queue:
+            free = self.max_inflight - len(inflight)
+            if free > 0 and self.queue:
+                work = self._plan(list(self.queue))
+                self.queue.clear()
+                for launch in work[free:]:  # no slot yet: wait in the queue
+                    self.queue.extend(launch)
+                for launch in work[:free]:
+                    inflight.add(self._start(launch))
+            if inflight:
+                wait = None if i >= n else max(0.0, arrivals[order[i]] - now)
+                if free > 0 and wait is not None:
+                    wait = min(wait, 1e-3)
+                done, inflight = concurrent.futures.wait(
+                    inflight, timeout=wait, return_when=concurrent.futures.FIRST_COMPLETED)
+                for f in done:
+                    served += len(f.result())
+            elif i < n:
                 time.sleep(max(0.0, min(1e-3, arrivals[order[i]] - now)))
-                continue
-            served += len(self.step())
         return time.perf_counter() - t_begin
 
 
