@@ -499,7 +499,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     // 16-bit slot counts when no sample set exceeds 65535 samples
     const bool c16 = !ws->force_c32 &&
                      *std::max_element(col_total.begin(), col_total.end()) <= 65535;
-    const size_t hot_need = (size_t) ts->hot.max_slots * TSG_SLOT_WORDS(K, c16) * 4 + stage;
+    const size_t hot_need =
+        (size_t) ts->hot.max_slots * TSG_SLOT_WORDS(K, c16, mode == TSG_STAT_BRANCH) * 4 + stage;
     const bool use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
     if (!use_hot) {
         std::lock_guard<std::mutex> lock(g_mutex);
@@ -1050,7 +1051,7 @@ int tsg_treeseq_init(tsg_treeseq **out, double sequence_length, uint64_t num_nod
                              derived_state_offset);
     if (ret != 0) return ret;
     ts->full_chunks = default_num_chunks(ts.get());
-    ret = tsg_build_hot_plan(ts.get(), default_hot_chunks(ts.get()));
+    ret = tsg_build_hot_plan(ts.get(), default_hot_chunks(ts.get()), true);
     if (ret != 0) return ret;
     *out = ts.release();
     return 0;
@@ -1075,7 +1076,7 @@ int tsg_treeseq_set_num_chunks(tsg_treeseq *ts, uint32_t num_chunks) {
     tsg_hot_device_free(&ts->hot.dev);
     ts->full_chunks = (int32_t) std::min<uint32_t>(num_chunks, INT32_MAX);
     ts->full_built = false;
-    return tsg_build_hot_plan(ts, ts->full_chunks);
+    return tsg_build_hot_plan(ts, ts->full_chunks, false);
 }
 
 // Testing hook: plan = 1 runs every sweep on the full plan, 2 on the hot

@@ -49,8 +49,9 @@ struct TsgQuery {
 #define TSG_STORE_LDS 1
 #define TSG_STORE_HOT 2
 #define TSG_STORE_HOT16 3
-// 32-bit words per hot slot: FP64 branch length + K counts (16-bit when c16)
-#define TSG_SLOT_WORDS(K, c16) (2 + ((c16) ? ((K) + 1) / 2 : (K)))
+// 32-bit words per hot slot: FP64 branch length (branch sweeps, bl) + K
+// counts (16-bit when c16)
+#define TSG_SLOT_WORDS(K, c16, bl) (((bl) ? 2 : 0) + ((c16) ? ((K) + 1) / 2 : (K)))
 // 32-bit words per full-plan node record: parent, FP64 branch length, K counts
 #define TSG_NODE_WORDS(K) (3 + (K))
 

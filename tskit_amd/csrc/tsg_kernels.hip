@@ -138,35 +138,42 @@ struct Nodes {
     }
 };
 
-// Hot-plan slot storage in LDS: rec[h*R..+1] = FP64 branch length, then the
-// slot's K counts, as 32-bit words or, when every sample set has at most
-// 65535 samples (C16), as 16-bit halves (R = 2 + K or 2 + ceil(K / 2)
-// words).  No parent pointers: the root paths are precomputed.
-template <bool C16>
+// Hot-plan slot storage in LDS: (BL, branch mode) rec[h*R..+1] = FP64
+// branch length, then the slot's K counts, as 32-bit words or, when every
+// sample set has at most 65535 samples (C16), as 16-bit halves (R =
+// TSG_SLOT_WORDS(K, C16, BL)).  Site sweeps keep no branch lengths.  No
+// parent pointers: the root paths are precomputed.
+template <bool C16, bool BL>
 struct Slots {
+    static constexpr int OFF = BL ? 2 : 0;  // words before the counts
     int32_t *rec;
     int32_t R;
-    __device__ double bl(int h) const { return ld_f64(rec + h * R); }
-    __device__ void set_bl(int h, double b) const { st_f64(rec + h * R, b); }
+    __device__ double bl(int h) const {
+        if constexpr (BL) return ld_f64(rec + h * R);
+        return 0.0;
+    }
+    __device__ void set_bl(int h, double b) const {
+        if constexpr (BL) st_f64(rec + h * R, b);
+    }
     __device__ int32_t get(int h, int k) const {
-        if (C16) return reinterpret_cast<const uint16_t *>(rec + h * R + 2)[k];
-        return rec[h * R + 2 + k];
+        if (C16) return reinterpret_cast<const uint16_t *>(rec + h * R + OFF)[k];
+        return rec[h * R + OFF + k];
     }
     __device__ void put(int h, int k, int32_t v) const {
         if (C16) {
-            reinterpret_cast<uint16_t *>(rec + h * R + 2)[k] = (uint16_t) v;
+            reinterpret_cast<uint16_t *>(rec + h * R + OFF)[k] = (uint16_t) v;
         } else {
-            rec[h * R + 2 + k] = v;
+            rec[h * R + OFF + k] = v;
         }
     }
     // atomic += v (v >= 0); with 16-bit halves no carry crosses into the
     // neighbour, since no count exceeds its set size
     __device__ void add(int h, int k, int32_t v) const {
         if (C16) {
-            atomicAdd(reinterpret_cast<unsigned *>(rec + h * R + 2 + (k >> 1)),
+            atomicAdd(reinterpret_cast<unsigned *>(rec + h * R + OFF + (k >> 1)),
                       (unsigned) v << (16 * (k & 1)));
         } else {
-            atomicAdd(&rec[h * R + 2 + k], v);
+            atomicAdd(&rec[h * R + OFF + k], v);
         }
     }
 };
@@ -194,14 +201,16 @@ __device__ void init_chunk(const TsgSweepArgs &a, const Nodes &nd, int64_t n0, i
 // add every weighted sample into its nearest hot ancestor-or-self, then add
 // each depth level of the hot forest into its parents, deepest level first
 // (the counts of a level are complete before it is added upward).
-template <bool C16>
-__device__ void init_hot(const TsgSweepArgs &a, const Slots<C16> &nd, int c, int nslot, int K,
-                         int lane) {
+template <bool C16, bool BL>
+__device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c, int nslot,
+                         int K, int lane) {
     const int64_t s0 = a.chunk_node0[c];
     const int words = nslot * nd.R;
     for (int i = lane; i < words; i += WAVE) nd.rec[i] = 0;
     __syncthreads();
-    for (int h = lane; h < nslot; h += WAVE) nd.set_bl(h, a.loc_bl0[s0 + h]);
+    if constexpr (BL) {
+        for (int h = lane; h < nslot; h += WAVE) nd.set_bl(h, a.loc_bl0[s0 + h]);
+    }
     // (sample, column) pairs, several per lane in flight: the slot gathers
     // are independent HBM reads
     const uint16_t *ss = a.sample_slot + (int64_t) c * a.num_samples;
@@ -564,9 +573,10 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     int32_t *zero = tot + K;                                             // [K]
     int32_t *act = zero + K;                                             // [span]
     constexpr bool HOTS = STORE == TSG_STORE_HOT || STORE == TSG_STORE_HOT16;
-    using NS = typename std::conditional<HOTS, Slots<STORE == TSG_STORE_HOT16>, Nodes>::type;
+    using NS = typename std::conditional<HOTS, Slots<STORE == TSG_STORE_HOT16, MODE == 0>,
+                                         Nodes>::type;
     NS nd;
-    nd.R = HOTS ? TSG_SLOT_WORDS(K, STORE == TSG_STORE_HOT16) : R;
+    nd.R = HOTS ? TSG_SLOT_WORDS(K, STORE == TSG_STORE_HOT16, MODE == 0) : R;
     if (STORE != TSG_STORE_GLOBAL) {
         nd.rec = smem + TSG_STAGE_INTS(K, span);
     } else {

@@ -641,12 +641,13 @@ void plan_hot_range(const tsg_treeseq *ts, tsg_hotplan &hp, int32_t cbeg, int32_
 // and derive the slot structure there, then apply them again.  Runs of
 // chunks are planned by separate threads, each replaying the edges up to its
 // first chunk.
-int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
+namespace {
+int build_hot_plan_bounds(tsg_treeseq *ts, std::vector<int32_t> bounds) {
     tsg_hotplan &hp = ts->hot;
     hp = tsg_hotplan();
     const int32_t T = ts->num_trees, E = ts->num_edges, N = ts->num_nodes;
     const int32_t n = ts->num_samples;
-    hp.chunk_tree0 = chunk_bounds(ts, num_chunks);
+    hp.chunk_tree0 = std::move(bounds);
     const int32_t C = (int32_t) hp.chunk_tree0.size() - 1;
     hp.num_chunks = C;
     hp.tr_off.assign(T + 1, 0);
@@ -778,4 +779,44 @@ int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks) {
     }
     hp.usable = true;
     return 0;
+}
+}  // namespace
+
+// Chunks are first cut by edge events (about equal); with balance, chunks
+// whose hot slots exceed 1.15x the mean are cut again (at tree boundaries,
+// by events) and the plan rebuilt: the sweep's LDS per workgroup is set by
+// the largest chunk, so trimming the tail of the slot distribution lets more
+// chunks share a CU.
+int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks, bool balance) {
+    int ret = build_hot_plan_bounds(ts, chunk_bounds(ts, num_chunks));
+    tsg_hotplan &hp = ts->hot;
+    if (ret != 0 || !balance || !hp.usable || hp.num_chunks < 2) return ret;
+    const int32_t C = hp.num_chunks, T = ts->num_trees;
+    const double mean = (double) hp.chunk_slot0[C] / C;
+    const double target = std::max(1.15 * mean, 64.0);
+    if ((double) hp.max_slots <= 1.05 * target) return 0;
+    std::vector<int32_t> nb{0};
+    for (int32_t c = 0; c < C; c++) {
+        const int32_t t0 = hp.chunk_tree0[c], t1 = hp.chunk_tree0[c + 1];
+        const double slots = (double) (hp.chunk_slot0[c + 1] - hp.chunk_slot0[c]);
+        const int32_t parts = std::min<int32_t>(t1 - t0, (int32_t) std::ceil(slots / target));
+        if (parts > 1) {
+            // events of the chunk's transitions t0+1 .. t1 (t1: the closing one)
+            const int32_t tl = std::min(t1, T - 1);
+            const int64_t ev = hp.tr_off[tl + 1] - hp.tr_off[t0 + 1];
+            int32_t made = 1;
+            for (int32_t t = t0 + 1; t < t1 && made < parts; t++) {
+                // cut before tree t when the events up to transition t reach
+                // the next share
+                if ((hp.tr_off[t + 1] - hp.tr_off[t0 + 1]) * parts >= ev * made) {
+                    nb.push_back(t);
+                    made++;
+                }
+            }
+        }
+        nb.push_back(t1);
+    }
+    const size_t nc = nb.size() - 1;
+    if ((int64_t) nc == C || (double) nc * ts->num_samples * 2 > (double) (5ll << 30)) return 0;
+    return build_hot_plan_bounds(ts, std::move(nb));
 }

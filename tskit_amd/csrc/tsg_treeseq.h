@@ -172,6 +172,6 @@ struct tsg_treeseq {
 };
 
 int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks);
-int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks);
+int tsg_build_hot_plan(tsg_treeseq *ts, int32_t num_chunks, bool balance);
 int tsg_upload(tsg_treeseq *ts);
 void tsg_device_free(TsgDevice *d);
