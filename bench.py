@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30
 
 
 def log(*a):
@@ -201,6 +202,52 @@ def served_accuracy(tss, om, kind, count=16, seed=9):
     return worst
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_sweep_traffic.json")
+SWEEP_KERNEL = "k_sweep<0, 3, 1>"
+
+
+def pmc_summary(fetch_csv, write_csv):
+    """HBM bytes per dispatch of the branch sweep from rocprofv3 counter
+    CSVs: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) averaged over its
+    dispatches, each scaled by bytes / counter of the calibration kernel
+    (torch.neg over COPY_BYTES: reads and writes COPY_BYTES)."""
+    import csv
+
+    def means(path):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+    out = {"workload": workload(False)["name"], "kernel": SWEEP_KERNEL}
+    for tag, path in [("fetch", fetch_csv), ("write", write_csv)]:
+        m = means(path)
+        sweep = [v for k, v in m.items() if SWEEP_KERNEL in k]
+        calib = max((v for k, v in m.items() if "neg" in k), default=None)
+        if not sweep or calib is None:
+            raise SystemExit(f"{path}: no {SWEEP_KERNEL} or calibration dispatches")
+        scale = COPY_BYTES / (calib * 1024.0)
+        out[f"{tag}_kib_per_dispatch"] = sweep[0]
+        out[f"calibration_{tag}_kib"] = calib
+        out[f"{tag}_scale"] = scale
+        out[f"{tag}_bytes"] = sweep[0] * 1024.0 * scale
+    out["traffic_bytes"] = out["fetch_bytes"] + out["write_bytes"]
+    os.makedirs(os.path.dirname(TRAFFIC_JSON), exist_ok=True)
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
+def measured_traffic(cfg_name):
+    """traffic_bytes of the committed counter summary for this workload."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("traffic_bytes") if t.get("workload") == cfg_name else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,7 +258,17 @@ def main():
     ap.add_argument("--chunks", type=int, default=0, help="override the chunk count")
     ap.add_argument("--no-serve", action="store_true", help="skip the served phase")
     ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
+                         "`bench.py --no-serve --no-cpu --copy-calibration` into "
+                         "profiles/k_sweep_traffic.json and exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the steps, copy 1 GiB device to device three times (the "
+                         "reference kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -285,10 +342,21 @@ def main():
     avg_launch_s = mode_ms["branch"] / 1e3
     achieved = alg_bytes / avg_launch_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None if args.small else measured_traffic(cfg["name"]),
                 "kernel": "k_sweep<0> (branch, fused batch)", "bytes_per_launch": alg_bytes,
                 "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                 "site_sweep_ms": round(mode_ms["site"], 4)}
+
+    if args.copy_calibration:
+        # an elementwise kernel reading and writing COPY_BYTES each
+        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"copy calibration: 3 x {COPY_BYTES} bytes read and written")
 
     served, serve_tss = None, None
     if not args.no_serve and not args.small:

@@ -20,6 +20,7 @@
 #include <new>
 #include <numeric>
 #include <unordered_set>
+#include <thread>
 #include <vector>
 
 #include "tsg_common.h"
@@ -753,6 +754,36 @@ int run_one(tsg_treeseq *ts, Query &q) {
 }
 
 // ---- device residency -----------------------------------------------------
+// The device layout of the hot plan's sample map (host: [C x n], chunk
+// major): per block of TSG_SS_CB chunks, per sample, the block's chunks --
+// ((c / CB) * n + s) * CB + c % CB, the last block padded.  Cache-blocked
+// transposition over threads.
+std::vector<uint16_t> block_sample_map(const std::vector<uint16_t> &map, int32_t C, int32_t n) {
+    const int64_t CB = TSG_SS_CB, nb = (C + CB - 1) / CB;
+    std::vector<uint16_t> out((size_t) (nb * CB * n), (uint16_t) TSG_NO_SLOT);
+    auto work = [&](int64_t b0, int64_t b1) {
+        constexpr int64_t SB = 256;
+        for (int64_t b = b0; b < b1; b++) {
+            const int64_t c0 = b * CB, c1 = std::min<int64_t>(C, c0 + CB);
+            for (int64_t s0 = 0; s0 < n; s0 += SB) {
+                const int64_t s1 = std::min<int64_t>(n, s0 + SB);
+                for (int64_t c = c0; c < c1; c++) {
+                    const uint16_t *src = map.data() + c * n;
+                    uint16_t *dst = out.data() + (b * n) * CB + (c - c0);
+                    for (int64_t s = s0; s < s1; s++) dst[s * CB] = src[s];
+                }
+            }
+        }
+    };
+    const int64_t P = std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t) std::thread::hardware_concurrency(), 16, nb}));
+    std::vector<std::thread> pool;
+    for (int64_t p = 1; p < P; p++) pool.emplace_back(work, nb * p / P, nb * (p + 1) / P);
+    work(0, nb / P);
+    for (auto &th : pool) th.join();
+    return out;
+}
+
 // Called with g_mutex held.
 int device_upload(tsg_treeseq *ts) {
     if (ts->dev.device >= 0) return 0;
@@ -799,7 +830,8 @@ int device_upload(tsg_treeseq *ts) {
             (ret = upload_ptr(&h.slot_bl0, hp.slot_bl0, hb)) ||
             (ret = upload_ptr(&h.chunk_lev0, hp.chunk_lev0, hb)) ||
             (ret = upload_ptr(&h.lev_slot0, hp.lev_slot0, hb)) ||
-            (ret = upload_ptr(&h.sample_slot, hp.sample_slot, hb)) ||
+            (ret = upload_ptr(&h.sample_slot, block_sample_map(hp.sample_slot, hp.num_chunks,
+                                                              ts->num_samples), hb)) ||
             (ret = upload_ptr(&h.tr_off, hp.tr_off, hb)) ||
             (ret = upload_ptr(&h.tr_nrem, hp.tr_nrem, hb)) ||
             (ret = upload_ptr(&h.ev_child, hp.ev_child, hb)) ||

@@ -28,6 +28,11 @@ struct TsgQuery {
     int64_t pbase;       // offset (in doubles) of its partial-sum block
 };
 
+// Chunks per block of the device sample map: one sample's slots in 64
+// consecutive chunks share a 128-byte line, which the chunks' workgroups --
+// placed on one XCD in chunk order -- read through that XCD's L2.
+#define TSG_SS_CB 64
+
 // Hops of an event's root path staged in LDS at once by the sweep kernel.
 #define TSG_PSEG 16
 
@@ -93,7 +98,9 @@ struct TsgSweepArgs {
     const int32_t *slot_par;     // nearest hot proper ancestor, or -1
     const int64_t *chunk_lev0;   // [C+1] into lev_slot0
     const int32_t *lev_slot0;    // per chunk: level boundaries, deepest first
-    const uint16_t *sample_slot; // [C * num_samples]
+    // per block of TSG_SS_CB chunks, per sample, the block's chunks: entry
+    // ((c / CB) * num_samples + s) * CB + c % CB
+    const uint16_t *sample_slot;
     const int32_t *col_total;    // [K] total weight (set size) per column
     const TsgOut *outs;          // [M]
     const TsgQuery *queries;     // [Q]

@@ -197,6 +197,21 @@ __device__ void init_chunk(const TsgSweepArgs &a, const Nodes &nd, int64_t n0, i
     __syncthreads();
 }
 
+// The chunk a sweep workgroup takes: workgroups go to the 8 XCDs in turn
+// (workgroup b to XCD b mod 8), so XCD x is given chunks x*per, x*per + 1,
+// ... in dispatch order and the chunks in flight on one XCD are neighbours
+// (their sample-map lines and plan data share that XCD's L2).  The grid is
+// 8 * per workgroups; those past the last chunk return at once.
+__device__ __forceinline__ int xcd_chunk(int b, int C) {
+    const int per = (C + 7) / 8;
+    return (b % 8) * per + b / 8;
+}
+
+// Chunk c's column of the device sample map (stride TSG_SS_CB per sample).
+__device__ __forceinline__ const uint16_t *sample_map(const TsgSweepArgs &a, int c) {
+    return a.sample_slot + (int64_t) (c / TSG_SS_CB) * a.num_samples * TSG_SS_CB + c % TSG_SS_CB;
+}
+
 // Hot plan: zero the slots' counts, load their start-tree branch lengths,
 // add every weighted sample into its nearest hot ancestor-or-self, then add
 // each depth level of the hot forest into its parents, deepest level first
@@ -212,8 +227,8 @@ __device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c,
         for (int h = lane; h < nslot; h += WAVE) nd.set_bl(h, a.loc_bl0[s0 + h]);
     }
     // (sample, column) pairs, several per lane in flight: the slot gathers
-    // are independent HBM reads
-    const uint16_t *ss = a.sample_slot + (int64_t) c * a.num_samples;
+    // are independent reads
+    const uint16_t *ss = sample_map(a, c);
     constexpr int U = 8;
     for (int i0 = lane; i0 < a.npairs; i0 += WAVE * U) {
         int ps[U], pc[U];
@@ -225,7 +240,9 @@ __device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c,
             pc[r] = (i < a.npairs) ? a.pair_col[i] : 0;
         }
 #pragma unroll
-        for (int r = 0; r < U; r++) ph[r] = (ps[r] >= 0) ? (unsigned) ss[ps[r]] : TSG_NO_SLOT;
+        for (int r = 0; r < U; r++) {
+            ph[r] = (ps[r] >= 0) ? (unsigned) ss[(int64_t) ps[r] * TSG_SS_CB] : TSG_NO_SLOT;
+        }
 #pragma unroll
         for (int r = 0; r < U; r++) {
             if (ph[r] != TSG_NO_SLOT) nd.add((int) ph[r], pc[r], 1);
@@ -543,7 +560,8 @@ __device__ __forceinline__ double branch_summary(const TsgOut &o, const int32_t 
 template <int MODE, int STORE, int MPL>
 __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     extern __shared__ int32_t smem[];
-    const int c = blockIdx.x;
+    const int c = xcd_chunk(blockIdx.x, a.num_chunks);
+    if (c >= a.num_chunks) return;
     const int lane = threadIdx.x;
     const int K = a.K;
     const int R = 3 + K;
@@ -1220,7 +1238,8 @@ __device__ void afs_flush_pieces(const TsgSweepArgs &a, const int32_t *cnt, doub
 template <bool BRANCH>
 __global__ void __launch_bounds__(WAVE) k_afs_hot(const TsgSweepArgs a) {
     extern __shared__ double dsm[];
-    const int c = blockIdx.x;
+    const int c = xcd_chunk(blockIdx.x, a.num_chunks);
+    if (c >= a.num_chunks) return;
     const int lane = threadIdx.x;
     const int K1 = a.afs_K + 1;
     const int64_t s0 = a.chunk_node0[c];
@@ -1241,9 +1260,9 @@ __global__ void __launch_bounds__(WAVE) k_afs_hot(const TsgSweepArgs a) {
             yins[h] = a.slot_ins0[s0 + h];
         }
     }
-    const uint16_t *ss = a.sample_slot + (int64_t) c * a.num_samples;
+    const uint16_t *ss = sample_map(a, c);
     for (int i = lane; i < a.npairs; i += WAVE) {
-        const unsigned h = ss[a.pair_sample[i]];
+        const unsigned h = ss[(int64_t) a.pair_sample[i] * TSG_SS_CB];
         if (h != TSG_NO_SLOT) atomicAdd(&X[h * K1 + a.pair_col[i]], 1);
     }
     __syncthreads();
@@ -1647,7 +1666,8 @@ hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
                                            g_lds_max);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_sweep<MODE, STORE, MPL>), dim3(a.num_chunks), dim3(WAVE), shmem, s, a);
+    const unsigned grid = 8u * (unsigned) ((a.num_chunks + 7) / 8);
+    hipLaunchKernelGGL((k_sweep<MODE, STORE, MPL>), dim3(grid), dim3(WAVE), shmem, s, a);
     return hipGetLastError();
 }
 
@@ -1720,7 +1740,7 @@ hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t s
 }
 
 hipError_t tsg_launch_afs_hot(const TsgSweepArgs &a, bool branch, size_t shmem, hipStream_t s) {
-    dim3 grid(a.num_chunks), block(WAVE);
+    dim3 grid(8u * (unsigned) ((a.num_chunks + 7) / 8)), block(WAVE);
     hipError_t e;
     if (branch) {
         if ((e = allow_lds(&k_afs_hot<true>, shmem)) != hipSuccess) return e;
