@@ -11,16 +11,23 @@ step counts 2 * E * K_q updates (each edge inserted and removed once, for
 each of the K_q sample sets the query takes), whatever the implementation
 fuses or repeats internally; the CPU baseline counts the same.
 
-Served metric (after the steps, unless --no-serve): 8 resident tree
-sequences -- the one above, one of the 1000-sample shape (L = 1e6, seed 1)
-and six of 2e3..7e4 samples (L = 2e6) -- and requests drawn uniformly (a
-tree sequence; diversity, divergence, Fst, f2, f3, f4 or the AFS; 1-64 sample
-sets of 10-1e4 samples; 1-1e4 windows; a mode).  Reported: queries/s of the
-batched serving loop with every request waiting (capacity) against one query
-per launch, and p50/p99 latency of both under Poisson arrivals at half the
-batched capacity.
+Dense phase (unless --no-dense): the site-mode relatedness matrix of 5000
+samples over ~2e5 sites on the int8 matrix cores and the joint 2-set allele
+frequency spectrum (200 + 300 samples, 100 windows), timed, with errors
+against the CPU when the CPU leg runs.
+
+Served metric (unless --no-serve): 8 resident tree sequences -- the one
+above, one of the 1000-sample shape (L = 1e6, seed 1) and six of 2e3..7e4
+samples (L = 2e6) -- and requests drawn uniformly (a tree sequence;
+diversity, divergence, Fst, f2, f3, f4 or the AFS; 1-64 sample sets of
+10-1e4 samples; 1-1e4 windows; a mode).  Reported, on the 8 and on the
+1e5-sample sequence alone: queries/s of the served form (fused launches, up
+to 4 in flight) and of the plain form (one request per launch, one at a
+time) with every request waiting, and p50/p99 latency of both under Poisson
+arrivals at half the plain capacity.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--no-serve]
+       [--no-dense] [--no-cpu]
 """
 import argparse
 import json
@@ -202,6 +209,83 @@ def served_accuracy(tss, om, kind, count=16, seed=9):
     return worst
 
 
+DENSE = dict(n=5000, L=2e7, seed=3, mu=2.75e-8, afs_sets=(200, 300), afs_windows=100)
+INT8_PEAK_TOPS = 5000.0
+
+
+def dense_phase(om, with_cpu):
+    """The dense workload: the site-mode relatedness matrix of 5000 samples
+    (each its own set) over ~2e5 biallelic sites (L = 2e7, seed 3, mu scaled
+    to give 2e5 sites) on the int8 matrix cores, and the joint 2-set allele
+    frequency spectrum (sets of 200 and 300 samples, 100 windows), site and
+    branch mode."""
+    import torch
+
+    import tskit_amd as ta
+
+    d = DENSE
+    tc = ta.simulate_tables(d["n"], d["L"], 1e4, 1e-8, d["mu"], d["seed"])
+    ts = tc.tree_sequence()
+    ts.ll_tree_sequence.upload()
+    S = tc.num_sites
+    out = {"workload": f"{d['n']} samples, L={d['L']:g}, {S} sites (seed {d['seed']}); "
+                       f"AFS sets {d['afs_sets']}, W={d['afs_windows']}",
+           "sites": S}
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps, r
+
+    t_rel, rel = timed(lambda: ts.genetic_relatedness_matrix())
+    ops = 2.0 * d["n"] ** 2 * S
+    out["relatedness_matrix_ms"] = t_rel * 1e3
+    out["relatedness_int_tops_end_to_end"] = ops / t_rel / 1e12
+    out["relatedness_frac_of_int8_peak_end_to_end"] = ops / t_rel / 1e12 / INT8_PEAK_TOPS
+    perm = np.random.default_rng(d["seed"]).permutation(d["n"])
+    a, b = d["afs_sets"]
+    sets = [perm[:a], perm[a:a + b]]
+    w = np.linspace(0, d["L"], d["afs_windows"] + 1)
+    afs = {}
+    for mode in ("site", "branch"):
+        t_afs, afs[mode] = timed(lambda: ts.allele_frequency_spectrum(sets, windows=w, mode=mode))
+        out[f"afs_{mode}_ms"] = t_afs * 1e3
+    log(f"dense: relatedness {d['n']}^2 x {S} sites {t_rel * 1e3:.1f} ms "
+        f"({out['relatedness_int_tops_end_to_end']:.0f} int TOP/s end to end); AFS site "
+        f"{out['afs_site_ms']:.2f} ms, branch {out['afs_branch_ms']:.2f} ms")
+    if with_cpu:
+        impl = om.Reference(tc) if om.reference_available() else om.Oracle(tc)
+        exp = impl.afs(sets, w, None, "site", True, False)[:, 0]
+        out["afs_site_bit_exact"] = bool(np.array_equal(afs["site"], exp))
+        exp = impl.afs(sets, w, None, "branch", True, False)[:, 0]
+        out["afs_branch_max_rel_err"] = float(np.max(np.abs(afs["branch"] - exp)) /
+                                              np.max(np.abs(exp)))
+        # relatedness: a 64 x 64 block of the centred closed form
+        # K = C - r 1^T / n - 1 r^T / n + sum(r) / n^2 (C = G^T G, r = G^T k,
+        # k the allele counts), span-normalised, in FP64 on the CPU from the
+        # oracle's genotype matrix
+        G = om.Oracle(tc).genotypes()
+        n = G.shape[1]
+        k = G.sum(axis=1, dtype=np.int64).astype(np.float64)
+        r = np.zeros(n)
+        for s0 in range(0, G.shape[0], 16384):
+            r += G[s0:s0 + 16384].T.astype(np.float64) @ k[s0:s0 + 16384]
+        idx = np.arange(64)
+        Gb = G[:, idx].astype(np.float64)
+        blk = (Gb.T @ Gb - r[idx, None] / n - r[None, idx] / n + (k @ k) / n**2) / d["L"]
+        out["relatedness_block_max_rel_err"] = float(
+            np.max(np.abs(rel[np.ix_(idx, idx)] - blk)) / np.max(np.abs(blk)))
+        del G
+        log(f"dense: AFS site bit-exact {out['afs_site_bit_exact']}, branch max rel err "
+            f"{out['afs_branch_max_rel_err']:.2e}, relatedness 64x64 block max rel err "
+            f"{out['relatedness_block_max_rel_err']:.2e}")
+    return out
+
+
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_sweep_traffic.json")
 SWEEP_KERNEL = "k_sweep<0, 3, 1>"
 
@@ -257,6 +341,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--chunks", type=int, default=0, help="override the chunk count")
     ap.add_argument("--no-serve", action="store_true", help="skip the served phase")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the relatedness-matrix / joint AFS phase")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
@@ -358,6 +444,10 @@ def main():
         del src, dst
         log(f"copy calibration: 3 x {COPY_BYTES} bytes read and written")
 
+    dense = None
+    if not args.no_dense and not args.small:
+        dense = dense_phase(om, not args.no_cpu)
+
     served, serve_tss = None, None
     if not args.no_serve and not args.small:
         served, serve_tss = served_phase(ts, args.serve_requests, 7)
@@ -392,7 +482,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32 counts; fp32 TwoSum sums; fp64 combine",
+        "dtype": "int32/uint16 counts; fp64 sums, combine and finals",
         "data": "synthetic (own Hudson coalescent + infinite-sites generator)",
         "config": {"workload": cfg["name"], "edges": E, "trees": ts.num_trees, "sites": S,
                    "nodes": N, "queries_per_step": queries_per_step,
@@ -403,6 +493,7 @@ def main():
         "cpu_baseline": cpu,
         "max_rel_err_vs_cpu": errs,
         "served": served,
+        "dense": dense,
         "setup_s": {"generate": round(t_gen, 2), "host_init": round(t_init, 2),
                     "upload": round(t_upload, 3)},
     }
