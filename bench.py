@@ -286,6 +286,117 @@ def dense_phase(om, with_cpu):
     return out
 
 
+def replicate_workload(count=256, seed0=1000):
+    """The replicate batch: `count` tree sequences with log-uniform 1e2..1e4
+    samples and L 1e5..1e7 (r = mu = 1e-8, seeds seed0..), each with 1-32
+    random disjoint sample sets of 10-5000 samples (clipped to fit) and
+    1-2000 equal windows; per replicate and mode the queries are diversity of
+    every set and divergence of consecutive sets."""
+    import tskit_amd as ta
+
+    rng = np.random.default_rng(seed0)
+    reps = []
+    for i in range(count):
+        n = int(round(10 ** rng.uniform(2, 4)))
+        L = float(round(10 ** rng.uniform(5, 7)))
+        tc = ta.simulate_tables(n, L, 1e4, 1e-8, 1e-8, seed0 + i)
+        k = int(rng.integers(1, 33))
+        size = int(rng.integers(10, 5001))
+        size = max(2, min(size, n // k))
+        k = max(1, min(k, n // size))
+        perm = rng.permutation(n)
+        sets = [perm[j * size:(j + 1) * size] for j in range(k)]
+        w = np.linspace(0, L, int(rng.integers(1, 2001)) + 1)
+        pairs = [(j, j + 1) for j in range(k - 1)]
+        reps.append(dict(tc=tc, sets=sets, windows=w, pairs=pairs))
+    return reps
+
+
+def replicate_phase(om, with_cpu, inflight=4):
+    """The replicate batch, plain (every query its own launch, one launch at a
+    time) and batched (one fused launch per replicate and mode, `inflight`
+    launches in flight on their own streams)."""
+    import concurrent.futures
+
+    import torch
+
+    t0 = time.perf_counter()
+    reps = replicate_workload()
+    for r in reps:
+        r["ts"] = r["tc"].tree_sequence()
+        r["ts"].ll_tree_sequence.upload()
+    t_setup = time.perf_counter() - t0
+    edges = sum(r["tc"].num_edges for r in reps)
+    updates = 0
+    for r in reps:
+        k = len(r["sets"])
+        updates += 2 * (2 * r["tc"].num_edges * k) * (2 if r["pairs"] else 1)
+
+    def plain(r):
+        ts, out = r["ts"], {}
+        for mode in ("branch", "site"):
+            out[("diversity", mode)] = ts.diversity(r["sets"], windows=r["windows"], mode=mode)
+            if r["pairs"]:
+                out[("divergence", mode)] = ts.divergence(r["sets"], indexes=r["pairs"],
+                                                          windows=r["windows"], mode=mode)
+        return out
+
+    def fused(r, mode):
+        b = r["ts"].batch()
+        p1 = b.diversity(r["sets"], windows=r["windows"], mode=mode)
+        p2 = b.divergence(r["sets"], indexes=r["pairs"], windows=r["windows"],
+                          mode=mode) if r["pairs"] else None
+        b.run()
+        return {("diversity", mode): p1.result(),
+                ("divergence", mode): p2.result() if p2 else None}
+
+    for r in reps[:8]:  # warm buffers
+        plain(r)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for r in reps:
+        plain(r)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t
+    results = [dict() for _ in reps]
+    with concurrent.futures.ThreadPoolExecutor(max_workers=inflight) as pool:
+        list(pool.map(lambda r: fused(r, "branch"), reps[:8]))
+        t = time.perf_counter()
+        futs = [(i, pool.submit(fused, r, mode)) for i, r in enumerate(reps)
+                for mode in ("branch", "site")]
+        for i, f in futs:
+            results[i].update(f.result())
+        torch.cuda.synchronize()
+        t_batched = time.perf_counter() - t
+    out = {"replicates": len(reps), "edges": edges,
+           "samples_min_max": [int(min(len(r["ts"].samples()) for r in reps)),
+                               int(max(len(r["ts"].samples()) for r in reps))],
+           "setup_s": t_setup, "plain_s": t_plain, "batched_s": t_batched,
+           "speedup": t_plain / t_batched, "updates_per_s_batched": updates / t_batched,
+           "queries_per_s_batched": sum(2 * (2 if r["pairs"] else 1) for r in reps) / t_batched,
+           "form": f"one fused launch per replicate and mode, {inflight} in flight"}
+    log(f"replicates: {len(reps)} tree sequences ({edges} edges) plain {t_plain * 1e3:.0f} ms, "
+        f"batched {t_batched * 1e3:.0f} ms ({t_plain / t_batched:.2f}x)")
+    if with_cpu:
+        worst = 0.0
+        for i in range(0, len(reps), 32):
+            r = reps[i]
+            impl = om.Reference(r["tc"]) if om.reference_available() else om.Oracle(r["tc"])
+            for (stat, mode), got in results[i].items():
+                if got is None:
+                    continue
+                idx = r["pairs"] if stat == "divergence" else None
+                exp = impl.stat(stat, r["sets"], idx, r["windows"], mode)
+                got = np.asarray(got, dtype=np.float64).reshape(exp.shape)
+                fin = np.isfinite(exp)
+                scale = max(float(np.max(np.abs(exp[fin]))), 1e-300) if np.any(fin) else 1.0
+                if np.any(fin):
+                    worst = max(worst, float(np.max(np.abs(got[fin] - exp[fin]))) / scale)
+        out["max_rel_err_vs_cpu_sampled"] = worst
+        log(f"replicates: max rel err of 8 sampled replicates vs CPU {worst:.2e}")
+    return out
+
+
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_sweep_traffic.json")
 SWEEP_KERNEL = "k_sweep<0, 3, 1>"
 
@@ -341,6 +452,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--chunks", type=int, default=0, help="override the chunk count")
     ap.add_argument("--no-serve", action="store_true", help="skip the served phase")
+    ap.add_argument("--no-replicates", action="store_true",
+                    help="skip the 256-replicate batch phase")
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the relatedness-matrix / joint AFS phase")
     ap.add_argument("--serve-requests", type=int, default=300)
@@ -447,6 +560,9 @@ def main():
     dense = None
     if not args.no_dense and not args.small:
         dense = dense_phase(om, not args.no_cpu)
+    replicates = None
+    if not args.no_replicates and not args.small:
+        replicates = replicate_phase(om, not args.no_cpu)
 
     served, serve_tss = None, None
     if not args.no_serve and not args.small:
@@ -494,6 +610,7 @@ def main():
         "max_rel_err_vs_cpu": errs,
         "served": served,
         "dense": dense,
+        "replicates": replicates,
         "setup_s": {"generate": round(t_gen, 2), "host_init": round(t_init, 2),
                     "upload": round(t_upload, 3)},
     }
