@@ -382,6 +382,27 @@ def test_relatedness_matrix(require_gpu, windows):
     assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
 
 
+@pytest.mark.parametrize("plan", ["hot", "full"])
+def test_relatedness_matrix_wide(require_gpu, plan):
+    """3000 samples: several workgroups per chunk split the sample bitsets
+    (hot plan, 2048 samples each); the hot and full genotype sweeps give the
+    same matrix, the oracle's closed form to FP64 rounding."""
+    tc = cases.generated(3000, 3e5, 23)
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    orc = Oracle(tc)
+    w = np.array([0, 1.2e5, 3e5])
+    _say("relatedness matrix wide", plan)
+    ll.set_sweep_plan(1 if plan == "full" else 0)
+    try:
+        got = ts.genetic_relatedness_matrix(windows=w)
+        assert ll.get_last_store() == (2 if plan == "hot" else 0)
+    finally:
+        ll.set_sweep_plan(0)
+    exp = orc.relatedness_matrix(w, True)
+    assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
+
+
 def test_relatedness_matrix_subset(require_gpu):
     tc = cases.generated(200, 3e5, 19)
     ts = tc.tree_sequence()

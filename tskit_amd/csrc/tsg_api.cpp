@@ -1575,16 +1575,31 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_
     int ret;
     {
         std::lock_guard<std::mutex> lock(g_mutex);
-        if ((ret = ensure_full(ts))) return ret;
+        if ((ret = device_upload(ts))) return ret;
     }
     Lease lease(ts);
     Workspace *ws = lease.ws;
     if (ws == nullptr) return TSG_ERR_DEVICE;
+    // genotype bitsets on the hot plan when a block of WB words per slot
+    // fits in LDS (WB = 64, 32 or 16 words: a workgroup's share of samples)
+    int WB = 0;
+    if (ts->hot.usable && !ws->force_full) {
+        for (int wb : {64, 32, 16}) {
+            if ((size_t) ts->hot.max_slots * wb * 4 <= (size_t) 48 * 1024) {
+                WB = wb;
+                break;
+            }
+        }
+    }
+    if (WB == 0) {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = ensure_full(ts))) return ret;
+    }
     hipStream_t s = ws->stream;
     const int slots = std::min(ts->num_chunks, 512);
-    const int64_t gstride = (int64_t) ts->max_nloc * (1 + NW);
+    const int64_t gstride = WB ? 1 : (int64_t) ts->max_nloc * (1 + NW);
     if ((ret = ensure(ws->gbits, (size_t) std::max(ts->num_sites, 1) * NW * 4)) ||
-        (ret = ensure(ws->gslab, (size_t) slots * gstride * 4)) ||
+        (ret = ensure(ws->gslab, (size_t) (WB ? 1 : slots) * gstride * 4)) ||
         (ret = upload(ws->sitecol, sitecol, s)) || (ret = upload(ws->wk0, wk0, s)) ||
         (ret = upload(ws->wk1, wk1, s)) || (ret = upload(ws->scale, scale, s)) ||
         (ret = upload(ws->sel, sel, s)) || (ret = ensure(ws->gt, (size_t) n_pad * Kp)) ||
@@ -1594,26 +1609,47 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_
         return ret;
     }
     const TsgDevice &d = ts->dev;
+    const TsgHotDevice &hd = ts->hot.dev;
     TsgGenoArgs g{};
-    g.chunk_tree0 = d.chunk_tree0;
-    g.chunk_node0 = d.chunk_node0;
-    g.loc_parent0 = d.loc_parent0;
-    g.tr_off = d.tr_off;
-    g.tr_nrem = d.tr_nrem;
-    g.ev_child = d.ev_child;
-    g.ev_parent = d.ev_parent;
     g.tree_site0 = d.tree_site0;
     g.site_mut0 = d.site_mut0;
-    g.mut_node = d.mut_node;
-    g.num_chunks = ts->num_chunks;
     g.num_samples = ns;
     g.NW = NW;
-    g.slots = slots;
-    g.g_rec = (uint32_t *) ws->gslab.p;
-    g.g_stride = gstride;
     g.gbits = (uint32_t *) ws->gbits.p;
+    if (WB) {
+        g.chunk_tree0 = hd.chunk_tree0;
+        g.chunk_node0 = hd.chunk_slot0;
+        g.tr_off = hd.tr_off;
+        g.tr_nrem = hd.tr_nrem;
+        g.ev_child = hd.ev_child;
+        g.ev_path0 = hd.ev_path0;
+        g.path = hd.path;
+        g.mut_node = hd.mut_slot;
+        g.slot_par = hd.slot_par;
+        g.chunk_lev0 = hd.chunk_lev0;
+        g.lev_slot0 = hd.lev_slot0;
+        g.sample_slot = hd.sample_slot;
+        g.num_chunks = ts->hot.num_chunks;
+        g.WB = WB;
+        ws->last_store = TSG_STORE_HOT;
+    } else {
+        g.chunk_tree0 = d.chunk_tree0;
+        g.chunk_node0 = d.chunk_node0;
+        g.loc_parent0 = d.loc_parent0;
+        g.tr_off = d.tr_off;
+        g.tr_nrem = d.tr_nrem;
+        g.ev_child = d.ev_child;
+        g.ev_parent = d.ev_parent;
+        g.mut_node = d.mut_node;
+        g.num_chunks = ts->num_chunks;
+        g.slots = slots;
+        g.g_rec = (uint32_t *) ws->gslab.p;
+        g.g_stride = gstride;
+        ws->last_store = TSG_STORE_GLOBAL;
+    }
     if (ws->profile) (void) hipEventRecord(ws->ev0, s);
-    if ((ret = hip_check(tsg_launch_geno_bits(g, s))) ||
+    if ((ret = hip_check(WB ? tsg_launch_geno_hot(g, (size_t) ts->hot.max_slots * WB * 4, s)
+                            : tsg_launch_geno_bits(g, s))) ||
         (ret = hip_check(tsg_launch_bits_to_gt((const uint32_t *) ws->gbits.p, NW,
                                                (const int32_t *) ws->sitecol.p, Kp, ns, n_pad,
                                                (int8_t *) ws->gt.p, s))) ||

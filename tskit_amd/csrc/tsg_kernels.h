@@ -158,6 +158,15 @@ struct TsgGenoArgs {
     uint32_t *g_rec;             // per slot: max_nloc x (1 + NW) words
     int64_t g_stride;
     uint32_t *gbits;             // [S x NW] derived-allele bitsets per site
+    // hot plan (k_geno_hot): slot ids in events, paths and mutations;
+    // chunk_node0 holds slot ranges; WB bitset words per workgroup
+    const int64_t *ev_path0;
+    const int32_t *path;
+    const int32_t *slot_par;
+    const int64_t *chunk_lev0;
+    const int32_t *lev_slot0;
+    const uint16_t *sample_slot;   // blocked, as TsgSweepArgs::sample_slot
+    int32_t WB;
 };
 
 struct TsgCombineArgs {

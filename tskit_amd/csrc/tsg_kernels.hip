@@ -1700,6 +1700,7 @@ hipError_t launch_sweep_store(const TsgSweepArgs &a, int store, int mpl, size_t 
 }  // namespace
 
 void tsg_set_lds_max(int bytes) { g_lds_max = std::max(bytes, 65536); }
+int tsg_lds_max() { return g_lds_max; }
 
 hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl, size_t shmem,
                             hipStream_t s) {

@@ -6,6 +6,7 @@
 
 // the device's dynamic LDS limit per workgroup (set once, before launches)
 void tsg_set_lds_max(int bytes);
+int tsg_lds_max();
 // store: TSG_STORE_GLOBAL / TSG_STORE_LDS / TSG_STORE_HOT
 hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl, size_t shmem,
                             hipStream_t s);
@@ -38,6 +39,8 @@ hipError_t tsg_launch_afs_branch_final(double *h, const double *win, int W, int6
                                        uint32_t flags, hipStream_t s);
 size_t tsg_final_struct_size();
 hipError_t tsg_launch_geno_bits(const TsgGenoArgs &a, hipStream_t s);
+// genotype bitsets on the hot plan (shmem = max slots x WB x 4 bytes)
+hipError_t tsg_launch_geno_hot(const TsgGenoArgs &a, size_t shmem, hipStream_t s);
 hipError_t tsg_launch_bits_to_gt(const uint32_t *gbits, int NW, const int32_t *site_of_col,
                                  int64_t Kp, int n, int n_pad, int8_t *gt, hipStream_t s);
 hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int64_t *win_k0,

@@ -23,6 +23,7 @@
 
 #include "tsg_common.h"
 #include "tsg_kernels.h"
+#include "tsg_launch.h"
 
 #define WAVE 64
 
@@ -91,6 +92,80 @@ __device__ void geno_bits_chunk(const TsgGenoArgs &a, int c, uint32_t *rec) {
                 const int node = a.mut_node[m0];
                 for (int w = lane; w < NW; w += WAVE) row[w] = word(node, w);
             }
+        }
+    }
+}
+
+// --- genotype bitsets on the hot plan ---------------------------------------
+// One workgroup per (chunk, block of WB bitset words = 32*WB samples): the
+// chunk's hot slots hold bitsets of the samples below them in LDS.  Seeding:
+// each sample of the block sets its bit in its nearest hot ancestor-or-self,
+// then the hot forest's levels OR upward, deepest first.  Events: lane w owns
+// word w of every slot, so an edge insertion ORs the child's word into each
+// node of its root path and a removal clears it (the child's samples lie
+// below every path node and nowhere else among them) -- no lane reads
+// another lane's words, so the event loop needs no barrier.  Each site's row
+// is its mutation node's bitset (at most one mutation per site on this
+// route; none gives an empty row).  Grid: 8 * ceil(C / 8) x blocks.
+__global__ void __launch_bounds__(WAVE) k_geno_hot(const TsgGenoArgs a) {
+    extern __shared__ uint32_t bits[];
+    const int C = a.num_chunks;
+    const int per = (C + 7) / 8;
+    const int c = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD-contiguous chunks
+    if (c >= C) return;
+    const int lane = threadIdx.x;
+    const int WB = a.WB;
+    const int w0 = blockIdx.y * WB;                // first word of the block
+    const int nwb = min(WB, a.NW - w0);            // words in this block
+    const int64_t s0 = a.chunk_node0[c];
+    const int S = (int) (a.chunk_node0[c + 1] - s0);
+    for (int i = lane; i < S * WB; i += WAVE) bits[i] = 0u;
+    __syncthreads();
+    const uint16_t *ss = a.sample_slot + (int64_t) (c / TSG_SS_CB) * a.num_samples * TSG_SS_CB +
+                         c % TSG_SS_CB;
+    const int j0 = w0 * 32, j1 = min(a.num_samples, (w0 + nwb) * 32);
+    for (int j = j0 + lane; j < j1; j += WAVE) {
+        const unsigned h = ss[(int64_t) j * TSG_SS_CB];
+        if (h != TSG_NO_SLOT) atomicOr(&bits[h * WB + ((j >> 5) - w0)], 1u << (j & 31));
+    }
+    __syncthreads();
+    const int32_t *lev = a.lev_slot0 + a.chunk_lev0[c];
+    const int nlev = (int) (a.chunk_lev0[c + 1] - a.chunk_lev0[c]) - 1;
+    for (int l = 0; l + 1 < nlev; l++) {
+        const int h0 = lev[l], h1 = lev[l + 1];
+        for (int idx = lane; idx < (h1 - h0) * nwb; idx += WAVE) {
+            const int h = h0 + idx / nwb, w = idx % nwb;
+            const int p = a.slot_par[s0 + h];
+            const uint32_t v = bits[h * WB + w];
+            if (p >= 0 && v != 0u) atomicOr(&bits[p * WB + w], v);
+        }
+        __syncthreads();
+    }
+    const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
+    for (int t = t0; t < t1; t++) {
+        if (t > t0) {
+            const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
+            const int64_t nrem = a.tr_nrem[t];
+            for (int64_t e = e0; e < e1; e++) {
+                const bool ins = (e - e0) >= nrem;
+                const int u = a.ev_child[e];
+                const int64_t p0 = a.ev_path0[e], p1 = a.ev_path0[e + 1];
+                if (lane < nwb) {
+                    const uint32_t xc = bits[u * WB + lane];
+                    if (xc != 0u) {
+                        for (int64_t q = p0; q < p1; q++) {
+                            uint32_t &b = bits[a.path[q] * WB + lane];
+                            b = ins ? (b | xc) : (b & ~xc);
+                        }
+                    }
+                }
+            }
+        }
+        const int sa = a.tree_site0[t], sb = a.tree_site0[t + 1];
+        for (int s = sa; s < sb; s++) {
+            uint32_t *row = a.gbits + (int64_t) s * a.NW + w0;
+            const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
+            if (lane < nwb) row[lane] = (m1 == m0) ? 0u : bits[a.mut_node[m0] * WB + lane];
         }
     }
 }
@@ -256,6 +331,20 @@ __global__ void k_total(int64_t *rowsum, int n, int W) {
 // --- launchers ------------------------------------------------------------------
 hipError_t tsg_launch_geno_bits(const TsgGenoArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_geno_bits, dim3(a.slots), dim3(WAVE), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_geno_hot(const TsgGenoArgs &a, size_t shmem, hipStream_t s) {
+    if (shmem > 65536) {
+        if ((int) shmem > tsg_lds_max()) return hipErrorInvalidValue;
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_geno_hot),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           tsg_lds_max());
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid(8u * (unsigned) ((a.num_chunks + 7) / 8),
+                    (unsigned) ((a.NW + a.WB - 1) / a.WB));
+    hipLaunchKernelGGL(k_geno_hot, grid, dim3(WAVE), shmem, s, a);
     return hipGetLastError();
 }
 
