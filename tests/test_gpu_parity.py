@@ -216,6 +216,43 @@ def test_node_stores(require_gpu, plan):
                orc.stat("f4", sets[:4], [(0, 1, 2, 3)], w, mode)[:, 0])
 
 
+@pytest.mark.parametrize("mode", ["branch", "site"])
+def test_gapped_stats(require_gpu, mode):
+    """Statistics on trees with several roots and samples missing over gaps
+    (nodes that lose their parent edge and regain it later)."""
+    tc = cases.gapped(200, 2e5, 3)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    sets = cases.sample_sets(200, 4, 40, 3)
+    w = np.linspace(0, tc.sequence_length, 9)
+    _say("gapped stats", mode)
+    _check("diversity", ts.diversity(sets, windows=w, mode=mode),
+           orc.stat("diversity", sets, None, w, mode))
+    idx = [(0, 1), (2, 3), (1, 1)]
+    _check("divergence", ts.divergence(sets, indexes=idx, windows=w, mode=mode),
+           orc.stat("divergence", sets, idx, w, mode))
+    _check("f4", ts.f4(sets, indexes=[(0, 1, 2, 3)], windows=w, mode=mode),
+           orc.stat("f4", sets, [(0, 1, 2, 3)], w, mode))
+
+
+def test_scale_workload(require_gpu):
+    """The full-size workload (1e5 samples, L=1e8, seed 2; K=8 sets of 1000,
+    1000 windows): diversity and all-pairs divergence in both modes against
+    the oracle, elementwise <= 1e-6 relative."""
+    tc = cases.generated(100000, 1e8, 2)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    sets = cases.sample_sets(100000, 8, 1000, 2)
+    w = np.linspace(0, 1e8, 1001)
+    pairs = [(i, j) for i in range(8) for j in range(i + 1, 8)]
+    for mode in ["branch", "site"]:
+        _say("scale workload", mode)
+        _check("diversity", ts.diversity(sets, windows=w, mode=mode),
+               orc.stat("diversity", sets, None, w, mode))
+        _check("divergence", ts.divergence(sets, indexes=pairs, windows=w, mode=mode),
+               orc.stat("divergence", sets, pairs, w, mode))
+
+
 def test_unused_sets_dropped(small):
     """Sets that no index tuple names do not become state columns; the
     remaining columns are renumbered (results as the oracle's)."""
