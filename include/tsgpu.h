@@ -89,6 +89,13 @@ int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t
     int64_t *tr_off, int32_t *tr_nrem, int32_t *ev_child, int32_t *ev_parent, double *ev_bl,
     int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc, int32_t *mut_allele,
     int32_t *mut_pallele, int32_t *site_nalleles);
+/* Genotypes of every sample at every site, out = [num_sites x num_samples]
+ * int32 allele indexes (0 = ancestral, then derived states in order of first
+ * appearance; -1 = missing: an isolated sample without a mutation on it,
+ * unless options has TSG_ISOLATED_NOT_MISSING = 1 << 1).  Replaces the
+ * per-site tsk_variant_decode loop behind TreeSequence.genotype_matrix
+ * (c/tskit/genotypes.c:473-595, python/tskit/trees.py:5563-5640). */
+int tsg_treeseq_genotype_matrix(tsg_treeseq_t *ts, uint32_t options, int32_t *out);
 /* host copy of each event's root path (local ids, the edge's parent first);
  * ev_path0 has (events + 1) entries */
 uint64_t tsg_treeseq_get_path_size(const tsg_treeseq_t *ts);

@@ -173,6 +173,16 @@ class Oracle(_Base):
             raise OracleError(err)
         return G
 
+    def genotype_matrix(self, isolated_as_missing=True):
+        """Allele index of every sample at every site, [S, n] int32 (-1:
+        missing)."""
+        G = np.zeros((len(self.tc.site_position), int(np.sum(self.tc.node_flags & 1))),
+                     dtype=np.int32)
+        err = self._lib.oracle_genotype_matrix(self.h, int(bool(isolated_as_missing)), _p(G))
+        if err != 0:
+            raise OracleError(err)
+        return G
+
     def relatedness_matrix(self, windows=None, span_normalise=True):
         """Site-mode relatedness of every sample, J (G^T G) J per window (exact
         integer cross product, FP64 centring)."""
@@ -226,6 +236,7 @@ class Reference(_Base):
         lib.ref_afs.argtypes = (
             [_vp, ctypes.c_uint64, _vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _vp,
              ctypes.c_uint32, _vp])
+        lib.ref_genotype_matrix.argtypes = [_vp, ctypes.c_int, _vp]
         lib.ref_divergence_matrix.argtypes = (
             [_vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _vp, ctypes.c_uint32, _vp])
         super().__init__(tc)
@@ -255,6 +266,14 @@ class Reference(_Base):
         return self._lib.ref_afs(
             self.h, len(sizes), _p(sizes), _p(flat), len(windows) - 1, _p(windows),
             len(time_windows) - 1, _p(time_windows), opts, _p(out))
+
+    def genotype_matrix(self, isolated_as_missing=True):
+        G = np.zeros((len(self.tc.site_position), int(np.sum(self.tc.node_flags & 1))),
+                     dtype=np.int32)
+        err = self._lib.ref_genotype_matrix(self.h, int(bool(isolated_as_missing)), _p(G))
+        if err != 0:
+            raise OracleError(err)
+        return G
 
     def divergence_matrix(self, samples, windows=None, mode="site", span_normalise=True):
         samples = np.ascontiguousarray(samples, np.int32)

@@ -128,6 +128,28 @@ int ref_afs(const ref_ts_t *r, uint64_t K, const uint64_t *sizes, const int32_t 
 
 /* Site-mode divergence matrix (trees.c:8753-8826); used to cross-check the
  * relatedness-matrix closed form on sub-blocks. */
+/* Genotypes of every sample at every site through the reference's variant
+ * decoder (tsk_variant_init / tsk_variant_decode, genotypes.h:120,157). */
+int ref_genotype_matrix(const ref_ts_t *r, int isolated_as_missing, int32_t *out) {
+    tsk_variant_t var;
+    const tsk_size_t n = tsk_treeseq_get_num_samples(&r->ts);
+    const tsk_size_t S = tsk_treeseq_get_num_sites(&r->ts);
+    tsk_size_t s;
+    int ret = tsk_variant_init(&var, &r->ts, NULL, 0, NULL,
+        isolated_as_missing ? 0 : TSK_ISOLATED_NOT_MISSING);
+    if (ret != 0) {
+        tsk_variant_free(&var);
+        return ret;
+    }
+    for (s = 0; s < S; s++) {
+        ret = tsk_variant_decode(&var, (tsk_id_t) s, 0);
+        if (ret != 0) break;
+        memcpy(out + s * n, var.genotypes, n * sizeof(int32_t));
+    }
+    tsk_variant_free(&var);
+    return ret;
+}
+
 int ref_divergence_matrix(const ref_ts_t *r, uint64_t num_samples, const int32_t *samples,
     uint64_t nw, const double *windows, uint32_t options, double *result) {
     return tsk_treeseq_divergence_matrix(&r->ts, num_samples, NULL, samples, nw, windows,

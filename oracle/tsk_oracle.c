@@ -679,6 +679,58 @@ int oracle_genotypes(const ots_t *ts, int8_t *G) {
     return 0;
 }
 
+/* ---- genotype decode (tsk_variant_decode, genotypes.c:473-595; missing
+ * data genotypes.c:413-436): per site, every sample starts at allele 0; with
+ * isolated_as_missing a sample that is a root without children in the tree
+ * covering the site gets -1; then each mutation in order sets the samples
+ * below its node to its allele index (first-seen order, as mut_allele).
+ * G is [S x n] int32. ------------------------------------------------------ */
+int oracle_genotype_matrix(const ots_t *ts, int isolated_as_missing, int32_t *G) {
+    const int32_t N = ts->N, E = ts->E;
+    int32_t *par = malloc((size_t) (N + 1) * sizeof(int32_t));
+    int32_t *nch = calloc((size_t) N + 1, sizeof(int32_t));
+    int32_t j, h, tj = 0, tk = 0, t = 0, s, u, m;
+    double tl = 0, tr;
+    for (j = 0; j < N; j++) par[j] = NUL;
+    while (tj < E || tl < ts->L) {
+        while (tk < E && ts->right[ts->O[tk]] == tl) {
+            h = ts->O[tk++];
+            par[ts->child[h]] = NUL;
+            nch[ts->parent[h]]--;
+        }
+        while (tj < E && ts->left[ts->I[tj]] == tl) {
+            h = ts->I[tj++];
+            par[ts->child[h]] = ts->parent[h];
+            nch[ts->parent[h]]++;
+        }
+        tr = ts->L;
+        if (tj < E && ts->left[ts->I[tj]] < tr) tr = ts->left[ts->I[tj]];
+        if (tk < E && ts->right[ts->O[tk]] < tr) tr = ts->right[ts->O[tk]];
+        for (s = ts->tree_site0[t]; s < ts->tree_site0[t + 1]; s++) {
+            int32_t *row = G + (size_t) s * ts->n;
+            for (j = 0; j < ts->n; j++) {
+                u = ts->samples[j];
+                row[j] = (isolated_as_missing && par[u] == NUL && nch[u] == 0) ? -1 : 0;
+            }
+            for (m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) {
+                for (j = 0; j < ts->n; j++) {
+                    for (u = ts->samples[j]; u != NUL; u = par[u]) {
+                        if (u == ts->mut_node[m]) {
+                            row[j] = ts->mut_allele[m];
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        t++;
+        tl = tr;
+    }
+    free(par);
+    free(nch);
+    return 0;
+}
+
 /* ---- AFS: trees.c:3469-3928 --------------------------------------------- */
 static void fold(int64_t *coord, const int64_t *dims, int32_t nd) { /* trees.c:3469-3496 */
     int32_t k;

@@ -440,6 +440,32 @@ def test_relatedness_matrix_wide(require_gpu, plan):
     assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
 
 
+@pytest.mark.parametrize("case", ["paper_ex", "single_tree_ex", "nonbinary_ex", "generated",
+                                  "gapped", "wide"])
+@pytest.mark.parametrize("missing", [True, False])
+def test_genotype_matrix(require_gpu, case, missing):
+    """Device genotype decode equals the oracle's (pinned to the reference's
+    tsk_variant_decode): multi-allelic sites with back and recurrent
+    mutations (single_tree_ex), isolated samples as missing or ancestral
+    (gapped), several 2048-sample blocks per chunk (wide).  Bit-exact."""
+    if case == "generated":
+        tc = cases.generated(150, 2e5, 29)
+    elif case == "gapped":
+        tc = cases.gapped(200, 2e5, 3)
+    elif case == "wide":
+        tc = cases.generated(5000, 1e5, 33)
+    else:
+        tc = getattr(cases, case)()
+    ts = tc.tree_sequence()
+    _say("genotype matrix", case, missing)
+    got = ts.genotype_matrix(isolated_as_missing=missing)
+    exp = Oracle(tc).genotype_matrix(missing)
+    assert got.dtype == np.int32 and np.array_equal(got, exp)
+    if case == "generated":
+        sel = ts.samples()[[5, 1, 77]]
+        assert np.array_equal(ts.genotype_matrix(samples=sel), exp[:, [5, 1, 77]])
+
+
 def test_relatedness_matrix_subset(require_gpu):
     tc = cases.generated(200, 3e5, 19)
     ts = tc.tree_sequence()

@@ -290,6 +290,30 @@ def test_brute_afs_site(tiny):
 def test_oracle_genotypes_brute_force(tiny):
     tc, orc = tiny
     assert np.array_equal(orc.genotypes(), _genotypes(tc))
+    # the decoder agrees on these single-mutation sites
+    assert np.array_equal(orc.genotype_matrix(), _genotypes(tc).astype(np.int32))
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["paper_ex", "single_tree_ex", "nonbinary_ex", "generated",
+                                  "gapped"])
+@pytest.mark.parametrize("missing", [True, False])
+def test_oracle_genotype_matrix_vs_reference(case, missing):
+    """The oracle's decoder equals the reference's tsk_variant_decode, with
+    back and recurrent mutations (single_tree_ex) and isolated samples
+    (gapped: samples without edges over stretches) as missing or not."""
+    if case == "generated":
+        tc = cases.generated(70, 1e5, 31)
+    elif case == "gapped":
+        tc = cases.gapped(120, 1e5, 11)
+    else:
+        tc = getattr(cases, case)()
+    orc, ref = Oracle(tc), om.Reference(tc)
+    got = orc.genotype_matrix(missing)
+    exp = ref.genotype_matrix(missing)
+    assert np.array_equal(got, exp)
+    if case == "gapped" and missing:
+        assert np.any(exp == -1)  # the case exercises missing data
 
 
 def _normalise_divergence(D):

@@ -49,7 +49,7 @@ struct Workspace {
     // first tree: counts and per-block running-sum partials
     DevBuf x0, r0part, bsum;
     // relatedness-matrix route
-    DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab;
+    DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab, geno;
     int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
     bool force_full = false;        // run sweeps on the full plan (testing hook)
     bool force_c32 = false;         // hot plan with 32-bit counts (testing hook)
@@ -906,7 +906,7 @@ void free_one_workspace(Workspace *ws) {
                       &ws->Rstart, &ws->raw, &ws->rawbase, &ws->lanebase, &ws->out, &ws->fin, &ws->ipar,
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
                       &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
-                      &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->x0,
+                      &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->geno, &ws->x0,
                       &ws->r0part, &ws->bsum};
     for (DevBuf *b : bufs) {
         if (b->p != nullptr) (void) hipFree(b->p);
@@ -1673,6 +1673,127 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_
         float ms = 0;
         (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
         ws->sweep_ms += ms;
+        ws->sweep_launches += 1;
+    }
+    return 0;
+}
+
+// ---- genotype decode ------------------------------------------------------
+// (site, sample index) pairs where a sample is isolated -- no edge names it as
+// parent or child over the site's position -- and no mutation of the site
+// sits on it: the reference marks such samples missing before applying the
+// site's mutations, and only a mutation on the node itself reaches an
+// isolated node (genotypes.c:413-436, 473-595).
+static void isolated_pairs(const tsg_treeseq *ts, std::vector<int32_t> &ms,
+                           std::vector<int32_t> &mj) {
+    const int32_t n = ts->num_samples;
+    std::vector<std::vector<std::pair<double, double>>> cover(n);
+    for (int32_t e = 0; e < ts->num_edges; e++) {
+        for (int32_t u : {ts->parent[e], ts->child[e]}) {
+            const int32_t j = ts->sample_index_map[u];
+            if (j != TSG_NULL) cover[j].emplace_back(ts->left[e], ts->right[e]);
+        }
+    }
+    auto sites_in = [&](double a, double b, int32_t j) {
+        const auto &pos = ts->site_pos;
+        size_t s = std::lower_bound(pos.begin(), pos.end(), a) - pos.begin();
+        const size_t s1 = std::lower_bound(pos.begin(), pos.end(), b) - pos.begin();
+        for (; s < s1; s++) {
+            bool mut_on = false;
+            for (int32_t m = ts->site_mut0[s]; m < ts->site_mut0[s + 1]; m++) {
+                mut_on |= ts->mut_node[m] == ts->samples[j];
+            }
+            if (!mut_on) {
+                ms.push_back((int32_t) s);
+                mj.push_back(j);
+            }
+        }
+    };
+    for (int32_t j = 0; j < n; j++) {
+        auto &iv = cover[j];
+        std::sort(iv.begin(), iv.end());
+        double x = 0.0;  // covered up to x so far
+        for (const auto &[l, r] : iv) {
+            if (l > x) sites_in(x, l, j);
+            x = std::max(x, r);
+        }
+        if (x < ts->L) sites_in(x, ts->L, j);
+    }
+}
+
+// Genotypes of every sample at every site, [S x n] int32, as
+// TreeSequence.genotype_matrix (python/tskit/trees.py:5563-5640) builds them
+// from tsk_variant_decode (genotypes.c:473-595): allele indexes in order of
+// first appearance (0 = ancestral), -1 for isolated samples unless options
+// has TSG_ISOLATED_NOT_MISSING.  Runs on the hot plan.
+int tsg_treeseq_genotype_matrix(tsg_treeseq *ts, uint32_t options, int32_t *out) {
+    if (ts->num_sites == 0) return 0;
+    int ret;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = device_upload(ts))) return ret;
+    }
+    if (!ts->hot.usable) return TSG_ERR_UNSUPPORTED_INPUT;
+    int WB = 0;
+    for (int wb : {64, 32, 16}) {
+        if ((size_t) ts->hot.max_slots * wb * 4 <= (size_t) 48 * 1024) {
+            WB = wb;
+            break;
+        }
+    }
+    if (WB == 0) return TSG_ERR_UNSUPPORTED_INPUT;
+    std::vector<int32_t> ms, mj;
+    if (!(options & TSG_ISOLATED_NOT_MISSING)) isolated_pairs(ts, ms, mj);
+    Lease lease(ts);
+    Workspace *ws = lease.ws;
+    if (ws == nullptr) return TSG_ERR_DEVICE;
+    hipStream_t s = ws->stream;
+    const int ns = ts->num_samples;
+    const size_t cells = (size_t) ts->num_sites * ns;
+    if ((ret = ensure(ws->geno, cells * 4)) || (ret = upload(ws->wk0, ms, s)) ||
+        (ret = upload(ws->wk1, mj, s))) {
+        return ret;
+    }
+    const TsgDevice &d = ts->dev;
+    const TsgHotDevice &hd = ts->hot.dev;
+    TsgGenoArgs g{};
+    g.tree_site0 = d.tree_site0;
+    g.site_mut0 = d.site_mut0;
+    g.num_samples = ns;
+    g.NW = (ns + 31) / 32;
+    g.chunk_tree0 = hd.chunk_tree0;
+    g.chunk_node0 = hd.chunk_slot0;
+    g.tr_off = hd.tr_off;
+    g.tr_nrem = hd.tr_nrem;
+    g.ev_child = hd.ev_child;
+    g.ev_path0 = hd.ev_path0;
+    g.path = hd.path;
+    g.mut_node = hd.mut_slot;
+    g.slot_par = hd.slot_par;
+    g.chunk_lev0 = hd.chunk_lev0;
+    g.lev_slot0 = hd.lev_slot0;
+    g.sample_slot = hd.sample_slot;
+    g.num_chunks = ts->hot.num_chunks;
+    g.WB = WB;
+    g.mut_allele = d.mut_allele;
+    g.geno = (int32_t *) ws->geno.p;
+    ws->last_store = TSG_STORE_HOT;
+    if (ws->profile) (void) hipEventRecord(ws->ev0, s);
+    if ((ret = hip_check(tsg_launch_genotypes(g, (size_t) ts->hot.max_slots * WB * 4,
+                                              (const int32_t *) ws->wk0.p,
+                                              (const int32_t *) ws->wk1.p, (int64_t) ms.size(),
+                                              s)))) {
+        return ret;
+    }
+    if (ws->profile) (void) hipEventRecord(ws->ev1, s);
+    if ((ret = hip_check(hipMemcpyAsync(out, ws->geno.p, cells * 4, hipMemcpyDeviceToHost, s))) ||
+        (ret = hip_check(hipStreamSynchronize(s)))) {
+        return ret;
+    }
+    if (ws->profile) {
+        float ms_ = 0;
+        (void) hipEventElapsedTime(&ms_, ws->ev0, ws->ev1);
+        ws->sweep_ms += ms_;
         ws->sweep_launches += 1;
     }
     return 0;

@@ -26,6 +26,9 @@
 // reference, trees.c:537-543); branch statistics then need
 // TSG_STAT_ALLOW_TIME_UNCALIBRATED.
 #define TSG_TS_TIME_UNCALIBRATED (1u << 0)
+// genotype decode: isolated samples take the ancestral allele, not -1
+// (TSK_ISOLATED_NOT_MISSING, genotypes.h:35)
+#define TSG_ISOLATED_NOT_MISSING (1u << 1)
 
 // --- error numbers (core.h) -------------------------------------------------
 #define TSG_ERR_GENERIC (-1)

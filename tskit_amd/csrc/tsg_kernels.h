@@ -167,6 +167,9 @@ struct TsgGenoArgs {
     const int32_t *lev_slot0;
     const uint16_t *sample_slot;   // blocked, as TsgSweepArgs::sample_slot
     int32_t WB;
+    // genotype decode: allele index per mutation, [S x num_samples] output
+    const int32_t *mut_allele;
+    int32_t *geno;
 };
 
 struct TsgCombineArgs {

@@ -41,6 +41,10 @@ size_t tsg_final_struct_size();
 hipError_t tsg_launch_geno_bits(const TsgGenoArgs &a, hipStream_t s);
 // genotype bitsets on the hot plan (shmem = max slots x WB x 4 bytes)
 hipError_t tsg_launch_geno_hot(const TsgGenoArgs &a, size_t shmem, hipStream_t s);
+// genotype decode on the hot plan into a.geno [S x n] int32, then -1 at the
+// listed (site, sample) pairs
+hipError_t tsg_launch_genotypes(const TsgGenoArgs &a, size_t shmem, const int32_t *miss_site,
+                                const int32_t *miss_sample, int64_t nmiss, hipStream_t s);
 hipError_t tsg_launch_bits_to_gt(const uint32_t *gbits, int NW, const int32_t *site_of_col,
                                  int64_t Kp, int n, int n_pad, int8_t *gt, hipStream_t s);
 hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int64_t *win_k0,

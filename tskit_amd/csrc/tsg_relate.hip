@@ -107,6 +107,13 @@ __device__ void geno_bits_chunk(const TsgGenoArgs &a, int c, uint32_t *rec) {
 // another lane's words, so the event loop needs no barrier.  Each site's row
 // is its mutation node's bitset (at most one mutation per site on this
 // route; none gives an empty row).  Grid: 8 * ceil(C / 8) x blocks.
+//
+// GENO: the genotype decode instead (tsk_variant_decode, genotypes.c:473-595):
+// every sample starts at allele 0 and each mutation of the site, in order,
+// sets the samples below its node to its allele index; lane w writes the 32
+// int32 genotypes of its word.  Missing data is marked afterwards from the
+// host's list of isolated samples (k_mark_missing).
+template <bool GENO>
 __global__ void __launch_bounds__(WAVE) k_geno_hot(const TsgGenoArgs a) {
     extern __shared__ uint32_t bits[];
     const int C = a.num_chunks;
@@ -163,9 +170,32 @@ __global__ void __launch_bounds__(WAVE) k_geno_hot(const TsgGenoArgs a) {
         }
         const int sa = a.tree_site0[t], sb = a.tree_site0[t + 1];
         for (int s = sa; s < sb; s++) {
-            uint32_t *row = a.gbits + (int64_t) s * a.NW + w0;
             const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
-            if (lane < nwb) row[lane] = (m1 == m0) ? 0u : bits[a.mut_node[m0] * WB + lane];
+            if constexpr (GENO) {
+                if (lane < nwb) {
+                    int32_t g[32];
+#pragma unroll
+                    for (int b = 0; b < 32; b++) g[b] = 0;
+                    for (int m = m0; m < m1; m++) {
+                        const uint32_t mask = bits[a.mut_node[m] * WB + lane];
+                        const int32_t al = a.mut_allele[m];
+#pragma unroll
+                        for (int b = 0; b < 32; b++) {
+                            if ((mask >> b) & 1u) g[b] = al;
+                        }
+                    }
+                    const int j0w = (w0 + lane) * 32;
+                    int32_t *row = a.geno + (int64_t) s * a.num_samples + j0w;
+                    const int nb = min(32, a.num_samples - j0w);
+#pragma unroll
+                    for (int b = 0; b < 32; b++) {
+                        if (b < nb) row[b] = g[b];
+                    }
+                }
+            } else {
+                uint32_t *row = a.gbits + (int64_t) s * a.NW + w0;
+                if (lane < nwb) row[lane] = (m1 == m0) ? 0u : bits[a.mut_node[m0] * WB + lane];
+            }
         }
     }
 }
@@ -334,17 +364,38 @@ hipError_t tsg_launch_geno_bits(const TsgGenoArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t tsg_launch_geno_hot(const TsgGenoArgs &a, size_t shmem, hipStream_t s) {
+template <bool GENO>
+hipError_t launch_geno_hot(const TsgGenoArgs &a, size_t shmem, hipStream_t s) {
     if (shmem > 65536) {
         if ((int) shmem > tsg_lds_max()) return hipErrorInvalidValue;
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_geno_hot),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_geno_hot<GENO>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            tsg_lds_max());
         if (e != hipSuccess) return e;
     }
     const dim3 grid(8u * (unsigned) ((a.num_chunks + 7) / 8),
                     (unsigned) ((a.NW + a.WB - 1) / a.WB));
-    hipLaunchKernelGGL(k_geno_hot, grid, dim3(WAVE), shmem, s, a);
+    hipLaunchKernelGGL((k_geno_hot<GENO>), grid, dim3(WAVE), shmem, s, a);
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_geno_hot(const TsgGenoArgs &a, size_t shmem, hipStream_t s) {
+    return launch_geno_hot<false>(a, shmem, s);
+}
+
+// Missing data: geno[site * n + sample] = -1 for each listed pair.
+__global__ void k_mark_missing(int32_t *geno, int n, const int32_t *site, const int32_t *sample,
+                               int64_t npairs) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npairs) geno[(int64_t) site[i] * n + sample[i]] = -1;
+}
+
+hipError_t tsg_launch_genotypes(const TsgGenoArgs &a, size_t shmem, const int32_t *miss_site,
+                                const int32_t *miss_sample, int64_t nmiss, hipStream_t s) {
+    hipError_t e = launch_geno_hot<true>(a, shmem, s);
+    if (e != hipSuccess || nmiss == 0) return e;
+    hipLaunchKernelGGL(k_mark_missing, dim3((unsigned) ((nmiss + 255) / 256)), dim3(256), 0, s,
+                       a.geno, a.num_samples, miss_site, miss_sample, nmiss);
     return hipGetLastError();
 }
 

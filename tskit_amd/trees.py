@@ -393,6 +393,17 @@ class LowLevelTreeSequence:
         return self._k_way(self._lib.tsg_treeseq_Fst, 2, sample_set_sizes, sample_sets, indexes,
                            windows, mode, span_normalise, polarised, centre)
 
+    def genotype_matrix(self, isolated_as_missing=True):
+        """Allele index of every sample at every site, [S, n] int32 (-1 for
+        missing data), decoded on the device (genotypes.c:473-595)."""
+        S, n = self.get_num_sites(), self.get_num_samples()
+        out = np.zeros((S, n), dtype=np.int32)
+        if S == 0:
+            return out
+        opts = 0 if isolated_as_missing else 2  # TSK_ISOLATED_NOT_MISSING
+        check(self._lib.tsg_treeseq_genotype_matrix(self._h, opts, ptr(out, ctypes.c_int32)))
+        return out
+
     def genetic_relatedness_matrix(self, sample_sets, windows, mode=None, span_normalise=1):
         """Relatedness of singleton sample sets on the int8 matrix-core route.
         ``sample_sets`` is None (every sample) or a list of node ids, each its
@@ -787,6 +798,28 @@ class TreeSequence:
         return self._k_way_sample_set_stat(
             self._ll_tree_sequence.Fst, 2, sample_sets, indexes=indexes, windows=windows,
             mode=mode, span_normalise=span_normalise)
+
+    def genotype_matrix(self, *, samples=None, isolated_as_missing=None, alleles=None,
+                        impute_missing_data=None):
+        """The genotypes of trees.py:5563-5640: an [S, n] int32 matrix of allele
+        indexes (0 = ancestral, then derived states in order of first
+        appearance at the site), -1 for an isolated sample without a mutation
+        on it when ``isolated_as_missing`` (the default).  ``samples`` selects
+        columns (sample nodes only); ``alleles`` must be None."""
+        if alleles is not None:
+            raise ValueError("alleles: only the default encoding is supported on this path")
+        if isolated_as_missing is None:
+            isolated_as_missing = not impute_missing_data
+        G = self._ll_tree_sequence.genotype_matrix(bool(isolated_as_missing))
+        if samples is None:
+            return G
+        index = {int(u): j for j, u in enumerate(self.samples())}
+        cols = []
+        for u in np.atleast_1d(np.asarray(samples)):
+            if int(u) not in index:
+                raise ValueError("samples: only sample nodes are supported on this path")
+            cols.append(index[int(u)])
+        return G[:, cols]
 
     def genetic_relatedness_matrix(self, sample_sets=None, windows=None, mode="site",
                                    span_normalise=True):
