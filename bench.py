@@ -246,6 +246,9 @@ def dense_phase(om, with_cpu):
     out["relatedness_matrix_ms"] = t_rel * 1e3
     out["relatedness_int_tops_end_to_end"] = ops / t_rel / 1e12
     out["relatedness_frac_of_int8_peak_end_to_end"] = ops / t_rel / 1e12 / INT8_PEAK_TOPS
+    t_geno, geno = timed(lambda: ts.genotype_matrix(), reps=2)
+    out["genotype_matrix_ms"] = t_geno * 1e3
+    out["genotype_matrix_bytes"] = int(geno.nbytes)
     perm = np.random.default_rng(d["seed"]).permutation(d["n"])
     a, b = d["afs_sets"]
     sets = [perm[:a], perm[a:a + b]]
@@ -280,6 +283,17 @@ def dense_phase(om, with_cpu):
         out["relatedness_block_max_rel_err"] = float(
             np.max(np.abs(rel[np.ix_(idx, idx)] - blk)) / np.max(np.abs(blk)))
         del G
+        if om.reference_available():
+            ref = om.Reference(tc)
+            t = time.perf_counter()
+            gexp = ref.genotype_matrix(True)
+            out["genotype_matrix_cpu_reference_s"] = time.perf_counter() - t
+            out["genotype_matrix_bit_exact"] = bool(np.array_equal(geno, gexp))
+            del gexp
+            log(f"dense: genotype matrix {out['genotype_matrix_ms']:.0f} ms (reference decode "
+                f"{out['genotype_matrix_cpu_reference_s']:.1f} s, 1 core), bit-exact "
+                f"{out['genotype_matrix_bit_exact']}")
+        del geno
         log(f"dense: AFS site bit-exact {out['afs_site_bit_exact']}, branch max rel err "
             f"{out['afs_branch_max_rel_err']:.2e}, relatedness 64x64 block max rel err "
             f"{out['relatedness_block_max_rel_err']:.2e}")

@@ -218,9 +218,11 @@ def reference_available():
 
 
 class Reference(_Base):
-    """The reference C library compiled from source (oracle/_ref)."""
+    """The reference C library compiled from source (oracle/_ref).  With
+    ``path`` the tree sequence comes from the reference's own ``.trees``
+    loader (``tc`` then only supplies shapes: site positions, sample count)."""
 
-    def __init__(self, tc):
+    def __init__(self, tc, path=None):
         lib = ctypes.CDLL(REF_LIB)
         self._lib = lib
         self._init = lib.ref_ts_init
@@ -239,7 +241,16 @@ class Reference(_Base):
         lib.ref_genotype_matrix.argtypes = [_vp, ctypes.c_int, _vp]
         lib.ref_divergence_matrix.argtypes = (
             [_vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _vp, ctypes.c_uint32, _vp])
-        super().__init__(tc)
+        lib.ref_ts_load.argtypes = [ctypes.POINTER(_vp), ctypes.c_char_p]
+        if path is None:
+            super().__init__(tc)
+            return
+        self.tc = tc
+        h = _vp()
+        err = lib.ref_ts_load(ctypes.byref(h), os.fsencode(path))
+        if err != 0:
+            raise OracleError(err)
+        self.h = h
 
     @property
     def num_trees(self):

@@ -128,6 +128,26 @@ int ref_afs(const ref_ts_t *r, uint64_t K, const uint64_t *sizes, const int32_t 
 
 /* Site-mode divergence matrix (trees.c:8753-8826); used to cross-check the
  * relatedness-matrix closed form on sub-blocks. */
+/* A tree sequence from a .trees file through the reference's own loader
+ * (tsk_table_collection_load, tables.c:11800ff), indexed as for ref_ts_init. */
+int ref_ts_load(ref_ts_t **out, const char *path) {
+    ref_ts_t *r = calloc(1, sizeof(ref_ts_t));
+    int ret;
+    *out = NULL;
+    ret = tsk_table_collection_load(&r->tables, path, 0);
+    if (ret != 0) goto fail;
+    ret = tsk_table_collection_build_index(&r->tables, 0);
+    if (ret != 0) goto fail;
+    ret = tsk_treeseq_init(&r->ts, &r->tables, 0);
+    if (ret != 0) goto fail;
+    *out = r;
+    return 0;
+fail:
+    tsk_table_collection_free(&r->tables);
+    free(r);
+    return ret;
+}
+
 /* Genotypes of every sample at every site through the reference's variant
  * decoder (tsk_variant_init / tsk_variant_decode, genotypes.h:120,157). */
 int ref_genotype_matrix(const ref_ts_t *r, int isolated_as_missing, int32_t *out) {

@@ -275,6 +275,37 @@ def test_unused_sets_dropped(small):
         _check("diversity", p2.result(), orc.stat("diversity", sets[2:4], None, w, mode))
 
 
+@pytest.mark.parametrize("name", ["basic_tree_seq", "minimal-example", "single-locus-example",
+                                  "tskit-0.3.3"])
+def test_trees_files(require_gpu, name):
+    """The reference's own .trees data files (tests/golden/trees: a docs
+    example, two SLiM outputs, a tskit 0.3.3 file), read by the kastore
+    reader: GPU statistics and genotypes against the oracle."""
+    import os
+
+    import tskit_amd as ta
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trees",
+                        name + ".trees")
+    tc = ta.load_tables(path)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    smp = ts.samples()
+    half = [smp[: len(smp) // 2], smp[len(smp) // 2:]]
+    w = np.linspace(0, tc.sequence_length, 3)
+    for mode in (["branch", "site"] if tc.num_sites else ["branch"]):
+        _say("trees file", name, mode)
+        got = ts.diversity(half, windows=w, mode=mode)
+        exp = orc.stat("diversity", half, None, w, mode)
+        assert np.allclose(got, exp, rtol=1e-9, atol=1e-12 * max(1.0, np.max(np.abs(exp))))
+        got = ts.divergence(half, indexes=[(0, 1)], windows=w, mode=mode)
+        exp = orc.stat("divergence", half, [(0, 1)], w, mode)
+        assert np.allclose(np.asarray(got).reshape(exp.shape), exp, rtol=1e-9,
+                           atol=1e-12 * max(1.0, np.max(np.abs(exp))))
+    if tc.num_sites:
+        assert np.array_equal(ts.genotype_matrix(), orc.genotype_matrix())
+
+
 @pytest.mark.parametrize("fixture", ["paper_ex", "single_tree_ex", "nonbinary_ex"])
 def test_reference_fixtures(require_gpu, fixture):
     tc = getattr(cases, fixture)()

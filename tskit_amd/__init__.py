@@ -6,11 +6,19 @@ genetic relatedness, ...), computed by hand-written HIP kernels behind a C ABI
 (``libtsgpu.so``, declared in ``include/tsgpu.h``).
 """
 from .exceptions import LibraryError, TskitException
+from .kastore import load as load_tables
 from .sim import random_sample_sets, sim_ancestry, simulate_tables
 from .tables import NODE_IS_SAMPLE, TableCollection, parse_text
 from .trees import TreeSequence
 
+def load(path):
+    """A TreeSequence from a ``.trees`` file (as tskit.load)."""
+    return load_tables(path).tree_sequence()
+
+
 __all__ = [
+    "load",
+    "load_tables",
     "LibraryError",
     "TskitException",
     "TableCollection",

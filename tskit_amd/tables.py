@@ -100,6 +100,12 @@ class TableCollection:
 
         return TreeSequence.load_tables(self)
 
+    def dump(self, path):
+        """Writes the tables as a ``.trees`` file (kastore; see kastore.py)."""
+        from . import kastore
+
+        kastore.dump(self, path)
+
     def column_digest(self):
         """SHA-256 over the columns, to check two hosts generated the same input."""
         import hashlib

@@ -561,6 +561,10 @@ class TreeSequence:
     def tables(self):
         return self._tables
 
+    def dump(self, path):
+        """Writes the tree sequence as a ``.trees`` file."""
+        self._tables.dump(path)
+
     @property
     def sequence_length(self):
         return self._tables.sequence_length
