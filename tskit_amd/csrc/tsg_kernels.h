@@ -38,16 +38,17 @@ struct TsgQuery {
 
 // 32-bit words of one TsgOut.
 #define TSG_OUT_INTS ((int) (sizeof(TsgOut) / 4))
-// Trees, events and path entries the sweep kernel stages in LDS per batch.
-#define TSG_TB 32
-#define TSG_EB 64
-#define TSG_PB 512
+// Trees, events and path entries the sweep kernel stages in LDS per batch
+// (small: the sweep's occupancy is set by its LDS per workgroup).
+#define TSG_TB 16
+#define TSG_EB 32
+#define TSG_PB 192
 // 32-bit words of the sweep kernel's LDS staging block for K state columns
 // and `span` outputs in the launch (layout in k_sweep); node records follow.
 #define TSG_STAGE_INTS(K, span)                                                               \
-    ((int64_t) (span) * (TSG_OUT_INTS + 1) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + \
-     (TSG_TB + 1) + 3 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) +   \
-     2 * TSG_PSEG + 8 * (int64_t) (K) + 2 * 64 + 2)
+    ((int64_t) (span) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + (TSG_TB + 1) +         \
+     3 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) + 2 * TSG_PSEG +  \
+     8 * (int64_t) (K) + 2 * 64 + 2)
 
 // Where a sweep keeps its node records (see k_sweep).
 #define TSG_STORE_GLOBAL 0

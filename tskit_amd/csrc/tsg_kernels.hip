@@ -284,6 +284,9 @@ struct LaneOut {
     double c0, c3;
     double l1[4];
     double q[10];
+    // TSG_KIND_SEG: column, set size, polarised, 1 / denominator
+    int si, sn, spol;
+    double sinv;
 };
 
 #define TSG_KIND_HOP 0
@@ -482,7 +485,7 @@ __device__ __forceinline__ double poly_walk(const LaneOut &L, const double *d, d
 }
 
 template <int MPL>
-__device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, int lane,
+__device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *gout, int c, int lane,
                               LaneOut (&lo)[MPL]) {
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
@@ -496,7 +499,7 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, 
         L.kind = TSG_KIND_HOP;
         if (m < a.m_end) {
             L.lm = m - a.m_begin;
-            const TsgOut &o = sout[L.lm];
+            const TsgOut o = gout[L.lm];
             const TsgQuery &q = a.queries[o.q];
             const int64_t cq = (int64_t) c * a.Q + o.q;
             L.win = a.windows + q.win_off;
@@ -506,6 +509,10 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *sout, int c, 
             L.slot0 = q.pbase + a.cq_poff[cq] * (int64_t) q.M + (m - q.mbase);
             L.mask = summary_columns(o, a.K);
             L.kind = moment_form(o, a.K, L);
+            L.si = o.i;
+            L.sn = o.ni;
+            L.spol = o.flags & 1;
+            L.sinv = o.inv;
         }
     }
 }
@@ -558,7 +565,7 @@ __device__ __forceinline__ double branch_summary(const TsgOut &o, const int32_t 
 //     written to LDS, and each output's owner lane adds them in hop order.
 // Running sums and window sums are FP64, as in the reference.
 template <int MODE, int STORE, int MPL>
-__global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
+__global__ void __launch_bounds__(WAVE, MPL == 1 ? 3 : 1) k_sweep(const TsgSweepArgs a) {
     extern __shared__ int32_t smem[];
     const int c = xcd_chunk(blockIdx.x, a.num_chunks);
     if (c >= a.num_chunks) return;
@@ -568,9 +575,10 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     const int span = a.m_end - a.m_begin;
     const int64_t n0 = a.chunk_node0[c];
     const int nloc = (int) (a.chunk_node0[c + 1] - n0);
-    // 8-byte items first
-    TsgOut *sout = reinterpret_cast<TsgOut *>(smem);                    // [span]
-    double *tbp = reinterpret_cast<double *>(sout + span);              // [TB+1] breakpoints
+    // 8-byte items first; each lane keeps its outputs' specs in registers
+    // (ow), the launch's specs stay in HBM (gout)
+    const TsgOut *gout = a.outs + a.m_begin;
+    double *tbp = reinterpret_cast<double *>(smem);                     // [TB+1] breakpoints
     int64_t *toff = reinterpret_cast<int64_t *>(tbp + TSG_TB + 1);       // [TB+1] tr_off
     int64_t *evp = toff + TSG_TB + 1;                                    // [EB+1] ev_path0
     double *s1v = reinterpret_cast<double *>(evp + TSG_EB + 1);          // [K] path sums
@@ -603,7 +611,6 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     const bool any_unpolarised = (a.flags & TSG_F_ANY_UNPOLARISED) != 0;
     const uint64_t below = (1ull << lane) - 1;
 
-    for (int m = lane; m < span; m += WAVE) sout[m] = a.outs[a.m_begin + m];
     for (int k = lane; k < K; k += WAVE) {
         tot[k] = a.col_total[k];
         zero[k] = 0;
@@ -615,13 +622,13 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
     }
 
     LaneOut lo[MPL];
-    setup_outputs<MPL>(a, sout, c, lane, lo);
+    setup_outputs<MPL>(a, gout, c, lane, lo);
     int wfirst[MPL];
     bool any_poly = false, any_c3 = false, any_seg = false;
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
         wfirst[j] = lo[j].wi;
-        if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(sout[lo[j].lm], zero, tot);
+        if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(gout[lo[j].lm], zero, tot);
         any_poly |= lo[j].kind == TSG_KIND_POLY;
         any_c3 |= lo[j].kind == TSG_KIND_POLY && lo[j].c3 != 0;
         any_seg |= lo[j].kind == TSG_KIND_SEG;
@@ -730,11 +737,10 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                         if (L.kind == TSG_KIND_POLY) {
                             f = poly_value(L, xc);
                         } else if (L.kind == TSG_KIND_SEG) {
-                            const TsgOut &o = sout[L.lm];
-                            const double xa = (double) xc[o.i];
-                            f = o.inv * ((o.flags & 1) ? seg_pol(xa, o.ni) : seg_unpol(xa, o.ni));
+                            const double xa = (double) xc[L.si];
+                            f = L.sinv * (L.spol ? seg_pol(xa, L.sn) : seg_unpol(xa, L.sn));
                         } else {
-                            f = branch_summary(sout[L.lm], xc, tot);
+                            f = branch_summary(gout[L.lm], xc, tot);
                         }
                     }
                     lo[j].R += b * f;
@@ -805,7 +811,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                         const int idx = r0 + lane;
                         if (idx < items) {
                             const int ai = idx / h, i = idx - ai * h;
-                            const TsgOut &o = sout[act[ai]];
+                            const TsgOut o = gout[act[ai]];
                             const double fo = branch_summary(o, xo + i * K, tot);
                             const double fn = branch_summary(o, xn + i * K, tot);
                             td[lane] = (double) sbl[i] * (fn - fo);
@@ -837,8 +843,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
                     LaneOut &L = lo[j];
                     if (L.kind == TSG_KIND_HOP || !(L.mask & cm)) continue;
                     if (L.kind == TSG_KIND_SEG) {
-                        const TsgOut &o = sout[L.lm];
-                        L.R += o.inv * ((o.flags & 1) ? gpv[o.i] : guv[o.i]);
+                        L.R += L.sinv * (L.spol ? gpv[L.si] : guv[L.si]);
                         continue;
                     }
                     double d[4];
@@ -905,7 +910,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(const TsgSweepArgs a) {
 #pragma unroll
                     for (int j = 0; j < MPL; j++) {
                         if (lo[j].lm < 0) continue;
-                        const TsgOut &o = sout[lo[j].lm];
+                        const TsgOut &o = gout[lo[j].lm];
                         if (al > 0 || !(o.flags & 1)) val[j] += eval_summary(o, xo, tot, false);
                     }
                     __syncthreads();
