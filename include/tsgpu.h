@@ -89,6 +89,15 @@ int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t
     int64_t *tr_off, int32_t *tr_nrem, int32_t *ev_child, int32_t *ev_parent, double *ev_bl,
     int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc, int32_t *mut_allele,
     int32_t *mut_pallele, int32_t *site_nalleles);
+/* tsk_treeseq_divergence_matrix (c/tskit/trees.h; trees.c:8902-9001) in
+ * site mode for sample sets of one sample each: result = [W x n x n], the
+ * number of sites in the window where two samples carry different alleles
+ * (divided by the window span with TSG_STAT_SPAN_NORMALISE); sites with at
+ * most one mutation; from the same int8 cross products as the relatedness
+ * matrix. */
+int tsg_treeseq_divergence_matrix(tsg_treeseq_t *ts, uint64_t num_sample_sets,
+    const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
 /* Genotypes of every sample at every site, out = [num_sites x num_samples]
  * int32 allele indexes (0 = ancestral, then derived states in order of first
  * appearance; -1 = missing: an isolated sample without a mutation on it,

@@ -497,6 +497,42 @@ def test_genotype_matrix(require_gpu, case, missing):
         assert np.array_equal(ts.genotype_matrix(samples=sel), exp[:, [5, 1, 77]])
 
 
+@pytest.mark.parametrize("windows", [None, "three"])
+def test_divergence_matrix(require_gpu, windows):
+    """Site divergence matrix of singleton sets: C_ii + C_jj - 2 C_ij from the
+    int8 cross products, against the reference's own tsk_treeseq_divergence_
+    matrix when oracle/_ref is present, else the oracle's genotypes."""
+    from oracle import oracle as om
+
+    tc = cases.generated(300, 5e5, 17)
+    ts = tc.tree_sequence()
+    w = None if windows is None else np.array([0, 1e5, 3.5e5, 5e5])
+    _say("divergence matrix", windows)
+    got = ts.divergence_matrix(windows=w)
+    wn = np.array([0, 5e5]) if w is None else w
+    if om.reference_available():
+        exp = om.Reference(tc).divergence_matrix(np.arange(300, dtype=np.int32), wn, "site")
+    else:
+        G = Oracle(tc).genotypes().astype(np.int64)
+        pos = tc.site_position
+        exp = []
+        for j in range(len(wn) - 1):
+            g = G[(pos >= wn[j]) & (pos < wn[j + 1])]
+            C = g.T @ g
+            a = np.diag(C)
+            exp.append((a[:, None] + a[None, :] - 2 * C) / (wn[j + 1] - wn[j]))
+        exp = np.array(exp, dtype=np.float64)
+    if w is None:
+        exp = exp[0]
+    assert got.shape == exp.shape
+    assert np.allclose(got, exp, rtol=1e-12, atol=1e-15)
+    sel = [[7], [3], [150]]
+    sub = ts.divergence_matrix(sample_sets=sel, windows=w)
+    full = got if w is None else got
+    idx = [7, 3, 150]
+    assert np.allclose(sub, full[..., idx, :][..., :, idx], rtol=1e-12, atol=1e-15)
+
+
 def test_relatedness_matrix_subset(require_gpu):
     tc = cases.generated(200, 3e5, 19)
     ts = tc.tree_sequence()

@@ -341,6 +341,25 @@ def test_relatedness_matrix_closed_form_vs_reference():
     assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
 
 
+@needs_ref
+def test_divergence_matrix_closed_form_vs_reference():
+    """The site divergence matrix of singleton sets is C_ii + C_jj - 2 C_ij
+    (C = G^T G of the 0/1 genotypes) per window span: the form the GPU route
+    computes, against the reference's tsk_treeseq_divergence_matrix."""
+    tc = cases.generated(60, 2e5, 43)
+    orc, ref = Oracle(tc), om.Reference(tc)
+    w = np.array([0, 7e4, 2e5])
+    exp = ref.divergence_matrix(np.arange(60, dtype=np.int32), w, "site", True)
+    G = orc.genotypes().astype(np.int64)
+    pos = tc.site_position
+    for j in range(2):
+        g = G[(pos >= w[j]) & (pos < w[j + 1])]
+        C = g.T @ g
+        a = np.diag(C)
+        got = (a[:, None] + a[None, :] - 2 * C) / (w[j + 1] - w[j])
+        assert np.allclose(got, exp[j], rtol=1e-13, atol=0)
+
+
 def test_identity_invariant():
     """test_stats.c:1061-1124: branch diversity-style identity -- the branch
     segregating-sites of all samples with span normalisation off equals the

@@ -87,6 +87,11 @@ SIGNATURES = {
         ],
     ),
     "tsg_treeseq_genotype_matrix": (ctypes.c_int, [c_void, ctypes.c_uint32, c_i32p]),
+    "tsg_treeseq_divergence_matrix": (
+        ctypes.c_int,
+        [c_void, ctypes.c_uint64, c_u64p, c_i32p, ctypes.c_uint64, c_f64p, ctypes.c_uint32,
+         c_f64p],
+    ),
     "tsg_treeseq_genetic_relatedness_matrix": (
         ctypes.c_int,
         [c_void, ctypes.c_uint64, c_u64p, c_i32p, ctypes.c_uint64, c_f64p, ctypes.c_uint32,

@@ -1500,11 +1500,13 @@ int tsg_treeseq_genetic_relatedness_proportion(tsg_treeseq *ts, uint64_t K, cons
 // for singleton sample sets and sites with at most one mutation: K = J C J,
 // C = G^T G on int8 MFMA.  sample_sets may be NULL (all samples); result is
 // W x n x n doubles.
-int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
-                                           const uint64_t *sample_set_sizes,
-                                           const int32_t *sample_sets, uint64_t nw,
-                                           const double *windows, uint32_t options,
-                                           double *result) {
+// Site-mode matrices of singleton sample sets from C = G^T G (int8 matrix
+// cores): the centred relatedness K = J C J / span, or with `divergence` the
+// pairwise count of differing sites C_ii + C_jj - 2 C_ij / span.
+static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
+                            const uint64_t *sample_set_sizes, const int32_t *sample_sets,
+                            uint64_t nw, const double *windows, uint32_t options, bool divergence,
+                            double *result) {
     bool site = options & TSG_STAT_SITE, branch = options & TSG_STAT_BRANCH;
     if (options & TSG_STAT_NODE) return TSG_ERR_UNSUPPORTED_STAT_MODE;
     if (!(site || branch)) site = true;
@@ -1657,10 +1659,16 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_
                                             (const int64_t *) ws->wk0.p,
                                             (const int64_t *) ws->wk1.p, W,
                                             (int32_t *) ws->cmat.p, s))) ||
-        (ret = hip_check(tsg_launch_centre((const int32_t *) ws->cmat.p,
-                                           (const int32_t *) ws->sel.p, (int64_t *) ws->rowsum.p,
-                                           n, n_pad, W, (const double *) ws->scale.p,
-                                           (double *) ws->kout.p, s)))) {
+        (ret = hip_check(divergence
+                             ? tsg_launch_divergence((const int32_t *) ws->cmat.p,
+                                                     (const int32_t *) ws->sel.p, n, n_pad, W,
+                                                     (const double *) ws->scale.p,
+                                                     (double *) ws->kout.p, s)
+                             : tsg_launch_centre((const int32_t *) ws->cmat.p,
+                                                 (const int32_t *) ws->sel.p,
+                                                 (int64_t *) ws->rowsum.p, n, n_pad, W,
+                                                 (const double *) ws->scale.p,
+                                                 (double *) ws->kout.p, s)))) {
         return ret;
     }
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);
@@ -1676,6 +1684,26 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_
         ws->sweep_launches += 1;
     }
     return 0;
+}
+
+int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
+                                           const uint64_t *sample_set_sizes,
+                                           const int32_t *sample_sets, uint64_t nw,
+                                           const double *windows, uint32_t options,
+                                           double *result) {
+    return site_gram_matrix(ts, num_sample_sets, sample_set_sizes, sample_sets, nw, windows,
+                            options, false, result);
+}
+
+// tsk_treeseq_divergence_matrix (trees.c:8902-9001), site mode, sample sets
+// of one sample each (the normalisation of fill_lower_triangle_count_normalise
+// then divides off-diagonal entries by 1 and leaves the zero diagonal).
+int tsg_treeseq_divergence_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
+                                  const uint64_t *sample_set_sizes, const int32_t *sample_sets,
+                                  uint64_t nw, const double *windows, uint32_t options,
+                                  double *result) {
+    return site_gram_matrix(ts, num_sample_sets, sample_set_sizes, sample_sets, nw, windows,
+                            options, true, result);
 }
 
 // ---- genotype decode ------------------------------------------------------

@@ -49,6 +49,9 @@ hipError_t tsg_launch_bits_to_gt(const uint32_t *gbits, int NW, const int32_t *s
                                  int64_t Kp, int n, int n_pad, int8_t *gt, hipStream_t s);
 hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int64_t *win_k0,
                               const int64_t *win_k1, int W, int32_t *C, hipStream_t s);
+// D = (C_ii + C_jj - 2 C_ij) * scale[w] over the selected samples
+hipError_t tsg_launch_divergence(const int32_t *C, const int32_t *sel, int n, int n_pad, int W,
+                                 const double *scale, double *D, hipStream_t s);
 hipError_t tsg_launch_centre(const int32_t *C, const int32_t *sel, int64_t *rowsum, int n,
                              int n_pad, int W, const double *scale, double *K, hipStream_t s);
 

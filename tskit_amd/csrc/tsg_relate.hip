@@ -349,6 +349,24 @@ __global__ void __launch_bounds__(256) k_centre(const int32_t *C, const int32_t 
     K[(int64_t) w * n * n + gid] = v * scale[w];
 }
 
+// Site divergence of singleton sets from the same cross products: samples i
+// and j differ at a 0/1 site iff g_i + g_j - 2 g_i g_j = 1, so the count of
+// differing sites is C_ii + C_jj - 2 C_ij (0 on the diagonal), times the
+// window's span scale (trees.c:8753-8900 with sets of one sample).
+__global__ void __launch_bounds__(256) k_divergence(const int32_t *C, const int32_t *sel, int n,
+                                                    int n_pad, const double *scale, double *D) {
+    const int64_t gid = (int64_t) blockIdx.x * 256 + threadIdx.x;
+    const int w = blockIdx.y;
+    if (gid >= (int64_t) n * n) return;
+    const int i = (int) (gid / n), j = (int) (gid % n);
+    const int32_t *Cw = C + (int64_t) w * n_pad * n_pad;
+    const int64_t si = sel[i], sj = sel[j];
+    const double v = (i == j) ? 0.0
+                              : (double) Cw[si * n_pad + si] + (double) Cw[sj * n_pad + sj] -
+                                    2.0 * (double) Cw[si * n_pad + sj];
+    D[(int64_t) w * n * n + gid] = v * scale[w];
+}
+
 // grand totals: one thread per window (n row sums each)
 __global__ void k_total(int64_t *rowsum, int n, int W) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -411,6 +429,14 @@ hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int
     const int tpd = n_pad / BT;
     dim3 grid((unsigned) (tpd * (tpd + 1) / 2), (unsigned) W);
     hipLaunchKernelGGL(k_syrk_i8, grid, dim3(256), 0, s, gt, Kp, n_pad, win_k0, win_k1, tpd, C);
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_divergence(const int32_t *C, const int32_t *sel, int n, int n_pad, int W,
+                                 const double *scale, double *D, hipStream_t s) {
+    const int64_t nn = (int64_t) n * n;
+    hipLaunchKernelGGL(k_divergence, dim3((unsigned) ((nn + 255) / 256), (unsigned) W), dim3(256),
+                       0, s, C, sel, n, n_pad, scale, D);
     return hipGetLastError();
 }
 

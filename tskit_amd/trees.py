@@ -404,6 +404,32 @@ class LowLevelTreeSequence:
         check(self._lib.tsg_treeseq_genotype_matrix(self._h, opts, ptr(out, ctypes.c_int32)))
         return out
 
+    def divergence_matrix(self, windows, sample_sets=None, sample_set_sizes=None, mode=None,
+                          span_normalise=1):
+        """_tskitmodule.c divergence_matrix: [W, n, n] over sample sets of one
+        sample each (site mode, int8 matrix cores)."""
+        opts = self._options(mode, span_normalise, False)
+        w = _parse_windows(windows)
+        if sample_sets is None:
+            n = self.get_num_samples()
+            sets_p = None
+            sizes_p = None
+        else:
+            sets = as_c(np.asarray(sample_sets), np.int32)
+            n = len(sets)
+            sets_p = ptr(sets, ctypes.c_int32)
+            sizes_p = None
+            if sample_set_sizes is not None:
+                sizes = as_c(np.asarray(sample_set_sizes), np.uint64)
+                n = len(sizes)
+                sizes_p = ptr(sizes, ctypes.c_uint64)
+        out = np.zeros((len(w) - 1, n, n), dtype=np.float64)
+        err = self._lib.tsg_treeseq_divergence_matrix(
+            self._h, n, sizes_p, sets_p, len(w) - 1, ptr(w, ctypes.c_double), opts,
+            ptr(out, ctypes.c_double))
+        check(err)
+        return out
+
     def genetic_relatedness_matrix(self, sample_sets, windows, mode=None, span_normalise=1):
         """Relatedness of singleton sample sets on the int8 matrix-core route.
         ``sample_sets`` is None (every sample) or a list of node ids, each its
@@ -824,6 +850,27 @@ class TreeSequence:
                 raise ValueError("samples: only sample nodes are supported on this path")
             cols.append(index[int(u)])
         return G[:, cols]
+
+    def divergence_matrix(self, sample_sets=None, *, windows=None, num_threads=0, mode=None,
+                          span_normalise=True):
+        """trees.py:8739-8828: mean pairwise divergence between sample sets,
+        [W, K, K] (or [K, K] without windows).  On this path: site mode and
+        sample sets of one sample each, from G^T G on the int8 matrix cores
+        (num_threads is accepted and ignored)."""
+        drop = windows is None
+        w = self.parse_windows(windows)
+        if sample_sets is None:
+            flat, sizes = None, None
+        else:
+            flat, sizes = [], []
+            for s in sample_sets:
+                s = np.atleast_1d(np.asarray(s))
+                flat.extend(int(x) for x in s)
+                sizes.append(len(s))
+        D = self._ll_tree_sequence.divergence_matrix(
+            w, sample_sets=flat, sample_set_sizes=sizes, mode=mode or "site",
+            span_normalise=span_normalise)
+        return D[0] if drop else D
 
     def genetic_relatedness_matrix(self, sample_sets=None, windows=None, mode="site",
                                    span_normalise=True):
