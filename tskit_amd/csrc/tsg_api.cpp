@@ -335,16 +335,31 @@ double summary_inv(int stat, const int32_t *n, const int32_t *c, bool same01) {
 int device_upload(tsg_treeseq *ts);
 int ensure_full(tsg_treeseq *ts);
 
-// Runs a batch of queries (same tree sequence, same mode) in one sweep.
-int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
+// A batch between its phases: prepared (inputs uploaded, sweep arguments
+// set), swept, finished (combine, finals, results copied out, not yet
+// synchronised).  A workspace and a stream carry it.
+struct BatchRun {
+    tsg_treeseq *ts = nullptr;
+    std::vector<Query *> *batch = nullptr;
+    Workspace *ws = nullptr;
+    hipStream_t s = nullptr;
+    uint32_t mode = 0;
+    int mode_id = 0, store = 0, M = 0, Q = 0, C = 0, slice = 256;
+    size_t shmem = 0;
+    TsgSweepArgs a{};
+    const int32_t *t0_parent = nullptr;
+    const double *t0_bl = nullptr;
+    int nloc0 = 0, nb0 = 0;
+    int64_t lanecount = 0, outcount = 0;
+    TsgCombineArgs ca{};
+    std::vector<TsgFinalHost> fin;
+};
+
+// Host work of a batch and its uploads on stream s: columns, outputs,
+// per-(chunk, query) window ranges, node store, sweep arguments, finals.
+int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, hipStream_t s,
+                  BatchRun &r) {
     int ret;
-    {
-        std::lock_guard<std::mutex> lock(g_mutex);
-        if ((ret = device_upload(ts))) return ret;
-    }
-    Lease lease(ts);
-    Workspace *ws = lease.ws;
-    if (ws == nullptr) return TSG_ERR_DEVICE;
     const int Q = (int) batch.size();
     const uint32_t mode = batch[0]->mode;
     // fused columns / outputs
@@ -548,8 +563,8 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         lanebase[qi] = lanecount;
         lanecount += (items * g + 63) / 64 * 64;
     }
-    hipStream_t s = ws->stream;
-    TsgSweepArgs a{};
+    TsgSweepArgs &a = r.a;
+    a = TsgSweepArgs{};
     if ((ret = upload_seeds(ws, pair_s, pair_c, ts->num_samples, s, a)) ||
         (ret = upload(ws->tot, col_total, s)) || (ret = upload(ws->outs, outs, s)) ||
         (ret = upload(ws->queries, qs, s)) || (ret = upload(ws->windows, wins, s)) ||
@@ -649,23 +664,25 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     a.flags = (noskip ? TSG_F_NOSKIP : 0u) | (any_unpolarised ? TSG_F_ANY_UNPOLARISED : 0u);
     a.g_rec = (int32_t *) ws->slab.p;
     a.g_stride = (int64_t) ts->max_nloc * R;
-    const int mode_id = (mode == TSG_STAT_BRANCH) ? 0 : 1;
-    if (ws->profile) (void) hipEventRecord(ws->ev0, s);
-    for (int m0 = 0; m0 < M; m0 += slice) {
-        a.m_begin = m0;
-        a.m_end = std::min(M, m0 + slice);
-        const int span = a.m_end - a.m_begin;
-        const int mpl = span <= 64 ? 1 : span <= 128 ? 2 : 4;
-        if ((ret = hip_check(tsg_launch_sweep(a, mode_id, store, mpl, shmem, s)))) return ret;
-    }
-    if (ws->profile) (void) hipEventRecord(ws->ev1, s);
-    if (mode == TSG_STAT_BRANCH) {
-        if ((ret = hip_check(tsg_launch_tree0(a, t0_parent, t0_bl, (int32_t *) ws->x0.p, nloc0,
-                                              (double *) ws->r0part.p, s)))) {
-            return ret;
-        }
-    }
-    TsgCombineArgs ca{};
+    r.ts = ts;
+    r.batch = &batch;
+    r.ws = ws;
+    r.s = s;
+    r.mode = mode;
+    r.mode_id = (mode == TSG_STAT_BRANCH) ? 0 : 1;
+    r.store = store;
+    r.shmem = shmem;
+    r.M = M;
+    r.Q = Q;
+    r.C = C;
+    r.slice = slice;
+    r.t0_parent = t0_parent;
+    r.t0_bl = t0_bl;
+    r.nloc0 = nloc0;
+    r.nb0 = nb0;
+    r.lanecount = lanecount;
+    TsgCombineArgs &ca = r.ca;
+    ca = TsgCombineArgs{};
     ca.bp = d.breakpoints;
     ca.chunk_tree0 = a.chunk_tree0;
     ca.num_chunks = C;
@@ -682,13 +699,10 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     ca.rawbase = (const int64_t *) ws->rawbase.p;
     ca.lanebase = (const int64_t *) ws->lanebase.p;
     ca.flags = 0;
-    if ((ret = hip_check(tsg_launch_combine(ca, (mode == TSG_STAT_BRANCH) ? a.dR : nullptr,
-                                            (double *) ws->Rstart.p, (double *) ws->bsum.p,
-                                            (const double *) ws->r0part.p, nb0, lanecount, s)))) {
-        return ret;
-    }
-    // finals
-    std::vector<TsgFinalHost> fin(Q);
+    // finals (FP64 composites), set up now: they read only the combined raw
+    // sums
+    std::vector<TsgFinalHost> &fin = r.fin;
+    fin.assign(Q, TsgFinalHost{});
     std::vector<int32_t> ipar;
     std::vector<double> dpar;
     int64_t outcount = 0;
@@ -711,31 +725,90 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
         }
         outcount += (int64_t) f.W * f.P;
     }
+    r.outcount = outcount;
     if ((ret = upload(ws->fin, fin, s)) || (ret = upload(ws->ipar, ipar, s)) ||
         (ret = upload(ws->dpar, dpar, s)) || (ret = ensure(ws->out, (size_t) outcount * 8))) {
         return ret;
     }
-    if ((ret = hip_check(tsg_launch_finals(ws->fin.p, Q, (const double *) ws->raw.p,
-                                           (const int32_t *) ws->ipar.p,
-                                           (const double *) ws->dpar.p, (double *) ws->out.p,
-                                           outcount, s)))) {
+    return 0;
+}
+
+// The sweep launches of a prepared batch: slices of at most r.slice outputs.
+int batch_sweep(BatchRun &r) {
+    TsgSweepArgs a = r.a;
+    for (int m0 = 0; m0 < r.M; m0 += r.slice) {
+        a.m_begin = m0;
+        a.m_end = std::min(r.M, m0 + r.slice);
+        const int span = a.m_end - a.m_begin;
+        const int mpl = span <= 64 ? 1 : span <= 128 ? 2 : 4;
+        int ret = hip_check(tsg_launch_sweep(a, r.mode_id, r.store, mpl, r.shmem, r.s));
+        if (ret != 0) return ret;
+    }
+    r.a.m_begin = 0;
+    r.a.m_end = r.M;
+    return 0;
+}
+
+// After the sweep: the first tree's running sum (branch), the chunk scan and
+// window combine, the finals and the copies of the results to the host (the
+// caller synchronises r.s).
+int batch_finish(BatchRun &r) {
+    int ret;
+    Workspace *ws = r.ws;
+    hipStream_t s = r.s;
+    const TsgSweepArgs &a = r.a;
+    if (r.mode == TSG_STAT_BRANCH) {
+        if ((ret = hip_check(tsg_launch_tree0(a, r.t0_parent, r.t0_bl, (int32_t *) ws->x0.p,
+                                              r.nloc0, (double *) ws->r0part.p, s)))) {
+            return ret;
+        }
+    }
+    if ((ret = hip_check(tsg_launch_combine(r.ca, (r.mode == TSG_STAT_BRANCH) ? a.dR : nullptr,
+                                            (double *) ws->Rstart.p, (double *) ws->bsum.p,
+                                            (const double *) ws->r0part.p, r.nb0, r.lanecount,
+                                            s)))) {
         return ret;
     }
-    for (int qi = 0; qi < Q; qi++) {
-        const TsgFinalHost &f = fin[qi];
-        if ((ret = hip_check(hipMemcpyAsync(batch[qi]->result, (double *) ws->out.p + f.outbase,
+    if ((ret = hip_check(tsg_launch_finals(ws->fin.p, r.Q, (const double *) ws->raw.p,
+                                           (const int32_t *) ws->ipar.p,
+                                           (const double *) ws->dpar.p, (double *) ws->out.p,
+                                           r.outcount, s)))) {
+        return ret;
+    }
+    for (int qi = 0; qi < r.Q; qi++) {
+        const TsgFinalHost &f = r.fin[qi];
+        if ((ret = hip_check(hipMemcpyAsync((*r.batch)[qi]->result,
+                                            (double *) ws->out.p + f.outbase,
                                             (size_t) f.W * f.P * 8, hipMemcpyDeviceToHost, s)))) {
             return ret;
         }
     }
-    if ((ret = hip_check(hipStreamSynchronize(s)))) return ret;
+    return 0;
+}
+
+// Runs a batch of queries (same tree sequence, same mode) in one sweep.
+int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
+    int ret;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = device_upload(ts))) return ret;
+    }
+    Lease lease(ts);
+    Workspace *ws = lease.ws;
+    if (ws == nullptr) return TSG_ERR_DEVICE;
+    BatchRun r;
+    if ((ret = batch_prepare(ts, batch, ws, ws->stream, r))) return ret;
+    if (ws->profile) (void) hipEventRecord(ws->ev0, r.s);
+    if ((ret = batch_sweep(r))) return ret;
+    if (ws->profile) (void) hipEventRecord(ws->ev1, r.s);
+    if ((ret = batch_finish(r)) || (ret = hip_check(hipStreamSynchronize(r.s)))) return ret;
     if (ws->profile) {
         float ms = 0;
         (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
         ws->sweep_ms += ms;
         ws->sweep_launches += 1;
-        ws->mode_ms[mode_id] += ms;
-        ws->mode_launches[mode_id] += 1;
+        ws->mode_ms[r.mode_id] += ms;
+        ws->mode_launches[r.mode_id] += 1;
     }
     return 0;
 }
