@@ -334,6 +334,8 @@ def replicate_phase(om, with_cpu, inflight=4):
 
     import torch
 
+    import tskit_amd as ta
+
     t0 = time.perf_counter()
     reps = replicate_workload()
     for r in reps:
@@ -382,15 +384,44 @@ def replicate_phase(om, with_cpu, inflight=4):
             results[i].update(f.result())
         torch.cuda.synchronize()
         t_batched = time.perf_counter() - t
+
+    def multi_batches():
+        bs, pend = [], []
+        for r in reps:
+            b = r["ts"].batch()
+            pm = {}
+            for mode in ("branch", "site"):
+                pm[("diversity", mode)] = b.diversity(r["sets"], windows=r["windows"], mode=mode)
+                if r["pairs"]:
+                    pm[("divergence", mode)] = b.divergence(r["sets"], indexes=r["pairs"],
+                                                            windows=r["windows"], mode=mode)
+            bs.append(b)
+            pend.append(pm)
+        return bs, pend
+
+    bs, _ = multi_batches()
+    ta.run_batches(bs[:8])
+    bs, pend = multi_batches()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ta.run_batches(bs)
+    torch.cuda.synchronize()
+    t_multi = time.perf_counter() - t
+    multi_ok = all(np.array_equal(np.asarray(pend[i][k].result()), np.asarray(results[i][k]))
+                   for i in range(len(reps)) for k in pend[i])
     out = {"replicates": len(reps), "edges": edges,
            "samples_min_max": [int(min(len(r["ts"].samples()) for r in reps)),
                                int(max(len(r["ts"].samples()) for r in reps))],
            "setup_s": t_setup, "plain_s": t_plain, "batched_s": t_batched,
            "speedup": t_plain / t_batched, "updates_per_s_batched": updates / t_batched,
            "queries_per_s_batched": sum(2 * (2 if r["pairs"] else 1) for r in reps) / t_batched,
-           "form": f"one fused launch per replicate and mode, {inflight} in flight"}
+           "form": f"one fused launch per replicate and mode, {inflight} in flight",
+           "multi_s": t_multi, "multi_speedup": t_plain / t_multi,
+           "multi_form": "one sweep launch per mode over all replicates",
+           "multi_equals_batched": bool(multi_ok)}
     log(f"replicates: {len(reps)} tree sequences ({edges} edges) plain {t_plain * 1e3:.0f} ms, "
-        f"batched {t_batched * 1e3:.0f} ms ({t_plain / t_batched:.2f}x)")
+        f"batched {t_batched * 1e3:.0f} ms ({t_plain / t_batched:.2f}x), one launch per mode "
+        f"{t_multi * 1e3:.0f} ms ({t_plain / t_multi:.2f}x, equal to batched: {multi_ok})")
     if with_cpu:
         worst = 0.0
         for i in range(0, len(reps), 32):

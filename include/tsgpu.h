@@ -243,6 +243,12 @@ int tsg_batch_free(tsg_batch_t *batch);
 int tsg_batch_clear(tsg_batch_t *batch);
 uint64_t tsg_batch_get_num_queries(const tsg_batch_t *batch);
 int tsg_batch_run(tsg_batch_t *batch);
+/* Runs n batches (on one tree sequence or several, e.g. simulation
+ * replicates) as one sweep launch per mode over all their trees, on one
+ * stream; a batch whose outputs need several sweep slices, or that does not
+ * fit the hot plan, launches on its own.  Results equal tsg_batch_run's on
+ * each batch.  Returns when every result has been written. */
+int tsg_batch_run_multi(tsg_batch_t **batches, uint64_t n);
 int tsg_batch_diversity(tsg_batch_t *batch, uint64_t num_sample_sets,
     const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
     const double *windows, uint32_t options, double *result);

@@ -596,3 +596,63 @@ def test_repeatable(small):
     a = ts.diversity(sets, windows="trees", mode="branch")
     b = ts.diversity(sets, windows="trees", mode="branch")
     assert np.array_equal(a, b)
+
+
+def _replicate_queries(b, tc, seed, wide=False):
+    n = int(np.sum(tc.node_flags & 1))
+    sets = cases.sample_sets(n, 25 if wide else 3, n // 25 if wide else min(20, n // 4), seed)
+    w = np.linspace(0, tc.sequence_length, 4 + seed % 3)
+    if wide:
+        pairs = [(i, j) for i in range(25) for j in range(i + 1, 25)]  # 300 outputs
+    else:
+        pairs = [(0, 1), (1, 2)]
+    out = []
+    for mode in ("branch", "site"):
+        out.append(b.diversity(sets, windows=w, mode=mode))
+        out.append(b.divergence(sets, indexes=pairs, windows=w, mode=mode))
+        if not wide:
+            out.append(b.Tajimas_D(sets, windows=w, mode=mode))
+            out.append(b.f3(sets, indexes=[(0, 1, 2)], windows=w, mode=mode))
+    return out
+
+
+def test_run_batches_multi(require_gpu):
+    """Batches on several tree sequences run as one sweep launch per mode give
+    bit-identical results to each batch run alone: replicates of different
+    shapes, a gapped sequence, two batches on one tree sequence, a batch too
+    wide for one sweep slice and one forced onto the full plan (both launch
+    on their own inside the same call)."""
+    import tskit_amd as ta
+
+    tcs = [cases.generated(60 + 20 * i, 1e5 + 5e4 * i, 40 + i) for i in range(5)]
+    tcs.append(cases.gapped(80, 2e5, 7))
+    tss = [tc.tree_sequence() for tc in tcs]
+    tss[4].ll_tree_sequence.set_sweep_plan(1)
+    specs = [(i, i, False) for i in range(len(tss))] + [(0, 9, False), (2, 11, True)]
+    alone = []
+    for ti, seed, wide in specs:
+        b = tss[ti].batch()
+        p = _replicate_queries(b, tcs[ti], seed, wide)
+        b.run()
+        alone.append([x.result() for x in p])
+    batches, pend = [], []
+    for ti, seed, wide in specs:
+        b = tss[ti].batch()
+        pend.append(_replicate_queries(b, tcs[ti], seed, wide))
+        batches.append(b)
+    res = ta.run_batches(batches)
+    _say("run_batches", len(batches))
+    assert len(res) == len(specs)
+    for k in range(len(specs)):
+        assert len(res[k]) == len(alone[k])
+        for got, exp, p in zip(res[k], alone[k], pend[k]):
+            assert np.array_equal(np.asarray(got), np.asarray(exp), equal_nan=True), specs[k]
+            assert np.array_equal(np.asarray(p.result()), np.asarray(exp), equal_nan=True)
+    orc = Oracle(tcs[5])
+    sets = cases.sample_sets(80, 3, 20, 5)
+    w = np.linspace(0, tcs[5].sequence_length, 4 + 5 % 3)
+    _check("diversity", res[5][0], orc.stat("diversity", sets, None, w, "branch"))
+    _check("divergence", res[5][5], orc.stat("divergence", sets, [(0, 1), (1, 2)], w, "site"))
+    with pytest.raises(RuntimeError):
+        ta.run_batches(batches[:1])
+    assert ta.run_batches([]) == []

@@ -9,7 +9,7 @@ from .exceptions import LibraryError, TskitException
 from .kastore import load as load_tables
 from .sim import random_sample_sets, sim_ancestry, simulate_tables
 from .tables import NODE_IS_SAMPLE, TableCollection, parse_text
-from .trees import TreeSequence
+from .trees import StatBatch, TreeSequence, run_batches
 
 def load(path):
     """A TreeSequence from a ``.trees`` file (as tskit.load)."""
@@ -23,6 +23,8 @@ __all__ = [
     "TskitException",
     "TableCollection",
     "TreeSequence",
+    "StatBatch",
+    "run_batches",
     "NODE_IS_SAMPLE",
     "parse_text",
     "sim_ancestry",

@@ -12,8 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -21,6 +23,7 @@
 #include <numeric>
 #include <unordered_set>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "tsg_common.h"
@@ -48,6 +51,9 @@ struct Workspace {
         Rstart, raw, rawbase, lanebase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
     // first tree: counts and per-block running-sum partials
     DevBuf x0, r0part, bsum;
+    // one launch over several batches' sweeps: their argument blocks and
+    // chunk prefixes
+    DevBuf margs, mcpre;
     // relatedness-matrix route
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab, geno;
     int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
@@ -980,7 +986,7 @@ void free_one_workspace(Workspace *ws) {
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
                       &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
                       &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->geno, &ws->x0,
-                      &ws->r0part, &ws->bsum};
+                      &ws->r0part, &ws->bsum, &ws->margs, &ws->mcpre};
     for (DevBuf *b : bufs) {
         if (b->p != nullptr) (void) hipFree(b->p);
     }
@@ -1953,6 +1959,107 @@ int tsg_batch_run(tsg_batch *b) {
         if (ret != 0) return ret;
     }
     return 0;
+}
+
+namespace {
+// The runs of tsg_batch_run_multi, queued on stream s0.  Runs on the hot
+// plan whose outputs fit one slice share one sweep launch per (mode, node
+// store, outputs per lane); the rest launch on their own.
+int run_multi(tsg_batch **bs, uint64_t n, std::vector<std::unique_ptr<Lease>> &leases,
+              std::deque<std::vector<Query *>> &groups, std::deque<BatchRun> &runs,
+              hipStream_t &s0) {
+    int ret;
+    for (uint64_t i = 0; i < n; i++) {
+        tsg_batch *b = bs[i];
+        if (b == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+        {
+            std::lock_guard<std::mutex> lock(g_mutex);
+            if ((ret = device_upload(b->ts))) return ret;
+        }
+        for (uint32_t mode : {TSG_STAT_BRANCH, TSG_STAT_SITE}) {
+            std::vector<Query *> g;
+            for (auto &q : b->queries) {
+                if (q->mode == mode) g.push_back(q.get());
+            }
+            if (g.empty()) continue;
+            leases.push_back(std::make_unique<Lease>(b->ts));
+            Workspace *ws = leases.back()->ws;
+            if (ws == nullptr) return TSG_ERR_DEVICE;
+            if (s0 == nullptr) s0 = ws->stream;
+            groups.push_back(std::move(g));
+            runs.emplace_back();
+            if ((ret = batch_prepare(b->ts, groups.back(), ws, s0, runs.back()))) return ret;
+        }
+    }
+    if (runs.empty()) return 0;
+    std::map<std::tuple<int, int, int>, std::vector<BatchRun *>> shared;
+    for (BatchRun &r : runs) {
+        const bool hot = r.store == TSG_STORE_HOT || r.store == TSG_STORE_HOT16;
+        if (!hot || r.M > r.slice || r.M <= 0 || r.C <= 0) {
+            if ((ret = batch_sweep(r))) return ret;
+            continue;
+        }
+        const int mpl = r.M <= 64 ? 1 : r.M <= 128 ? 2 : 4;
+        shared[{r.mode_id, r.store, mpl}].push_back(&r);
+    }
+    if (!shared.empty()) {
+        std::vector<TsgSweepArgs> args;
+        std::vector<int32_t> cpre;
+        for (auto &kv : shared) {
+            int64_t total = 0;
+            cpre.push_back(0);
+            for (BatchRun *r : kv.second) {
+                args.push_back(r->a);
+                args.back().m_begin = 0;
+                args.back().m_end = r->M;
+                total += r->C;
+                if (total > INT32_MAX / 2) return TSG_ERR_BAD_PARAM_VALUE;
+                cpre.push_back((int32_t) total);
+            }
+        }
+        Workspace *ws0 = runs.front().ws;
+        if ((ret = upload(ws0->margs, args, s0)) || (ret = upload(ws0->mcpre, cpre, s0))) {
+            return ret;
+        }
+        size_t aoff = 0, coff = 0;
+        for (auto &kv : shared) {
+            const int nrun = (int) kv.second.size();
+            size_t shmem = 0;
+            for (BatchRun *r : kv.second) shmem = std::max(shmem, r->shmem);
+            const int32_t total = cpre[coff + nrun];
+            const int mode_id = std::get<0>(kv.first), store = std::get<1>(kv.first);
+            const int mpl = std::get<2>(kv.first);
+            if ((ret = hip_check(tsg_launch_sweep_multi(
+                     (const TsgSweepArgs *) ws0->margs.p + aoff,
+                     (const int32_t *) ws0->mcpre.p + coff, nrun, total, mode_id, store, mpl,
+                     shmem, s0)))) {
+                return ret;
+            }
+            aoff += nrun;
+            coff += nrun + 1;
+        }
+    }
+    for (BatchRun &r : runs) {
+        if ((ret = batch_finish(r))) return ret;
+    }
+    return 0;
+}
+}  // namespace
+
+int tsg_batch_run_multi(tsg_batch **batches, uint64_t n) {
+    if (n > 0 && batches == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+    std::vector<std::unique_ptr<Lease>> leases;
+    std::deque<std::vector<Query *>> groups;
+    std::deque<BatchRun> runs;
+    hipStream_t s0 = nullptr;
+    int ret = run_multi(batches, n, leases, groups, runs, s0);
+    // the leases end (workspaces return to their pools) only once all work
+    // queued on s0 has finished
+    if (s0 != nullptr) {
+        const int sret = hip_check(hipStreamSynchronize(s0));
+        if (ret == 0) ret = sret;
+    }
+    return ret;
 }
 
 #define BATCH_ONE_WAY(name, fn)                                                               \

@@ -565,10 +565,8 @@ __device__ __forceinline__ double branch_summary(const TsgOut &o, const int32_t 
 //     written to LDS, and each output's owner lane adds them in hop order.
 // Running sums and window sums are FP64, as in the reference.
 template <int MODE, int STORE, int MPL>
-__global__ void __launch_bounds__(WAVE, MPL == 1 ? 3 : 1) k_sweep(const TsgSweepArgs a) {
+__device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     extern __shared__ int32_t smem[];
-    const int c = xcd_chunk(blockIdx.x, a.num_chunks);
-    if (c >= a.num_chunks) return;
     const int lane = threadIdx.x;
     const int K = a.K;
     const int R = 3 + K;
@@ -940,6 +938,37 @@ __global__ void __launch_bounds__(WAVE, MPL == 1 ? 3 : 1) k_sweep(const TsgSweep
             L.wi++;
         }
     }
+}
+
+template <int MODE, int STORE, int MPL>
+__global__ void __launch_bounds__(WAVE, MPL == 1 ? 3 : 1) k_sweep(const TsgSweepArgs a) {
+    const int c = xcd_chunk(blockIdx.x, a.num_chunks);
+    if (c >= a.num_chunks) return;
+    sweep_chunk<MODE, STORE, MPL>(a, c);
+}
+
+// Many sweeps in one launch (a batch per tree sequence): the chunks of run r
+// are global chunks cpre[r] .. cpre[r+1]-1, each swept with that run's
+// arguments.  All runs share MODE, STORE and MPL; the dynamic LDS is the
+// largest run's.
+template <int MODE, int STORE, int MPL>
+__global__ void __launch_bounds__(WAVE, MPL == 1 ? 3 : 1)
+    k_sweep_multi(const TsgSweepArgs *__restrict__ args, const int32_t *__restrict__ cpre,
+                  int nrun) {
+    const int total = cpre[nrun];
+    const int g = xcd_chunk(blockIdx.x, total);
+    if (g >= total) return;
+    int lo = 0, hi = nrun - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cpre[mid] <= g) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    const TsgSweepArgs &a = args[lo];
+    sweep_chunk<MODE, STORE, MPL>(a, g - cpre[lo]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1703,6 +1732,55 @@ hipError_t launch_sweep_store(const TsgSweepArgs &a, int store, int mpl, size_t 
     }
 }
 }  // namespace
+
+namespace {
+template <int MODE, int STORE, int MPL>
+hipError_t launch_multi_one(const TsgSweepArgs *args, const int32_t *cpre, int nrun, int total,
+                            size_t shmem, hipStream_t s) {
+    const void *fn = reinterpret_cast<const void *>(&k_sweep_multi<MODE, STORE, MPL>);
+    if (shmem > 65536) {
+        if ((int) shmem > g_lds_max) return hipErrorInvalidValue;
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           g_lds_max);
+        if (e != hipSuccess) return e;
+    }
+    const unsigned grid = 8u * (unsigned) ((total + 7) / 8);
+    hipLaunchKernelGGL((k_sweep_multi<MODE, STORE, MPL>), dim3(grid), dim3(WAVE), shmem, s, args,
+                       cpre, nrun);
+    return hipGetLastError();
+}
+
+template <int MODE, int STORE>
+hipError_t launch_multi_mpl(const TsgSweepArgs *args, const int32_t *cpre, int nrun, int total,
+                            int mpl, size_t shmem, hipStream_t s) {
+    switch (mpl) {
+    case 1:
+        return launch_multi_one<MODE, STORE, 1>(args, cpre, nrun, total, shmem, s);
+    case 2:
+        return launch_multi_one<MODE, STORE, 2>(args, cpre, nrun, total, shmem, s);
+    default:
+        return launch_multi_one<MODE, STORE, 4>(args, cpre, nrun, total, shmem, s);
+    }
+}
+}  // namespace
+
+hipError_t tsg_launch_sweep_multi(const TsgSweepArgs *args, const int32_t *cpre, int nrun,
+                                  int total_chunks, int mode, int store, int mpl, size_t shmem,
+                                  hipStream_t s) {
+    if (store == TSG_STORE_HOT16) {
+        return mode == 0 ? launch_multi_mpl<0, TSG_STORE_HOT16>(args, cpre, nrun, total_chunks,
+                                                                  mpl, shmem, s)
+                         : launch_multi_mpl<1, TSG_STORE_HOT16>(args, cpre, nrun, total_chunks,
+                                                                  mpl, shmem, s);
+    }
+    if (store == TSG_STORE_HOT) {
+        return mode == 0 ? launch_multi_mpl<0, TSG_STORE_HOT>(args, cpre, nrun, total_chunks, mpl,
+                                                                shmem, s)
+                         : launch_multi_mpl<1, TSG_STORE_HOT>(args, cpre, nrun, total_chunks, mpl,
+                                                                shmem, s);
+    }
+    return hipErrorInvalidValue;
+}
 
 void tsg_set_lds_max(int bytes) { g_lds_max = std::max(bytes, 65536); }
 int tsg_lds_max() { return g_lds_max; }

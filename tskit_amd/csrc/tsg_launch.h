@@ -7,6 +7,11 @@
 // the device's dynamic LDS limit per workgroup (set once, before launches)
 void tsg_set_lds_max(int bytes);
 int tsg_lds_max();
+// one launch over the chunks of nrun sweeps (hot-plan stores only): args
+// and cpre (nrun + 1 chunk prefix) in device memory
+hipError_t tsg_launch_sweep_multi(const TsgSweepArgs *args, const int32_t *cpre, int nrun,
+                                  int total_chunks, int mode, int store, int mpl, size_t shmem,
+                                  hipStream_t s);
 // store: TSG_STORE_GLOBAL / TSG_STORE_LDS / TSG_STORE_HOT
 hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl, size_t shmem,
                             hipStream_t s);

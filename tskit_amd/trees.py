@@ -564,6 +564,31 @@ class StatBatch:
         return [p.result() for p in self._pending]
 
 
+def run_batches(batches):
+    """Runs several StatBatch objects (on one tree sequence or several, e.g.
+    simulation replicates) as one device sweep launch per mode over all of
+    them; returns each batch's results, as StatBatch.run() would."""
+    batches = list(batches)
+    if not batches:
+        return []
+    for b in batches:
+        if not isinstance(b, StatBatch):
+            raise TypeError("run_batches takes StatBatch objects")
+        if b.done:
+            raise RuntimeError("this batch has already run")
+    if len({id(b) for b in batches}) != len(batches):
+        raise ValueError("a batch appears twice")
+    lib = batches[0]._ll._lib
+    handles = (ctypes.c_void_p * len(batches))(*[b._ll._b.value for b in batches])
+    check(lib.tsg_batch_run_multi(handles, len(batches)))
+    out = []
+    for b in batches:
+        check(lib.tsg_batch_clear(b._ll._b))
+        b.done = True
+        out.append([p.result() for p in b._pending])
+    return out
+
+
 class TreeSequence:
     """A tree sequence resident on the GPU, with the reference's statistics API."""
 
