@@ -108,7 +108,10 @@ class _Base:
         else:
             idx = np.ascontiguousarray(np.asarray(indexes, dtype=np.int32).reshape(-1, t))
             M = idx.shape[0]
-        out = np.zeros((len(windows) - 1, M), dtype=np.float64)
+        shape = (len(windows) - 1, M)
+        if mode == "node":
+            shape = (len(windows) - 1, len(self.tc.node_time), M)
+        out = np.zeros(shape, dtype=np.float64)
         err = self._stat(sid, sizes, flat, t, M, idx, windows, opts, out)
         if err != 0:
             raise OracleError(err)

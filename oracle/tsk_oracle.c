@@ -442,6 +442,78 @@ static int branch_stat(const ots_t *ts, const double *W, const wf_t *f, int32_t 
     return 0;
 }
 
+/* ---- node statistics: trees.c:1788-1918 ---------------------------------
+ * Each node's summary of its own state integrated along the genome: the
+ * value held since the node's last update is added, over the span since
+ * then, whenever its state changes (it is on the root path above an inserted
+ * or removed edge) and at every window boundary.  result: [nw x N x M]. */
+static int node_stat(const ots_t *ts, const double *W, const wf_t *f, int32_t nw,
+    const double *windows, double *result) {
+    const int32_t N = ts->N, E = ts->E, K = f->p->K, M = f->p->M;
+    int32_t *par = malloc((size_t) (N + 1) * sizeof(int32_t));
+    double *X = calloc((size_t) N * K + 1, sizeof(double));
+    double *Y = calloc((size_t) N * M + 1, sizeof(double));
+    double *last = calloc((size_t) N + 1, sizeof(double));
+    double *z = calloc((size_t) K + 1, sizeof(double));
+    int32_t j, k, m, u, c, tj = 0, tk = 0, h, wi = 0;
+    double tl = 0, tr, wr;
+    for (j = 0; j < N; j++) par[j] = NUL;
+    wsummary(f, z, Y); /* f(0) for every node */
+    for (j = 1; j < N; j++) memcpy(Y + (size_t) j * M, Y, (size_t) M * sizeof(double));
+    for (j = 0; j < ts->n; j++) {
+        u = ts->samples[j];
+        memcpy(X + (size_t) u * K, W + (size_t) j * K, (size_t) K * sizeof(double));
+        wsummary(f, X + (size_t) u * K, Y + (size_t) u * M);
+    }
+    memset(result, 0, (size_t) nw * N * M * sizeof(double));
+#define FLUSH(node, pos)                                                              \
+    do {                                                                              \
+        for (m = 0; m < M; m++) {                                                     \
+            result[((size_t) wi * N + (node)) * M + m] +=                             \
+                ((pos) - last[node]) * Y[(size_t) (node) * M + m];                    \
+        }                                                                             \
+        last[node] = (pos);                                                           \
+    } while (0)
+    while (tj < E || tl < ts->L) {
+        while (tk < E && ts->right[ts->O[tk]] == tl) {
+            h = ts->O[tk++];
+            c = ts->child[h];
+            for (u = ts->parent[h]; u != NUL; u = par[u]) {
+                FLUSH(u, tl);
+                for (k = 0; k < K; k++) X[(size_t) u * K + k] -= X[(size_t) c * K + k];
+                wsummary(f, X + (size_t) u * K, Y + (size_t) u * M);
+            }
+            par[c] = NUL;
+        }
+        while (tj < E && ts->left[ts->I[tj]] == tl) {
+            h = ts->I[tj++];
+            c = ts->child[h];
+            par[c] = ts->parent[h];
+            for (u = ts->parent[h]; u != NUL; u = par[u]) {
+                FLUSH(u, tl);
+                for (k = 0; k < K; k++) X[(size_t) u * K + k] += X[(size_t) c * K + k];
+                wsummary(f, X + (size_t) u * K, Y + (size_t) u * M);
+            }
+        }
+        tr = ts->L;
+        if (tj < E && ts->left[ts->I[tj]] < tr) tr = ts->left[ts->I[tj]];
+        if (tk < E && ts->right[ts->O[tk]] < tr) tr = ts->right[ts->O[tk]];
+        while (wi < nw && windows[wi + 1] <= tr) {
+            wr = windows[wi + 1];
+            for (u = 0; u < N; u++) FLUSH(u, wr);
+            wi++;
+        }
+        tl = tr;
+    }
+#undef FLUSH
+    free(par);
+    free(X);
+    free(Y);
+    free(last);
+    free(z);
+    return 0;
+}
+
 /* ---- site sweep: trees.c:1525-1776 -------------------------------------- */
 static int site_stat(const ots_t *ts, const double *W, const sf_t *p, int polarised,
     int32_t nw, const double *windows, double *result) {
@@ -574,9 +646,9 @@ int oracle_sample_count_stat(const ots_t *ts, int stat, int32_t K, const uint64_
     }
     /* general_stat checks: trees.c:2048-2078 */
     if (!(site || branch || node)) site = 1;
-    if (site + branch + node > 1 || node) {
+    if (site + branch + node > 1) {
         free(W);
-        return (site + branch + node > 1) ? -902 : -909; /* node mode: out of scope */
+        return -902;
     }
     if (windows == NULL) {
         nw = 1;
@@ -602,8 +674,8 @@ int oracle_sample_count_stat(const ots_t *ts, int stat, int32_t K, const uint64_
     p.tuple = tuple;
     p.n = n;
     p.idx = idx;
-    if (branch) {
-        if (ts->time_uncalibrated && !(options & OPT_ALLOW_UNCAL)) {
+    if (branch || node) {
+        if (branch && ts->time_uncalibrated && !(options & OPT_ALLOW_UNCAL)) {
             ret = -910;
             goto out;
         }
@@ -614,14 +686,16 @@ int oracle_sample_count_stat(const ots_t *ts, int stat, int32_t K, const uint64_
         wf.total = total;
         wf.tmp_x = tx;
         wf.tmp_y = ty;
-        ret = branch_stat(ts, W, &wf, nw, windows, result);
+        ret = node ? node_stat(ts, W, &wf, nw, windows, result)
+                   : branch_stat(ts, W, &wf, nw, windows, result);
     } else {
         ret = site_stat(ts, W, &p, !!(options & OPT_POLARISED), nw, windows, result);
     }
-    if (ret == 0 && (options & OPT_SPAN_NORMALISE)) { /* trees.c:1920-1934 */
+    if (ret == 0 && (options & OPT_SPAN_NORMALISE)) { /* trees.c:1920-1934, 2085-2091 */
+        const size_t row = (size_t) p.M * (node ? (size_t) ts->N : 1);
         for (j = 0; j < nw; j++) {
             const double span = windows[j + 1] - windows[j];
-            for (k = 0; k < p.M; k++) result[(size_t) j * p.M + k] /= span;
+            for (size_t r = 0; r < row; r++) result[(size_t) j * row + r] /= span;
         }
     }
 out:

@@ -178,10 +178,12 @@ def rel_err(got, exp, scale=None):
     assert got.shape == exp.shape, (got.shape, exp.shape)
     nan_g, nan_e = np.isnan(got), np.isnan(exp)
     assert np.array_equal(nan_g, nan_e), "NaN pattern differs"
+    inf_e = np.isinf(exp)
+    assert np.array_equal(got[inf_e], exp[inf_e]), "infinities differ"
     if scale is None:
         scale = np.abs(exp)
     scale = np.maximum(np.broadcast_to(scale, exp.shape), 1e-300)
-    fin = ~nan_e
+    fin = np.isfinite(exp)
     if not np.any(fin):
         return 0.0
     return float(np.max(np.abs(got[fin] - exp[fin]) / scale[fin]))

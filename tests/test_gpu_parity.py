@@ -32,7 +32,8 @@ def _check(name, got, exp):
     if name in POSITIVE or name == "genetic_relatedness_nc":
         err = rel_err(got, exp)
     else:
-        scale = np.max(np.abs(exp), axis=tuple(range(exp.ndim - 1)), keepdims=True)
+        fin = np.where(np.isfinite(exp), np.abs(exp), 0.0)
+        scale = np.max(fin, axis=tuple(range(exp.ndim - 1)), keepdims=True)
         err = rel_err(got, exp, scale=np.maximum(scale, 1e-300))
     assert err <= TOL, f"{name}: relative error {err:.3e}"
     return err
@@ -656,3 +657,73 @@ def test_run_batches_multi(require_gpu):
     with pytest.raises(RuntimeError):
         ta.run_batches(batches[:1])
     assert ta.run_batches([]) == []
+
+
+def _node_case(name):
+    if name == "generated":
+        return cases.generated(120, 2e5, 23)
+    if name == "gapped":
+        return cases.gapped(120, 2e5, 23, count=30)
+    return getattr(cases, name)()
+
+
+@pytest.mark.parametrize("name", ["generated", "gapped", "paper_ex", "nonbinary_ex",
+                                  "single_tree_ex"])
+def test_node_mode(require_gpu, name):
+    """Node statistics (trees.c:1788-1918) against the oracle, which is pinned
+    to the reference's node outputs: one- and k-way summaries, polarised and
+    not, span-normalised and not, several windows and none, and the composed
+    Fst and Tajima's D; shapes as the reference's (windows, nodes, outputs)."""
+    tc = _node_case(name)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    n = int(np.sum(tc.node_flags & 1))
+    N = tc.num_nodes
+    k = 3 if n >= 6 else 2
+    size = max(1, min(25, n // k))
+    sets = cases.sample_sets(n, k, size, 31)
+    L = tc.sequence_length
+    w = np.array([0, 0.23 * L, 0.5 * L, L])
+    _say("node", name, N)
+    pairs = [(0, 1), (1, 0)] if k == 2 else [(0, 1), (1, 2), (2, 2)]
+    for sn in (True, False):
+        got = ts.diversity(sets, windows=w, mode="node", span_normalise=sn)
+        assert got.shape == (3, N, k)
+        _check("diversity", got, orc.stat("diversity", sets, None, w, "node", sn))
+        got = ts.divergence(sets, indexes=pairs, windows=w, mode="node", span_normalise=sn)
+        assert got.shape == (3, N, len(pairs))
+        _check("divergence", got, orc.stat("divergence", sets, pairs, w, "node", sn))
+        got = ts.segregating_sites(sets, windows=w, mode="node", span_normalise=sn)
+        _check("segregating_sites", got, orc.stat("segregating_sites", sets, None, w, "node", sn))
+    got = ts.genetic_relatedness(sets, indexes=pairs, windows=w, mode="node", polarised=True,
+                                 proportion=False)
+    _check("genetic_relatedness", got,
+           orc.stat("genetic_relatedness", sets, pairs, w, "node", True, True))
+    got = ts.genetic_relatedness(sets, indexes=pairs, windows=w, mode="node", polarised=False,
+                                 proportion=False, centre=False)
+    _check("genetic_relatedness noncentred", got,
+           orc.stat("genetic_relatedness", sets, pairs, w, "node", True, False, False))
+    if k >= 3 and size >= 2:
+        got = ts.f3(sets, indexes=[(0, 1, 2)], windows=w, mode="node")
+        _check("f3", got, orc.stat("f3", sets, [(0, 1, 2)], w, "node"))
+    # no windows: (nodes, outputs); a single set: (nodes,)
+    got = ts.diversity(sets, mode="node")
+    assert got.shape == (N, k)
+    _check("diversity whole", got, orc.stat("diversity", sets, None, None, "node")[0])
+    assert ts.diversity(sets[0], mode="node").shape == (N,)
+    # composed statistics, per (window, node)
+    div = orc.stat("diversity", sets, None, w, "node").reshape(-1, k)
+    dvg = orc.stat("divergence", sets, pairs, w, "node").reshape(-1, len(pairs))
+    got = ts.Fst(sets, indexes=pairs, windows=w, mode="node")
+    _check("Fst", got, fst(div, dvg, pairs).reshape(3, N, len(pairs)))
+    T = orc.stat("diversity", sets, None, w, "node", False).reshape(-1, k)
+    S = orc.stat("segregating_sites", sets, None, w, "node", False).reshape(-1, k)
+    got = ts.Tajimas_D(sets, windows=w, mode="node")
+    _check("Tajimas_D", got, tajimas_d(T, S, [len(s) for s in sets]).reshape(3, N, k))
+
+
+def test_node_mode_not_batched(small):
+    tc, ts, orc = small
+    b = ts.batch()
+    with pytest.raises(Exception):
+        b.diversity(cases.sample_sets(200, 2, 10, 1), mode="node")

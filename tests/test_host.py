@@ -157,7 +157,7 @@ def test_stat_errors_match_reference_numbers(paper):
     assert _err(lambda: ll.diversity([2], [0, 1], [0, 5])) == -901        # windows end
     assert _err(lambda: ll.diversity([2], [0, 1], [1, 10])) == -901       # windows start
     assert _err(lambda: ll.diversity([2], [0, 1], [0, 5, 5, 10])) == -901  # not increasing
-    assert _err(lambda: ll.diversity([2], [0, 1], w, mode="node")) == -909
+    assert _err(lambda: ll.diversity([2], [0, 1], [0, 5], mode="node")) == -901  # node mode checks too
     assert _err(lambda: ll.divergence([2, 2], [0, 1, 2, 3], [(0, 2)], w)) == -907
     assert _err(lambda: ll.f4([1, 1, 1, 1], [0, 1, 2, 3], [(0, 1, 2, 4)], w)) == -907
     assert _err(lambda: ll.allele_frequency_spectrum(

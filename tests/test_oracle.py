@@ -374,3 +374,64 @@ def test_identity_invariant():
                 total += (b - a) * (tc.node_time[par[u]] - tc.node_time[u])
     got = orc.stat("segregating_sites", [list(range(10))], None, None, "branch", False)[0, 0]
     assert np.isclose(got, total, rtol=1e-12)
+
+
+NODE_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "node_reference.npz")
+
+
+@pytest.mark.parametrize("key", ["generated", "gapped"])
+def test_golden_node_stats(key):
+    """Node statistics of the oracle against the reference's own outputs
+    (tests/golden/make_node_golden.py)."""
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_node_golden as mk
+
+    g = np.load(NODE_GOLDEN)
+    tc = mk.inputs()[key]
+    assert tc.column_digest() == bytes(g[f"{key}_digest"]).decode(), "generator output changed"
+    orc = Oracle(tc)
+    sets = _sets(g["sets"], 4)
+    for name, stat, idx, pol, sn in mk.CHECKS:
+        got = orc.stat(stat, sets, idx, g["windows"], "node", sn, pol)
+        exp = g[f"{key}_{name}"]
+        assert got.shape == exp.shape == (3, tc.num_nodes, exp.shape[2])
+        assert np.allclose(got, exp, rtol=1e-12, atol=1e-12 * np.max(np.abs(exp))), name
+
+
+@needs_ref
+@pytest.mark.parametrize("fixture", ["paper_ex", "nonbinary_ex", "single_tree_ex"])
+def test_oracle_node_vs_reference_fixtures(fixture):
+    tc = getattr(cases, fixture)()
+    orc, ref = Oracle(tc), om.Reference(tc)
+    n = int(np.sum(tc.node_flags & 1))
+    sets = [np.arange(0, n // 2, dtype=np.int32), np.arange(n // 2, n, dtype=np.int32)]
+    w = np.linspace(0, tc.sequence_length, 4)
+    for name, idx in [("diversity", None), ("segregating_sites", None),
+                      ("divergence", [(0, 1), (1, 1)]), ("f2", [(0, 1)])]:
+        for pol in (False, True):
+            for sn in (False, True):
+                a = orc.stat(name, sets, idx, w, "node", sn, pol)
+                b = ref.stat(name, sets, idx, w, "node", sn, pol)
+                assert np.allclose(a, b, rtol=1e-13, atol=1e-13, equal_nan=True), name
+
+
+def test_node_single_tree_identity():
+    """On one tree the branch statistic is the node statistic weighted by
+    each node's branch length (trees.c:1352-1523 vs 1788-1918): a dropped
+    piece, a wrong window or a missed update fails this."""
+    tc = cases.single_tree_ex()
+    orc = Oracle(tc)
+    sets = [np.array([0, 1, 2], np.int32), np.array([1, 2, 3], np.int32)]
+    par = np.full(tc.num_nodes, -1)
+    par[tc.edge_child] = tc.edge_parent
+    bl = np.where(par >= 0, tc.node_time[par] - tc.node_time, 0.0)
+    for name, idx in [("diversity", None), ("divergence", [(0, 1)]), ("Y1", None)]:
+        for pol in (False, True):
+            node = orc.stat(name, sets, idx, None, "node", False, pol)
+            branch = orc.stat(name, sets, idx, None, "branch", False, pol)
+            assert node.shape == (1, tc.num_nodes, node.shape[2])
+            assert np.all(np.isfinite(node))
+            assert np.allclose(np.einsum("wnm,n->wm", node, bl), branch, rtol=1e-13), name

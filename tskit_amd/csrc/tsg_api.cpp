@@ -281,7 +281,6 @@ int parse_mode_windows(const tsg_treeseq *ts, uint32_t options, uint64_t nw, con
     const bool node = options & TSG_STAT_NODE;
     if (!(site || branch || node)) site = true;
     if ((int) site + (int) branch + (int) node > 1) return TSG_ERR_MULTIPLE_STAT_MODES;
-    if (node) return TSG_ERR_UNSUPPORTED_STAT_MODE;  // node mode is not on this path
     if (windows == nullptr) {
         q.windows = {0.0, ts->L};
     } else {
@@ -289,7 +288,7 @@ int parse_mode_windows(const tsg_treeseq *ts, uint32_t options, uint64_t nw, con
         if (ret != 0) return ret;
         q.windows.assign(windows, windows + nw + 1);
     }
-    q.mode = branch ? TSG_STAT_BRANCH : TSG_STAT_SITE;
+    q.mode = node ? TSG_STAT_NODE : branch ? TSG_STAT_BRANCH : TSG_STAT_SITE;
     if (branch && ts->time_uncalibrated && !(options & TSG_STAT_ALLOW_TIME_UNCALIBRATED)) {
         return TSG_ERR_TIME_UNCALIBRATED;
     }
@@ -361,20 +360,26 @@ struct BatchRun {
     std::vector<TsgFinalHost> fin;
 };
 
-// Host work of a batch and its uploads on stream s: columns, outputs,
-// per-(chunk, query) window ranges, node store, sweep arguments, finals.
-int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, hipStream_t s,
-                  BatchRun &r) {
-    int ret;
-    const int Q = (int) batch.size();
-    const uint32_t mode = batch[0]->mode;
-    // fused columns / outputs
+// The fused state columns and outputs of a batch: one column per distinct
+// sample set, one TsgOut per raw output, per-query windows and offsets.
+struct Fused {
     std::vector<int32_t> pair_s, pair_c, col_total;
     std::vector<TsgOut> outs;
-    std::vector<TsgQuery> qs(Q);
+    std::vector<TsgQuery> qs;
     std::vector<double> wins;
     bool noskip = false, any_unpolarised = false;
     int K = 0, M = 0;
+};
+
+void fuse_queries(std::vector<Query *> &batch, Fused &f) {
+    const int Q = (int) batch.size();
+    std::vector<int32_t> &pair_s = f.pair_s, &pair_c = f.pair_c, &col_total = f.col_total;
+    std::vector<TsgOut> &outs = f.outs;
+    std::vector<TsgQuery> &qs = f.qs;
+    qs.assign(Q, TsgQuery{});
+    std::vector<double> &wins = f.wins;
+    bool &noskip = f.noskip, &any_unpolarised = f.any_unpolarised;
+    int &K = f.K, &M = f.M;
     // Identical sample sets (same samples, any order) share one state column
     // across the batch.  A query with a centred relatedness block keeps its
     // own contiguous columns: its summary averages over them.
@@ -509,6 +514,23 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
             K = K2;
         }
     }
+}
+
+// Host work of a batch and its uploads on stream s: columns, outputs,
+// per-(chunk, query) window ranges, node store, sweep arguments, finals.
+int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, hipStream_t s,
+                  BatchRun &r) {
+    int ret;
+    const int Q = (int) batch.size();
+    const uint32_t mode = batch[0]->mode;
+    Fused fz;
+    fuse_queries(batch, fz);
+    std::vector<int32_t> &pair_s = fz.pair_s, &pair_c = fz.pair_c, &col_total = fz.col_total;
+    std::vector<TsgOut> &outs = fz.outs;
+    std::vector<TsgQuery> &qs = fz.qs;
+    std::vector<double> &wins = fz.wins;
+    const bool noskip = fz.noskip, any_unpolarised = fz.any_unpolarised;
+    const int K = fz.K, M = fz.M;
     // Plan and node store: the hot plan with its slots in LDS when the
     // largest chunk's slots fit; otherwise the full plan, in LDS when its
     // largest chunk fits, else in per-chunk HBM slabs.
@@ -823,7 +845,14 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
 // instead of being run (see tsg_batch_*).
 thread_local std::vector<std::unique_ptr<Query>> *g_capture = nullptr;
 
+int run_node(tsg_treeseq *ts, Query &q);
+
 int run_one(tsg_treeseq *ts, Query &q) {
+    if (q.mode == TSG_STAT_NODE) {
+        // node statistics are not fused into batches
+        if (g_capture != nullptr) return TSG_ERR_UNSUPPORTED_STAT_MODE;
+        return run_node(ts, q);
+    }
     if (g_capture != nullptr) {
         g_capture->push_back(std::make_unique<Query>(std::move(q)));
         return 0;
@@ -979,6 +1008,134 @@ int ensure_hot_afs(tsg_treeseq *ts) {
     return 0;
 }
 
+// Node statistics tables of the hot plan (with the AFS tables they share).
+int ensure_hot_node(tsg_treeseq *ts) {
+    int ret = ensure_hot_afs(ts);
+    if (ret != 0) return ret;
+    TsgHotDevice &h = ts->hot.dev;
+    if (h.node_lid != nullptr) return 0;
+    const tsg_hotplan &hp = ts->hot;
+    size_t &hb = h.bytes;
+    if ((ret = upload_ptr(&h.slot_node, hp.slot_node, hb)) ||
+        (ret = upload_ptr(&h.node_last, hp.node_last, hb)) ||
+        (ret = upload_ptr(&h.node_lid, hp.node_lid, hb))) {
+        return ret;
+    }
+    return 0;
+}
+
+// Node statistics (trees.c:1788-1918, with the unpolarised wrapper of
+// trees.c:1944-1972 and the span normalisation of trees.c:2085-2091) on the
+// hot plan; the result is W x num_nodes x P, composed per (window, node) row
+// as the batch path composes per window.
+int run_node(tsg_treeseq *ts, Query &q) {
+    int ret;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = device_upload(ts))) return ret;
+    }
+    const tsg_hotplan &hp = ts->hot;
+    if (!hp.usable) return TSG_ERR_UNSUPPORTED_INPUT;
+    std::vector<Query *> one{&q};
+    Fused fz;
+    fuse_queries(one, fz);
+    const int K = fz.K, M = fz.M;
+    const int W = (int) q.windows.size() - 1;
+    const int64_t N = ts->num_nodes;
+    if ((double) W * (double) N >= (double) INT32_MAX) return TSG_ERR_UNSUPPORTED_INPUT;
+    const size_t shmem = (size_t) hp.max_slots * (8 + 4 * (size_t) K) + 64;
+    Lease lease(ts);
+    Workspace *ws = lease.ws;
+    if (ws == nullptr) return TSG_ERR_DEVICE;
+    if (shmem > (size_t) ws->lds_limit) return TSG_ERR_UNSUPPORTED_INPUT;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = ensure_hot_node(ts))) return ret;
+    }
+    hipStream_t s = ws->stream;
+    const TsgHotDevice &h = hp.dev;
+    const TsgDevice &d = ts->dev;
+    const int64_t nres = (int64_t) W * N * M;
+    const int64_t nout = (int64_t) W * N * q.P;
+    const int nl = (int) hp.tl_parent.size();
+    TsgSweepArgs a{};
+    if ((ret = upload_seeds(ws, fz.pair_s, fz.pair_c, ts->num_samples, s, a)) ||
+        (ret = upload(ws->tot, fz.col_total, s)) || (ret = upload(ws->outs, fz.outs, s)) ||
+        (ret = upload(ws->windows, q.windows, s)) || (ret = ensure(ws->raw, (size_t) nres * 8)) ||
+        (ret = ensure(ws->out, (size_t) nout * 8)) ||
+        (ret = ensure(ws->x0, ((size_t) nl + 1) * K * 4))) {
+        return ret;
+    }
+    a.chunk_tree0 = h.chunk_tree0;
+    a.chunk_node0 = h.chunk_slot0;
+    a.tr_off = h.tr_off;
+    a.tr_nrem = h.tr_nrem;
+    a.ev_child = h.ev_child;
+    a.ev_path0 = h.ev_path0;
+    a.path = h.path;
+    a.slot_par = h.slot_par;
+    a.chunk_lev0 = h.chunk_lev0;
+    a.lev_slot0 = h.lev_slot0;
+    a.sample_slot = h.sample_slot;
+    a.slot_last0 = h.slot_last0;
+    a.bp = d.breakpoints;
+    a.num_trees = ts->num_trees;
+    a.num_chunks = hp.num_chunks;
+    a.num_samples = ts->num_samples;
+    a.K = K;
+    a.M = M;
+    a.Q = 1;
+    a.col_total = (const int32_t *) ws->tot.p;
+    a.outs = (const TsgOut *) ws->outs.p;
+    a.windows = (const double *) ws->windows.p;
+    a.afs_W = W;
+    TsgLastTree lt{};
+    lt.parent = h.tl_parent;
+    lt.nl = nl;
+    lt.counts = (int32_t *) ws->x0.p;
+    double *raw = (double *) ws->raw.p;
+    if ((ret = hip_check(tsg_launch_fill_f64(raw, nres, s)))) return ret;
+    if (ws->profile) (void) hipEventRecord(ws->ev0, s);
+    if ((ret = hip_check(tsg_launch_node(a, h.slot_node, raw, N, shmem, lt, h.node_last,
+                                         h.node_lid, ts->L, s)))) {
+        return ret;
+    }
+    if (ws->profile) (void) hipEventRecord(ws->ev1, s);
+    ws->last_store = TSG_STORE_HOT;
+    const uint32_t fflags = q.span_normalise ? TSG_F_SPAN_NORMALISE : 0u;
+    if ((ret = hip_check(tsg_launch_afs_branch_final(raw, a.windows, W, N * M, fflags, s)))) {
+        return ret;
+    }
+    std::vector<TsgFinalHost> fin(1);
+    fin[0].kind = q.final_kind;
+    fin[0].Kq = (int32_t) q.sets.size();
+    fin[0].P = q.P;
+    fin[0].W = (int32_t) (W * N);
+    fin[0].rawbase = 0;
+    fin[0].outbase = 0;
+    fin[0].pbase = 0;
+    fin[0].Mraw = M;
+    if ((ret = upload(ws->fin, fin, s)) || (ret = upload(ws->ipar, q.fin_ipar, s)) ||
+        (ret = upload(ws->dpar, q.fin_dpar, s))) {
+        return ret;
+    }
+    if ((ret = hip_check(tsg_launch_finals(ws->fin.p, 1, raw, (const int32_t *) ws->ipar.p,
+                                           (const double *) ws->dpar.p, (double *) ws->out.p,
+                                           nout, s))) ||
+        (ret = hip_check(hipMemcpyAsync(q.result, ws->out.p, (size_t) nout * 8,
+                                        hipMemcpyDeviceToHost, s))) ||
+        (ret = hip_check(hipStreamSynchronize(s)))) {
+        return ret;
+    }
+    if (ws->profile) {
+        float ms = 0;
+        (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
+        ws->sweep_ms += ms;
+        ws->sweep_launches += 1;
+    }
+    return 0;
+}
+
 void free_one_workspace(Workspace *ws) {
     DevBuf *bufs[] = {&ws->seed_off, &ws->seed_ps, &ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
                       &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
@@ -1024,7 +1181,8 @@ void tsg_hot_device_free(TsgHotDevice *h) {
                     h->lev_slot0, h->sample_slot, h->tr_off, h->tr_nrem, h->ev_child, h->ev_bl,
                     h->ev_path0, h->path, h->mut_slot, h->t0_parent, h->t0_bl,
                     h->slot_nsd, h->slot_tu, h->slot_last0, h->slot_ins0, h->slot_tp0,
-                    h->tl_parent, h->tl_time, h->tl_last, h->tl_ins};
+                    h->tl_parent, h->tl_time, h->tl_last, h->tl_ins, h->slot_node,
+                    h->node_last, h->node_lid};
     for (void *p : ptrs) {
         if (p != nullptr) (void) hipFree(p);
     }

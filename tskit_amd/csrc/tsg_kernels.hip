@@ -1419,6 +1419,134 @@ __global__ void k_afs_tl_flush(const TsgSweepArgs a, const TsgLastTree lt, doubl
 }
 
 // ---------------------------------------------------------------------------
+// Node statistics (trees.c:1788-1918): for every node u and output m, the
+// summary f_m(x(u)) of the node's own state, integrated along the genome in
+// each window.  A node's state changes only where an event's root path
+// passes through it, so f is constant between its updates; the value is
+// added at each update over the span since the previous one, cut at the
+// window boundaries that span crosses.  The reference adds the same value
+// at the same updates and at every window boundary: the same pieces, and a
+// zero-length update adds f * 0 in both (NaN when f is not finite).  The
+// flush points are the branch AFS's (the path nodes, and a removed edge's
+// child, whose state does not change there: an extra cut of a constant
+// piece), so the planner's last-flush positions per slot apply; intervals
+// still open at the end are added from the last tree.  Results go to
+// res[(window * N + node) * M + m] by FP64 atomics (the order of the adds,
+// and so the last bits, varies between runs).
+namespace {
+
+__device__ void node_flush(const TsgSweepArgs &a, double *res, int64_t N, int64_t node, int m,
+                           double f, double from, double x) {
+    const double *win = a.windows;
+    const int W = a.afs_W;
+    int wi = afs_window_of(win, W, from);
+    double p = from;
+    while (wi < W && win[wi + 1] <= x) {
+        const double q = win[wi + 1];
+        atomicAdd(&res[((int64_t) wi * N + node) * a.M + m], f * (q - p));
+        p = q;
+        wi++;
+    }
+    if (wi < W) atomicAdd(&res[((int64_t) wi * N + node) * a.M + m], f * (x - p));
+}
+
+}  // namespace
+
+// One wave per chunk.  LDS: per slot its last flush position (double), then
+// the counts [S x K].
+__global__ void __launch_bounds__(WAVE) k_node_hot(const TsgSweepArgs a,
+                                                   const int32_t *__restrict__ slot_node,
+                                                   double *res, int64_t N) {
+    extern __shared__ double dsm[];
+    const int c = xcd_chunk(blockIdx.x, a.num_chunks);
+    if (c >= a.num_chunks) return;
+    const int lane = threadIdx.x;
+    const int K = a.K, M = a.M;
+    const int64_t s0 = a.chunk_node0[c];
+    const int S = (int) (a.chunk_node0[c + 1] - s0);
+    double *last = dsm;
+    int32_t *X = reinterpret_cast<int32_t *>(dsm + S);
+    for (int i = lane; i < S * K; i += WAVE) X[i] = 0;
+    for (int h = lane; h < S; h += WAVE) last[h] = a.slot_last0[s0 + h];
+    __syncthreads();
+    const uint16_t *ss = sample_map(a, c);
+    for (int i = lane; i < a.npairs; i += WAVE) {
+        const unsigned h = ss[(int64_t) a.pair_sample[i] * TSG_SS_CB];
+        if (h != TSG_NO_SLOT) atomicAdd(&X[h * K + a.pair_col[i]], 1);
+    }
+    __syncthreads();
+    const int32_t *lev = a.lev_slot0 + a.chunk_lev0[c];
+    const int nlev = (int) (a.chunk_lev0[c + 1] - a.chunk_lev0[c]) - 1;
+    for (int l = 0; l + 1 < nlev; l++) {
+        const int h0 = lev[l], h1 = lev[l + 1];
+        for (int idx = lane; idx < (h1 - h0) * K; idx += WAVE) {
+            const int h = h0 + idx / K, k = idx % K;
+            const int p = a.slot_par[s0 + h];
+            const int32_t v = X[h * K + k];
+            if (p >= 0 && v != 0) atomicAdd(&X[p * K + k], v);
+        }
+        __syncthreads();
+    }
+    const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
+    // the closing transition too: its updates happen at the chunk end
+    const int tend = (t1 < a.num_trees) ? t1 + 1 : t1;
+    for (int t = t0 + 1; t < tend; t++) {
+        const double x = a.bp[t];
+        const int64_t e0 = a.tr_off[t], e1 = a.tr_off[t + 1];
+        const int64_t nrem = a.tr_nrem[t];
+        for (int64_t e = e0; e < e1; e++) {
+            const bool ins = (e - e0) >= nrem;
+            const int u = a.ev_child[e];
+            const int64_t p0 = a.ev_path0[e];
+            const int plen = (int) (a.ev_path0[e + 1] - p0);
+            const int nn = plen + (ins ? 0 : 1);
+            for (int idx = lane; idx < nn * M; idx += WAVE) {
+                const int j = idx / M, m = idx % M;
+                const int w = (j < plen) ? a.path[p0 + j] : u;
+                const double f = branch_summary(a.outs[m], X + w * K, a.col_total);
+                node_flush(a, res, N, slot_node[s0 + w], m, f, last[w], x);
+            }
+            __syncthreads();
+            for (int j = lane; j < nn; j += WAVE) last[(j < plen) ? a.path[p0 + j] : u] = x;
+            for (int idx = lane; idx < plen * K; idx += WAVE) {
+                const int j = idx / K, k = idx % K;
+                const int w = a.path[p0 + j];
+                const int32_t v = X[u * K + k];
+                X[w * K + k] += ins ? v : -v;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Counts of the last tree over the K state columns: thread i adds its
+// (sample, column) pair along the sample's root path.
+__global__ void k_node_tl_counts(const TsgSweepArgs a, const TsgLastTree lt) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t) a.npairs) return;
+    const int k = a.pair_col[i];
+    for (int u = a.pair_sample[i]; u >= 0; u = lt.parent[u]) {
+        atomicAdd(&lt.counts[(int64_t) u * a.K + k], 1);
+    }
+}
+
+// The final piece [last flush, L) of every node u and output m, one thread
+// each; a node not in the last tree has state 0 there.
+__global__ void k_node_tl_flush(const TsgSweepArgs a, const TsgLastTree lt,
+                                const double *__restrict__ node_last,
+                                const int32_t *__restrict__ node_lid, double *res, int64_t N,
+                                const int32_t *zeros, double L) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * a.M) return;
+    const int64_t u = i / a.M;
+    const int m = (int) (i % a.M);
+    const int32_t q = node_lid[u];
+    const int32_t *x = (q >= 0) ? lt.counts + (int64_t) q * a.K : zeros;
+    const double f = branch_summary(a.outs[m], x, a.col_total);
+    node_flush(a, res, N, u, m, f, node_last[u], L);
+}
+
+// ---------------------------------------------------------------------------
 // Running branch sum of the first tree (chunk 0's start tree), evaluated in
 // parallel instead of by replaying its edge insertions one at a time:
 //   R(tree 0) = sum over nodes u with a parent of bl(u) * F(x(u)),
@@ -1920,6 +2048,34 @@ hipError_t tsg_launch_afs_site_final(const unsigned long long *h, double *out, c
     const int64_t n = (int64_t) W * afs_size;
     hipLaunchKernelGGL(k_afs_site_final, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, h,
                        out, win, W, afs_size, flags);
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_node(const TsgSweepArgs &a, const int32_t *slot_node, double *res,
+                           int64_t N, size_t shmem, const TsgLastTree &lt, const double *node_last,
+                           const int32_t *node_lid, double L, hipStream_t s) {
+    hipError_t e;
+    if ((e = allow_lds(&k_node_hot, shmem)) != hipSuccess) return e;
+    if (a.num_chunks > 0) {
+        const unsigned grid = 8u * (unsigned) ((a.num_chunks + 7) / 8);
+        hipLaunchKernelGGL(k_node_hot, dim3(grid), dim3(WAVE), shmem, s, a, slot_node, res, N);
+    }
+    // counts of the nl last-tree nodes, then one row of zeros
+    const int64_t words = ((int64_t) lt.nl + 1) * a.K;
+    const int32_t *zeros = lt.counts + (int64_t) lt.nl * a.K;
+    if (words > 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned) ((words + 255) / 256)), dim3(256), 0, s,
+                           lt.counts, words);
+    }
+    if (a.npairs > 0) {
+        hipLaunchKernelGGL(k_node_tl_counts, dim3((unsigned) ((a.npairs + 255) / 256)), dim3(256),
+                           0, s, a, lt);
+    }
+    const int64_t items = N * a.M;
+    if (items > 0) {
+        hipLaunchKernelGGL(k_node_tl_flush, dim3((unsigned) ((items + 255) / 256)), dim3(256), 0,
+                           s, a, lt, node_last, node_lid, res, N, zeros, L);
+    }
     return hipGetLastError();
 }
 

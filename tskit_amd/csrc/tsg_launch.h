@@ -40,6 +40,12 @@ hipError_t tsg_launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);
 hipError_t tsg_launch_fill_f64(double *p, int64_t n, hipStream_t s);
 hipError_t tsg_launch_afs_site_final(const unsigned long long *h, double *out, const double *win,
                                      int W, int64_t afs_size, uint32_t flags, hipStream_t s);
+// node statistics on the hot plan: the chunk sweep, the last tree's counts
+// and the final flush of every node (res zeroed by the caller; lt.counts
+// holds (lt.nl + 1) x K ints)
+hipError_t tsg_launch_node(const TsgSweepArgs &a, const int32_t *slot_node, double *res,
+                           int64_t N, size_t shmem, const TsgLastTree &lt, const double *node_last,
+                           const int32_t *node_lid, double L, hipStream_t s);
 hipError_t tsg_launch_afs_branch_final(double *h, const double *win, int W, int64_t per_window,
                                        uint32_t flags, hipStream_t s);
 size_t tsg_final_struct_size();

@@ -776,6 +776,8 @@ int build_hot_plan_bounds(tsg_treeseq *ts, std::vector<int32_t> bounds) {
             hp.tl_ins[q] = gins[u];
             if (gpar[u] != TSG_NULL) hp.tl_parent[q] = lid[gpar[u]];
         }
+        hp.node_last = glast;
+        hp.node_lid = lid;
     }
     hp.usable = true;
     return 0;

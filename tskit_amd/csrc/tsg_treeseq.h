@@ -66,6 +66,10 @@ struct TsgHotDevice {
     double *tl_time = nullptr;
     double *tl_last = nullptr;
     double *tl_ins = nullptr;
+    // node statistics tables (uploaded on first use)
+    int32_t *slot_node = nullptr;
+    double *node_last = nullptr;
+    int32_t *node_lid = nullptr;
     size_t bytes = 0;
 };
 
@@ -116,6 +120,10 @@ struct tsg_hotplan {
     std::vector<double> slot_tp0;  // time of the node's parent, -inf: none
     std::vector<int32_t> tl_parent;
     std::vector<double> tl_time, tl_last, tl_ins;
+    // every node: its last flush position (as slot_last0) over the whole
+    // sequence, and its id in the last tree's arrays (-1: not in it)
+    std::vector<double> node_last;
+    std::vector<int32_t> node_lid;
     int32_t max_slots = 0, max_path = 0, max_levels = 0;
     TsgHotDevice dev;
 };

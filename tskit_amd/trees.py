@@ -301,12 +301,19 @@ class LowLevelTreeSequence:
             opts |= STAT_NONCENTRED
         return opts
 
+    def _result_shape(self, num_windows, opts, P):
+        """(windows, P), or (windows, nodes, P) for node statistics
+        (_tskitmodule.c:6638-6660)."""
+        if opts & STAT_NODE:
+            return (num_windows, self.get_num_nodes(), P)
+        return (num_windows, P)
+
     def _one_way(self, fn, sample_set_sizes, sample_sets, windows, mode, span_normalise,
                  polarised):
         opts = self._options(mode, span_normalise, polarised)
         sizes, sets = _parse_sample_sets(sample_set_sizes, sample_sets)
         w = _parse_windows(windows)
-        out = np.zeros((len(w) - 1, len(sizes)), dtype=np.float64)
+        out = np.zeros(self._result_shape(len(w) - 1, opts, len(sizes)), dtype=np.float64)
         fn, h = self._route(fn)
         err = fn(h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
                  len(w) - 1, ptr(w, ctypes.c_double), opts, ptr(out, ctypes.c_double))
@@ -319,7 +326,7 @@ class LowLevelTreeSequence:
         sizes, sets = _parse_sample_sets(sample_set_sizes, sample_sets)
         w = _parse_windows(windows)
         idx = _parse_indexes(indexes, k)
-        out = np.zeros((len(w) - 1, idx.shape[0]), dtype=np.float64)
+        out = np.zeros(self._result_shape(len(w) - 1, opts, idx.shape[0]), dtype=np.float64)
         fn, h = self._route(fn)
         err = fn(h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
                  idx.shape[0], ptr(idx, ctypes.c_int32), len(w) - 1, ptr(w, ctypes.c_double),
