@@ -488,6 +488,41 @@ def measured_traffic(cfg_name):
     return t.get("traffic_bytes") if t.get("workload") == cfg_name else None
 
 
+def node_phase(ts, tc, sets, om, with_cpu, reps=3):
+    """Node-mode diversity of the K sets over the whole sequence (one value per
+    node and set): end-to-end time per call, its node kernels' time from HIP
+    events, and the largest relative error against the CPU."""
+    import torch
+
+    ll = ts.ll_tree_sequence
+    got = ts.diversity(sets, mode="node")  # first call uploads the node tables
+    torch.cuda.synchronize()
+    ll.set_profiling(True)
+    t = time.perf_counter()
+    for _ in range(reps):
+        got = ts.diversity(sets, mode="node")
+    torch.cuda.synchronize()
+    t_call = (time.perf_counter() - t) / reps
+    kern_ms, launches = ll.get_profile()
+    ll.set_profiling(False)
+    out = {"stat": "diversity", "mode": "node", "sets": len(sets), "shape": list(got.shape),
+           "call_ms": round(t_call * 1e3, 3),
+           "kernels_ms": round(kern_ms / max(launches, 1), 3)}
+    msg = (f"node: diversity of {len(sets)} sets at {got.shape[0]} nodes {t_call * 1e3:.1f} ms "
+           f"per call, node kernels {out['kernels_ms']:.2f} ms")
+    if with_cpu:
+        kind = "reference" if om.reference_available() else "port"
+        impl = om.Reference(tc) if kind == "reference" else om.Oracle(tc)
+        t = time.perf_counter()
+        exp = impl.stat("diversity", sets, None, None, "node")[0]
+        out["cpu_s"] = round(time.perf_counter() - t, 3)
+        out["cpu_kind"] = kind
+        out["max_rel_err"] = float(np.max(np.abs(got - exp)) / np.max(np.abs(exp)))
+        msg += f"; CPU ({kind}) {out['cpu_s']:.2f} s, max rel err {out['max_rel_err']:.2e}"
+    log(msg)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -602,6 +637,10 @@ def main():
         del src, dst
         log(f"copy calibration: 3 x {COPY_BYTES} bytes read and written")
 
+    node = None
+    if not args.small:
+        node = node_phase(ts, tc, sets, om, not args.no_cpu)
+
     dense = None
     if not args.no_dense and not args.small:
         dense = dense_phase(om, not args.no_cpu)
@@ -655,6 +694,7 @@ def main():
         "max_rel_err_vs_cpu": errs,
         "served": served,
         "dense": dense,
+        "node": node,
         "replicates": replicates,
         "setup_s": {"generate": round(t_gen, 2), "host_init": round(t_init, 2),
                     "upload": round(t_upload, 3)},

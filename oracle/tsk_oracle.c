@@ -7,6 +7,8 @@
  * Each function follows the reference file:line it cites.  It is pinned by
  *   - the reference C compiled from source (oracle/_ref, see Makefile) on
  *     generated tree sequences (tests/test_oracle.py),
+ *   - golden vectors the reference wrote (tests/golden/: the small workload's
+ *     statistics and AFS, and node statistics of node_stat),
  *   - the reference's own hand-computed fixtures (tests/golden/),
  *   - brute-force pairwise definitions on tiny trees (tests/test_oracle.py).
  */
