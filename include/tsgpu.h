@@ -90,11 +90,13 @@ int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t
     int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc, int32_t *mut_allele,
     int32_t *mut_pallele, int32_t *site_nalleles);
 /* tsk_treeseq_divergence_matrix (c/tskit/trees.h; trees.c:8902-9001) in
- * site mode for sample sets of one sample each: result = [W x n x n], the
- * number of sites in the window where two samples carry different alleles
- * (divided by the window span with TSG_STAT_SPAN_NORMALISE); sites with at
- * most one mutation; from the same int8 cross products as the relatedness
- * matrix. */
+ * site mode for disjoint sample sets (sample_set_sizes NULL: one sample
+ * each; sample_sets NULL: every sample): result = [W x K x K], per pair of
+ * sets the number of (site, sample pair) with different alleles in the
+ * window, over the pairs of the two sets, divided by the number of pairs
+ * (trees.c:8876-8899) and, with TSG_STAT_SPAN_NORMALISE, by the window span;
+ * sites with at most one mutation; from the same int8 cross products as the
+ * relatedness matrix. */
 int tsg_treeseq_divergence_matrix(tsg_treeseq_t *ts, uint64_t num_sample_sets,
     const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
     const double *windows, uint32_t options, double *result);

@@ -244,6 +244,8 @@ class Reference(_Base):
         lib.ref_genotype_matrix.argtypes = [_vp, ctypes.c_int, _vp]
         lib.ref_divergence_matrix.argtypes = (
             [_vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _vp, ctypes.c_uint32, _vp])
+        lib.ref_divergence_matrix_sets.argtypes = (
+            [_vp, ctypes.c_uint64, _vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint32, _vp])
         lib.ref_ts_load.argtypes = [ctypes.POINTER(_vp), ctypes.c_char_p]
         if path is None:
             super().__init__(tc)
@@ -301,6 +303,25 @@ class Reference(_Base):
         out = np.zeros((len(windows) - 1, n, n))
         err = self._lib.ref_divergence_matrix(self.h, n, _p(samples), len(windows) - 1,
                                               _p(windows), opts, _p(out))
+        if err != 0:
+            raise OracleError(err)
+        return out
+
+    def divergence_matrix_sets(self, sample_sets, windows=None, mode="site",
+                               span_normalise=True):
+        """tsk_treeseq_divergence_matrix over sample sets: [W, K, K]."""
+        sizes = np.array([len(s) for s in sample_sets], dtype=np.uint64)
+        flat = np.ascontiguousarray(np.concatenate(sample_sets).astype(np.int32))
+        if windows is None:
+            windows = np.array([0.0, self.tc.sequence_length])
+        windows = np.ascontiguousarray(windows, np.float64)
+        opts = {"site": SITE, "branch": BRANCH}[mode]
+        if span_normalise:
+            opts |= SPAN_NORMALISE
+        k = len(sizes)
+        out = np.zeros((len(windows) - 1, k, k))
+        err = self._lib.ref_divergence_matrix_sets(self.h, k, _p(sizes), _p(flat),
+                                                   len(windows) - 1, _p(windows), opts, _p(out))
         if err != 0:
             raise OracleError(err)
         return out

@@ -175,3 +175,10 @@ int ref_divergence_matrix(const ref_ts_t *r, uint64_t num_samples, const int32_t
     return tsk_treeseq_divergence_matrix(&r->ts, num_samples, NULL, samples, nw, windows,
         options, result);
 }
+
+int ref_divergence_matrix_sets(const ref_ts_t *r, uint64_t num_sets, const uint64_t *sizes,
+    const int32_t *sets, uint64_t nw, const double *windows, uint32_t options,
+    double *result) {
+    return tsk_treeseq_divergence_matrix(&r->ts, num_sets, sizes, sets, nw, windows, options,
+        result);
+}

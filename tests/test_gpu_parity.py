@@ -727,3 +727,36 @@ def test_node_mode_not_batched(small):
     b = ts.batch()
     with pytest.raises(Exception):
         b.diversity(cases.sample_sets(200, 2, 10, 1), mode="node")
+
+
+@pytest.mark.parametrize("windows", [None, "three"])
+def test_set_divergence_matrix(require_gpu, windows):
+    """Site divergence matrix between disjoint sample sets of several samples
+    (one a singleton): the pairwise matrix from the int8 cross products summed
+    per pair of sets and count-normalised, against the closed form on the
+    oracle's genotypes (pinned to the reference's tsk_treeseq_divergence_
+    matrix) and the reference itself when oracle/_ref is present."""
+    import test_oracle as to
+    from oracle import oracle as om
+
+    tc = cases.generated(300, 5e5, 17)
+    ts = tc.tree_sequence()
+    w = None if windows is None else np.array([0, 1e5, 3.5e5, 5e5])
+    wn = np.array([0, 5e5]) if w is None else w
+    rng = np.random.default_rng(9)
+    perm = rng.permutation(300).astype(np.int32)
+    sets = [perm[:40], perm[40:41], perm[41:140], perm[140:300]]
+    _say("set divergence matrix", windows)
+    got = ts.divergence_matrix(sample_sets=sets, windows=w)
+    G = Oracle(tc).genotypes().astype(np.int64)
+    exp = to._set_divergence(G, tc.site_position, wn, sets)
+    if w is None:
+        exp = exp[0]
+    assert got.shape == exp.shape
+    assert np.allclose(got, exp, rtol=1e-12, atol=0)
+    if om.reference_available():
+        ref = om.Reference(tc).divergence_matrix_sets(sets, wn, "site")
+        assert np.allclose(got, ref if w is not None else ref[0], rtol=1e-12, atol=0)
+    # a set's samples in another order give the same matrix
+    again = ts.divergence_matrix(sample_sets=[s[::-1] for s in sets], windows=w)
+    assert np.allclose(again, got, rtol=1e-13, atol=0)

@@ -435,3 +435,42 @@ def test_node_single_tree_identity():
             assert node.shape == (1, tc.num_nodes, node.shape[2])
             assert np.all(np.isfinite(node))
             assert np.allclose(np.einsum("wnm,n->wm", node, bl), branch, rtol=1e-13), name
+
+
+def _set_divergence(G, pos, w, sets):
+    """Closed form of the site divergence matrix between disjoint sample sets
+    for 0/1 genotypes: per window the count of differing genotype pairs over
+    A x B (a pair within one set counted in both orders), divided by |A||B|,
+    or |A|(|A|-1) on the diagonal when not zero, and by the window span."""
+    out = []
+    for j in range(len(w) - 1):
+        g = G[(pos >= w[j]) & (pos < w[j + 1])]
+        C = g.T @ g
+        a = np.diag(C)
+        P = (a[:, None] + a[None, :] - 2 * C).astype(np.float64)
+        K = len(sets)
+        D = np.zeros((K, K))
+        for x in range(K):
+            for y in range(K):
+                v = P[np.ix_(sets[x], sets[y])].sum()
+                na, nb = len(sets[x]), len(sets[y])
+                den = na * (na - 1) if x == y else na * nb
+                D[x, y] = v / den if den != 0 else v
+        out.append(D / (w[j + 1] - w[j]))
+    return np.array(out)
+
+
+@needs_ref
+def test_set_divergence_matrix_closed_form_vs_reference():
+    """Sample sets of several samples (and one singleton) in the reference's
+    tsk_treeseq_divergence_matrix: the closed form the GPU route computes."""
+    tc = cases.generated(60, 2e5, 43)
+    orc, ref = Oracle(tc), om.Reference(tc)
+    w = np.array([0, 7e4, 2e5])
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(60).astype(np.int32)
+    sets = [perm[:10], perm[10:11], perm[11:30], perm[30:52]]
+    exp = ref.divergence_matrix_sets(sets, w, "site", True)
+    G = orc.genotypes().astype(np.int64)
+    got = _set_divergence(G, tc.site_position, w, sets)
+    assert np.allclose(got, exp, rtol=1e-13, atol=0)

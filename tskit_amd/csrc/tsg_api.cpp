@@ -1760,10 +1760,16 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
         }
         win.assign(windows, windows + nw + 1);
     }
-    // sample sets (trees.c:8833-8873): singletons only on this route
-    std::vector<int32_t> sel;
+    // sample sets (trees.c:8833-8873): disjoint sets of samples; the
+    // relatedness matrix takes singletons only.  sel lists the sets' samples
+    // in order, set k holding sel[set_off[k] .. set_off[k + 1]).
+    std::vector<int32_t> sel, set_off{0};
+    bool singletons = true;
     if (sample_sets == nullptr) {
-        for (int32_t j = 0; j < ts->num_samples; j++) sel.push_back(j);
+        for (int32_t j = 0; j < ts->num_samples; j++) {
+            sel.push_back(j);
+            set_off.push_back(j + 1);
+        }
     } else {
         std::vector<char> seen(ts->num_samples, 0);
         uint64_t i = 0;
@@ -1776,18 +1782,23 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
                 if (si == TSG_NULL) return TSG_ERR_BAD_SAMPLES;
                 if (seen[si]) return TSG_ERR_DUPLICATE_SAMPLE;
                 seen[si] = 1;
-                if (sz == 1) sel.push_back(si);
+                sel.push_back(si);
             }
-            if (sz != 1) return TSG_ERR_UNSUPPORTED_INPUT;
+            singletons &= (sz == 1);
+            set_off.push_back((int32_t) sel.size());
         }
+        if (!singletons && !divergence) return TSG_ERR_UNSUPPORTED_INPUT;
     }
+    const int nsets = (int) set_off.size() - 1;
     if (branch) return TSG_ERR_UNSUPPORTED_STAT_MODE;  // the branch matrix is not on this route
     for (int32_t s = 0; s < ts->num_sites; s++) {
         if (ts->site_mut0[s + 1] - ts->site_mut0[s] > 1) return TSG_ERR_NOT_BIALLELIC;
     }
     const int W = (int) win.size() - 1;
     const int n = (int) sel.size();
-    if (n == 0) return 0;
+    if (n == 0 || nsets == 0) return 0;
+    // grids: windows on y, set pairs on x
+    if (W > 65535 || (int64_t) nsets * nsets > INT32_MAX) return TSG_ERR_UNSUPPORTED_INPUT;
     const int ns = ts->num_samples;
     const int n_pad = ((ns + 127) / 128) * 128;
     const int NW = (ns + 31) / 32;
@@ -1846,6 +1857,12 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
         (ret = ensure(ws->rowsum, (size_t) (W * (int64_t) n + W) * 8)) ||
         (ret = ensure(ws->kout, (size_t) W * n * (size_t) n * 8))) {
         return ret;
+    }
+    if (!singletons) {
+        if ((ret = ensure(ws->out, (size_t) W * nsets * (size_t) nsets * 8)) ||
+            (ret = upload(ws->ipar, set_off, s))) {
+            return ret;
+        }
     }
     const TsgDevice &d = ts->dev;
     const TsgHotDevice &hd = ts->hot.dev;
@@ -1908,8 +1925,18 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
                                                  (double *) ws->kout.p, s)))) {
         return ret;
     }
+    // sample sets of several samples: per pair of sets the sum over their
+    // samples' pairs, count-normalised (trees.c:8876-8899)
+    if (!singletons &&
+        (ret = hip_check(tsg_launch_set_divergence((const double *) ws->kout.p, n, W,
+                                                   (const int32_t *) ws->ipar.p, nsets,
+                                                   (double *) ws->out.p, s)))) {
+        return ret;
+    }
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);
-    if ((ret = hip_check(hipMemcpyAsync(result, ws->kout.p, (size_t) W * n * (size_t) n * 8,
+    const void *res_dev = singletons ? ws->kout.p : ws->out.p;
+    const int nr = singletons ? n : nsets;
+    if ((ret = hip_check(hipMemcpyAsync(result, res_dev, (size_t) W * nr * (size_t) nr * 8,
                                         hipMemcpyDeviceToHost, s)))) {
         return ret;
     }

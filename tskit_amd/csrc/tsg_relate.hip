@@ -367,6 +367,41 @@ __global__ void __launch_bounds__(256) k_divergence(const int32_t *C, const int3
     D[(int64_t) w * n * n + gid] = v * scale[w];
 }
 
+// Divergence between sample sets from the pairwise matrix D [W x n x n] of
+// their samples (set k = rows off[k] .. off[k + 1]): one workgroup per
+// (A, B, window) sums D over A x B in a fixed order, then divides by the
+// number of pairs, |A||B| (A != B) or |A|(|A| - 1) when that is not zero
+// (trees.c:8876-8899; the reference counts a pair within one set twice, as
+// D's two triangles do).
+__global__ void __launch_bounds__(256) k_set_divergence(const double *D, int n,
+                                                        const int32_t *off, int nsets,
+                                                        double *out) {
+    __shared__ double part[256];
+    const int A = blockIdx.x / nsets, B = blockIdx.x % nsets;
+    const int w = blockIdx.y;
+    const int a0 = off[A], a1 = off[A + 1], b0 = off[B], b1 = off[B + 1];
+    const int nb = b1 - b0;
+    const int64_t cnt = (int64_t) (a1 - a0) * nb;
+    const double *Dw = D + (int64_t) w * n * n;
+    double s = 0.0;
+    for (int64_t t = threadIdx.x; t < cnt; t += 256) {
+        const int i = a0 + (int) (t / nb), j = b0 + (int) (t % nb);
+        s += Dw[(int64_t) i * n + j];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int) threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double na = (double) (a1 - a0), nbd = (double) nb;
+        const double denom = (A == B) ? na * (na - 1) : na * nbd;
+        const double v = part[0];
+        out[((int64_t) w * nsets + A) * nsets + B] = (A == B && denom == 0) ? v : v / denom;
+    }
+}
+
 // grand totals: one thread per window (n row sums each)
 __global__ void k_total(int64_t *rowsum, int n, int W) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -437,6 +472,13 @@ hipError_t tsg_launch_divergence(const int32_t *C, const int32_t *sel, int n, in
     const int64_t nn = (int64_t) n * n;
     hipLaunchKernelGGL(k_divergence, dim3((unsigned) ((nn + 255) / 256), (unsigned) W), dim3(256),
                        0, s, C, sel, n, n_pad, scale, D);
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_set_divergence(const double *D, int n, int W, const int32_t *off, int nsets,
+                                     double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_divergence, dim3((unsigned) nsets * (unsigned) nsets, (unsigned) W),
+                       dim3(256), 0, s, D, n, off, nsets, out);
     return hipGetLastError();
 }
 

@@ -413,8 +413,8 @@ class LowLevelTreeSequence:
 
     def divergence_matrix(self, windows, sample_sets=None, sample_set_sizes=None, mode=None,
                           span_normalise=1):
-        """_tskitmodule.c divergence_matrix: [W, n, n] over sample sets of one
-        sample each (site mode, int8 matrix cores)."""
+        """_tskitmodule.c divergence_matrix: [W, K, K] over disjoint sample
+        sets (site mode, int8 matrix cores)."""
         opts = self._options(mode, span_normalise, False)
         w = _parse_windows(windows)
         if sample_sets is None:
@@ -885,10 +885,10 @@ class TreeSequence:
 
     def divergence_matrix(self, sample_sets=None, *, windows=None, num_threads=0, mode=None,
                           span_normalise=True):
-        """trees.py:8739-8828: mean pairwise divergence between sample sets,
-        [W, K, K] (or [K, K] without windows).  On this path: site mode and
-        sample sets of one sample each, from G^T G on the int8 matrix cores
-        (num_threads is accepted and ignored)."""
+        """trees.py:8739-8828: mean pairwise divergence between disjoint sample
+        sets, [W, K, K] (or [K, K] without windows).  On this path: site mode,
+        from G^T G on the int8 matrix cores, the pairs of each two sets summed
+        on the device (num_threads is accepted and ignored)."""
         drop = windows is None
         w = self.parse_windows(windows)
         if sample_sets is None:
