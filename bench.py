@@ -135,7 +135,8 @@ def served_phase(big_ts, count, seed):
            for n, L, sd in SERVE_SHAPES]
     tss.append(big_ts)
     log(f"serve: 8 resident tree sequences generated in {time.perf_counter() - t0:.1f}s")
-    forms = {"served": dict(max_inflight=4), "plain": dict(one_per_launch=True, max_inflight=1)}
+    forms = {"served": dict(max_inflight=4), "multi": dict(max_inflight=4, multi=True),
+             "plain": dict(one_per_launch=True, max_inflight=1)}
 
     def capacity(pool, form, n_req, sd):
         srv = serve.StatServer(pool, **forms[form])
@@ -153,17 +154,20 @@ def served_phase(big_ts, count, seed):
     for name, pool, n_req in [("mix8", tss, count), ("big", [big_ts], max(count // 3, 40))]:
         capacity(pool, "served", max(n_req // 4, 20), seed + 100)  # warm buffers
         qs, ls, resident = capacity(pool, "served", n_req, seed)
+        qm, lm, _ = capacity(pool, "multi", n_req, seed)
         qp, lp, _ = capacity(pool, "plain", n_req, seed)
-        log(f"serve {name}: capacity served {qs:.1f} q/s ({ls} launches), plain {qp:.1f} q/s "
-            f"({lp} launches): {qs / qp:.2f}x")
+        log(f"serve {name}: capacity served {qs:.1f} q/s ({ls} launches), multi {qm:.1f} q/s "
+            f"({lm} jobs), plain {qp:.1f} q/s ({lp} launches): {qs / qp:.2f}x, "
+            f"{qm / qp:.2f}x")
         res = {"requests": n_req, "resident_bytes": resident, "qps_served": qs,
                "qps_plain": qp, "speedup": qs / qp, "launches_served": ls,
-               "launches_plain": lp}
+               "launches_plain": lp, "qps_multi": qm, "speedup_multi": qm / qp,
+               "jobs_multi": lm}
         offered = 0.5 * qp
         rng = np.random.default_rng(seed + 1)
         arrivals = np.cumsum(rng.exponential(1.0 / offered, n_req))
         res["offered_qps"] = offered
-        for form in ["served", "plain"]:
+        for form in ["served", "multi", "plain"]:
             srv = serve.StatServer(pool, **forms[form])
             reqs = serve.random_requests(pool, n_req, seed + 2)
             wall = srv.run_open_loop(reqs, arrivals)

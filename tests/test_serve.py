@@ -27,11 +27,14 @@ def resident(require_gpu):
     return tcs, [tc.tree_sequence() for tc in tcs]
 
 
-def test_mixed_equals_alone(resident):
+@pytest.mark.parametrize("multi", [False, True])
+def test_mixed_equals_alone(resident, multi):
     tcs, tss = resident
     reqs = serve.random_requests(tss, 60, seed=5, max_sets=8, max_set_size=60, max_windows=12)
     alone = serve.random_requests(tss, 60, seed=5, max_sets=8, max_set_size=60, max_windows=12)
-    mixed = serve.StatServer(tss, max_inflight=4)  # fused, concurrent launches
+    # fused, concurrent launches; with multi, one sweep launch per mode over
+    # several tree sequences
+    mixed = serve.StatServer(tss, max_inflight=4, multi=multi)
     for r in reqs:
         mixed.submit(r)
     done = mixed.step()
@@ -72,10 +75,11 @@ def test_served_results_match_oracle(resident):
         assert np.max(np.abs(got - exp)[fin] / np.broadcast_to(scale, exp.shape)[fin]) <= 1e-6
 
 
-def test_open_loop_latencies(resident):
+@pytest.mark.parametrize("multi", [False, True])
+def test_open_loop_latencies(resident, multi):
     tcs, tss = resident
     reqs = serve.random_requests(tss, 40, seed=7, max_sets=4, max_set_size=40, max_windows=6)
-    srv = serve.StatServer(tss)
+    srv = serve.StatServer(tss, multi=multi)
     arrivals = np.cumsum(np.random.default_rng(1).exponential(1 / 400.0, len(reqs)))
     wall = srv.run_open_loop(reqs, arrivals)
     st = serve.latency_stats(reqs)

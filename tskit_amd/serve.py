@@ -9,7 +9,10 @@ sample sets and ``max_outputs`` outputs, admitted while the launch's
 estimated device memory stays within ``launch_budget`` bytes; a request that
 does not fit starts the next launch.  Allele frequency spectra run one per
 launch (they are not fused).  Up to ``max_inflight`` launches run
-concurrently on their own streams.  ``one_per_launch=True, max_inflight=1``
+concurrently on their own streams.  With ``multi=True`` the fused launches
+of a planning round are dealt into at most ``max_inflight`` jobs, and each
+job runs all of its launches -- on any of the tree sequences -- as one sweep
+launch per mode (``run_batches``).  ``one_per_launch=True, max_inflight=1``
 serves every request alone, one launch at a time: the plain path the served
 loop is measured against.
 
@@ -82,7 +85,7 @@ class StatServer:
     while launches are in flight are admitted as soon as a slot frees."""
 
     def __init__(self, tree_sequences, launch_budget=2 << 30, max_columns=64, max_outputs=4096,
-                 one_per_launch=False, max_inflight=4):
+                 one_per_launch=False, max_inflight=4, multi=False):
         self.ts = list(tree_sequences)
         for ts in self.ts:
             ts.ll_tree_sequence.upload()
@@ -93,6 +96,7 @@ class StatServer:
         self.max_outputs = max_outputs
         self.one_per_launch = one_per_launch
         self.max_inflight = max(1, int(max_inflight))
+        self.multi = bool(multi) and not one_per_launch
         self.pool = (concurrent.futures.ThreadPoolExecutor(max_workers=self.max_inflight)
                      if self.max_inflight > 1 else None)
         self.queue = collections.deque()
@@ -127,9 +131,11 @@ class StatServer:
             launches.append(cur)
         return launches
 
-    def _plan(self, pending):
-        """Launches for the pending requests: fused groups per (tree
-        sequence, mode), each allele frequency spectrum alone."""
+    def _plan(self, pending, njobs=None):
+        """Jobs for the pending requests, each a list of launches: fused
+        groups per (tree sequence, mode), each allele frequency spectrum
+        alone; with ``multi`` the fused launches are dealt into at most
+        ``njobs`` jobs (default ``max_inflight``), else one launch a job."""
         groups = collections.OrderedDict()
         alone = []
         for r in pending:
@@ -137,40 +143,52 @@ class StatServer:
                 groups.setdefault((r.ts_id, r.mode), []).append(r)
             else:
                 alone.append([r])
-        work = []
+        fused = []
         for (tid, _), reqs in groups.items():
-            work.extend(self._pack(reqs, self.chunks[tid]))
-        work.extend(alone)
-        return work
+            fused.extend(self._pack(reqs, self.chunks[tid]))
+        if not self.multi:
+            return [[launch] for launch in fused] + [[a] for a in alone]
+        nj = max(1, min(njobs or self.max_inflight, len(fused)))
+        jobs = [fused[j::nj] for j in range(nj)]
+        return [j for j in jobs if j] + [[a] for a in alone]
 
-    def _run(self, launch):
+    def _run(self, job):
+        from .trees import run_batches
+
         t0 = time.perf_counter()
-        ts = self.ts[launch[0].ts_id]
-        if len(launch) == 1 and launch[0].stat not in FUSABLE:
-            r = launch[0]
+        reqs = [r for launch in job for r in launch]
+        if len(job) == 1 and len(job[0]) == 1 and job[0][0].stat not in FUSABLE:
+            r = job[0][0]
             r.t_start = t0
-            r.result = getattr(ts, r.stat)(**r.kwargs)
+            r.result = getattr(self.ts[r.ts_id], r.stat)(**r.kwargs)
         else:
-            b = ts.batch()
-            pend = [getattr(b, r.stat)(**r.kwargs) for r in launch]
-            b.run()
-            for r, p in zip(launch, pend):
+            batches, pend = [], []
+            for launch in job:
+                b = self.ts[launch[0].ts_id].batch()
+                pend.extend(getattr(b, r.stat)(**r.kwargs) for r in launch)
+                batches.append(b)
+            if len(batches) == 1:
+                batches[0].run()
+            else:
+                run_batches(batches)
+            for r, p in zip(reqs, pend):
                 r.t_start = t0
                 r.result = p.result()
         t1 = time.perf_counter()
-        for r in launch:
+        for r in reqs:
             r.t_done = t1
-        return launch
+        return reqs
 
-    def _start(self, launch):
+    def _start(self, job):
         self.launches += 1
-        for r in launch:
-            r.launch = self.launches
+        for launch in job:
+            for r in launch:
+                r.launch = self.launches
         if self.pool is None:
             f = concurrent.futures.Future()
-            f.set_result(self._run(launch))
+            f.set_result(self._run(job))
             return f
-        return self.pool.submit(self._run, launch)
+        return self.pool.submit(self._run, job)
 
     def step(self):
         """Serves every request queued now; returns the requests completed."""
@@ -178,7 +196,7 @@ class StatServer:
             return []
         pending = list(self.queue)
         self.queue.clear()
-        futures = [self._start(launch) for launch in self._plan(pending)]
+        futures = [self._start(job) for job in self._plan(pending)]
         done = []
         for f in futures:
             done.extend(f.result())
@@ -204,12 +222,13 @@ class StatServer:
                 i += 1
             free = self.max_inflight - len(inflight)
             if free > 0 and self.queue:
-                work = self._plan(list(self.queue))
+                work = self._plan(list(self.queue), free)
                 self.queue.clear()
-                for launch in work[free:]:  # no slot yet: wait in the queue
-                    self.queue.extend(launch)
-                for launch in work[:free]:
-                    inflight.add(self._start(launch))
+                for job in work[free:]:  # no slot yet: wait in the queue
+                    for launch in job:
+                        self.queue.extend(launch)
+                for job in work[:free]:
+                    inflight.add(self._start(job))
             if inflight:
                 wait = None if i >= n else max(0.0, arrivals[order[i]] - now)
                 if free > 0 and wait is not None:
