@@ -89,8 +89,11 @@ int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t
     int64_t *tr_off, int32_t *tr_nrem, int32_t *ev_child, int32_t *ev_parent, double *ev_bl,
     int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc, int32_t *mut_allele,
     int32_t *mut_pallele, int32_t *site_nalleles);
-/* tsk_treeseq_divergence_matrix (c/tskit/trees.h; trees.c:8902-9001) in
- * site mode for disjoint sample sets (sample_set_sizes NULL: one sample
+/* tsk_treeseq_divergence_matrix (c/tskit/trees.h; trees.c:8902-9001).
+ * Branch mode (trees.c:8579-8745): from the branch length every two samples
+ * share (one fused sweep of the noncentred relatedness of singleton sets),
+ * path length G_ii + G_jj - 2 G_ij, per pair of sets and count-normalised as
+ * below; at most 2^22 sample pairs.  Site mode for disjoint sample sets (sample_set_sizes NULL: one sample
  * each; sample_sets NULL: every sample): result = [W x K x K], per pair of
  * sets the number of (site, sample pair) with different alleles in the
  * window, over the pairs of the two sets, divided by the number of pairs

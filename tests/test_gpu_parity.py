@@ -760,3 +760,46 @@ def test_set_divergence_matrix(require_gpu, windows):
     # a set's samples in another order give the same matrix
     again = ts.divergence_matrix(sample_sets=[s[::-1] for s in sets], windows=w)
     assert np.allclose(again, got, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("case", ["generated", "gapped"])
+def test_branch_divergence_matrix(require_gpu, case):
+    """Branch divergence matrix (trees.c:8579-8745) between sample sets and of
+    singleton sets, over windows that span the sequence and windows that do
+    not: against the shared-branch-length closed form on the oracle (pinned
+    to the reference's MRCA route) and the reference itself when oracle/_ref
+    is present."""
+    import test_oracle as to
+    from oracle import oracle as om
+
+    tc = cases.generated(40, 2e5, 53) if case == "generated" else cases.gapped(40, 2e5, 53, 10)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    n = 40
+    L = tc.sequence_length
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(n).astype(np.int32)
+    sets = [perm[:9], perm[9:10], perm[10:25], perm[25:40]]
+    pairs = [(i, j) for i in range(n) for j in range(i, n)]
+    single = [np.array([i], np.int32) for i in range(n)]
+    fw = np.array([0, 2e4, 7e4, 1.5e5, L])
+    G = orc.stat("genetic_relatedness", single, pairs, fw, "branch", False, True, False)
+    exp_all = to._branch_divergence_from_shared(G, n, fw, sets)
+    exp_one = to._branch_divergence_from_shared(G, n, fw, [[i] for i in range(n)])
+    _say("branch divergence matrix", case)
+    got = ts.divergence_matrix(sample_sets=sets, windows=fw, mode="branch")
+    assert np.allclose(got, exp_all, rtol=1e-9, atol=0)
+    part = np.array([2e4, 7e4, 1.5e5])  # inner windows only
+    got = ts.divergence_matrix(sample_sets=sets, windows=part, mode="branch")
+    assert np.allclose(got, exp_all[1:3], rtol=1e-9, atol=0)
+    got = ts.divergence_matrix(windows=part, mode="branch")
+    assert np.allclose(got, exp_one[1:3], rtol=1e-9, atol=1e-300)
+    whole = ts.divergence_matrix(sample_sets=sets, mode="branch")
+    exp_whole = to._branch_divergence_from_shared(
+        orc.stat("genetic_relatedness", single, pairs, None, "branch", False, True, False),
+        n, np.array([0, L]), sets)[0]
+    assert np.allclose(whole, exp_whole, rtol=1e-9, atol=0)
+    if om.reference_available():
+        ref = om.Reference(tc)
+        assert np.allclose(ts.divergence_matrix(sample_sets=sets, windows=part, mode="branch"),
+                           ref.divergence_matrix_sets(sets, part, "branch"), rtol=1e-9, atol=0)

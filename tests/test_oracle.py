@@ -474,3 +474,55 @@ def test_set_divergence_matrix_closed_form_vs_reference():
     G = orc.genotypes().astype(np.int64)
     got = _set_divergence(G, tc.site_position, w, sets)
     assert np.allclose(got, exp, rtol=1e-13, atol=0)
+
+
+def _branch_divergence_from_shared(G, n, w, sets):
+    """Branch divergence matrix between disjoint sample sets from the shared
+    branch length of every two samples (G [W, n(n+1)/2] in (i <= j) order):
+    path length G_ii + G_jj - 2 G_ij, summed per pair of sets, divided by the
+    number of pairs and by the window span."""
+    iu = np.triu_indices(n)
+    out = []
+    for j in range(len(w) - 1):
+        S = np.zeros((n, n))
+        S[iu] = G[j]
+        S = S + np.triu(S, 1).T
+        d = np.diag(S)
+        P = d[:, None] + d[None, :] - 2 * S
+        np.fill_diagonal(P, 0.0)
+        K = len(sets)
+        D = np.zeros((K, K))
+        for x in range(K):
+            for y in range(K):
+                na, nb = len(sets[x]), len(sets[y])
+                den = na * (na - 1) if x == y else na * nb
+                v = P[np.ix_(sets[x], sets[y])].sum()
+                D[x, y] = v / den if den != 0 else v
+        out.append(D / (w[j + 1] - w[j]))
+    return np.array(out)
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["generated", "gapped"])
+def test_branch_divergence_matrix_shared_length_vs_reference(case):
+    """The reference's branch divergence matrix (per-tree MRCA path lengths,
+    or paths to the own roots across subtrees) equals G_ii + G_jj - 2 G_ij of
+    the branch-mode noncentred relatedness of singleton sets, per pair of
+    sets: the route the GPU takes."""
+    tc = cases.generated(30, 1e5, 47) if case == "generated" else cases.gapped(30, 1e5, 47, 8)
+    orc, ref = Oracle(tc), om.Reference(tc)
+    n = 30
+    w = np.array([0, 3e4, 1e5])
+    rng = np.random.default_rng(4)
+    perm = rng.permutation(n).astype(np.int32)
+    sets = [perm[:6], perm[6:7], perm[7:20], perm[20:30]]
+    pairs = [(i, j) for i in range(n) for j in range(i, n)]
+    single = [np.array([i], np.int32) for i in range(n)]
+    G = orc.stat("genetic_relatedness", single, pairs, w, "branch", False, True, False)
+    # sets hold node ids; samples 0..n-1 are nodes 0..n-1 here
+    got = _branch_divergence_from_shared(G, n, w, sets)
+    exp = ref.divergence_matrix_sets(sets, w, "branch", True)
+    assert np.allclose(got, exp, rtol=1e-11, atol=0)
+    exp1 = ref.divergence_matrix(np.arange(n, dtype=np.int32), w, "branch", True)
+    got1 = _branch_divergence_from_shared(G, n, w, [[i] for i in range(n)])
+    assert np.allclose(got1, exp1, rtol=1e-11, atol=1e-300)

@@ -1740,6 +1740,91 @@ int tsg_treeseq_genetic_relatedness_proportion(tsg_treeseq *ts, uint64_t K, cons
 // Site-mode matrices of singleton sample sets from C = G^T G (int8 matrix
 // cores): the centred relatedness K = J C J / span, or with `divergence` the
 // pairwise count of differing sites C_ii + C_jj - 2 C_ij / span.
+// tsk_treeseq_divergence_matrix in branch mode (trees.c:8579-8745, which
+// sums the path length between every two samples over the trees, from the
+// samples to their MRCA or, in different subtrees, to their own roots).
+// Here from the shared branch length
+//   G_ij = integral over the window of sum_u bl(u) [i below u] [j below u],
+// the branch-mode noncentred relatedness of singleton sets (summary x_i x_j,
+// trees.c:4729-4753 with weight 1), computed by the fused sweep in output
+// slices: the path length is G_ii + G_jj - 2 G_ij in every tree (G_ij = 0
+// for samples in different subtrees).  Windows that do not span the
+// sequence are padded at both ends for the sweep.  The pairs are then summed
+// per pair of sets and count-normalised (trees.c:8876-8899) on the host, an
+// O(W n^2) pass over the sweep's result.
+static int branch_divergence_matrix(tsg_treeseq *ts, const std::vector<int32_t> &sel,
+                                    const std::vector<int32_t> &set_off,
+                                    const std::vector<double> &win, bool span_normalise,
+                                    double *result) {
+    const int n = (int) sel.size();
+    const int nsets = (int) set_off.size() - 1;
+    const int W = (int) win.size() - 1;
+    const int64_t M = (int64_t) n * (n + 1) / 2;
+    if (M > (1 << 22)) return TSG_ERR_UNSUPPORTED_INPUT;
+    std::vector<double> fw;
+    int wfirst = 0;
+    if (win[0] > 0) {
+        fw.push_back(0.0);
+        wfirst = 1;
+    }
+    fw.insert(fw.end(), win.begin(), win.end());
+    if (win.back() < ts->L) fw.push_back(ts->L);
+    Query q;
+    q.mode = TSG_STAT_BRANCH;
+    q.span_normalise = false;
+    q.sets.resize(n);
+    for (int i = 0; i < n; i++) q.sets[i] = {sel[i]};
+    Block b;
+    b.stat = TSG_STAT_RELATEDNESS_NONCENTRED;
+    b.tuple = 2;
+    b.polarised = true;
+    b.idx.reserve((size_t) 2 * M);
+    for (int i = 0; i < n; i++) {
+        for (int j = i; j < n; j++) {
+            b.idx.push_back(i);
+            b.idx.push_back(j);
+        }
+    }
+    q.blocks.push_back(std::move(b));
+    q.windows = fw;
+    q.final_kind = TSG_FINAL_NONE;
+    q.P = (int) M;
+    std::vector<double> G((size_t) (fw.size() - 1) * M);
+    q.result = G.data();
+    std::vector<Query *> batch{&q};
+    int ret = run_batch(ts, batch);
+    if (ret != 0) return ret;
+    auto pair = [&](int i, int j) -> int64_t {  // (i <= j) in the output order above
+        return (int64_t) i * n - (int64_t) i * (i - 1) / 2 + (j - i);
+    };
+    std::vector<double> D((size_t) n * n);
+    for (int w = 0; w < W; w++) {
+        const double *g = G.data() + (size_t) (w + wfirst) * M;
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < n; j++) {
+                const int lo = std::min(i, j), hi = std::max(i, j);
+                D[(size_t) i * n + j] =
+                    (i == j) ? 0.0 : g[pair(lo, lo)] + g[pair(hi, hi)] - 2.0 * g[pair(lo, hi)];
+            }
+        }
+        const double span = win[w + 1] - win[w];
+        double *out = result + (size_t) w * nsets * nsets;
+        for (int A = 0; A < nsets; A++) {
+            for (int B = 0; B < nsets; B++) {
+                double v = 0.0;
+                for (int i = set_off[A]; i < set_off[A + 1]; i++) {
+                    for (int j = set_off[B]; j < set_off[B + 1]; j++) v += D[(size_t) i * n + j];
+                }
+                const double na = set_off[A + 1] - set_off[A], nb = set_off[B + 1] - set_off[B];
+                const double denom = (A == B) ? na * (na - 1) : na * nb;
+                if (!(A == B && denom == 0)) v /= denom;
+                out[(size_t) A * nsets + B] = span_normalise ? v / span : v;
+            }
+        }
+    }
+    return 0;
+}
+
 static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
                             const uint64_t *sample_set_sizes, const int32_t *sample_sets,
                             uint64_t nw, const double *windows, uint32_t options, bool divergence,
@@ -1790,7 +1875,15 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
         if (!singletons && !divergence) return TSG_ERR_UNSUPPORTED_INPUT;
     }
     const int nsets = (int) set_off.size() - 1;
-    if (branch) return TSG_ERR_UNSUPPORTED_STAT_MODE;  // the branch matrix is not on this route
+    if (branch) {
+        if (!divergence) return TSG_ERR_UNSUPPORTED_STAT_MODE;  // relatedness: site mode only
+        if (ts->time_uncalibrated && !(options & TSG_STAT_ALLOW_TIME_UNCALIBRATED)) {
+            return TSG_ERR_TIME_UNCALIBRATED;
+        }
+        if (sel.empty() || nsets == 0) return 0;
+        return branch_divergence_matrix(ts, sel, set_off, win,
+                                        (options & TSG_STAT_SPAN_NORMALISE) != 0, result);
+    }
     for (int32_t s = 0; s < ts->num_sites; s++) {
         if (ts->site_mut0[s + 1] - ts->site_mut0[s] > 1) return TSG_ERR_NOT_BIALLELIC;
     }
