@@ -803,3 +803,49 @@ def test_branch_divergence_matrix(require_gpu, case):
         ref = om.Reference(tc)
         assert np.allclose(ts.divergence_matrix(sample_sets=sets, windows=part, mode="branch"),
                            ref.divergence_matrix_sets(sets, part, "branch"), rtol=1e-9, atol=0)
+
+
+def _relatedness_from_divergence(D, n):
+    """trees.py:9080-9102 restated: self comparisons into the diagonal, then
+    double centring."""
+    B = D.copy()
+    np.fill_diagonal(B, np.diag(B) * (n - 1) / n)
+    K = B + np.mean(B)
+    y = np.mean(B, axis=0)
+    K -= y[:, None] + y[None, :]
+    return K / -2
+
+
+def test_relatedness_matrix_branch_and_sets(require_gpu):
+    """genetic_relatedness_matrix in branch mode and over sets of several
+    samples: the reference's composition of the divergence matrix, which is
+    checked against the closed forms pinned to the reference."""
+    import test_oracle as to
+
+    tc = cases.generated(40, 2e5, 53)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    n = 40
+    L = tc.sequence_length
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(n).astype(np.int32)
+    sets = [perm[:9], perm[9:10], perm[10:25], perm[25:40]]
+    sizes = np.array([len(s) for s in sets])
+    fw = np.array([0, 7e4, L])
+    pairs = [(i, j) for i in range(n) for j in range(i, n)]
+    single = [np.array([i], np.int32) for i in range(n)]
+    G = orc.stat("genetic_relatedness", single, pairs, fw, "branch", False, True, False)
+    Db = to._branch_divergence_from_shared(G, n, fw, sets)
+    got = ts.genetic_relatedness_matrix(sample_sets=sets, windows=fw, mode="branch")
+    exp = np.array([_relatedness_from_divergence(Db[j], sizes) for j in range(2)])
+    assert np.allclose(got, exp, rtol=1e-9, atol=1e-12 * np.max(np.abs(exp)))
+    Ds = to._set_divergence(orc.genotypes().astype(np.int64), tc.site_position, fw, sets)
+    got = ts.genetic_relatedness_matrix(sample_sets=sets, windows=fw, mode="site")
+    exp = np.array([_relatedness_from_divergence(Ds[j], sizes) for j in range(2)])
+    assert np.allclose(got, exp, rtol=1e-9, atol=1e-12 * np.max(np.abs(exp)))
+    got = ts.genetic_relatedness_matrix(mode="branch")
+    D1 = to._branch_divergence_from_shared(
+        orc.stat("genetic_relatedness", single, pairs, None, "branch", False, True, False),
+        n, np.array([0, L]), [[i] for i in range(n)])[0]
+    exp = _relatedness_from_divergence(D1, np.ones(n))
+    assert np.allclose(got, exp, rtol=1e-9, atol=1e-12 * np.max(np.abs(exp)))

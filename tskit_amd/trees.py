@@ -906,17 +906,37 @@ class TreeSequence:
 
     def genetic_relatedness_matrix(self, sample_sets=None, windows=None, mode="site",
                                    span_normalise=True):
-        """Site-mode relatedness matrix (trees.py:8978-9102) for singleton sample
-        sets: G^T G on the int8 matrix cores, FP64 double centring.  Returns
-        [n, n], or [W, n, n] with windows."""
+        """Relatedness matrix (trees.py:8978-9102).  Site mode with singleton
+        sample sets: G^T G on the int8 matrix cores, FP64 double centring on
+        the device.  Otherwise (branch mode, or sets of several samples), as
+        the reference: the divergence matrix of the sets, its diagonal
+        corrected to include self comparisons and double-centred
+        (trees.py:9080-9102).  Returns [K, K], or [W, K, K] with windows."""
+        single = sample_sets is None or all(
+            len(np.atleast_1d(np.asarray(s))) == 1 for s in sample_sets)
+        if mode not in (None, "site") or not single:
+            D = self.divergence_matrix(sample_sets, windows=windows, mode=mode,
+                                       span_normalise=span_normalise)
+            if sample_sets is None:
+                n = np.ones(self.num_samples)
+            else:
+                n = np.array([len(np.atleast_1d(np.asarray(x))) for x in sample_sets])
+
+            def _normalise(B):
+                if len(B) == 0:
+                    return B
+                B = B.copy()
+                np.fill_diagonal(B, np.diag(B) * (n - 1) / n)
+                K = B + np.mean(B)
+                y = np.mean(B, axis=0)
+                K -= y[:, np.newaxis] + y[np.newaxis, :]
+                return K / -2
+
+            if windows is None:
+                return _normalise(D)
+            return np.array([_normalise(D[j]) for j in range(D.shape[0])])
         if sample_sets is not None:
-            flat = []
-            for s in sample_sets:
-                s = np.atleast_1d(np.asarray(s))
-                if len(s) != 1:
-                    raise ValueError("only single-sample sets are supported on this path")
-                flat.append(int(s[0]))
-            sample_sets = flat
+            sample_sets = [int(np.atleast_1d(np.asarray(s))[0]) for s in sample_sets]
         drop = windows is None
         w = self.parse_windows(windows)
         out = self._ll_tree_sequence.genetic_relatedness_matrix(
