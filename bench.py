@@ -75,16 +75,16 @@ def algorithmic_bytes(E, N, S, n, queries, W):
 
 
 def run_gpu_step(ts, sets, windows, pairs, quads):
-    """One step: Fst (all pairs), Tajima's D and f4, fused per mode."""
-    out = {}
+    """One step: Fst (all pairs), Tajima's D and f4, fused per mode (one
+    batch holding both modes: one sweep per mode)."""
+    b = ts.batch()
+    pend = {}
     for mode in ("branch", "site"):
-        b = ts.batch()
-        p_fst = b.Fst(sets, indexes=pairs, windows=windows, mode=mode)
-        p_tjd = b.Tajimas_D(sets, windows=windows, mode=mode)
-        p_f4 = b.f4(sets, indexes=quads, windows=windows, mode=mode)
-        b.run()
-        out[mode] = (p_fst.result(), p_tjd.result(), p_f4.result())
-    return out
+        pend[mode] = (b.Fst(sets, indexes=pairs, windows=windows, mode=mode),
+                      b.Tajimas_D(sets, windows=windows, mode=mode),
+                      b.f4(sets, indexes=quads, windows=windows, mode=mode))
+    b.run()
+    return {mode: tuple(p.result() for p in ps) for mode, ps in pend.items()}
 
 
 def cpu_step(impl, sets, windows, pairs, quads, fst, tajimas_d):

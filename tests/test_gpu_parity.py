@@ -849,3 +849,4 @@ def test_relatedness_matrix_branch_and_sets(require_gpu):
         n, np.array([0, L]), [[i] for i in range(n)])[0]
     exp = _relatedness_from_divergence(D1, np.ones(n))
     assert np.allclose(got, exp, rtol=1e-9, atol=1e-12 * np.max(np.abs(exp)))
+

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -1043,6 +1044,7 @@ int run_node(tsg_treeseq *ts, Query &q) {
     const int W = (int) q.windows.size() - 1;
     const int64_t N = ts->num_nodes;
     if ((double) W * (double) N >= (double) INT32_MAX) return TSG_ERR_UNSUPPORTED_INPUT;
+    // per slot: last flush position and K counts
     const size_t shmem = (size_t) hp.max_slots * (8 + 4 * (size_t) K) + 64;
     Lease lease(ts);
     Workspace *ws = lease.ws;
@@ -2240,13 +2242,16 @@ int tsg_batch_run(tsg_batch *b) {
 }
 
 namespace {
-// The runs of tsg_batch_run_multi, queued on stream s0.  Runs on the hot
-// plan whose outputs fit one slice share one sweep launch per (mode, node
-// store, outputs per lane); the rest launch on their own.
+// The runs of tsg_batch_run_multi: prepared in parallel, each on its
+// workspace's stream; runs on the hot plan whose outputs fit one slice share
+// one sweep launch per (mode, node store, outputs per lane) on the first
+// run's stream, the rest launch on their own there; then each run finishes
+// on its own stream.
 int run_multi(tsg_batch **bs, uint64_t n, std::vector<std::unique_ptr<Lease>> &leases,
-              std::deque<std::vector<Query *>> &groups, std::deque<BatchRun> &runs,
-              hipStream_t &s0) {
+              std::deque<std::vector<Query *>> &groups, std::deque<BatchRun> &runs) {
     int ret;
+    std::vector<tsg_treeseq *> run_ts;
+    std::vector<Workspace *> run_ws;
     for (uint64_t i = 0; i < n; i++) {
         tsg_batch *b = bs[i];
         if (b == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
@@ -2263,13 +2268,40 @@ int run_multi(tsg_batch **bs, uint64_t n, std::vector<std::unique_ptr<Lease>> &l
             leases.push_back(std::make_unique<Lease>(b->ts));
             Workspace *ws = leases.back()->ws;
             if (ws == nullptr) return TSG_ERR_DEVICE;
-            if (s0 == nullptr) s0 = ws->stream;
             groups.push_back(std::move(g));
             runs.emplace_back();
-            if ((ret = batch_prepare(b->ts, groups.back(), ws, s0, runs.back()))) return ret;
+            run_ts.push_back(b->ts);
+            run_ws.push_back(ws);
         }
     }
-    if (runs.empty()) return 0;
+    const size_t R = runs.size();
+    if (R == 0) return 0;
+    // host work and uploads of the runs on up to 8 host threads, each run on
+    // its workspace's own stream; the shared launches wait for all of them
+    std::vector<int> rets(R, 0);
+    {
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next.fetch_add(1)) < R;) {
+                Workspace *ws = run_ws[i];
+                rets[i] = batch_prepare(run_ts[i], groups[i], ws, ws->stream, runs[i]);
+                if (rets[i] == 0) rets[i] = hip_check(hipEventRecord(ws->ev0, ws->stream));
+            }
+        };
+        std::vector<std::thread> pool;
+        const size_t T = std::min<size_t>(R, 8);
+        for (size_t t = 1; t < T; t++) pool.emplace_back(work);
+        work();
+        for (std::thread &th : pool) th.join();
+    }
+    for (size_t i = 0; i < R; i++) {
+        if (rets[i] != 0) return rets[i];
+    }
+    hipStream_t s0 = run_ws[0]->stream;
+    for (size_t i = 1; i < R; i++) {
+        if ((ret = hip_check(hipStreamWaitEvent(s0, run_ws[i]->ev0, 0)))) return ret;
+    }
+    for (BatchRun &r : runs) r.s = s0;
     std::map<std::tuple<int, int, int>, std::vector<BatchRun *>> shared;
     for (BatchRun &r : runs) {
         const bool hot = r.store == TSG_STORE_HOT || r.store == TSG_STORE_HOT16;
@@ -2317,7 +2349,13 @@ int run_multi(tsg_batch **bs, uint64_t n, std::vector<std::unique_ptr<Lease>> &l
             coff += nrun + 1;
         }
     }
-    for (BatchRun &r : runs) {
+    // the per-run combine, finals and copies on the runs' own streams, after
+    // the sweeps
+    if ((ret = hip_check(hipEventRecord(run_ws[0]->ev1, s0)))) return ret;
+    for (size_t i = 0; i < R; i++) {
+        BatchRun &r = runs[i];
+        r.s = run_ws[i]->stream;
+        if (i > 0 && (ret = hip_check(hipStreamWaitEvent(r.s, run_ws[0]->ev1, 0)))) return ret;
         if ((ret = batch_finish(r))) return ret;
     }
     return 0;
@@ -2329,12 +2367,12 @@ int tsg_batch_run_multi(tsg_batch **batches, uint64_t n) {
     std::vector<std::unique_ptr<Lease>> leases;
     std::deque<std::vector<Query *>> groups;
     std::deque<BatchRun> runs;
-    hipStream_t s0 = nullptr;
-    int ret = run_multi(batches, n, leases, groups, runs, s0);
+    int ret = run_multi(batches, n, leases, groups, runs);
     // the leases end (workspaces return to their pools) only once all work
-    // queued on s0 has finished
-    if (s0 != nullptr) {
-        const int sret = hip_check(hipStreamSynchronize(s0));
+    // queued on their streams has finished
+    for (auto &l : leases) {
+        if (l->ws == nullptr) continue;
+        const int sret = hip_check(hipStreamSynchronize(l->ws->stream));
         if (ret == 0) ret = sret;
     }
     return ret;
