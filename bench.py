@@ -641,16 +641,16 @@ def main():
         del src, dst
         log(f"copy calibration: 3 x {COPY_BYTES} bytes read and written")
 
-    node = None
-    if not args.small:
-        node = node_phase(ts, tc, sets, om, not args.no_cpu)
-
     dense = None
     if not args.no_dense and not args.small:
         dense = dense_phase(om, not args.no_cpu)
     replicates = None
     if not args.no_replicates and not args.small:
         replicates = replicate_phase(om, not args.no_cpu)
+
+    node = None
+    if not args.small:
+        node = node_phase(ts, tc, sets, om, not args.no_cpu)
 
     served, serve_tss = None, None
     if not args.no_serve and not args.small:
