@@ -30,6 +30,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--no-serve
        [--no-dense] [--no-cpu]
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -76,7 +77,8 @@ def algorithmic_bytes(E, N, S, n, queries, W):
 
 def run_gpu_step(ts, sets, windows, pairs, quads):
     """One step: Fst (all pairs), Tajima's D and f4, fused per mode (one
-    batch holding both modes: one sweep per mode)."""
+    batch holding both modes: the site sweep's host work overlaps the
+    branch sweep)."""
     b = ts.batch()
     pend = {}
     for mode in ("branch", "site"):
@@ -601,12 +603,17 @@ def main():
         run_gpu_step(ts, sets, windows, pairs, quads)
         log(f"warmup {i + 1}/{args.warmup} done (node store {ll.get_last_store()})")
     ll.set_profiling(True)
+    # Python's cyclic garbage collector is off while the steps are timed (as
+    # timeit does): its occasional pauses are tens of milliseconds
+    gc.collect()
+    gc.disable()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         gpu_out = run_gpu_step(ts, sets, windows, pairs, quads)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    gc.enable()
     sweep_ms, launches = ll.get_profile()
     per_mode = ll.get_profile_modes()
     ll.set_profiling(False)

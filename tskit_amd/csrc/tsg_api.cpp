@@ -2228,17 +2228,72 @@ uint64_t tsg_batch_get_num_queries(const tsg_batch *b) { return b->queries.size(
 // Runs every query of the batch: one fused sweep for the branch-mode
 // queries and one for the site-mode queries.  Results land in the buffers
 // given when the queries were added.
+namespace {
+// Both modes of a batch: each mode's run on its own leased workspace and
+// stream, the second mode's host work and uploads done while the first
+// mode's sweep runs; the second sweep starts when the first has ended (the
+// two do not share the device), then both finishes.
+int run_two_modes(tsg_treeseq *ts, std::vector<std::vector<Query *>> &groups,
+                  std::vector<std::unique_ptr<Lease>> &leases, std::vector<BatchRun> &runs) {
+    int ret;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = device_upload(ts))) return ret;
+    }
+    runs.resize(groups.size());
+    for (size_t i = 0; i < groups.size(); i++) {
+        leases.push_back(std::make_unique<Lease>(ts));
+        Workspace *ws = leases.back()->ws;
+        if (ws == nullptr) return TSG_ERR_DEVICE;
+        BatchRun &r = runs[i];
+        if ((ret = batch_prepare(ts, groups[i], ws, ws->stream, r))) return ret;
+        if (i > 0 && (ret = hip_check(hipStreamWaitEvent(r.s, leases[i - 1]->ws->ev1, 0)))) {
+            return ret;
+        }
+        if (ws->profile) (void) hipEventRecord(ws->ev0, r.s);
+        if ((ret = batch_sweep(r))) return ret;
+        if ((ret = hip_check(hipEventRecord(ws->ev1, r.s)))) return ret;
+    }
+    for (BatchRun &r : runs) {
+        if ((ret = batch_finish(r))) return ret;
+    }
+    return 0;
+}
+}  // namespace
+
 int tsg_batch_run(tsg_batch *b) {
+    std::vector<std::vector<Query *>> groups;
     for (uint32_t mode : {TSG_STAT_BRANCH, TSG_STAT_SITE}) {
         std::vector<Query *> group;
         for (auto &q : b->queries) {
             if (q->mode == mode) group.push_back(q.get());
         }
-        if (group.empty()) continue;
-        int ret = run_batch(b->ts, group);
-        if (ret != 0) return ret;
+        if (!group.empty()) groups.push_back(std::move(group));
     }
-    return 0;
+    if (groups.empty()) return 0;
+    if (groups.size() == 1) return run_batch(b->ts, groups[0]);
+    std::vector<std::unique_ptr<Lease>> leases;
+    std::vector<BatchRun> runs;
+    int ret = run_two_modes(b->ts, groups, leases, runs);
+    // every stream drains before its workspace returns to the pool
+    for (size_t i = 0; i < leases.size(); i++) {
+        Workspace *ws = leases[i]->ws;
+        if (ws == nullptr) continue;
+        const int sret = hip_check(hipStreamSynchronize(ws->stream));
+        if (ret == 0) ret = sret;
+        if (ret == 0 && ws->profile && i < runs.size()) {
+            float ms = 0;
+            (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
+            ws->sweep_ms += ms;
+            ws->sweep_launches += 1;
+            ws->mode_ms[runs[i].mode_id] += ms;
+            ws->mode_launches[runs[i].mode_id] += 1;
+        }
+    }
+    // last lease first: the pool hands the same workspace to the same mode
+    // next time, so its buffers are already sized for it
+    while (!leases.empty()) leases.pop_back();
+    return ret;
 }
 
 namespace {
