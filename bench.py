@@ -4,7 +4,8 @@ Workload (the scale configuration): a Hudson-coalescent tree sequence with
 1e5 haploid samples, L = 1e8, Ne = 1e4, r = mu = 1e-8 (seed 2), K = 8
 disjoint random sample sets of 1000 samples and 1000 equal windows.  One
 step computes, for branch and site mode, Fst over all 28 pairs, Tajima's D
-of every set and f4 over sets (0,1;2,3), as one fused batch per mode.
+of every set and f4 over sets (0,1;2,3), as one fused sweep per mode (one
+batch; the site sweep's host work overlaps the branch sweep).
 
 Metric: edge x sample-set updates per second.  Every statistic query of a
 step counts 2 * E * K_q updates (each edge inserted and removed once, for
@@ -16,6 +17,11 @@ samples over ~2e5 sites on the int8 matrix cores and the joint 2-set allele
 frequency spectrum (200 + 300 samples, 100 windows), timed, with errors
 against the CPU when the CPU leg runs.
 
+Replicate phase (unless --no-replicates): 256 generated tree sequences,
+plain, batched per replicate, and one sweep launch per mode over all of them
+(run_batches).  Node phase: node-mode diversity of the 8 sets on the scale
+tree sequence.
+
 Served metric (unless --no-serve): 8 resident tree sequences -- the one
 above, one of the 1000-sample shape (L = 1e6, seed 1) and six of 2e3..7e4
 samples (L = 2e6) -- and requests drawn uniformly (a tree sequence;
@@ -24,7 +30,8 @@ diversity, divergence, Fst, f2, f3, f4 or the AFS; 1-64 sample sets of
 1e5-sample sequence alone: queries/s of the served form (fused launches, up
 to 4 in flight) and of the plain form (one request per launch, one at a
 time) with every request waiting, and p50/p99 latency of both under Poisson
-arrivals at half the plain capacity.
+arrivals at half the plain capacity; also the multi-sequence form (each job
+one sweep launch per mode over several tree sequences).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--no-serve]
        [--no-dense] [--no-cpu]
@@ -532,8 +539,8 @@ def node_phase(ts, tc, sets, om, with_cpu, reps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--small", action="store_true", help="the 1000-sample workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--chunks", type=int, default=0, help="override the chunk count")
