@@ -55,6 +55,11 @@ struct Workspace {
     // one launch over several batches' sweeps: their argument blocks and
     // chunk prefixes
     DevBuf margs, mcpre;
+    // a batch's host arrays, packed into one pinned buffer and sent with one
+    // copy into one device buffer
+    DevBuf arena;
+    void *stage_h = nullptr;
+    size_t stage_bytes = 0;
     // relatedness-matrix route
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab, geno;
     int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
@@ -173,28 +178,89 @@ struct Lease {
 
 // Groups (sample, column) pairs by sample so each weighted sample walks its
 // root path once: seed_sample[i] has columns seed_col[seed_off[i]..seed_off[i+1]).
-int upload_seeds(Workspace *ws, const std::vector<int32_t> &pair_s,
-                 const std::vector<int32_t> &pair_c, int32_t num_samples, hipStream_t s,
-                 TsgSweepArgs &a) {
-    (void) num_samples;
-    // pairs by sample (columns of one sample in their original order): the
-    // CSR form (one entry per weighted sample) and the flat pair list
+struct Seeds {
+    std::vector<int32_t> seed_sample, seed_off, col, flat_s;
+};
+
+// pairs by sample (columns of one sample in their original order): the CSR
+// form (one entry per weighted sample) and the flat pair list
+void seeds_host(const std::vector<int32_t> &pair_s, const std::vector<int32_t> &pair_c,
+                Seeds &sd) {
     std::vector<int32_t> order(pair_s.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t x, int32_t y) { return pair_s[x] < pair_s[y]; });
-    std::vector<int32_t> seed_sample, seed_off{0}, col(order.size()), flat_s(order.size());
+    sd.seed_sample.clear();
+    sd.seed_off.assign(1, 0);
+    sd.col.resize(order.size());
+    sd.flat_s.resize(order.size());
     for (size_t i = 0; i < order.size(); i++) {
         const int32_t si = pair_s[order[i]];
-        col[i] = pair_c[order[i]];
-        flat_s[i] = si;
-        if (seed_sample.empty() || seed_sample.back() != si) {
-            if (!seed_sample.empty()) seed_off.push_back((int32_t) i);
-            seed_sample.push_back(si);
+        sd.col[i] = pair_c[order[i]];
+        sd.flat_s[i] = si;
+        if (sd.seed_sample.empty() || sd.seed_sample.back() != si) {
+            if (!sd.seed_sample.empty()) sd.seed_off.push_back((int32_t) i);
+            sd.seed_sample.push_back(si);
         }
     }
-    seed_off.push_back((int32_t) order.size());
-    if (seed_sample.empty()) seed_off.assign(1, 0);
+    sd.seed_off.push_back((int32_t) order.size());
+    if (sd.seed_sample.empty()) sd.seed_off.assign(1, 0);
+}
+
+// The host arrays of a batch at 256-byte offsets of one pinned buffer, sent
+// with one copy into the workspace's arena (a batch of many small arrays
+// otherwise costs one transfer each).  The workspace is leased: no earlier
+// copy from its staging buffer is still in flight.
+struct Pack {
+    struct Part {
+        const void *p;
+        size_t bytes, off;
+    };
+    std::vector<Part> parts;
+    size_t total = 0;
+    template <typename T>
+    int add(const std::vector<T> &v) {
+        const size_t off = (total + 255) & ~(size_t) 255;
+        const size_t bytes = v.size() * sizeof(T);
+        parts.push_back({v.data(), bytes, off});
+        total = off + std::max<size_t>(bytes, 8);
+        return (int) parts.size() - 1;
+    }
+    int send(Workspace *ws, hipStream_t s) const {
+        int ret = ensure(ws->arena, total);
+        if (ret != 0) return ret;
+        if (ws->stage_bytes < total) {
+            if (ws->stage_h != nullptr) (void) hipHostFree(ws->stage_h);
+            ws->stage_h = nullptr;
+            ws->stage_bytes = 0;
+            const size_t want = std::max(total + total / 2, (size_t) 1 << 16);
+            if (hipHostMalloc(&ws->stage_h, want) != hipSuccess) {
+                (void) hipGetLastError();
+                ws->stage_h = nullptr;
+                return TSG_ERR_NO_MEMORY;
+            }
+            ws->stage_bytes = want;
+        }
+        for (const Part &pt : parts) {
+            if (pt.bytes) std::memcpy((char *) ws->stage_h + pt.off, pt.p, pt.bytes);
+        }
+        return hip_check(
+            hipMemcpyAsync(ws->arena.p, ws->stage_h, total, hipMemcpyHostToDevice, s));
+    }
+    template <typename T>
+    T *dev(const Workspace *ws, int i) const {
+        return reinterpret_cast<T *>((char *) ws->arena.p + parts[i].off);
+    }
+};
+
+int upload_seeds(Workspace *ws, const std::vector<int32_t> &pair_s,
+                 const std::vector<int32_t> &pair_c, int32_t num_samples, hipStream_t s,
+                 TsgSweepArgs &a) {
+    (void) num_samples;
+    Seeds sd;
+    seeds_host(pair_s, pair_c, sd);
+    const std::vector<int32_t> &seed_sample = sd.seed_sample, &seed_off = sd.seed_off;
+    const std::vector<int32_t> &col = sd.col, &flat_s = sd.flat_s;
     int ret;
     if ((ret = upload(ws->pair_s, seed_sample, s)) || (ret = upload(ws->pair_c, col, s)) ||
         (ret = upload(ws->seed_off, seed_off, s)) || (ret = upload(ws->seed_ps, flat_s, s))) {
@@ -359,6 +425,9 @@ struct BatchRun {
     int64_t lanecount = 0, outcount = 0;
     TsgCombineArgs ca{};
     std::vector<TsgFinalHost> fin;
+    const void *fin_dev = nullptr;
+    const int32_t *ipar_dev = nullptr;
+    const double *dpar_dev = nullptr;
 };
 
 // The fused state columns and outputs of a batch: one column per distinct
@@ -594,13 +663,17 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     }
     TsgSweepArgs &a = r.a;
     a = TsgSweepArgs{};
-    if ((ret = upload_seeds(ws, pair_s, pair_c, ts->num_samples, s, a)) ||
-        (ret = upload(ws->tot, col_total, s)) || (ret = upload(ws->outs, outs, s)) ||
-        (ret = upload(ws->queries, qs, s)) || (ret = upload(ws->windows, wins, s)) ||
-        (ret = upload(ws->cq_wfirst, cq_wfirst, s)) || (ret = upload(ws->cq_nwin, cq_nwin, s)) ||
-        (ret = upload(ws->cq_poff, cq_poff, s)) || (ret = upload(ws->rawbase, rawbase, s)) ||
-        (ret = upload(ws->lanebase, lanebase, s)) ||
-        (ret = ensure(ws->partial, (size_t) pcount * 8)) ||
+    // the host arrays go over in one copy at the end (pk); their device
+    // pointers are set then
+    Seeds sd;
+    seeds_host(pair_s, pair_c, sd);
+    Pack pk;
+    const int p_ss = pk.add(sd.seed_sample), p_so = pk.add(sd.seed_off);
+    const int p_col = pk.add(sd.col), p_fs = pk.add(sd.flat_s);
+    const int p_tot = pk.add(col_total), p_outs = pk.add(outs), p_qs = pk.add(qs);
+    const int p_win = pk.add(wins), p_wf = pk.add(cq_wfirst), p_nw = pk.add(cq_nwin);
+    const int p_po = pk.add(cq_poff), p_rb = pk.add(rawbase), p_lb = pk.add(lanebase);
+    if ((ret = ensure(ws->partial, (size_t) pcount * 8)) ||
         (ret = ensure(ws->raw, (size_t) rawcount * 8))) {
         return ret;
     }
@@ -681,13 +754,6 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     a.K = K;
     a.M = M;
     a.Q = Q;
-    a.col_total = (const int32_t *) ws->tot.p;
-    a.outs = (const TsgOut *) ws->outs.p;
-    a.queries = (const TsgQuery *) ws->queries.p;
-    a.windows = (const double *) ws->windows.p;
-    a.cq_wfirst = (const int32_t *) ws->cq_wfirst.p;
-    a.cq_nwin = (const int32_t *) ws->cq_nwin.p;
-    a.cq_poff = (const int64_t *) ws->cq_poff.p;
     a.partial = (double *) ws->partial.p;
     a.dR = (double *) ws->dR.p;
     a.flags = (noskip ? TSG_F_NOSKIP : 0u) | (any_unpolarised ? TSG_F_ANY_UNPOLARISED : 0u);
@@ -717,16 +783,9 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     ca.num_chunks = C;
     ca.Q = Q;
     ca.M = M;
-    ca.queries = a.queries;
-    ca.windows = a.windows;
-    ca.cq_wfirst = a.cq_wfirst;
-    ca.cq_nwin = a.cq_nwin;
-    ca.cq_poff = a.cq_poff;
     ca.partial = a.partial;
     ca.Rstart = (mode == TSG_STAT_BRANCH) ? (const double *) ws->Rstart.p : nullptr;
     ca.raw = (double *) ws->raw.p;
-    ca.rawbase = (const int64_t *) ws->rawbase.p;
-    ca.lanebase = (const int64_t *) ws->lanebase.p;
     ca.flags = 0;
     // finals (FP64 composites), set up now: they read only the combined raw
     // sums
@@ -755,10 +814,32 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
         outcount += (int64_t) f.W * f.P;
     }
     r.outcount = outcount;
-    if ((ret = upload(ws->fin, fin, s)) || (ret = upload(ws->ipar, ipar, s)) ||
-        (ret = upload(ws->dpar, dpar, s)) || (ret = ensure(ws->out, (size_t) outcount * 8))) {
-        return ret;
-    }
+    const int p_fin = pk.add(fin), p_ip = pk.add(ipar), p_dp = pk.add(dpar);
+    if ((ret = ensure(ws->out, (size_t) outcount * 8)) || (ret = pk.send(ws, s))) return ret;
+    a.npairs = (int32_t) sd.flat_s.size();
+    a.pair_sample = pk.dev<const int32_t>(ws, p_fs);
+    a.pair_col = pk.dev<const int32_t>(ws, p_col);
+    a.nseed = (int32_t) sd.seed_sample.size();
+    a.seed_sample = pk.dev<const int32_t>(ws, p_ss);
+    a.seed_col = pk.dev<const int32_t>(ws, p_col);
+    a.seed_off = pk.dev<const int32_t>(ws, p_so);
+    a.col_total = pk.dev<const int32_t>(ws, p_tot);
+    a.outs = pk.dev<const TsgOut>(ws, p_outs);
+    a.queries = pk.dev<const TsgQuery>(ws, p_qs);
+    a.windows = pk.dev<const double>(ws, p_win);
+    a.cq_wfirst = pk.dev<const int32_t>(ws, p_wf);
+    a.cq_nwin = pk.dev<const int32_t>(ws, p_nw);
+    a.cq_poff = pk.dev<const int64_t>(ws, p_po);
+    ca.queries = a.queries;
+    ca.windows = a.windows;
+    ca.cq_wfirst = a.cq_wfirst;
+    ca.cq_nwin = a.cq_nwin;
+    ca.cq_poff = a.cq_poff;
+    ca.rawbase = pk.dev<const int64_t>(ws, p_rb);
+    ca.lanebase = pk.dev<const int64_t>(ws, p_lb);
+    r.fin_dev = pk.dev<const void>(ws, p_fin);
+    r.ipar_dev = pk.dev<const int32_t>(ws, p_ip);
+    r.dpar_dev = pk.dev<const double>(ws, p_dp);
     return 0;
 }
 
@@ -798,9 +879,8 @@ int batch_finish(BatchRun &r) {
                                             s)))) {
         return ret;
     }
-    if ((ret = hip_check(tsg_launch_finals(ws->fin.p, r.Q, (const double *) ws->raw.p,
-                                           (const int32_t *) ws->ipar.p,
-                                           (const double *) ws->dpar.p, (double *) ws->out.p,
+    if ((ret = hip_check(tsg_launch_finals(r.fin_dev, r.Q, (const double *) ws->raw.p,
+                                           r.ipar_dev, r.dpar_dev, (double *) ws->out.p,
                                            r.outcount, s)))) {
         return ret;
     }
@@ -1145,10 +1225,11 @@ void free_one_workspace(Workspace *ws) {
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
                       &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
                       &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->geno, &ws->x0,
-                      &ws->r0part, &ws->bsum, &ws->margs, &ws->mcpre};
+                      &ws->r0part, &ws->bsum, &ws->margs, &ws->mcpre, &ws->arena};
     for (DevBuf *b : bufs) {
         if (b->p != nullptr) (void) hipFree(b->p);
     }
+    if (ws->stage_h != nullptr) (void) hipHostFree(ws->stage_h);
     if (ws->ev0) (void) hipEventDestroy(ws->ev0);
     if (ws->ev1) (void) hipEventDestroy(ws->ev1);
     if (ws->stream) (void) hipStreamDestroy(ws->stream);
