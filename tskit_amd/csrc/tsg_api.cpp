@@ -60,6 +60,9 @@ struct Workspace {
     DevBuf arena;
     void *stage_h = nullptr;
     size_t stage_bytes = 0;
+    // a batch's results, copied back in one transfer (pinned)
+    void *out_h = nullptr;
+    size_t out_h_bytes = 0;
     // relatedness-matrix route
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab, geno;
     int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
@@ -884,15 +887,33 @@ int batch_finish(BatchRun &r) {
                                            r.outcount, s)))) {
         return ret;
     }
+    // all results in one copy into the workspace's pinned buffer; each
+    // query's part goes to its result buffer after the stream has finished
+    // (batch_deliver)
+    const size_t bytes = (size_t) std::max<int64_t>(r.outcount, 1) * 8;
+    if (ws->out_h_bytes < bytes) {
+        if (ws->out_h != nullptr) (void) hipHostFree(ws->out_h);
+        ws->out_h = nullptr;
+        ws->out_h_bytes = 0;
+        const size_t want = std::max(bytes + bytes / 2, (size_t) 1 << 16);
+        if (hipHostMalloc(&ws->out_h, want) != hipSuccess) {
+            (void) hipGetLastError();
+            ws->out_h = nullptr;
+            return TSG_ERR_NO_MEMORY;
+        }
+        ws->out_h_bytes = want;
+    }
+    return hip_check(hipMemcpyAsync(ws->out_h, ws->out.p, (size_t) r.outcount * 8,
+                                    hipMemcpyDeviceToHost, s));
+}
+
+// After r.s has finished: each query's results from the pinned copy.
+void batch_deliver(const BatchRun &r) {
     for (int qi = 0; qi < r.Q; qi++) {
         const TsgFinalHost &f = r.fin[qi];
-        if ((ret = hip_check(hipMemcpyAsync((*r.batch)[qi]->result,
-                                            (double *) ws->out.p + f.outbase,
-                                            (size_t) f.W * f.P * 8, hipMemcpyDeviceToHost, s)))) {
-            return ret;
-        }
+        std::memcpy((*r.batch)[qi]->result, (const double *) r.ws->out_h + f.outbase,
+                    (size_t) f.W * f.P * 8);
     }
-    return 0;
 }
 
 // Runs a batch of queries (same tree sequence, same mode) in one sweep.
@@ -911,6 +932,7 @@ int run_batch(tsg_treeseq *ts, std::vector<Query *> &batch) {
     if ((ret = batch_sweep(r))) return ret;
     if (ws->profile) (void) hipEventRecord(ws->ev1, r.s);
     if ((ret = batch_finish(r)) || (ret = hip_check(hipStreamSynchronize(r.s)))) return ret;
+    batch_deliver(r);
     if (ws->profile) {
         float ms = 0;
         (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
@@ -1230,6 +1252,7 @@ void free_one_workspace(Workspace *ws) {
         if (b->p != nullptr) (void) hipFree(b->p);
     }
     if (ws->stage_h != nullptr) (void) hipHostFree(ws->stage_h);
+    if (ws->out_h != nullptr) (void) hipHostFree(ws->out_h);
     if (ws->ev0) (void) hipEventDestroy(ws->ev0);
     if (ws->ev1) (void) hipEventDestroy(ws->ev1);
     if (ws->stream) (void) hipStreamDestroy(ws->stream);
@@ -2371,6 +2394,9 @@ int tsg_batch_run(tsg_batch *b) {
             ws->mode_launches[runs[i].mode_id] += 1;
         }
     }
+    if (ret == 0) {
+        for (const BatchRun &r : runs) batch_deliver(r);
+    }
     // last lease first: the pool hands the same workspace to the same mode
     // next time, so its buffers are already sized for it
     while (!leases.empty()) leases.pop_back();
@@ -2510,6 +2536,9 @@ int tsg_batch_run_multi(tsg_batch **batches, uint64_t n) {
         if (l->ws == nullptr) continue;
         const int sret = hip_check(hipStreamSynchronize(l->ws->stream));
         if (ret == 0) ret = sret;
+    }
+    if (ret == 0) {
+        for (const BatchRun &r : runs) batch_deliver(r);
     }
     return ret;
 }
