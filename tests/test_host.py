@@ -324,3 +324,30 @@ def test_plan_bounds_config1():
     ts = tc.tree_sequence()
     plan = ts.ll_tree_sequence.get_plan()
     plan_emulator.check_plan_bounds(plan, ts.num_trees, ts.num_samples)
+
+
+def test_matrix_and_batch_errors(paper):
+    """Argument errors of the divergence matrix (trees.c:8902-8960, 8833-8873),
+    node statistics inside a batch, and run_batches: raised before any device
+    work, with the reference's numbers where it has them."""
+    import tskit_amd as tam
+
+    ll = paper.ll_tree_sequence
+    w = [0, 10]
+    assert _err(lambda: ll.divergence_matrix(w, mode="node")) == -909
+    assert _err(lambda: ll.divergence_matrix(w, sample_sets=[0, 0])) == -600
+    assert _err(lambda: ll.divergence_matrix(w, sample_sets=[0, 5])) == -601
+    assert _err(lambda: ll.divergence_matrix(w, sample_sets=[0, 100])) == -202
+    assert _err(lambda: ll.divergence_matrix(
+        w, sample_sets=[0, 1, 1, 2], sample_set_sizes=[2, 2])) == -600
+    assert _err(lambda: ll.divergence_matrix([0, 5, 5], mode="site")) == -901
+    b = paper.batch()
+    with pytest.raises(tam.LibraryError) as e:
+        b.diversity([[0, 1]], mode="node")
+    assert e.value.err == -909
+    assert tam.run_batches([]) == []
+    with pytest.raises(TypeError):
+        tam.run_batches([object()])
+    lib = paper.ll_tree_sequence._lib
+    assert lib.tsg_batch_run_multi(None, 0) == 0
+    assert lib.tsg_batch_run_multi(None, 1) == -4
