@@ -147,7 +147,13 @@ int tsg_treeseq_get_profile(const tsg_treeseq_t *ts, double *sweep_ms, uint64_t 
 /* the same split by mode: [0] branch sweeps, [1] site sweeps */
 int tsg_treeseq_get_profile_modes(const tsg_treeseq_t *ts, double *ms, uint64_t *launches);
 
-/* ---- one-way statistics ------------------------------------------------ */
+/* ---- statistics ----------------------------------------------------------
+ * Every statistics call takes the arguments of its reference counterpart.
+ * options: TSG_STAT_SITE / TSG_STAT_BRANCH / TSG_STAT_NODE (one of them;
+ * none = site), TSG_STAT_POLARISED, TSG_STAT_SPAN_NORMALISE.  result is
+ * [W x P], or [W x num_nodes x P] in node mode (trees.c:1788-1918, on the
+ * hot plan; not in a batch).
+ * ---- one-way statistics ------------------------------------------------ */
 /* tsk_treeseq_diversity (trees.h:1103, trees.c:3950-3958) */
 int tsg_treeseq_diversity(tsg_treeseq_t *ts, uint64_t num_sample_sets,
     const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
