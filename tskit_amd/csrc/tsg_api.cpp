@@ -329,17 +329,20 @@ int parse_sample_sets(const tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, 
         }
     }
     out.assign(K, {});
-    std::vector<int32_t> sorted;
+    // a sample twice in one set: marks in a per-sample table, cleared after
+    // each set
+    std::vector<uint8_t> seen(ts->num_samples, 0);
     j = 0;
     for (uint64_t k = 0; k < K; k++) {
         out[k].reserve(sizes[k]);
         for (uint64_t l = 0; l < sizes[k]; l++) out[k].push_back(ts->sample_index_map[sets[j++]]);
-        // a sample twice in one set (sort, then compare neighbours)
-        sorted = out[k];
-        std::sort(sorted.begin(), sorted.end());
-        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
-            return TSG_ERR_DUPLICATE_SAMPLE;
+        bool dup = false;
+        for (int32_t si : out[k]) {
+            dup |= seen[si] != 0;
+            seen[si] = 1;
         }
+        for (int32_t si : out[k]) seen[si] = 0;
+        if (dup) return TSG_ERR_DUPLICATE_SAMPLE;
     }
     return 0;
 }
@@ -457,6 +460,9 @@ void fuse_queries(std::vector<Query *> &batch, Fused &f) {
     // across the batch.  A query with a centred relatedness block keeps its
     // own contiguous columns: its summary averages over them.
     std::map<std::vector<int32_t>, int32_t> shared_col;
+    // the same set given again in the same order (queries of one batch
+    // usually share their sets) is matched without sorting it again
+    std::map<std::vector<int32_t>, int32_t> seen_as_given;
     for (int qi = 0; qi < Q; qi++) {
         Query &q = *batch[qi];
         const int cbase = K;
@@ -466,14 +472,21 @@ void fuse_queries(std::vector<Query *> &batch, Fused &f) {
         for (size_t t = 0; t < q.sets.size(); t++) {
             const auto &s = q.sets[t];
             if (!own) {
+                auto given = seen_as_given.find(s);
+                if (given != seen_as_given.end()) {
+                    gcol[t] = given->second;
+                    continue;
+                }
                 std::vector<int32_t> key(s);
                 std::sort(key.begin(), key.end());
                 auto it = shared_col.find(key);
                 if (it != shared_col.end()) {
                     gcol[t] = it->second;
+                    seen_as_given.emplace(s, it->second);
                     continue;
                 }
                 shared_col.emplace(std::move(key), K);
+                seen_as_given.emplace(s, K);
             }
             for (int32_t si : s) {
                 pair_s.push_back(si);
