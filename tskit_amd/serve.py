@@ -116,7 +116,9 @@ class StatServer:
         launches, cur, cols, outs, mem = [], [], set(), 0, 0
         for r in reqs:
             k, m, nw = r.shape()
-            keys = {tuple(np.sort(np.asarray(s)).tolist()) for s in r.kwargs.get("sample_sets")}
+            # a set's identity: its sorted samples, as bytes (hashed in C)
+            keys = {np.sort(np.asarray(s, dtype=np.int64)).tobytes()
+                    for s in r.kwargs.get("sample_sets")}
             b = launch_bytes(k, m, nw, chunks)
             full = cur and (len(cols | keys) > self.max_columns or outs + m > self.max_outputs
                             or mem + b > self.launch_budget)
