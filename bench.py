@@ -28,7 +28,7 @@ samples (L = 2e6) -- and requests drawn uniformly (a tree sequence;
 diversity, divergence, Fst, f2, f3, f4 or the AFS; 1-64 sample sets of
 10-1e4 samples; 1-1e4 windows; a mode).  Reported, on the 8 and on the
 1e5-sample sequence alone: queries/s of the served form (fused launches, up
-to 4 in flight) and of the plain form (one request per launch, one at a
+to 8 in flight) and of the plain form (one request per launch, one at a
 time) with every request waiting, and p50/p99 latency of both under Poisson
 arrivals at half the plain capacity; also the multi-sequence form (each job
 one sweep launch per mode over several tree sequences).
@@ -133,7 +133,7 @@ SERVE_SHAPES = [(1000, 1e6, 1), (2000, 2e6, 51), (5000, 2e6, 52), (10000, 2e6, 5
 
 def served_phase(big_ts, count, seed):
     """Capacity and latency of the serving loop (see the module docstring):
-    the served form (fused launches, up to 4 in flight on their own streams)
+    the served form (fused launches, up to 8 in flight on their own streams)
     against the plain path (one request per launch, one launch at a time),
     on the 8 resident tree sequences and on the 1e5-sample one alone."""
     import tskit_amd as ta
@@ -144,7 +144,7 @@ def served_phase(big_ts, count, seed):
            for n, L, sd in SERVE_SHAPES]
     tss.append(big_ts)
     log(f"serve: 8 resident tree sequences generated in {time.perf_counter() - t0:.1f}s")
-    forms = {"served": dict(max_inflight=4), "multi": dict(max_inflight=4, multi=True),
+    forms = {"served": dict(max_inflight=8), "multi": dict(max_inflight=8, multi=True),
              "plain": dict(one_per_launch=True, max_inflight=1)}
 
     def capacity(pool, form, n_req, sd):

@@ -85,7 +85,7 @@ class StatServer:
     while launches are in flight are admitted as soon as a slot frees."""
 
     def __init__(self, tree_sequences, launch_budget=2 << 30, max_columns=64, max_outputs=4096,
-                 one_per_launch=False, max_inflight=4, multi=False):
+                 one_per_launch=False, max_inflight=8, multi=False):
         self.ts = list(tree_sequences)
         for ts in self.ts:
             ts.ll_tree_sequence.upload()
