@@ -112,19 +112,27 @@ def cpu_step(impl, sets, windows, pairs, quads, fst, tajimas_d):
 
 
 def max_rel_err(gpu, cpu):
-    worst = {}
+    """Per statistic and mode: (a) the largest |gpu - cpu| of an output over
+    the windows divided by that output's largest |cpu| (the normaliser of
+    sign-changing statistics: Fst and Tajima's D of sets drawn from one
+    population, and f4, scatter around 0), and (b) the largest elementwise
+    |gpu - cpu| / |cpu| over the values with |cpu| >= 1e-3 of that output's
+    largest (below that the relative error measures the cancellation of
+    1 - ratio, not the sweep)."""
+    per_output, elementwise = {}, {}
     for mode in gpu:
         for name, g, c in zip(("Fst", "Tajimas_D", "f4"), gpu[mode], cpu[mode]):
-            g = np.asarray(g, dtype=np.float64)
+            g = np.asarray(g, dtype=np.float64).reshape(np.shape(c))
             c = np.asarray(c, dtype=np.float64)
             fin = np.isfinite(c)
-            if name == "f4":
-                scale = np.max(np.abs(c[fin])) if np.any(fin) else 1.0
-            else:
-                scale = np.maximum(1.0, np.abs(c[fin]))
-            e = float(np.max(np.abs(g[fin] - c[fin]) / scale)) if np.any(fin) else 0.0
-            worst[f"{name}_{mode}"] = e
-    return worst
+            a = np.where(fin, np.abs(c), 0.0)
+            scale = np.maximum(np.max(a, axis=0, keepdims=True), 1e-300)
+            d = np.where(fin, np.abs(g - c), 0.0)
+            per_output[f"{name}_{mode}"] = float(np.max(d / scale))
+            big = fin & (a >= 1e-3 * scale)
+            elementwise[f"{name}_{mode}"] = (float(np.max(d[big] / a[big]))
+                                             if np.any(big) else 0.0)
+    return per_output, elementwise
 
 
 SERVE_SHAPES = [(1000, 1e6, 1), (2000, 2e6, 51), (5000, 2e6, 52), (10000, 2e6, 53),
@@ -604,6 +612,9 @@ def main():
     # queries per step (per mode): Fst, Tajima's D and f4, each over the K sets
     kq = [K, K, K]
     updates_per_step = 2 * sum(2 * E * k for k in kq)
+    # the fused sweep of a mode updates each of the K distinct state columns
+    # once per edge event: 2 modes x 2E x K column updates per step
+    column_updates_per_step = 2 * 2 * E * K
     queries_per_step = 2 * len(kq)
 
     for i in range(args.warmup):
@@ -671,7 +682,7 @@ def main():
         served, serve_tss = served_phase(ts, args.serve_requests, 7)
 
     cpu = None
-    errs = {}
+    errs, errs_elem = {}, {}
     if not args.no_cpu:
         kind = "reference" if om.reference_available() else "port"
         impl = om.Reference(tc) if kind == "reference" else om.Oracle(tc)
@@ -682,8 +693,9 @@ def main():
                "cores": 1, "kind": kind, "seconds": round(t_cpu, 3),
                "sample": "one full step (Fst over 28 pairs, Tajima's D, f4; branch and site) "
                          "on the same tree sequence, single thread"}
-        errs = max_rel_err(gpu_out, cpu_out)
-        log(f"cpu step {t_cpu:.2f}s ({kind}); max rel err {max(errs.values()):.2e}")
+        errs, errs_elem = max_rel_err(gpu_out, cpu_out)
+        log(f"cpu step {t_cpu:.2f}s ({kind}); max error / max|value| per output "
+            f"{max(errs.values()):.2e}, elementwise rel err {max(errs_elem.values()):.2e}")
         if served is not None:
             served["max_rel_err_vs_cpu"] = served_accuracy(serve_tss, om, kind)
             log(f"serve: max rel err of sampled served results vs CPU "
@@ -707,9 +719,16 @@ def main():
                    "chunks": hp["chunks"], "node_store": ["hbm", "lds", "lds-hot", "lds-hot16"][
                        ll.get_last_store()]},
         "queries_per_s": queries_per_step * args.steps / elapsed,
+        "column_updates_per_s": column_updates_per_step * args.steps / elapsed,
+        "value_note": "value counts 2 E K_q updates for each of the 3 queries of a mode (the "
+                      "reference's sweeps, one per query); column_updates_per_s counts each "
+                      "distinct state column once per edge event",
         "roofline": roofline,
         "cpu_baseline": cpu,
         "max_rel_err_vs_cpu": errs,
+        "max_rel_err_vs_cpu_note": "per output: max |gpu - cpu| over windows / max |cpu| "
+                                   "of that output",
+        "max_elementwise_rel_err_vs_cpu": errs_elem,
         "served": served,
         "dense": dense,
         "node": node,

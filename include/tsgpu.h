@@ -9,8 +9,11 @@
  * (tsk_id_t), options uint32_t (tsk_flags_t); option bits and error numbers
  * have the reference's values (c/tskit/trees.h:47-56, c/tskit/core.h).
  * Results are written to caller-owned host buffers, C order, W x M doubles
- * (AFS: W x TW x prod(n_k + 1)).  Calls on one handle are serialised by the
- * library; the handle is never mutated by a statistics call.
+ * (AFS: W x TW x prod(n_k + 1)).  Statistics calls on one handle may run
+ * concurrently from several host threads (each takes its own device
+ * workspace and stream) and never mutate it; tsg_treeseq_set_num_chunks and
+ * tsg_treeseq_free wait until the calls in flight on the handle have
+ * returned, and calls that start meanwhile wait for them.
  */
 #ifndef TSGPU_H
 #define TSGPU_H

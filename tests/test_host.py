@@ -351,3 +351,32 @@ def test_matrix_and_batch_errors(paper):
     lib = paper.ll_tree_sequence._lib
     assert lib.tsg_batch_run_multi(None, 0) == 0
     assert lib.tsg_batch_run_multi(None, 1) == -4
+
+
+def test_serve_request_normalise_and_pack():
+    """The serving loop's request bookkeeping on the host: no sample sets
+    means every sample, a flat list is one set; identical sets share a
+    state column when requests are packed."""
+    from tskit_amd import serve
+
+    class FakeTS:
+        def samples(self):
+            return np.arange(10, dtype=np.int32)
+
+    ts = FakeTS()
+    a = serve.Request(0, 0, "diversity", {"mode": "site"})
+    a.normalise(ts)
+    assert len(a.sets) == 1 and np.array_equal(a.sets[0], np.arange(10))
+    assert a.shape() == (1, 1, 1)
+    b = serve.Request(1, 0, "diversity", {"sample_sets": [3, 1, 2], "mode": "site"})
+    b.normalise(ts)
+    assert len(b.sets) == 1 and b.shape()[0] == 1
+    c = serve.Request(2, 0, "divergence", {"sample_sets": [[1, 2, 3], [4, 5]],
+                                           "indexes": [(0, 1)], "windows": [0, 5, 10]})
+    c.normalise(ts)
+    assert c.shape() == (2, 1, 2)
+    srv = serve.StatServer.__new__(serve.StatServer)
+    srv.max_columns, srv.max_outputs, srv.launch_budget = 64, 4096, 1 << 30
+    srv.one_per_launch = False
+    launches = srv._pack([a, b, c], 10)
+    assert len(launches) == 1 and len(launches[0]) == 3

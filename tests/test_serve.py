@@ -85,3 +85,30 @@ def test_open_loop_latencies(resident, multi):
     st = serve.latency_stats(reqs)
     assert wall > 0 and all(r.result is not None for r in reqs)
     assert 0 < st["p50_ms"] <= st["p99_ms"]
+
+
+def test_default_and_flat_sets_and_errors(resident):
+    """Requests with no sample sets (every sample) and with one flat set are
+    packed like any other; a request the library rejects (an index out of
+    range) gets its error, and the requests of the other launches their
+    results."""
+    tcs, tss = resident
+    w = np.linspace(0, tss[0].sequence_length, 4)
+    flat = tss[0].samples()[:50]
+    reqs = [serve.Request(0, 0, "diversity", {"windows": w, "mode": "branch"}),
+            serve.Request(1, 0, "diversity", {"sample_sets": flat, "windows": w, "mode": "site"}),
+            serve.Request(2, 1, "divergence", {"sample_sets": [tss[1].samples()[:9],
+                                                               tss[1].samples()[9:20]],
+                                               "indexes": [(0, 5)], "windows": None,
+                                               "mode": "site"}),
+            serve.Request(3, 2, "diversity", {"sample_sets": [tss[2].samples()[:30]],
+                                              "mode": "branch"})]
+    srv = serve.StatServer(tss, max_inflight=2)
+    for r in reqs:
+        srv.submit(r)
+    srv.step()
+    srv.close()
+    assert _same(reqs[0].result, tss[0].diversity(windows=w, mode="branch"))
+    assert _same(reqs[1].result, tss[0].diversity(flat, windows=w, mode="site"))
+    assert reqs[2].result is None and reqs[2].error is not None
+    assert _same(reqs[3].result, tss[2].diversity([tss[2].samples()[:30]], mode="branch"))

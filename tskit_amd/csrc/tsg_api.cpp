@@ -15,6 +15,7 @@
 #include <atomic>
 #include <climits>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -89,8 +90,56 @@ struct Resident {
 };
 
 // Guards device residency (uploads, plan builds), the workspace pools and
-// settings; never held while a query runs.
+// settings, and the per-handle call counts; never held while a query runs.
 std::mutex g_mutex;
+
+// Statistics calls on one handle may run concurrently from several host
+// threads (each leases its own workspace and stream).  A call holds a Use
+// for its whole duration; the calls that rebuild or free the handle's plans
+// (tsg_treeseq_set_num_chunks, tsg_treeseq_free) take it exclusively: they
+// wait until no call is in flight on the handle, and calls that start
+// meanwhile wait for them.
+std::condition_variable g_calls_cv;
+// handles this thread is inside (a call made from within another on the same
+// handle is counted once)
+thread_local std::vector<const tsg_treeseq *> t_using;
+
+struct Use {
+    const tsg_treeseq *ts = nullptr;
+    explicit Use(const tsg_treeseq *t) {
+        if (t == nullptr || std::find(t_using.begin(), t_using.end(), t) != t_using.end()) {
+            return;
+        }
+        std::unique_lock<std::mutex> lock(g_mutex);
+        g_calls_cv.wait(lock, [&] { return !t->exclusive; });
+        t->calls++;
+        ts = t;
+        t_using.push_back(t);
+    }
+    ~Use() {
+        if (ts == nullptr) return;
+        t_using.erase(std::find(t_using.begin(), t_using.end(), ts));
+        std::lock_guard<std::mutex> lock(g_mutex);
+        ts->calls--;
+        g_calls_cv.notify_all();
+    }
+    Use(const Use &) = delete;
+    Use &operator=(const Use &) = delete;
+};
+
+// Waits, holding g_mutex, until no call is in flight on the handle; new
+// calls on it wait until release().
+std::unique_lock<std::mutex> exclusive(tsg_treeseq *ts) {
+    std::unique_lock<std::mutex> lock(g_mutex);
+    g_calls_cv.wait(lock, [&] { return !ts->exclusive; });
+    ts->exclusive = true;
+    g_calls_cv.wait(lock, [&] { return ts->calls == 0; });
+    return lock;
+}
+void release(tsg_treeseq *ts) {
+    ts->exclusive = false;
+    g_calls_cv.notify_all();
+}
 
 int hip_check(hipError_t e) { return e == hipSuccess ? 0 : TSG_ERR_DEVICE; }
 
@@ -171,6 +220,9 @@ struct Lease {
     }
     ~Lease() {
         if (ws == nullptr) return;
+        // an error path may return with copies or kernels still queued: the
+        // next lease must find the stream (and its staging buffers) idle
+        (void) hipStreamSynchronize(ws->stream);
         std::lock_guard<std::mutex> lock(g_mutex);
         if (ws->last_store >= 0) res->last_store = ws->last_store;
         res->idle.push_back(ws);
@@ -964,6 +1016,7 @@ thread_local std::vector<std::unique_ptr<Query>> *g_capture = nullptr;
 int run_node(tsg_treeseq *ts, Query &q);
 
 int run_one(tsg_treeseq *ts, Query &q) {
+    Use use(ts);
     if (q.mode == TSG_STAT_NODE) {
         // node statistics are not fused into batches
         if (g_capture != nullptr) return TSG_ERR_UNSUPPORTED_STAT_MODE;
@@ -1447,9 +1500,12 @@ int tsg_treeseq_init(tsg_treeseq **out, double sequence_length, uint64_t num_nod
 
 int tsg_treeseq_free(tsg_treeseq *ts) {
     if (ts == nullptr) return 0;
-    free_workspace(ts);
-    tsg_device_free(&ts->dev);
-    tsg_hot_device_free(&ts->hot.dev);
+    {
+        std::unique_lock<std::mutex> lock = exclusive(ts);
+        free_workspace(ts);
+        tsg_device_free(&ts->dev);
+        tsg_hot_device_free(&ts->hot.dev);
+    }
     delete ts;
     return 0;
 }
@@ -1458,13 +1514,15 @@ int tsg_treeseq_free(tsg_treeseq *ts) {
 // full plan is rebuilt on its next use.
 int tsg_treeseq_set_num_chunks(tsg_treeseq *ts, uint32_t num_chunks) {
     if (ts == nullptr || num_chunks == 0) return TSG_ERR_BAD_PARAM_VALUE;
-    std::lock_guard<std::mutex> lock(g_mutex);
+    std::unique_lock<std::mutex> lock = exclusive(ts);
     free_workspace(ts);
     tsg_device_free(&ts->dev);
     tsg_hot_device_free(&ts->hot.dev);
     ts->full_chunks = (int32_t) std::min<uint32_t>(num_chunks, INT32_MAX);
     ts->full_built = false;
-    return tsg_build_hot_plan(ts, ts->full_chunks, false);
+    const int ret = tsg_build_hot_plan(ts, ts->full_chunks, false);
+    release(ts);
+    return ret;
 }
 
 // Testing hook: plan = 1 runs every sweep on the full plan, 2 on the hot
@@ -1472,6 +1530,7 @@ int tsg_treeseq_set_num_chunks(tsg_treeseq *ts, uint32_t num_chunks) {
 // its slots fit in LDS, with 16-bit counts when every set has <= 65535
 // samples).
 int tsg_treeseq_set_sweep_plan(tsg_treeseq *ts, int plan) {
+    Use use(ts);
     std::lock_guard<std::mutex> lock(g_mutex);
     int ret = device_upload(ts);
     if (ret != 0) return ret;
@@ -1490,6 +1549,7 @@ int tsg_treeseq_get_last_store(const tsg_treeseq *ts) {
 }
 
 int tsg_treeseq_upload(tsg_treeseq *ts) {
+    Use use(ts);
     std::lock_guard<std::mutex> lock(g_mutex);
     return device_upload(ts);
 }
@@ -1529,6 +1589,7 @@ int tsg_treeseq_get_index(const tsg_treeseq *ts, int32_t *I, int32_t *O) {
 // machines without a GPU.  sizes = {chunks, local nodes, events, trees,
 // sites, mutations}.
 int tsg_treeseq_get_plan_sizes(const tsg_treeseq *ts, uint64_t *sizes) {
+    Use use(ts);
     ts = full_plan(ts);
     sizes[0] = (uint64_t) ts->num_chunks;
     sizes[1] = ts->loc_parent0.size();
@@ -1545,6 +1606,7 @@ int tsg_treeseq_copy_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *
                           int32_t *ev_child, int32_t *ev_parent, double *ev_bl,
                           int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc,
                           int32_t *mut_allele, int32_t *mut_pallele, int32_t *site_nalleles) {
+    Use use(ts);
     ts = full_plan(ts);
     auto cp = [](auto *dst, const auto &v) { std::copy(v.begin(), v.end(), dst); };
     cp(chunk_tree0, ts->chunk_tree0);
@@ -1571,6 +1633,7 @@ int tsg_treeseq_copy_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64_t *
 uint64_t tsg_treeseq_get_path_size(const tsg_treeseq *ts) { return full_plan(ts)->path.size(); }
 
 int tsg_treeseq_copy_paths(const tsg_treeseq *ts, int64_t *ev_path0, int32_t *path) {
+    Use use(ts);
     ts = full_plan(ts);
     std::copy(ts->ev_path0.begin(), ts->ev_path0.end(), ev_path0);
     std::copy(ts->path.begin(), ts->path.end(), path);
@@ -1581,6 +1644,7 @@ int tsg_treeseq_copy_paths(const tsg_treeseq *ts, int64_t *ev_path0, int32_t *pa
 // slots, level entries, events, path entries, first-tree nodes, max slots
 // per chunk, max levels per chunk}.
 int tsg_treeseq_get_hot_plan_sizes(const tsg_treeseq *ts, uint64_t *sizes) {
+    Use use(ts);
     const tsg_hotplan &hp = ts->hot;
     sizes[0] = hp.usable ? 1 : 0;
     sizes[1] = (uint64_t) hp.num_chunks;
@@ -1601,6 +1665,7 @@ int tsg_treeseq_copy_hot_plan(const tsg_treeseq *ts, int32_t *chunk_tree0, int64
                               int32_t *tr_nrem, int32_t *ev_child, double *ev_bl,
                               int64_t *ev_path0, int32_t *path, int32_t *mut_slot,
                               int32_t *t0_parent, double *t0_bl) {
+    Use use(ts);
     const tsg_hotplan &hp = ts->hot;
     if (!hp.usable) return TSG_ERR_UNSUPPORTED_INPUT;
     auto cp = [](auto *dst, const auto &v) { std::copy(v.begin(), v.end(), dst); };
@@ -1627,6 +1692,7 @@ int tsg_treeseq_copy_hot_afs_plan(const tsg_treeseq *ts, int32_t *slot_node, int
                                   double *slot_last0, double *slot_ins0, double *slot_tp0,
                                   int32_t *tl_parent, double *tl_time, double *tl_last,
                                   double *tl_ins) {
+    Use use(ts);
     const tsg_hotplan &hp = ts->hot;
     if (!hp.usable) return TSG_ERR_UNSUPPORTED_INPUT;
     auto cp = [](auto *dst, const auto &v) { std::copy(v.begin(), v.end(), dst); };
@@ -1645,6 +1711,7 @@ int tsg_treeseq_copy_hot_afs_plan(const tsg_treeseq *ts, int32_t *slot_node, int
 // Profiling of the sweep kernel: HIP events on the library's stream around
 // every sweep launch.
 int tsg_treeseq_set_profiling(tsg_treeseq *ts, int enable) {
+    Use use(ts);
     int ret = tsg_treeseq_upload(ts);
     if (ret != 0) return ret;
     std::lock_guard<std::mutex> lock(g_mutex);
@@ -1660,6 +1727,7 @@ int tsg_treeseq_set_profiling(tsg_treeseq *ts, int enable) {
     return 0;
 }
 int tsg_treeseq_get_profile(const tsg_treeseq *ts, double *sweep_ms, uint64_t *launches) {
+    Use use(ts);
     std::lock_guard<std::mutex> lock(g_mutex);
     const Resident *res = res_of(ts);
     *sweep_ms = 0.0;
@@ -1674,6 +1742,7 @@ int tsg_treeseq_get_profile(const tsg_treeseq *ts, double *sweep_ms, uint64_t *l
 
 // The same per statistics mode: ms[0] / launches[0] branch sweeps, [1] site.
 int tsg_treeseq_get_profile_modes(const tsg_treeseq *ts, double *ms, uint64_t *launches) {
+    Use use(ts);
     std::lock_guard<std::mutex> lock(g_mutex);
     const Resident *res = res_of(ts);
     for (int m = 0; m < 2; m++) {
@@ -2167,6 +2236,7 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq *ts, uint64_t num_sample_
                                            const int32_t *sample_sets, uint64_t nw,
                                            const double *windows, uint32_t options,
                                            double *result) {
+    Use use(ts);
     return site_gram_matrix(ts, num_sample_sets, sample_set_sizes, sample_sets, nw, windows,
                             options, false, result);
 }
@@ -2178,6 +2248,7 @@ int tsg_treeseq_divergence_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
                                   const uint64_t *sample_set_sizes, const int32_t *sample_sets,
                                   uint64_t nw, const double *windows, uint32_t options,
                                   double *result) {
+    Use use(ts);
     return site_gram_matrix(ts, num_sample_sets, sample_set_sizes, sample_sets, nw, windows,
                             options, true, result);
 }
@@ -2231,6 +2302,7 @@ static void isolated_pairs(const tsg_treeseq *ts, std::vector<int32_t> &ms,
 // first appearance (0 = ancestral), -1 for isolated samples unless options
 // has TSG_ISOLATED_NOT_MISSING.  Runs on the hot plan.
 int tsg_treeseq_genotype_matrix(tsg_treeseq *ts, uint32_t options, int32_t *out) {
+    Use use(ts);
     if (ts->num_sites == 0) return 0;
     int ret;
     {
@@ -2379,6 +2451,7 @@ int run_two_modes(tsg_treeseq *ts, std::vector<std::vector<Query *>> &groups,
 }  // namespace
 
 int tsg_batch_run(tsg_batch *b) {
+    Use use(b->ts);
     std::vector<std::vector<Query *>> groups;
     for (uint32_t mode : {TSG_STAT_BRANCH, TSG_STAT_SITE}) {
         std::vector<Query *> group;
@@ -2539,6 +2612,12 @@ int run_multi(tsg_batch **bs, uint64_t n, std::vector<std::unique_ptr<Lease>> &l
 
 int tsg_batch_run_multi(tsg_batch **batches, uint64_t n) {
     if (n > 0 && batches == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+    // every tree sequence of the call stays in use until its leases end
+    std::vector<std::unique_ptr<Use>> uses;
+    for (uint64_t i = 0; i < n; i++) {
+        if (batches[i] == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+        uses.push_back(std::make_unique<Use>(batches[i]->ts));
+    }
     std::vector<std::unique_ptr<Lease>> leases;
     std::deque<std::vector<Query *>> groups;
     std::deque<BatchRun> runs;
@@ -2593,6 +2672,7 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
                                           const double *windows, uint64_t ntw,
                                           const double *time_windows, uint32_t options,
                                           double *result) {
+    Use use(ts);
     bool site = options & TSG_STAT_SITE, branch = options & TSG_STAT_BRANCH;
     if (options & TSG_STAT_NODE) return TSG_ERR_UNSUPPORTED_STAT_MODE;
     if (!(site || branch)) site = true;

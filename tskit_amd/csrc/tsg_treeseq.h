@@ -177,6 +177,11 @@ struct tsg_treeseq {
 
     TsgDevice dev;
     void *workspace = nullptr;            // per-handle device scratch (tsg_api.cpp)
+    // statistics calls in flight on the handle, and whether a call that
+    // rebuilds or frees its plans is waiting for them (guarded by the
+    // library's mutex, tsg_api.cpp)
+    mutable int32_t calls = 0;
+    mutable bool exclusive = false;
 };
 
 int tsg_build_plan(tsg_treeseq *ts, int32_t num_chunks);

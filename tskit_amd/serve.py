@@ -35,7 +35,7 @@ class Request:
     sequence ``ts_id``."""
 
     __slots__ = ("rid", "ts_id", "stat", "kwargs", "t_arrival", "t_start", "t_done", "result",
-                 "launch")
+                 "error", "launch", "sets")
 
     def __init__(self, rid, ts_id, stat, kwargs, t_arrival=0.0):
         self.rid = rid
@@ -46,15 +46,27 @@ class Request:
         self.t_start = None
         self.t_done = None
         self.result = None
+        self.error = None  # the exception the request's launch raised, if any
         self.launch = None
+        self.sets = None   # the sample sets as a list of arrays (set on submit)
 
     @property
     def mode(self):
         return self.kwargs.get("mode", "site")
 
+    def normalise(self, ts):
+        """The request's sample sets as a list of arrays, as the statistic
+        reads them: none means every sample, a flat list one set."""
+        sets = self.kwargs.get("sample_sets")
+        if sets is None:
+            sets = [ts.samples()]
+        elif len(sets) > 0 and np.ndim(sets[0]) == 0:
+            sets = [sets]
+        self.sets = [np.asarray(x, dtype=np.int64) for x in sets]
+
     def shape(self):
         """(sample sets, outputs, windows) of the request."""
-        sets = self.kwargs.get("sample_sets")
+        sets = self.sets if self.sets is not None else self.kwargs.get("sample_sets")
         k = len(sets) if sets is not None else 1
         idx = self.kwargs.get("indexes")
         if self.stat in ("diversity", "segregating_sites", "Y1", "Tajimas_D"):
@@ -108,6 +120,7 @@ class StatServer:
             self.pool = None
 
     def submit(self, req):
+        req.normalise(self.ts[req.ts_id])
         self.queue.append(req)
 
     def _pack(self, reqs, chunks):
@@ -117,8 +130,7 @@ class StatServer:
         for r in reqs:
             k, m, nw = r.shape()
             # a set's identity: its sorted samples, as bytes (hashed in C)
-            keys = {np.sort(np.asarray(s, dtype=np.int64)).tobytes()
-                    for s in r.kwargs.get("sample_sets")}
+            keys = {np.sort(s).tobytes() for s in r.sets}
             b = launch_bytes(k, m, nw, chunks)
             full = cur and (len(cols | keys) > self.max_columns or outs + m > self.max_outputs
                             or mem + b > self.launch_budget)
@@ -155,27 +167,35 @@ class StatServer:
         return [j for j in jobs if j] + [[a] for a in alone]
 
     def _run(self, job):
+        """Runs one job; an error (a request the library rejects) is stored
+        on the job's requests, whose results stay None, and does not stop
+        the other jobs of the round."""
         from .trees import run_batches
 
         t0 = time.perf_counter()
         reqs = [r for launch in job for r in launch]
-        if len(job) == 1 and len(job[0]) == 1 and job[0][0].stat not in FUSABLE:
-            r = job[0][0]
+        for r in reqs:
             r.t_start = t0
-            r.result = getattr(self.ts[r.ts_id], r.stat)(**r.kwargs)
-        else:
-            batches, pend = [], []
-            for launch in job:
-                b = self.ts[launch[0].ts_id].batch()
-                pend.extend(getattr(b, r.stat)(**r.kwargs) for r in launch)
-                batches.append(b)
-            if len(batches) == 1:
-                batches[0].run()
+        try:
+            if len(job) == 1 and len(job[0]) == 1 and job[0][0].stat not in FUSABLE:
+                r = job[0][0]
+                r.result = getattr(self.ts[r.ts_id], r.stat)(**r.kwargs)
             else:
-                run_batches(batches)
-            for r, p in zip(reqs, pend):
-                r.t_start = t0
-                r.result = p.result()
+                batches, pend = [], []
+                for launch in job:
+                    b = self.ts[launch[0].ts_id].batch()
+                    pend.extend(getattr(b, r.stat)(**r.kwargs) for r in launch)
+                    batches.append(b)
+                if len(batches) == 1:
+                    batches[0].run()
+                else:
+                    run_batches(batches)
+                for r, p in zip(reqs, pend):
+                    r.result = p.result()
+        except Exception as exc:  # noqa: BLE001 -- reported per request
+            for r in reqs:
+                r.result = None
+                r.error = exc
         t1 = time.perf_counter()
         for r in reqs:
             r.t_done = t1
