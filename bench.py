@@ -28,10 +28,17 @@ samples (L = 2e6) -- and requests drawn uniformly (a tree sequence;
 diversity, divergence, Fst, f2, f3, f4 or the AFS; 1-64 sample sets of
 10-1e4 samples; 1-1e4 windows; a mode).  Reported, on the 8 and on the
 1e5-sample sequence alone: queries/s of the served form (fused launches, up
-to 8 in flight) and of the plain form (one request per launch, one at a
-time) with every request waiting, and p50/p99 latency of both under Poisson
-arrivals at half the plain capacity; also the multi-sequence form (each job
-one sweep launch per mode over several tree sequences).
+to 8 in flight, admitted by device memory) and of the plain form (one
+request per launch, one at a time) with every request waiting, and p50/p99
+latency of both under Poisson arrivals at 0.25, 0.5, 1 and 2 times the plain
+capacity; also the capacity of the multi-sequence form (each job one sweep
+launch per mode over several tree sequences).
+
+CPU baseline (unless --no-cpu): the reference C library (oracle/_ref, built
+from the reference's sources) on the host's cores as window-partitioned
+processes -- each clips the tree sequence to its block of windows and runs
+the step's statistics there -- with the clip + init and statistics times
+reported separately, and the same step in one process beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--no-serve]
        [--no-dense] [--no-cpu]
@@ -96,42 +103,25 @@ def run_gpu_step(ts, sets, windows, pairs, quads):
     return {mode: tuple(p.result() for p in ps) for mode, ps in pend.items()}
 
 
-def cpu_step(impl, sets, windows, pairs, quads, fst, tajimas_d):
-    """The same step with the CPU implementation (the reference C library
-    when built, else the oracle), composites as the reference's Python
-    layer does them (two sweeps each)."""
-    out = {}
-    for mode in ("branch", "site"):
-        div = impl.stat("diversity", sets, None, windows, mode)
-        dvg = impl.stat("divergence", sets, pairs, windows, mode)
-        T = impl.stat("diversity", sets, None, windows, mode, False)
-        S = impl.stat("segregating_sites", sets, None, windows, mode, False)
-        f4 = impl.stat("f4", sets, quads, windows, mode)
-        out[mode] = (fst(div, dvg, pairs), tajimas_d(T, S, [len(s) for s in sets]), f4)
-    return out
-
-
 def max_rel_err(gpu, cpu):
-    """Per statistic and mode: (a) the largest |gpu - cpu| of an output over
-    the windows divided by that output's largest |cpu| (the normaliser of
-    sign-changing statistics: Fst and Tajima's D of sets drawn from one
-    population, and f4, scatter around 0), and (b) the largest elementwise
-    |gpu - cpu| / |cpu| over the values with |cpu| >= 1e-3 of that output's
-    largest (below that the relative error measures the cancellation of
-    1 - ratio, not the sweep)."""
+    """Per statistic and mode (keys "Fst_branch", ...): (a) the largest
+    |gpu - cpu| of an output over the windows divided by that output's
+    largest |cpu| (the normaliser of sign-changing statistics: Fst and
+    Tajima's D of sets drawn from one population, and f4, scatter around 0),
+    and (b) the largest elementwise |gpu - cpu| / |cpu| over the values with
+    |cpu| >= 1e-3 of that output's largest (below that the relative error
+    measures the cancellation of 1 - ratio, not the sweep)."""
     per_output, elementwise = {}, {}
-    for mode in gpu:
-        for name, g, c in zip(("Fst", "Tajimas_D", "f4"), gpu[mode], cpu[mode]):
-            g = np.asarray(g, dtype=np.float64).reshape(np.shape(c))
-            c = np.asarray(c, dtype=np.float64)
-            fin = np.isfinite(c)
-            a = np.where(fin, np.abs(c), 0.0)
-            scale = np.maximum(np.max(a, axis=0, keepdims=True), 1e-300)
-            d = np.where(fin, np.abs(g - c), 0.0)
-            per_output[f"{name}_{mode}"] = float(np.max(d / scale))
-            big = fin & (a >= 1e-3 * scale)
-            elementwise[f"{name}_{mode}"] = (float(np.max(d[big] / a[big]))
-                                             if np.any(big) else 0.0)
+    for key, c in cpu.items():
+        c = np.asarray(c, dtype=np.float64)
+        g = np.asarray(gpu[key], dtype=np.float64).reshape(c.shape)
+        fin = np.isfinite(c)
+        a = np.where(fin, np.abs(c), 0.0)
+        scale = np.maximum(np.max(a, axis=0, keepdims=True), 1e-300)
+        d = np.where(fin, np.abs(g - c), 0.0)
+        per_output[key] = float(np.max(d / scale))
+        big = fin & (a >= 1e-3 * scale)
+        elementwise[key] = float(np.max(d[big] / a[big])) if np.any(big) else 0.0
     return per_output, elementwise
 
 
@@ -180,21 +170,27 @@ def served_phase(big_ts, count, seed):
                "qps_plain": qp, "speedup": qs / qp, "launches_served": ls,
                "launches_plain": lp, "qps_multi": qm, "speedup_multi": qm / qp,
                "jobs_multi": lm}
-        offered = 0.5 * qp
-        rng = np.random.default_rng(seed + 1)
-        arrivals = np.cumsum(rng.exponential(1.0 / offered, n_req))
-        res["offered_qps"] = offered
-        for form in ["served", "multi", "plain"]:
-            srv = serve.StatServer(pool, **forms[form])
-            reqs = serve.random_requests(pool, n_req, seed + 2)
-            wall = srv.run_open_loop(reqs, arrivals)
-            srv.close()
-            st = serve.latency_stats(reqs)
-            res[f"p50_ms_{form}"] = st["p50_ms"]
-            res[f"p99_ms_{form}"] = st["p99_ms"]
-            res[f"achieved_qps_{form}"] = n_req / wall
-            log(f"serve {name}: offered {offered:.1f} q/s, {form}: p50 {st['p50_ms']:.2f} ms "
-                f"p99 {st['p99_ms']:.2f} ms")
+        # Poisson arrivals at a sweep of offered rates (fractions of the plain
+        # capacity): p50 / p99 latency and achieved rate of both forms
+        sweep = []
+        for frac in (0.25, 0.5, 1.0, 2.0):
+            offered = frac * qp
+            rng = np.random.default_rng(seed + 1)
+            arrivals = np.cumsum(rng.exponential(1.0 / offered, n_req))
+            point = {"offered_qps": offered, "offered_frac_of_plain": frac}
+            for form in ["served", "plain"]:
+                srv = serve.StatServer(pool, **forms[form])
+                reqs = serve.random_requests(pool, n_req, seed + 2)
+                wall = srv.run_open_loop(reqs, arrivals)
+                srv.close()
+                st = serve.latency_stats(reqs)
+                point[f"p50_ms_{form}"] = st["p50_ms"]
+                point[f"p99_ms_{form}"] = st["p99_ms"]
+                point[f"achieved_qps_{form}"] = n_req / wall
+                log(f"serve {name}: offered {offered:.1f} q/s, {form}: p50 {st['p50_ms']:.2f} ms "
+                    f"p99 {st['p99_ms']:.2f} ms, achieved {n_req / wall:.1f} q/s")
+            sweep.append(point)
+        res["latency_sweep"] = sweep
         out[name] = res
     return out, tss
 
@@ -684,18 +680,41 @@ def main():
     cpu = None
     errs, errs_elem = {}, {}
     if not args.no_cpu:
+        import functools
+
+        from oracle import partitioned
+
         kind = "reference" if om.reference_available() else "port"
-        impl = om.Reference(tc) if kind == "reference" else om.Oracle(tc)
+        step = functools.partial(partitioned.scale_step, sets=sets, pairs=pairs, quads=quads)
+        if kind == "reference":
+            # the reference as P window-partitioned processes on the host's
+            # cores (general_stat needs windows spanning the sequence:
+            # each process clips to its block and drops the outer windows)
+            cpu_out, t_cpu, t_init, t_stat, P = partitioned.run(tc, windows, step)
+        else:
+            t0 = time.perf_counter()
+            cpu_out = step(om.Oracle(tc), windows)
+            t_cpu, t_init, P = time.perf_counter() - t0, 0.0, 1
+            t_stat = t_cpu
         t0 = time.perf_counter()
-        cpu_out = cpu_step(impl, sets, windows, pairs, quads, om.fst, om.tajimas_d)
-        t_cpu = time.perf_counter() - t0
+        impl = om.Reference(tc) if kind == "reference" else om.Oracle(tc)
+        step(impl, windows)
+        t_one = time.perf_counter() - t0
         cpu = {"value": updates_per_step / t_cpu, "unit": "edge×sample-set updates/s",
-               "cores": 1, "kind": kind, "seconds": round(t_cpu, 3),
-               "sample": "one full step (Fst over 28 pairs, Tajima's D, f4; branch and site) "
-                         "on the same tree sequence, single thread"}
-        errs, errs_elem = max_rel_err(gpu_out, cpu_out)
-        log(f"cpu step {t_cpu:.2f}s ({kind}); max error / max|value| per output "
-            f"{max(errs.values()):.2e}, elementwise rel err {max(errs_elem.values()):.2e}")
+               "cores": P,
+               "kind": ("reference, window-partitioned" if kind == "reference" else "port"),
+               "seconds": round(t_cpu, 3), "clip_init_seconds_max": round(t_init, 3),
+               "stat_seconds_max": round(t_stat, 3), "single_thread_seconds": round(t_one, 3),
+               "sample": f"one full step (Fst over 28 pairs, Tajima's D, f4; branch and site) on "
+                         f"the same tree sequence: {P} processes, each the reference on its "
+                         f"block of windows (edges and sites clipped to it); wall time of "
+                         f"the parallel run"}
+        gpu_dict = {f"{name}_{mode}": v for mode, vals in gpu_out.items()
+                    for name, v in zip(("Fst", "Tajimas_D", "f4"), vals)}
+        errs, errs_elem = max_rel_err(gpu_dict, cpu_out)
+        log(f"cpu step {t_cpu:.2f}s wall on {P} processes (clip+init max {t_init:.2f}s, stats "
+            f"max {t_stat:.2f}s; one process {t_one:.2f}s, {kind}); max error / max|value| per "
+            f"output {max(errs.values()):.2e}, elementwise rel err {max(errs_elem.values()):.2e}")
         if served is not None:
             served["max_rel_err_vs_cpu"] = served_accuracy(serve_tss, om, kind)
             log(f"serve: max rel err of sampled served results vs CPU "

@@ -93,16 +93,19 @@ int tsg_treeseq_copy_plan(const tsg_treeseq_t *ts, int32_t *chunk_tree0, int64_t
     int32_t *tree_site0, int32_t *site_mut0, int32_t *mut_node_loc, int32_t *mut_allele,
     int32_t *mut_pallele, int32_t *site_nalleles);
 /* tsk_treeseq_divergence_matrix (c/tskit/trees.h; trees.c:8902-9001).
- * Branch mode (trees.c:8579-8745): from the branch length every two samples
- * share (one fused sweep of the noncentred relatedness of singleton sets),
- * path length G_ii + G_jj - 2 G_ij, per pair of sets and count-normalised as
- * below; at most 2^22 sample pairs.  Site mode for disjoint sample sets (sample_set_sizes NULL: one sample
+ * Branch mode (trees.c:8579-8745): from the branch length G_ij every two
+ * samples share (the relatedness vector of identity weights on the hot
+ * plan), path length G_ii + G_jj - 2 G_ij, per pair of sets and
+ * count-normalised as below.  Site mode for disjoint sample sets (sample_set_sizes NULL: one sample
  * each; sample_sets NULL: every sample): result = [W x K x K], per pair of
  * sets the number of (site, sample pair) with different alleles in the
  * window, over the pairs of the two sets, divided by the number of pairs
  * (trees.c:8876-8899) and, with TSG_STAT_SPAN_NORMALISE, by the window span;
- * sites with at most one mutation; from the same int8 cross products as the
- * relatedness matrix. */
+ * from the same int8 cross products as the relatedness matrix: per window
+ * P = 2 G1'G1 + G2'G2 (G1 the derived-allele rows of the sites with at most
+ * two alleles, G2 the any-derived row and one row per derived allele of the
+ * others), samples i and j differing at (P_ii + P_jj) / 2 - P_ij sites
+ * (trees.c:8684-8826 groups the alleles the same way). */
 int tsg_treeseq_divergence_matrix(tsg_treeseq_t *ts, uint64_t num_sample_sets,
     const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
     const double *windows, uint32_t options, double *result);
@@ -226,6 +229,20 @@ int tsg_treeseq_Fst(tsg_treeseq_t *ts, uint64_t num_sample_sets,
     const int32_t *index_tuples, uint64_t num_windows, const double *windows,
     uint32_t options, double *result);
 
+/* ---- relatedness vector -------------------------------------------------
+ * tsk_treeseq_genetic_relatedness_vector (trees.h; trees.c:10402-10816,
+ * entry 10783), same arguments in the same order (result before options):
+ * weights [num_samples x num_weights] (rows in sample order), result
+ * [num_windows x num_focal_nodes x num_weights] = per window the product of
+ * the branch relatedness matrix and the weights, centred (weights over the
+ * samples, result over the focal nodes) unless TSG_STAT_NONCENTRED.  Branch
+ * mode only (TSG_STAT_SITE / TSG_STAT_NODE: TSG_ERR_UNSUPPORTED_STAT_MODE);
+ * windows need not span the sequence.  Focal nodes must be samples here
+ * (otherwise TSG_ERR_UNSUPPORTED_INPUT). */
+int tsg_treeseq_genetic_relatedness_vector(tsg_treeseq_t *ts, uint64_t num_weights,
+    const double *weights, uint64_t num_windows, const double *windows,
+    uint64_t num_focal_nodes, const int32_t *focal_nodes, double *result, uint32_t options);
+
 /* ---- allele frequency spectrum ----------------------------------------- */
 /* tsk_treeseq_allele_frequency_spectrum (trees.h:1112, trees.c:3815-3928);
  * at most 16 sample sets. */
@@ -237,9 +254,10 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq_t *ts, uint64_t num_sample
 /* ---- relatedness matrix ---------------------------------------------------
  * genetic_relatedness_matrix(mode="site") of python/tskit/trees.py:8978-9102
  * (the divergence matrix of c/tskit/trees.c:8900-9001 double-centred) for
- * singleton sample sets (sample_sets NULL = every sample) and sites with at
- * most one mutation, computed as K = J (G^T G) J with the cross product on
- * int8 matrix cores (exact int32) and FP64 centring.  Windows need not span
+ * singleton sample sets (sample_sets NULL = every sample), computed as
+ * K = J P J / 2 with P the int8 matrix-core cross product of the divergence
+ * matrix above (exact int32; K = J G^T G J when every site has at most two
+ * alleles) and FP64 centring.  Windows need not span
  * the genome (as the reference's divergence matrix).  result: W x n x n. */
 int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq_t *ts, uint64_t num_sample_sets,
     const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,

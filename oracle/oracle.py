@@ -28,6 +28,7 @@ TUPLE = {0: 0, 1: 0, 2: 0, 3: 2, 4: 2, 5: 2, 6: 2, 7: 2, 8: 3, 9: 3, 10: 4}
 
 SITE, BRANCH, NODE = 1, 2, 4
 POLARISED, SPAN_NORMALISE, ALLOW_UNCAL = 1 << 10, 1 << 11, 1 << 12
+NONCENTRED = 1 << 14
 
 _vp = ctypes.c_void_p
 
@@ -113,6 +114,28 @@ class _Base:
             shape = (len(windows) - 1, len(self.tc.node_time), M)
         out = np.zeros(shape, dtype=np.float64)
         err = self._stat(sid, sizes, flat, t, M, idx, windows, opts, out)
+        if err != 0:
+            raise OracleError(err)
+        return out
+
+    def relatedness_vector(self, W, windows=None, mode="branch", span_normalise=True,
+                           centre=True, nodes=None):
+        """genetic_relatedness_vector (trees.c:10402-10816): W [n, K] weights of
+        the samples; result [nw, len(nodes), K] (nodes default: the samples)."""
+        W = np.ascontiguousarray(np.asarray(W, dtype=np.float64).reshape(len(W), -1))
+        if windows is None:
+            windows = np.array([0.0, self.tc.sequence_length])
+        windows = np.ascontiguousarray(windows, np.float64)
+        if nodes is None:
+            nodes = np.flatnonzero(self.tc.node_flags & 1)
+        nodes = np.ascontiguousarray(np.asarray(nodes, dtype=np.int32))
+        opts = {"site": SITE, "branch": BRANCH, "node": NODE}[mode]
+        if span_normalise:
+            opts |= SPAN_NORMALISE
+        if not centre:
+            opts |= NONCENTRED
+        out = np.zeros((len(windows) - 1, len(nodes), W.shape[1]))
+        err = self._relvec(W, windows, nodes, opts, out)
         if err != 0:
             raise OracleError(err)
         return out
@@ -210,6 +233,11 @@ class Oracle(_Base):
             self.h, sid, len(sizes), _p(sizes), _p(flat), t, M, _p(idx), len(windows) - 1,
             _p(windows), ctypes.c_uint32(opts), _p(out))
 
+    def _relvec(self, W, windows, nodes, opts, out):
+        return self._lib.oracle_relatedness_vector(
+            self.h, W.shape[1], _p(W), len(windows) - 1, _p(windows), len(nodes), _p(nodes),
+            ctypes.c_uint32(opts), _p(out))
+
     def _afs(self, sizes, flat, windows, time_windows, opts, out):
         return self._lib.oracle_afs(
             self.h, len(sizes), _p(sizes), _p(flat), len(windows) - 1, _p(windows),
@@ -277,6 +305,11 @@ class Reference(_Base):
         return self._lib.ref_sample_count_stat(
             self.h, sid, len(sizes), _p(sizes), _p(flat), M, _p(idx), len(windows) - 1,
             _p(windows), opts, _p(out))
+
+    def _relvec(self, W, windows, nodes, opts, out):
+        return self._lib.ref_relatedness_vector(
+            self.h, ctypes.c_uint64(W.shape[1]), _p(W), ctypes.c_uint64(len(windows) - 1),
+            _p(windows), ctypes.c_uint64(len(nodes)), _p(nodes), ctypes.c_uint32(opts), _p(out))
 
     def _afs(self, sizes, flat, windows, time_windows, opts, out):
         return self._lib.ref_afs(

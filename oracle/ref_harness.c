@@ -182,3 +182,12 @@ int ref_divergence_matrix_sets(const ref_ts_t *r, uint64_t num_sets, const uint6
     return tsk_treeseq_divergence_matrix(&r->ts, num_sets, sizes, sets, nw, windows, options,
         result);
 }
+
+/* tsk_treeseq_genetic_relatedness_vector (trees.c:10783): weights [n x K],
+ * result [nw x nf x K] */
+int ref_relatedness_vector(const ref_ts_t *r, uint64_t K, const double *weights, uint64_t nw,
+    const double *windows, uint64_t nf, const int32_t *focal, uint32_t options,
+    double *result) {
+    return tsk_treeseq_genetic_relatedness_vector(
+        &r->ts, K, weights, nw, windows, nf, focal, result, options);
+}

@@ -1053,3 +1053,104 @@ int oracle_afs(const ots_t *ts, int32_t K, const uint64_t *sizes, const int32_t 
     free(last);
     return 0;
 }
+
+/* ---- genetic_relatedness_vector, branch mode: trees.c:10402-10816 --------
+ * For window w, focal node f and weight column k:
+ *   y[w][f][k] = integral over the window of sum over the ancestors u of f
+ *                (f included) of branch_length(u) * Z(u, k),
+ * Z(u, k) the sum of the weights of the samples below u, the weights first
+ * centred over the samples (trees.c:10517-10540, unless NONCENTRED), the
+ * result then centred over the focal nodes (trees.c:10676-10697, unless
+ * NONCENTRED) and span-normalised (trees.c:10763-10765).  Stated directly:
+ * every tree is rebuilt from its edges and Z summed by walking each sample's
+ * root path, so the reference's lazy push-down (trees.c:10553-10605) is not
+ * followed.  Windows need not span the sequence (check_windows with no
+ * full-span rule, trees.c:1244-1286).  result: [nw x nf x K]. */
+int oracle_relatedness_vector(const ots_t *ts, int32_t K, const double *weights, int32_t nw,
+    const double *windows, int32_t nf, const int32_t *focal, uint32_t options,
+    double *result) {
+    const int32_t N = ts->N, E = ts->E, n = ts->n;
+    int32_t *par;
+    double *Z, *wc, *mean;
+    int32_t j, k, u, h, tj = 0, tk = 0, wi;
+    double tl = 0, tr;
+    if (options & (OPT_SITE | OPT_NODE)) return -909;
+    if (nw < 1) return -900;
+    if (windows[0] < 0 || windows[nw] > ts->L) return -901;
+    for (j = 0; j < nw; j++) {
+        if (windows[j] >= windows[j + 1]) return -901;
+    }
+    for (j = 0; j < nf; j++) {
+        if (focal[j] < 0 || focal[j] >= N) return -202;
+    }
+    par = malloc((size_t) (N + 1) * sizeof(int32_t));
+    Z = calloc((size_t) N * K + 1, sizeof(double));
+    wc = malloc((size_t) n * K * sizeof(double) + 8);
+    mean = calloc((size_t) K + 1, sizeof(double));
+    for (j = 0; j < N; j++) par[j] = NUL;
+    memcpy(wc, weights, (size_t) n * K * sizeof(double));
+    if (!(options & (1u << 14))) {
+        for (j = 0; j < n; j++) {
+            for (k = 0; k < K; k++) mean[k] += wc[(size_t) j * K + k];
+        }
+        for (k = 0; k < K; k++) mean[k] /= (double) n;
+        for (j = 0; j < n; j++) {
+            for (k = 0; k < K; k++) wc[(size_t) j * K + k] -= mean[k];
+        }
+    }
+    memset(result, 0, (size_t) nw * nf * K * sizeof(double));
+    while (tl < ts->L) {
+        while (tk < E && ts->right[ts->O[tk]] == tl) {
+            par[ts->child[ts->O[tk]]] = NUL;
+            tk++;
+        }
+        while (tj < E && ts->left[ts->I[tj]] == tl) {
+            h = ts->I[tj++];
+            par[ts->child[h]] = ts->parent[h];
+        }
+        tr = ts->L;
+        if (tj < E && ts->left[ts->I[tj]] < tr) tr = ts->left[ts->I[tj]];
+        if (tk < E && ts->right[ts->O[tk]] < tr) tr = ts->right[ts->O[tk]];
+        /* this tree's Z: each sample's weights up its root path */
+        memset(Z, 0, (size_t) N * K * sizeof(double));
+        for (j = 0; j < n; j++) {
+            for (u = ts->samples[j]; u != NUL; u = par[u]) {
+                for (k = 0; k < K; k++) Z[(size_t) u * K + k] += wc[(size_t) j * K + k];
+            }
+        }
+        for (wi = 0; wi < nw; wi++) {
+            const double a = tl > windows[wi] ? tl : windows[wi];
+            const double b = tr < windows[wi + 1] ? tr : windows[wi + 1];
+            if (!(a < b)) continue;
+            for (j = 0; j < nf; j++) {
+                double *row = result + ((size_t) wi * nf + j) * K;
+                for (u = focal[j]; u != NUL && par[u] != NUL; u = par[u]) {
+                    const double bl = ts->time[par[u]] - ts->time[u];
+                    for (k = 0; k < K; k++) row[k] += (b - a) * bl * Z[(size_t) u * K + k];
+                }
+            }
+        }
+        tl = tr;
+    }
+    if (!(options & (1u << 14)) && nf > 0) {
+        for (wi = 0; wi < nw; wi++) {
+            for (k = 0; k < K; k++) {
+                double s = 0;
+                for (j = 0; j < nf; j++) s += result[((size_t) wi * nf + j) * K + k];
+                s /= (double) nf;
+                for (j = 0; j < nf; j++) result[((size_t) wi * nf + j) * K + k] -= s;
+            }
+        }
+    }
+    if (options & OPT_SPAN_NORMALISE) {
+        for (wi = 0; wi < nw; wi++) {
+            const double span = windows[wi + 1] - windows[wi];
+            for (size_t r = 0; r < (size_t) nf * K; r++) result[(size_t) wi * nf * K + r] /= span;
+        }
+    }
+    free(par);
+    free(Z);
+    free(wc);
+    free(mean);
+    return 0;
+}

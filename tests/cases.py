@@ -187,3 +187,60 @@ def rel_err(got, exp, scale=None):
     if not np.any(fin):
         return 0.0
     return float(np.max(np.abs(got[fin] - exp[fin]) / scale[fin]))
+
+
+@functools.lru_cache(maxsize=None)
+def multiallelic(n, L, seed, every=3):
+    """A generated tree sequence whose every `every`-th site gets more
+    mutations on samples: a new allele "2" on one sample, "3" on another and
+    a back mutation "0" on a third carrier, so sites have 2 to 4 alleles
+    (recurrent, back and stacked mutations).  A mutation on a sample that
+    carries the site's first derived allele has that mutation as parent."""
+    from oracle.oracle import Oracle
+
+    src = generated(n, L, seed)
+    G = Oracle(src).genotypes()
+    rng = np.random.default_rng(seed + 77)
+    S = len(src.site_position)
+    first = np.searchsorted(src.mutation_site, np.arange(S))
+    msite, mnode, mpar, states = [], [], [], []
+    for s in range(S):
+        m0 = int(first[s])
+        if m0 >= len(src.mutation_site) or src.mutation_site[m0] != s:
+            continue
+        base = len(msite)
+        msite.append(s)
+        mnode.append(int(src.mutation_node[m0]))
+        mpar.append(-1)
+        states.append(b"1")
+        if s % every:
+            continue
+        carriers = np.flatnonzero(G[s] == 1)
+        others = np.flatnonzero(G[s] == 0)
+        picks = []
+        if len(others):
+            picks.append((int(rng.choice(others)), b"2", -1))
+        if len(carriers):
+            picks.append((int(rng.choice(carriers)), b"3", base))
+        if len(carriers) > 1:
+            picks.append((int(rng.choice(carriers)), b"0", base))
+        used = set()
+        for v, st, p in picks:
+            if v in used:
+                continue
+            used.add(v)
+            msite.append(s)
+            mnode.append(v)
+            mpar.append(p)
+            states.append(st)
+    tc = ta.TableCollection(src.sequence_length)
+    tc.set_nodes(src.node_flags, src.node_time)
+    tc.set_edges(src.edge_left, src.edge_right, src.edge_parent, src.edge_child)
+    tc.site_position = src.site_position.copy()
+    tc.ancestral_state, tc.ancestral_state_offset = src.ancestral_state, src.ancestral_state_offset
+    tc.mutation_site = np.array(msite, np.int32)
+    tc.mutation_node = np.array(mnode, np.int32)
+    tc.mutation_parent = np.array(mpar, np.int32)
+    tc.derived_state = np.frombuffer(b"".join(states), np.int8).copy()
+    tc.derived_state_offset = np.arange(len(states) + 1, dtype=np.uint64)
+    return tc

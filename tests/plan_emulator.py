@@ -500,3 +500,155 @@ def run_hot_afs(hp, plan_sites, bp, windows, time_windows, sets, num_samples, no
                              hp["tl_ins"][q], L)
     span = np.diff(windows)
     return hist / span.reshape((W,) + (1,) * (hist.ndim - 1))
+
+
+def run_hot_relvec(hp, bp, windows, weights, focal):
+    """The genetic_relatedness_vector program of tsg_relvec.hip executed on
+    the CPU, step by step as the kernels do it: per chunk the lazy push-down
+    over the hot slots (k_relvec_chunk), writing E and the window-segment
+    integrals I per slot; the first tree's D_0 (k_relvec_tree0_*); then per
+    focal sample the prefix over chunks (k_relvec_esum, k_relvec_assemble).
+    weights [n, K] as the kernels take them (already centred or not); focal
+    are sample indexes.  Returns y [W, nf, K], before the final centring and
+    span normalisation."""
+    ct0, cs0, cl0 = hp["chunk_tree0"], hp["chunk_slot0"], hp["chunk_lev0"]
+    C = len(ct0) - 1
+    T = len(bp) - 1
+    n, K = weights.shape
+    W = len(windows) - 1
+    E = [None] * C
+    I = [dict() for _ in range(C)]
+    for c in range(C):
+        s0, s1 = cs0[c], cs0[c + 1]
+        S = s1 - s0
+        sp = hp["slot_par"][s0:s1]
+        par = sp.astype(np.int64).copy()
+        bl = hp["slot_bl0"][s0:s1].astype(np.float64).copy()
+        t0, t1 = ct0[c], ct0[c + 1]
+        ca, cb = bp[t0], bp[t1]
+        xs = np.full(S, ca)
+        w = np.zeros((S, K))
+        v = np.zeros((S, K))
+        top = np.zeros(S, bool)
+        tlast = min(t1, T - 1)
+        if t0 + 1 <= tlast:
+            e0, e1 = hp["tr_off"][t0 + 1], hp["tr_off"][tlast + 1]
+            top[hp["ev_child"][e0:e1]] = True
+            top[hp["path"][hp["ev_path0"][e0]:hp["ev_path0"][e1]]] = True
+        ss = hp["sample_slot"][c * n:(c + 1) * n]
+        for s in range(n):
+            if ss[s] != NO_SLOT:
+                w[ss[s]] += weights[s]
+        lev = hp["lev_slot0"][cl0[c]:cl0[c + 1]]
+        nlev = len(lev) - 1
+        for j in range(nlev - 1):
+            for h in range(lev[j], lev[j + 1]):
+                if par[h] >= 0:
+                    w[par[h]] += w[h]
+        d0 = np.zeros((S, K))
+        for j in range(nlev - 1, -1, -1):
+            for h in range(lev[j], lev[j + 1]):
+                d0[h] = (d0[par[h]] if par[h] >= 0 else 0.0) + bl[h] * w[h]
+        seg0 = ca
+        bi = 0
+        while bi <= W and windows[bi] <= ca:
+            bi += 1
+
+        def readout(x, m):
+            nonlocal seg0, v
+            v += (bl * (x - xs))[:, None] * w
+            xs[:] = x
+            if 0 <= m < W and x > seg0:
+                r = np.zeros((S, K))
+                for h in range(S):
+                    u = h
+                    while u >= 0:
+                        r[h] += v[u]
+                        u = par[u]
+                I[c][m] = r - (x - seg0) * d0
+            v = np.zeros((S, K))
+            seg0 = x
+
+        def boundaries_to(x):
+            nonlocal bi
+            while bi <= W and windows[bi] <= x:
+                readout(windows[bi], bi - 1)
+                bi += 1
+
+        def transition(t):
+            x = bp[t]
+            e0, e1 = hp["tr_off"][t], hp["tr_off"][t + 1]
+            nrem = hp["tr_nrem"][t]
+            for e in range(e0, e1):
+                ins = (e - e0) >= nrem
+                u = hp["ev_child"][e]
+                sg = 1.0 if ins else -1.0
+                if not ins:
+                    v[u] += bl[u] * (x - xs[u]) * w[u]
+                    xs[u] = x
+                    bl[u] = 0.0
+                    par[u] = -1
+                path = hp["path"][hp["ev_path0"][e]:hp["ev_path0"][e + 1]]
+                for p in path:
+                    v[p] += bl[p] * (x - xs[p]) * w[p]
+                    xs[p] = x
+                    v[u] -= sg * v[p]
+                    w[p] += sg * w[u]
+                if ins:
+                    xs[u] = x
+                    par[u] = path[0]
+                    bl[u] = hp["ev_bl"][e]
+
+        for t in range(t0 + 1, t1):
+            boundaries_to(bp[t])
+            transition(t)
+        boundaries_to(cb)
+        if seg0 < cb:
+            readout(cb, bi - 1)
+        if t1 < T:
+            transition(t1)
+        Ec = np.zeros((S, K))
+        for h in range(S):
+            u = h
+            while u >= 0:
+                Ec[h] += bl[u] * w[u]
+                u = par[u]
+        Ec -= d0
+        for h in range(S):
+            if top[h]:
+                continue
+            q = sp[h]
+            while q >= 0 and not top[q]:
+                q = sp[q]
+            Ec[h] = Ec[q] if q >= 0 else 0.0
+            for m in I[c]:
+                I[c][m][h] = I[c][m][q] if q >= 0 else 0.0
+        E[c] = Ec
+    # first tree
+    t0p, t0b = hp["t0_parent"], hp["t0_bl"]
+    z0 = np.zeros((len(t0p), K))
+    for s in range(n):
+        u = s
+        while u >= 0:
+            z0[u] += weights[s]
+            u = t0p[u]
+    y = np.zeros((W, len(focal), K))
+    for j, s in enumerate(focal):
+        P = np.zeros(K)
+        u = s
+        while u >= 0 and t0p[u] >= 0:
+            P += t0b[u] * z0[u]
+            u = t0p[u]
+        for c in range(C):
+            h = hp["sample_slot"][c * n + s]
+            ca, cb = bp[ct0[c]], bp[ct0[c + 1]]
+            for m in range(W):
+                sa, sb = max(ca, windows[m]), min(cb, windows[m + 1])
+                if not sa < sb:
+                    continue
+                y[m, j] += (sb - sa) * P
+                if h != NO_SLOT:
+                    y[m, j] += I[c][m][h]
+            if h != NO_SLOT:
+                P += E[c][h]
+    return y

@@ -830,3 +830,131 @@ def test_relatedness_matrix_branch_and_sets(require_gpu):
     assert np.allclose(got, exp, rtol=1e-9, atol=1e-12 * np.max(np.abs(exp)))
 
 
+
+
+@pytest.mark.parametrize("case", ["generated", "gapped", "paper_ex", "wide"])
+def test_relatedness_vector(require_gpu, case):
+    """genetic_relatedness_vector (trees.c:10402-10816) on the hot plan
+    against the compiled reference when oracle/_ref is present and the
+    oracle (pinned to it and to the closed form G w): windows spanning the
+    sequence and not, centred and not, span-normalised and not, a subset of
+    samples as focal nodes, 20 weight columns (two column groups), one and
+    several chunks.  Tolerance: <= 1e-6 of the largest |value| of each
+    window (the result is centred, so it changes sign)."""
+    from oracle import oracle as om
+
+    tc = {"generated": lambda: cases.generated(150, 3e5, 41),
+          "gapped": lambda: cases.gapped(120, 2e5, 43, count=20),
+          "paper_ex": cases.paper_ex,
+          "wide": lambda: cases.generated(3000, 1e6, 13)}[case]()
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    ref = om.Reference(tc) if om.reference_available() else None
+    n = ts.num_samples
+    L = tc.sequence_length
+    rng = np.random.default_rng(6)
+    smp = ts.samples()
+
+    def check(got, W, win, sn, centre, nodes=None):
+        for impl in [orc] + ([ref] if ref is not None else []):
+            exp = impl.relatedness_vector(W, win, "branch", sn, centre, nodes)
+            if win is None:
+                exp = exp[0]
+            assert got.shape == exp.shape
+            scale = np.max(np.abs(exp), axis=tuple(range(1, exp.ndim)) if win is not None
+                           else None, keepdims=True)
+            err = rel_err(got, exp, np.maximum(scale, 1e-300))
+            assert err <= TOL, (type(impl).__name__, err)
+
+    for chunks in ([None, 1] if case != "wide" else [None]):
+        if chunks:
+            ts.ll_tree_sequence.set_num_chunks(chunks)
+        for win in [None, np.linspace(0, L, 6), np.array([0.05 * L, 0.4 * L, 0.9 * L])]:
+            W = rng.normal(size=(n, 3))
+            _say("relatedness vector", case, chunks, None if win is None else len(win))
+            for sn in (True, False):
+                for centre in (True, False):
+                    got = ts.genetic_relatedness_vector(W, windows=win, mode="branch",
+                                                        span_normalise=sn, centre=centre)
+                    check(got, W, win, sn, centre)
+        sub = smp[::-1][: max(2, n // 3)]
+        got = ts.genetic_relatedness_vector(W, windows=np.linspace(0, L, 3), mode="branch",
+                                            centre=False, nodes=sub)
+        check(got, W, np.linspace(0, L, 3), True, False, sub)
+    W20 = rng.normal(size=(n, 20))
+    _say("relatedness vector 20 columns", case)
+    got = ts.genetic_relatedness_vector(W20, windows=np.linspace(0, L, 4), mode="branch")
+    check(got, W20, np.linspace(0, L, 4), True, True)
+
+
+@pytest.mark.parametrize("case", ["multiallelic", "single_tree_ex", "wide"])
+@pytest.mark.parametrize("plan", ["hot", "full"])
+def test_multiallelic_matrices(require_gpu, case, plan):
+    """Site divergence and relatedness matrices with back, recurrent and
+    stacked mutations (2 to 4 alleles per site): the extra allele rows go
+    into a second int8 cross product.  Against the genotype closed form on
+    the oracle's genotypes (pinned to tsk_variant_decode) and the reference's
+    tsk_treeseq_divergence_matrix when oracle/_ref is present; the
+    relatedness matrix against the reference's composition of the divergence
+    matrix (trees.py:9080-9102).  Integer cross products: rtol 1e-12."""
+    import test_oracle as to
+    from oracle import oracle as om
+
+    tc = {"multiallelic": lambda: cases.multiallelic(150, 3e5, 5),
+          "single_tree_ex": cases.single_tree_ex,
+          "wide": lambda: cases.multiallelic(3000, 2e5, 9, every=2)}[case]()
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    G = Oracle(tc).genotype_matrix(False)
+    n = G.shape[1]
+    L = tc.sequence_length
+    w = np.array([0, 0.4 * L, L])
+    _say("multiallelic matrices", case, plan)
+    ll.set_sweep_plan(1 if plan == "full" else 0)
+    try:
+        got = ts.divergence_matrix(windows=w)
+        exp = to._allele_divergence(G, tc.site_position, w) / np.diff(w)[:, None, None]
+        assert np.allclose(got, exp, rtol=1e-12, atol=0)
+        if om.reference_available() and n <= 300:
+            ref = om.Reference(tc).divergence_matrix(np.arange(n, dtype=np.int32), w, "site")
+            assert np.allclose(got, ref, rtol=1e-12, atol=0)
+        if n >= 4:
+            sets = [np.arange(0, n // 2, dtype=np.int32),
+                    np.arange(n // 2, n - 1, dtype=np.int32), np.array([n - 1], np.int32)]
+            got = ts.divergence_matrix(sample_sets=sets, windows=w)
+            exp = to._allele_divergence(G, tc.site_position, w, [list(s) for s in sets])
+            assert np.allclose(got, exp / np.diff(w)[:, None, None], rtol=1e-12, atol=0)
+        K = ts.genetic_relatedness_matrix(windows=w)
+        D = to._allele_divergence(G, tc.site_position, w) / np.diff(w)[:, None, None]
+        expK = np.array([_relatedness_from_divergence(D[j], np.ones(n)) for j in range(2)])
+        assert np.allclose(K, expK, rtol=1e-9, atol=1e-12 * np.max(np.abs(expK)))
+    finally:
+        ll.set_sweep_plan(0)
+
+
+def test_branch_divergence_matrix_wide(require_gpu):
+    """The branch divergence matrix of 3000 singleton samples (4.5e6 sample
+    pairs, beyond the fused-sweep route's 2^22): symmetric with a zero
+    diagonal, and a block of 30 samples equal to the shared-branch-length
+    closed form on the oracle (pinned to the reference's MRCA route), over
+    windows that do not span the sequence."""
+    import test_oracle as to
+
+    tc = cases.generated(3000, 1e5, 61)
+    ts = tc.tree_sequence()
+    orc = Oracle(tc)
+    L = tc.sequence_length
+    w = np.array([0.1 * L, 0.6 * L, 0.95 * L])
+    _say("branch divergence matrix wide")
+    D = ts.divergence_matrix(windows=w, mode="branch")
+    assert D.shape == (2, 3000, 3000)
+    assert np.array_equal(D, np.transpose(D, (0, 2, 1)))
+    assert np.all(D[:, np.arange(3000), np.arange(3000)] == 0)
+    pick = np.sort(np.random.default_rng(4).choice(3000, 30, replace=False))
+    single = [np.array([i], np.int32) for i in pick]
+    pairs = [(i, j) for i in range(30) for j in range(i, 30)]
+    fw = np.concatenate([[0.0], w, [L]])
+    G = orc.stat("genetic_relatedness", single, pairs, fw, "branch", False, True, False)
+    exp = to._branch_divergence_from_shared(G, 30, fw, [[i] for i in range(30)])[1:3]
+    got = D[:, pick][:, :, pick]
+    assert np.allclose(got, exp, rtol=1e-9, atol=1e-12 * np.max(np.abs(exp)))

@@ -380,3 +380,77 @@ def test_serve_request_normalise_and_pack():
     srv.one_per_launch = False
     launches = srv._pack([a, b, c], 10)
     assert len(launches) == 1 and len(launches[0]) == 3
+
+
+@pytest.mark.parametrize("case,chunks", [("generated", 13), ("generated", 1), ("gapped", 9),
+                                         ("paper_ex", 2)])
+def test_hot_plan_relvec_executes_to_oracle(case, chunks):
+    """The relatedness-vector program of tsg_relvec.hip (per-chunk lazy
+    push-down over the hot slots, first-tree D_0, prefix over chunks),
+    executed on the CPU by the plan emulator, equals the oracle's
+    genetic_relatedness_vector: windows spanning the sequence, windows that
+    do not, a window boundary inside a chunk."""
+    tc = {"generated": lambda: cases.generated(60, 1e5, 3),
+          "gapped": lambda: cases.gapped(60, 2e5, 3, count=10),
+          "paper_ex": cases.paper_ex}[case]()
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    ll.set_num_chunks(chunks)
+    hp = ll.get_hot_plan()
+    bp = ts.breakpoints(as_array=True)
+    n = ts.num_samples
+    weights = np.random.default_rng(3).normal(size=(n, 2))
+    L = tc.sequence_length
+    orc = Oracle(tc)
+    smp = np.flatnonzero(tc.node_flags & 1)
+    for win in [np.array([0, L]), np.array([0, 0.3 * L, 0.31 * L, L]),
+                np.array([0.1 * L, 0.5 * L, 0.77 * L])]:
+        focal = np.arange(n)[::-1]
+        y = plan_emulator.run_hot_relvec(hp, bp, win, weights, focal)
+        exp = orc.relatedness_vector(weights, win, "branch", False, False, smp[focal])
+        assert np.allclose(y, exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
+
+
+def test_relatedness_vector_errors(paper):
+    """Argument errors of genetic_relatedness_vector, in the reference's order
+    and numbers (trees.c:10783-10816, 1244-1286, 10491-10496)."""
+    ll = paper.ll_tree_sequence
+    W = np.ones((4, 1))
+    assert _err(lambda: ll.genetic_relatedness_vector(W, [0, 10], mode="site")) == -909
+    assert _err(lambda: ll.genetic_relatedness_vector(W, [0, 10], mode="node")) == -909
+    assert _err(lambda: ll.genetic_relatedness_vector(W, [0, 11], mode="branch")) == -901
+    assert _err(lambda: ll.genetic_relatedness_vector(W, [-1, 5], mode="branch")) == -901
+    assert _err(lambda: ll.genetic_relatedness_vector(W, [0, 5, 5], mode="branch")) == -901
+    assert _err(lambda: ll.genetic_relatedness_vector(W, [0, 10], mode="branch",
+                                                      nodes=[0, 99])) == -202
+    with pytest.raises(ValueError):
+        ll.genetic_relatedness_vector(np.ones((3, 1)), [0, 10], mode="branch")
+    with pytest.raises(ValueError):
+        paper.genetic_relatedness_vector(np.ones(4), mode="branch", nodes=[0, 1])
+    with pytest.raises(ta.LibraryError):
+        paper.genetic_relatedness_vector(np.ones(4))  # mode="site", as the reference
+
+
+def test_serve_memory_admission():
+    """A job starts only while the launches in flight and its own fit the
+    device-memory budget, or when nothing is in flight."""
+    from tskit_amd import serve
+
+    class FakeTS:
+        def samples(self):
+            return np.arange(1000, dtype=np.int32)
+
+    srv = serve.StatServer.__new__(serve.StatServer)
+    srv.chunks = [100]
+    r = serve.Request(0, 0, "diversity", {"sample_sets": [np.arange(500)], "windows": [0, 1, 2]})
+    r.normalise(FakeTS())
+    job = [[r]]
+    need = srv.job_bytes(job)
+    assert need == serve.launch_bytes(1, 1, 2, 100, 500)
+    srv.memory_budget, srv.inflight_bytes = need + 10, 0
+    assert srv.admits(job, 0) and srv.admits(job, 1)
+    srv.inflight_bytes = 11
+    assert not srv.admits(job, 1)
+    assert srv.admits(job, 0)  # nothing in flight: always
+    srv.memory_budget = None
+    assert srv.admits(job, 5)

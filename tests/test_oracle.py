@@ -526,3 +526,182 @@ def test_branch_divergence_matrix_shared_length_vs_reference(case):
     exp1 = ref.divergence_matrix(np.arange(n, dtype=np.int32), w, "branch", True)
     got1 = _branch_divergence_from_shared(G, n, w, [[i] for i in range(n)])
     assert np.allclose(got1, exp1, rtol=1e-11, atol=1e-300)
+
+
+RELVEC_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "relvec_reference.npz")
+
+
+@pytest.mark.parametrize("key", ["generated", "gapped"])
+def test_golden_relatedness_vector(key):
+    """genetic_relatedness_vector of the oracle against the reference's own
+    outputs (tests/golden/make_relvec_golden.py): windows spanning the
+    sequence and not, centred and not, every node as a focal node."""
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_relvec_golden as mk
+
+    g = np.load(RELVEC_GOLDEN)
+    tc = mk.inputs()[key]
+    assert tc.column_digest() == bytes(g[f"{key}_digest"]).decode(), "generator output changed"
+    orc = Oracle(tc)
+    W = mk.weights(int(np.sum(tc.node_flags & 1)))
+    for name, wk, centre, sn, fk in mk.CHECKS:
+        got = orc.relatedness_vector(W, mk.windows()[wk], "branch", sn, centre, mk.focal(tc, fk))
+        exp = g[f"{key}_{name}"]
+        assert got.shape == exp.shape
+        assert np.allclose(got, exp, rtol=1e-12, atol=1e-12 * np.max(np.abs(exp))), name
+
+
+@needs_ref
+@pytest.mark.parametrize("fixture", ["paper_ex", "single_tree_ex", "nonbinary_ex", "generated"])
+def test_relatedness_vector_vs_reference(fixture):
+    tc = cases.generated(40, 3e4, 12) if fixture == "generated" else getattr(cases, fixture)()
+    orc, ref = Oracle(tc), om.Reference(tc)
+    n = int(np.sum(tc.node_flags & 1))
+    W = np.random.default_rng(2).normal(size=(n, 2))
+    L = tc.sequence_length
+    for w in [None, np.array([0.2 * L, 0.5 * L, 0.9 * L])]:
+        for centre in (True, False):
+            a = orc.relatedness_vector(W, w, "branch", True, centre)
+            b = ref.relatedness_vector(W, w, "branch", True, centre)
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.max(np.abs(b)))
+    for mode in ("site", "node"):
+        with pytest.raises(om.OracleError):
+            orc.relatedness_vector(W, None, mode)
+        with pytest.raises(om.OracleError):
+            ref.relatedness_vector(W, None, mode)
+
+
+@pytest.mark.parametrize("case", ["generated", "gapped", "paper_ex"])
+def test_relatedness_vector_closed_form(case):
+    """Uncentred, the relatedness vector is G w with G the branch length each
+    pair of samples shares (the noncentred relatedness of singleton sets,
+    trees.c:4755-4773, itself checked against the reference); centred, it is
+    J G J w with J = I - 11'/n (trees.py:9157-9225)."""
+    tc = {"generated": lambda: cases.generated(30, 4e4, 8),
+          "gapped": lambda: cases.gapped(30, 4e4, 8, count=6),
+          "paper_ex": cases.paper_ex}[case]()
+    orc = Oracle(tc)
+    smp = np.flatnonzero(tc.node_flags & 1)
+    n = len(smp)
+    single = [np.array([i], np.int32) for i in smp]
+    pairs = [(i, j) for i in range(n) for j in range(n)]
+    L = tc.sequence_length
+    w = np.array([0, 0.4 * L, L])
+    G = orc.stat("genetic_relatedness", single, pairs, w, "branch", False, True, False)
+    W = np.random.default_rng(7).normal(size=(n, 2))
+    J = np.eye(n) - 1.0 / n
+    raw = orc.relatedness_vector(W, w, "branch", False, False)
+    cen = orc.relatedness_vector(W, w, "branch", False, True)
+    for j in range(2):
+        Gj = G[j].reshape(n, n)
+        assert np.allclose(raw[j], Gj @ W, rtol=1e-12, atol=1e-12 * np.max(np.abs(raw[j])))
+        assert np.allclose(cen[j], J @ Gj @ J @ W, rtol=1e-10,
+                           atol=1e-12 * np.max(np.abs(Gj @ W)))
+
+
+@needs_ref
+def test_window_partitioned_reference_equals_whole():
+    """The CPU baseline's window-partitioned reference (each process clips
+    edges and sites to its block of windows, trees.c:1255-1266 forcing the
+    outer windows) gives the single-process reference's values."""
+    import functools
+
+    from oracle import partitioned
+
+    tc = cases.generated(300, 4e5, 21)
+    sets = cases.sample_sets(300, 4, 40, 2)
+    w = np.linspace(0, 4e5, 11)
+    pairs = [(0, 1), (2, 3), (1, 3)]
+    quads = [(0, 1, 2, 3)]
+    step = functools.partial(partitioned.scale_step, sets=sets, pairs=pairs, quads=quads)
+    whole = step(om.Reference(tc), w)
+    got, wall, t_init, t_stat, P = partitioned.run(tc, w, step, processes=3)
+    assert P == 3 and wall > 0 and t_init > 0 and t_stat > 0
+    for k, v in whole.items():
+        fin = np.isfinite(v)
+        assert np.array_equal(fin, np.isfinite(got[k])), k
+        scale = np.max(np.abs(v[fin]))
+        assert np.max(np.abs(got[k][fin] - v[fin])) <= 1e-10 * scale, k
+
+
+def _allele_divergence(G, pos, windows, sets=None):
+    """Site divergence matrix from genotypes G [S, n] (isolated samples
+    ancestral): per window the number of (site, sample pair) with different
+    alleles, summed over each pair of sets and divided by its number of pairs
+    (trees.c:8684-8826, 8876-8899)."""
+    n = G.shape[1]
+    if sets is None:
+        sets = [[i] for i in range(n)]
+    out = []
+    for j in range(len(windows) - 1):
+        g = G[(pos >= windows[j]) & (pos < windows[j + 1])]
+        # sites minus the pairs carrying the same allele
+        D = np.full((n, n), len(g), dtype=np.int64)
+        for a in range(int(g.max()) + 1 if len(g) else 0):
+            B = (g == a).astype(np.int64)
+            D -= B.T @ B
+        D = D.astype(np.float64)
+        K = len(sets)
+        R = np.zeros((K, K))
+        for a in range(K):
+            for b in range(K):
+                v = D[np.ix_(sets[a], sets[b])].sum()
+                na, nb = len(sets[a]), len(sets[b])
+                den = na * (na - 1) if a == b else na * nb
+                R[a, b] = v / den if den else v
+        out.append(R)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("case", ["multiallelic", "single_tree_ex"])
+def test_multiallelic_rows_identity(case):
+    """The row decomposition the matrix kernels use for sites with three or
+    more alleles: with P = 2 G1'G1 + G2'G2 (G1: derived rows of sites with at
+    most two alleles; G2: any-derived row and one row per derived allele of
+    the others), (P_ii + P_jj)/2 - P_ij counts the sites where samples i and
+    j carry different alleles, on the oracle's genotypes (pinned to
+    tsk_variant_decode)."""
+    tc = cases.multiallelic(40, 6e4, 4) if case == "multiallelic" else cases.single_tree_ex()
+    G = Oracle(tc).genotype_matrix(False)
+    na = G.max(axis=1) + 1
+    # single_tree_ex: two alleles per site, but back and recurrent mutations
+    assert np.any(na >= 3) or case == "single_tree_ex"
+    rows1 = [(G[s] != 0).astype(np.int64) for s in range(len(G)) if na[s] <= 2]
+    rows2 = []
+    for s in np.flatnonzero(na >= 3):
+        rows2.append((G[s] != 0).astype(np.int64))
+        rows2.extend((G[s] == a).astype(np.int64) for a in range(1, na[s]))
+    G1 = np.array(rows1).reshape(-1, G.shape[1])
+    G2 = np.array(rows2).reshape(-1, G.shape[1])
+    P = 2 * G1.T @ G1 + G2.T @ G2
+    d = np.diag(P)
+    assert np.all(d % 2 == 0)
+    D = (d[:, None] + d[None, :]) // 2 - P
+    exp = (G[:, :, None] != G[:, None, :]).sum(axis=0)
+    assert np.array_equal(D, exp)
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["multiallelic", "single_tree_ex"])
+def test_multiallelic_divergence_matrix_vs_reference(case):
+    """The genotype closed form of the site divergence matrix against the
+    reference's tsk_treeseq_divergence_matrix on sites with back, recurrent
+    and stacked mutations (2 to 4 alleles), singletons and sets."""
+    tc = cases.multiallelic(40, 6e4, 4) if case == "multiallelic" else cases.single_tree_ex()
+    ref = om.Reference(tc)
+    G = Oracle(tc).genotype_matrix(False)
+    n = G.shape[1]
+    L = tc.sequence_length
+    w = np.array([0, 0.35 * L, L])
+    got = ref.divergence_matrix(np.arange(n, dtype=np.int32), w, "site")
+    exp = _allele_divergence(G, tc.site_position, w) / np.diff(w)[:, None, None]
+    assert np.allclose(got, exp, rtol=1e-12, atol=0)
+    if n >= 4:
+        sets = [np.arange(0, n // 2, dtype=np.int32), np.arange(n // 2, n - 1, dtype=np.int32),
+                np.array([n - 1], dtype=np.int32)]
+        got = ref.divergence_matrix_sets(sets, w, "site")
+        exp = _allele_divergence(G, tc.site_position, w, [list(s) for s in sets])
+        assert np.allclose(got, exp / np.diff(w)[:, None, None], rtol=1e-12, atol=0)

@@ -112,3 +112,27 @@ def test_default_and_flat_sets_and_errors(resident):
     assert _same(reqs[1].result, tss[0].diversity(flat, windows=w, mode="site"))
     assert reqs[2].result is None and reqs[2].error is not None
     assert _same(reqs[3].result, tss[2].diversity([tss[2].samples()[:30]], mode="branch"))
+
+
+def test_memory_admission_defers_launches(resident):
+    """With a device-memory budget below two launches, launches wait for
+    each other (counted in `deferred`) and every request is still served,
+    with the results of an unconstrained server."""
+    tcs, tss = resident
+    reqs = serve.random_requests(tss, 30, seed=9, max_sets=6, max_set_size=50, max_windows=8,
+                                 stats=("diversity", "divergence", "f2"))
+    free = serve.StatServer(tss, max_inflight=4)
+    for r in reqs:
+        free.submit(r)
+    free.step()
+    free.close()
+    tight = serve.random_requests(tss, 30, seed=9, max_sets=6, max_set_size=50, max_windows=8,
+                                  stats=("diversity", "divergence", "f2"))
+    srv = serve.StatServer(tss, max_inflight=4, memory_budget=1)
+    for r in tight:
+        srv.submit(r)
+    done = srv.step()
+    srv.close()
+    assert len(done) == len(tight) and srv.deferred > 0 and srv.inflight_bytes == 0
+    for a, b in zip(reqs, tight):
+        assert _same(a.result, b.result)

@@ -30,6 +30,7 @@
 
 #include "tsg_common.h"
 #include "tsg_launch.h"
+#include "tsg_relvec.h"
 #include "tsg_treeseq.h"
 
 void tsg_build_index(const double *left, const double *right, const int32_t *parent,
@@ -64,8 +65,12 @@ struct Workspace {
     // a batch's results, copied back in one transfer (pinned)
     void *out_h = nullptr;
     size_t out_h_bytes = 0;
+    // relatedness-vector route
+    DevBuf rv_E, rv_I, rv_z0, rv_d0, rv_esum, rv_y;
     // relatedness-matrix route
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab, geno;
+    // ... the rows of sites with three or more alleles
+    DevBuf sitecol2, gt2, cmat2, wk0b, wk1b, row2;
     int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
     bool force_full = false;        // run sweeps on the full plan (testing hook)
     bool force_c32 = false;         // hot plan with 32-bit counts (testing hook)
@@ -161,6 +166,12 @@ int ensure(DevBuf &b, size_t bytes) {
     }
     b.bytes = want;
     return 0;
+}
+
+// The workspace's buffer b of at least `bytes`, allocated or grown.
+int ensure(Workspace *ws, DevBuf &b, size_t bytes) {
+    (void) ws;
+    return ensure(b, bytes);
 }
 
 template <typename T>
@@ -282,7 +293,7 @@ struct Pack {
         return (int) parts.size() - 1;
     }
     int send(Workspace *ws, hipStream_t s) const {
-        int ret = ensure(ws->arena, total);
+        int ret = ensure(ws, ws->arena, total);
         if (ret != 0) return ret;
         if (ws->stage_bytes < total) {
             if (ws->stage_h != nullptr) (void) hipHostFree(ws->stage_h);
@@ -741,8 +752,8 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     const int p_tot = pk.add(col_total), p_outs = pk.add(outs), p_qs = pk.add(qs);
     const int p_win = pk.add(wins), p_wf = pk.add(cq_wfirst), p_nw = pk.add(cq_nwin);
     const int p_po = pk.add(cq_poff), p_rb = pk.add(rawbase), p_lb = pk.add(lanebase);
-    if ((ret = ensure(ws->partial, (size_t) pcount * 8)) ||
-        (ret = ensure(ws->raw, (size_t) rawcount * 8))) {
+    if ((ret = ensure(ws, ws->partial, (size_t) pcount * 8)) ||
+        (ret = ensure(ws, ws->raw, (size_t) rawcount * 8))) {
         return ret;
     }
     // the first tree, samples first (its running branch sum is evaluated
@@ -751,11 +762,11 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
                               : (int) (ts->chunk_node0[1] - ts->chunk_node0[0]);
     const int nb0 = tsg_tree0_blocks(nloc0);
     if (mode == TSG_STAT_BRANCH) {
-        if ((ret = ensure(ws->dR, (size_t) C * M * 8)) ||
-            (ret = ensure(ws->Rstart, (size_t) C * M * 8)) ||
-            (ret = ensure(ws->x0, (size_t) nloc0 * K * 4)) ||
-            (ret = ensure(ws->r0part, (size_t) nb0 * M * 8)) ||
-            (ret = ensure(ws->bsum, (size_t) tsg_scan_blocks(C) * M * 8))) {
+        if ((ret = ensure(ws, ws->dR, (size_t) C * M * 8)) ||
+            (ret = ensure(ws, ws->Rstart, (size_t) C * M * 8)) ||
+            (ret = ensure(ws, ws->x0, (size_t) nloc0 * K * 4)) ||
+            (ret = ensure(ws, ws->r0part, (size_t) nb0 * M * 8)) ||
+            (ret = ensure(ws, ws->bsum, (size_t) tsg_scan_blocks(C) * M * 8))) {
             return ret;
         }
     }
@@ -883,7 +894,7 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     }
     r.outcount = outcount;
     const int p_fin = pk.add(fin), p_ip = pk.add(ipar), p_dp = pk.add(dpar);
-    if ((ret = ensure(ws->out, (size_t) outcount * 8)) || (ret = pk.send(ws, s))) return ret;
+    if ((ret = ensure(ws, ws->out, (size_t) outcount * 8)) || (ret = pk.send(ws, s))) return ret;
     a.npairs = (int32_t) sd.flat_s.size();
     a.pair_sample = pk.dev<const int32_t>(ws, p_fs);
     a.pair_col = pk.dev<const int32_t>(ws, p_col);
@@ -1313,7 +1324,9 @@ void free_one_workspace(Workspace *ws) {
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
                       &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
                       &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->geno, &ws->x0,
-                      &ws->r0part, &ws->bsum, &ws->margs, &ws->mcpre, &ws->arena};
+                      &ws->r0part, &ws->bsum, &ws->margs, &ws->mcpre, &ws->arena,
+                      &ws->rv_E, &ws->rv_I, &ws->rv_z0, &ws->rv_d0, &ws->rv_esum, &ws->rv_y,
+                      &ws->sitecol2, &ws->gt2, &ws->cmat2, &ws->wk0b, &ws->wk1b, &ws->row2};
     for (DevBuf *b : bufs) {
         if (b->p != nullptr) (void) hipFree(b->p);
     }
@@ -1430,7 +1443,6 @@ const char *tsg_strerror(int err) {
     case TSG_ERR_NO_DEVICE: return "No usable GPU device";
     case TSG_ERR_DEVICE_MEMORY: return "Device memory allocation failed";
     case TSG_ERR_UNSUPPORTED_INPUT: return "Input shape not supported by the device path";
-    case TSG_ERR_NOT_BIALLELIC: return "Every site must carry exactly one mutation";
     default: return "Error in tree sequence input or statistics arguments";
     }
 }
@@ -1940,6 +1952,21 @@ int tsg_treeseq_genetic_relatedness_proportion(tsg_treeseq *ts, uint64_t K, cons
 // sequence are padded at both ends for the sweep.  The pairs are then summed
 // per pair of sets and count-normalised (trees.c:8876-8899) on the host, an
 // O(W n^2) pass over the sweep's result.
+namespace {
+int relvec_run(tsg_treeseq *ts, int K, const std::vector<double> &wc,
+               const std::vector<int32_t> &ident, const std::vector<double> &win,
+               const std::vector<int32_t> &fs, bool centre, bool span_normalise,
+               double *result);
+}  // namespace
+
+// Branch divergence matrix (trees.c:8579-8745): per window the branch length
+// G_ij that samples i and j share, integrated along the genome, gives their
+// path length G_ii + G_jj - 2 G_ij.  G is the relatedness vector of the
+// identity weights (sample j weighs 1 in column j; tsg_relvec.hip), noncentred;
+// then per pair of sets the sum over their samples' pairs, count-normalised
+// (trees.c:8876-8899).  A tree sequence without a usable hot plan takes the
+// fused sweep of the noncentred relatedness of singleton sets instead
+// (n(n+1)/2 outputs, at most 2^22 of them).
 static int branch_divergence_matrix(tsg_treeseq *ts, const std::vector<int32_t> &sel,
                                     const std::vector<int32_t> &set_off,
                                     const std::vector<double> &win, bool span_normalise,
@@ -1947,52 +1974,69 @@ static int branch_divergence_matrix(tsg_treeseq *ts, const std::vector<int32_t> 
     const int n = (int) sel.size();
     const int nsets = (int) set_off.size() - 1;
     const int W = (int) win.size() - 1;
-    const int64_t M = (int64_t) n * (n + 1) / 2;
-    if (M > (1 << 22)) return TSG_ERR_UNSUPPORTED_INPUT;
-    std::vector<double> fw;
-    int wfirst = 0;
-    if (win[0] > 0) {
-        fw.push_back(0.0);
-        wfirst = 1;
-    }
-    fw.insert(fw.end(), win.begin(), win.end());
-    if (win.back() < ts->L) fw.push_back(ts->L);
-    Query q;
-    q.mode = TSG_STAT_BRANCH;
-    q.span_normalise = false;
-    q.sets.resize(n);
-    for (int i = 0; i < n; i++) q.sets[i] = {sel[i]};
-    Block b;
-    b.stat = TSG_STAT_RELATEDNESS_NONCENTRED;
-    b.tuple = 2;
-    b.polarised = true;
-    b.idx.reserve((size_t) 2 * M);
-    for (int i = 0; i < n; i++) {
-        for (int j = i; j < n; j++) {
-            b.idx.push_back(i);
-            b.idx.push_back(j);
+    // G [W x n x n]
+    std::vector<double> G;
+    int ret;
+    if (ts->hot.usable) {
+        if ((double) W * n * (double) n >= 1e9) return TSG_ERR_UNSUPPORTED_INPUT;
+        G.assign((size_t) W * n * n, 0.0);
+        if ((ret = relvec_run(ts, n, {}, sel, win, sel, false, false, G.data()))) return ret;
+    } else {
+        const int64_t M = (int64_t) n * (n + 1) / 2;
+        if (M > (1 << 22)) return TSG_ERR_UNSUPPORTED_INPUT;
+        std::vector<double> fw;
+        int wfirst = 0;
+        if (win[0] > 0) {
+            fw.push_back(0.0);
+            wfirst = 1;
+        }
+        fw.insert(fw.end(), win.begin(), win.end());
+        if (win.back() < ts->L) fw.push_back(ts->L);
+        Query q;
+        q.mode = TSG_STAT_BRANCH;
+        q.span_normalise = false;
+        q.sets.resize(n);
+        for (int i = 0; i < n; i++) q.sets[i] = {sel[i]};
+        Block b;
+        b.stat = TSG_STAT_RELATEDNESS_NONCENTRED;
+        b.tuple = 2;
+        b.polarised = true;
+        b.idx.reserve((size_t) 2 * M);
+        for (int i = 0; i < n; i++) {
+            for (int j = i; j < n; j++) {
+                b.idx.push_back(i);
+                b.idx.push_back(j);
+            }
+        }
+        q.blocks.push_back(std::move(b));
+        q.windows = fw;
+        q.final_kind = TSG_FINAL_NONE;
+        q.P = (int) M;
+        std::vector<double> Gp((size_t) (fw.size() - 1) * M);
+        q.result = Gp.data();
+        std::vector<Query *> batch{&q};
+        if ((ret = run_batch(ts, batch))) return ret;
+        auto pair = [&](int i, int j) -> int64_t {  // (i <= j) in the output order above
+            return (int64_t) i * n - (int64_t) i * (i - 1) / 2 + (j - i);
+        };
+        G.assign((size_t) W * n * n, 0.0);
+        for (int w = 0; w < W; w++) {
+            const double *g = Gp.data() + (size_t) (w + wfirst) * M;
+            for (int i = 0; i < n; i++) {
+                for (int j = 0; j < n; j++) {
+                    G[((size_t) w * n + i) * n + j] = g[pair(std::min(i, j), std::max(i, j))];
+                }
+            }
         }
     }
-    q.blocks.push_back(std::move(b));
-    q.windows = fw;
-    q.final_kind = TSG_FINAL_NONE;
-    q.P = (int) M;
-    std::vector<double> G((size_t) (fw.size() - 1) * M);
-    q.result = G.data();
-    std::vector<Query *> batch{&q};
-    int ret = run_batch(ts, batch);
-    if (ret != 0) return ret;
-    auto pair = [&](int i, int j) -> int64_t {  // (i <= j) in the output order above
-        return (int64_t) i * n - (int64_t) i * (i - 1) / 2 + (j - i);
-    };
     std::vector<double> D((size_t) n * n);
     for (int w = 0; w < W; w++) {
-        const double *g = G.data() + (size_t) (w + wfirst) * M;
+        const double *g = G.data() + (size_t) w * n * n;
         for (int i = 0; i < n; i++) {
             for (int j = 0; j < n; j++) {
-                const int lo = std::min(i, j), hi = std::max(i, j);
-                D[(size_t) i * n + j] =
-                    (i == j) ? 0.0 : g[pair(lo, lo)] + g[pair(hi, hi)] - 2.0 * g[pair(lo, hi)];
+                D[(size_t) i * n + j] = (i == j) ? 0.0
+                                                 : g[(size_t) i * n + i] + g[(size_t) j * n + j] -
+                                                       2.0 * g[(size_t) i * n + j];
             }
         }
         const double span = win[w + 1] - win[w];
@@ -2072,9 +2116,19 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
         return branch_divergence_matrix(ts, sel, set_off, win,
                                         (options & TSG_STAT_SPAN_NORMALISE) != 0, result);
     }
+    // sites with three or more alleles get extra bit rows: their any-derived
+    // row (row s) and one row per derived allele (rows site_row2[s] ..)
+    std::vector<int32_t> site_row2(std::max(ts->num_sites, 1), -1);
+    int64_t nrows = ts->num_sites;
+    bool multi = false;
     for (int32_t s = 0; s < ts->num_sites; s++) {
-        if (ts->site_mut0[s + 1] - ts->site_mut0[s] > 1) return TSG_ERR_NOT_BIALLELIC;
+        if (ts->site_nalleles[s] >= 3) {
+            site_row2[s] = (int32_t) nrows;
+            nrows += ts->site_nalleles[s] - 1;
+            multi = true;
+        }
     }
+    if (nrows >= INT32_MAX) return TSG_ERR_UNSUPPORTED_INPUT;
     const int W = (int) win.size() - 1;
     const int n = (int) sel.size();
     if (n == 0 || nsets == 0) return 0;
@@ -2083,21 +2137,36 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
     const int ns = ts->num_samples;
     const int n_pad = ((ns + 127) / 128) * 128;
     const int NW = (ns + 31) / 32;
-    // per window: its sites, padded to a multiple of 64 columns of Gt
-    std::vector<int64_t> wk0(W), wk1(W);
-    std::vector<int32_t> sitecol;
+    // per window: the bit rows of its sites, padded to a multiple of 64
+    // columns of Gt -- sites with at most two alleles in Gt, the rows of the
+    // others in Gt2
+    std::vector<int64_t> wk0(W), wk1(W), wk0b(W), wk1b(W);
+    std::vector<int32_t> sitecol, sitecol2;
     for (int w = 0; w < W; w++) {
         const int32_t s0 = (int32_t) (std::lower_bound(ts->site_pos.begin(), ts->site_pos.end(), win[w]) -
                                       ts->site_pos.begin());
         const int32_t s1 = (int32_t) (std::lower_bound(ts->site_pos.begin(), ts->site_pos.end(), win[w + 1]) -
                                       ts->site_pos.begin());
         wk0[w] = (int64_t) sitecol.size();
-        for (int32_t s = s0; s < s1; s++) sitecol.push_back(s);
+        wk0b[w] = (int64_t) sitecol2.size();
+        for (int32_t s = s0; s < s1; s++) {
+            if (site_row2[s] < 0) {
+                sitecol.push_back(s);
+            } else {
+                sitecol2.push_back(s);
+                for (int al = 1; al < ts->site_nalleles[s]; al++) {
+                    sitecol2.push_back(site_row2[s] + al - 1);
+                }
+            }
+        }
         while (sitecol.size() % 64 != 0) sitecol.push_back(-1);
+        while (sitecol2.size() % 64 != 0) sitecol2.push_back(-1);
         wk1[w] = (int64_t) sitecol.size();
+        wk1b[w] = (int64_t) sitecol2.size();
     }
     if (sitecol.empty()) sitecol.assign(64, -1);
-    const int64_t Kp = (int64_t) sitecol.size();
+    if (sitecol2.empty()) sitecol2.assign(64, -1);
+    const int64_t Kp = (int64_t) sitecol.size(), Kp2 = (int64_t) sitecol2.size();
     std::vector<double> scale(W, 1.0);
     if (options & TSG_STAT_SPAN_NORMALISE) {
         for (int w = 0; w < W; w++) scale[w] = 1.0 / (win[w + 1] - win[w]);
@@ -2129,7 +2198,13 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
     hipStream_t s = ws->stream;
     const int slots = std::min(ts->num_chunks, 512);
     const int64_t gstride = WB ? 1 : (int64_t) ts->max_nloc * (1 + NW);
-    if ((ret = ensure(ws->gbits, (size_t) std::max(ts->num_sites, 1) * NW * 4)) ||
+    if (multi && ((ret = upload(ws->sitecol2, sitecol2, s)) || (ret = upload(ws->wk0b, wk0b, s)) ||
+                  (ret = upload(ws->wk1b, wk1b, s)) || (ret = upload(ws->row2, site_row2, s)) ||
+                  (ret = ensure(ws->gt2, (size_t) n_pad * Kp2)) ||
+                  (ret = ensure(ws->cmat2, (size_t) W * n_pad * n_pad * 4)))) {
+        return ret;
+    }
+    if ((ret = ensure(ws->gbits, (size_t) std::max<int64_t>(nrows, 1) * NW * 4)) ||
         (ret = ensure(ws->gslab, (size_t) (WB ? 1 : slots) * gstride * 4)) ||
         (ret = upload(ws->sitecol, sitecol, s)) || (ret = upload(ws->wk0, wk0, s)) ||
         (ret = upload(ws->wk1, wk1, s)) || (ret = upload(ws->scale, scale, s)) ||
@@ -2153,6 +2228,9 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
     g.num_samples = ns;
     g.NW = NW;
     g.gbits = (uint32_t *) ws->gbits.p;
+    g.mut_allele = d.mut_allele;
+    g.site_nalleles = d.site_nalleles;
+    g.site_row2 = multi ? (const int32_t *) ws->row2.p : nullptr;
     if (WB) {
         g.chunk_tree0 = hd.chunk_tree0;
         g.chunk_node0 = hd.chunk_slot0;
@@ -2193,13 +2271,28 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
         (ret = hip_check(tsg_launch_syrk_i8((const int8_t *) ws->gt.p, Kp, n_pad,
                                             (const int64_t *) ws->wk0.p,
                                             (const int64_t *) ws->wk1.p, W,
-                                            (int32_t *) ws->cmat.p, s))) ||
-        (ret = hip_check(divergence
-                             ? tsg_launch_divergence((const int32_t *) ws->cmat.p,
+                                            (int32_t *) ws->cmat.p, s)))) {
+        return ret;
+    }
+    const int32_t *C2 = nullptr;
+    if (multi) {
+        if ((ret = hip_check(tsg_launch_bits_to_gt((const uint32_t *) ws->gbits.p, NW,
+                                                   (const int32_t *) ws->sitecol2.p, Kp2, ns,
+                                                   n_pad, (int8_t *) ws->gt2.p, s))) ||
+            (ret = hip_check(tsg_launch_syrk_i8((const int8_t *) ws->gt2.p, Kp2, n_pad,
+                                                (const int64_t *) ws->wk0b.p,
+                                                (const int64_t *) ws->wk1b.p, W,
+                                                (int32_t *) ws->cmat2.p, s)))) {
+            return ret;
+        }
+        C2 = (const int32_t *) ws->cmat2.p;
+    }
+    if ((ret = hip_check(divergence
+                             ? tsg_launch_divergence((const int32_t *) ws->cmat.p, C2,
                                                      (const int32_t *) ws->sel.p, n, n_pad, W,
                                                      (const double *) ws->scale.p,
                                                      (double *) ws->kout.p, s)
-                             : tsg_launch_centre((const int32_t *) ws->cmat.p,
+                             : tsg_launch_centre((const int32_t *) ws->cmat.p, C2,
                                                  (const int32_t *) ws->sel.p,
                                                  (int64_t *) ws->rowsum.p, n, n_pad, W,
                                                  (const double *) ws->scale.p,
@@ -2301,6 +2394,187 @@ static void isolated_pairs(const tsg_treeseq *ts, std::vector<int32_t> &ms,
 // from tsk_variant_decode (genotypes.c:473-595): allele indexes in order of
 // first appearance (0 = ancestral), -1 for isolated samples unless options
 // has TSG_ISOLATED_NOT_MISSING.  Runs on the hot plan.
+namespace {
+// The relatedness-vector program (tsg_relvec.hip) for K weight columns --
+// dense weights wc [num_samples x K] (already centred or not), or identity
+// columns (ident[k] = the sample column k weighs 1 at) -- over windows win
+// (need not span the sequence), focal samples fs (sample indexes): result
+// [W x nf x K] on the host, centred over the focal nodes when `centre`,
+// span-normalised when `span_normalise`.  The hot plan must be usable.
+int relvec_run(tsg_treeseq *ts, int K, const std::vector<double> &wc,
+               const std::vector<int32_t> &ident, const std::vector<double> &win,
+               const std::vector<int32_t> &fs, bool centre, bool span_normalise,
+               double *result) {
+    const int W = (int) win.size() - 1;
+    const int64_t nf = (int64_t) fs.size();
+    const int n = ts->num_samples;
+    int ret;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if ((ret = device_upload(ts))) return ret;
+    }
+    const tsg_hotplan &hp = ts->hot;
+    if (!hp.usable) return TSG_ERR_UNSUPPORTED_INPUT;
+    Lease lease(ts);
+    Workspace *ws = lease.ws;
+    if (ws == nullptr) return TSG_ERR_DEVICE;
+    int Kb = std::min(K, TSG_RELVEC_KMAX);
+    while (Kb > 1 && tsg_relvec_shmem(hp.max_slots, Kb) > (size_t) ws->lds_limit) Kb--;
+    if (tsg_relvec_shmem(hp.max_slots, Kb) > (size_t) ws->lds_limit) {
+        return TSG_ERR_UNSUPPORTED_INPUT;
+    }
+    const int C = hp.num_chunks;
+    // per chunk: the windows it overlaps and where its segment sums go
+    std::vector<int32_t> wfirst(C), nwin(C);
+    std::vector<int64_t> ioff(C + 1, 0);
+    {
+        int m = 0;
+        for (int c = 0; c < C; c++) {
+            const double ca = ts->breakpoints[hp.chunk_tree0[c]];
+            const double cb = ts->breakpoints[hp.chunk_tree0[c + 1]];
+            while (m < W && win[m + 1] <= ca) m++;
+            int m1 = m;
+            while (m1 < W && win[m1] < cb) m1++;
+            wfirst[c] = m;
+            nwin[c] = std::max(0, m1 - m);
+            const int64_t S = hp.chunk_slot0[c + 1] - hp.chunk_slot0[c];
+            ioff[c + 1] = ioff[c] + (int64_t) nwin[c] * S * Kb;
+        }
+    }
+    const int nranges = (int) std::max<int64_t>(
+        1, std::min<int64_t>({(65536 + nf - 1) / nf, 256, (int64_t) C}));
+    const int nloc0 = (int) hp.t0_parent.size();
+    hipStream_t s = ws->stream;
+    // (the packer keeps pointers to the vectors until it sends them)
+    Pack pk;
+    const int p_w = pk.add(wc), p_id = pk.add(ident), p_win = pk.add(win);
+    const int p_wf = pk.add(wfirst), p_nw = pk.add(nwin), p_io = pk.add(ioff);
+    const int p_fs = pk.add(fs);
+    if ((ret = ensure(ws, ws->rv_E, (size_t) hp.chunk_slot0[C] * Kb * 8)) ||
+        (ret = ensure(ws, ws->rv_I, (size_t) std::max<int64_t>(ioff[C], 1) * 8)) ||
+        (ret = ensure(ws, ws->rv_z0, (size_t) std::max(nloc0, 1) * Kb * 8)) ||
+        (ret = ensure(ws, ws->rv_d0, (size_t) nf * Kb * 8)) ||
+        (ret = ensure(ws, ws->rv_esum, (size_t) nranges * nf * Kb * 8)) ||
+        (ret = ensure(ws, ws->rv_y, (size_t) W * nf * K * 8)) || (ret = pk.send(ws, s))) {
+        return ret;
+    }
+    const TsgHotDevice &h = hp.dev;
+    TsgRelvecArgs a{};
+    a.chunk_tree0 = h.chunk_tree0;
+    a.chunk_slot0 = h.chunk_slot0;
+    a.slot_par = h.slot_par;
+    a.slot_bl0 = h.slot_bl0;
+    a.chunk_lev0 = h.chunk_lev0;
+    a.lev_slot0 = h.lev_slot0;
+    a.sample_slot = h.sample_slot;
+    a.tr_off = h.tr_off;
+    a.tr_nrem = h.tr_nrem;
+    a.ev_child = h.ev_child;
+    a.ev_bl = h.ev_bl;
+    a.ev_path0 = h.ev_path0;
+    a.path = h.path;
+    a.bp = ts->dev.breakpoints;
+    a.num_trees = ts->num_trees;
+    a.num_chunks = C;
+    a.num_samples = n;
+    a.weights = wc.empty() ? nullptr : pk.dev<const double>(ws, p_w);
+    a.ident = ident.empty() ? nullptr : pk.dev<const int32_t>(ws, p_id);
+    a.K = K;
+    a.windows = pk.dev<const double>(ws, p_win);
+    a.num_windows = W;
+    a.wfirst = pk.dev<const int32_t>(ws, p_wf);
+    a.nwin = pk.dev<const int32_t>(ws, p_nw);
+    a.ioff = pk.dev<const int64_t>(ws, p_io);
+    a.focal = pk.dev<const int32_t>(ws, p_fs);
+    a.nf = (int32_t) nf;
+    a.nranges = nranges;
+    a.nloc0 = nloc0;
+    a.E = (double *) ws->rv_E.p;
+    a.I = (double *) ws->rv_I.p;
+    double *y = (double *) ws->rv_y.p;
+    if ((ret = hip_check(tsg_launch_relvec_fill(y, (int64_t) W * nf * K, s)))) return ret;
+    if (ws->profile) (void) hipEventRecord(ws->ev0, s);
+    for (int k0 = 0; k0 < K; k0 += Kb) {
+        a.k0 = k0;
+        a.Kb = std::min(Kb, K - k0);
+        if ((ret = hip_check(tsg_launch_relvec(a, tsg_relvec_shmem(hp.max_slots, a.Kb),
+                                               h.t0_parent, h.t0_bl, nloc0,
+                                               (double *) ws->rv_z0.p, (double *) ws->rv_d0.p,
+                                               (double *) ws->rv_esum.p, y, s)))) {
+            return ret;
+        }
+    }
+    if (ws->profile) (void) hipEventRecord(ws->ev1, s);
+    if ((ret = hip_check(tsg_launch_relvec_final(y, a.windows, W, (int) nf, K, centre,
+                                                 span_normalise, s))) ||
+        (ret = hip_check(hipMemcpyAsync(result, y, (size_t) W * nf * K * 8,
+                                        hipMemcpyDeviceToHost, s))) ||
+        (ret = hip_check(hipStreamSynchronize(s)))) {
+        return ret;
+    }
+    if (ws->profile) {
+        float ms = 0;
+        (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
+        ws->sweep_ms += ms;
+        ws->sweep_launches += 1;
+    }
+    return 0;
+}
+}  // namespace
+
+// genetic_relatedness_vector, branch mode (trees.c:10402-10816, entry
+// 10783): arguments in the reference's order (result before options).
+// Weights [num_samples x num_weights], centred over the samples unless
+// TSG_STAT_NONCENTRED; windows need not span the sequence; result
+// [num_windows x num_focal_nodes x num_weights], centred over the focal
+// nodes unless noncentred and span-normalised with TSG_STAT_SPAN_NORMALISE.
+// On the hot plan (tsg_relvec.hip); focal nodes must be samples.
+int tsg_treeseq_genetic_relatedness_vector(tsg_treeseq *ts, uint64_t num_weights,
+                                           const double *weights, uint64_t num_windows,
+                                           const double *windows, uint64_t num_focal_nodes,
+                                           const int32_t *focal_nodes, double *result,
+                                           uint32_t options) {
+    Use use(ts);
+    if (options & (TSG_STAT_SITE | TSG_STAT_NODE)) return TSG_ERR_UNSUPPORTED_STAT_MODE;
+    // tsk_treeseq_check_windows without the full-span rule (trees.c:1244-1286)
+    if (num_windows < 1) return TSG_ERR_BAD_NUM_WINDOWS;
+    if (windows[0] < 0 || windows[num_windows] > ts->L) return TSG_ERR_BAD_WINDOWS;
+    for (uint64_t j = 0; j < num_windows; j++) {
+        if (!(windows[j] < windows[j + 1])) return TSG_ERR_BAD_WINDOWS;
+    }
+    const int64_t nf = (int64_t) num_focal_nodes;
+    std::vector<int32_t> fs((size_t) nf);
+    for (int64_t j = 0; j < nf; j++) {  // trees.c:10491-10496
+        if (focal_nodes[j] < 0 || focal_nodes[j] >= ts->num_nodes) {
+            return TSG_ERR_NODE_OUT_OF_BOUNDS;
+        }
+    }
+    for (int64_t j = 0; j < nf; j++) {
+        fs[(size_t) j] = ts->sample_index_map[focal_nodes[j]];
+        if (fs[(size_t) j] < 0) return TSG_ERR_UNSUPPORTED_INPUT;
+    }
+    const int W = (int) num_windows;
+    const int K = (int) num_weights;
+    const int n = ts->num_samples;
+    if (K == 0 || nf == 0) return 0;
+    if ((double) W * (double) nf * (double) K >= 1e9) return TSG_ERR_UNSUPPORTED_INPUT;
+    // centre the weights (trees.c:10517-10540)
+    std::vector<double> wc(weights, weights + (size_t) n * K);
+    if (!(options & TSG_STAT_NONCENTRED)) {
+        std::vector<double> mean(K, 0.0);
+        for (int j = 0; j < n; j++) {
+            for (int k = 0; k < K; k++) mean[k] += wc[(size_t) j * K + k];
+        }
+        for (int k = 0; k < K; k++) mean[k] /= (double) n;
+        for (int j = 0; j < n; j++) {
+            for (int k = 0; k < K; k++) wc[(size_t) j * K + k] -= mean[k];
+        }
+    }
+    return relvec_run(ts, K, wc, {}, std::vector<double>(windows, windows + W + 1), fs,
+                      !(options & TSG_STAT_NONCENTRED),
+                      (options & TSG_STAT_SPAN_NORMALISE) != 0, result);
+}
+
 int tsg_treeseq_genotype_matrix(tsg_treeseq *ts, uint32_t options, int32_t *out) {
     Use use(ts);
     if (ts->num_sites == 0) return 0;

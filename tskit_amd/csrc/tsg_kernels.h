@@ -171,6 +171,13 @@ struct TsgGenoArgs {
     // genotype decode: allele index per mutation, [S x num_samples] output
     const int32_t *mut_allele;
     int32_t *geno;
+    // bit rows of sites with three or more alleles (the divergence and
+    // relatedness matrices): such a site's row in gbits is its any-derived
+    // row, and rows site_row2[s] .. + nalleles - 2 of gbits hold the rows of
+    // its derived alleles 1, 2, ...; -1 for sites with at most two alleles
+    // (null: every site has at most two)
+    const int32_t *site_row2;
+    const int32_t *site_nalleles;
 };
 
 struct TsgCombineArgs {

@@ -61,14 +61,16 @@ hipError_t tsg_launch_bits_to_gt(const uint32_t *gbits, int NW, const int32_t *s
 hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int64_t *win_k0,
                               const int64_t *win_k1, int W, int32_t *C, hipStream_t s);
 // D = (C_ii + C_jj - 2 C_ij) * scale[w] over the selected samples
-hipError_t tsg_launch_divergence(const int32_t *C, const int32_t *sel, int n, int n_pad, int W,
-                                 const double *scale, double *D, hipStream_t s);
+hipError_t tsg_launch_divergence(const int32_t *C, const int32_t *C2, const int32_t *sel, int n,
+                                 int n_pad, int W, const double *scale, double *D,
+                                 hipStream_t s);
 // divergence between sample sets (rows off[k] .. off[k + 1] of the pairwise
 // matrix D [W x n x n]), count-normalised: out [W x nsets x nsets]
 hipError_t tsg_launch_set_divergence(const double *D, int n, int W, const int32_t *off, int nsets,
                                      double *out, hipStream_t s);
-hipError_t tsg_launch_centre(const int32_t *C, const int32_t *sel, int64_t *rowsum, int n,
-                             int n_pad, int W, const double *scale, double *K, hipStream_t s);
+hipError_t tsg_launch_centre(const int32_t *C, const int32_t *C2, const int32_t *sel,
+                             int64_t *rowsum, int n, int n_pad, int W, const double *scale,
+                             double *K, hipStream_t s);
 
 // Mirrors TsgFinal in tsg_kernels.hip.
 struct TsgFinalHost {

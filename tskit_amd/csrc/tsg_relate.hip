@@ -1,14 +1,18 @@
 // Site-mode genetic relatedness matrix on the matrix cores (gfx950).
 //
-// For sites carrying at most one mutation the reference's
+// For sites with at most two alleles the reference's
 // genetic_relatedness_matrix(mode="site") (python/tskit/trees.py:8978-9102,
 // from the divergence matrix of c/tskit/trees.c:8753-8826) reduces to
 //     K = J C J / span,   C = G^T G,   J = I - 11^T / n,
 // where G is the site x sample 0/1 matrix of derived alleles: the divergence
 // matrix is D = a 1^T + 1 a^T - 2C (a = derived counts per sample) and the
-// double centring -1/2 J D J removes the a terms exactly.  C is an exact
-// integer cross product (entries <= number of sites < 2^31), computed here
-// with int8 MFMA and int32 accumulation; the centring is FP64.
+// double centring -1/2 J D J removes the a terms exactly.  A site with more
+// alleles differs between samples i and j iff d_i + d_j - d_i d_j -
+// sum_a B_ai B_aj = 1 (d its any-derived row, B_a its row of allele a >= 1),
+// so its rows d and B_a go into a second cross product C2 and, with
+// P = 2C + C2, D = (P_ii + P_jj)/2 - P_ij and K = J P J / 2.  C and C2 are
+// exact integer cross products (entries <= number of rows < 2^31), computed
+// here with int8 MFMA and int32 accumulation; the centring is FP64.
 //
 // Kernels:
 //   k_geno_bits  one wavefront per chunk: the chunk sweep with a per-node
@@ -28,6 +32,43 @@
 #define WAVE 64
 
 // --- genotype bitsets ------------------------------------------------------
+// Word w of site s's bit rows from the bitsets of its mutations' nodes
+// (below(node) = samples below it), following tsk_variant_decode's rule
+// (genotypes.c:473-595): samples start at allele 0 and each mutation, in
+// order, sets the samples below its node to its allele.  A site with at most
+// two alleles gets one row, its derived allele; a site with three or more
+// gets its any-derived row in gbits[s] and one row per derived allele at
+// gbits[site_row2[s] + allele - 1].
+template <typename Below>
+__device__ __forceinline__ void site_rows(const TsgGenoArgs &a, int s, int w, int NW,
+                                          Below below) {
+    const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
+    const int r2 = a.site_row2 ? a.site_row2[s] : -1;
+    if (r2 < 0) {
+        uint32_t b1 = 0u;
+        for (int m = m0; m < m1; m++) {
+            const uint32_t b = below(a.mut_node[m]);
+            b1 = (a.mut_allele[m] != 0) ? (b1 | b) : (b1 & ~b);
+        }
+        a.gbits[(int64_t) s * NW + w] = b1;
+        return;
+    }
+    const int na = a.site_nalleles[s];
+    uint32_t *rows = a.gbits + (int64_t) r2 * NW + w;
+    for (int al = 1; al < na; al++) rows[(int64_t) (al - 1) * NW] = 0u;
+    for (int m = m0; m < m1; m++) {
+        const uint32_t b = below(a.mut_node[m]);
+        const int al = a.mut_allele[m];
+        for (int q = 1; q < na; q++) {
+            uint32_t &r = rows[(int64_t) (q - 1) * NW];
+            r = (q == al) ? (r | b) : (r & ~b);
+        }
+    }
+    uint32_t d = 0u;
+    for (int al = 1; al < na; al++) d |= rows[(int64_t) (al - 1) * NW];
+    a.gbits[(int64_t) s * NW + w] = d;
+}
+
 __device__ void geno_bits_chunk(const TsgGenoArgs &a, int c, uint32_t *rec);
 
 // Each workgroup owns one slab and loops over chunks blockIdx.x,
@@ -84,13 +125,8 @@ __device__ void geno_bits_chunk(const TsgGenoArgs &a, int c, uint32_t *rec) {
         }
         const int s0 = a.tree_site0[t], s1 = a.tree_site0[t + 1];
         for (int s = s0; s < s1; s++) {
-            uint32_t *row = a.gbits + (int64_t) s * NW;
-            const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
-            if (m1 == m0) {
-                for (int w = lane; w < NW; w += WAVE) row[w] = 0u;
-            } else {
-                const int node = a.mut_node[m0];
-                for (int w = lane; w < NW; w += WAVE) row[w] = word(node, w);
+            for (int w = lane; w < NW; w += WAVE) {
+                site_rows(a, s, w, NW, [&](int node) { return word(node, w); });
             }
         }
     }
@@ -105,8 +141,7 @@ __device__ void geno_bits_chunk(const TsgGenoArgs &a, int c, uint32_t *rec) {
 // node of its root path and a removal clears it (the child's samples lie
 // below every path node and nowhere else among them) -- no lane reads
 // another lane's words, so the event loop needs no barrier.  Each site's row
-// is its mutation node's bitset (at most one mutation per site on this
-// route; none gives an empty row).  Grid: 8 * ceil(C / 8) x blocks.
+// rows come from its mutations' node bitsets (site_rows).  Grid: 8 * ceil(C / 8) x blocks.
 //
 // GENO: the genotype decode instead (tsk_variant_decode, genotypes.c:473-595):
 // every sample starts at allele 0 and each mutation of the site, in order,
@@ -193,8 +228,10 @@ __global__ void __launch_bounds__(WAVE) k_geno_hot(const TsgGenoArgs a) {
                     }
                 }
             } else {
-                uint32_t *row = a.gbits + (int64_t) s * a.NW + w0;
-                if (lane < nwb) row[lane] = (m1 == m0) ? 0u : bits[a.mut_node[m0] * WB + lane];
+                if (lane < nwb) {
+                    site_rows(a, s, w0 + lane, a.NW,
+                              [&](int node) { return bits[node * WB + lane]; });
+                }
             }
         }
     }
@@ -315,13 +352,23 @@ __global__ void __launch_bounds__(256) k_syrk_i8(const int8_t *gt, int64_t Kp, i
 // --- centring (FP64) ----------------------------------------------------------
 // The n selected samples (sample indexes sel[0..n)) form the matrix.
 // rowsum[w][a] = sum_b C[w][sel_a][sel_b] (exact in int64); block per (a, w).
-__global__ void __launch_bounds__(256) k_rowsum(const int32_t *C, const int32_t *sel, int n,
-                                                int n_pad, int64_t *rowsum) {
+// With sites of three or more alleles the matrices take P = 2 C + C2 (C
+// over the rows of the other sites, C2 over the any-derived and per-allele
+// rows of those sites, C2 null when there are none): two samples differ at
+// a site in (P_ii + P_jj) / 2 - P_ij sites, and the relatedness is
+// J P J / 2; without C2, P = 2C and these reduce to the biallelic forms.
+__device__ __forceinline__ int64_t cross(const int32_t *C, const int32_t *C2, int64_t idx) {
+    return C2 ? 2 * (int64_t) C[idx] + (int64_t) C2[idx] : (int64_t) C[idx];
+}
+
+__global__ void __launch_bounds__(256) k_rowsum(const int32_t *C, const int32_t *C2,
+                                                const int32_t *sel, int n, int n_pad,
+                                                int64_t *rowsum) {
     __shared__ long long part[256];
     const int i = blockIdx.x, w = blockIdx.y;
-    const int32_t *row = C + ((int64_t) w * n_pad + sel[i]) * n_pad;
+    const int64_t row = ((int64_t) w * n_pad + sel[i]) * n_pad;
     long long s = 0;
-    for (int j = threadIdx.x; j < n; j += 256) s += row[sel[j]];
+    for (int j = threadIdx.x; j < n; j += 256) s += cross(C, C2, row + sel[j]);
     part[threadIdx.x] = s;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
@@ -332,9 +379,9 @@ __global__ void __launch_bounds__(256) k_rowsum(const int32_t *C, const int32_t 
 }
 
 // K[w][i][j] = (C_ij - r_i/n - r_j/n + t/n^2) * scale[w], t = sum_i r_i.
-__global__ void __launch_bounds__(256) k_centre(const int32_t *C, const int32_t *sel,
-                                                const int64_t *rowsum, int n, int n_pad,
-                                                const double *scale, double *K) {
+__global__ void __launch_bounds__(256) k_centre(const int32_t *C, const int32_t *C2,
+                                                const int32_t *sel, const int64_t *rowsum, int n,
+                                                int n_pad, const double *scale, double *K) {
     const int64_t gid = (int64_t) blockIdx.x * 256 + threadIdx.x;
     const int w = blockIdx.y;
     if (gid >= (int64_t) n * n) return;
@@ -344,26 +391,34 @@ __global__ void __launch_bounds__(256) k_centre(const int32_t *C, const int32_t 
     // it is passed through rowsum[n * W + w] by the host-side launcher
     const double t = (double) rowsum[(int64_t) gridDim.y * n + w];
     const double dn = (double) n;
-    const double cij = (double) C[((int64_t) w * n_pad + sel[i]) * n_pad + sel[j]];
+    const double cij = (double) cross(C, C2, ((int64_t) w * n_pad + sel[i]) * n_pad + sel[j]);
     const double v = cij - (double) rs[i] / dn - (double) rs[j] / dn + t / (dn * dn);
-    K[(int64_t) w * n * n + gid] = v * scale[w];
+    K[(int64_t) w * n * n + gid] = (C2 ? 0.5 * v : v) * scale[w];
 }
 
 // Site divergence of singleton sets from the same cross products: samples i
 // and j differ at a 0/1 site iff g_i + g_j - 2 g_i g_j = 1, so the count of
 // differing sites is C_ii + C_jj - 2 C_ij (0 on the diagonal), times the
 // window's span scale (trees.c:8753-8900 with sets of one sample).
-__global__ void __launch_bounds__(256) k_divergence(const int32_t *C, const int32_t *sel, int n,
-                                                    int n_pad, const double *scale, double *D) {
+__global__ void __launch_bounds__(256) k_divergence(const int32_t *C, const int32_t *C2,
+                                                    const int32_t *sel, int n, int n_pad,
+                                                    const double *scale, double *D) {
     const int64_t gid = (int64_t) blockIdx.x * 256 + threadIdx.x;
     const int w = blockIdx.y;
     if (gid >= (int64_t) n * n) return;
     const int i = (int) (gid / n), j = (int) (gid % n);
-    const int32_t *Cw = C + (int64_t) w * n_pad * n_pad;
+    const int64_t base = (int64_t) w * n_pad * n_pad;
     const int64_t si = sel[i], sj = sel[j];
-    const double v = (i == j) ? 0.0
-                              : (double) Cw[si * n_pad + si] + (double) Cw[sj * n_pad + sj] -
-                                    2.0 * (double) Cw[si * n_pad + sj];
+    double v = 0.0;
+    if (i != j) {
+        const int64_t pii = cross(C, C2, base + si * n_pad + si);
+        const int64_t pjj = cross(C, C2, base + sj * n_pad + sj);
+        const int64_t pij = cross(C, C2, base + si * n_pad + sj);
+        // without C2: C_ii + C_jj - 2 C_ij; with it (P_ii + P_jj) / 2 - P_ij
+        // (P_ii is even: a site's any-derived row and its allele rows count
+        // each carrier once each)
+        v = C2 ? (double) ((pii + pjj) / 2 - pij) : (double) (pii + pjj - 2 * pij);
+    }
     D[(int64_t) w * n * n + gid] = v * scale[w];
 }
 
@@ -467,11 +522,12 @@ hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int
     return hipGetLastError();
 }
 
-hipError_t tsg_launch_divergence(const int32_t *C, const int32_t *sel, int n, int n_pad, int W,
-                                 const double *scale, double *D, hipStream_t s) {
+hipError_t tsg_launch_divergence(const int32_t *C, const int32_t *C2, const int32_t *sel, int n,
+                                 int n_pad, int W, const double *scale, double *D,
+                                 hipStream_t s) {
     const int64_t nn = (int64_t) n * n;
     hipLaunchKernelGGL(k_divergence, dim3((unsigned) ((nn + 255) / 256), (unsigned) W), dim3(256),
-                       0, s, C, sel, n, n_pad, scale, D);
+                       0, s, C, C2, sel, n, n_pad, scale, D);
     return hipGetLastError();
 }
 
@@ -482,13 +538,14 @@ hipError_t tsg_launch_set_divergence(const double *D, int n, int W, const int32_
     return hipGetLastError();
 }
 
-hipError_t tsg_launch_centre(const int32_t *C, const int32_t *sel, int64_t *rowsum, int n,
-                             int n_pad, int W, const double *scale, double *K, hipStream_t s) {
-    hipLaunchKernelGGL(k_rowsum, dim3((unsigned) n, (unsigned) W), dim3(256), 0, s, C, sel, n,
+hipError_t tsg_launch_centre(const int32_t *C, const int32_t *C2, const int32_t *sel,
+                             int64_t *rowsum, int n, int n_pad, int W, const double *scale,
+                             double *K, hipStream_t s) {
+    hipLaunchKernelGGL(k_rowsum, dim3((unsigned) n, (unsigned) W), dim3(256), 0, s, C, C2, sel, n,
                        n_pad, rowsum);
     hipLaunchKernelGGL(k_total, dim3((unsigned) ((W + 63) / 64)), dim3(64), 0, s, rowsum, n, W);
     const int64_t nn = (int64_t) n * n;
     hipLaunchKernelGGL(k_centre, dim3((unsigned) ((nn + 255) / 256), (unsigned) W), dim3(256), 0, s,
-                       C, sel, rowsum, n, n_pad, scale, K);
+                       C, C2, sel, rowsum, n, n_pad, scale, K);
     return hipGetLastError();
 }

@@ -9,7 +9,11 @@ sample sets and ``max_outputs`` outputs, admitted while the launch's
 estimated device memory stays within ``launch_budget`` bytes; a request that
 does not fit starts the next launch.  Allele frequency spectra run one per
 launch (they are not fused).  Up to ``max_inflight`` launches run
-concurrently on their own streams.  With ``multi=True`` the fused launches
+concurrently on their own streams, and a launch is admitted only while the
+device memory the launches in flight are estimated to need, plus its own,
+stays within ``memory_budget`` (by default the device's free memory when the
+server starts, less ``reserve_bytes``); a launch that does not fit waits
+for others to finish (one always runs, so the loop makes progress).  With ``multi=True`` the fused launches
 of a planning round are dealt into at most ``max_inflight`` jobs, and each
 job runs all of its launches -- on any of the tree sequences -- as one sweep
 launch per mode (``run_batches``).  ``one_per_launch=True, max_inflight=1``
@@ -80,11 +84,24 @@ class Request:
         return k, m, nw
 
 
-def launch_bytes(k, m, nw, chunks):
+def launch_bytes(k, m, nw, chunks, samples=0):
     """Device bytes one fused launch needs for a query of m outputs over nw
     windows on a plan of `chunks` chunks: window partials (chunks + windows)
-    x m, the combined result and finals, and the per-chunk branch sums."""
-    return 8 * m * (3 * chunks + 3 * nw) + 64 * k
+    x m, the combined result and finals, the per-chunk branch sums, and the
+    (sample, column) seeds and per-(chunk, query) window tables."""
+    return 8 * m * (3 * chunks + 3 * nw) + 64 * k + 16 * samples + 16 * chunks
+
+
+def device_free_bytes():
+    """Free device memory of the current GPU (bytes), or None without one."""
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            return int(torch.cuda.mem_get_info()[0])
+    except Exception:  # noqa: BLE001 -- no device: no memory admission
+        return None
+    return None
 
 
 class StatServer:
@@ -97,11 +114,18 @@ class StatServer:
     while launches are in flight are admitted as soon as a slot frees."""
 
     def __init__(self, tree_sequences, launch_budget=2 << 30, max_columns=64, max_outputs=4096,
-                 one_per_launch=False, max_inflight=8, multi=False):
+                 one_per_launch=False, max_inflight=8, multi=False, memory_budget=None,
+                 reserve_bytes=1 << 30):
         self.ts = list(tree_sequences)
         for ts in self.ts:
             ts.ll_tree_sequence.upload()
         self.resident_bytes = sum(ts.ll_tree_sequence.get_device_bytes() for ts in self.ts)
+        if memory_budget is None:
+            free = device_free_bytes()
+            memory_budget = None if free is None else max(0, free - int(reserve_bytes))
+        self.memory_budget = memory_budget
+        self.inflight_bytes = 0
+        self.deferred = 0  # launches that waited for device memory
         self.chunks = [ts.ll_tree_sequence.get_hot_plan_sizes()["chunks"] for ts in self.ts]
         self.launch_budget = launch_budget
         self.max_columns = max_columns
@@ -131,7 +155,7 @@ class StatServer:
             k, m, nw = r.shape()
             # a set's identity: its sorted samples, as bytes (hashed in C)
             keys = {np.sort(s).tobytes() for s in r.sets}
-            b = launch_bytes(k, m, nw, chunks)
+            b = launch_bytes(k, m, nw, chunks, sum(len(x) for x in r.sets))
             full = cur and (len(cols | keys) > self.max_columns or outs + m > self.max_outputs
                             or mem + b > self.launch_budget)
             if full or self.one_per_launch and cur:
@@ -201,27 +225,61 @@ class StatServer:
             r.t_done = t1
         return reqs
 
+    def job_bytes(self, job):
+        """Estimated device bytes of a job's launches."""
+        total = 0
+        for launch in job:
+            chunks = self.chunks[launch[0].ts_id]
+            for r in launch:
+                k, m, nw = r.shape()
+                total += launch_bytes(k, m, nw, chunks, sum(len(x) for x in r.sets))
+        return total
+
+    def admits(self, job, inflight):
+        """A job may start when its memory fits beside the jobs in flight (or
+        nothing is in flight)."""
+        if self.memory_budget is None or inflight == 0:
+            return True
+        return self.inflight_bytes + self.job_bytes(job) <= self.memory_budget
+
     def _start(self, job):
         self.launches += 1
         for launch in job:
             for r in launch:
                 r.launch = self.launches
+        need = self.job_bytes(job)
+        self.inflight_bytes += need
         if self.pool is None:
             f = concurrent.futures.Future()
             f.set_result(self._run(job))
-            return f
-        return self.pool.submit(self._run, job)
+        else:
+            f = self.pool.submit(self._run, job)
+        f.bytes = need
+        return f
+
+    def _finish(self, f):
+        self.inflight_bytes -= getattr(f, "bytes", 0)
+        return f.result()
 
     def step(self):
-        """Serves every request queued now; returns the requests completed."""
+        """Serves every request queued now, admitting jobs by slot and device
+        memory; returns the requests completed."""
         if not self.queue:
             return []
         pending = list(self.queue)
         self.queue.clear()
-        futures = [self._start(job) for job in self._plan(pending)]
-        done = []
-        for f in futures:
-            done.extend(f.result())
+        work = collections.deque(self._plan(pending))
+        inflight, done = set(), []
+        while work or inflight:
+            while work and len(inflight) < self.max_inflight and self.admits(work[0],
+                                                                             len(inflight)):
+                inflight.add(self._start(work.popleft()))
+            if work and len(inflight) < self.max_inflight:
+                self.deferred += 1
+            fin, inflight = concurrent.futures.wait(
+                inflight, return_when=concurrent.futures.FIRST_COMPLETED)
+            for f in fin:
+                done.extend(self._finish(f))
         return done
 
     def run_open_loop(self, requests, arrivals):
@@ -246,11 +304,17 @@ class StatServer:
             if free > 0 and self.queue:
                 work = self._plan(list(self.queue), free)
                 self.queue.clear()
-                for job in work[free:]:  # no slot yet: wait in the queue
-                    for launch in job:
-                        self.queue.extend(launch)
-                for job in work[:free]:
-                    inflight.add(self._start(job))
+                started = 0
+                for job in work:
+                    if started < free and self.admits(job, len(inflight)):
+                        inflight.add(self._start(job))
+                        started += 1
+                    else:  # no slot or no memory yet: wait in the queue
+                        if started < free:
+                            self.deferred += 1
+                        for launch in job:
+                            self.queue.extend(launch)
+                free -= started
             if inflight:
                 wait = None if i >= n else max(0.0, arrivals[order[i]] - now)
                 if free > 0 and wait is not None:
@@ -258,7 +322,7 @@ class StatServer:
                 done, inflight = concurrent.futures.wait(
                     inflight, timeout=wait, return_when=concurrent.futures.FIRST_COMPLETED)
                 for f in done:
-                    served += len(f.result())
+                    served += len(self._finish(f))
             elif i < n:
                 time.sleep(max(0.0, min(1e-3, arrivals[order[i]] - now)))
         return time.perf_counter() - t_begin

@@ -457,6 +457,27 @@ class LowLevelTreeSequence:
         check(err)
         return out
 
+    def genetic_relatedness_vector(self, weights, windows, mode=None, span_normalise=1, centre=1,
+                                   nodes=None):
+        """_tskitmodule.c:7197-7283 (kwlist weights, windows, mode,
+        span_normalise, centre, nodes): [W, len(nodes), num_weights]."""
+        opts = self._options(mode, span_normalise, False, centre)
+        w = _parse_windows(windows)
+        W = np.asarray(weights, dtype=np.float64)
+        if W.ndim != 2:
+            raise ValueError("weights must be a 2D array")
+        W = as_c(W, np.float64)
+        if W.shape[0] != self.get_num_samples():
+            raise ValueError("First dimension must be num_samples")
+        focal = as_c(np.asarray(nodes if nodes is not None else self.get_samples()).reshape(-1),
+                     np.int32)
+        out = np.zeros((len(w) - 1, len(focal), W.shape[1]), dtype=np.float64)
+        err = self._lib.tsg_treeseq_genetic_relatedness_vector(
+            self._h, W.shape[1], ptr(W, ctypes.c_double), len(w) - 1, ptr(w, ctypes.c_double),
+            len(focal), ptr(focal, ctypes.c_int32), ptr(out, ctypes.c_double), opts)
+        check(err)
+        return out
+
     def allele_frequency_spectrum(self, sample_set_sizes, sample_sets, windows, time_windows,
                                   mode=None, span_normalise=1, polarised=0):
         """_tskitmodule.c:6977-7062"""
@@ -903,6 +924,29 @@ class TreeSequence:
             w, sample_sets=flat, sample_set_sizes=sizes, mode=mode or "site",
             span_normalise=span_normalise)
         return D[0] if drop else D
+
+    def genetic_relatedness_vector(self, W, windows=None, mode="site", span_normalise=True,
+                                   centre=True, nodes=None):
+        """Product of the branch relatedness matrix and the weights W (one row
+        per sample), trees.py:9157-9224 and 8459-8490: [W, n, K], or [n, K]
+        without windows; ``nodes`` (samples here) needs centre=False.  Branch
+        mode only, as the reference's C (mode="site" raises its error)."""
+        if (not hasattr(W, "__len__")) or (len(W) != self.num_samples):
+            raise ValueError("First weight dimension must be equal to number of samples.")
+        W = np.asarray(W)
+        if len(W.shape) == 1:
+            W = W.reshape(W.shape[0], 1)
+        if nodes is None:
+            nodes = list(self.samples())
+        elif centre:
+            raise ValueError("If `nodes` is provided, must have centre=False.")
+        try:
+            nodes = safe_np_int_cast(nodes, np.int32)
+        except Exception:
+            raise ValueError("Could not interpret `nodes` as a list of node IDs.")
+        return self._run_windowed_stat(
+            windows, self._ll_tree_sequence.genetic_relatedness_vector, W, mode=mode,
+            span_normalise=span_normalise, centre=centre, nodes=nodes)
 
     def genetic_relatedness_matrix(self, sample_sets=None, windows=None, mode="site",
                                    span_normalise=True):
