@@ -281,6 +281,21 @@ int tsg_batch_run(tsg_batch_t *batch);
  * fit the hot plan, launches on its own.  Results equal tsg_batch_run's on
  * each batch.  Returns when every result has been written. */
 int tsg_batch_run_multi(tsg_batch_t **batches, uint64_t n);
+/* tsg_batch_run without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc, hipFree, hipHostMalloc or other allocating device routine:
+ * every device buffer of the run comes from a 64 MiB device array that each
+ * GPU the library runs on holds until the process ends, and host arrays go
+ * over from ordinary host memory.  The tree sequence must already be
+ * resident on the current device (tsg_treeseq_upload; otherwise
+ * TSG_ERR_NOT_RESIDENT = -30007), and its queries must run on the hot plan
+ * (otherwise TSG_ERR_UNSUPPORTED_INPUT); a batch needing more than the
+ * 64 MiB returns TSG_ERR_SCRATCH_TOO_SMALL (-30006) with no result written.
+ * Modes run one after the other on the null stream; results equal
+ * tsg_batch_run's.  Calls on one GPU share its array: calls that overlap
+ * on one GPU race, and the caller serialises them. */
+int tsg_batch_run_noalloc(tsg_batch_t *batch);
 int tsg_batch_diversity(tsg_batch_t *batch, uint64_t num_sample_sets,
     const uint64_t *sample_set_sizes, const int32_t *sample_sets, uint64_t num_windows,
     const double *windows, uint32_t options, double *result);

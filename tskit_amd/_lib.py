@@ -103,6 +103,7 @@ SIGNATURES = {
     "tsg_batch_get_num_queries": (ctypes.c_uint64, [c_void]),
     "tsg_batch_run": (ctypes.c_int, [c_void]),
     "tsg_batch_run_multi": (ctypes.c_int, [c_void, ctypes.c_uint64]),
+    "tsg_batch_run_noalloc": (ctypes.c_int, [c_void]),
     "tsg_treeseq_genetic_relatedness_vector": (
         ctypes.c_int,
         [c_void, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_i32p,

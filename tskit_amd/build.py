@@ -17,6 +17,7 @@ ARCH = os.environ.get("TSG_OFFLOAD_ARCH", "gfx950")
 SOURCES = [
     "tsg_kernels.hip",
     "tsg_relate.hip",
+    "tsg_noalloc.hip",
     "tsg_relvec.hip",
     "tsg_api.cpp",
     "tsg_treeseq.cpp",

@@ -71,6 +71,12 @@ struct Workspace {
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab, geno;
     // ... the rows of sites with three or more alleles
     DevBuf sitecol2, gt2, cmat2, wk0b, wk1b, row2;
+    // no-allocation runs (tsg_batch_run_fixed): device buffers are carved
+    // from a caller-provided region instead of allocated, and host arrays go
+    // over from ordinary host memory
+    char *fixed = nullptr;
+    size_t fixed_cap = 0, fixed_used = 0;
+    std::vector<char> fixed_stage;
     int lds_limit = 65536;          // dynamic LDS a sweep workgroup may use
     bool force_full = false;        // run sweeps on the full plan (testing hook)
     bool force_c32 = false;         // hot plan with 32-bit counts (testing hook)
@@ -168,10 +174,18 @@ int ensure(DevBuf &b, size_t bytes) {
     return 0;
 }
 
-// The workspace's buffer b of at least `bytes`, allocated or grown.
+// The workspace's buffer b of at least `bytes`: allocated (or grown), or,
+// in a no-allocation run, the next 256-byte-aligned piece of the fixed
+// region (TSG_ERR_SCRATCH_TOO_SMALL when it is used up).
 int ensure(Workspace *ws, DevBuf &b, size_t bytes) {
-    (void) ws;
-    return ensure(b, bytes);
+    if (ws->fixed == nullptr) return ensure(b, bytes);
+    if (bytes == 0) bytes = 8;
+    const size_t off = (ws->fixed_used + 255) & ~(size_t) 255;
+    if (off > ws->fixed_cap || bytes > ws->fixed_cap - off) return TSG_ERR_SCRATCH_TOO_SMALL;
+    b.p = ws->fixed + off;
+    b.bytes = bytes;
+    ws->fixed_used = off + bytes;
+    return 0;
 }
 
 template <typename T>
@@ -295,6 +309,16 @@ struct Pack {
     int send(Workspace *ws, hipStream_t s) const {
         int ret = ensure(ws, ws->arena, total);
         if (ret != 0) return ret;
+        if (ws->fixed != nullptr) {
+            // no pinned staging: one copy from host memory the workspace
+            // keeps until the run has synchronised
+            ws->fixed_stage.assign(total, 0);
+            for (const Part &pt : parts) {
+                if (pt.bytes) std::memcpy(ws->fixed_stage.data() + pt.off, pt.p, pt.bytes);
+            }
+            return hip_check(hipMemcpyAsync(ws->arena.p, ws->fixed_stage.data(), total,
+                                            hipMemcpyHostToDevice, s));
+        }
         if (ws->stage_bytes < total) {
             if (ws->stage_h != nullptr) (void) hipHostFree(ws->stage_h);
             ws->stage_h = nullptr;
@@ -696,6 +720,9 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
         (size_t) ts->hot.max_slots * TSG_SLOT_WORDS(K, c16, mode == TSG_STAT_BRANCH) * 4 + stage;
     const bool use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
     if (!use_hot) {
+        // the full plan is built and uploaded on first need: not in a
+        // no-allocation run
+        if (ws->fixed != nullptr) return TSG_ERR_UNSUPPORTED_INPUT;
         std::lock_guard<std::mutex> lock(g_mutex);
         if ((ret = ensure_full(ts))) return ret;
     }
@@ -963,6 +990,20 @@ int batch_finish(BatchRun &r) {
                                            r.outcount, s)))) {
         return ret;
     }
+    if (ws->fixed != nullptr) {
+        // no pinned buffer: each query's results straight into its buffer
+        for (int qi = 0; qi < r.Q; qi++) {
+            const TsgFinalHost &f = r.fin[qi];
+            const size_t n = (size_t) f.W * f.P * 8;
+            if (n == 0) continue;
+            if ((ret = hip_check(hipMemcpyAsync((*r.batch)[qi]->result,
+                                                (const double *) ws->out.p + f.outbase, n,
+                                                hipMemcpyDeviceToHost, s)))) {
+                return ret;
+            }
+        }
+        return 0;
+    }
     // all results in one copy into the workspace's pinned buffer; each
     // query's part goes to its result buffer after the stream has finished
     // (batch_deliver)
@@ -985,6 +1026,7 @@ int batch_finish(BatchRun &r) {
 
 // After r.s has finished: each query's results from the pinned copy.
 void batch_deliver(const BatchRun &r) {
+    if (r.ws->fixed != nullptr) return;  // copied by batch_finish
     for (int qi = 0; qi < r.Q; qi++) {
         const TsgFinalHost &f = r.fin[qi];
         std::memcpy((*r.batch)[qi]->result, (const double *) r.ws->out_h + f.outbase,
@@ -1443,6 +1485,10 @@ const char *tsg_strerror(int err) {
     case TSG_ERR_NO_DEVICE: return "No usable GPU device";
     case TSG_ERR_DEVICE_MEMORY: return "Device memory allocation failed";
     case TSG_ERR_UNSUPPORTED_INPUT: return "Input shape not supported by the device path";
+    case TSG_ERR_SCRATCH_TOO_SMALL:
+        return "The batch needs more device scratch than the no-allocation entry point holds";
+    case TSG_ERR_NOT_RESIDENT:
+        return "The tree sequence is not resident on the current device (call tsg_treeseq_upload)";
     default: return "Error in tree sequence input or statistics arguments";
     }
 }
@@ -2760,6 +2806,53 @@ int tsg_batch_run(tsg_batch *b) {
     // last lease first: the pool hands the same workspace to the same mode
     // next time, so its buffers are already sized for it
     while (!leases.empty()) leases.pop_back();
+    return ret;
+}
+
+// A batch run with no device or pinned allocation (tsg_batch_run_noalloc,
+// tsg_noalloc.hip): every device buffer of the run is carved from
+// [scratch, scratch + cap), a region of device memory that stays allocated
+// for the life of the process.  The tree sequence must already be resident
+// on the current device and its statistics must run on the hot plan.  The
+// modes run one after the other on the null stream; the call returns when
+// every result has been written.
+__attribute__((visibility("hidden"))) int tsg_batch_run_fixed(tsg_batch *b, void *scratch,
+                                                             size_t cap) {
+    if (b == nullptr || scratch == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+    tsg_treeseq *ts = b->ts;
+    Use use(ts);
+    int dev = 0;
+    if (hip_check(hipGetDevice(&dev))) return TSG_ERR_DEVICE;
+    Workspace w;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        const Resident *res = res_of(ts);
+        if (res == nullptr || ts->dev.device != dev || !ts->hot.usable) {
+            return TSG_ERR_NOT_RESIDENT;
+        }
+        w.lds_limit = res->lds_limit;
+        w.force_c32 = res->force_c32;
+        w.force_full = res->force_full;
+    }
+    w.fixed = (char *) scratch;
+    w.fixed_cap = cap;
+    w.stream = nullptr;
+    int ret = 0;
+    for (uint32_t mode : {TSG_STAT_BRANCH, TSG_STAT_SITE}) {
+        std::vector<Query *> group;
+        for (auto &q : b->queries) {
+            if (q->mode == mode) group.push_back(q.get());
+        }
+        if (group.empty()) continue;
+        w.fixed_used = 0;
+        BatchRun r;
+        ret = batch_prepare(ts, group, &w, w.stream, r);
+        if (ret == 0) ret = batch_sweep(r);
+        if (ret == 0) ret = batch_finish(r);
+        const int sret = hip_check(hipStreamSynchronize(w.stream));
+        if (ret == 0) ret = sret;
+        if (ret != 0) break;
+    }
     return ret;
 }
 

@@ -80,6 +80,8 @@
 #define TSG_ERR_NO_DEVICE (-30002)       // no usable GPU
 #define TSG_ERR_DEVICE_MEMORY (-30003)   // device allocation failed / over budget
 #define TSG_ERR_UNSUPPORTED_INPUT (-30004)  // input shape this path does not take
+#define TSG_ERR_SCRATCH_TOO_SMALL (-30006)  // no-allocation run: scratch used up
+#define TSG_ERR_NOT_RESIDENT (-30007)       // no-allocation run: handle not uploaded
 
 // --- summary-function registry ----------------------------------------------
 // One id per reference summary function (trees.c:3934-5291).  Each stat
