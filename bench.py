@@ -353,12 +353,25 @@ def replicate_phase(om, with_cpu, inflight=4):
 
     import tskit_amd as ta
 
+    from tskit_amd import autotune
+
     t0 = time.perf_counter()
     reps = replicate_workload()
     for r in reps:
         r["ts"] = r["tc"].tree_sequence()
-        r["ts"].ll_tree_sequence.upload()
     t_setup = time.perf_counter() - t0
+    # chunk length by the autotuner: the first replicate of each shape bucket
+    # is tuned, the others take its setting
+    t0 = time.perf_counter()
+    table = {}
+    for r in reps:
+        if autotune.shape_key(r["ts"]) not in table:
+            autotune.tune(r["ts"], reps=2, table=table)
+        autotune.apply(r["ts"], table=table)
+        r["ts"].ll_tree_sequence.upload()
+    t_tune = time.perf_counter() - t0
+    log(f"replicates: autotuned {len(table)} shape buckets in {t_tune:.1f}s: "
+        + ", ".join(f"{k}:{v['events_per_chunk']}" for k, v in sorted(table.items())))
     edges = sum(r["tc"].num_edges for r in reps)
     updates = 0
     for r in reps:
@@ -416,16 +429,30 @@ def replicate_phase(om, with_cpu, inflight=4):
             pend.append(pm)
         return bs, pend
 
-    bs, _ = multi_batches()
-    ta.run_batches(bs[:8])
+    # the single launch over all replicates is tuned jointly (its total work
+    # decides, not one replicate's latency); a different chunking rounds
+    # differently, so it is compared with the batched results to a tolerance
+    t0 = time.perf_counter()
+    joint = autotune.tune_batch([r["ts"] for r in reps],
+                                lambda: ta.run_batches(multi_batches()[0]))
+    t_tune_multi = time.perf_counter() - t0
     bs, pend = multi_batches()
     torch.cuda.synchronize()
     t = time.perf_counter()
     ta.run_batches(bs)
     torch.cuda.synchronize()
     t_multi = time.perf_counter() - t
-    multi_ok = all(np.array_equal(np.asarray(pend[i][k].result()), np.asarray(results[i][k]))
-                   for i in range(len(reps)) for k in pend[i])
+    multi_diff = 0.0
+    for i in range(len(reps)):
+        for k in pend[i]:
+            a = np.asarray(pend[i][k].result(), dtype=np.float64)
+            b = np.asarray(results[i][k], dtype=np.float64)
+            fin = np.isfinite(b)
+            scale = max(float(np.max(np.abs(b[fin]))), 1e-300) if np.any(fin) else 1.0
+            if not np.array_equal(np.isfinite(a), fin):
+                multi_diff = float("inf")
+            elif np.any(fin):
+                multi_diff = max(multi_diff, float(np.max(np.abs(a[fin] - b[fin]))) / scale)
     out = {"replicates": len(reps), "edges": edges,
            "samples_min_max": [int(min(len(r["ts"].samples()) for r in reps)),
                                int(max(len(r["ts"].samples()) for r in reps))],
@@ -434,11 +461,19 @@ def replicate_phase(om, with_cpu, inflight=4):
            "queries_per_s_batched": sum(2 * (2 if r["pairs"] else 1) for r in reps) / t_batched,
            "form": f"one fused launch per replicate and mode, {inflight} in flight",
            "multi_s": t_multi, "multi_speedup": t_plain / t_multi,
-           "multi_form": "one sweep launch per mode over all replicates",
-           "multi_equals_batched": bool(multi_ok)}
+           "multi_form": "one sweep launch per mode over all replicates, one launch per "
+                         "finish phase over all of them, one copy back",
+           "tuned_events_per_chunk": {k: v["events_per_chunk"] for k, v in table.items()},
+           "tune_s": round(t_tune, 2),
+           "multi_events_per_chunk": joint["events_per_chunk"],
+           "multi_tune_seconds": joint["seconds"], "multi_tune_s": round(t_tune_multi, 2),
+           "multi_max_diff_vs_batched": multi_diff}
     log(f"replicates: {len(reps)} tree sequences ({edges} edges) plain {t_plain * 1e3:.0f} ms, "
         f"batched {t_batched * 1e3:.0f} ms ({t_plain / t_batched:.2f}x), one launch per mode "
-        f"{t_multi * 1e3:.0f} ms ({t_plain / t_multi:.2f}x, equal to batched: {multi_ok})")
+        f"{t_multi * 1e3:.0f} ms ({t_plain / t_multi:.2f}x; jointly tuned "
+        f"{joint['events_per_chunk']} events per chunk: "
+        + ", ".join(f"{k}: {v * 1e3:.0f} ms" for k, v in joint["seconds"].items())
+        + f"; max diff vs batched {multi_diff:.2e})")
     if with_cpu:
         worst = 0.0
         for i in range(0, len(reps), 32):
@@ -558,6 +593,8 @@ def main():
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "`bench.py --no-serve --no-cpu --copy-calibration` into "
                          "profiles/k_sweep_traffic.json and exit")
+    ap.add_argument("--dump", metavar="NPZ",
+                    help="write the last timed step's statistics to an .npz file")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the steps, copy 1 GiB device to device three times (the "
                          "reference kernel that scales FETCH_SIZE / WRITE_SIZE counters)")
@@ -628,6 +665,10 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     gc.enable()
+    if args.dump:
+        np.savez(args.dump, **{f"{name}_{mode}": np.asarray(v)
+                               for mode, vals in gpu_out.items()
+                               for name, v in zip(("Fst", "Tajimas_D", "f4"), vals)})
     sweep_ms, launches = ll.get_profile()
     per_mode = ll.get_profile_modes()
     ll.set_profiling(False)

@@ -958,3 +958,32 @@ def test_branch_divergence_matrix_wide(require_gpu):
     exp = to._branch_divergence_from_shared(G, 30, fw, [[i] for i in range(30)])[1:3]
     got = D[:, pick][:, :, pick]
     assert np.allclose(got, exp, rtol=1e-9, atol=1e-12 * np.max(np.abs(exp)))
+
+
+def test_autotune(require_gpu, tmp_path):
+    """The chunk-count autotuner times its candidates on a tree sequence,
+    records the fastest per shape in its JSON cache and applies it to
+    another tree sequence of the same shape; statistics on the tuned plan
+    equal the oracle's."""
+    import json
+
+    from tskit_amd import autotune
+
+    path = str(tmp_path / "tune.json")
+    tc = cases.generated(300, 3e5, 71)
+    ts = tc.tree_sequence()
+    _say("autotune")
+    got = autotune.tune(ts, candidates=(64, 256), reps=2, path=path)
+    assert got["events_per_chunk"] in (64, 256)
+    assert ts.ll_tree_sequence.get_hot_plan_sizes()["chunks"] == got["chunks"]
+    table = json.load(open(path))
+    assert autotune.shape_key(ts) in table
+    ts2 = cases.generated(300, 3e5, 71).tree_sequence()
+    assert autotune.apply(ts2, path=path) == got["events_per_chunk"]
+    assert ts2.ll_tree_sequence.get_hot_plan_sizes()["chunks"] == got["chunks"]
+    orc = Oracle(tc)
+    sets = cases.sample_sets(300, 3, 40, 2)
+    w = np.linspace(0, 3e5, 9)
+    for mode in ("branch", "site"):
+        _check("diversity", ts2.diversity(sets, windows=w, mode=mode),
+               orc.stat("diversity", sets, None, w, mode))

@@ -55,8 +55,11 @@ struct Workspace {
     // first tree: counts and per-block running-sum partials
     DevBuf x0, r0part, bsum;
     // one launch over several batches' sweeps: their argument blocks and
-    // chunk prefixes
-    DevBuf margs, mcpre;
+    // chunk prefixes; their finish phases (run blocks, block prefixes) and
+    // results, copied back in one transfer
+    DevBuf margs, mcpre, mfin, mpre, mout;
+    void *mout_h = nullptr;
+    size_t mout_h_bytes = 0;
     // a batch's host arrays, packed into one pinned buffer and sent with one
     // copy into one device buffer
     DevBuf arena;
@@ -521,6 +524,10 @@ struct BatchRun {
     const void *fin_dev = nullptr;
     const int32_t *ipar_dev = nullptr;
     const double *dpar_dev = nullptr;
+    // results in host memory after the run has synchronised: the workspace's
+    // pinned buffer, or (several runs finished together) this run's part of
+    // a shared one
+    const double *results_h = nullptr;
 };
 
 // The fused state columns and outputs of a batch: one column per distinct
@@ -1026,6 +1033,13 @@ int batch_finish(BatchRun &r) {
 
 // After r.s has finished: each query's results from the pinned copy.
 void batch_deliver(const BatchRun &r) {
+    if (r.results_h != nullptr) {
+        for (int qi = 0; qi < r.Q; qi++) {
+            const TsgFinalHost &f = r.fin[qi];
+            std::memcpy((*r.batch)[qi]->result, r.results_h + f.outbase, (size_t) f.W * f.P * 8);
+        }
+        return;
+    }
     if (r.ws->fixed != nullptr) return;  // copied by batch_finish
     for (int qi = 0; qi < r.Q; qi++) {
         const TsgFinalHost &f = r.fin[qi];
@@ -1366,7 +1380,8 @@ void free_one_workspace(Workspace *ws) {
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
                       &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum, &ws->scale,
                       &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->geno, &ws->x0,
-                      &ws->r0part, &ws->bsum, &ws->margs, &ws->mcpre, &ws->arena,
+                      &ws->r0part, &ws->bsum, &ws->margs, &ws->mcpre, &ws->mfin, &ws->mpre,
+                      &ws->mout, &ws->arena,
                       &ws->rv_E, &ws->rv_I, &ws->rv_z0, &ws->rv_d0, &ws->rv_esum, &ws->rv_y,
                       &ws->sitecol2, &ws->gt2, &ws->cmat2, &ws->wk0b, &ws->wk1b, &ws->row2};
     for (DevBuf *b : bufs) {
@@ -1374,6 +1389,7 @@ void free_one_workspace(Workspace *ws) {
     }
     if (ws->stage_h != nullptr) (void) hipHostFree(ws->stage_h);
     if (ws->out_h != nullptr) (void) hipHostFree(ws->out_h);
+    if (ws->mout_h != nullptr) (void) hipHostFree(ws->mout_h);
     if (ws->ev0) (void) hipEventDestroy(ws->ev0);
     if (ws->ev1) (void) hipEventDestroy(ws->ev1);
     if (ws->stream) (void) hipStreamDestroy(ws->stream);
@@ -2080,15 +2096,17 @@ static int branch_divergence_matrix(tsg_treeseq *ts, const std::vector<int32_t> 
         const double *g = G.data() + (size_t) w * n * n;
         for (int i = 0; i < n; i++) {
             for (int j = 0; j < n; j++) {
+                // G from the relatedness vector is symmetric only to rounding:
+                // both off-diagonal entries enter, so D is exactly symmetric
                 D[(size_t) i * n + j] = (i == j) ? 0.0
                                                  : g[(size_t) i * n + i] + g[(size_t) j * n + j] -
-                                                       2.0 * g[(size_t) i * n + j];
+                                                       (g[(size_t) i * n + j] + g[(size_t) j * n + i]);
             }
         }
         const double span = win[w + 1] - win[w];
         double *out = result + (size_t) w * nsets * nsets;
         for (int A = 0; A < nsets; A++) {
-            for (int B = 0; B < nsets; B++) {
+            for (int B = A; B < nsets; B++) {  // (B, A) mirrors (A, B)
                 double v = 0.0;
                 for (int i = set_off[A]; i < set_off[A + 1]; i++) {
                     for (int j = set_off[B]; j < set_off[B + 1]; j++) v += D[(size_t) i * n + j];
@@ -2096,7 +2114,8 @@ static int branch_divergence_matrix(tsg_treeseq *ts, const std::vector<int32_t> 
                 const double na = set_off[A + 1] - set_off[A], nb = set_off[B + 1] - set_off[B];
                 const double denom = (A == B) ? na * (na - 1) : na * nb;
                 if (!(A == B && denom == 0)) v /= denom;
-                out[(size_t) A * nsets + B] = span_normalise ? v / span : v;
+                out[(size_t) A * nsets + B] = out[(size_t) B * nsets + A] =
+                    span_normalise ? v / span : v;
             }
         }
     }
@@ -2964,14 +2983,74 @@ int run_multi(tsg_batch **bs, uint64_t n, std::vector<std::unique_ptr<Lease>> &l
             coff += nrun + 1;
         }
     }
-    // the per-run combine, finals and copies on the runs' own streams, after
-    // the sweeps
-    if ((ret = hip_check(hipEventRecord(run_ws[0]->ev1, s0)))) return ret;
+    // the finish phases of every run (first-tree sums, chunk scan, window
+    // combine, finals), one launch per phase over all runs, after the
+    // sweeps; every run's results into one device buffer, copied back in one
+    // transfer
+    Workspace *ws0 = run_ws[0];
+    std::vector<TsgFinishRun> fr(R);
+    std::vector<int64_t> pre((size_t) TSG_FINISH_PHASES * (R + 1), 0), totals(TSG_FINISH_PHASES);
+    std::vector<int64_t> outoff(R + 1, 0);
+    for (size_t i = 0; i < R; i++) outoff[i + 1] = outoff[i] + std::max<int64_t>(runs[i].outcount, 0);
+    if ((ret = ensure(ws0->mout, (size_t) std::max<int64_t>(outoff[R], 1) * 8))) return ret;
     for (size_t i = 0; i < R; i++) {
-        BatchRun &r = runs[i];
-        r.s = run_ws[i]->stream;
-        if (i > 0 && (ret = hip_check(hipStreamWaitEvent(r.s, run_ws[0]->ev1, 0)))) return ret;
-        if ((ret = batch_finish(r))) return ret;
+        const BatchRun &r = runs[i];
+        Workspace *ws = run_ws[i];
+        TsgFinishRun &f = fr[i];
+        const bool branch = r.mode == TSG_STAT_BRANCH;
+        f.a = r.a;
+        f.ca = r.ca;
+        f.t0_parent = r.t0_parent;
+        f.t0_bl = r.t0_bl;
+        f.x0 = (int32_t *) ws->x0.p;
+        f.r0part = (double *) ws->r0part.p;
+        f.bsum = (double *) ws->bsum.p;
+        f.Rstart = (double *) ws->Rstart.p;
+        f.dR = branch ? r.a.dR : nullptr;
+        f.nloc = r.nloc0;
+        f.nb = r.nb0;
+        f.nblk = tsg_scan_blocks(r.C);
+        f.lanecount = r.lanecount;
+        f.fin = r.fin_dev;
+        f.nq = r.Q;
+        f.raw = (const double *) ws->raw.p;
+        f.ipar = r.ipar_dev;
+        f.dpar = r.dpar_dev;
+        f.out = (double *) ws0->mout.p + outoff[i];
+        f.outcount = r.outcount;
+        int64_t blocks[TSG_FINISH_PHASES];
+        tsg_finish_blocks(f, blocks);
+        for (int ph = 0; ph < TSG_FINISH_PHASES; ph++) {
+            int64_t *p = pre.data() + (size_t) ph * (R + 1);
+            p[i + 1] = p[i] + blocks[ph];
+        }
+    }
+    for (int ph = 0; ph < TSG_FINISH_PHASES; ph++) {
+        totals[ph] = pre[(size_t) ph * (R + 1) + R];
+    }
+    const size_t bytes = (size_t) std::max<int64_t>(outoff[R], 1) * 8;
+    if (ws0->mout_h_bytes < bytes) {
+        if (ws0->mout_h != nullptr) (void) hipHostFree(ws0->mout_h);
+        ws0->mout_h = nullptr;
+        ws0->mout_h_bytes = 0;
+        const size_t want = std::max(bytes + bytes / 2, (size_t) 1 << 16);
+        if (hipHostMalloc(&ws0->mout_h, want) != hipSuccess) {
+            (void) hipGetLastError();
+            return TSG_ERR_NO_MEMORY;
+        }
+        ws0->mout_h_bytes = want;
+    }
+    if ((ret = upload(ws0->mfin, fr, s0)) || (ret = upload(ws0->mpre, pre, s0)) ||
+        (ret = hip_check(tsg_launch_finish_multi((const TsgFinishRun *) ws0->mfin.p,
+                                                 (const int64_t *) ws0->mpre.p, totals.data(),
+                                                 (int) R, s0))) ||
+        (ret = hip_check(hipMemcpyAsync(ws0->mout_h, ws0->mout.p, (size_t) outoff[R] * 8,
+                                        hipMemcpyDeviceToHost, s0)))) {
+        return ret;
+    }
+    for (size_t i = 0; i < R; i++) {
+        runs[i].s = s0;
+        runs[i].results_h = (const double *) ws0->mout_h + outoff[i];
     }
     return 0;
 }

@@ -31,7 +31,9 @@ struct TsgQuery {
 // Chunks per block of the device sample map: one sample's slots in 64
 // consecutive chunks share a 128-byte line, which the chunks' workgroups --
 // placed on one XCD in chunk order -- read through that XCD's L2.
+#ifndef TSG_SS_CB
 #define TSG_SS_CB 64
+#endif
 
 // Hops of an event's root path staged in LDS at once by the sweep kernel.
 #define TSG_PSEG 16
@@ -196,3 +198,30 @@ struct TsgCombineArgs {
     const int64_t *lanebase;     // per query: first combine lane (multiples of 64)
     uint32_t flags;
 };
+
+// One run's share of a finish phase launched over many runs at once
+// (tsg_launch_finish_multi): the first tree's running sum (branch), the
+// chunk scan and window combine, the finals.
+struct TsgFinishRun {
+    TsgSweepArgs a;
+    TsgCombineArgs ca;
+    const int32_t *t0_parent;
+    const double *t0_bl;
+    int32_t *x0;
+    double *r0part, *bsum, *Rstart;
+    const double *dR;         // null: site mode (no running sums)
+    int32_t nloc, nb, nblk;
+    int64_t lanecount;        // combine lanes
+    const void *fin;          // TsgFinal per query
+    int32_t nq;
+    const double *raw;
+    const int32_t *ipar;
+    const double *dpar;
+    double *out;
+    int64_t outcount;
+};
+
+// Block-count prefixes of the phases of tsg_launch_finish_multi, per run
+// (each [nrun + 1]): fill, tree0 counts, tree0 sums, scan blocks, scan top,
+// scan apply, combine, finals.
+#define TSG_FINISH_PHASES 8

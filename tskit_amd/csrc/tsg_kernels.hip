@@ -37,6 +37,8 @@
 //     added) carries the reference's NaN into the sums.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <type_traits>
 
 #include "tsg_common.h"
@@ -178,6 +180,24 @@ struct Slots {
     }
 };
 
+// The sweep's workgroup is one wavefront.  Its lanes run in lockstep and the
+// LDS executes one wavefront's instructions in issue order, so an LDS write
+// is seen by every later LDS read of the wavefront without a hardware
+// barrier; wave_sync only keeps the compiler from moving memory operations
+// across the point.  Node records in HBM (the full plan's global store) take
+// a real barrier.
+#ifndef TSG_SWEEP_SYNC
+#define TSG_SWEEP_SYNC 1
+#endif
+__device__ __forceinline__ void wave_sync() {
+#if TSG_SWEEP_SYNC
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+#else
+    __syncthreads();
+#endif
+}
+
 // Materialise the chunk's start tree and seed the counts: each weighted
 // sample walks its root path once, adding 1 to every column it belongs to.
 __device__ void init_chunk(const TsgSweepArgs &a, const Nodes &nd, int64_t n0, int nloc,
@@ -222,7 +242,7 @@ __device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c,
     const int64_t s0 = a.chunk_node0[c];
     const int words = nslot * nd.R;
     for (int i = lane; i < words; i += WAVE) nd.rec[i] = 0;
-    __syncthreads();
+    wave_sync();
     if constexpr (BL) {
         for (int h = lane; h < nslot; h += WAVE) nd.set_bl(h, a.loc_bl0[s0 + h]);
     }
@@ -248,7 +268,7 @@ __device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c,
             if (ph[r] != TSG_NO_SLOT) nd.add((int) ph[r], pc[r], 1);
         }
     }
-    __syncthreads();
+    wave_sync();
     const int32_t *lev = a.lev_slot0 + a.chunk_lev0[c];
     const int nlev = (int) (a.chunk_lev0[c + 1] - a.chunk_lev0[c]) - 1;
     // the last level holds the roots of the hot forest: nothing above them
@@ -260,7 +280,7 @@ __device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c,
             const int32_t v = nd.get(h, k);
             if (p >= 0 && v != 0) nd.add(p, k, v);
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -276,7 +296,8 @@ struct LaneOut {
     // how a walk's contribution is formed (see k_sweep): TSG_KIND_POLY, the
     // summary (unpolarised wrapper included) as a polynomial in the counts
     // of four column slots v_s = x[pc[s]]:
-    //   F = c0 + sum_s l1[s] v_s + sum_{s<=t} q[qidx(s,t)] v_s v_t + c3 v_0^3;
+    //   F = c0 + l1.v + v'Hv / 2 + c3 v_0^3,
+    // H symmetric, its entry (s, t), s <= t, in q[qidx(s,t)];
     // TSG_KIND_SEG, segregating sites from per-column indicator sums;
     // TSG_KIND_HOP, evaluated hop by hop
     int kind;
@@ -284,9 +305,8 @@ struct LaneOut {
     double c0, c3;
     double l1[4];
     double q[10];
-    // TSG_KIND_SEG: column, set size, polarised, 1 / denominator
-    int si, sn, spol;
-    double sinv;
+    // TSG_KIND_SEG reuses the slots: column pc[0], 1 / denominator c0,
+    // polarised pc[1] (the set size is the column total)
 };
 
 #define TSG_KIND_HOP 0
@@ -437,51 +457,85 @@ __device__ int moment_form(const TsgOut &o, int K, LaneOut &L) {
     default:
         return TSG_KIND_HOP;
     }
+    // q holds the coefficients of v_s v_t (s <= t); H has the off-diagonal
+    // ones and twice the diagonal ones
     const double inv = o.inv;
     L.c0 *= inv;
     L.c3 *= inv;
     for (int s = 0; s < 4; s++) l1[s] *= inv;
     for (int s = 0; s < 10; s++) q[s] *= inv;
+    for (int s = 0; s < 4; s++) q[qidx(s, s)] *= 2;
     return TSG_KIND_POLY;
+}
+
+// The parts of F at the slot counts v = x[pc] (TSG_KIND_POLY) that a walk
+// reuses: A = l1.v + c3 v_0^3, u = H v and B = v'Hv / 2, so F = c0 + A + B.
+struct PolyParts {
+    double A, B, v0;
+    double u[4];
+};
+
+__device__ __forceinline__ PolyParts poly_parts(const LaneOut &L, const int32_t *x) {
+    PolyParts P;
+    double v[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) v[s] = (double) x[L.pc[s]];
+    double bq = 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        double us = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) us = fma(L.q[s <= t ? qidx(s, t) : qidx(t, s)], v[t], us);
+        P.u[s] = us;
+        bq = fma(v[s], us, bq);
+    }
+    double a = L.c3 * v[0] * v[0] * v[0];
+#pragma unroll
+    for (int s = 0; s < 4; s++) a = fma(L.l1[s], v[s], a);
+    P.A = a;
+    P.B = 0.5 * bq;
+    P.v0 = v[0];
+    return P;
 }
 
 // F at the counts x (TSG_KIND_POLY)
 __device__ __forceinline__ double poly_value(const LaneOut &L, const int32_t *x) {
-    double v[4];
-#pragma unroll
-    for (int s = 0; s < 4; s++) v[s] = (double) x[L.pc[s]];
-    double f = L.c0 + L.c3 * v[0] * v[0] * v[0];
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-        f += L.l1[s] * v[s];
-#pragma unroll
-        for (int t = s; t < 4; t++) f += L.q[qidx(s, t)] * v[s] * v[t];
-    }
-    return f;
+    const PolyParts P = poly_parts(L, x);
+    return L.c0 + P.A + P.B;
 }
 
-// sum_h bl_h (F(x_h + d) - F(x_h)) from the path sums S0, S1[k] = sum bl x[k]
-// and S2[k] = sum bl x[k]^2; d[s] = the signed moved count of slot s:
-//   S0 (l1.d + d'Qd + c3 d0^3) + sum_{s<=t} q_st (d_t S1_s + d_s S1_t)
-//   + c3 (3 d0^2 S1_0 + 3 d0 S2_0)
-__device__ __forceinline__ double poly_walk(const LaneOut &L, const double *d, double s0,
-                                            const double *s1v, const double *s2v) {
-    double sv[4];
+// sum_h bl_h (F(x_h + d) - F(x_h)) for the moved counts d = sg v (sg = +-1)
+// from the path sums S0 = sum bl, S1[k] = sum bl x[k], S2[k] = sum bl x[k]^2:
+// F(x + d) - F(x) = l1.d + d'Hd / 2 + x.Hd + c3 (d_0^3 + 3 x_0 d_0^2 +
+// 3 x_0^2 d_0), so the sum is
+//   S0 (sg A + B) + sg S1.u + 3 c3 (v_0^2 S1_0 + sg v_0 S2_0).
+__device__ __forceinline__ double poly_walk(const LaneOut &L, const PolyParts &P, double sg,
+                                            double s0, const double *s1v, const double *s2v,
+                                            bool c3) {
+    double g = 0.0;
 #pragma unroll
-    for (int s = 0; s < 4; s++) sv[s] = s1v[L.pc[s]];
-    double lin = L.c3 * d[0] * d[0] * d[0];
-    double grad = L.c3 * (3 * d[0] * d[0] * sv[0] + 3 * d[0] * s2v[L.pc[0]]);
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-        lin += L.l1[s] * d[s];
-#pragma unroll
-        for (int t = s; t < 4; t++) {
-            const double qst = L.q[qidx(s, t)];
-            lin += qst * d[s] * d[t];
-            grad += qst * (d[t] * sv[s] + d[s] * sv[t]);
-        }
+    for (int s = 0; s < 4; s++) g = fma(s1v[L.pc[s]], P.u[s], g);
+    double w = fma(s0, fma(sg, P.A, P.B), sg * g);
+    if (c3) {
+        const int p = L.pc[0];
+        w = fma(3.0 * L.c3, P.v0 * fma(P.v0, s1v[p], sg * s2v[p]), w);
     }
-    return s0 * lin + grad;
+    return w;
+}
+
+// Sum of v over the G hop groups (G a power of two) of the count-update
+// layout, lane = g K + k: lanes of group 0 receive each column's sum, in a
+// fixed order.
+__device__ __forceinline__ double group_sum(double v, int K, int G) {
+    for (int gs = G >> 1; gs >= 1; gs >>= 1) v += __shfl_down(v, gs * K);
+    return v;
+}
+
+// Segregating-sites numerators as integers: polarised [x > 0] (n - x),
+// unpolarised n [0 < x < n] (seg_pol, seg_unpol)
+__device__ __forceinline__ int32_t iseg_pol(int32_t x, int32_t n) { return x > 0 ? n - x : 0; }
+__device__ __forceinline__ int32_t iseg_unpol(int32_t x, int32_t n) {
+    return (x > 0 && x < n) ? n : 0;
 }
 
 template <int MPL>
@@ -509,10 +563,11 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *gout, int c, 
             L.slot0 = q.pbase + a.cq_poff[cq] * (int64_t) q.M + (m - q.mbase);
             L.mask = summary_columns(o, a.K);
             L.kind = moment_form(o, a.K, L);
-            L.si = o.i;
-            L.sn = o.ni;
-            L.spol = o.flags & 1;
-            L.sinv = o.inv;
+            if (L.kind == TSG_KIND_SEG) {
+                L.pc[0] = o.i;
+                L.pc[1] = o.flags & 1;
+                L.c0 = o.inv;
+            }
         }
     }
 }
@@ -567,6 +622,14 @@ __device__ __forceinline__ double branch_summary(const TsgOut &o, const int32_t 
 template <int MODE, int STORE, int MPL>
 __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     extern __shared__ int32_t smem[];
+#define SWEEP_SYNC()                                     \
+    do {                                                 \
+        if constexpr (STORE == TSG_STORE_GLOBAL) {       \
+            __syncthreads();                             \
+        } else {                                         \
+            wave_sync();                                 \
+        }                                                \
+    } while (0)
     const int lane = threadIdx.x;
     const int K = a.K;
     const int R = 3 + K;
@@ -622,19 +685,35 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     LaneOut lo[MPL];
     setup_outputs<MPL>(a, gout, c, lane, lo);
     int wfirst[MPL];
-    bool any_poly = false, any_c3 = false, any_seg = false;
+    bool any_poly = false, any_c3 = false, any_segp = false, any_segu = false, any_hop = false;
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
         wfirst[j] = lo[j].wi;
         if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(gout[lo[j].lm], zero, tot);
         any_poly |= lo[j].kind == TSG_KIND_POLY;
         any_c3 |= lo[j].kind == TSG_KIND_POLY && lo[j].c3 != 0;
-        any_seg |= lo[j].kind == TSG_KIND_SEG;
+        any_segp |= lo[j].kind == TSG_KIND_SEG && lo[j].pc[1];
+        any_segu |= lo[j].kind == TSG_KIND_SEG && !lo[j].pc[1];
+        any_hop |= lo[j].lm >= 0 && lo[j].kind == TSG_KIND_HOP;
     }
     any_poly = MODE == 0 && __any(any_poly);
     any_c3 = MODE == 0 && __any(any_c3);
-    any_seg = MODE == 0 && __any(any_seg);
+    any_segp = MODE == 0 && __any(any_segp);
+    any_segu = MODE == 0 && __any(any_segu);
+    any_hop = MODE == 0 && __any(any_hop);
+    const bool any_seg = any_segp || any_segu;
     const bool any_agg = any_poly || any_seg;
+    // count-update layout: with K <= 64 columns, G hop groups of K lanes
+    // (lane = g K + k, G the largest power of two with G K <= 64) update G
+    // hops of a path at once and accumulate their column's path sums in
+    // registers; with more columns one group strides over them.  The
+    // path-sum kinds (POLY, SEG) only occur with K <= 64.
+    const bool kwide = K > WAVE || K < 1;
+    const int G = kwide ? 1 : 1 << (31 - __clz(WAVE / K));
+    const int ug = kwide ? 0 : lane / K;
+    const int uk = kwide ? lane : lane - ug * K;
+    const bool uon = ug < G;
+    const int32_t totk = (!kwide && uon) ? tot[uk] : 0;
 
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
     // the chunk's events: transitions t0+1 .. t1 (the last is the closing
@@ -646,7 +725,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     // tb .. tb + TB - 1 of tr_nrem (never past index num_trees)
     int tb = -1;
     auto stage_trees = [&](int t) {
-        __syncthreads();
+        SWEEP_SYNC();
         tb = t;
         for (int l = lane; l <= TSG_TB; l += WAVE) {
             const int x = t + l;
@@ -657,7 +736,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                 if (l < TSG_TB) tnrem[l] = (x < a.num_trees) ? a.tr_nrem[x] : 0;
             }
         }
-        __syncthreads();
+        SWEEP_SYNC();
     };
     auto tree = [&](int t) {  // make entries t and t + 1 resident
         if (tb < 0 || t < tb || t + 1 > tb + TSG_TB) stage_trees(t);
@@ -670,14 +749,14 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     int neb = 0;
     bool gpath = false;
     auto stage_events = [&](int64_t e) {
-        __syncthreads();
+        SWEEP_SYNC();
         const int n = (int) min((int64_t) TSG_EB, ev_end - e);
         if (lane < n) {
             evc[lane] = a.ev_child[e + lane];
             evb[lane] = a.ev_bl[e + lane];
         }
         for (int l = lane; l <= n; l += WAVE) evp[l] = a.ev_path0[e + l];
-        __syncthreads();
+        SWEEP_SYNC();
         const int64_t base = evp[0];
         // events whose paths end inside pbuf (a prefix, offsets increase)
         const bool fits = lane < n && evp[lane + 1] - base <= TSG_PB;
@@ -688,7 +767,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
         pb0 = base;
         const int plen = gpath ? 0 : (int) (evp[neb] - base);
         for (int i = lane; i < plen; i += WAVE) pbuf[i] = a.path[base + i];
-        __syncthreads();
+        SWEEP_SYNC();
     };
 
     auto apply_transition = [&](int t) {
@@ -719,36 +798,27 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                 }
                 cm = __any(nz) ? ~0ull : 0ull;
             }
-            __syncthreads();
-            // the edge's own term bl * F(x_child): needed even for a zero-weight
-            // subtree, because unpolarised summaries need not vanish at zero
-            // (non-centred relatedness: F(0) = f(0) + f(total) = 1)
+            SWEEP_SYNC();
+            // the edge's own term bl * F(x_child) (needed even for a zero-weight
+            // subtree, because unpolarised summaries need not vanish at zero:
+            // non-centred relatedness has F(0) = f(0) + f(total) = 1) is added
+            // with the walk's, after the path update, which it does not
+            // depend on (the path starts at the parent)
             const double sg = ins ? 1.0 : -1.0;
-            if (MODE == 0) {
-                const double b = sg * (double) (ins ? ebl : nd.bl(u));
-#pragma unroll
-                for (int j = 0; j < MPL; j++) {
-                    const LaneOut &L = lo[j];
-                    if (L.lm < 0) continue;
-                    double f = L.f0;
-                    if (L.mask & cm) {
-                        if (L.kind == TSG_KIND_POLY) {
-                            f = poly_value(L, xc);
-                        } else if (L.kind == TSG_KIND_SEG) {
-                            const double xa = (double) xc[L.si];
-                            f = L.sinv * (L.spol ? seg_pol(xa, L.sn) : seg_unpol(xa, L.sn));
-                        } else {
-                            f = branch_summary(gout[L.lm], xc, tot);
-                        }
-                    }
-                    lo[j].R += b * f;
-                }
-            }
-            __syncthreads();
+            const double b = MODE == 0 ? sg * (double) (ins ? ebl : nd.bl(u)) : 0.0;
+            SWEEP_SYNC();
             // after seeding nothing reads parent pointers (the root paths
             // are precomputed), so only the branch length is kept current
             if (lane == 0) nd.set_bl(u, ins ? ebl : 0.0);
-            if (cm == 0) continue;  // no count changes, no summary changes
+            if (cm == 0) {  // no count changes, no summary changes
+                if (MODE == 0) {
+#pragma unroll
+                    for (int j = 0; j < MPL; j++) {
+                        if (lo[j].lm >= 0) lo[j].R += b * lo[j].f0;
+                    }
+                }
+                continue;
+            }
             // outputs whose summaries can change, compacted into act[]
             int pos[MPL];
             int na = 0;
@@ -763,46 +833,60 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             }
             const int64_t p0 = evp[le], p1 = evp[le + 1];
             const int32_t *pth = gpath ? a.path + p0 : pbuf + (p0 - pb0);
-            __syncthreads();
-            // path sums (lane k < K: column k): S0 = sum bl, S1 = sum bl x,
-            // S2 = sum bl x^2, and the segregating-sites indicator changes
+            SWEEP_SYNC();
+            // path sums of this lane's column uk over its hops: S0 = sum bl,
+            // S1 = sum bl x, S2 = sum bl x^2 (x the count before the event),
+            // and the segregating-sites numerator changes
             double s0 = 0, s1 = 0, s2 = 0, gp = 0, gu = 0;
+            const int32_t dx = (!kwide && uon) ? (ins ? xc[uk] : -xc[uk]) : 0;
             for (int64_t sg0 = 0; sg0 < p1 - p0; sg0 += TSG_PSEG) {
                 const int h = (int) min((int64_t) TSG_PSEG, p1 - p0 - sg0);
-                for (int idx = lane; idx < h * K; idx += WAVE) {
-                    const int i = idx / K, k = idx - i * K;
-                    const int node = pth[sg0 + i];
-                    const int32_t old = nd.get(node, k);
-                    const int32_t nw = ins ? old + xc[k] : old - xc[k];
-                    nd.put(node, k, nw);
-                    if (MODE == 0) {
-                        xo[idx] = old;
-                        xn[idx] = nw;
-                    }
-                }
-                if (MODE == 0) {
-                    for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(pth[sg0 + i]);
-                }
-                __syncthreads();
-                if (any_agg) {
+                if (kwide) {
                     for (int i = 0; i < h; i++) {
-                        const double bw = (double) sbl[i];
-                        s0 += bw;
-                        if (lane < K) {
-                            const double xold = (double) xo[i * K + lane];
-                            if (any_poly) {
-                                s1 += bw * xold;
-                                if (any_c3) s2 += bw * xold * xold;
+                        const int node = pth[sg0 + i];
+                        for (int k = lane; k < K; k += WAVE) {
+                            const int32_t old = nd.get(node, k);
+                            const int32_t nw = ins ? old + xc[k] : old - xc[k];
+                            nd.put(node, k, nw);
+                            if (MODE == 0 && any_hop) {
+                                xo[i * K + k] = old;
+                                xn[i * K + k] = nw;
                             }
-                            if (any_seg) {
-                                const double xnew = (double) xn[i * K + lane];
-                                const double nk = (double) tot[lane];
-                                gp += bw * (seg_pol(xnew, nk) - seg_pol(xold, nk));
-                                gu += bw * (seg_unpol(xnew, nk) - seg_unpol(xold, nk));
+                        }
+                    }
+                } else if (uon) {
+                    for (int i = ug; i < h; i += G) {
+                        const int node = pth[sg0 + i];
+                        const int32_t old = nd.get(node, uk);
+                        const int32_t nw = old + dx;
+                        nd.put(node, uk, nw);
+                        if (MODE == 0 && any_hop) {
+                            xo[i * K + uk] = old;
+                            xn[i * K + uk] = nw;
+                        }
+                        if (MODE == 0 && any_agg) {
+                            const double bw = nd.bl(node);
+                            if (any_poly) {
+                                const double xold = (double) old;
+                                s0 += bw;
+                                s1 = fma(bw, xold, s1);
+                                if (any_c3) s2 = fma(bw * xold, xold, s2);
+                            }
+                            if (any_segp) {
+                                const int32_t dp = iseg_pol(nw, totk) - iseg_pol(old, totk);
+                                gp = fma(bw, (double) dp, gp);
+                            }
+                            if (any_segu) {
+                                const int32_t du = iseg_unpol(nw, totk) - iseg_unpol(old, totk);
+                                gu = fma(bw, (double) du, gu);
                             }
                         }
                     }
                 }
+                if (MODE == 0 && any_hop) {
+                    for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(pth[sg0 + i]);
+                }
+                SWEEP_SYNC();
                 if (MODE == 0 && na > 0) {
                     const int items = na * h;
                     for (int r0 = 0; r0 < items; r0 += WAVE) {
@@ -814,42 +898,55 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                             const double fn = branch_summary(o, xn + i * K, tot);
                             td[lane] = (double) sbl[i] * (fn - fo);
                         }
-                        __syncthreads();
+                        SWEEP_SYNC();
 #pragma unroll
                         for (int j = 0; j < MPL; j++) {
                             if (pos[j] < 0) continue;
                             const int q0 = max(pos[j] * h, r0), q1 = min(pos[j] * h + h, r0 + WAVE);
                             for (int q = q0; q < q1; q++) lo[j].R += td[q - r0];
                         }
-                        __syncthreads();
+                        SWEEP_SYNC();
                     }
                 }
             }
-            if (any_agg) {
-                if (lane < K) {
-                    s1v[lane] = s1;
-                    s2v[lane] = s2;
-                    gpv[lane] = gp;
-                    guv[lane] = gu;
+            if (MODE == 0) {
+                if (any_agg) {
+                    if (any_poly) {
+                        s0 = __shfl(group_sum(s0, K, G), 0);
+                        s1 = group_sum(s1, K, G);
+                        if (any_c3) s2 = group_sum(s2, K, G);
+                    }
+                    if (any_segp) gp = group_sum(gp, K, G);
+                    if (any_segu) gu = group_sum(gu, K, G);
+                    if (lane < K) {
+                        s1v[lane] = s1;
+                        s2v[lane] = s2;
+                        gpv[lane] = gp;
+                        guv[lane] = gu;
+                    }
+                    SWEEP_SYNC();
                 }
-                __syncthreads();
-                // sum_h bl_h (F(x_h + d) - F(x_h)) for F = c1.x + x'C2x + c3 x_a^3:
-                // S0 (c1.d + d'C2d + c3 d_a^3) + sum over columns of
-                // S1 * (gradient of d'C2x) + 3 c3 (S1_a d_a^2 + S2_a d_a)
+                // child term, then the walk's closed form (POLY, SEG)
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
                     LaneOut &L = lo[j];
-                    if (L.kind == TSG_KIND_HOP || !(L.mask & cm)) continue;
-                    if (L.kind == TSG_KIND_SEG) {
-                        L.R += L.sinv * (L.spol ? gpv[L.si] : guv[L.si]);
-                        continue;
+                    if (L.lm < 0) continue;
+                    if (!(L.mask & cm)) {
+                        L.R += b * L.f0;
+                    } else if (L.kind == TSG_KIND_POLY) {
+                        const PolyParts P = poly_parts(L, xc);
+                        L.R += b * (L.c0 + P.A + P.B);
+                        L.R += poly_walk(L, P, sg, s0, s1v, s2v, any_c3);
+                    } else if (L.kind == TSG_KIND_SEG) {
+                        const int si = L.pc[0];
+                        const double xa = (double) xc[si], sn = (double) tot[si];
+                        L.R += b * (L.c0 * (L.pc[1] ? seg_pol(xa, sn) : seg_unpol(xa, sn)));
+                        L.R += L.c0 * (L.pc[1] ? gpv[si] : guv[si]);
+                    } else {
+                        L.R += b * branch_summary(gout[L.lm], xc, tot);
                     }
-                    double d[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) d[q] = sg * (double) xc[L.pc[q]];
-                    L.R += poly_walk(L, d, s0, s1v, s2v);
                 }
-                __syncthreads();
+                SWEEP_SYNC();
             }
         }
     };
@@ -904,14 +1001,14 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                         }
                         xo[k] = w;
                     }
-                    __syncthreads();
+                    SWEEP_SYNC();
 #pragma unroll
                     for (int j = 0; j < MPL; j++) {
                         if (lo[j].lm < 0) continue;
                         const TsgOut &o = gout[lo[j].lm];
                         if (al > 0 || !(o.flags & 1)) val[j] += eval_summary(o, xo, tot, false);
                     }
-                    __syncthreads();
+                    SWEEP_SYNC();
                 }
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
@@ -938,10 +1035,17 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             L.wi++;
         }
     }
+#undef SWEEP_SYNC
 }
 
+// Waves per SIMD the one-output-per-lane sweep is compiled for (its VGPR
+// budget: 3 waves allow 168)
+#ifndef TSG_SWEEP_WAVES1
+#define TSG_SWEEP_WAVES1 3
+#endif
+
 template <int MODE, int STORE, int MPL>
-__global__ void __launch_bounds__(WAVE, MPL == 1 ? 3 : 1) k_sweep(const TsgSweepArgs a) {
+__global__ void __launch_bounds__(WAVE, MPL == 1 ? TSG_SWEEP_WAVES1 : 1) k_sweep(const TsgSweepArgs a) {
     const int c = xcd_chunk(blockIdx.x, a.num_chunks);
     if (c >= a.num_chunks) return;
     sweep_chunk<MODE, STORE, MPL>(a, c);
@@ -952,7 +1056,7 @@ __global__ void __launch_bounds__(WAVE, MPL == 1 ? 3 : 1) k_sweep(const TsgSweep
 // arguments.  All runs share MODE, STORE and MPL; the dynamic LDS is the
 // largest run's.
 template <int MODE, int STORE, int MPL>
-__global__ void __launch_bounds__(WAVE, MPL == 1 ? 3 : 1)
+__global__ void __launch_bounds__(WAVE, MPL == 1 ? TSG_SWEEP_WAVES1 : 1)
     k_sweep_multi(const TsgSweepArgs *__restrict__ args, const int32_t *__restrict__ cpre,
                   int nrun) {
     const int total = cpre[nrun];
@@ -1557,8 +1661,9 @@ __global__ void k_node_tl_flush(const TsgSweepArgs a, const TsgLastTree lt,
 //
 // Counts: one thread per weighted sample walks its root path in the
 // snapshot, adding 1 to each of its columns.
-__global__ void k_tree0_counts(const TsgSweepArgs a, const int32_t *t0_parent, int32_t *x0) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tree0_counts(const TsgSweepArgs &a, const int32_t *t0_parent,
+                                             int32_t *x0, int bx) {
+    const int i = bx * 256 + threadIdx.x;
     if (i >= a.nseed) return;
     const int c0 = a.seed_off[i], c1 = a.seed_off[i + 1];
     for (int u = a.seed_sample[i]; u >= 0; u = t0_parent[u]) {
@@ -1566,15 +1671,20 @@ __global__ void k_tree0_counts(const TsgSweepArgs a, const int32_t *t0_parent, i
     }
 }
 
+__global__ void __launch_bounds__(256) k_tree0_counts(const TsgSweepArgs a,
+                                                      const int32_t *t0_parent, int32_t *x0) {
+    tree0_counts(a, t0_parent, x0, blockIdx.x);
+}
+
 #define TSG_T0_NODES 64
 
 // grid (ceil(nloc / TSG_T0_NODES), ceil(M / 64)); lane = output.
-__global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *t0_parent,
-                                                    const double *t0_bl, const int32_t *x0,
-                                                    int nloc, double *part) {
-    const int m = blockIdx.y * WAVE + threadIdx.x;
+__device__ __forceinline__ void tree0_sum(const TsgSweepArgs &a, const int32_t *t0_parent,
+                                          const double *t0_bl, const int32_t *x0, int nloc,
+                                          double *part, int bx, int by) {
+    const int m = by * WAVE + threadIdx.x;
     if (m >= a.M) return;
-    const int u0 = blockIdx.x * TSG_T0_NODES;
+    const int u0 = bx * TSG_T0_NODES;
     const int u1 = min(nloc, u0 + TSG_T0_NODES);
     const TsgOut o = a.outs[m];
     LaneOut L;
@@ -1591,7 +1701,13 @@ __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const 
         }
         acc += (double) t0_bl[u] * f;
     }
-    part[(int64_t) blockIdx.x * a.M + m] = acc;
+    part[(int64_t) bx * a.M + m] = acc;
+}
+
+__global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *t0_parent,
+                                                    const double *t0_bl, const int32_t *x0,
+                                                    int nloc, double *part) {
+    tree0_sum(a, t0_parent, t0_bl, x0, nloc, part, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -1607,23 +1723,26 @@ __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const 
 // block kernels: (blocks, ceil(M / 64)), lane = output.
 #define TSG_SCAN_B 128
 
-__global__ void k_scan_blocks(const double *dR, double *bsum, int C, int M) {
-    const int m = blockIdx.y * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void scan_blocks(const double *dR, double *bsum, int C, int M, int bx,
+                                            int by) {
+    const int m = by * 64 + threadIdx.x;
     if (m >= M) return;
-    const int c0 = blockIdx.x * TSG_SCAN_B, c1 = min(C, c0 + TSG_SCAN_B);
+    const int c0 = bx * TSG_SCAN_B, c1 = min(C, c0 + TSG_SCAN_B);
     double acc = 0;
     for (int c = c0; c < c1; c++) acc += dR[(int64_t) c * M + m];
-    bsum[(int64_t) blockIdx.x * M + m] = acc;
+    bsum[(int64_t) bx * M + m] = acc;
+}
+
+__global__ void __launch_bounds__(64) k_scan_blocks(const double *dR, double *bsum, int C, int M) {
+    scan_blocks(dR, bsum, C, M, blockIdx.x, blockIdx.y);
 }
 
 // one workgroup of TSG_SCAN_TOP threads per output: the first tree's
 // partials are summed by a fixed-shape tree (deterministic), then one lane
 // scans the block sums
 #define TSG_SCAN_TOP 256
-__global__ void __launch_bounds__(TSG_SCAN_TOP)
-k_scan_top(double *bsum, int nblk, int M, const double *r0part, int nb) {
-    __shared__ double red[TSG_SCAN_TOP];
-    const int m = blockIdx.x;
+__device__ __forceinline__ void scan_top(double *bsum, int nblk, int M, const double *r0part,
+                                         int nb, int m, double *red) {
     const int t = threadIdx.x;
     double acc = 0;
     for (int b = t; b < nb; b += TSG_SCAN_TOP) acc += r0part[(int64_t) b * M + m];
@@ -1642,23 +1761,35 @@ k_scan_top(double *bsum, int nblk, int M, const double *r0part, int nb) {
     }
 }
 
-__global__ void k_scan_apply(const double *dR, const double *bsum, double *Rstart, int C, int M) {
-    const int m = blockIdx.y * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(TSG_SCAN_TOP)
+k_scan_top(double *bsum, int nblk, int M, const double *r0part, int nb) {
+    __shared__ double red[TSG_SCAN_TOP];
+    scan_top(bsum, nblk, M, r0part, nb, blockIdx.x, red);
+}
+
+__device__ __forceinline__ void scan_apply(const double *dR, const double *bsum, double *Rstart,
+                                           int C, int M, int bx, int by) {
+    const int m = by * 64 + threadIdx.x;
     if (m >= M) return;
-    const int c0 = blockIdx.x * TSG_SCAN_B, c1 = min(C, c0 + TSG_SCAN_B);
-    double acc = bsum[(int64_t) blockIdx.x * M + m];
+    const int c0 = bx * TSG_SCAN_B, c1 = min(C, c0 + TSG_SCAN_B);
+    double acc = bsum[(int64_t) bx * M + m];
     for (int c = c0; c < c1; c++) {
         Rstart[(int64_t) c * M + m] = acc;
         acc += dR[(int64_t) c * M + m];
     }
 }
 
+__global__ void __launch_bounds__(64) k_scan_apply(const double *dR, const double *bsum,
+                                                   double *Rstart, int C, int M) {
+    scan_apply(dR, bsum, Rstart, C, M, blockIdx.x, blockIdx.y);
+}
+
 // One group of Qr.lanes lanes per (query, window, output), aligned within
 // the wave: lane j of the group sums the window's chunks j, j + lanes, ...,
 // then the group adds its lanes by a fixed butterfly (deterministic).  A
 // window spanning many chunks (few windows) gets up to a wave.
-__global__ void k_combine(const TsgCombineArgs a, int64_t total) {
-    const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void combine(const TsgCombineArgs &a, int64_t total, int64_t bx) {
+    const int64_t gid = bx * 256 + threadIdx.x;
     if (gid >= total) return;  // total is a multiple of 64: whole waves leave
     int qlo = 0, qhi = a.Q - 1;
     while (qlo < qhi) {
@@ -1720,6 +1851,10 @@ __global__ void k_combine(const TsgCombineArgs a, int64_t total) {
     }
 }
 
+__global__ void __launch_bounds__(256) k_combine(const TsgCombineArgs a, int64_t total) {
+    combine(a, total, blockIdx.x);
+}
+
 // FP64 composites of the reference's Python layer.
 struct TsgFinal {
     int32_t kind;       // tsg_final_id
@@ -1731,9 +1866,10 @@ struct TsgFinal {
     int32_t Mraw;
 };
 
-__global__ void k_finals(const TsgFinal *fin, int nq, const double *raw, const int32_t *ipar,
-                         const double *dpar, double *out, int64_t total) {
-    const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void finals(const TsgFinal *fin, int nq, const double *raw,
+                                       const int32_t *ipar, const double *dpar, double *out,
+                                       int64_t total, int64_t bx) {
+    const int64_t gid = bx * 256 + threadIdx.x;
     if (gid >= total) return;
     int q = 0;
     int64_t rem = gid;
@@ -1772,6 +1908,12 @@ __global__ void k_finals(const TsgFinal *fin, int nq, const double *raw, const i
         v = r[p];
     }
     out[F.outbase + (int64_t) w * F.P + p] = v;
+}
+
+__global__ void __launch_bounds__(256) k_finals(const TsgFinal *fin, int nq, const double *raw,
+                                                const int32_t *ipar, const double *dpar,
+                                                double *out, int64_t total) {
+    finals(fin, nq, raw, ipar, dpar, out, total, blockIdx.x);
 }
 
 // Site AFS finalise: half-unit counts to doubles (exact), span normalise.
@@ -2028,6 +2170,132 @@ hipError_t tsg_launch_finals(const void *fin, int nq, const double *raw, const i
     const int64_t blocks = (total + 255) / 256;
     hipLaunchKernelGGL(k_finals, dim3((unsigned) blocks), dim3(256), 0, s,
                        reinterpret_cast<const TsgFinal *>(fin), nq, raw, ipar, dpar, out, total);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The finish phases of many runs, one launch per phase: the grid covers the
+// blocks of every run (prefix pre[r]), a block finds its run by binary search
+// and does what the single-run kernel's block of the same index does, so the
+// sums are the same, in the same order.
+namespace {
+__device__ __forceinline__ int run_of(const int64_t *pre, int nrun, int64_t b) {
+    int lo = 0, hi = nrun - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= b) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_finish_multi_256(const TsgFinishRun *runs,
+                                                          const int64_t *pre, int nrun,
+                                                          int phase) {
+    const int64_t *p = pre + (int64_t) phase * (nrun + 1);
+    const int64_t b = blockIdx.x;
+    if (b >= p[nrun]) return;
+    const int r = run_of(p, nrun, b);
+    const TsgFinishRun &R = runs[r];
+    const int64_t lb = b - p[r];
+    switch (phase) {
+    case 0: {  // clear the first tree's counts
+        const int64_t i = lb * 256 + threadIdx.x;
+        if (i < (int64_t) R.nloc * R.a.K) R.x0[i] = 0;
+        break;
+    }
+    case 1:
+        tree0_counts(R.a, R.t0_parent, R.x0, (int) lb);
+        break;
+    case 6:
+        combine(R.ca, R.lanecount, lb);
+        break;
+    case 7:
+        finals(reinterpret_cast<const TsgFinal *>(R.fin), R.nq, R.raw, R.ipar, R.dpar, R.out,
+               R.outcount, lb);
+        break;
+    default:
+        break;
+    }
+}
+
+__global__ void __launch_bounds__(TSG_SCAN_TOP) k_finish_multi_top(const TsgFinishRun *runs,
+                                                                   const int64_t *pre, int nrun) {
+    __shared__ double red[TSG_SCAN_TOP];
+    const int64_t *p = pre + (int64_t) 4 * (nrun + 1);
+    const int64_t b = blockIdx.x;
+    if (b >= p[nrun]) return;
+    const int r = run_of(p, nrun, b);
+    const TsgFinishRun &R = runs[r];
+    scan_top(R.bsum, R.nblk, R.a.M, R.r0part, R.nb, (int) (b - p[r]), red);
+}
+
+__global__ void __launch_bounds__(64) k_finish_multi_64(const TsgFinishRun *runs,
+                                                        const int64_t *pre, int nrun, int phase) {
+    const int64_t *p = pre + (int64_t) phase * (nrun + 1);
+    const int64_t b = blockIdx.x;
+    if (b >= p[nrun]) return;
+    const int r = run_of(p, nrun, b);
+    const TsgFinishRun &R = runs[r];
+    const int64_t lb = b - p[r];
+    switch (phase) {
+    case 2: {  // tree0 sums: grid (nb, ceil(M / 64))
+        const int gx = R.nb;
+        tree0_sum(R.a, R.t0_parent, R.t0_bl, R.x0, R.nloc, R.r0part, (int) (lb % gx),
+                  (int) (lb / gx));
+        break;
+    }
+    case 3: {
+        const int gx = R.nblk;
+        scan_blocks(R.dR, R.bsum, R.a.num_chunks, R.a.M, (int) (lb % gx), (int) (lb / gx));
+        break;
+    }
+    case 5: {
+        const int gx = R.nblk;
+        scan_apply(R.dR, R.bsum, R.Rstart, R.a.num_chunks, R.a.M, (int) (lb % gx),
+                   (int) (lb / gx));
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+// Blocks of run r in each phase (the single-run launchers' grids).
+void tsg_finish_blocks(const TsgFinishRun &R, int64_t *blocks) {
+    const bool br = R.dR != nullptr;
+    const int64_t mb = (R.a.M + 63) / 64;
+    blocks[0] = br ? ((int64_t) R.nloc * R.a.K + 255) / 256 : 0;
+    blocks[1] = br ? ((int64_t) R.a.nseed + 255) / 256 : 0;
+    blocks[2] = (br && R.a.M > 0) ? (int64_t) R.nb * mb : 0;
+    blocks[3] = br ? (int64_t) R.nblk * mb : 0;
+    blocks[4] = br ? (int64_t) R.a.M : 0;
+    blocks[5] = br ? (int64_t) R.nblk * mb : 0;
+    blocks[6] = (R.lanecount + 255) / 256;
+    blocks[7] = (R.outcount + 255) / 256;
+}
+
+hipError_t tsg_launch_finish_multi(const TsgFinishRun *runs, const int64_t *pre,
+                                   const int64_t *totals, int nrun, hipStream_t s) {
+    for (int ph = 0; ph < TSG_FINISH_PHASES; ph++) {
+        const int64_t nb = totals[ph];
+        if (nb <= 0) continue;
+        if (nb > INT32_MAX) return hipErrorInvalidValue;
+        if (ph == 4) {
+            hipLaunchKernelGGL(k_finish_multi_top, dim3((unsigned) nb), dim3(TSG_SCAN_TOP), 0, s,
+                               runs, pre, nrun);
+        } else if (ph == 2 || ph == 3 || ph == 5) {
+            hipLaunchKernelGGL(k_finish_multi_64, dim3((unsigned) nb), dim3(64), 0, s, runs, pre,
+                               nrun, ph);
+        } else {
+            hipLaunchKernelGGL(k_finish_multi_256, dim3((unsigned) nb), dim3(256), 0, s, runs, pre,
+                               nrun, ph);
+        }
+    }
     return hipGetLastError();
 }
 

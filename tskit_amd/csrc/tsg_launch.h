@@ -72,6 +72,13 @@ hipError_t tsg_launch_centre(const int32_t *C, const int32_t *C2, const int32_t 
                              int64_t *rowsum, int n, int n_pad, int W, const double *scale,
                              double *K, hipStream_t s);
 
+// Finish phases of many runs, one launch per phase: runs [nrun] and the
+// per-phase block prefixes pre [TSG_FINISH_PHASES x (nrun + 1)] in device
+// memory, totals[phase] = pre[phase][nrun] on the host.
+void tsg_finish_blocks(const TsgFinishRun &run, int64_t *blocks);
+hipError_t tsg_launch_finish_multi(const TsgFinishRun *runs, const int64_t *pre,
+                                   const int64_t *totals, int nrun, hipStream_t s);
+
 // Mirrors TsgFinal in tsg_kernels.hip.
 struct TsgFinalHost {
     int32_t kind;
