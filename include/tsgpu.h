@@ -243,6 +243,59 @@ int tsg_treeseq_genetic_relatedness_vector(tsg_treeseq_t *ts, uint64_t num_weigh
     const double *weights, uint64_t num_windows, const double *windows,
     uint64_t num_focal_nodes, const int32_t *focal_nodes, double *result, uint32_t options);
 
+/* ---- general_stat with real-valued weights ----------------------------
+ * tsk_treeseq_general_stat (trees.h:1035, trees.c:2036-2095) with the summary
+ * function taken from a registry by id instead of a general_stat_func_t and
+ * its f_params (trees.h:1032-1037): sample_weights [num_samples x state_dim]
+ * (rows in sample order), result [num_windows x result_dim].  Branch or site
+ * mode (TSG_STAT_NODE: TSG_ERR_UNSUPPORTED_STAT_MODE), TSG_STAT_POLARISED,
+ * TSG_STAT_SPAN_NORMALISE; windows must span the sequence.  Summaries (and
+ * their parameters, the reference's param structs):
+ *   TSG_SUMMARY_TRAIT_COVARIANCE (11)   trees.c:3961-3974; result_dim =
+ *       state_dim; x^2 / (2 (n - 1)^2), n = number of samples.
+ *   TSG_SUMMARY_TRAIT_CORRELATION (12)  trees.c:4026-4045; result_dim =
+ *       state_dim - 1, the last column the frequency.
+ *   TSG_SUMMARY_TRAIT_LINEAR_MODEL (13) trees.c:4108-4152; state = [traits
+ *       (result_dim) | covariates | 1]; dparams = V [result_dim x covariates].
+ *   TSG_SUMMARY_RELATEDNESS_WEIGHTED (14) trees.c:4801-4820 and
+ *   TSG_SUMMARY_RELATEDNESS_WEIGHTED_NONCENTRED (15) trees.c:4822-4837;
+ *       index_tuples [result_dim x 2], the last column the frequency;
+ *       dparams = the total weights [state_dim] (centred only).
+ * State columns are FP64 in LDS on the hot plan: a tree sequence whose
+ * largest chunk's slots do not fit returns TSG_ERR_UNSUPPORTED_INPUT.  Not
+ * inside a batch (TSG_ERR_UNSUPPORTED_INPUT). */
+#define TSG_SUMMARY_TRAIT_COVARIANCE 11
+#define TSG_SUMMARY_TRAIT_CORRELATION 12
+#define TSG_SUMMARY_TRAIT_LINEAR_MODEL 13
+#define TSG_SUMMARY_RELATEDNESS_WEIGHTED 14
+#define TSG_SUMMARY_RELATEDNESS_WEIGHTED_NONCENTRED 15
+int tsg_treeseq_general_stat(tsg_treeseq_t *ts, uint64_t state_dim,
+    const double *sample_weights, uint64_t result_dim, int32_t summary,
+    const int32_t *index_tuples, const double *dparams, uint64_t num_windows,
+    const double *windows, uint32_t options, double *result);
+
+/* tsk_treeseq_trait_covariance (trees.h; trees.c:3976-4022): the weights
+ * centred on the host, then general_stat.  Same arguments and order. */
+int tsg_treeseq_trait_covariance(tsg_treeseq_t *ts, uint64_t num_weights,
+    const double *weights, uint64_t num_windows, const double *windows, uint32_t options,
+    double *result);
+/* tsk_treeseq_trait_correlation (trees.c:4047-4103): centred and scaled,
+ * plus a column of 1/n. */
+int tsg_treeseq_trait_correlation(tsg_treeseq_t *ts, uint64_t num_weights,
+    const double *weights, uint64_t num_windows, const double *windows, uint32_t options,
+    double *result);
+/* tsk_treeseq_trait_linear_model (trees.c:4154-4220): V = W^T Z on the host,
+ * state [W | Z | 1]; the covariates Z are taken as standardised. */
+int tsg_treeseq_trait_linear_model(tsg_treeseq_t *ts, uint64_t num_weights,
+    const double *weights, uint64_t num_covariates, const double *covariates,
+    uint64_t num_windows, const double *windows, uint32_t options, double *result);
+/* tsk_treeseq_genetic_relatedness_weighted (trees.c:4839-4895), result
+ * before options as there; TSG_STAT_NONCENTRED selects the non-centred
+ * summary. */
+int tsg_treeseq_genetic_relatedness_weighted(tsg_treeseq_t *ts, uint64_t num_weights,
+    const double *weights, uint64_t num_index_tuples, const int32_t *index_tuples,
+    uint64_t num_windows, const double *windows, double *result, uint32_t options);
+
 /* ---- allele frequency spectrum ----------------------------------------- */
 /* tsk_treeseq_allele_frequency_spectrum (trees.h:1112, trees.c:3815-3928);
  * at most 16 sample sets. */

@@ -140,6 +140,36 @@ class _Base:
             raise OracleError(err)
         return out
 
+    def weighted_stat(self, name, W, windows=None, mode="site", span_normalise=True,
+                      polarised=False, Z=None, indexes=None, centre=True):
+        """The reference's weighted statistics (trees.c:3976-4220, 4839-4895):
+        name in trait_covariance, trait_correlation, trait_linear_model (Z the
+        covariates, taken as standardised), genetic_relatedness_weighted
+        (indexes [m x 2]); W [n, K].  Result [nw, outputs]."""
+        W = np.ascontiguousarray(np.asarray(W, dtype=np.float64).reshape(len(W), -1))
+        if windows is None:
+            windows = np.array([0.0, self.tc.sequence_length])
+        windows = np.ascontiguousarray(windows, np.float64)
+        opts = {"site": SITE, "branch": BRANCH, "node": NODE}[mode]
+        if span_normalise:
+            opts |= SPAN_NORMALISE
+        if polarised:
+            opts |= POLARISED
+        if not centre:
+            opts |= NONCENTRED
+        idx = None
+        if indexes is not None:
+            idx = np.ascontiguousarray(np.asarray(indexes, dtype=np.int32).reshape(-1, 2))
+        M = len(idx) if idx is not None else W.shape[1]
+        shape = (len(windows) - 1, M) if mode != "node" else (
+            len(windows) - 1, len(self.tc.node_time), M)
+        out = np.zeros(shape)
+        Zc = None if Z is None else np.ascontiguousarray(np.asarray(Z, dtype=np.float64))
+        err = self._weighted(name, W, Zc, idx, windows, opts, out)
+        if err != 0:
+            raise OracleError(err)
+        return out
+
     def afs(self, sample_sets, windows=None, time_windows=None, mode="site",
             span_normalise=True, polarised=False):
         sizes = np.array([len(s) for s in sample_sets], dtype=np.uint64)
@@ -238,6 +268,34 @@ class Oracle(_Base):
             self.h, W.shape[1], _p(W), len(windows) - 1, _p(windows), len(nodes), _p(nodes),
             ctypes.c_uint32(opts), _p(out))
 
+    def _weighted(self, name, W, Z, idx, windows, opts, out):
+        # the reference's weight preparation (trees.c:3976-4220, 4839-4895),
+        # restated in numpy, then the restated general_stat
+        n, K = W.shape
+        dpar = None
+        if name == "trait_covariance":
+            stat, Wn, M = 11, W - W.mean(axis=0), K
+        elif name == "trait_correlation":
+            mean = W.mean(axis=0)
+            msq = (np.sum(W * W, axis=0) - mean * mean * n) / (n - 1)
+            Wn = np.column_stack([(W - mean) / np.sqrt(msq), np.full(n, 1.0 / n)])
+            stat, M = 12, K
+        elif name == "trait_linear_model":
+            V = W.T @ Z
+            Wn = np.column_stack([W, Z, np.ones(n)])
+            stat, M, dpar = 13, K, V
+        else:
+            Wn = np.column_stack([W, np.full(n, 1.0 / n)])
+            dpar = np.append(W.sum(axis=0), 1.0)
+            stat = 15 if opts & NONCENTRED else 14
+            M = len(idx)
+        Wn = np.ascontiguousarray(Wn, dtype=np.float64)
+        dpar = None if dpar is None else np.ascontiguousarray(dpar, dtype=np.float64)
+        return self._lib.oracle_general_stat(
+            self.h, stat, Wn.shape[1], _p(Wn), M, None if idx is None else _p(idx),
+            None if dpar is None else _p(dpar), len(windows) - 1, _p(windows),
+            ctypes.c_uint32(opts & ~NONCENTRED), _p(out))
+
     def _afs(self, sizes, flat, windows, time_windows, opts, out):
         return self._lib.oracle_afs(
             self.h, len(sizes), _p(sizes), _p(flat), len(windows) - 1, _p(windows),
@@ -310,6 +368,16 @@ class Reference(_Base):
         return self._lib.ref_relatedness_vector(
             self.h, ctypes.c_uint64(W.shape[1]), _p(W), ctypes.c_uint64(len(windows) - 1),
             _p(windows), ctypes.c_uint64(len(nodes)), _p(nodes), ctypes.c_uint32(opts), _p(out))
+
+    def _weighted(self, name, W, Z, idx, windows, opts, out):
+        which = ["trait_covariance", "trait_correlation", "trait_linear_model",
+                 "genetic_relatedness_weighted"].index(name)
+        nc = 0 if Z is None else Z.shape[1]
+        return self._lib.ref_weighted_stat(
+            self.h, which, ctypes.c_uint64(W.shape[1]), _p(W), ctypes.c_uint64(nc),
+            None if Z is None else _p(Z), ctypes.c_uint64(0 if idx is None else len(idx)),
+            None if idx is None else _p(idx), ctypes.c_uint64(len(windows) - 1), _p(windows),
+            ctypes.c_uint32(opts), _p(out))
 
     def _afs(self, sizes, flat, windows, time_windows, opts, out):
         return self._lib.ref_afs(

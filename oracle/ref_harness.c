@@ -191,3 +191,24 @@ int ref_relatedness_vector(const ref_ts_t *r, uint64_t K, const double *weights,
     return tsk_treeseq_genetic_relatedness_vector(
         &r->ts, K, weights, nw, windows, nf, focal, result, options);
 }
+
+/* The reference's weighted statistics: which = 0 trait_covariance
+ * (trees.c:3976), 1 trait_correlation (4047), 2 trait_linear_model (4154,
+ * covariates Z [n x nc]), 3 genetic_relatedness_weighted (4839, idx
+ * [nidx x 2]); weights [n x K]. */
+int ref_weighted_stat(const ref_ts_t *r, int which, uint64_t K, const double *weights,
+    uint64_t nc, const double *Z, uint64_t nidx, const int32_t *idx, uint64_t nw,
+    const double *windows, uint32_t options, double *result) {
+    switch (which) {
+    case 0:
+        return tsk_treeseq_trait_covariance(&r->ts, K, weights, nw, windows, options, result);
+    case 1:
+        return tsk_treeseq_trait_correlation(&r->ts, K, weights, nw, windows, options, result);
+    case 2:
+        return tsk_treeseq_trait_linear_model(
+            &r->ts, K, weights, nc, Z, nw, windows, options, result);
+    default:
+        return tsk_treeseq_genetic_relatedness_weighted(
+            &r->ts, K, weights, nidx, idx, nw, windows, result, options);
+    }
+}

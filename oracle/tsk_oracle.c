@@ -21,7 +21,9 @@
 
 enum {
     ST_DIVERSITY = 0, ST_SEGSITES, ST_Y1, ST_DIVERGENCE, ST_RELATEDNESS,
-    ST_RELATEDNESS_NC, ST_Y2, ST_F2, ST_Y3, ST_F3, ST_F4
+    ST_RELATEDNESS_NC, ST_Y2, ST_F2, ST_Y3, ST_F3, ST_F4,
+    /* summaries of real-valued weights (oracle_general_stat) */
+    ST_TRAIT_COV, ST_TRAIT_CORR, ST_TRAIT_LM, ST_REL_W, ST_REL_W_NC
 };
 #define OPT_SITE (1u << 0)
 #define OPT_BRANCH (1u << 1)
@@ -235,6 +237,10 @@ typedef struct {
     int32_t K, M, tuple;
     const double *n;      /* sample set sizes as doubles */
     const int32_t *idx;   /* [M x tuple] */
+    /* weighted summaries: number of samples, FP64 parameters (V rows of the
+     * linear model, total weights of weighted relatedness) */
+    double nsamp;
+    const double *dpar;
 } sf_t;
 
 static void summary(const sf_t *p, const double *x, double *y) {
@@ -324,6 +330,54 @@ static void summary(const sf_t *p, const double *x, double *y) {
             y[m] = (x[i] * (x[i] - 1) * (nj - x[j]) * (nk - x[k])
                        - x[i] * (ni - x[i]) * (nj - x[j]) * x[k])
                    / (ni * (ni - 1) * nj * nk);
+        }
+        break;
+    case ST_TRAIT_COV: /* trees.c:3961-3974 */
+        for (m = 0; m < p->K; m++) {
+            y[m] = (x[m] * x[m]) / (2 * (p->nsamp - 1) * (p->nsamp - 1));
+        }
+        break;
+    case ST_TRAIT_CORR: { /* trees.c:4026-4045 */
+        const double q = x[p->K - 1];
+        for (m = 0; m < p->K - 1; m++) {
+            y[m] = (q > 0.0 && q < 1.0)
+                       ? (x[m] * x[m]) / (2 * (q * (1 - q)) * p->nsamp * (p->nsamp - 1))
+                       : 0.0;
+        }
+        break;
+    }
+    case ST_TRAIT_LM: { /* trees.c:4108-4152 */
+        const int32_t nc = p->K - p->M - 1;
+        const double mm = x[p->K - 1];
+        for (i = 0; i < p->M; i++) {
+            y[i] = 0.0;
+            if (mm > 0.0 && mm < p->nsamp) {
+                const double *v = p->dpar + (size_t) i * nc;
+                double a = x[i], denom = mm, z;
+                for (j = 0; j < nc; j++) {
+                    z = x[p->M + j];
+                    a -= z * v[j];
+                    denom -= z * z;
+                }
+                y[i] = (denom < 1e-8) ? 0.0 : (a * a) / (2 * denom * denom);
+            }
+        }
+        break;
+    }
+    case ST_REL_W: { /* trees.c:4801-4820 */
+        const double pn = x[p->K - 1];
+        for (m = 0; m < p->M; m++) {
+            i = p->idx[2 * m];
+            j = p->idx[2 * m + 1];
+            y[m] = (x[i] - p->dpar[i] * pn) * (x[j] - p->dpar[j] * pn);
+        }
+        break;
+    }
+    case ST_REL_W_NC: /* trees.c:4822-4837 */
+        for (m = 0; m < p->M; m++) {
+            i = p->idx[2 * m];
+            j = p->idx[2 * m + 1];
+            y[m] = x[i] * x[j];
         }
         break;
     case ST_F4: /* trees.c:5266-5291 */
@@ -703,6 +757,69 @@ int oracle_sample_count_stat(const ots_t *ts, int stat, int32_t K, const uint64_
 out:
     free(W);
     free(n);
+    free(total);
+    free(tx);
+    free(ty);
+    return ret;
+}
+
+/* general_stat of real-valued weights W [n x K] (trees.c:2036-2095) with a
+ * weighted summary (ST_TRAIT_COV .. ST_REL_W_NC): M outputs, idx [M x 2]
+ * for the relatedness summaries, dpar their parameters. */
+int oracle_general_stat(const ots_t *ts, int stat, int32_t K, const double *W, int32_t M,
+    const int32_t *idx, const double *dpar, int32_t nw, const double *windows,
+    uint32_t options, double *result) {
+    int site = !!(options & OPT_SITE), branch = !!(options & OPT_BRANCH);
+    int node = !!(options & OPT_NODE);
+    double dw[2];
+    double *total = NULL, *tx = NULL, *ty = NULL;
+    int32_t j, k, ret = 0;
+    sf_t p;
+    wf_t wf;
+    if (!(site || branch || node)) site = 1;
+    if (site + branch + node > 1) return -902;
+    if (K < 1) return -903;
+    if (M < 1) return -904;
+    if (windows == NULL) {
+        nw = 1;
+        dw[0] = 0;
+        dw[1] = ts->L;
+        windows = dw;
+    } else if ((ret = check_windows(ts, nw, windows)) != 0) {
+        return ret;
+    }
+    memset(&p, 0, sizeof(p));
+    p.stat = stat;
+    p.K = K;
+    p.M = M;
+    p.tuple = 2;
+    p.idx = idx;
+    p.nsamp = (double) ts->n;
+    p.dpar = dpar;
+    total = calloc((size_t) K, sizeof(double));
+    for (j = 0; j < ts->n; j++) {
+        for (k = 0; k < K; k++) total[k] += W[(size_t) j * K + k];
+    }
+    if (branch || node) {
+        tx = calloc((size_t) K + 1, sizeof(double));
+        ty = calloc((size_t) M + 1, sizeof(double));
+        wf.p = &p;
+        wf.polarised = !!(options & OPT_POLARISED);
+        wf.total = total;
+        wf.tmp_x = tx;
+        wf.tmp_y = ty;
+        ret = node ? node_stat(ts, W, &wf, nw, windows, result)
+                   : branch_stat(ts, W, &wf, nw, windows, result);
+    } else {
+        ret = site_stat(ts, W, &p, !!(options & OPT_POLARISED), nw, windows, result);
+    }
+    if (ret == 0 && (options & OPT_SPAN_NORMALISE)) {
+        const size_t row = (size_t) M * (node ? (size_t) ts->N : 1);
+        for (j = 0; j < nw; j++) {
+            const double span = windows[j + 1] - windows[j];
+            for (size_t r = 0; r < row; r++) result[(size_t) j * row + r] /= span;
+        }
+    }
     free(total);
     free(tx);
     free(ty);

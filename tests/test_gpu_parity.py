@@ -987,3 +987,160 @@ def test_autotune(require_gpu, tmp_path):
     for mode in ("branch", "site"):
         _check("diversity", ts2.diversity(sets, windows=w, mode=mode),
                orc.stat("diversity", sets, None, w, mode))
+    # one setting for tree sequences swept together (run_batches)
+    import tskit_amd as ta
+
+    def batches():
+        out = []
+        for t in (ts, ts2):
+            b = t.batch()
+            b.diversity(sets, windows=w, mode="branch")
+            out.append(b)
+        return out
+
+    joint = autotune.tune_batch([ts, ts2], lambda: ta.run_batches(batches()),
+                                candidates=(64, 256), reps=1)
+    assert joint["events_per_chunk"] in (64, 256)
+    res = ta.run_batches(batches())
+    exp = orc.stat("diversity", sets, None, w, "branch")
+    for r in res:
+        _check("diversity", r[0], exp)
+
+
+@pytest.mark.parametrize("case", ["generated", "gapped", "multiallelic", "paper_ex"])
+def test_weighted_stats(require_gpu, case):
+    """general_stat of real-valued weights (FP64 state columns on the hot
+    plan): trait covariance, correlation and linear model, weighted genetic
+    relatedness centred and not, both modes and polarisations, windows
+    spanning the sequence and in pieces, one and several chunks, against the
+    oracle (pinned to the compiled reference and to closed forms) and the
+    compiled reference when present.  Tolerance: <= 1e-9 of the largest
+    |value| of each window (the traits are centred: values change sign).
+    A repeated call is bit-identical."""
+    from oracle import oracle as om
+
+    tc = {"generated": lambda: cases.generated(150, 3e5, 41),
+          "gapped": lambda: cases.gapped(120, 2e5, 43, count=20),
+          "multiallelic": lambda: cases.multiallelic(100, 2e5, 5),
+          "paper_ex": cases.paper_ex}[case]()
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    orc = Oracle(tc)
+    ref = om.Reference(tc) if om.reference_available() else None
+    n = ts.num_samples
+    L = tc.sequence_length
+    rng = np.random.default_rng(9)
+    W = rng.normal(size=(n, 3))
+    Z = rng.normal(size=(n, 2))
+    calls = [("trait_covariance", {}, lambda w, m, p: ll.trait_covariance(W, w, m, p)),
+             ("trait_correlation", {}, lambda w, m, p: ll.trait_correlation(W, w, m, p)),
+             ("trait_linear_model", {"Z": Z},
+              lambda w, m, p: ll.trait_linear_model(W, Z, w, m, p)),
+             ("genetic_relatedness_weighted", {"indexes": [(0, 1), (1, 2), (2, 2)]},
+              lambda w, m, p: ll.genetic_relatedness_weighted(W, [(0, 1), (1, 2), (2, 2)], w,
+                                                              m, 1, p, 1)),
+             ("genetic_relatedness_weighted", {"indexes": [(0, 2)], "centre": False},
+              lambda w, m, p: ll.genetic_relatedness_weighted(W, [(0, 2)], w, m, 1, p, 0))]
+    for chunks in (None, 1):
+        if chunks:
+            ll.set_num_chunks(chunks)
+        for w in [np.array([0.0, L]), np.array([0, 0.3 * L, 0.35 * L, L])]:
+            for mode in ("branch", "site"):
+                for pol in (False, True):
+                    _say("weighted", case, chunks, len(w), mode, pol)
+                    for name, kw, gpu in calls:
+                        got = gpu(w, mode, pol)
+                        for impl in [orc] + ([ref] if ref is not None else []):
+                            exp = impl.weighted_stat(name, W, w, mode, True, pol, **kw)
+                            scale = np.max(np.abs(exp), axis=1, keepdims=True)
+                            err = rel_err(got, exp, np.maximum(scale, 1e-300))
+                            assert err <= 1e-9, (name, type(impl).__name__, err)
+                        assert np.array_equal(got, gpu(w, mode, pol)), name
+
+
+def test_general_stat_registry(require_gpu):
+    """TreeSequence.general_stat with a registry summary equals the
+    statistic it defines (trait covariance of centred weights), the strict
+    check rejects a summary that is not zero at the total weight, a Python
+    callable is refused, node mode is not taken, and the Python wrappers
+    drop dimensions as the reference's."""
+    import tskit_amd as ta
+    from tskit_amd import summaries
+
+    tc = cases.generated(120, 2e5, 17)
+    ts = tc.tree_sequence()
+    n = ts.num_samples
+    rng = np.random.default_rng(4)
+    W = rng.normal(size=(n, 2))
+    Wc = W - W.mean(axis=0)
+    w = np.linspace(0, tc.sequence_length, 4)
+    for mode in ("branch", "site"):
+        a = ts.general_stat(Wc, summaries.trait_covariance(n), 2, windows=w, mode=mode)
+        b = ts.trait_covariance(W, windows=w, mode=mode)
+        assert a.shape == b.shape == (3, 2)
+        assert rel_err(a, b, np.max(np.abs(b))) <= 1e-12
+    assert ts.trait_covariance(W, mode="branch").shape == (2,)
+    assert ts.genetic_relatedness_weighted(W, mode="branch").shape == ()
+    assert ts.genetic_relatedness_weighted(W, indexes=[(0, 1), (1, 1)],
+                                           mode="branch").shape == (2,)
+    assert ts.trait_linear_model(W, mode="site").shape == (2,)
+    with pytest.raises(ValueError):
+        ts.general_stat(W, summaries.trait_covariance(n), 2)  # not centred: f(total) != 0
+    with pytest.raises(TypeError):
+        ts.general_stat(Wc, lambda x: x, 2)
+    with pytest.raises(ta.LibraryError):
+        ts.general_stat(Wc, summaries.trait_covariance(n), 2, mode="node")
+    with pytest.raises(ta.LibraryError):
+        ts.ll_tree_sequence.trait_covariance(np.zeros((n, 0)), [0, tc.sequence_length])
+
+
+def test_device_results(require_gpu):
+    """A result buffer in device memory is filled on the device (no copy to
+    the host): a direct call, a batch and a multi-batch run write the same
+    values into torch device tensors as into host buffers."""
+    import ctypes
+
+    import torch
+
+    from tskit_amd._lib import as_c, ptr
+
+    tc = cases.generated(150, 2e5, 23)
+    ts = tc.tree_sequence()
+    ll = ts.ll_tree_sequence
+    lib, h = ll._lib, ll._h
+    sets = cases.sample_sets(150, 3, 20, 8)
+    sizes = as_c(np.array([len(s) for s in sets], dtype=np.uint64), np.uint64)
+    flat = as_c(np.concatenate(sets).astype(np.int32), np.int32)
+    w = as_c(np.linspace(0, tc.sequence_length, 6), np.float64)
+    c_f64p = ctypes.POINTER(ctypes.c_double)
+    BRANCH, SITE, SPAN = 1 << 1, 1 << 0, 1 << 11
+    for opts in (BRANCH | SPAN, SITE | SPAN):
+        host = np.zeros((5, 3))
+        assert lib.tsg_treeseq_diversity(h, 3, ptr(sizes, ctypes.c_uint64),
+                                         ptr(flat, ctypes.c_int32), 5, ptr(w, ctypes.c_double),
+                                         opts, ptr(host, ctypes.c_double)) == 0
+        dev = torch.full((5, 3), -1.0, dtype=torch.float64, device="cuda")
+        _say("device result", opts)
+        assert lib.tsg_treeseq_diversity(h, 3, ptr(sizes, ctypes.c_uint64),
+                                         ptr(flat, ctypes.c_int32), 5, ptr(w, ctypes.c_double),
+                                         opts, ctypes.cast(dev.data_ptr(), c_f64p)) == 0
+        assert np.array_equal(dev.cpu().numpy(), host)
+        # in a batch (one device, one host result), and two batches run together
+        devs = [torch.full((5, 3), -1.0, dtype=torch.float64, device="cuda") for _ in range(2)]
+        hosts = [np.zeros((5, 3)) for _ in range(2)]
+        bs = []
+        for k in range(2):
+            b = ctypes.c_void_p()
+            assert lib.tsg_batch_init(ctypes.byref(b), h) == 0
+            for out in (ctypes.cast(devs[k].data_ptr(), c_f64p), ptr(hosts[k], ctypes.c_double)):
+                assert lib.tsg_batch_diversity(b, 3, ptr(sizes, ctypes.c_uint64),
+                                               ptr(flat, ctypes.c_int32), 5,
+                                               ptr(w, ctypes.c_double), opts, out) == 0
+            bs.append(b)
+        assert lib.tsg_batch_run(bs[0]) == 0
+        handles = (ctypes.c_void_p * 1)(bs[1].value)
+        assert lib.tsg_batch_run_multi(handles, 1) == 0
+        for k in range(2):
+            assert np.array_equal(hosts[k], host)
+            assert np.array_equal(devs[k].cpu().numpy(), host)
+            lib.tsg_batch_free(bs[k])

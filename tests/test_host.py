@@ -431,6 +431,38 @@ def test_relatedness_vector_errors(paper):
         paper.genetic_relatedness_vector(np.ones(4))  # mode="site", as the reference
 
 
+def test_weighted_stat_errors(paper):
+    """Argument errors of general_stat and the weighted statistics, in the
+    reference's order and numbers (trees.c:2048-2075, 3976-3995,
+    4839-4860), raised before any device work."""
+    from tskit_amd import summaries
+
+    ll = paper.ll_tree_sequence
+    W = np.ones((4, 2))
+    w = [0, 10]
+    cov = summaries.trait_covariance(4)
+    assert _err(lambda: ll.general_stat(np.ones((4, 0)), cov, 1, w)) == -903
+    assert _err(lambda: ll.general_stat(W, cov, 0, w)) == -904
+    assert _err(lambda: ll.general_stat(W, cov, 2, [0, 5])) == -901
+    assert _err(lambda: ll.general_stat(W, cov, 3, w)) == -904      # result_dim != state_dim
+    assert _err(lambda: ll.general_stat(W, cov, 2, w, mode="node")) == -909
+    bad = summaries.relatedness_weighted_noncentred([(0, 2)])
+    assert _err(lambda: ll.general_stat(W, bad, 1, w)) == -907
+    assert _err(lambda: ll.trait_covariance(np.ones((4, 0)), w)) == -913
+    assert _err(lambda: ll.trait_correlation(np.ones((4, 0)), w)) == -913
+    assert _err(lambda: ll.genetic_relatedness_weighted(np.ones((4, 0)), [(0, 0)], w)) == -913
+    with pytest.raises(ValueError):
+        ll.trait_covariance(np.ones((3, 2)), w)
+    with pytest.raises(ValueError):
+        paper.trait_covariance(np.ones((3, 2)))
+    with pytest.raises(ValueError):
+        paper.trait_correlation(np.ones((4, 2)))  # zero variance
+    with pytest.raises(ValueError):
+        paper.genetic_relatedness_weighted(np.ones((4, 3)))  # three columns, no indexes
+    with pytest.raises(ValueError):
+        paper.trait_linear_model(np.ones((4, 1)), Z=np.ones((3, 1)))
+
+
 def test_serve_memory_admission():
     """A job starts only while the launches in flight and its own fit the
     device-memory budget, or when nothing is in flight."""

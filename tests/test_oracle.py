@@ -602,6 +602,87 @@ def test_relatedness_vector_closed_form(case):
                            atol=1e-12 * np.max(np.abs(Gj @ W)))
 
 
+WEIGHTED = ["trait_covariance", "trait_correlation", "trait_linear_model",
+            "genetic_relatedness_weighted"]
+
+
+def _weighted_calls(n, rng):
+    """(name, keyword arguments) of each weighted statistic on random
+    weights of n samples."""
+    Z = rng.normal(size=(n, 2))
+    return [("trait_covariance", {}), ("trait_correlation", {}),
+            ("trait_linear_model", {"Z": Z}),
+            ("genetic_relatedness_weighted", {"indexes": [(0, 1), (1, 2), (2, 2)]}),
+            ("genetic_relatedness_weighted", {"indexes": [(0, 2), (1, 1)], "centre": False})]
+
+
+@needs_ref
+@pytest.mark.parametrize("fixture", ["paper_ex", "nonbinary_ex", "generated", "multiallelic"])
+def test_weighted_stats_vs_reference(fixture):
+    """The restated weighted statistics (general_stat of real-valued weights
+    with the trait and weighted-relatedness summaries, trees.c:3961-4220,
+    4801-4895) against the compiled reference, both modes and
+    polarisations, windows spanning the sequence in several pieces."""
+    tc = {"generated": lambda: cases.generated(40, 3e4, 12),
+          "multiallelic": lambda: cases.multiallelic(30, 2e4, 3)}.get(
+              fixture, getattr(cases, fixture, None))()
+    orc, ref = Oracle(tc), om.Reference(tc)
+    n = int(np.sum(tc.node_flags & 1))
+    rng = np.random.default_rng(3)
+    W = rng.normal(size=(n, 3))
+    L = tc.sequence_length
+    for w in [None, np.array([0, 0.3 * L, 0.35 * L, L])]:
+        for mode in ("branch", "site"):
+            for pol in (False, True):
+                for name, kw in _weighted_calls(n, rng):
+                    a = orc.weighted_stat(name, W, w, mode, True, pol, **kw)
+                    b = ref.weighted_stat(name, W, w, mode, True, pol, **kw)
+                    assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.max(np.abs(b))), name
+    with pytest.raises(om.OracleError):
+        orc.weighted_stat("trait_covariance", W, np.array([0.1, L]))
+    with pytest.raises(om.OracleError):
+        ref.weighted_stat("trait_covariance", W, np.array([0.1, L]))
+
+
+@pytest.mark.parametrize("case", ["generated", "gapped", "paper_ex"])
+def test_weighted_stats_closed_form(case):
+    """Branch mode, polarised, unnormalised: with G the branch length each
+    pair of samples shares (the non-centred relatedness of singleton sets,
+    itself checked against the reference), trait covariance is
+    w'Gw / (2 (n-1)^2) for the centred trait w, and weighted relatedness is
+    v_i'G v_j with v the centred weights (or the raw weights, non-centred):
+    the summaries' sums over the nodes, sum bl x x', written as quadratic
+    forms."""
+    tc = {"generated": lambda: cases.generated(30, 4e4, 8),
+          "gapped": lambda: cases.gapped(30, 4e4, 8, count=6),
+          "paper_ex": cases.paper_ex}[case]()
+    orc = Oracle(tc)
+    smp = np.flatnonzero(tc.node_flags & 1)
+    n = len(smp)
+    single = [np.array([i], np.int32) for i in smp]
+    pairs = [(i, j) for i in range(n) for j in range(n)]
+    L = tc.sequence_length
+    w = np.array([0, 0.4 * L, L])
+    G = orc.stat("genetic_relatedness", single, pairs, w, "branch", False, True, False)
+    W = np.random.default_rng(7).normal(size=(n, 3))
+    Wc = W - W.mean(axis=0)
+    cov = orc.weighted_stat("trait_covariance", W, w, "branch", False, True)
+    idx = [(0, 1), (2, 2), (1, 0)]
+    rel = orc.weighted_stat("genetic_relatedness_weighted", W, w, "branch", False, True,
+                            indexes=idx)
+    reln = orc.weighted_stat("genetic_relatedness_weighted", W, w, "branch", False, True,
+                             indexes=idx, centre=False)
+    for j in range(2):
+        Gj = G[j].reshape(n, n)
+        exp = np.einsum("sk,st,tk->k", Wc, Gj, Wc) / (2 * (n - 1) ** 2)
+        assert np.allclose(cov[j], exp, rtol=1e-10, atol=1e-12 * np.max(np.abs(exp)))
+        for m, (a, b) in enumerate(idx):
+            e1 = Wc[:, a] @ Gj @ Wc[:, b]
+            e2 = W[:, a] @ Gj @ W[:, b]
+            assert np.isclose(rel[j, m], e1, rtol=1e-10, atol=1e-12 * np.max(np.abs(Gj)))
+            assert np.isclose(reln[j, m], e2, rtol=1e-10, atol=1e-12 * np.max(np.abs(Gj)))
+
+
 @needs_ref
 def test_window_partitioned_reference_equals_whole():
     """The CPU baseline's window-partitioned reference (each process clips

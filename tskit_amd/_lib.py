@@ -109,6 +109,29 @@ SIGNATURES = {
         [c_void, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_i32p,
          c_f64p, ctypes.c_uint32],
     ),
+    "tsg_treeseq_general_stat": (
+        ctypes.c_int,
+        [c_void, ctypes.c_uint64, c_f64p, ctypes.c_uint64, ctypes.c_int32, c_i32p, c_f64p,
+         ctypes.c_uint64, c_f64p, ctypes.c_uint32, c_f64p],
+    ),
+    "tsg_treeseq_trait_covariance": (
+        ctypes.c_int,
+        [c_void, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_f64p, ctypes.c_uint32, c_f64p],
+    ),
+    "tsg_treeseq_trait_correlation": (
+        ctypes.c_int,
+        [c_void, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_f64p, ctypes.c_uint32, c_f64p],
+    ),
+    "tsg_treeseq_trait_linear_model": (
+        ctypes.c_int,
+        [c_void, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_f64p,
+         ctypes.c_uint32, c_f64p],
+    ),
+    "tsg_treeseq_genetic_relatedness_weighted": (
+        ctypes.c_int,
+        [c_void, ctypes.c_uint64, c_f64p, ctypes.c_uint64, c_i32p, ctypes.c_uint64, c_f64p,
+         c_f64p, ctypes.c_uint32],
+    ),
     "tsg_batch_diversity": (ctypes.c_int, ONE_WAY),
     "tsg_batch_segregating_sites": (ctypes.c_int, ONE_WAY),
     "tsg_batch_Y1": (ctypes.c_int, ONE_WAY),

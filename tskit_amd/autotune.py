@@ -3,10 +3,11 @@
 The sweep cuts a tree sequence's trees into chunks, one wavefront each; a
 chunk is a serial chain of edge events, so more, shorter chunks run faster
 as long as their hot slots stay small enough for several chunks per CU.
-The best events-per-chunk depends on the tree sequence's shape.
-``tune(ts)`` times a probe query (diversity and divergence of two sample
-sets over ten windows, branch mode, with HIP-event timing of the sweep
-kernel) at a few candidate chunk counts, keeps the fastest and records it,
+The best events-per-chunk depends on the tree sequence's shape, and on the
+host work per chunk that every call also pays (window ranges per chunk and
+query).  ``tune(ts)`` times a probe query (diversity and divergence of two
+sample sets over ten windows, branch mode, one fused batch) end to end at a
+few candidate chunk counts, keeps the fastest and records it,
 keyed by the shape (log2 buckets of samples and edges), in a JSON cache;
 ``apply(ts)`` sets a tree sequence's chunk count from the cache.  The cache
 file is ``$TSKIT_AMD_TUNE_CACHE`` or ``~/.cache/tskit_amd/tune.json``.
@@ -57,32 +58,36 @@ def chunks_for(ts, events_per_chunk):
 
 
 def _probe(ts, reps):
-    ll = ts.ll_tree_sequence
+    """Wall-clock milliseconds of the probe batch (host work, sweep and
+    finish), the fastest of `reps` runs after a warm one."""
+    import time
+
     n = ts.num_samples
     smp = ts.samples()
     half = max(1, n // 2)
     sets = [smp[:half], smp[half:]] if n >= 2 else [smp, smp]
     w = np.linspace(0, ts.sequence_length, 11)
-    b = ts.batch()
-    b.diversity(sets, windows=w, mode="branch")
-    b.divergence(sets, windows=w, mode="branch")
-    b.run()  # warm the workspace
-    ll.set_profiling(True)
-    for _ in range(reps):
+
+    def run():
         b = ts.batch()
         b.diversity(sets, windows=w, mode="branch")
         b.divergence(sets, windows=w, mode="branch")
         b.run()
-    ms, launches = ll.get_profile()
-    ll.set_profiling(False)
-    return ms / max(launches, 1)
+
+    run()  # warm the workspace
+    best = float("inf")
+    for _ in range(reps):
+        t = time.perf_counter()
+        run()
+        best = min(best, time.perf_counter() - t)
+    return best * 1e3
 
 
 def tune(ts, candidates=CANDIDATES, reps=3, path=None, table=None):
     """Times the probe at each candidate events-per-chunk, applies the
     fastest to ts and records it in the cache (or in `table`, when given,
     without writing).  Returns {"events_per_chunk": best, "chunks": ...,
-    "sweep_ms": {candidate: ms}}."""
+    "probe_ms": {candidate: ms}}."""
     own = table is None
     if own:
         table = load(path)
@@ -92,7 +97,7 @@ def tune(ts, candidates=CANDIDATES, reps=3, path=None, table=None):
         times[epc] = _probe(ts, reps)
     best = min(times, key=times.get)
     ts.ll_tree_sequence.set_num_chunks(chunks_for(ts, best))
-    entry = {"events_per_chunk": int(best), "sweep_ms": {str(k): v for k, v in times.items()}}
+    entry = {"events_per_chunk": int(best), "probe_ms": {str(k): v for k, v in times.items()}}
     table[shape_key(ts)] = entry
     if own:
         save(table, path)
@@ -130,6 +135,8 @@ def tune_batch(tss, run, candidates=CANDIDATES, reps=2):
     for ts in tss:
         ts.ll_tree_sequence.set_num_chunks(chunks_for(ts, pick))
         ts.ll_tree_sequence.upload()
+    run()  # warm the workspaces at the chosen plan
+    torch.cuda.synchronize()
     return {"events_per_chunk": int(pick), "seconds": {str(k): v for k, v in times.items()}}
 
 

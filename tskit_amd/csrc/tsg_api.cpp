@@ -68,6 +68,10 @@ struct Workspace {
     // a batch's results, copied back in one transfer (pinned)
     void *out_h = nullptr;
     size_t out_h_bytes = 0;
+    // large results into pageable host memory: two pinned pieces in turn
+    // (copy_out)
+    void *cp_h[2] = {nullptr, nullptr};
+    hipEvent_t cp_ev[2] = {nullptr, nullptr};
     // relatedness-vector route
     DevBuf rv_E, rv_I, rv_z0, rv_d0, rv_esum, rv_y;
     // relatedness-matrix route
@@ -263,12 +267,13 @@ struct Lease {
 // root path once: seed_sample[i] has columns seed_col[seed_off[i]..seed_off[i+1]).
 struct Seeds {
     std::vector<int32_t> seed_sample, seed_off, col, flat_s;
+    std::vector<double> w;  // the pairs' weights, in flat order (weighted runs)
 };
 
 // pairs by sample (columns of one sample in their original order): the CSR
 // form (one entry per weighted sample) and the flat pair list
 void seeds_host(const std::vector<int32_t> &pair_s, const std::vector<int32_t> &pair_c,
-                Seeds &sd) {
+                Seeds &sd, const std::vector<double> *pair_w = nullptr) {
     std::vector<int32_t> order(pair_s.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
@@ -277,10 +282,12 @@ void seeds_host(const std::vector<int32_t> &pair_s, const std::vector<int32_t> &
     sd.seed_off.assign(1, 0);
     sd.col.resize(order.size());
     sd.flat_s.resize(order.size());
+    if (pair_w != nullptr) sd.w.resize(order.size());
     for (size_t i = 0; i < order.size(); i++) {
         const int32_t si = pair_s[order[i]];
         sd.col[i] = pair_c[order[i]];
         sd.flat_s[i] = si;
+        if (pair_w != nullptr) sd.w[i] = (*pair_w)[order[i]];
         if (sd.seed_sample.empty() || sd.seed_sample.back() != si) {
             if (!sd.seed_sample.empty()) sd.seed_off.push_back((int32_t) i);
             sd.seed_sample.push_back(si);
@@ -389,7 +396,15 @@ struct Query {
     std::vector<int32_t> fin_ipar;
     std::vector<double> fin_dpar;
     double *result = nullptr;
+    // real-valued sample weights (tsg_treeseq_general_stat): K columns,
+    // [n x K] by sample index, the outputs over them and their parameters
+    bool weighted = false;
+    int wK = 0;
+    std::vector<double> wts;
+    std::vector<TsgOut> wouts;
+    std::vector<double> opar;
     int M() const {
+        if (weighted) return (int) wouts.size();
         int m = 0;
         for (const Block &b : blocks) m += (b.tuple == 0) ? (int) sets.size() : (int) (b.idx.size() / b.tuple);
         return m;
@@ -528,7 +543,112 @@ struct BatchRun {
     // pinned buffer, or (several runs finished together) this run's part of
     // a shared one
     const double *results_h = nullptr;
+    // weighted runs: the first tree's sums of the weights [nloc0 x K] (device)
+    const double *x0w_dev = nullptr;
+    // per query: its result buffer is device memory (filled by a device
+    // copy on the run's stream, not from the pinned host copy)
+    std::vector<char> on_dev;
 };
+
+// Whether p is device memory (a caller may hand a device buffer for the
+// results: it is then filled on the device, no copy to the host).
+bool is_device_ptr(const void *p) {
+    hipPointerAttribute_t attr;
+    if (p == nullptr || hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void) hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice;
+}
+
+// `bytes` of device memory src into the caller's dst, waiting for stream s
+// to finish: a device copy when dst is device memory, one transfer when it
+// is pinned host memory or small; otherwise through two pinned pieces in
+// turn, the transfer of one piece running while host threads copy the
+// previous one into dst (pageable memory takes its transfers slowly and its
+// first touch is paid in the copy; spread over threads, both go faster).
+constexpr size_t TSG_COPY_PIECE = (size_t) 64 << 20;
+
+int copy_out(Workspace *ws, hipStream_t s, void *dst, const void *src, size_t bytes) {
+    int ret;
+    if (bytes == 0) return hip_check(hipStreamSynchronize(s));
+    hipPointerAttribute_t attr;
+    hipMemoryType type = hipMemoryTypeUnregistered;
+    if (hipPointerGetAttributes(&attr, dst) == hipSuccess) {
+        type = attr.type;
+    } else {
+        (void) hipGetLastError();
+    }
+    if (type == hipMemoryTypeDevice || type == hipMemoryTypeHost || bytes <= TSG_COPY_PIECE ||
+        ws->fixed != nullptr) {
+        const hipMemcpyKind kind =
+            type == hipMemoryTypeDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        if ((ret = hip_check(hipMemcpyAsync(dst, src, bytes, kind, s)))) return ret;
+        return hip_check(hipStreamSynchronize(s));
+    }
+    for (int b = 0; b < 2; b++) {
+        if (ws->cp_h[b] == nullptr && hipHostMalloc(&ws->cp_h[b], TSG_COPY_PIECE) != hipSuccess) {
+            (void) hipGetLastError();
+            ws->cp_h[b] = nullptr;
+            return TSG_ERR_NO_MEMORY;
+        }
+        if (ws->cp_ev[b] == nullptr &&
+            (ret = hip_check(hipEventCreateWithFlags(&ws->cp_ev[b], hipEventDisableTiming)))) {
+            return ret;
+        }
+    }
+    const unsigned hc = std::thread::hardware_concurrency();
+    const size_t T = std::max<size_t>(1, std::min<size_t>(8, hc == 0 ? 1 : hc));
+    auto host_copy = [&](char *to, const char *from, size_t n) {
+        const size_t per = (n + T - 1) / T;
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < T && t * per < n; t++) {
+            pool.emplace_back([=]() { std::memcpy(to + t * per, from + t * per, std::min(per, n - t * per)); });
+        }
+        std::memcpy(to, from, std::min(per, n));
+        for (auto &th : pool) th.join();
+    };
+    const size_t np = (bytes + TSG_COPY_PIECE - 1) / TSG_COPY_PIECE;
+    for (size_t i = 0; i <= np; i++) {
+        if (i < np) {
+            const size_t off = i * TSG_COPY_PIECE, n = std::min(TSG_COPY_PIECE, bytes - off);
+            const int b = (int) (i & 1);
+            if ((ret = hip_check(hipMemcpyAsync(ws->cp_h[b], (const char *) src + off, n,
+                                                hipMemcpyDeviceToHost, s))) ||
+                (ret = hip_check(hipEventRecord(ws->cp_ev[b], s)))) {
+                (void) hipStreamSynchronize(s);
+                return ret;
+            }
+        }
+        if (i >= 1) {  // piece i - 1 has arrived in its pinned buffer
+            const size_t off = (i - 1) * TSG_COPY_PIECE, n = std::min(TSG_COPY_PIECE, bytes - off);
+            const int b = (int) ((i - 1) & 1);
+            if ((ret = hip_check(hipEventSynchronize(ws->cp_ev[b])))) {
+                (void) hipStreamSynchronize(s);
+                return ret;
+            }
+            host_copy((char *) dst + off, (const char *) ws->cp_h[b], n);
+        }
+    }
+    return hip_check(hipStreamSynchronize(s));
+}
+
+// Device copies of the results of the queries whose buffers are device
+// memory, from the run's result block `out` (queued on stream s).
+int copy_device_results(BatchRun &r, const double *out, hipStream_t s) {
+    r.on_dev.assign(r.Q, 0);
+    for (int qi = 0; qi < r.Q; qi++) {
+        double *dst = (*r.batch)[qi]->result;
+        if (!is_device_ptr(dst)) continue;
+        r.on_dev[qi] = 1;
+        const TsgFinalHost &f = r.fin[qi];
+        const size_t n = (size_t) f.W * f.P * 8;
+        if (n == 0) continue;
+        int ret = hip_check(hipMemcpyAsync(dst, out + f.outbase, n, hipMemcpyDeviceToDevice, s));
+        if (ret != 0) return ret;
+    }
+    return 0;
+}
 
 // The fused state columns and outputs of a batch: one column per distinct
 // sample set, one TsgOut per raw output, per-query windows and offsets.
@@ -539,10 +659,53 @@ struct Fused {
     std::vector<double> wins;
     bool noskip = false, any_unpolarised = false;
     int K = 0, M = 0;
+    // real-valued weights: per pair its weight, column totals, parameters
+    bool weighted = false;
+    std::vector<double> pair_w, col_total_w, opar;
 };
+
+// A query of real-valued weights runs alone: its K columns as given (pairs
+// of the nonzero weights, by sample), totals summed in sample order as the
+// reference does (trees.c:1990-1998).
+void fuse_weighted(const Query &q, Fused &f) {
+    const int K = q.wK;
+    const size_t n = K > 0 ? q.wts.size() / (size_t) K : 0;
+    f.weighted = true;
+    f.K = K;
+    f.col_total_w.assign(K, 0.0);
+    f.col_total.assign(K, 0);
+    for (size_t s = 0; s < n; s++) {
+        for (int k = 0; k < K; k++) {
+            const double w = q.wts[s * K + k];
+            f.col_total_w[k] += w;
+            if (w == 0.0) continue;
+            f.pair_s.push_back((int32_t) s);
+            f.pair_c.push_back(k);
+            f.pair_w.push_back(w);
+        }
+    }
+    f.outs = q.wouts;
+    f.M = (int) f.outs.size();
+    f.opar = q.opar;
+    for (TsgOut &o : f.outs) {
+        o.q = 0;
+        if (!std::isfinite(o.inv)) f.noskip = true;
+        if (!(o.flags & 1)) f.any_unpolarised = true;
+    }
+    f.qs.assign(1, TsgQuery{});
+    TsgQuery &tq = f.qs[0];
+    tq.W = (int32_t) q.windows.size() - 1;
+    tq.M = f.M;
+    tq.flags = q.span_normalise ? TSG_F_SPAN_NORMALISE : 0u;
+    f.wins = q.windows;
+}
 
 void fuse_queries(std::vector<Query *> &batch, Fused &f) {
     const int Q = (int) batch.size();
+    if (Q == 1 && batch[0]->weighted) {
+        fuse_weighted(*batch[0], f);
+        return;
+    }
     std::vector<int32_t> &pair_s = f.pair_s, &pair_c = f.pair_c, &col_total = f.col_total;
     std::vector<TsgOut> &outs = f.outs;
     std::vector<TsgQuery> &qs = f.qs;
@@ -719,13 +882,20 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     // outputs per launch: up to 256 (four per lane; wider launches would
     // spill the per-output polynomial coefficients out of registers)
     const int slice = 256;
-    const size_t stage = (size_t) TSG_STAGE_INTS(K, std::min(M, slice)) * 4;
+    // real-valued weights: FP64 state columns, on the hot plan only
+    const bool wgt = fz.weighted;
+    const size_t stage = (size_t) (wgt ? TSG_STAGE_INTS_W(K, std::min(M, slice))
+                                       : TSG_STAGE_INTS(K, std::min(M, slice))) * 4;
     // 16-bit slot counts when no sample set exceeds 65535 samples
-    const bool c16 = !ws->force_c32 &&
+    const bool c16 = !wgt && !ws->force_c32 &&
                      *std::max_element(col_total.begin(), col_total.end()) <= 65535;
+    const bool bl = mode == TSG_STAT_BRANCH;
     const size_t hot_need =
-        (size_t) ts->hot.max_slots * TSG_SLOT_WORDS(K, c16, mode == TSG_STAT_BRANCH) * 4 + stage;
-    const bool use_hot = ts->hot.usable && !ws->force_full && hot_need <= (size_t) ws->lds_limit;
+        (size_t) ts->hot.max_slots * (wgt ? TSG_SLOT_WORDS_W(K, bl) : TSG_SLOT_WORDS(K, c16, bl)) *
+            4 + stage;
+    const bool use_hot = ts->hot.usable && (wgt || !ws->force_full) &&
+                         hot_need <= (size_t) ws->lds_limit;
+    if (wgt && !use_hot) return TSG_ERR_UNSUPPORTED_INPUT;
     if (!use_hot) {
         // the full plan is built and uploaded on first need: not in a
         // no-allocation run
@@ -779,7 +949,21 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     // the host arrays go over in one copy at the end (pk); their device
     // pointers are set then
     Seeds sd;
-    seeds_host(pair_s, pair_c, sd);
+    seeds_host(pair_s, pair_c, sd, wgt ? &fz.pair_w : nullptr);
+    // weighted branch runs: the first tree's sums of the weights, each
+    // sample's weights added up its root path in sample order (the hot
+    // plan's first tree numbers samples first), so the sums are the same on
+    // every run
+    std::vector<double> x0w;
+    if (wgt && mode == TSG_STAT_BRANCH) {
+        const std::vector<int32_t> &par0 = ts->hot.t0_parent;
+        x0w.assign(par0.size() * (size_t) K, 0.0);
+        for (size_t i = 0; i < sd.flat_s.size(); i++) {
+            const int32_t k = sd.col[i];
+            const double w = sd.w[i];
+            for (int32_t u = sd.flat_s[i]; u >= 0; u = par0[u]) x0w[(size_t) u * K + k] += w;
+        }
+    }
     Pack pk;
     const int p_ss = pk.add(sd.seed_sample), p_so = pk.add(sd.seed_off);
     const int p_col = pk.add(sd.col), p_fs = pk.add(sd.flat_s);
@@ -804,7 +988,7 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
             return ret;
         }
     }
-    int store = c16 ? TSG_STORE_HOT16 : TSG_STORE_HOT;
+    int store = wgt ? TSG_STORE_HOTW : c16 ? TSG_STORE_HOT16 : TSG_STORE_HOT;
     size_t shmem = hot_need;
     const TsgDevice &d = ts->dev;
     const TsgHotDevice &h = ts->hot.dev;
@@ -928,6 +1112,13 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     }
     r.outcount = outcount;
     const int p_fin = pk.add(fin), p_ip = pk.add(ipar), p_dp = pk.add(dpar);
+    int p_pw = -1, p_tw = -1, p_op = -1, p_x0 = -1;
+    if (wgt) {
+        p_pw = pk.add(sd.w);
+        p_tw = pk.add(fz.col_total_w);
+        p_op = pk.add(fz.opar);
+        if (!x0w.empty()) p_x0 = pk.add(x0w);
+    }
     if ((ret = ensure(ws, ws->out, (size_t) outcount * 8)) || (ret = pk.send(ws, s))) return ret;
     a.npairs = (int32_t) sd.flat_s.size();
     a.pair_sample = pk.dev<const int32_t>(ws, p_fs);
@@ -953,6 +1144,12 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     r.fin_dev = pk.dev<const void>(ws, p_fin);
     r.ipar_dev = pk.dev<const int32_t>(ws, p_ip);
     r.dpar_dev = pk.dev<const double>(ws, p_dp);
+    if (wgt) {
+        a.pair_w = pk.dev<const double>(ws, p_pw);
+        a.col_total_w = pk.dev<const double>(ws, p_tw);
+        a.opar = pk.dev<const double>(ws, p_op);
+        r.x0w_dev = p_x0 >= 0 ? pk.dev<const double>(ws, p_x0) : nullptr;
+    }
     return 0;
 }
 
@@ -981,8 +1178,12 @@ int batch_finish(BatchRun &r) {
     hipStream_t s = r.s;
     const TsgSweepArgs &a = r.a;
     if (r.mode == TSG_STAT_BRANCH) {
-        if ((ret = hip_check(tsg_launch_tree0(a, r.t0_parent, r.t0_bl, (int32_t *) ws->x0.p,
-                                              r.nloc0, (double *) ws->r0part.p, s)))) {
+        if ((ret = hip_check(
+                 r.x0w_dev != nullptr
+                     ? tsg_launch_tree0_w(a, r.t0_parent, r.t0_bl, r.x0w_dev, r.nloc0,
+                                          (double *) ws->r0part.p, s)
+                     : tsg_launch_tree0(a, r.t0_parent, r.t0_bl, (int32_t *) ws->x0.p, r.nloc0,
+                                        (double *) ws->r0part.p, s)))) {
             return ret;
         }
     }
@@ -997,9 +1198,11 @@ int batch_finish(BatchRun &r) {
                                            r.outcount, s)))) {
         return ret;
     }
+    if ((ret = copy_device_results(r, (const double *) ws->out.p, s))) return ret;
     if (ws->fixed != nullptr) {
         // no pinned buffer: each query's results straight into its buffer
         for (int qi = 0; qi < r.Q; qi++) {
+            if (r.on_dev[qi]) continue;
             const TsgFinalHost &f = r.fin[qi];
             const size_t n = (size_t) f.W * f.P * 8;
             if (n == 0) continue;
@@ -1033,8 +1236,10 @@ int batch_finish(BatchRun &r) {
 
 // After r.s has finished: each query's results from the pinned copy.
 void batch_deliver(const BatchRun &r) {
+    auto dev = [&](int qi) { return qi < (int) r.on_dev.size() && r.on_dev[qi]; };
     if (r.results_h != nullptr) {
         for (int qi = 0; qi < r.Q; qi++) {
+            if (dev(qi)) continue;
             const TsgFinalHost &f = r.fin[qi];
             std::memcpy((*r.batch)[qi]->result, r.results_h + f.outbase, (size_t) f.W * f.P * 8);
         }
@@ -1042,6 +1247,7 @@ void batch_deliver(const BatchRun &r) {
     }
     if (r.ws->fixed != nullptr) return;  // copied by batch_finish
     for (int qi = 0; qi < r.Q; qi++) {
+        if (dev(qi)) continue;
         const TsgFinalHost &f = r.fin[qi];
         std::memcpy((*r.batch)[qi]->result, (const double *) r.ws->out_h + f.outbase,
                     (size_t) f.W * f.P * 8);
@@ -1359,9 +1565,7 @@ int run_node(tsg_treeseq *ts, Query &q) {
     if ((ret = hip_check(tsg_launch_finals(ws->fin.p, 1, raw, (const int32_t *) ws->ipar.p,
                                            (const double *) ws->dpar.p, (double *) ws->out.p,
                                            nout, s))) ||
-        (ret = hip_check(hipMemcpyAsync(q.result, ws->out.p, (size_t) nout * 8,
-                                        hipMemcpyDeviceToHost, s))) ||
-        (ret = hip_check(hipStreamSynchronize(s)))) {
+        (ret = copy_out(ws, s, q.result, ws->out.p, (size_t) nout * 8))) {
         return ret;
     }
     if (ws->profile) {
@@ -1390,6 +1594,10 @@ void free_one_workspace(Workspace *ws) {
     if (ws->stage_h != nullptr) (void) hipHostFree(ws->stage_h);
     if (ws->out_h != nullptr) (void) hipHostFree(ws->out_h);
     if (ws->mout_h != nullptr) (void) hipHostFree(ws->mout_h);
+    for (int b = 0; b < 2; b++) {
+        if (ws->cp_h[b] != nullptr) (void) hipHostFree(ws->cp_h[b]);
+        if (ws->cp_ev[b]) (void) hipEventDestroy(ws->cp_ev[b]);
+    }
     if (ws->ev0) (void) hipEventDestroy(ws->ev0);
     if (ws->ev1) (void) hipEventDestroy(ws->ev1);
     if (ws->stream) (void) hipStreamDestroy(ws->stream);
@@ -1495,6 +1703,9 @@ const char *tsg_strerror(int err) {
     case TSG_ERR_EMPTY_SAMPLE_SET: return "Samples cannot be empty";
     case TSG_ERR_UNSUPPORTED_STAT_MODE: return "Requested statistics mode not supported for this method";
     case TSG_ERR_TIME_UNCALIBRATED: return "Statistics using branch lengths cannot be calculated when time_units is 'uncalibrated'";
+    case TSG_ERR_STAT_POLARISED_UNSUPPORTED: return "The polarised option is not supported by this statistic";
+    case TSG_ERR_STAT_SPAN_NORMALISE_UNSUPPORTED: return "The span_normalise option is not supported by this statistic";
+    case TSG_ERR_INSUFFICIENT_WEIGHTS: return "Insufficient weights provided (at least 1 required)";
     case TSG_ERR_BAD_TIME_WINDOWS_DIM: return "Must have at least one time window";
     case TSG_ERR_BAD_TIME_WINDOWS: return "Time windows must be strictly increasing and begin at 0";
     case TSG_ERR_DEVICE: return "A HIP runtime call failed";
@@ -2375,11 +2586,7 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);
     const void *res_dev = singletons ? ws->kout.p : ws->out.p;
     const int nr = singletons ? n : nsets;
-    if ((ret = hip_check(hipMemcpyAsync(result, res_dev, (size_t) W * nr * (size_t) nr * 8,
-                                        hipMemcpyDeviceToHost, s)))) {
-        return ret;
-    }
-    if ((ret = hip_check(hipStreamSynchronize(s)))) return ret;
+    if ((ret = copy_out(ws, s, result, res_dev, (size_t) W * nr * (size_t) nr * 8))) return ret;
     if (ws->profile) {
         float ms = 0;
         (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);
@@ -2572,9 +2779,7 @@ int relvec_run(tsg_treeseq *ts, int K, const std::vector<double> &wc,
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);
     if ((ret = hip_check(tsg_launch_relvec_final(y, a.windows, W, (int) nf, K, centre,
                                                  span_normalise, s))) ||
-        (ret = hip_check(hipMemcpyAsync(result, y, (size_t) W * nf * K * 8,
-                                        hipMemcpyDeviceToHost, s))) ||
-        (ret = hip_check(hipStreamSynchronize(s)))) {
+        (ret = copy_out(ws, s, result, y, (size_t) W * nf * K * 8))) {
         return ret;
     }
     if (ws->profile) {
@@ -2640,6 +2845,204 @@ int tsg_treeseq_genetic_relatedness_vector(tsg_treeseq *ts, uint64_t num_weights
                       (options & TSG_STAT_SPAN_NORMALISE) != 0, result);
 }
 
+// ---- general_stat with real-valued weights (trees.c:2036-2095) -----------
+// The summary function is chosen from the registry by id (tsg_stat_id, the
+// weighted ones) with its parameters, in place of general_stat_func_t and
+// f_params (trees.h:1032-1037):
+//   TSG_STAT_TRAIT_COVARIANCE        result_dim = state_dim (trees.c:3961)
+//   TSG_STAT_TRAIT_CORRELATION       result_dim = state_dim - 1, the last
+//                                    column the frequency (trees.c:4026)
+//   TSG_STAT_TRAIT_LINEAR_MODEL      state = [traits | covariates | 1],
+//                                    dparams = V [result_dim x covariates]
+//                                    (trees.c:4108)
+//   TSG_STAT_RELATEDNESS_WEIGHTED(_NONCENTRED)  index_tuples [result_dim x 2],
+//                                    dparams = total weights [state_dim], the
+//                                    last column the frequency (trees.c:4801)
+// Branch and site mode on the hot plan (FP64 state columns); node mode and
+// statistics inside a batch are not taken.
+int tsg_treeseq_general_stat(tsg_treeseq *ts, uint64_t state_dim, const double *sample_weights,
+                             uint64_t result_dim, int32_t summary, const int32_t *index_tuples,
+                             const double *dparams, uint64_t num_windows, const double *windows,
+                             uint32_t options, double *result) {
+    // the reference's order of checks (trees.c:2048-2075)
+    const int nmodes = !!(options & TSG_STAT_SITE) + !!(options & TSG_STAT_BRANCH) +
+                       !!(options & TSG_STAT_NODE);
+    if (nmodes > 1) return TSG_ERR_MULTIPLE_STAT_MODES;
+    if (state_dim < 1) return TSG_ERR_BAD_STATE_DIMS;
+    if (result_dim < 1) return TSG_ERR_BAD_RESULT_DIMS;
+    Query q;
+    int ret = parse_mode_windows(ts, options, num_windows, windows, q);
+    if (ret != 0) return ret;
+    if (options & TSG_STAT_NODE) return TSG_ERR_UNSUPPORTED_STAT_MODE;
+    if (g_capture != nullptr) return TSG_ERR_UNSUPPORTED_INPUT;
+    if (state_dim > 4096 || result_dim > (1u << 20)) return TSG_ERR_UNSUPPORTED_INPUT;
+    const int K = (int) state_dim, P = (int) result_dim;
+    const int n = ts->num_samples;
+    const bool pol = (options & TSG_STAT_POLARISED) != 0;
+    q.weighted = true;
+    q.wK = K;
+    q.wts.assign(sample_weights, sample_weights + (size_t) n * K);
+    q.wouts.resize(P);
+    for (int m = 0; m < P; m++) {
+        TsgOut &o = q.wouts[m];
+        o = TsgOut{};
+        o.stat = summary;
+        o.flags = pol ? 1 : 0;
+        o.inv = 1.0;
+        o.i = o.j = o.k = o.l = m < K ? m : 0;
+        o.ni = n;
+        o.par = 0;
+    }
+    switch (summary) {
+    case TSG_STAT_TRAIT_COVARIANCE:
+        if (P != K) return TSG_ERR_BAD_RESULT_DIMS;
+        for (TsgOut &o : q.wouts) o.inv = 1.0 / (2 * ((double) n - 1) * ((double) n - 1));
+        break;
+    case TSG_STAT_TRAIT_CORRELATION:
+        if (P != K - 1) return TSG_ERR_BAD_RESULT_DIMS;
+        for (TsgOut &o : q.wouts) {
+            o.l = K - 1;
+            o.inv = 1.0 / (2 * (double) n * ((double) n - 1));
+        }
+        break;
+    case TSG_STAT_TRAIT_LINEAR_MODEL: {
+        if (P > K - 1) return TSG_ERR_BAD_RESULT_DIMS;
+        const int nc = K - P - 1;
+        if (nc > 0 && dparams == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+        if (nc > 0) q.opar.assign(dparams, dparams + (size_t) P * nc);
+        for (int m = 0; m < P; m++) {
+            TsgOut &o = q.wouts[m];
+            o.cbase = P;
+            o.ccount = nc;
+            o.l = K - 1;
+            o.par = m * nc;
+        }
+        break;
+    }
+    case TSG_STAT_RELATEDNESS_WEIGHTED:
+    case TSG_STAT_RELATEDNESS_WEIGHTED_NONCENTRED: {
+        if (index_tuples == nullptr) return TSG_ERR_INSUFFICIENT_INDEX_TUPLES;
+        const bool centred = summary == TSG_STAT_RELATEDNESS_WEIGHTED;
+        if (centred && dparams == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+        for (int m = 0; m < P; m++) {
+            const int32_t i = index_tuples[2 * m], j = index_tuples[2 * m + 1];
+            if (i < 0 || i >= K || j < 0 || j >= K) return TSG_ERR_BAD_SAMPLE_SET_INDEX;
+            TsgOut &o = q.wouts[m];
+            o.i = i;
+            o.j = j;
+            o.l = K - 1;
+            if (centred) {
+                o.par = (int32_t) q.opar.size();
+                q.opar.push_back(dparams[i]);
+                q.opar.push_back(dparams[j]);
+            }
+        }
+        break;
+    }
+    default:
+        return TSG_ERR_BAD_PARAM_VALUE;
+    }
+    if (q.opar.empty()) q.opar.push_back(0.0);
+    q.P = P;
+    q.result = result;
+    return run_one(ts, q);
+}
+
+// The reference's weighted statistics (trees.c:3976-4220, 4839-4895): their
+// weight preparation on the host, then general_stat.
+int tsg_treeseq_trait_covariance(tsg_treeseq *ts, uint64_t num_weights, const double *weights,
+                                 uint64_t num_windows, const double *windows, uint32_t options,
+                                 double *result) {
+    if (num_weights == 0) return TSG_ERR_INSUFFICIENT_WEIGHTS;
+    const size_t n = (size_t) ts->num_samples, K = num_weights;
+    std::vector<double> mean(K, 0.0), w(n * K);
+    for (size_t j = 0; j < n; j++) {
+        for (size_t k = 0; k < K; k++) mean[k] += weights[j * K + k];
+    }
+    for (size_t k = 0; k < K; k++) mean[k] /= (double) n;
+    for (size_t j = 0; j < n; j++) {
+        for (size_t k = 0; k < K; k++) w[j * K + k] = weights[j * K + k] - mean[k];
+    }
+    return tsg_treeseq_general_stat(ts, K, w.data(), K, TSG_STAT_TRAIT_COVARIANCE, nullptr,
+                                    nullptr, num_windows, windows, options, result);
+}
+
+int tsg_treeseq_trait_correlation(tsg_treeseq *ts, uint64_t num_weights, const double *weights,
+                                  uint64_t num_windows, const double *windows, uint32_t options,
+                                  double *result) {
+    if (num_weights < 1) return TSG_ERR_INSUFFICIENT_WEIGHTS;
+    const size_t n = (size_t) ts->num_samples, K = num_weights;
+    std::vector<double> mean(K, 0.0), msq(K, 0.0), sd(K), w(n * (K + 1));
+    for (size_t j = 0; j < n; j++) {
+        for (size_t k = 0; k < K; k++) {
+            const double x = weights[j * K + k];
+            mean[k] += x;
+            msq[k] += x * x;
+        }
+    }
+    for (size_t k = 0; k < K; k++) {
+        mean[k] /= (double) n;
+        msq[k] -= mean[k] * mean[k] * (double) n;
+        msq[k] /= (double) (n - 1);
+        sd[k] = std::sqrt(msq[k]);
+    }
+    for (size_t j = 0; j < n; j++) {
+        for (size_t k = 0; k < K; k++) w[j * (K + 1) + k] = (weights[j * K + k] - mean[k]) / sd[k];
+        w[j * (K + 1) + K] = 1.0 / (double) n;
+    }
+    return tsg_treeseq_general_stat(ts, K + 1, w.data(), K, TSG_STAT_TRAIT_CORRELATION, nullptr,
+                                    nullptr, num_windows, windows, options, result);
+}
+
+int tsg_treeseq_trait_linear_model(tsg_treeseq *ts, uint64_t num_weights, const double *weights,
+                                   uint64_t num_covariates, const double *covariates,
+                                   uint64_t num_windows, const double *windows, uint32_t options,
+                                   double *result) {
+    if (num_weights < 1) return TSG_ERR_INSUFFICIENT_WEIGHTS;
+    const size_t n = (size_t) ts->num_samples, K = num_weights, C = num_covariates;
+    // V = W^T Z, in the reference's order (trees.c:4175-4185); the covariates
+    // are taken as already standardised
+    std::vector<double> V(K * C, 0.0), w(n * (K + C + 1));
+    for (size_t s = 0; s < n; s++) {
+        for (size_t i = 0; i < K; i++) {
+            for (size_t j = 0; j < C; j++) V[i * C + j] += weights[s * K + i] * covariates[s * C + j];
+        }
+    }
+    for (size_t s = 0; s < n; s++) {
+        double *row = w.data() + s * (K + C + 1);
+        for (size_t i = 0; i < K; i++) row[i] = weights[s * K + i];
+        for (size_t i = 0; i < C; i++) row[K + i] = covariates[s * C + i];
+        row[K + C] = 1.0;
+    }
+    return tsg_treeseq_general_stat(ts, K + C + 1, w.data(), K, TSG_STAT_TRAIT_LINEAR_MODEL,
+                                    nullptr, V.empty() ? nullptr : V.data(), num_windows, windows,
+                                    options, result);
+}
+
+int tsg_treeseq_genetic_relatedness_weighted(tsg_treeseq *ts, uint64_t num_weights,
+                                             const double *weights, uint64_t num_index_tuples,
+                                             const int32_t *index_tuples, uint64_t num_windows,
+                                             const double *windows, double *result,
+                                             uint32_t options) {
+    if (num_weights == 0) return TSG_ERR_INSUFFICIENT_WEIGHTS;
+    const size_t n = (size_t) ts->num_samples, K = num_weights;
+    // W with a column of 1/n (trees.c:4862-4872)
+    std::vector<double> tot(K + 1, 0.0), w(n * (K + 1));
+    for (size_t j = 0; j < n; j++) {
+        for (size_t k = 0; k < K; k++) {
+            w[j * (K + 1) + k] = weights[j * K + k];
+            tot[k] += weights[j * K + k];
+        }
+        w[j * (K + 1) + K] = 1.0 / (double) n;
+    }
+    tot[K] = 1.0;
+    const int32_t id = (options & TSG_STAT_NONCENTRED) ? TSG_STAT_RELATEDNESS_WEIGHTED_NONCENTRED
+                                                       : TSG_STAT_RELATEDNESS_WEIGHTED;
+    return tsg_treeseq_general_stat(ts, K + 1, w.data(), num_index_tuples, id, index_tuples,
+                                    tot.data(), num_windows, windows,
+                                    options & ~TSG_STAT_NONCENTRED, result);
+}
+
 int tsg_treeseq_genotype_matrix(tsg_treeseq *ts, uint32_t options, int32_t *out) {
     Use use(ts);
     if (ts->num_sites == 0) return 0;
@@ -2701,10 +3104,7 @@ int tsg_treeseq_genotype_matrix(tsg_treeseq *ts, uint32_t options, int32_t *out)
         return ret;
     }
     if (ws->profile) (void) hipEventRecord(ws->ev1, s);
-    if ((ret = hip_check(hipMemcpyAsync(out, ws->geno.p, cells * 4, hipMemcpyDeviceToHost, s))) ||
-        (ret = hip_check(hipStreamSynchronize(s)))) {
-        return ret;
-    }
+    if ((ret = copy_out(ws, s, out, ws->geno.p, cells * 4))) return ret;
     if (ws->profile) {
         float ms_ = 0;
         (void) hipEventElapsedTime(&ms_, ws->ev0, ws->ev1);
@@ -3051,6 +3451,9 @@ int run_multi(tsg_batch **bs, uint64_t n, std::vector<std::unique_ptr<Lease>> &l
     for (size_t i = 0; i < R; i++) {
         runs[i].s = s0;
         runs[i].results_h = (const double *) ws0->mout_h + outoff[i];
+        if ((ret = copy_device_results(runs[i], (const double *) ws0->mout.p + outoff[i], s0))) {
+            return ret;
+        }
     }
     return 0;
 }
@@ -3343,11 +3746,7 @@ int tsg_treeseq_allele_frequency_spectrum(tsg_treeseq *ts, uint64_t K, const uin
         }
         res_dev = (double *) ws->out.p;
     }
-    if ((ret = hip_check(hipMemcpyAsync(result, res_dev, (size_t) hist_n * 8,
-                                        hipMemcpyDeviceToHost, s)))) {
-        return ret;
-    }
-    if ((ret = hip_check(hipStreamSynchronize(s)))) return ret;
+    if ((ret = copy_out(ws, s, result, res_dev, (size_t) hist_n * 8))) return ret;
     if (ws->profile) {
         float ms = 0;
         (void) hipEventElapsedTime(&ms, ws->ev0, ws->ev1);

@@ -72,6 +72,9 @@
 #define TSG_ERR_EMPTY_SAMPLE_SET (-908)
 #define TSG_ERR_UNSUPPORTED_STAT_MODE (-909)
 #define TSG_ERR_TIME_UNCALIBRATED (-910)
+#define TSG_ERR_STAT_POLARISED_UNSUPPORTED (-911)
+#define TSG_ERR_STAT_SPAN_NORMALISE_UNSUPPORTED (-912)
+#define TSG_ERR_INSUFFICIENT_WEIGHTS (-913)
 #define TSG_ERR_BAD_TIME_WINDOWS_DIM (-922)
 #define TSG_ERR_BAD_TIME_WINDOWS (-924)
 
@@ -98,6 +101,12 @@ enum tsg_stat_id {
     TSG_STAT_Y3,                  // trees.c:5177-5199   (triples)
     TSG_STAT_F3,                  // trees.c:5219-5242   (triples)
     TSG_STAT_F4,                  // trees.c:5266-5291   (quads)
+    // summaries of real-valued sample weights (tsg_treeseq_general_stat)
+    TSG_STAT_TRAIT_COVARIANCE,    // trees.c:3961-3974
+    TSG_STAT_TRAIT_CORRELATION,   // trees.c:4026-4045
+    TSG_STAT_TRAIT_LINEAR_MODEL,  // trees.c:4108-4152
+    TSG_STAT_RELATEDNESS_WEIGHTED,             // trees.c:4801-4820 (pairs)
+    TSG_STAT_RELATEDNESS_WEIGHTED_NONCENTRED,  // trees.c:4822-4837 (pairs)
     TSG_NUM_STAT_IDS
 };
 

@@ -12,7 +12,7 @@ struct TsgOut {
     int32_t ni, nj, nk, nl;  // sample-set sizes of those columns
     int32_t cbase, ccount;   // the query's own columns (relatedness mean)
     int32_t flags;       // bit 0: polarised
-    int32_t pad;
+    int32_t par;         // weighted summaries: offset of their parameters in opar
     double inv;          // 1 / denominator of the summary function
 };
 
@@ -47,19 +47,28 @@ struct TsgQuery {
 #define TSG_PB 192
 // 32-bit words of the sweep kernel's LDS staging block for K state columns
 // and `span` outputs in the launch (layout in k_sweep); node records follow.
+// (an even count: the node records that follow may hold FP64 values)
 #define TSG_STAGE_INTS(K, span)                                                               \
-    ((int64_t) (span) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + (TSG_TB + 1) +         \
-     3 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) + 2 * TSG_PSEG +  \
-     8 * (int64_t) (K) + 2 * 64 + 2)
+    (((int64_t) (span) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + (TSG_TB + 1) +        \
+      3 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) + 2 * TSG_PSEG + \
+      8 * (int64_t) (K) + 2 * 64 + 3) & ~(int64_t) 1)
+// ... with FP64 state columns: the moved counts, a path segment's counts
+// before / after, the column totals and the zero state take two words each
+#define TSG_STAGE_INTS_W(K, span) \
+    (TSG_STAGE_INTS(K, span) + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K))
 
 // Where a sweep keeps its node records (see k_sweep).
 #define TSG_STORE_GLOBAL 0
 #define TSG_STORE_LDS 1
 #define TSG_STORE_HOT 2
 #define TSG_STORE_HOT16 3
+// the hot plan with FP64 state columns: real-valued sample weights
+#define TSG_STORE_HOTW 4
 // 32-bit words per hot slot: FP64 branch length (branch sweeps, bl) + K
 // counts (16-bit when c16)
 #define TSG_SLOT_WORDS(K, c16, bl) (((bl) ? 2 : 0) + ((c16) ? ((K) + 1) / 2 : (K)))
+// ... and with FP64 state (TSG_STORE_HOTW): the branch length, K doubles
+#define TSG_SLOT_WORDS_W(K, bl) (((bl) ? 2 : 0) + 2 * (K))
 // 32-bit words per full-plan node record: parent, FP64 branch length, K counts
 #define TSG_NODE_WORDS(K) (3 + (K))
 
@@ -105,6 +114,11 @@ struct TsgSweepArgs {
     // ((c / CB) * num_samples + s) * CB + c % CB
     const uint16_t *sample_slot;
     const int32_t *col_total;    // [K] total weight (set size) per column
+    // real-valued weights (TSG_STORE_HOTW): each (sample, column) pair's
+    // weight, the columns' total weights, the weighted summaries' parameters
+    const double *pair_w;
+    const double *col_total_w;
+    const double *opar;
     const TsgOut *outs;          // [M]
     const TsgQuery *queries;     // [Q]
     const double *windows;       // concatenated

@@ -54,7 +54,12 @@ namespace {
 // counts below 2^17.6 (n <= ~2e5 samples per set) is exact in FP64, so the
 // numerators are the exact integer polynomials, scaled once by
 // 1/denominator.  (f2 / f3 / f4 multiply two exact factors, one rounding.)
-__device__ double eval_summary(const TsgOut &o, const int32_t *x, const int32_t *tot, bool comp) {
+//
+// State columns of real-valued weights (T = double) take the weighted
+// summaries, whose parameters beyond one denominator are in opar.
+template <typename T>
+__device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool comp,
+                               const double *opar = nullptr) {
 #define XV(col) ((double) (comp ? (tot[col] - x[col]) : x[col]))
     const double xi = XV(o.i);
     const double ni = (double) o.ni;
@@ -110,8 +115,41 @@ __device__ double eval_summary(const TsgOut &o, const int32_t *x, const int32_t 
         return (a * b * o.inv);
     }
     default:
-        return 0.0;
+        break;
     }
+    if constexpr (std::is_same<T, double>::value) {
+    switch (o.stat) {
+    case TSG_STAT_TRAIT_COVARIANCE:  // x^2 / (2 (n - 1)^2)
+        return xi * xi * o.inv;
+    case TSG_STAT_TRAIT_CORRELATION: {  // x^2 / (2 p (1 - p) n (n - 1)), p = x_l
+        const double p = XV(o.l);
+        return (p > 0.0 && p < 1.0) ? (xi * xi) * o.inv / (p * (1 - p)) : 0.0;
+    }
+    case TSG_STAT_TRAIT_LINEAR_MODEL: {
+        // trait i, covariates cbase .. cbase+ccount-1 with their row of V
+        // (opar + par), the sample count m = x_l of n
+        const double m = XV(o.l), n = (double) o.ni;
+        if (!(m > 0.0 && m < n)) return 0.0;
+        const double *v = opar + o.par;
+        double a = xi, denom = m;
+        for (int c = 0; c < o.ccount; c++) {
+            const double z = XV(o.cbase + c);
+            a -= z * v[c];
+            denom -= z * z;
+        }
+        return denom < 1e-8 ? 0.0 : (a * a) / (2 * denom * denom);
+    }
+    case TSG_STAT_RELATEDNESS_WEIGHTED: {  // (x_i - a x_l)(x_j - b x_l)
+        const double pn = XV(o.l), xj = XV(o.j);
+        return (xi - opar[o.par] * pn) * (xj - opar[o.par + 1] * pn);
+    }
+    case TSG_STAT_RELATEDNESS_WEIGHTED_NONCENTRED:
+        return xi * XV(o.j);
+    default:
+        break;
+    }
+    }
+    return 0.0;
 #undef XV
 }
 
@@ -148,6 +186,7 @@ struct Nodes {
 template <bool C16, bool BL>
 struct Slots {
     static constexpr int OFF = BL ? 2 : 0;  // words before the counts
+    static constexpr bool WEIGHTED = false;
     int32_t *rec;
     int32_t R;
     __device__ double bl(int h) const {
@@ -178,6 +217,28 @@ struct Slots {
             atomicAdd(&rec[h * R + OFF + k], v);
         }
     }
+};
+
+// The same with FP64 state columns (real-valued sample weights): rec[h*R..+1]
+// = branch length (BL), then K doubles (R = TSG_SLOT_WORDS_W(K, BL), even;
+// rec is 8-byte aligned).
+template <bool BL>
+struct SlotsW {
+    static constexpr int OFF = BL ? 2 : 0;
+    static constexpr bool WEIGHTED = true;
+    int32_t *rec;
+    int32_t R;
+    __device__ double bl(int h) const {
+        if constexpr (BL) return ld_f64(rec + h * R);
+        return 0.0;
+    }
+    __device__ void set_bl(int h, double b) const {
+        if constexpr (BL) st_f64(rec + h * R, b);
+    }
+    __device__ double *col(int h) const { return reinterpret_cast<double *>(rec + h * R + OFF); }
+    __device__ double get(int h, int k) const { return col(h)[k]; }
+    __device__ void put(int h, int k, double v) const { col(h)[k] = v; }
+    __device__ void add(int h, int k, double v) const { atomicAdd(col(h) + k, v); }
 };
 
 // The sweep's workgroup is one wavefront.  Its lanes run in lockstep and the
@@ -236,14 +297,15 @@ __device__ __forceinline__ const uint16_t *sample_map(const TsgSweepArgs &a, int
 // add every weighted sample into its nearest hot ancestor-or-self, then add
 // each depth level of the hot forest into its parents, deepest level first
 // (the counts of a level are complete before it is added upward).
-template <bool C16, bool BL>
-__device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c, int nslot,
-                         int K, int lane) {
+// With real-valued weights (SlotsW) a pair adds its weight instead of 1.
+template <class NS>
+__device__ void init_hot(const TsgSweepArgs &a, const NS &nd, int c, int nslot, int K,
+                         int lane) {
     const int64_t s0 = a.chunk_node0[c];
     const int words = nslot * nd.R;
     for (int i = lane; i < words; i += WAVE) nd.rec[i] = 0;
     wave_sync();
-    if constexpr (BL) {
+    if constexpr (NS::OFF != 0) {
         for (int h = lane; h < nslot; h += WAVE) nd.set_bl(h, a.loc_bl0[s0 + h]);
     }
     // (sample, column) pairs, several per lane in flight: the slot gathers
@@ -265,7 +327,12 @@ __device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c,
         }
 #pragma unroll
         for (int r = 0; r < U; r++) {
-            if (ph[r] != TSG_NO_SLOT) nd.add((int) ph[r], pc[r], 1);
+            if (ph[r] == TSG_NO_SLOT) continue;
+            if constexpr (NS::WEIGHTED) {
+                nd.add((int) ph[r], pc[r], a.pair_w[i0 + r * WAVE]);
+            } else {
+                nd.add((int) ph[r], pc[r], 1);
+            }
         }
     }
     wave_sync();
@@ -277,7 +344,7 @@ __device__ void init_hot(const TsgSweepArgs &a, const Slots<C16, BL> &nd, int c,
         for (int idx = lane; idx < (h1 - h0) * K; idx += WAVE) {
             const int h = h0 + idx / K, k = idx % K;
             const int p = a.slot_par[s0 + h];
-            const int32_t v = nd.get(h, k);
+            const auto v = nd.get(h, k);
             if (p >= 0 && v != 0) nd.add(p, k, v);
         }
         wave_sync();
@@ -349,6 +416,21 @@ __device__ uint64_t summary_columns(const TsgOut &o, int K) {
         m |= 1ull << o.j;
         for (int c = o.cbase; c < o.cbase + o.ccount; c++) m |= 1ull << c;
         break;
+    case TSG_STAT_TRAIT_COVARIANCE:
+        break;
+    case TSG_STAT_TRAIT_CORRELATION:
+        m |= 1ull << o.l;
+        break;
+    case TSG_STAT_TRAIT_LINEAR_MODEL:
+        m |= 1ull << o.l;
+        for (int c = o.cbase; c < o.cbase + o.ccount; c++) m |= 1ull << c;
+        break;
+    case TSG_STAT_RELATEDNESS_WEIGHTED:
+        m |= (1ull << o.j) | (1ull << o.l);
+        break;
+    case TSG_STAT_RELATEDNESS_WEIGHTED_NONCENTRED:
+        m |= 1ull << o.j;
+        break;
     default:
         return ~0ull;
     }
@@ -369,7 +451,12 @@ __device__ __forceinline__ int qidx(int s, int t) {
 // quartic terms cancel.  The coefficients
 // were derived by expanding f(x) + f(total - x).  Segregating sites
 // (4222-4236) is TSG_KIND_SEG; the rest are TSG_KIND_HOP.
-__device__ int moment_form(const TsgOut &o, int K, LaneOut &L) {
+// Weighted summaries (real-valued columns; totw the columns' total weights,
+// opar their parameters): trait covariance (trees.c:3961-3974) and weighted
+// relatedness (4801-4837) are quadratic too.
+template <bool WGT = false>
+__device__ int moment_form(const TsgOut &o, int K, LaneOut &L, const double *totw = nullptr,
+                           const double *opar = nullptr) {
     if (K > 64 || !isfinite(o.inv)) return TSG_KIND_HOP;
     const bool pol = (o.flags & 1) != 0;
     const double ni = (double) o.ni, nj = (double) o.nj, nk = (double) o.nk, nl = (double) o.nl;
@@ -455,7 +542,52 @@ __device__ int moment_form(const TsgOut &o, int K, LaneOut &L) {
         q[qidx(1, 3)] = ni * nk;
         break;
     default:
+        if constexpr (!WGT) return TSG_KIND_HOP;
+        break;
+    }
+    if constexpr (WGT) {
+    if (o.stat >= TSG_STAT_TRAIT_COVARIANCE) switch (o.stat) {
+    case TSG_STAT_TRAIT_COVARIANCE: {  // x^2 [+ (T - x)^2]
+        const double t = totw[o.i];
+        q[0] = pol ? 1 : 2;
+        if (!pol) {
+            L.c0 = t * t;
+            l1[0] = -2 * t;
+        }
+        break;
+    }
+    case TSG_STAT_RELATEDNESS_WEIGHTED_NONCENTRED: {  // x_i x_j [+ (T_i - x_i)(T_j - x_j)]
+        q[qidx(0, 1)] = pol ? 1 : 2;
+        if (!pol) {
+            const double ti = totw[o.i], tj = totw[o.j];
+            L.c0 = ti * tj;
+            l1[0] = -tj;
+            l1[1] = -ti;
+        }
+        break;
+    }
+    case TSG_STAT_RELATEDNESS_WEIGHTED: {
+        // slots (x_i, x_j, x_l): (x_i - a x_l)(x_j - b x_l); unpolarised adds
+        // (A - x_i + a x_l)(B - x_j + b x_l), A = T_i - a T_l, B = T_j - b T_l
+        L.pc[2] = o.l;
+        const double ca = opar[o.par], cb = opar[o.par + 1];
+        const double g = pol ? 1 : 2;
+        q[qidx(0, 1)] = g;
+        q[qidx(0, 2)] = -g * cb;
+        q[qidx(1, 2)] = -g * ca;
+        q[qidx(2, 2)] = g * ca * cb;
+        if (!pol) {
+            const double A = totw[o.i] - ca * totw[o.l], B = totw[o.j] - cb * totw[o.l];
+            L.c0 = A * B;
+            l1[0] = -B;
+            l1[1] = -A;
+            l1[2] = A * cb + B * ca;
+        }
+        break;
+    }
+    default:
         return TSG_KIND_HOP;
+    }
     }
     // q holds the coefficients of v_s v_t (s <= t); H has the off-diagonal
     // ones and twice the diagonal ones
@@ -475,7 +607,8 @@ struct PolyParts {
     double u[4];
 };
 
-__device__ __forceinline__ PolyParts poly_parts(const LaneOut &L, const int32_t *x) {
+template <typename T>
+__device__ __forceinline__ PolyParts poly_parts(const LaneOut &L, const T *x) {
     PolyParts P;
     double v[4];
 #pragma unroll
@@ -499,7 +632,8 @@ __device__ __forceinline__ PolyParts poly_parts(const LaneOut &L, const int32_t 
 }
 
 // F at the counts x (TSG_KIND_POLY)
-__device__ __forceinline__ double poly_value(const LaneOut &L, const int32_t *x) {
+template <typename T>
+__device__ __forceinline__ double poly_value(const LaneOut &L, const T *x) {
     const PolyParts P = poly_parts(L, x);
     return L.c0 + P.A + P.B;
 }
@@ -533,12 +667,14 @@ __device__ __forceinline__ double group_sum(double v, int K, int G) {
 
 // Segregating-sites numerators as integers: polarised [x > 0] (n - x),
 // unpolarised n [0 < x < n] (seg_pol, seg_unpol)
-__device__ __forceinline__ int32_t iseg_pol(int32_t x, int32_t n) { return x > 0 ? n - x : 0; }
-__device__ __forceinline__ int32_t iseg_unpol(int32_t x, int32_t n) {
-    return (x > 0 && x < n) ? n : 0;
+template <typename T>
+__device__ __forceinline__ T iseg_pol(T x, T n) { return x > 0 ? n - x : T(0); }
+template <typename T>
+__device__ __forceinline__ T iseg_unpol(T x, T n) {
+    return (x > 0 && x < n) ? n : T(0);
 }
 
-template <int MPL>
+template <int MPL, bool WGT>
 __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *gout, int c, int lane,
                               LaneOut (&lo)[MPL]) {
 #pragma unroll
@@ -562,7 +698,7 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *gout, int c, 
             L.Mq = q.M;
             L.slot0 = q.pbase + a.cq_poff[cq] * (int64_t) q.M + (m - q.mbase);
             L.mask = summary_columns(o, a.K);
-            L.kind = moment_form(o, a.K, L);
+            L.kind = moment_form<WGT>(o, a.K, L, a.col_total_w, a.opar);
             if (L.kind == TSG_KIND_SEG) {
                 L.pc[0] = o.i;
                 L.pc[1] = o.flags & 1;
@@ -578,10 +714,11 @@ __device__ __forceinline__ void flush_window(const TsgSweepArgs &a, LaneOut &L, 
 }
 
 // Unpolarised summaries add f(total - x) (trees.c:1944-1972).
-__device__ __forceinline__ double branch_summary(const TsgOut &o, const int32_t *x,
-                                                 const int32_t *tot) {
-    double f = eval_summary(o, x, tot, false);
-    if (!(o.flags & 1)) f = f + eval_summary(o, x, tot, true);
+template <typename T>
+__device__ __forceinline__ double branch_summary(const TsgOut &o, const T *x, const T *tot,
+                                                 const double *opar = nullptr) {
+    double f = eval_summary(o, x, tot, false, opar);
+    if (!(o.flags & 1)) f = f + eval_summary(o, x, tot, true, opar);
     return f;
 }
 
@@ -649,23 +786,29 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     double *td = guv + K;                                                // [WAVE] hop terms
     double *evb = td + WAVE;                                             // [EB] event branch length
     double *sbl = evb + TSG_EB;                                          // [PSEG] path branch lengths
-    int32_t *tnrem = reinterpret_cast<int32_t *>(sbl + TSG_PSEG);        // [TB]
+    // state values: counts, or FP64 sums of real-valued weights
+    constexpr bool WGT = STORE == TSG_STORE_HOTW;
+    using Cnt = typename std::conditional<WGT, double, int32_t>::type;
+    Cnt *xc = reinterpret_cast<Cnt *>(sbl + TSG_PSEG);                  // [K] moved subtree
+    Cnt *xo = xc + K;                                                    // [PSEG x K]
+    Cnt *xn = xo + TSG_PSEG * K;                                         // [PSEG x K]
+    Cnt *tot = xn + TSG_PSEG * K;                                        // [K]
+    Cnt *zero = tot + K;                                                 // [K]
+    int32_t *tnrem = reinterpret_cast<int32_t *>(zero + K);              // [TB]
     int32_t *ts0 = tnrem + TSG_TB;                                       // [TB+1] tree_site0
     int32_t *evc = ts0 + TSG_TB + 1;                                     // [EB] event child
     int32_t *pbuf = evc + TSG_EB;                                        // [PB] path entries
-    int32_t *xc = pbuf + TSG_PB;                                         // [K]
-    int32_t *xo = xc + K;                                                // [PSEG x K]
-    int32_t *xn = xo + TSG_PSEG * K;                                     // [PSEG x K]
-    int32_t *tot = xn + TSG_PSEG * K;                                    // [K]
-    int32_t *zero = tot + K;                                             // [K]
-    int32_t *act = zero + K;                                             // [span]
-    constexpr bool HOTS = STORE == TSG_STORE_HOT || STORE == TSG_STORE_HOT16;
-    using NS = typename std::conditional<HOTS, Slots<STORE == TSG_STORE_HOT16, MODE == 0>,
-                                         Nodes>::type;
+    int32_t *act = pbuf + TSG_PB;                                        // [span]
+    constexpr bool HOTS = STORE == TSG_STORE_HOT || STORE == TSG_STORE_HOT16 || WGT;
+    using NS = typename std::conditional<
+        WGT, SlotsW<MODE == 0>,
+        typename std::conditional<HOTS, Slots<STORE == TSG_STORE_HOT16, MODE == 0>,
+                                  Nodes>::type>::type;
     NS nd;
-    nd.R = HOTS ? TSG_SLOT_WORDS(K, STORE == TSG_STORE_HOT16, MODE == 0) : R;
+    nd.R = WGT ? TSG_SLOT_WORDS_W(K, MODE == 0)
+               : (HOTS ? TSG_SLOT_WORDS(K, STORE == TSG_STORE_HOT16, MODE == 0) : R);
     if (STORE != TSG_STORE_GLOBAL) {
-        nd.rec = smem + TSG_STAGE_INTS(K, span);
+        nd.rec = smem + (WGT ? TSG_STAGE_INTS_W(K, span) : TSG_STAGE_INTS(K, span));
     } else {
         nd.rec = a.g_rec + (int64_t) c * a.g_stride;
     }
@@ -673,7 +816,11 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     const uint64_t below = (1ull << lane) - 1;
 
     for (int k = lane; k < K; k += WAVE) {
-        tot[k] = a.col_total[k];
+        if constexpr (WGT) {
+            tot[k] = a.col_total_w[k];
+        } else {
+            tot[k] = a.col_total[k];
+        }
         zero[k] = 0;
     }
     if constexpr (HOTS) {
@@ -683,13 +830,13 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     }
 
     LaneOut lo[MPL];
-    setup_outputs<MPL>(a, gout, c, lane, lo);
+    setup_outputs<MPL, WGT>(a, gout, c, lane, lo);
     int wfirst[MPL];
     bool any_poly = false, any_c3 = false, any_segp = false, any_segu = false, any_hop = false;
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
         wfirst[j] = lo[j].wi;
-        if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(gout[lo[j].lm], zero, tot);
+        if (MODE == 0 && lo[j].lm >= 0) lo[j].f0 = branch_summary(gout[lo[j].lm], zero, tot, a.opar);
         any_poly |= lo[j].kind == TSG_KIND_POLY;
         any_c3 |= lo[j].kind == TSG_KIND_POLY && lo[j].c3 != 0;
         any_segp |= lo[j].kind == TSG_KIND_SEG && lo[j].pc[1];
@@ -713,7 +860,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     const int ug = kwide ? 0 : lane / K;
     const int uk = kwide ? lane : lane - ug * K;
     const bool uon = ug < G;
-    const int32_t totk = (!kwide && uon) ? tot[uk] : 0;
+    const Cnt totk = (!kwide && uon) ? tot[uk] : Cnt(0);
 
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
     // the chunk's events: transitions t0+1 .. t1 (the last is the closing
@@ -783,7 +930,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             // columns the moved subtree carries
             uint64_t cm;
             if (K <= WAVE) {
-                int32_t xk = 0;
+                Cnt xk = 0;
                 if (lane < K) {
                     xk = nd.get(u, lane);
                     xc[lane] = xk;
@@ -792,7 +939,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             } else {
                 bool nz = false;
                 for (int k = lane; k < K; k += WAVE) {
-                    const int32_t xk = nd.get(u, k);
+                    const Cnt xk = nd.get(u, k);
                     xc[k] = xk;
                     nz |= (xk != 0);
                 }
@@ -838,15 +985,15 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             // S1 = sum bl x, S2 = sum bl x^2 (x the count before the event),
             // and the segregating-sites numerator changes
             double s0 = 0, s1 = 0, s2 = 0, gp = 0, gu = 0;
-            const int32_t dx = (!kwide && uon) ? (ins ? xc[uk] : -xc[uk]) : 0;
+            const Cnt dx = (!kwide && uon) ? (ins ? xc[uk] : -xc[uk]) : Cnt(0);
             for (int64_t sg0 = 0; sg0 < p1 - p0; sg0 += TSG_PSEG) {
                 const int h = (int) min((int64_t) TSG_PSEG, p1 - p0 - sg0);
                 if (kwide) {
                     for (int i = 0; i < h; i++) {
                         const int node = pth[sg0 + i];
                         for (int k = lane; k < K; k += WAVE) {
-                            const int32_t old = nd.get(node, k);
-                            const int32_t nw = ins ? old + xc[k] : old - xc[k];
+                            const Cnt old = nd.get(node, k);
+                            const Cnt nw = ins ? old + xc[k] : old - xc[k];
                             nd.put(node, k, nw);
                             if (MODE == 0 && any_hop) {
                                 xo[i * K + k] = old;
@@ -857,8 +1004,8 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                 } else if (uon) {
                     for (int i = ug; i < h; i += G) {
                         const int node = pth[sg0 + i];
-                        const int32_t old = nd.get(node, uk);
-                        const int32_t nw = old + dx;
+                        const Cnt old = nd.get(node, uk);
+                        const Cnt nw = old + dx;
                         nd.put(node, uk, nw);
                         if (MODE == 0 && any_hop) {
                             xo[i * K + uk] = old;
@@ -873,11 +1020,11 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                                 if (any_c3) s2 = fma(bw * xold, xold, s2);
                             }
                             if (any_segp) {
-                                const int32_t dp = iseg_pol(nw, totk) - iseg_pol(old, totk);
+                                const Cnt dp = iseg_pol(nw, totk) - iseg_pol(old, totk);
                                 gp = fma(bw, (double) dp, gp);
                             }
                             if (any_segu) {
-                                const int32_t du = iseg_unpol(nw, totk) - iseg_unpol(old, totk);
+                                const Cnt du = iseg_unpol(nw, totk) - iseg_unpol(old, totk);
                                 gu = fma(bw, (double) du, gu);
                             }
                         }
@@ -894,8 +1041,8 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                         if (idx < items) {
                             const int ai = idx / h, i = idx - ai * h;
                             const TsgOut o = gout[act[ai]];
-                            const double fo = branch_summary(o, xo + i * K, tot);
-                            const double fn = branch_summary(o, xn + i * K, tot);
+                            const double fo = branch_summary(o, xo + i * K, tot, a.opar);
+                            const double fn = branch_summary(o, xn + i * K, tot, a.opar);
                             td[lane] = (double) sbl[i] * (fn - fo);
                         }
                         SWEEP_SYNC();
@@ -943,7 +1090,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                         L.R += b * (L.c0 * (L.pc[1] ? seg_pol(xa, sn) : seg_unpol(xa, sn)));
                         L.R += L.c0 * (L.pc[1] ? gpv[si] : guv[si]);
                     } else {
-                        L.R += b * branch_summary(gout[L.lm], xc, tot);
+                        L.R += b * branch_summary(gout[L.lm], xc, tot, a.opar);
                     }
                 }
                 SWEEP_SYNC();
@@ -993,7 +1140,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                 // polarised outputs skip the ancestral allele (trees.c:1639)
                 for (int al = any_unpolarised ? 0 : 1; al < na; al++) {
                     for (int k = lane; k < K; k += WAVE) {
-                        int32_t w = (al == 0) ? tot[k] : 0;
+                        Cnt w = (al == 0) ? tot[k] : Cnt(0);
                         for (int mm = m0; mm < m1; mm++) {
                             const int node = a.mut_node[mm];
                             if (a.mut_allele[mm] == al) w += nd.get(node, k);
@@ -1006,7 +1153,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                     for (int j = 0; j < MPL; j++) {
                         if (lo[j].lm < 0) continue;
                         const TsgOut &o = gout[lo[j].lm];
-                        if (al > 0 || !(o.flags & 1)) val[j] += eval_summary(o, xo, tot, false);
+                        if (al > 0 || !(o.flags & 1)) val[j] += eval_summary(o, xo, tot, false, a.opar);
                     }
                     SWEEP_SYNC();
                 }
@@ -1679,8 +1826,11 @@ __global__ void __launch_bounds__(256) k_tree0_counts(const TsgSweepArgs a,
 #define TSG_T0_NODES 64
 
 // grid (ceil(nloc / TSG_T0_NODES), ceil(M / 64)); lane = output.
+// (T = double: the first tree's sums of real-valued weights, x0 computed on
+// the host)
+template <typename T>
 __device__ __forceinline__ void tree0_sum(const TsgSweepArgs &a, const int32_t *t0_parent,
-                                          const double *t0_bl, const int32_t *x0, int nloc,
+                                          const double *t0_bl, const T *x0, int nloc,
                                           double *part, int bx, int by) {
     const int m = by * WAVE + threadIdx.x;
     if (m >= a.M) return;
@@ -1688,24 +1838,32 @@ __device__ __forceinline__ void tree0_sum(const TsgSweepArgs &a, const int32_t *
     const int u1 = min(nloc, u0 + TSG_T0_NODES);
     const TsgOut o = a.outs[m];
     LaneOut L;
-    const bool mom = moment_form(o, a.K, L) == TSG_KIND_POLY;
+    const bool mom = moment_form<std::is_same<T, double>::value>(o, a.K, L, a.col_total_w,
+                                                                  a.opar) == TSG_KIND_POLY;
+    const T *tot;
+    if constexpr (std::is_same<T, double>::value) {
+        tot = a.col_total_w;
+    } else {
+        tot = a.col_total;
+    }
     double acc = 0;
     for (int u = u0; u < u1; u++) {
         if (t0_parent[u] < 0) continue;
-        const int32_t *x = x0 + (int64_t) u * a.K;
+        const T *x = x0 + (int64_t) u * a.K;
         double f;
         if (mom) {  // the polynomial form, as in the sweep
             f = poly_value(L, x);
         } else {
-            f = branch_summary(o, x, a.col_total);
+            f = branch_summary(o, x, tot, a.opar);
         }
         acc += (double) t0_bl[u] * f;
     }
     part[(int64_t) bx * a.M + m] = acc;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *t0_parent,
-                                                    const double *t0_bl, const int32_t *x0,
+                                                    const double *t0_bl, const T *x0,
                                                     int nloc, double *part) {
     tree0_sum(a, t0_parent, t0_bl, x0, nloc, part, blockIdx.x, blockIdx.y);
 }
@@ -1995,6 +2153,8 @@ hipError_t launch_sweep_store(const TsgSweepArgs &a, int store, int mpl, size_t 
         return launch_sweep_mpl<MODE, TSG_STORE_HOT>(a, mpl, shmem, s);
     case TSG_STORE_HOT16:
         return launch_sweep_mpl<MODE, TSG_STORE_HOT16>(a, mpl, shmem, s);
+    case TSG_STORE_HOTW:
+        return launch_sweep_mpl<MODE, TSG_STORE_HOTW>(a, mpl, shmem, s);
     case TSG_STORE_LDS:
         return launch_sweep_mpl<MODE, TSG_STORE_LDS>(a, mpl, shmem, s);
     default:
@@ -2140,8 +2300,19 @@ hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, con
     }
     const int nb = tsg_tree0_blocks(nloc);
     if (nb > 0 && a.M > 0) {
-        hipLaunchKernelGGL(k_tree0_sum, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), 0, s, a,
-                           t0_parent, t0_bl, x0, nloc, part);
+        hipLaunchKernelGGL(k_tree0_sum<int32_t>, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), 0,
+                           s, a, t0_parent, t0_bl, (const int32_t *) x0, nloc, part);
+    }
+    return hipGetLastError();
+}
+
+hipError_t tsg_launch_tree0_w(const TsgSweepArgs &a, const int32_t *t0_parent,
+                              const double *t0_bl, const double *x0, int nloc, double *part,
+                              hipStream_t s) {
+    const int nb = tsg_tree0_blocks(nloc);
+    if (nb > 0 && a.M > 0) {
+        hipLaunchKernelGGL(k_tree0_sum<double>, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), 0,
+                           s, a, t0_parent, t0_bl, x0, nloc, part);
     }
     return hipGetLastError();
 }

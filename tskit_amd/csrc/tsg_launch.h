@@ -29,6 +29,11 @@ int tsg_tree0_blocks(int nloc);
 // part [tsg_tree0_blocks(nloc) x M]
 hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, const double *t0_bl,
                             int32_t *x0, int nloc, double *part, hipStream_t s);
+// the same for FP64 state columns (real-valued weights), the first tree's
+// sums x0 [nloc x K] given
+hipError_t tsg_launch_tree0_w(const TsgSweepArgs &a, const int32_t *t0_parent,
+                              const double *t0_bl, const double *x0, int nloc, double *part,
+                              hipStream_t s);
 // chunk blocks of the running-sum scan (bsum holds blocks x M doubles)
 int tsg_scan_blocks(int num_chunks);
 hipError_t tsg_launch_combine(const TsgCombineArgs &a, const double *dR, double *Rstart,

@@ -478,6 +478,74 @@ class LowLevelTreeSequence:
         check(err)
         return out
 
+    def _weights(self, W):
+        W = np.asarray(W, dtype=np.float64)
+        if W.ndim != 2:
+            raise ValueError("weights must be a 2D array")
+        if W.shape[0] != self.get_num_samples():
+            raise ValueError("First dimension must be num_samples")
+        return as_c(W, np.float64)
+
+    def general_stat(self, weights, summary, output_dim, windows, mode=None, polarised=0,
+                     span_normalise=0):
+        """_tskitmodule.c:6665-6745, with ``summary`` a registry entry
+        (tskit_amd.summaries) in place of the Python callable: [W, output_dim]."""
+        opts = self._options(mode, span_normalise, polarised)
+        W = self._weights(weights)
+        w = _parse_windows(windows)
+        idx = summary.index_tuples
+        dp = summary.dparams
+        out = np.zeros((len(w) - 1, int(output_dim)), dtype=np.float64)
+        err = self._lib.tsg_treeseq_general_stat(
+            self._h, W.shape[1], ptr(W, ctypes.c_double), int(output_dim), summary.stat_id,
+            None if idx is None else ptr(idx, ctypes.c_int32),
+            None if dp is None else ptr(dp, ctypes.c_double), len(w) - 1,
+            ptr(w, ctypes.c_double), opts, ptr(out, ctypes.c_double))
+        check(err)
+        return out
+
+    def _weighted(self, fn, weights, windows, mode, polarised, span_normalise, extra=()):
+        opts = self._options(mode, span_normalise, polarised)
+        W = self._weights(weights)
+        w = _parse_windows(windows)
+        out = np.zeros((len(w) - 1, W.shape[1]), dtype=np.float64)
+        err = fn(self._h, W.shape[1], ptr(W, ctypes.c_double), *extra, len(w) - 1,
+                 ptr(w, ctypes.c_double), opts, ptr(out, ctypes.c_double))
+        check(err)
+        return out
+
+    def trait_covariance(self, weights, windows, mode=None, polarised=0, span_normalise=1):
+        """_tskitmodule.c:6748-6819 (kwlist weights, windows, mode, polarised,
+        span_normalise): [W, num_weights]"""
+        return self._weighted(self._lib.tsg_treeseq_trait_covariance, weights, windows, mode,
+                              polarised, span_normalise)
+
+    def trait_correlation(self, weights, windows, mode=None, polarised=0, span_normalise=1):
+        """_tskitmodule.c:6748-6819: [W, num_weights]"""
+        return self._weighted(self._lib.tsg_treeseq_trait_correlation, weights, windows, mode,
+                              polarised, span_normalise)
+
+    def trait_linear_model(self, weights, covariates, windows, mode=None, polarised=0,
+                           span_normalise=1):
+        """_tskitmodule.c:6821-6905: [W, num_weights]"""
+        Z = self._weights(covariates)
+        return self._weighted(self._lib.tsg_treeseq_trait_linear_model, weights, windows, mode,
+                              polarised, span_normalise, (Z.shape[1], ptr(Z, ctypes.c_double)))
+
+    def genetic_relatedness_weighted(self, weights, indexes, windows, mode=None,
+                                     span_normalise=1, polarised=0, centre=1):
+        """_tskitmodule.c genetic_relatedness_weighted: [W, num_indexes]"""
+        opts = self._options(mode, span_normalise, polarised, centre)
+        W = self._weights(weights)
+        idx = as_c(np.asarray(indexes, dtype=np.int32).reshape(-1, 2), np.int32)
+        w = _parse_windows(windows)
+        out = np.zeros((len(w) - 1, idx.shape[0]), dtype=np.float64)
+        err = self._lib.tsg_treeseq_genetic_relatedness_weighted(
+            self._h, W.shape[1], ptr(W, ctypes.c_double), idx.shape[0], ptr(idx, ctypes.c_int32),
+            len(w) - 1, ptr(w, ctypes.c_double), ptr(out, ctypes.c_double), opts)
+        check(err)
+        return out
+
     def allele_frequency_spectrum(self, sample_set_sizes, sample_sets, windows, time_windows,
                                   mode=None, span_normalise=1, polarised=0):
         """_tskitmodule.c:6977-7062"""
@@ -947,6 +1015,105 @@ class TreeSequence:
         return self._run_windowed_stat(
             windows, self._ll_tree_sequence.genetic_relatedness_vector, W, mode=mode,
             span_normalise=span_normalise, centre=centre, nodes=nodes)
+
+    def general_stat(self, W, f, output_dim, windows=None, polarised=False, mode=None,
+                     span_normalise=True, strict=True):
+        """Windowed statistic of weights W (one row per sample) and a summary
+        function (trees.py:7917-8004).  ``f`` is an entry of the device
+        summary registry (``tskit_amd.summaries``), not an arbitrary Python
+        callable; the strict check of f(0) and f(total weight) runs as in the
+        reference.  Branch and site mode."""
+        from . import summaries
+
+        if not isinstance(f, summaries.Summary):
+            raise TypeError("f must be a tskit_amd.summaries entry: the summary runs on the "
+                            "GPU, which cannot call a Python function")
+        if mode is None:
+            mode = "site"
+        if strict:
+            total_weights = np.sum(W, axis=0)
+            for x in [total_weights, total_weights * 0.0]:
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    fx = np.array(f(x))
+                fx[np.isnan(fx)] = 0.0
+                if not np.allclose(fx, np.zeros((output_dim,))):
+                    raise ValueError("Summary function does not return zero for both "
+                                     "zero weight and total weight.")
+        return self._run_windowed_stat(
+            windows, self._ll_tree_sequence.general_stat, W, f, output_dim,
+            polarised=polarised, span_normalise=span_normalise, mode=mode)
+
+    def trait_covariance(self, W, windows=None, mode="site", span_normalise=True):
+        """Covariance of each trait (column of W) with the genotypes
+        (trees.py:9559-9619): [W, num_traits], or [num_traits] without
+        windows."""
+        if W.shape[0] != self.num_samples:
+            raise ValueError("First trait dimension must be equal to number of samples.")
+        return self._run_windowed_stat(windows, self._ll_tree_sequence.trait_covariance, W,
+                                       mode=mode, span_normalise=span_normalise)
+
+    def trait_correlation(self, W, windows=None, mode="site", span_normalise=True):
+        """Correlation of each trait with the genotypes (trees.py:9621-9700)."""
+        if W.shape[0] != self.num_samples:
+            raise ValueError("First trait dimension must be equal to number of samples.")
+        sds = np.std(W, axis=0)
+        if np.any(sds == 0):
+            raise ValueError("Weight columns must have positive variance", sds)
+        return self._run_windowed_stat(windows, self._ll_tree_sequence.trait_correlation, W,
+                                       mode=mode, span_normalise=span_normalise)
+
+    def trait_regression(self, *args, **kwargs):
+        """Deprecated alias of trait_linear_model (trees.py:9702)."""
+        return self.trait_linear_model(*args, **kwargs)
+
+    def trait_linear_model(self, W, Z=None, windows=None, mode="site", span_normalise=True):
+        """Squared coefficient of the genotypes in a linear model of each
+        trait on the genotypes and the covariates Z (trees.py:9705-9788); Z
+        gets an intercept column and is orthonormalised as there."""
+        if W.shape[0] != self.num_samples:
+            raise ValueError("First trait dimension must be equal to number of samples.")
+        if Z is None:
+            Z = np.ones((self.num_samples, 1))
+        else:
+            tZ = np.column_stack([Z, np.ones((Z.shape[0], 1))])
+            if np.linalg.matrix_rank(tZ) == tZ.shape[1]:
+                Z = tZ
+        if Z.shape[0] != self.num_samples:
+            raise ValueError("First dimension of Z must equal the number of samples.")
+        if np.linalg.matrix_rank(Z) < Z.shape[1]:
+            raise ValueError("Matrix of covariates is computationally singular.")
+        K = np.linalg.cholesky(np.matmul(Z.T, Z)).T
+        Z = np.matmul(Z, np.linalg.inv(K))
+        return self._run_windowed_stat(windows, self._ll_tree_sequence.trait_linear_model, W,
+                                       Z, mode=mode, span_normalise=span_normalise)
+
+    def genetic_relatedness_weighted(self, W, indexes=None, windows=None, mode="site",
+                                     span_normalise=True, polarised=False, centre=True):
+        """Weighted genetic relatedness W_i^T C W_j for each pair (i, j) of
+        weight columns (trees.py:9104-9155 and 8414-8457)."""
+        if len(W) != self.num_samples:
+            raise ValueError("First trait dimension must be equal to number of samples.")
+        k = 2
+        W = np.asarray(W)
+        if indexes is None:
+            if W.shape[1] != k:
+                raise ValueError("Must specify indexes if there are not exactly {} columns "
+                                 "in W.".format(k))
+            indexes = np.arange(k, dtype=np.int32)
+        drop_dimension = False
+        indexes = safe_np_int_cast(indexes, np.int32)
+        if len(indexes.shape) == 1:
+            indexes = indexes.reshape((1, indexes.shape[0]))
+            drop_dimension = True
+        if len(indexes.shape) != 2 or indexes.shape[1] != k:
+            raise ValueError("Indexes must be convertable to a 2D numpy array with {} "
+                             "columns".format(k))
+        stat = self._run_windowed_stat(
+            windows, self._ll_tree_sequence.genetic_relatedness_weighted, W, indexes,
+            mode=mode, span_normalise=span_normalise, polarised=polarised, centre=centre)
+        if drop_dimension:
+            stat = stat.reshape(stat.shape[:-1])
+        return stat
 
     def genetic_relatedness_matrix(self, sample_sets=None, windows=None, mode="site",
                                    span_normalise=True):
