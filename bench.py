@@ -259,10 +259,16 @@ def dense_phase(om, with_cpu):
         return (time.perf_counter() - t) / reps, r
 
     t_rel, rel = timed(lambda: ts.genetic_relatedness_matrix())
+    # GEMM-equivalent ops (2 n^2 S) and the ops the int8 SYRK executes: the
+    # upper-triangular 128 x 128 tiles (diagonal ones whole) over the sites
+    # padded to a multiple of 64
     ops = 2.0 * d["n"] ** 2 * S
+    T = (d["n"] + 127) // 128
+    executed = 2.0 * (T * (T + 1) // 2) * 128 * 128 * ((S + 63) // 64 * 64)
     out["relatedness_matrix_ms"] = t_rel * 1e3
     out["relatedness_int_tops_end_to_end"] = ops / t_rel / 1e12
     out["relatedness_frac_of_int8_peak_end_to_end"] = ops / t_rel / 1e12 / INT8_PEAK_TOPS
+    out["relatedness_syrk_executed_ops"] = executed
     t_geno, geno = timed(lambda: ts.genotype_matrix(), reps=2)
     out["genotype_matrix_ms"] = t_geno * 1e3
     out["genotype_matrix_bytes"] = int(geno.nbytes)
@@ -501,15 +507,24 @@ SWEEP_KERNEL = "k_sweep<0, 3, 1>"
 def pmc_summary(fetch_csv, write_csv):
     """HBM bytes per dispatch of the branch sweep from rocprofv3 counter
     CSVs: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) averaged over its
-    dispatches, each scaled by bytes / counter of the calibration kernel
-    (torch.neg over COPY_BYTES: reads and writes COPY_BYTES)."""
+    timed dispatches (the first, the warm-up, first touches the workspace),
+    each scaled by bytes / counter of the calibration kernel (torch.neg over
+    COPY_BYTES: reads and writes COPY_BYTES)."""
     import csv
 
     def means(path):
         acc = {}
         for r in csv.DictReader(open(path)):
-            acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
-        return {k: sum(v) / len(v) for k, v in acc.items()}
+            per = acc.setdefault(r["Kernel_Name"], {})
+            d = int(r["Dispatch_Id"])
+            per[d] = per.get(d, 0.0) + float(r["Counter_Value"])
+        out = {}
+        for k, per in acc.items():
+            vals = [per[d] for d in sorted(per)]
+            if len(vals) > 1 and "neg" not in k:
+                vals = vals[1:]
+            out[k] = sum(vals) / len(vals)
+        return out
 
     out = {"workload": workload(False)["name"], "kernel": SWEEP_KERNEL}
     for tag, path in [("fetch", fetch_csv), ("write", write_csv)]:

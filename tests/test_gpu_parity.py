@@ -1144,3 +1144,51 @@ def test_device_results(require_gpu):
             assert np.array_equal(hosts[k], host)
             assert np.array_equal(devs[k].cpu().numpy(), host)
             lib.tsg_batch_free(bs[k])
+
+
+@pytest.mark.parametrize("mode", ["branch", "site"])
+def test_fused_results_bit_identical(small, mode):
+    """A query's results are the same bits alone and fused with unrelated
+    queries into one sweep with more state columns (K = 2 against K = 13):
+    the path sums add each hop's terms in hop order whatever the lane
+    layout, which depends on K.  Covers the closed forms (diversity, f4,
+    non-centred relatedness, polarised Y1), the segregating-sites form
+    (both polarisations) and the hop-by-hop form (polarised f2)."""
+    tc, ts, orc = small
+    ll = ts.ll_tree_sequence
+    sets = cases.sample_sets(200, 13, 12, 31)
+    two = sets[:2]
+    w = np.linspace(0, tc.sequence_length, 5)
+    sizes2 = np.array([len(s) for s in two], dtype=np.uint64)
+    flat2 = np.concatenate(two).astype(np.int32)
+
+    alone = [ts.diversity(two, windows=w, mode=mode),
+             ts.segregating_sites(two, windows=w, mode=mode),
+             ts.f4([two[0], two[1], two[0], two[1]], windows=w, mode=mode),
+             ts.genetic_relatedness(two, windows=w, mode=mode, centre=False, proportion=False),
+             ll.Y1(sizes2, flat2, w, mode=mode, span_normalise=1, polarised=1),
+             ll.segregating_sites(sizes2, flat2, w, mode=mode, span_normalise=1, polarised=1),
+             ll.f2(sizes2, flat2, np.array([(0, 1)], dtype=np.int32), w, mode=mode,
+                   span_normalise=1, polarised=1)]
+    b = ts.batch()
+    p = [b.diversity(two, windows=w, mode=mode),
+         b.segregating_sites(two, windows=w, mode=mode),
+         b.f4([two[0], two[1], two[0], two[1]], windows=w, mode=mode),
+         b.genetic_relatedness(two, windows=w, mode=mode, centre=False, proportion=False),
+         b.diversity(sets[2:], windows=w, mode=mode),
+         b.divergence(sets[2:], indexes=[(0, 1), (3, 7)], windows=w, mode=mode)]
+    b.run()
+    _say("fused bit-identical", mode)
+    for k in range(4):
+        assert np.array_equal(p[k].result(), alone[k], equal_nan=True), k
+    # the polarised ones through a batch of the low-level calls
+    bl = ts.batch()
+    bl.diversity(sets[2:], windows=w, mode=mode)
+    bll = bl._ll
+    r1 = bll.Y1(sizes2, flat2, w, mode=mode, span_normalise=1, polarised=1)
+    r2 = bll.segregating_sites(sizes2, flat2, w, mode=mode, span_normalise=1, polarised=1)
+    r3 = bll.f2(sizes2, flat2, np.array([(0, 1)], dtype=np.int32), w, mode=mode,
+                span_normalise=1, polarised=1)
+    bl.run()
+    for k, r in enumerate((r1, r2, r3)):
+        assert np.array_equal(r, alone[4 + k], equal_nan=True), 4 + k

@@ -657,12 +657,43 @@ __device__ __forceinline__ double poly_walk(const LaneOut &L, const PolyParts &P
     return w;
 }
 
-// Sum of v over the G hop groups (G a power of two) of the count-update
-// layout, lane = g K + k: lanes of group 0 receive each column's sum, in a
-// fixed order.
-__device__ __forceinline__ double group_sum(double v, int K, int G) {
-    for (int gs = G >> 1; gs >= 1; gs >>= 1) v += __shfl_down(v, gs * K);
-    return v;
+// v of lane + D (DPP row_shl:D, within the lane's row of 16 lanes)
+template <int D>
+__device__ __forceinline__ double lane_plus(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x100 + D, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x100 + D, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// S + q_0 + q_1 + ... + q_{G-1}, added one at a time in that order, where
+// q_g is q of lane + g: a column's G hop groups sit in adjacent lanes (lane
+// = k G + g, G <= 16 a power of two, within one row of 16), so the head lane
+// of the column adds the terms of one round of G hops in hop order.  The
+// whole wave runs it (uniform control flow); other lanes' S is not used.
+__device__ __forceinline__ double hop_chain(double S, double q, int G) {
+    S = S + q;
+    if (G > 1) S = S + lane_plus<1>(q);
+    if (G > 2) {
+        S = S + lane_plus<2>(q);
+        S = S + lane_plus<3>(q);
+    }
+    if (G > 4) {
+        S = S + lane_plus<4>(q);
+        S = S + lane_plus<5>(q);
+        S = S + lane_plus<6>(q);
+        S = S + lane_plus<7>(q);
+    }
+    if (G > 8) {
+        S = S + lane_plus<8>(q);
+        S = S + lane_plus<9>(q);
+        S = S + lane_plus<10>(q);
+        S = S + lane_plus<11>(q);
+        S = S + lane_plus<12>(q);
+        S = S + lane_plus<13>(q);
+        S = S + lane_plus<14>(q);
+        S = S + lane_plus<15>(q);
+    }
+    return S;
 }
 
 // Segregating-sites numerators as integers: polarised [x > 0] (n - x),
@@ -850,16 +881,17 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     any_hop = MODE == 0 && __any(any_hop);
     const bool any_seg = any_segp || any_segu;
     const bool any_agg = any_poly || any_seg;
-    // count-update layout: with K <= 64 columns, G hop groups of K lanes
-    // (lane = g K + k, G the largest power of two with G K <= 64) update G
-    // hops of a path at once and accumulate their column's path sums in
-    // registers; with more columns one group strides over them.  The
+    // count-update layout: with K <= 64 columns, each column has G lanes
+    // (lane = k G + g, G the largest power of two <= 16 with G K <= 64) that
+    // update G hops of a path at once (round by round); a column's path sums
+    // add the rounds' per-hop terms in hop order (hop_chain), so they do not
+    // depend on K.  With more columns one group strides over them.  The
     // path-sum kinds (POLY, SEG) only occur with K <= 64.
     const bool kwide = K > WAVE || K < 1;
-    const int G = kwide ? 1 : 1 << (31 - __clz(WAVE / K));
-    const int ug = kwide ? 0 : lane / K;
-    const int uk = kwide ? lane : lane - ug * K;
-    const bool uon = ug < G;
+    const int G = kwide ? 1 : min(16, 1 << (31 - __clz(WAVE / K)));
+    const int uk = kwide ? lane : lane / G;
+    const int ug = kwide ? 0 : lane - uk * G;
+    const bool uon = kwide || uk < K;
     const Cnt totk = (!kwide && uon) ? tot[uk] : Cnt(0);
 
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
@@ -1001,32 +1033,46 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                             }
                         }
                     }
-                } else if (uon) {
-                    for (int i = ug; i < h; i += G) {
-                        const int node = pth[sg0 + i];
-                        const Cnt old = nd.get(node, uk);
-                        const Cnt nw = old + dx;
-                        nd.put(node, uk, nw);
-                        if (MODE == 0 && any_hop) {
-                            xo[i * K + uk] = old;
-                            xn[i * K + uk] = nw;
+                } else {
+                    for (int r0 = 0; r0 < h; r0 += G) {
+                        const int i = r0 + ug;
+                        // this lane's hop's terms: bl, bl x, bl x^2 (x the
+                        // count before the event), bl times the change of
+                        // the segregating-sites numerators
+                        double q0 = 0, q1 = 0, q2 = 0, qp = 0, qu = 0;
+                        if (uon && i < h) {
+                            const int node = pth[sg0 + i];
+                            const Cnt old = nd.get(node, uk);
+                            const Cnt nw = old + dx;
+                            nd.put(node, uk, nw);
+                            if (MODE == 0 && any_hop) {
+                                xo[i * K + uk] = old;
+                                xn[i * K + uk] = nw;
+                            }
+                            if (MODE == 0 && any_agg) {
+                                const double bw = nd.bl(node);
+                                q0 = bw;
+                                if (any_poly) {
+                                    const double xold = (double) old;
+                                    q1 = bw * xold;
+                                    if (any_c3) q2 = q1 * xold;
+                                }
+                                if (any_segp) {
+                                    qp = bw * (double) (iseg_pol(nw, totk) - iseg_pol(old, totk));
+                                }
+                                if (any_segu) {
+                                    qu = bw * (double) (iseg_unpol(nw, totk) - iseg_unpol(old, totk));
+                                }
+                            }
                         }
                         if (MODE == 0 && any_agg) {
-                            const double bw = nd.bl(node);
                             if (any_poly) {
-                                const double xold = (double) old;
-                                s0 += bw;
-                                s1 = fma(bw, xold, s1);
-                                if (any_c3) s2 = fma(bw * xold, xold, s2);
+                                s0 = hop_chain(s0, q0, G);
+                                s1 = hop_chain(s1, q1, G);
+                                if (any_c3) s2 = hop_chain(s2, q2, G);
                             }
-                            if (any_segp) {
-                                const Cnt dp = iseg_pol(nw, totk) - iseg_pol(old, totk);
-                                gp = fma(bw, (double) dp, gp);
-                            }
-                            if (any_segu) {
-                                const Cnt du = iseg_unpol(nw, totk) - iseg_unpol(old, totk);
-                                gu = fma(bw, (double) du, gu);
-                            }
+                            if (any_segp) gp = hop_chain(gp, qp, G);
+                            if (any_segu) gu = hop_chain(gu, qu, G);
                         }
                     }
                 }
@@ -1058,18 +1104,17 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             }
             if (MODE == 0) {
                 if (any_agg) {
+                    // the head lane of column k (lane k G) holds its sums;
+                    // S0 is column 0's, lane 0's
                     if (any_poly) {
-                        s0 = __shfl(group_sum(s0, K, G), 0);
-                        s1 = group_sum(s1, K, G);
-                        if (any_c3) s2 = group_sum(s2, K, G);
+                        s0 = __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(s0)),
+                                              __builtin_amdgcn_readfirstlane(__double2loint(s0)));
                     }
-                    if (any_segp) gp = group_sum(gp, K, G);
-                    if (any_segu) gu = group_sum(gu, K, G);
-                    if (lane < K) {
-                        s1v[lane] = s1;
-                        s2v[lane] = s2;
-                        gpv[lane] = gp;
-                        guv[lane] = gu;
+                    if (uon && ug == 0) {
+                        s1v[uk] = s1;
+                        s2v[uk] = s2;
+                        gpv[uk] = gp;
+                        guv[uk] = gu;
                     }
                     SWEEP_SYNC();
                 }
