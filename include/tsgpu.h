@@ -8,8 +8,11 @@
  * `tsg_treeseq_t *`.  Types: sizes are uint64_t (tsk_size_t), ids int32_t
  * (tsk_id_t), options uint32_t (tsk_flags_t); option bits and error numbers
  * have the reference's values (c/tskit/trees.h:47-56, c/tskit/core.h).
- * Results are written to caller-owned host buffers, C order, W x M doubles
- * (AFS: W x TW x prod(n_k + 1)).  Statistics calls on one handle may run
+ * Results are written to caller-owned buffers, C order, W x M doubles
+ * (AFS: W x TW x prod(n_k + 1)): host memory, or device memory, which is
+ * then filled on the device (every entry point but the branch-mode
+ * divergence matrix, whose result is formed on the host).  Statistics calls
+ * on one handle may run
  * concurrently from several host threads (each takes its own device
  * workspace and stream) and never mutate it; tsg_treeseq_set_num_chunks and
  * tsg_treeseq_free wait until the calls in flight on the handle have

@@ -76,8 +76,6 @@ struct Workspace {
     DevBuf rv_E, rv_I, rv_z0, rv_d0, rv_esum, rv_y;
     // relatedness-matrix route
     DevBuf gbits, sitecol, gt, cmat, rowsum, scale, kout, wk0, wk1, sel, gslab, geno;
-    // ... the SYRK's per-part slices when its k range is split
-    DevBuf cslices;
     // ... the rows of sites with three or more alleles
     DevBuf sitecol2, gt2, cmat2, wk0b, wk1b, row2;
     // no-allocation runs (tsg_batch_run_fixed): device buffers are carved
@@ -1584,7 +1582,7 @@ void free_one_workspace(Workspace *ws) {
                       &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
                       &ws->Rstart, &ws->raw, &ws->rawbase, &ws->lanebase, &ws->out, &ws->fin, &ws->ipar,
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
-                      &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->cslices, &ws->rowsum,
+                      &ws->gbits, &ws->sitecol, &ws->gt, &ws->cmat, &ws->rowsum,
                       &ws->scale,
                       &ws->kout, &ws->wk0, &ws->wk1, &ws->sel, &ws->gslab, &ws->geno, &ws->x0,
                       &ws->r0part, &ws->bsum, &ws->margs, &ws->mcpre, &ws->mfin, &ws->mpre,
@@ -2416,7 +2414,6 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
     const int ns = ts->num_samples;
     const int n_pad = ((ns + 127) / 128) * 128;
     const int NW = (ns + 31) / 32;
-    const int nsplit = tsg_syrk_parts(n_pad, W);
     // per window: the bit rows of its sites, padded to a multiple of 64
     // columns of Gt -- sites with at most two alleles in Gt, the rows of the
     // others in Gt2
@@ -2490,8 +2487,6 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
         (ret = upload(ws->wk1, wk1, s)) || (ret = upload(ws->scale, scale, s)) ||
         (ret = upload(ws->sel, sel, s)) || (ret = ensure(ws->gt, (size_t) n_pad * Kp)) ||
         (ret = ensure(ws->cmat, (size_t) W * n_pad * n_pad * 4)) ||
-        (nsplit > 1 &&
-         (ret = ensure(ws->cslices, (size_t) nsplit * W * n_pad * n_pad * 4))) ||
         (ret = ensure(ws->rowsum, (size_t) (W * (int64_t) n + W) * 8)) ||
         (ret = ensure(ws->kout, (size_t) W * n * (size_t) n * 8))) {
         return ret;
@@ -2553,8 +2548,7 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
         (ret = hip_check(tsg_launch_syrk_i8((const int8_t *) ws->gt.p, Kp, n_pad,
                                             (const int64_t *) ws->wk0.p,
                                             (const int64_t *) ws->wk1.p, W,
-                                            (int32_t *) ws->cmat.p, (int32_t *) ws->cslices.p,
-                                            nsplit, s)))) {
+                                            (int32_t *) ws->cmat.p, s)))) {
         return ret;
     }
     const int32_t *C2 = nullptr;
@@ -2565,8 +2559,7 @@ static int site_gram_matrix(tsg_treeseq *ts, uint64_t num_sample_sets,
             (ret = hip_check(tsg_launch_syrk_i8((const int8_t *) ws->gt2.p, Kp2, n_pad,
                                                 (const int64_t *) ws->wk0b.p,
                                                 (const int64_t *) ws->wk1b.p, W,
-                                                (int32_t *) ws->cmat2.p,
-                                                (int32_t *) ws->cslices.p, nsplit, s)))) {
+                                                (int32_t *) ws->cmat2.p, s)))) {
             return ret;
         }
         C2 = (const int32_t *) ws->cmat2.p;

@@ -63,13 +63,8 @@ hipError_t tsg_launch_genotypes(const TsgGenoArgs &a, size_t shmem, const int32_
                                 const int32_t *miss_sample, int64_t nmiss, hipStream_t s);
 hipError_t tsg_launch_bits_to_gt(const uint32_t *gbits, int NW, const int32_t *site_of_col,
                                  int64_t Kp, int n, int n_pad, int8_t *gt, hipStream_t s);
-// parts of the k range the SYRK splits into (workgroups to fill the device)
-int tsg_syrk_parts(int n_pad, int W);
-// C[w] = Gt Gt^T over window w's k range; with nsplit > 1 the parts go to
-// slices [nsplit x W x n_pad x n_pad] first, then are summed into C
 hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int64_t *win_k0,
-                              const int64_t *win_k1, int W, int32_t *C, int32_t *slices,
-                              int nsplit, hipStream_t s);
+                              const int64_t *win_k1, int W, int32_t *C, hipStream_t s);
 // D = (C_ii + C_jj - 2 C_ij) * scale[w] over the selected samples
 hipError_t tsg_launch_divergence(const int32_t *C, const int32_t *C2, const int32_t *sel, int n,
                                  int n_pad, int W, const double *scale, double *D,

@@ -282,22 +282,14 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 #define BK 64         // k per LDS stage
 #define LDSP 80       // padded LDS row (bytes): 64 + 16
 
-// One workgroup (4 waves) per upper-triangular 128x128 tile of one window
-// and one of `nsplit` parts of the window's k range; wave (wy, wx) computes
-// the 64x64 quadrant as 2x2 tiles of 32x32.  The k loop is double-buffered:
-// the global loads of stage k+1 are in flight in registers while the MFMAs
-// of stage k read LDS, and one barrier per stage hands the buffers over.
-// nsplit == 1: the tile and its mirror go to C; otherwise the tile (upper
-// half only) goes to its part's slice of `slices`, summed and mirrored by
-// k_syrk_reduce.
+// One workgroup (4 waves) per upper-triangular 128x128 tile of one window.
+// Wave (wy, wx) computes the 64x64 quadrant as 2x2 tiles of 32x32.
 __global__ void __launch_bounds__(256) k_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad,
                                                  const int64_t *win_k0, const int64_t *win_k1,
-                                                 int tiles_per_dim, int nsplit, int32_t *C,
-                                                 int32_t *slices) {
-    __shared__ __attribute__((aligned(16))) int8_t As[2][BT * LDSP];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][BT * LDSP];
+                                                 int tiles_per_dim, int32_t *C) {
+    __shared__ __attribute__((aligned(16))) int8_t As[BT * LDSP];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[BT * LDSP];
     const int w = blockIdx.y;  // window
-    const int part = blockIdx.z;
     // invert the upper-triangular tile index
     int t = blockIdx.x, ti = 0;
     int rowlen = tiles_per_dim;
@@ -315,59 +307,35 @@ __global__ void __launch_bounds__(256) k_syrk_i8(const int8_t *gt, int64_t Kp, i
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
             for (int r = 0; r < 16; r++) acc[a][b][r] = 0;
-    // this part's stages of BK (window k ranges are multiples of 64)
-    const int64_t kbeg = win_k0[w];
-    const int64_t nst = (win_k1[w] - kbeg) / BK;
-    const int64_t per = (nst + nsplit - 1) / nsplit;
-    const int64_t st0 = min(nst, (int64_t) part * per), st1 = min(nst, st0 + per);
-    // a stage is 128 rows x 64 bytes of A (rows i0..) and of B (rows j0..):
-    // 512 16-byte pieces each, two per thread
-    int4 ra[2], rb[2];
-    auto gload = [&](int64_t st) {
-        const int64_t k = kbeg + st * BK;
-        for (int u = 0; u < 2; u++) {
-            const int q = tid + u * 256, r = q >> 2, seg = (q & 3) * 16;
-            ra[u] = *reinterpret_cast<const int4 *>(gt + (int64_t) (i0 + r) * Kp + k + seg);
-            rb[u] = *reinterpret_cast<const int4 *>(gt + (int64_t) (j0 + r) * Kp + k + seg);
+    const int64_t kbeg = win_k0[w], kend = win_k1[w];
+    for (int64_t k = kbeg; k < kend; k += BK) {
+        // stage 128 rows x 64 bytes of A (rows i0..) and B (rows j0..)
+        for (int q = tid; q < BT * 4; q += 256) {
+            const int r = q >> 2, seg = (q & 3) * 16;
+            *reinterpret_cast<int4 *>(As + r * LDSP + seg) =
+                *reinterpret_cast<const int4 *>(gt + (int64_t) (i0 + r) * Kp + k + seg);
+            *reinterpret_cast<int4 *>(Bs + r * LDSP + seg) =
+                *reinterpret_cast<const int4 *>(gt + (int64_t) (j0 + r) * Kp + k + seg);
         }
-    };
-    auto lstore = [&](int buf) {
-        for (int u = 0; u < 2; u++) {
-            const int q = tid + u * 256, r = q >> 2, seg = (q & 3) * 16;
-            *reinterpret_cast<int4 *>(As[buf] + r * LDSP + seg) = ra[u];
-            *reinterpret_cast<int4 *>(Bs[buf] + r * LDSP + seg) = rb[u];
-        }
-    };
-    if (st0 < st1) {
-        gload(st0);
-        lstore(0);
-    }
-    __syncthreads();
-    for (int64_t st = st0; st < st1; st++) {
-        const int buf = (int) ((st - st0) & 1);
-        const bool more = st + 1 < st1;
-        if (more) gload(st + 1);
+        __syncthreads();
         for (int kk = 0; kk < BK; kk += 32) {
             const int ko = kk + 16 * (lane >> 5);
             v4i af[2], bf[2];
             for (int a = 0; a < 2; a++) {
                 const int r = wy * 64 + a * 32 + (lane & 31);
-                af[a] = *reinterpret_cast<const v4i *>(As[buf] + r * LDSP + ko);
+                af[a] = *reinterpret_cast<const v4i *>(As + r * LDSP + ko);
             }
             for (int b = 0; b < 2; b++) {
                 const int r = wx * 64 + b * 32 + (lane & 31);
-                bf[b] = *reinterpret_cast<const v4i *>(Bs[buf] + r * LDSP + ko);
+                bf[b] = *reinterpret_cast<const v4i *>(Bs + r * LDSP + ko);
             }
             for (int a = 0; a < 2; a++)
                 for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
         }
-        if (more) lstore(buf ^ 1);
         __syncthreads();
     }
-    const int64_t nn = (int64_t) n_pad * n_pad;
-    int32_t *Cw = (nsplit == 1) ? C + (int64_t) w * nn
-                                : slices + ((int64_t) part * gridDim.y + w) * nn;
+    int32_t *Cw = C + (int64_t) w * n_pad * n_pad;
     for (int a = 0; a < 2; a++) {
         for (int b = 0; b < 2; b++) {
             const int col = j0 + wx * 64 + b * 32 + (lane & 31);
@@ -375,34 +343,9 @@ __global__ void __launch_bounds__(256) k_syrk_i8(const int8_t *gt, int64_t Kp, i
                 const int row = i0 + wy * 64 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                 const int32_t v = acc[a][b][r];
                 Cw[(int64_t) row * n_pad + col] = v;
-                if (nsplit == 1) Cw[(int64_t) col * n_pad + row] = v;
+                Cw[(int64_t) col * n_pad + row] = v;
             }
         }
-    }
-}
-
-// C[w] = the sum of the parts' slices, mirrored: a 64x64 block of C below the
-// tile diagonal is the transpose of the block above it, read row-wise into
-// LDS and written transposed.  Integer sums: exact, in any order.
-__global__ void __launch_bounds__(256) k_syrk_reduce(const int32_t *slices, int nsplit, int W,
-                                                     int n_pad, int32_t *C) {
-    __shared__ int32_t blk[64][65];
-    const int bi = blockIdx.y, bj = blockIdx.x, w = blockIdx.z;
-    const int64_t nn = (int64_t) n_pad * n_pad;
-    // the computed (upper) block this one reads: tiles of BT = 2 blocks
-    const bool upper = (bi / (BT / 64)) <= (bj / (BT / 64));
-    const int ri = upper ? bi : bj, rj = upper ? bj : bi;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int r = ty; r < 64; r += 4) {
-        const int64_t idx = (int64_t) (ri * 64 + r) * n_pad + rj * 64 + tx;
-        int32_t v = 0;
-        for (int p = 0; p < nsplit; p++) v += slices[((int64_t) p * W + w) * nn + idx];
-        blk[r][tx] = v;
-    }
-    __syncthreads();
-    int32_t *Cw = C + (int64_t) w * nn;
-    for (int r = ty; r < 64; r += 4) {
-        Cw[(int64_t) (bi * 64 + r) * n_pad + bj * 64 + tx] = upper ? blk[r][tx] : blk[tx][r];
     }
 }
 
@@ -571,26 +514,11 @@ hipError_t tsg_launch_bits_to_gt(const uint32_t *gbits, int NW, const int32_t *s
     return hipGetLastError();
 }
 
-int tsg_syrk_parts(int n_pad, int W) {
-    // enough workgroups for ~12 per CU of 256 CUs, at most 8 parts
-    const int64_t tpd = n_pad / BT, tiles = tpd * (tpd + 1) / 2 * (int64_t) W;
-    int p = 1;
-    while (p < 8 && tiles * p < 3072) p *= 2;
-    return p;
-}
-
 hipError_t tsg_launch_syrk_i8(const int8_t *gt, int64_t Kp, int n_pad, const int64_t *win_k0,
-                              const int64_t *win_k1, int W, int32_t *C, int32_t *slices,
-                              int nsplit, hipStream_t s) {
+                              const int64_t *win_k1, int W, int32_t *C, hipStream_t s) {
     const int tpd = n_pad / BT;
-    if (nsplit > 1 && slices == nullptr) return hipErrorInvalidValue;
-    dim3 grid((unsigned) (tpd * (tpd + 1) / 2), (unsigned) W, (unsigned) nsplit);
-    hipLaunchKernelGGL(k_syrk_i8, grid, dim3(256), 0, s, gt, Kp, n_pad, win_k0, win_k1, tpd,
-                       nsplit, C, slices);
-    if (nsplit > 1) {
-        dim3 rgrid((unsigned) (n_pad / 64), (unsigned) (n_pad / 64), (unsigned) W);
-        hipLaunchKernelGGL(k_syrk_reduce, rgrid, dim3(256), 0, s, slices, nsplit, W, n_pad, C);
-    }
+    dim3 grid((unsigned) (tpd * (tpd + 1) / 2), (unsigned) W);
+    hipLaunchKernelGGL(k_syrk_i8, grid, dim3(256), 0, s, gt, Kp, n_pad, win_k0, win_k1, tpd, C);
     return hipGetLastError();
 }
 
