@@ -13,10 +13,14 @@ LIB_PATH = os.path.join(_HERE, "libtsgpu.so")
 
 _lib = None
 
-c_i32p = ctypes.POINTER(ctypes.c_int32)
-c_u32p = ctypes.POINTER(ctypes.c_uint32)
-c_u64p = ctypes.POINTER(ctypes.c_uint64)
-c_f64p = ctypes.POINTER(ctypes.c_double)
+# Array arguments are declared as untyped addresses and passed as plain
+# integers: a typed ctypes pointer costs one ctypes.cast per argument, which
+# at ~10 pointer arguments per statistics call was most of the host time of
+# a batch of queries.  The names record the element type each entry expects.
+c_i32p = ctypes.c_void_p
+c_u32p = ctypes.c_void_p
+c_u64p = ctypes.c_void_p
+c_f64p = ctypes.c_void_p
 c_void = ctypes.c_void_p
 
 # name -> (restype, argtypes); every symbol declared in include/tsgpu.h
@@ -180,10 +184,12 @@ def lib():
 
 
 def ptr(array, ctype):
-    """Pointer to the data of a C-contiguous numpy array (None for None)."""
+    """Address of the data of a C-contiguous numpy array of element type
+    ``ctype`` (None for None).  The caller keeps the array alive for the
+    duration of the call it is passed to."""
     if array is None:
         return None
-    return array.ctypes.data_as(ctypes.POINTER(ctype))
+    return array.__array_interface__["data"][0]
 
 
 def as_c(array, dtype):

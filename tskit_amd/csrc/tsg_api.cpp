@@ -884,8 +884,14 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     const int slice = 256;
     // real-valued weights: FP64 state columns, on the hot plan only
     const bool wgt = fz.weighted;
-    const size_t stage = (size_t) (wgt ? TSG_STAGE_INTS_W(K, std::min(M, slice))
-                                       : TSG_STAGE_INTS(K, std::min(M, slice))) * 4;
+    // the per-hop staging arrays only for branch sweeps with hop-by-hop
+    // summaries (smaller LDS blocks: more resident workgroups)
+    const bool hop = mode == TSG_STAT_BRANCH &&
+                     tsg_sweep_needs_hop(outs.data(), (int64_t) outs.size(), K, wgt ? 1 : 0,
+                                         wgt ? fz.col_total_w.data() : nullptr,
+                                         wgt ? fz.opar.data() : nullptr) != 0;
+    const size_t stage = (size_t) (wgt ? TSG_STAGE_INTS_W(K, std::min(M, slice), hop)
+                                       : TSG_STAGE_INTS(K, std::min(M, slice), hop)) * 4;
     // 16-bit slot counts when no sample set exceeds 65535 samples
     const bool c16 = !wgt && !ws->force_c32 &&
                      *std::max_element(col_total.begin(), col_total.end()) <= 65535;
@@ -1054,6 +1060,7 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     a.partial = (double *) ws->partial.p;
     a.dR = (double *) ws->dR.p;
     a.flags = (noskip ? TSG_F_NOSKIP : 0u) | (any_unpolarised ? TSG_F_ANY_UNPOLARISED : 0u);
+    a.no_hop = hop ? 0 : 1;
     a.g_rec = (int32_t *) ws->slab.p;
     a.g_stride = (int64_t) ts->max_nloc * R;
     r.ts = ts;

@@ -48,14 +48,17 @@ struct TsgQuery {
 // 32-bit words of the sweep kernel's LDS staging block for K state columns
 // and `span` outputs in the launch (layout in k_sweep); node records follow.
 // (an even count: the node records that follow may hold FP64 values)
-#define TSG_STAGE_INTS(K, span)                                                               \
-    (((int64_t) (span) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + (TSG_TB + 1) +        \
-      3 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K) + 2 * TSG_PSEG + \
-      8 * (int64_t) (K) + 2 * 64 + 3) & ~(int64_t) 1)
-// ... with FP64 state columns: the moved counts, a path segment's counts
-// before / after, the column totals and the zero state take two words each
-#define TSG_STAGE_INTS_W(K, span) \
-    (TSG_STAGE_INTS(K, span) + 4 * (int64_t) (K) + 2 * TSG_PSEG * (int64_t) (K))
+// `hop`: the block also holds the per-hop counts and terms of summaries
+// evaluated hop by hop (branch sweeps with such outputs only).
+#define TSG_STAGE_INTS(K, span, hop)                                                        \
+    (((int64_t) (span) + 4 * (TSG_TB + 1) + 2 * (TSG_EB + 1) + TSG_TB + (TSG_TB + 1) +      \
+      3 * TSG_EB + TSG_PB + 4 * (int64_t) (K) + 8 * (int64_t) (K) + 3 +                    \
+      ((hop) ? 2 * TSG_PSEG * (int64_t) (K) + 2 * TSG_PSEG + 2 * 64 : 0)) & ~(int64_t) 1)
+// ... with FP64 state columns: the moved counts, the counts before (and,
+// with `hop`, after) each path hop, the column totals and the zero state take
+// two words each
+#define TSG_STAGE_INTS_W(K, span, hop) \
+    (TSG_STAGE_INTS(K, span, hop) + 4 * (int64_t) (K) + ((hop) ? 2 * TSG_PSEG * (int64_t) (K) : 0))
 
 // Where a sweep keeps its node records (see k_sweep).
 #define TSG_STORE_GLOBAL 0
@@ -128,6 +131,9 @@ struct TsgSweepArgs {
     double *partial;             // per (query, chunk, window) partial sums
     double *dR;                  // [C*M] branch running-sum change over chunk
     uint32_t flags;
+    // 1: no output of the batch is evaluated hop by hop, and the LDS staging
+    // block leaves out the per-hop arrays (TSG_STAGE_INTS(K, span, 0))
+    int32_t no_hop;
     // global-memory node records (when they do not fit in LDS)
     int32_t *g_rec;
     int64_t g_stride;            // ints per chunk slab

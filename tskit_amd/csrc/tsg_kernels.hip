@@ -438,7 +438,7 @@ __device__ uint64_t summary_columns(const TsgOut &o, int K) {
 }
 
 // index of the slot pair (s, t), s <= t, in LaneOut::q
-__device__ __forceinline__ int qidx(int s, int t) {
+__host__ __device__ __forceinline__ int qidx(int s, int t) {
     return s == 0 ? t : (s == 1 ? 3 + t : (s == 2 ? 5 + t : 9));
 }
 
@@ -455,9 +455,9 @@ __device__ __forceinline__ int qidx(int s, int t) {
 // opar their parameters): trait covariance (trees.c:3961-3974) and weighted
 // relatedness (4801-4837) are quadratic too.
 template <bool WGT = false>
-__device__ int moment_form(const TsgOut &o, int K, LaneOut &L, const double *totw = nullptr,
-                           const double *opar = nullptr) {
-    if (K > 64 || !isfinite(o.inv)) return TSG_KIND_HOP;
+__host__ __device__ int moment_form(const TsgOut &o, int K, LaneOut &L,
+                                    const double *totw = nullptr, const double *opar = nullptr) {
+    if (K > 64 || !__builtin_isfinite(o.inv)) return TSG_KIND_HOP;
     const bool pol = (o.flags & 1) != 0;
     const double ni = (double) o.ni, nj = (double) o.nj, nk = (double) o.nk, nl = (double) o.nl;
     L.pc[0] = o.i;
@@ -814,16 +814,19 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     double *s2v = s1v + K;                                               // [K]
     double *gpv = s2v + K;                                               // [K]
     double *guv = gpv + K;                                               // [K]
+    // the per-hop arrays (td, sbl, xo beyond K, xn) only when some output
+    // is evaluated hop by hop
+    const bool hop_stage = MODE == 0 && a.no_hop == 0;
     double *td = guv + K;                                                // [WAVE] hop terms
-    double *evb = td + WAVE;                                             // [EB] event branch length
+    double *evb = td + (hop_stage ? WAVE : 0);                           // [EB] event branch length
     double *sbl = evb + TSG_EB;                                          // [PSEG] path branch lengths
     // state values: counts, or FP64 sums of real-valued weights
     constexpr bool WGT = STORE == TSG_STORE_HOTW;
     using Cnt = typename std::conditional<WGT, double, int32_t>::type;
-    Cnt *xc = reinterpret_cast<Cnt *>(sbl + TSG_PSEG);                  // [K] moved subtree
-    Cnt *xo = xc + K;                                                    // [PSEG x K]
-    Cnt *xn = xo + TSG_PSEG * K;                                         // [PSEG x K]
-    Cnt *tot = xn + TSG_PSEG * K;                                        // [K]
+    Cnt *xc = reinterpret_cast<Cnt *>(sbl + (hop_stage ? TSG_PSEG : 0)); // [K] moved subtree
+    Cnt *xo = xc + K;                                                    // [PSEG x K] or [K]
+    Cnt *xn = xo + (hop_stage ? TSG_PSEG * K : K);                       // [PSEG x K] or none
+    Cnt *tot = xn + (hop_stage ? TSG_PSEG * K : 0);                      // [K]
     Cnt *zero = tot + K;                                                 // [K]
     int32_t *tnrem = reinterpret_cast<int32_t *>(zero + K);              // [TB]
     int32_t *ts0 = tnrem + TSG_TB;                                       // [TB+1] tree_site0
@@ -839,7 +842,8 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     nd.R = WGT ? TSG_SLOT_WORDS_W(K, MODE == 0)
                : (HOTS ? TSG_SLOT_WORDS(K, STORE == TSG_STORE_HOT16, MODE == 0) : R);
     if (STORE != TSG_STORE_GLOBAL) {
-        nd.rec = smem + (WGT ? TSG_STAGE_INTS_W(K, span) : TSG_STAGE_INTS(K, span));
+        nd.rec = smem + (WGT ? TSG_STAGE_INTS_W(K, span, hop_stage)
+                             : TSG_STAGE_INTS(K, span, hop_stage));
     } else {
         nd.rec = a.g_rec + (int64_t) c * a.g_stride;
     }
@@ -878,7 +882,9 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     any_c3 = MODE == 0 && __any(any_c3);
     any_segp = MODE == 0 && __any(any_segp);
     any_segu = MODE == 0 && __any(any_segu);
-    any_hop = MODE == 0 && __any(any_hop);
+    // (without the per-hop arrays no output is of that kind: the host
+    // decides no_hop with the same moment_form)
+    any_hop = hop_stage && __any(any_hop);
     const bool any_seg = any_segp || any_segu;
     const bool any_agg = any_poly || any_seg;
     // count-update layout: with K <= 64 columns, each column has G lanes
@@ -2264,6 +2270,17 @@ hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl,
                             hipStream_t s) {
     return (mode == 0) ? launch_sweep_store<0>(a, store, mpl, shmem, s)
                        : launch_sweep_store<1>(a, store, mpl, shmem, s);
+}
+
+int tsg_sweep_needs_hop(const TsgOut *outs, int64_t m, int K, int weighted, const double *totw,
+                        const double *opar) {
+    for (int64_t i = 0; i < m; i++) {
+        LaneOut L;
+        const int kind = weighted ? moment_form<true>(outs[i], K, L, totw, opar)
+                                  : moment_form<false>(outs[i], K, L);
+        if (kind == TSG_KIND_HOP) return 1;
+    }
+    return 0;
 }
 
 namespace {

@@ -15,6 +15,13 @@ hipError_t tsg_launch_sweep_multi(const TsgSweepArgs *args, const int32_t *cpre,
 // store: TSG_STORE_GLOBAL / TSG_STORE_LDS / TSG_STORE_HOT
 hipError_t tsg_launch_sweep(const TsgSweepArgs &a, int mode, int store, int mpl, size_t shmem,
                             hipStream_t s);
+// 1 when some output among outs[0..m) (host memory) is a summary the branch
+// sweep evaluates hop by hop, so that its LDS staging block must hold the
+// per-hop arrays (TSG_STAGE_INTS(K, span, 1)); decided by the same code the
+// kernel runs.  weighted: FP64 state columns with totals totw and summary
+// parameters opar (host memory).
+int tsg_sweep_needs_hop(const TsgOut *outs, int64_t m, int K, int weighted, const double *totw,
+                        const double *opar);
 hipError_t tsg_launch_afs(const TsgSweepArgs &a, bool branch, bool lds, size_t shmem,
                           hipStream_t s);
 // the AFS on the hot plan: one wave per chunk, shmem bytes of LDS

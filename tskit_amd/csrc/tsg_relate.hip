@@ -457,13 +457,20 @@ __global__ void __launch_bounds__(256) k_set_divergence(const double *D, int n,
     }
 }
 
-// grand totals: one thread per window (n row sums each)
-__global__ void k_total(int64_t *rowsum, int n, int W) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= W) return;
+// grand totals: one workgroup per window sums its n row sums (exact int64,
+// so the order of the additions does not matter)
+__global__ void __launch_bounds__(256) k_total(int64_t *rowsum, int n, int W) {
+    __shared__ long long part[256];
+    const int w = blockIdx.x;
     long long s = 0;
-    for (int i = 0; i < n; i++) s += rowsum[(int64_t) w * n + i];
-    rowsum[(int64_t) W * n + w] = s;
+    for (int i = threadIdx.x; i < n; i += 256) s += rowsum[(int64_t) w * n + i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int) threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rowsum[(int64_t) W * n + w] = part[0];
 }
 
 // --- launchers ------------------------------------------------------------------
@@ -543,7 +550,7 @@ hipError_t tsg_launch_centre(const int32_t *C, const int32_t *C2, const int32_t 
                              double *K, hipStream_t s) {
     hipLaunchKernelGGL(k_rowsum, dim3((unsigned) n, (unsigned) W), dim3(256), 0, s, C, C2, sel, n,
                        n_pad, rowsum);
-    hipLaunchKernelGGL(k_total, dim3((unsigned) ((W + 63) / 64)), dim3(64), 0, s, rowsum, n, W);
+    hipLaunchKernelGGL(k_total, dim3((unsigned) W), dim3(256), 0, s, rowsum, n, W);
     const int64_t nn = (int64_t) n * n;
     hipLaunchKernelGGL(k_centre, dim3((unsigned) ((nn + 255) / 256), (unsigned) W), dim3(256), 0, s,
                        C, C2, sel, rowsum, n, n_pad, scale, K);
