@@ -61,7 +61,9 @@ template <typename T>
 __device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool comp,
                                const double *opar = nullptr) {
 #define XV(col) ((double) (comp ? (tot[col] - x[col]) : x[col]))
-    const double xi = XV(o.i);
+    // the four slots' values are read together, before the summary's case
+    // (every slot names a valid column, unused ones the first)
+    const double xi = XV(o.i), xj = XV(o.j), xk = XV(o.k), xl = XV(o.l);
     const double ni = (double) o.ni;
     switch (o.stat) {
     case TSG_STAT_DIVERSITY:
@@ -70,45 +72,38 @@ __device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool c
         return (xi > 0) ? (ni - xi) * o.inv : 0.0;
     case TSG_STAT_Y1:
         return ((xi * (ni - xi) * (ni - xi - 1)) * o.inv);
-    case TSG_STAT_DIVERGENCE: {
-        const double xj = XV(o.j);
+    case TSG_STAT_DIVERGENCE:
         return ((xi * ((double) o.nj - xj)) * o.inv);
-    }
     case TSG_STAT_RELATEDNESS: {
         double sum = 0;
         for (int c = o.cbase; c < o.cbase + o.ccount; c++) {
             sum += XV(c) / (double) tot[c];
         }
         const double mean = sum / (double) o.ccount;
-        const double xj = XV(o.j);
         return ((xi / ni - mean) * (xj / (double) o.nj - mean));
     }
-    case TSG_STAT_RELATEDNESS_NONCENTRED: {
-        const double xj = XV(o.j);
+    case TSG_STAT_RELATEDNESS_NONCENTRED:
         return ((xi * xj) * o.inv);
-    }
     case TSG_STAT_Y2: {
-        const double xj = XV(o.j), nj = (double) o.nj;
+        const double nj = (double) o.nj;
         return ((xi * (nj - xj) * (nj - xj - 1)) * o.inv);
     }
     case TSG_STAT_F2: {
-        const double xj = XV(o.j), nj = (double) o.nj;
+        const double nj = (double) o.nj;
         const double a = xi * (nj - xj);
         const double b = (xi - 1) * (nj - xj - 1) - (ni - xi) * xj;
         return (a * b * o.inv);
     }
     case TSG_STAT_Y3: {
-        const double xj = XV(o.j), xk = XV(o.k);
         return ((xi * ((double) o.nj - xj) * ((double) o.nk - xk)) * o.inv);
     }
     case TSG_STAT_F3: {
-        const double xj = XV(o.j), xk = XV(o.k), nk = (double) o.nk;
+        const double nk = (double) o.nk;
         const double a = xi * ((double) o.nj - xj);
         const double b = (xi - 1) * (nk - xk) - (ni - xi) * xk;
         return (a * b * o.inv);
     }
     case TSG_STAT_F4: {
-        const double xj = XV(o.j), xk = XV(o.k), xl = XV(o.l);
         const double nk = (double) o.nk, nl = (double) o.nl;
         const double a = xi * ((double) o.nj - xj);
         const double b = xk * (nl - xl) - xl * (nk - xk);
@@ -122,13 +117,13 @@ __device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool c
     case TSG_STAT_TRAIT_COVARIANCE:  // x^2 / (2 (n - 1)^2)
         return xi * xi * o.inv;
     case TSG_STAT_TRAIT_CORRELATION: {  // x^2 / (2 p (1 - p) n (n - 1)), p = x_l
-        const double p = XV(o.l);
+        const double p = xl;
         return (p > 0.0 && p < 1.0) ? (xi * xi) * o.inv / (p * (1 - p)) : 0.0;
     }
     case TSG_STAT_TRAIT_LINEAR_MODEL: {
         // trait i, covariates cbase .. cbase+ccount-1 with their row of V
         // (opar + par), the sample count m = x_l of n
-        const double m = XV(o.l), n = (double) o.ni;
+        const double m = xl, n = (double) o.ni;
         if (!(m > 0.0 && m < n)) return 0.0;
         const double *v = opar + o.par;
         double a = xi, denom = m;
@@ -140,11 +135,11 @@ __device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool c
         return denom < 1e-8 ? 0.0 : (a * a) / (2 * denom * denom);
     }
     case TSG_STAT_RELATEDNESS_WEIGHTED: {  // (x_i - a x_l)(x_j - b x_l)
-        const double pn = XV(o.l), xj = XV(o.j);
+        const double pn = xl;
         return (xi - opar[o.par] * pn) * (xj - opar[o.par + 1] * pn);
     }
     case TSG_STAT_RELATEDNESS_WEIGHTED_NONCENTRED:
-        return xi * XV(o.j);
+        return xi * xj;
     default:
         break;
     }
@@ -302,6 +297,16 @@ template <class NS>
 __device__ void init_hot(const TsgSweepArgs &a, const NS &nd, int c, int nslot, int K,
                          int lane) {
     const int64_t s0 = a.chunk_node0[c];
+    // parents of the slots this lane owns in the level pass (slot lane + 64 r),
+    // read now so that their loads overlap the seeding
+    constexpr int PR = 8;
+    const bool own = nslot <= PR * WAVE;
+    int par[PR];
+#pragma unroll
+    for (int r = 0; r < PR; r++) {
+        const int h = lane + r * WAVE;
+        par[r] = (own && h < nslot) ? a.slot_par[s0 + h] : -1;
+    }
     const int words = nslot * nd.R;
     for (int i = lane; i < words; i += WAVE) nd.rec[i] = 0;
     wave_sync();
@@ -311,7 +316,7 @@ __device__ void init_hot(const TsgSweepArgs &a, const NS &nd, int c, int nslot, 
     // (sample, column) pairs, several per lane in flight: the slot gathers
     // are independent reads
     const uint16_t *ss = sample_map(a, c);
-    constexpr int U = 8;
+    constexpr int U = 16;
     for (int i0 = lane; i0 < a.npairs; i0 += WAVE * U) {
         int ps[U], pc[U];
         unsigned ph[U];
@@ -341,11 +346,23 @@ __device__ void init_hot(const TsgSweepArgs &a, const NS &nd, int c, int nslot, 
     // the last level holds the roots of the hot forest: nothing above them
     for (int l = 0; l + 1 < nlev; l++) {
         const int h0 = lev[l], h1 = lev[l + 1];
-        for (int idx = lane; idx < (h1 - h0) * K; idx += WAVE) {
-            const int h = h0 + idx / K, k = idx % K;
-            const int p = a.slot_par[s0 + h];
-            const auto v = nd.get(h, k);
-            if (p >= 0 && v != 0) nd.add(p, k, v);
+        if (own) {
+#pragma unroll
+            for (int r = 0; r < PR; r++) {
+                const int h = lane + r * WAVE;
+                if (h < h0 || h >= h1 || par[r] < 0) continue;
+                for (int k = 0; k < K; k++) {
+                    const auto v = nd.get(h, k);
+                    if (v != 0) nd.add(par[r], k, v);
+                }
+            }
+        } else {
+            for (int idx = lane; idx < (h1 - h0) * K; idx += WAVE) {
+                const int h = h0 + idx / K, k = idx % K;
+                const int p = a.slot_par[s0 + h];
+                const auto v = nd.get(h, k);
+                if (p >= 0 && v != 0) nd.add(p, k, v);
+            }
         }
         wave_sync();
     }
@@ -435,6 +452,18 @@ __device__ uint64_t summary_columns(const TsgOut &o, int K) {
         return ~0ull;
     }
     return m;
+}
+
+// A column mask in the sweep's count-update layout, where column k's count
+// is held by lanes k G .. k G + G - 1 and the ballot of the moved counts
+// sets bit k G: column k's bit moved to bit k G (all ones stay all ones).
+__device__ __forceinline__ uint64_t spread_columns(uint64_t m, int K, int G) {
+    if (G == 1 || m == ~0ull) return m;
+    uint64_t r = 0;
+    for (int k = 0; k < K; k++) {
+        if ((m >> k) & 1) r |= 1ull << (k * G);
+    }
+    return r;
 }
 
 // index of the slot pair (s, t), s <= t, in LaneOut::q
@@ -668,8 +697,9 @@ __device__ __forceinline__ double lane_plus(double v) {
 // S + q_0 + q_1 + ... + q_{G-1}, added one at a time in that order, where
 // q_g is q of lane + g: a column's G hop groups sit in adjacent lanes (lane
 // = k G + g, G <= 16 a power of two, within one row of 16), so the head lane
-// of the column adds the terms of one round of G hops in hop order.  The
-// whole wave runs it (uniform control flow); other lanes' S is not used.
+// of the column adds the terms of one round of G hops in hop order (lanes
+// past the path's last hop hold zeros).  The whole wave runs it (uniform
+// control flow); other lanes' S is not used.
 __device__ __forceinline__ double hop_chain(double S, double q, int G) {
     S = S + q;
     if (G > 1) S = S + lane_plus<1>(q);
@@ -706,7 +736,7 @@ __device__ __forceinline__ T iseg_unpol(T x, T n) {
 }
 
 template <int MPL, bool WGT>
-__device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *gout, int c, int lane,
+__device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *gout, int c, int lane, int G,
                               LaneOut (&lo)[MPL]) {
 #pragma unroll
     for (int j = 0; j < MPL; j++) {
@@ -728,7 +758,7 @@ __device__ void setup_outputs(const TsgSweepArgs &a, const TsgOut *gout, int c, 
             L.wend = L.wi + a.cq_nwin[cq];
             L.Mq = q.M;
             L.slot0 = q.pbase + a.cq_poff[cq] * (int64_t) q.M + (m - q.mbase);
-            L.mask = summary_columns(o, a.K);
+            L.mask = spread_columns(summary_columns(o, a.K), a.K, G);
             L.kind = moment_form<WGT>(o, a.K, L, a.col_total_w, a.opar);
             if (L.kind == TSG_KIND_SEG) {
                 L.pc[0] = o.i;
@@ -864,8 +894,21 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
         init_chunk(a, nd, n0, nloc, K, lane);
     }
 
+    // count-update layout: with K <= 64 columns, each column has G lanes
+    // (lane = k G + g, G the largest power of two <= 16 with G K <= 64) that
+    // update G hops of a path at once (round by round); a column's path sums
+    // add the rounds' per-hop terms in hop order (hop_chain), so they do not
+    // depend on K.  With more columns one group strides over them.  The
+    // path-sum kinds (POLY, SEG) only occur with K <= 64.  The outputs'
+    // column masks are in this layout (spread_columns).
+    const bool kwide = K > WAVE || K < 1;
+    const int G = kwide ? 1 : min(16, 1 << (31 - __clz(WAVE / K)));
+    const int uk = kwide ? lane : lane / G;
+    const int ug = kwide ? 0 : lane - uk * G;
+    const bool uon = kwide || uk < K;
+
     LaneOut lo[MPL];
-    setup_outputs<MPL, WGT>(a, gout, c, lane, lo);
+    setup_outputs<MPL, WGT>(a, gout, c, lane, G, lo);
     int wfirst[MPL];
     bool any_poly = false, any_c3 = false, any_segp = false, any_segu = false, any_hop = false;
 #pragma unroll
@@ -887,17 +930,6 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     any_hop = hop_stage && __any(any_hop);
     const bool any_seg = any_segp || any_segu;
     const bool any_agg = any_poly || any_seg;
-    // count-update layout: with K <= 64 columns, each column has G lanes
-    // (lane = k G + g, G the largest power of two <= 16 with G K <= 64) that
-    // update G hops of a path at once (round by round); a column's path sums
-    // add the rounds' per-hop terms in hop order (hop_chain), so they do not
-    // depend on K.  With more columns one group strides over them.  The
-    // path-sum kinds (POLY, SEG) only occur with K <= 64.
-    const bool kwide = K > WAVE || K < 1;
-    const int G = kwide ? 1 : min(16, 1 << (31 - __clz(WAVE / K)));
-    const int uk = kwide ? lane : lane / G;
-    const int ug = kwide ? 0 : lane - uk * G;
-    const bool uon = kwide || uk < K;
     const Cnt totk = (!kwide && uon) ? tot[uk] : Cnt(0);
 
     const int t0 = a.chunk_tree0[c], t1 = a.chunk_tree0[c + 1];
@@ -965,21 +997,24 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             const bool ins = (e - e0) >= nrem;
             const int u = evc[le];
             const double ebl = evb[le];
-            // columns the moved subtree carries
+            // the event's path range and the child's branch length are read
+            // with its counts (independent LDS reads in flight together)
+            const int64_t p0 = evp[le], p1 = evp[le + 1];
+            const double ubl = MODE == 0 ? nd.bl(u) : 0.0;
+            // columns the moved subtree carries: each lane reads the count of
+            // its update column uk; the ballot sets column k's bit at k G
             uint64_t cm;
-            if (K <= WAVE) {
-                Cnt xk = 0;
-                if (lane < K) {
-                    xk = nd.get(u, lane);
-                    xc[lane] = xk;
-                }
-                cm = __ballot(xk != 0);
+            Cnt xk = 0;
+            if (!kwide) {
+                if (uon) xk = nd.get(u, uk);
+                if (uon && ug == 0) xc[uk] = xk;
+                cm = __ballot(uon && ug == 0 && xk != 0);
             } else {
                 bool nz = false;
                 for (int k = lane; k < K; k += WAVE) {
-                    const Cnt xk = nd.get(u, k);
-                    xc[k] = xk;
-                    nz |= (xk != 0);
+                    const Cnt xw = nd.get(u, k);
+                    xc[k] = xw;
+                    nz |= (xw != 0);
                 }
                 cm = __any(nz) ? ~0ull : 0ull;
             }
@@ -990,8 +1025,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             // with the walk's, after the path update, which it does not
             // depend on (the path starts at the parent)
             const double sg = ins ? 1.0 : -1.0;
-            const double b = MODE == 0 ? sg * (double) (ins ? ebl : nd.bl(u)) : 0.0;
-            SWEEP_SYNC();
+            const double b = MODE == 0 ? sg * (double) (ins ? ebl : ubl) : 0.0;
             // after seeding nothing reads parent pointers (the root paths
             // are precomputed), so only the branch length is kept current
             if (lane == 0) nd.set_bl(u, ins ? ebl : 0.0);
@@ -1016,19 +1050,28 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                 if (on) act[pos[j]] = lo[j].lm;
                 na += __popcll(bal);
             }
-            const int64_t p0 = evp[le], p1 = evp[le + 1];
-            const int32_t *pth = gpath ? a.path + p0 : pbuf + (p0 - pb0);
+            // the path's hops are read from LDS: from the event batch, or,
+            // for a path longer than the batch buffer (gpath: the batch is
+            // this event alone), segment by segment copied into its start
+            int poff = 0;
+            auto path_node = [&](int i) -> int { return pbuf[poff + i]; };
             SWEEP_SYNC();
             // path sums of this lane's column uk over its hops: S0 = sum bl,
             // S1 = sum bl x, S2 = sum bl x^2 (x the count before the event),
             // and the segregating-sites numerator changes
             double s0 = 0, s1 = 0, s2 = 0, gp = 0, gu = 0;
-            const Cnt dx = (!kwide && uon) ? (ins ? xc[uk] : -xc[uk]) : Cnt(0);
+            const Cnt dx = (!kwide && uon) ? (ins ? xk : -xk) : Cnt(0);
             for (int64_t sg0 = 0; sg0 < p1 - p0; sg0 += TSG_PSEG) {
                 const int h = (int) min((int64_t) TSG_PSEG, p1 - p0 - sg0);
+                if (gpath) {
+                    if (lane < h) pbuf[lane] = a.path[p0 + sg0 + lane];
+                    SWEEP_SYNC();
+                } else {
+                    poff = (int) (p0 - pb0 + sg0);
+                }
                 if (kwide) {
                     for (int i = 0; i < h; i++) {
-                        const int node = pth[sg0 + i];
+                        const int node = path_node(i);
                         for (int k = lane; k < K; k += WAVE) {
                             const Cnt old = nd.get(node, k);
                             const Cnt nw = ins ? old + xc[k] : old - xc[k];
@@ -1047,8 +1090,11 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                         // the segregating-sites numerators
                         double q0 = 0, q1 = 0, q2 = 0, qp = 0, qu = 0;
                         if (uon && i < h) {
-                            const int node = pth[sg0 + i];
+                            const int node = path_node(i);
+                            // the count and the branch length are read
+                            // together, before the count's update
                             const Cnt old = nd.get(node, uk);
+                            const double bw = (MODE == 0 && any_agg) ? nd.bl(node) : 0.0;
                             const Cnt nw = old + dx;
                             nd.put(node, uk, nw);
                             if (MODE == 0 && any_hop) {
@@ -1056,7 +1102,6 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                                 xn[i * K + uk] = nw;
                             }
                             if (MODE == 0 && any_agg) {
-                                const double bw = nd.bl(node);
                                 q0 = bw;
                                 if (any_poly) {
                                     const double xold = (double) old;
@@ -1083,7 +1128,7 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                     }
                 }
                 if (MODE == 0 && any_hop) {
-                    for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(pth[sg0 + i]);
+                    for (int i = lane; i < h; i += WAVE) sbl[i] = nd.bl(path_node(i));
                 }
                 SWEEP_SYNC();
                 if (MODE == 0 && na > 0) {
@@ -1117,10 +1162,10 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                                               __builtin_amdgcn_readfirstlane(__double2loint(s0)));
                     }
                     if (uon && ug == 0) {
-                        s1v[uk] = s1;
-                        s2v[uk] = s2;
-                        gpv[uk] = gp;
-                        guv[uk] = gu;
+                        if (any_poly) s1v[uk] = s1;
+                        if (any_c3) s2v[uk] = s2;
+                        if (any_segp) gpv[uk] = gp;
+                        if (any_segu) guv[uk] = gu;
                     }
                     SWEEP_SYNC();
                 }
@@ -1149,6 +1194,18 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
         }
     };
 
+    // site sweeps: each lane's output specs in registers (read once, not per
+    // site), and the right end of its current window
+    TsgOut ow[MPL];
+    double wnext[MPL];
+#pragma unroll
+    for (int j = 0; j < MPL; j++) {
+        wnext[j] = 0.0;
+        if (MODE == 1 && lo[j].lm >= 0) {
+            ow[j] = gout[lo[j].lm];
+            if (lo[j].wi < lo[j].wend) wnext[j] = lo[j].win[lo[j].wi + 1];
+        }
+    }
     for (int t = t0; t < t1; t++) {
         if (t > t0) apply_transition(t);
         const int lt = tree(t);
@@ -1174,36 +1231,64 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
             const int s0 = ts0[lt], s1 = ts0[lt + 1];
             for (int s = s0; s < s1; s++) {
                 const double pos = a.site_pos[s];
+                const int na = a.site_nalleles[s];
+                const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
 #pragma unroll
                 for (int j = 0; j < MPL; j++) {
                     LaneOut &L = lo[j];
                     if (L.lm < 0) continue;
-                    while (L.wi < L.wend && L.win[L.wi + 1] <= pos) {
+                    while (L.wi < L.wend && wnext[j] <= pos) {
                         flush_window(a, L, wfirst[j]);
                         L.wi++;
+                        wnext[j] = (L.wi < L.wend) ? L.win[L.wi + 1] : 0.0;
                     }
                 }
-                const int na = a.site_nalleles[s];
-                const int m0 = a.site_mut0[s], m1 = a.site_mut0[s + 1];
                 double val[MPL];
 #pragma unroll
                 for (int j = 0; j < MPL; j++) val[j] = 0.0;
+                // with at most 4 alleles (and one column per lane) each lane
+                // forms its column's allele counts in one pass over the
+                // site's mutations: w[a] = [a = 0] total + the counts below
+                // the mutations to allele a - those below mutations from it
+                const bool fast = na <= 4 && !kwide;
+                Cnt w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+                if (fast && lane < K) {
+                    w0 = tot[lane];
+                    for (int mm = m0; mm < m1; mm++) {
+                        const int node = a.mut_node[mm];
+                        const int ma = a.mut_allele[mm], mp = a.mut_pallele[mm];
+                        const Cnt xv = nd.get(node, lane);
+                        // (added, then taken away, as the general form does)
+                        w0 = (ma == 0) ? w0 + xv : w0;
+                        w0 = (mp == 0) ? w0 - xv : w0;
+                        w1 = (ma == 1) ? w1 + xv : w1;
+                        w1 = (mp == 1) ? w1 - xv : w1;
+                        w2 = (ma == 2) ? w2 + xv : w2;
+                        w2 = (mp == 2) ? w2 - xv : w2;
+                        w3 = (ma == 3) ? w3 + xv : w3;
+                        w3 = (mp == 3) ? w3 - xv : w3;
+                    }
+                }
                 // polarised outputs skip the ancestral allele (trees.c:1639)
                 for (int al = any_unpolarised ? 0 : 1; al < na; al++) {
-                    for (int k = lane; k < K; k += WAVE) {
-                        Cnt w = (al == 0) ? tot[k] : Cnt(0);
-                        for (int mm = m0; mm < m1; mm++) {
-                            const int node = a.mut_node[mm];
-                            if (a.mut_allele[mm] == al) w += nd.get(node, k);
-                            if (a.mut_pallele[mm] == al) w -= nd.get(node, k);
+                    if (fast) {
+                        if (lane < K) xo[lane] = al == 0 ? w0 : (al == 1 ? w1 : (al == 2 ? w2 : w3));
+                    } else {
+                        for (int k = lane; k < K; k += WAVE) {
+                            Cnt w = (al == 0) ? tot[k] : Cnt(0);
+                            for (int mm = m0; mm < m1; mm++) {
+                                const int node = a.mut_node[mm];
+                                if (a.mut_allele[mm] == al) w += nd.get(node, k);
+                                if (a.mut_pallele[mm] == al) w -= nd.get(node, k);
+                            }
+                            xo[k] = w;
                         }
-                        xo[k] = w;
                     }
                     SWEEP_SYNC();
 #pragma unroll
                     for (int j = 0; j < MPL; j++) {
                         if (lo[j].lm < 0) continue;
-                        const TsgOut &o = gout[lo[j].lm];
+                        const TsgOut &o = ow[j];
                         if (al > 0 || !(o.flags & 1)) val[j] += eval_summary(o, xo, tot, false, a.opar);
                     }
                     SWEEP_SYNC();
