@@ -435,8 +435,9 @@ int parse_sample_sets(const tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, 
     }
     out.assign(K, {});
     // a sample twice in one set: marks in a per-sample table, cleared after
-    // each set
-    std::vector<uint8_t> seen(ts->num_samples, 0);
+    // each set (one table per thread, reused: all zeros between calls)
+    thread_local std::vector<uint8_t> seen;
+    if (seen.size() < (size_t) ts->num_samples) seen.assign((size_t) ts->num_samples, 0);
     j = 0;
     for (uint64_t k = 0; k < K; k++) {
         out[k].reserve(sizes[k]);

@@ -57,13 +57,24 @@ namespace {
 //
 // State columns of real-valued weights (T = double) take the weighted
 // summaries, whose parameters beyond one denominator are in opar.
-template <typename T>
+template <typename T, bool PRE = false>
 __device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool comp,
                                const double *opar = nullptr) {
 #define XV(col) ((double) (comp ? (tot[col] - x[col]) : x[col]))
-    // the four slots' values are read together, before the summary's case
-    // (every slot names a valid column, unused ones the first)
-    const double xi = XV(o.i), xj = XV(o.j), xk = XV(o.k), xl = XV(o.l);
+    // PRE: the four slots' values are read together, before the summary's
+    // case (every slot names a valid column, unused ones the first), for
+    // callers whose lanes take different cases; otherwise each case reads
+    // the slots it uses (fewer live registers)
+    const double xi = XV(o.i);
+    double pj = 0.0, pk = 0.0, pl = 0.0;
+    if constexpr (PRE) {
+        pj = XV(o.j);
+        pk = XV(o.k);
+        pl = XV(o.l);
+    }
+#define XJ (PRE ? pj : XV(o.j))
+#define XK (PRE ? pk : XV(o.k))
+#define XL (PRE ? pl : XV(o.l))
     const double ni = (double) o.ni;
     switch (o.stat) {
     case TSG_STAT_DIVERSITY:
@@ -73,40 +84,40 @@ __device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool c
     case TSG_STAT_Y1:
         return ((xi * (ni - xi) * (ni - xi - 1)) * o.inv);
     case TSG_STAT_DIVERGENCE:
-        return ((xi * ((double) o.nj - xj)) * o.inv);
+        return ((xi * ((double) o.nj - XJ)) * o.inv);
     case TSG_STAT_RELATEDNESS: {
         double sum = 0;
         for (int c = o.cbase; c < o.cbase + o.ccount; c++) {
             sum += XV(c) / (double) tot[c];
         }
         const double mean = sum / (double) o.ccount;
-        return ((xi / ni - mean) * (xj / (double) o.nj - mean));
+        return ((xi / ni - mean) * (XJ / (double) o.nj - mean));
     }
     case TSG_STAT_RELATEDNESS_NONCENTRED:
-        return ((xi * xj) * o.inv);
+        return ((xi * XJ) * o.inv);
     case TSG_STAT_Y2: {
         const double nj = (double) o.nj;
-        return ((xi * (nj - xj) * (nj - xj - 1)) * o.inv);
+        return ((xi * (nj - XJ) * (nj - XJ - 1)) * o.inv);
     }
     case TSG_STAT_F2: {
         const double nj = (double) o.nj;
-        const double a = xi * (nj - xj);
-        const double b = (xi - 1) * (nj - xj - 1) - (ni - xi) * xj;
+        const double a = xi * (nj - XJ);
+        const double b = (xi - 1) * (nj - XJ - 1) - (ni - xi) * XJ;
         return (a * b * o.inv);
     }
     case TSG_STAT_Y3: {
-        return ((xi * ((double) o.nj - xj) * ((double) o.nk - xk)) * o.inv);
+        return ((xi * ((double) o.nj - XJ) * ((double) o.nk - XK)) * o.inv);
     }
     case TSG_STAT_F3: {
         const double nk = (double) o.nk;
-        const double a = xi * ((double) o.nj - xj);
-        const double b = (xi - 1) * (nk - xk) - (ni - xi) * xk;
+        const double a = xi * ((double) o.nj - XJ);
+        const double b = (xi - 1) * (nk - XK) - (ni - xi) * XK;
         return (a * b * o.inv);
     }
     case TSG_STAT_F4: {
         const double nk = (double) o.nk, nl = (double) o.nl;
-        const double a = xi * ((double) o.nj - xj);
-        const double b = xk * (nl - xl) - xl * (nk - xk);
+        const double a = xi * ((double) o.nj - XJ);
+        const double b = XK * (nl - XL) - XL * (nk - XK);
         return (a * b * o.inv);
     }
     default:
@@ -117,13 +128,13 @@ __device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool c
     case TSG_STAT_TRAIT_COVARIANCE:  // x^2 / (2 (n - 1)^2)
         return xi * xi * o.inv;
     case TSG_STAT_TRAIT_CORRELATION: {  // x^2 / (2 p (1 - p) n (n - 1)), p = x_l
-        const double p = xl;
+        const double p = XL;
         return (p > 0.0 && p < 1.0) ? (xi * xi) * o.inv / (p * (1 - p)) : 0.0;
     }
     case TSG_STAT_TRAIT_LINEAR_MODEL: {
         // trait i, covariates cbase .. cbase+ccount-1 with their row of V
         // (opar + par), the sample count m = x_l of n
-        const double m = xl, n = (double) o.ni;
+        const double m = XL, n = (double) o.ni;
         if (!(m > 0.0 && m < n)) return 0.0;
         const double *v = opar + o.par;
         double a = xi, denom = m;
@@ -135,16 +146,19 @@ __device__ double eval_summary(const TsgOut &o, const T *x, const T *tot, bool c
         return denom < 1e-8 ? 0.0 : (a * a) / (2 * denom * denom);
     }
     case TSG_STAT_RELATEDNESS_WEIGHTED: {  // (x_i - a x_l)(x_j - b x_l)
-        const double pn = xl;
-        return (xi - opar[o.par] * pn) * (xj - opar[o.par + 1] * pn);
+        const double pn = XL;
+        return (xi - opar[o.par] * pn) * (XJ - opar[o.par + 1] * pn);
     }
     case TSG_STAT_RELATEDNESS_WEIGHTED_NONCENTRED:
-        return xi * xj;
+        return xi * XJ;
     default:
         break;
     }
     }
     return 0.0;
+#undef XJ
+#undef XK
+#undef XL
 #undef XV
 }
 
@@ -1289,7 +1303,9 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
                     for (int j = 0; j < MPL; j++) {
                         if (lo[j].lm < 0) continue;
                         const TsgOut &o = ow[j];
-                        if (al > 0 || !(o.flags & 1)) val[j] += eval_summary(o, xo, tot, false, a.opar);
+                        if (al > 0 || !(o.flags & 1)) {
+                            val[j] += eval_summary<Cnt, true>(o, xo, tot, false, a.opar);
+                        }
                     }
                     SWEEP_SYNC();
                 }

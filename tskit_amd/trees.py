@@ -86,6 +86,14 @@ def safe_np_int_cast(int_array, dtype):
         int_array = np.array(int_array)
     if int_array.size == 0:
         return int_array.astype(dtype)
+    src = int_array.dtype
+    if src.kind in "iu" and not np.can_cast(src, dtype, casting="safe"):
+        # integers of a wider type: the bounds check without the failed safe
+        # cast's exception (same result, same errors)
+        bounds = np.iinfo(dtype)
+        if int_array.min() < bounds.min or int_array.max() > bounds.max:
+            raise OverflowError(f"Cannot convert safely to {dtype} type")
+        return int_array.astype(dtype, casting="unsafe")
     try:
         return int_array.astype(dtype, casting="safe")
     except TypeError:
