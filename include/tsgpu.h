@@ -161,7 +161,7 @@ int tsg_treeseq_get_profile_modes(const tsg_treeseq_t *ts, double *ms, uint64_t 
  * options: TSG_STAT_SITE / TSG_STAT_BRANCH / TSG_STAT_NODE (one of them;
  * none = site), TSG_STAT_POLARISED, TSG_STAT_SPAN_NORMALISE.  result is
  * [W x P], or [W x num_nodes x P] in node mode (trees.c:1788-1918, on the
- * hot plan; not in a batch).
+ * hot plan; in a batch it runs on its own after the fused sweeps).
  * ---- one-way statistics ------------------------------------------------ */
 /* tsk_treeseq_diversity (trees.h:1103, trees.c:3950-3958) */
 int tsg_treeseq_diversity(tsg_treeseq_t *ts, uint64_t num_sample_sets,
@@ -321,7 +321,8 @@ int tsg_treeseq_genetic_relatedness_matrix(tsg_treeseq_t *ts, uint64_t num_sampl
 
 /* ---- batched form (no counterpart in the reference) ---------------------
  * A batch collects queries on one tree sequence; tsg_batch_run executes them
- * as one fused sweep per mode (branch / site), sharing the path walks.  Each
+ * as one fused sweep per mode (branch / site), sharing the path walks, then
+ * its node-mode queries one by one (not fused).  Each
  * tsg_batch_<stat> takes the arguments of tsg_treeseq_<stat>, validates them
  * the same way (same error numbers) and records the query; `result` must
  * stay valid until tsg_batch_run returns. */
@@ -346,7 +347,8 @@ int tsg_batch_run_multi(tsg_batch_t **batches, uint64_t n);
  * over from ordinary host memory.  The tree sequence must already be
  * resident on the current device (tsg_treeseq_upload; otherwise
  * TSG_ERR_NOT_RESIDENT = -30007), and its queries must run on the hot plan
- * (otherwise TSG_ERR_UNSUPPORTED_INPUT); a batch needing more than the
+ * (otherwise TSG_ERR_UNSUPPORTED_INPUT) and be of branch or site mode
+ * (node mode: TSG_ERR_UNSUPPORTED_STAT_MODE); a batch needing more than the
  * 64 MiB returns TSG_ERR_SCRATCH_TOO_SMALL (-30006) with no result written.
  * Modes run one after the other on the null stream; results equal
  * tsg_batch_run's.  Calls on one GPU share its array: calls that overlap

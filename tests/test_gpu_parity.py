@@ -701,11 +701,24 @@ def test_node_mode(require_gpu, name):
     _check("Tajimas_D", got, tajimas_d(T, S, [len(s) for s in sets]).reshape(3, N, k))
 
 
-def test_node_mode_not_batched(small):
+def test_node_mode_in_batch(small):
+    """A node statistic recorded in a batch runs on its own after the fused
+    sweeps: the same values as the direct call (node sums are FP64 atomics,
+    so to rounding), and the batch's fused results are unchanged."""
     tc, ts, orc = small
+    sets = cases.sample_sets(200, 2, 10, 1)
+    w = np.linspace(0, tc.sequence_length, 4)
     b = ts.batch()
-    with pytest.raises(Exception):
-        b.diversity(cases.sample_sets(200, 2, 10, 1), mode="node")
+    p_node = b.diversity(sets, windows=w, mode="node")
+    p_br = b.divergence(sets, windows=w, mode="branch")
+    p_site = b.Tajimas_D(sets, windows=w, mode="site")
+    b.run()
+    node = ts.diversity(sets, windows=w, mode="node")
+    assert p_node.result().shape == node.shape
+    assert np.allclose(p_node.result(), node, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(p_br.result(), ts.divergence(sets, windows=w, mode="branch"))
+    assert np.array_equal(p_site.result(), ts.Tajimas_D(sets, windows=w, mode="site"),
+                          equal_nan=True)
 
 
 @pytest.mark.parametrize("windows", [None, "three"])

@@ -274,10 +274,20 @@ struct Seeds {
 // form (one entry per weighted sample) and the flat pair list
 void seeds_host(const std::vector<int32_t> &pair_s, const std::vector<int32_t> &pair_c,
                 Seeds &sd, const std::vector<double> *pair_w = nullptr) {
-    std::vector<int32_t> order(pair_s.size());
+    // the pairs' order by sample, stable: a least-significant-digit radix
+    // sort over 11-bit digits of the (non-negative) sample index
+    const size_t np = pair_s.size();
+    std::vector<int32_t> order(np), tmp(np);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t x, int32_t y) { return pair_s[x] < pair_s[y]; });
+    int32_t smax = 0;
+    for (int32_t v : pair_s) smax = std::max(smax, v);
+    for (int shift = 0; shift == 0 || (smax >> shift) != 0; shift += 11) {
+        size_t count[2049] = {0};
+        for (size_t i = 0; i < np; i++) count[((pair_s[order[i]] >> shift) & 2047) + 1]++;
+        for (int d = 0; d < 2048; d++) count[d + 1] += count[d];
+        for (size_t i = 0; i < np; i++) tmp[count[(pair_s[order[i]] >> shift) & 2047]++] = order[i];
+        order.swap(tmp);
+    }
     sd.seed_sample.clear();
     sd.seed_off.assign(1, 0);
     sd.col.resize(order.size());
@@ -912,28 +922,48 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     }
     const int C = use_hot ? ts->hot.num_chunks : ts->num_chunks;
     const int32_t *ct0 = use_hot ? ts->hot.chunk_tree0.data() : ts->chunk_tree0.data();
-    // per (chunk, query) window ranges
+    // per (chunk, query) window ranges; the chunks' spans are read once, and
+    // queries with the same windows as an earlier query of the batch take
+    // its ranges
     std::vector<int32_t> cq_wfirst((size_t) C * Q), cq_nwin((size_t) C * Q);
     std::vector<int64_t> cq_poff((size_t) C * Q);
+    std::vector<double> cspan((size_t) C + 1);
+    for (int c = 0; c <= C; c++) cspan[c] = ts->breakpoints[ct0[c]];
+    std::vector<int64_t> qslots(Q, 0);
     int64_t pcount = 0;
     for (int qi = 0; qi < Q; qi++) {
         const double *w = wins.data() + qs[qi].win_off;
         const int W = qs[qi].W;
         qs[qi].pbase = pcount;
-        int64_t slots = 0;
-        // windows touched by each chunk: both ends move forward with c
-        int wf = 0, wl = 0;
-        for (int c = 0; c < C; c++) {
-            const double cs = ts->breakpoints[ct0[c]];
-            const double ce = ts->breakpoints[ct0[c + 1]];
-            while (wf + 1 < W && w[wf + 1] <= cs) wf++;  // last window starting <= cs
-            if (wl < wf) wl = wf;
-            while (wl + 1 < W && w[wl + 1] < ce) wl++;   // last window starting < ce
-            cq_wfirst[(size_t) c * Q + qi] = wf;
-            cq_nwin[(size_t) c * Q + qi] = wl - wf + 1;
-            cq_poff[(size_t) c * Q + qi] = slots;
-            slots += wl - wf + 1;
+        int same = -1;
+        for (int pj = 0; pj < qi && same < 0; pj++) {
+            if (qs[pj].W == W && std::equal(w, w + W + 1, wins.data() + qs[pj].win_off)) same = pj;
         }
+        int64_t slots = 0;
+        if (same >= 0) {
+            for (int c = 0; c < C; c++) {
+                const size_t from = (size_t) c * Q + same, to = (size_t) c * Q + qi;
+                cq_wfirst[to] = cq_wfirst[from];
+                cq_nwin[to] = cq_nwin[from];
+                cq_poff[to] = cq_poff[from];
+            }
+            slots = qslots[same];
+        } else {
+            // windows touched by each chunk: both ends move forward with c
+            int wf = 0, wl = 0;
+            for (int c = 0; c < C; c++) {
+                const double cs = cspan[c], ce = cspan[c + 1];
+                while (wf + 1 < W && w[wf + 1] <= cs) wf++;  // last window starting <= cs
+                if (wl < wf) wl = wf;
+                while (wl + 1 < W && w[wl + 1] < ce) wl++;   // last window starting < ce
+                const size_t at = (size_t) c * Q + qi;
+                cq_wfirst[at] = wf;
+                cq_nwin[at] = wl - wf + 1;
+                cq_poff[at] = slots;
+                slots += wl - wf + 1;
+            }
+        }
+        qslots[qi] = slots;
         pcount += slots * qs[qi].M;
     }
     // combine: lanes per (window, output) ~ chunks per window, a power of 2
@@ -1299,8 +1329,12 @@ int run_node(tsg_treeseq *ts, Query &q);
 int run_one(tsg_treeseq *ts, Query &q) {
     Use use(ts);
     if (q.mode == TSG_STAT_NODE) {
-        // node statistics are not fused into batches
-        if (g_capture != nullptr) return TSG_ERR_UNSUPPORTED_STAT_MODE;
+        // node statistics are not fused: a batch runs them on their own,
+        // after its fused sweeps
+        if (g_capture != nullptr) {
+            g_capture->push_back(std::make_unique<Query>(std::move(q)));
+            return 0;
+        }
         return run_node(ts, q);
     }
     if (g_capture != nullptr) {
@@ -2158,16 +2192,21 @@ int tsg_treeseq_Tajimas_D(tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, co
     q.blocks.push_back(Block{TSG_STAT_DIVERSITY, 0, {}, false});
     q.blocks.push_back(Block{TSG_STAT_SEGREGATING_SITES, 0, {}, false});
     q.final_kind = TSG_FINAL_TAJIMAS_D;
+    // h(n) = sum_{i<n} 1/i and g(n) = sum_{i<n} 1/i^2, accumulated in
+    // increasing i: prefix sums kept per thread and extended on demand (the
+    // same additions in the same order as a fresh loop)
+    thread_local std::vector<double> hsum{0.0}, gsum{0.0};
     for (uint64_t k = 0; k < K; k++) {
-        const double n = (double) sizes[k];
-        double h = 0, g = 0;
-        for (uint64_t i = 1; i < sizes[k]; i++) {
-            h += 1.0 / (double) i;
-            g += 1.0 / ((double) i * (double) i);
+        const uint64_t nk = sizes[k];
+        while (hsum.size() < std::max<uint64_t>(nk, 1)) {
+            const double i = (double) hsum.size();
+            hsum.push_back(hsum.back() + 1.0 / i);
+            gsum.push_back(gsum.back() + 1.0 / (i * i));
         }
-        q.fin_dpar.push_back(n);
-        q.fin_dpar.push_back(h);
-        q.fin_dpar.push_back(g);
+        const size_t at = (size_t) (nk > 0 ? nk - 1 : 0);
+        q.fin_dpar.push_back((double) nk);
+        q.fin_dpar.push_back(hsum[at]);
+        q.fin_dpar.push_back(gsum[at]);
     }
     q.P = (int) K;
     q.result = result;
@@ -3196,9 +3235,19 @@ int run_two_modes(tsg_treeseq *ts, std::vector<std::vector<Query *>> &groups,
     }
     return 0;
 }
-}  // namespace
 
-int tsg_batch_run(tsg_batch *b) {
+// The node-mode queries of a batch, each on its own (they are not fused).
+int run_node_queries(tsg_batch *b) {
+    for (auto &q : b->queries) {
+        if (q->mode != TSG_STAT_NODE) continue;
+        const int ret = run_node(b->ts, *q);
+        if (ret != 0) return ret;
+    }
+    return 0;
+}
+
+// The branch and site queries of a batch: one fused sweep per mode.
+int run_fused_modes(tsg_batch *b) {
     Use use(b->ts);
     std::vector<std::vector<Query *>> groups;
     for (uint32_t mode : {TSG_STAT_BRANCH, TSG_STAT_SITE}) {
@@ -3236,6 +3285,15 @@ int tsg_batch_run(tsg_batch *b) {
     while (!leases.empty()) leases.pop_back();
     return ret;
 }
+}  // namespace
+
+int tsg_batch_run(tsg_batch *b) {
+    if (b == nullptr) return TSG_ERR_BAD_PARAM_VALUE;
+    Use use(b->ts);
+    int ret = run_fused_modes(b);
+    if (ret == 0) ret = run_node_queries(b);
+    return ret;
+}
 
 // A batch run with no device or pinned allocation (tsg_batch_run_noalloc,
 // tsg_noalloc.hip): every device buffer of the run is carved from
@@ -3265,6 +3323,10 @@ __attribute__((visibility("hidden"))) int tsg_batch_run_fixed(tsg_batch *b, void
     w.fixed = (char *) scratch;
     w.fixed_cap = cap;
     w.stream = nullptr;
+    // node statistics allocate their per-node buffers: not in this form
+    for (auto &q : b->queries) {
+        if (q->mode == TSG_STAT_NODE) return TSG_ERR_UNSUPPORTED_STAT_MODE;
+    }
     int ret = 0;
     for (uint32_t mode : {TSG_STAT_BRANCH, TSG_STAT_SITE}) {
         std::vector<Query *> group;
@@ -3490,6 +3552,7 @@ int tsg_batch_run_multi(tsg_batch **batches, uint64_t n) {
     if (ret == 0) {
         for (const BatchRun &r : runs) batch_deliver(r);
     }
+    for (uint64_t i = 0; i < n && ret == 0; i++) ret = run_node_queries(batches[i]);
     return ret;
 }
 
