@@ -1159,6 +1159,40 @@ def test_device_results(require_gpu):
             lib.tsg_batch_free(bs[k])
 
 
+@pytest.mark.parametrize("same_sets", [True, False])
+def test_two_mode_batch_seed_sharing(require_gpu, same_sets):
+    """A batch of branch and site queries on the same sample sets seeds the
+    site sweep's start counts from the branch sweep's (integer counts, so
+    the site results are the same bits as a site-only batch); with other
+    sets per mode each run seeds its own."""
+    tc = cases.generated(3000, 1e6, 13)
+    ts = tc.tree_sequence()
+    sets = cases.sample_sets(3000, 6, 200, 21)
+    site_sets = sets if same_sets else cases.sample_sets(3000, 6, 200, 22)
+    w = np.linspace(0, tc.sequence_length, 6)
+    idx = [(0, 1), (2, 5)]
+
+    def queries(b, mode, ss):
+        return [b.diversity(ss, windows=w, mode=mode),
+                b.divergence(ss, indexes=idx, windows=w, mode=mode),
+                b.Tajimas_D(ss, windows=w, mode=mode),
+                b.f4(ss[:4], windows=w, mode=mode)]
+
+    b = ts.batch()
+    both_branch = queries(b, "branch", sets)
+    both_site = queries(b, "site", site_sets)
+    b.run()
+    for mode, ss, got in (("branch", sets, both_branch), ("site", site_sets, both_site)):
+        one = ts.batch()
+        alone = queries(one, mode, ss)
+        one.run()
+        _say("seed sharing", same_sets, mode)
+        for g, a in zip(got, alone):
+            assert np.array_equal(g.result(), a.result(), equal_nan=True)
+    orc = Oracle(tc)
+    _check("diversity", both_site[0].result(), orc.stat("diversity", site_sets, None, w, "site"))
+
+
 @pytest.mark.parametrize("mode", ["branch", "site"])
 def test_fused_results_bit_identical(small, mode):
     """A query's results are the same bits alone and fused with unrelated

@@ -50,6 +50,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf seed_off, seed_ps;
+    DevBuf seedbuf;                 // seeded slot counts shared with the next run
     DevBuf pair_s, pair_c, tot, outs, queries, windows, cq_wfirst, cq_nwin, cq_poff, partial, dR,
         Rstart, raw, rawbase, lanebase, out, fin, ipar, dpar, slab, last, hist, dims, twin;
     // first tree: counts and per-block running-sum partials
@@ -870,10 +871,20 @@ void fuse_queries(std::vector<Query *> &batch, Fused &f) {
     }
 }
 
+// The seeded start-tree counts of a batch's first run, for its second run
+// when both sweep the same (sample, column) pairs on the same node store.
+struct SeedShare {
+    int32_t *buf = nullptr;
+    int store = -1, K = 0;
+    std::vector<int32_t> pair_s, pair_c;
+};
+
 // Host work of a batch and its uploads on stream s: columns, outputs,
 // per-(chunk, query) window ranges, node store, sweep arguments, finals.
+// share_out: this run writes its seeded counts for a later run (share_in).
 int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, hipStream_t s,
-                  BatchRun &r) {
+                  BatchRun &r, SeedShare *share_out = nullptr,
+                  const SeedShare *share_in = nullptr) {
     int ret;
     const int Q = (int) batch.size();
     const uint32_t mode = batch[0]->mode;
@@ -1161,6 +1172,25 @@ int batch_prepare(tsg_treeseq *ts, std::vector<Query *> &batch, Workspace *ws, h
     a.npairs = (int32_t) sd.flat_s.size();
     a.pair_sample = pk.dev<const int32_t>(ws, p_fs);
     a.pair_col = pk.dev<const int32_t>(ws, p_col);
+    a.seed_out = nullptr;
+    a.seed_in = nullptr;
+    if (use_hot && !wgt && ws->fixed == nullptr) {
+        if (share_out != nullptr) {
+            const int cw = (store == TSG_STORE_HOT16) ? (K + 1) / 2 : K;
+            if (ensure(ws->seedbuf, (size_t) ts->hot.chunk_slot0[C] * cw * 4) == 0) {
+                a.seed_out = (int32_t *) ws->seedbuf.p;
+                share_out->buf = a.seed_out;
+                share_out->store = store;
+                share_out->K = K;
+                share_out->pair_s = fz.pair_s;
+                share_out->pair_c = fz.pair_c;
+            }
+        } else if (share_in != nullptr && share_in->buf != nullptr && share_in->store == store &&
+                   share_in->K == K && share_in->pair_s == fz.pair_s &&
+                   share_in->pair_c == fz.pair_c) {
+            a.seed_in = share_in->buf;
+        }
+    }
     a.nseed = (int32_t) sd.seed_sample.size();
     a.seed_sample = pk.dev<const int32_t>(ws, p_ss);
     a.seed_col = pk.dev<const int32_t>(ws, p_col);
@@ -1620,7 +1650,7 @@ int run_node(tsg_treeseq *ts, Query &q) {
 }
 
 void free_one_workspace(Workspace *ws) {
-    DevBuf *bufs[] = {&ws->seed_off, &ws->seed_ps, &ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
+    DevBuf *bufs[] = {&ws->seed_off, &ws->seed_ps, &ws->seedbuf, &ws->pair_s, &ws->pair_c, &ws->tot, &ws->outs, &ws->queries, &ws->windows,
                       &ws->cq_wfirst, &ws->cq_nwin, &ws->cq_poff, &ws->partial, &ws->dR,
                       &ws->Rstart, &ws->raw, &ws->rawbase, &ws->lanebase, &ws->out, &ws->fin, &ws->ipar,
                       &ws->dpar, &ws->slab, &ws->last, &ws->hist, &ws->dims, &ws->twin,
@@ -3217,12 +3247,18 @@ int run_two_modes(tsg_treeseq *ts, std::vector<std::vector<Query *>> &groups,
         if ((ret = device_upload(ts))) return ret;
     }
     runs.resize(groups.size());
+    SeedShare share;
     for (size_t i = 0; i < groups.size(); i++) {
         leases.push_back(std::make_unique<Lease>(ts));
         Workspace *ws = leases.back()->ws;
         if (ws == nullptr) return TSG_ERR_DEVICE;
         BatchRun &r = runs[i];
-        if ((ret = batch_prepare(ts, groups[i], ws, ws->stream, r))) return ret;
+        // the second run (it waits for the first sweep's end) takes the
+        // first run's seeded counts when its pairs are the same
+        if ((ret = batch_prepare(ts, groups[i], ws, ws->stream, r, i == 0 ? &share : nullptr,
+                                 i == 1 ? &share : nullptr))) {
+            return ret;
+        }
         if (i > 0 && (ret = hip_check(hipStreamWaitEvent(r.s, leases[i - 1]->ws->ev1, 0)))) {
             return ret;
         }

@@ -116,6 +116,13 @@ struct TsgSweepArgs {
     // per block of TSG_SS_CB chunks, per sample, the block's chunks: entry
     // ((c / CB) * num_samples + s) * CB + c % CB
     const uint16_t *sample_slot;
+    // the start-tree counts of the hot slots, shared by a batch's two runs
+    // (branch, then site) with the same (sample, column) pairs: the first
+    // writes its seeded counts (per slot its count words, at the chunk's
+    // first slot x count words), the second reads them instead of seeding;
+    // null otherwise
+    int32_t *seed_out;
+    const int32_t *seed_in;
     const int32_t *col_total;    // [K] total weight (set size) per column
     // real-valued weights (TSG_STORE_HOTW): each (sample, column) pair's
     // weight, the columns' total weights, the weighted summaries' parameters

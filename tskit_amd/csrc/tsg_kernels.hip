@@ -311,6 +311,20 @@ template <class NS>
 __device__ void init_hot(const TsgSweepArgs &a, const NS &nd, int c, int nslot, int K,
                          int lane) {
     const int64_t s0 = a.chunk_node0[c];
+    const int CW = nd.R - NS::OFF;  // count words per slot
+    if (a.seed_in != nullptr) {
+        // the counts the batch's other run seeded for the same pairs
+        if constexpr (NS::OFF != 0) {
+            for (int h = lane; h < nslot; h += WAVE) nd.set_bl(h, a.loc_bl0[s0 + h]);
+        }
+        const int32_t *src = a.seed_in + s0 * CW;
+        for (int i = lane; i < nslot * CW; i += WAVE) {
+            const int h = i / CW;
+            nd.rec[h * nd.R + NS::OFF + (i - h * CW)] = src[i];
+        }
+        wave_sync();
+        return;
+    }
     // parents of the slots this lane owns in the level pass (slot lane + 64 r),
     // read now so that their loads overlap the seeding
     constexpr int PR = 8;
@@ -379,6 +393,13 @@ __device__ void init_hot(const TsgSweepArgs &a, const NS &nd, int c, int nslot, 
             }
         }
         wave_sync();
+    }
+    if (a.seed_out != nullptr) {
+        int32_t *dst = a.seed_out + s0 * CW;
+        for (int i = lane; i < nslot * CW; i += WAVE) {
+            const int h = i / CW;
+            dst[i] = nd.rec[h * nd.R + NS::OFF + (i - h * CW)];
+        }
     }
 }
 
