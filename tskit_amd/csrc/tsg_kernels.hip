@@ -1997,6 +1997,8 @@ __global__ void __launch_bounds__(256) k_tree0_counts(const TsgSweepArgs a,
 }
 
 #define TSG_T0_NODES 64
+// column counts up to which k_tree0_sum stages its nodes in LDS
+#define TSG_T0_STAGE_K 64
 
 // grid (ceil(nloc / TSG_T0_NODES), ceil(M / 64)); lane = output.
 // (T = double: the first tree's sums of real-valued weights, x0 computed on
@@ -2004,11 +2006,29 @@ __global__ void __launch_bounds__(256) k_tree0_counts(const TsgSweepArgs a,
 template <typename T>
 __device__ __forceinline__ void tree0_sum(const TsgSweepArgs &a, const int32_t *t0_parent,
                                           const double *t0_bl, const T *x0, int nloc,
-                                          double *part, int bx, int by) {
+                                          double *part, int bx, int by, bool stage) {
     const int m = by * WAVE + threadIdx.x;
-    if (m >= a.M) return;
     const int u0 = bx * TSG_T0_NODES;
     const int u1 = min(nloc, u0 + TSG_T0_NODES);
+    // stage (the launch gives TSG_T0_NODES x (12 + K sizeof(T)) bytes of
+    // dynamic LDS) and K <= 64: the block's nodes (branch length, counts,
+    // whether they have a parent) are first read into LDS together,
+    // coalesced; the sum then reads them from there, in the same node order
+    const bool staged = stage && a.K <= TSG_T0_STAGE_K;
+    extern __shared__ int32_t smem[];
+    double *sbl = reinterpret_cast<double *>(smem);                     // [T0_NODES]
+    T *sx = reinterpret_cast<T *>(sbl + TSG_T0_NODES);                   // [T0_NODES x K]
+    int32_t *sroot = reinterpret_cast<int32_t *>(sx + TSG_T0_NODES * a.K);  // [T0_NODES]
+    if (staged) {
+        const int n = u1 - u0;
+        for (int i = threadIdx.x; i < n * a.K; i += WAVE) sx[i] = x0[(int64_t) u0 * a.K + i];
+        if ((int) threadIdx.x < n) {
+            sbl[threadIdx.x] = t0_bl[u0 + threadIdx.x];
+            sroot[threadIdx.x] = t0_parent[u0 + threadIdx.x] < 0;
+        }
+        __syncthreads();
+    }
+    if (m >= a.M) return;
     const TsgOut o = a.outs[m];
     LaneOut L;
     const bool mom = moment_form<std::is_same<T, double>::value>(o, a.K, L, a.col_total_w,
@@ -2021,15 +2041,16 @@ __device__ __forceinline__ void tree0_sum(const TsgSweepArgs &a, const int32_t *
     }
     double acc = 0;
     for (int u = u0; u < u1; u++) {
-        if (t0_parent[u] < 0) continue;
-        const T *x = x0 + (int64_t) u * a.K;
+        const int j = u - u0;
+        if (staged ? sroot[j] != 0 : t0_parent[u] < 0) continue;
+        const T *x = staged ? sx + (int64_t) j * a.K : x0 + (int64_t) u * a.K;
         double f;
         if (mom) {  // the polynomial form, as in the sweep
             f = poly_value(L, x);
         } else {
             f = branch_summary(o, x, tot, a.opar);
         }
-        acc += (double) t0_bl[u] * f;
+        acc += (staged ? sbl[j] : t0_bl[u]) * f;
     }
     part[(int64_t) bx * a.M + m] = acc;
 }
@@ -2038,7 +2059,7 @@ template <typename T>
 __global__ void __launch_bounds__(WAVE) k_tree0_sum(const TsgSweepArgs a, const int32_t *t0_parent,
                                                     const double *t0_bl, const T *x0,
                                                     int nloc, double *part) {
-    tree0_sum(a, t0_parent, t0_bl, x0, nloc, part, blockIdx.x, blockIdx.y);
+    tree0_sum(a, t0_parent, t0_bl, x0, nloc, part, blockIdx.x, blockIdx.y, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -2484,7 +2505,9 @@ hipError_t tsg_launch_tree0(const TsgSweepArgs &a, const int32_t *t0_parent, con
     }
     const int nb = tsg_tree0_blocks(nloc);
     if (nb > 0 && a.M > 0) {
-        hipLaunchKernelGGL(k_tree0_sum<int32_t>, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), 0,
+        const size_t sh = a.K <= TSG_T0_STAGE_K
+                              ? (size_t) TSG_T0_NODES * (8 + 4 + 4 * (size_t) a.K) : 0;
+        hipLaunchKernelGGL(k_tree0_sum<int32_t>, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), sh,
                            s, a, t0_parent, t0_bl, (const int32_t *) x0, nloc, part);
     }
     return hipGetLastError();
@@ -2495,7 +2518,9 @@ hipError_t tsg_launch_tree0_w(const TsgSweepArgs &a, const int32_t *t0_parent,
                               hipStream_t s) {
     const int nb = tsg_tree0_blocks(nloc);
     if (nb > 0 && a.M > 0) {
-        hipLaunchKernelGGL(k_tree0_sum<double>, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), 0,
+        const size_t sh = a.K <= TSG_T0_STAGE_K
+                              ? (size_t) TSG_T0_NODES * (8 + 4 + 8 * (size_t) a.K) : 0;
+        hipLaunchKernelGGL(k_tree0_sum<double>, dim3(nb, (a.M + WAVE - 1) / WAVE), dim3(WAVE), sh,
                            s, a, t0_parent, t0_bl, x0, nloc, part);
     }
     return hipGetLastError();
@@ -2601,7 +2626,7 @@ __global__ void __launch_bounds__(64) k_finish_multi_64(const TsgFinishRun *runs
     case 2: {  // tree0 sums: grid (nb, ceil(M / 64))
         const int gx = R.nb;
         tree0_sum(R.a, R.t0_parent, R.t0_bl, R.x0, R.nloc, R.r0part, (int) (lb % gx),
-                  (int) (lb / gx));
+                  (int) (lb / gx), false);
         break;
     }
     case 3: {
