@@ -852,7 +852,9 @@ __device__ __forceinline__ double branch_summary(const TsgOut &o, const T *x, co
 //     terms of a path segment are spread over the lanes (hop fastest),
 //     written to LDS, and each output's owner lane adds them in hop order.
 // Running sums and window sums are FP64, as in the reference.
-template <int MODE, int STORE, int MPL>
+// GC: the count-update layout's G when the launch fixes it at compile time
+// (0: chosen from K at run time).
+template <int MODE, int STORE, int MPL, int GC = 0>
 __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     extern __shared__ int32_t smem[];
 #define SWEEP_SYNC()                                     \
@@ -936,8 +938,8 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     // depend on K.  With more columns one group strides over them.  The
     // path-sum kinds (POLY, SEG) only occur with K <= 64.  The outputs'
     // column masks are in this layout (spread_columns).
-    const bool kwide = K > WAVE || K < 1;
-    const int G = kwide ? 1 : min(16, 1 << (31 - __clz(WAVE / K)));
+    const bool kwide = GC > 0 ? false : (K > WAVE || K < 1);
+    const int G = GC > 0 ? GC : (kwide ? 1 : min(16, 1 << (31 - __clz(WAVE / K))));
     const int uk = kwide ? lane : lane / G;
     const int ug = kwide ? 0 : lane - uk * G;
     const bool uon = kwide || uk < K;
@@ -1364,11 +1366,11 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
 #define TSG_SWEEP_WAVES1 3
 #endif
 
-template <int MODE, int STORE, int MPL>
+template <int MODE, int STORE, int MPL, int GC = 0>
 __global__ void __launch_bounds__(WAVE, MPL == 1 ? TSG_SWEEP_WAVES1 : 1) k_sweep(const TsgSweepArgs a) {
     const int c = xcd_chunk(blockIdx.x, a.num_chunks);
     if (c >= a.num_chunks) return;
-    sweep_chunk<MODE, STORE, MPL>(a, c);
+    sweep_chunk<MODE, STORE, MPL, GC>(a, c);
 }
 
 // Many sweeps in one launch (a batch per tree sequence): the chunks of run r
@@ -2313,9 +2315,9 @@ namespace {
 // undo each other's setting.
 int g_lds_max = 65536;
 
-template <int MODE, int STORE, int MPL>
-hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
-    const void *fn = reinterpret_cast<const void *>(&k_sweep<MODE, STORE, MPL>);
+template <int MODE, int STORE, int MPL, int GC>
+hipError_t launch_gc(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
+    const void *fn = reinterpret_cast<const void *>(&k_sweep<MODE, STORE, MPL, GC>);
     if (shmem > 65536) {
         if ((int) shmem > g_lds_max) return hipErrorInvalidValue;
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2323,8 +2325,28 @@ hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     const unsigned grid = 8u * (unsigned) ((a.num_chunks + 7) / 8);
-    hipLaunchKernelGGL((k_sweep<MODE, STORE, MPL>), dim3(grid), dim3(WAVE), shmem, s, a);
+    hipLaunchKernelGGL((k_sweep<MODE, STORE, MPL, GC>), dim3(grid), dim3(WAVE), shmem, s, a);
     return hipGetLastError();
+}
+
+// Sweeps of one output per lane on the hot plan run a build with the
+// count-update layout's G (hop groups per column: 16 for up to 4 columns, 8
+// for 5-8, 4 for 9-16, 2 for 17-32) fixed at compile time: the hop chains'
+// steps and the lane arithmetic need no run-time tests (measured on the
+// scale workload, K = 8: branch sweep 3.33 -> 2.81 ms).  Branch sweeps with
+// G = 16 keep the run-time form (the 15 unrolled chain steps spill).
+template <int MODE, int STORE, int MPL>
+hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
+    if constexpr (MPL == 1 && (STORE == TSG_STORE_HOT16 || STORE == TSG_STORE_HOT ||
+                               STORE == TSG_STORE_HOTW)) {
+        if constexpr (MODE == 1) {
+            if (a.K >= 1 && a.K <= 4) return launch_gc<MODE, STORE, MPL, 16>(a, shmem, s);
+        }
+        if (a.K >= 5 && a.K <= 8) return launch_gc<MODE, STORE, MPL, 8>(a, shmem, s);
+        if (a.K >= 9 && a.K <= 16) return launch_gc<MODE, STORE, MPL, 4>(a, shmem, s);
+        if (a.K >= 17 && a.K <= 32) return launch_gc<MODE, STORE, MPL, 2>(a, shmem, s);
+    }
+    return launch_gc<MODE, STORE, MPL, 0>(a, shmem, s);
 }
 
 template <int MODE, int STORE>
