@@ -501,7 +501,9 @@ def replicate_phase(om, with_cpu, inflight=4):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_sweep_traffic.json")
-SWEEP_KERNEL = "k_sweep<0, 3, 1>"
+# the branch sweep of the scale workload: hot slots with 16-bit counts, one
+# output per lane, 8 hop groups per column (K = 8)
+SWEEP_KERNEL = "k_sweep<0, 3, 1, 8>"
 
 
 def pmc_summary(fetch_csv, write_csv):
@@ -743,7 +745,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                 "traffic": None if args.small else measured_traffic(cfg["name"]),
-                "kernel": "k_sweep<0> (branch, fused batch)", "bytes_per_launch": alg_bytes,
+                "kernel": "k_sweep<0, 3, 1, 8> (branch, fused batch)", "bytes_per_launch": alg_bytes,
                 "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                 "site_sweep_ms": round(mode_ms["site"], 4)}
 
