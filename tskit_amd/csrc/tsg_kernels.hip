@@ -932,7 +932,8 @@ __device__ __forceinline__ void sweep_chunk(const TsgSweepArgs &a, int c) {
     }
 
     // count-update layout: with K <= 64 columns, each column has G lanes
-    // (lane = k G + g, G the largest power of two <= 16 with G K <= 64) that
+    // (lane = k G + g, G the largest power of two <= 16 with G K <= 64, or
+    // the launch's compile-time GC) that
     // update G hops of a path at once (round by round); a column's path sums
     // add the rounds' per-hop terms in hop order (hop_chain), so they do not
     // depend on K.  With more columns one group strides over them.  The
@@ -2333,8 +2334,10 @@ hipError_t launch_gc(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
 // count-update layout's G (hop groups per column: 16 for up to 4 columns, 8
 // for 5-8, 4 for 9-16, 2 for 17-32) fixed at compile time: the hop chains'
 // steps and the lane arithmetic need no run-time tests (measured on the
-// scale workload, K = 8: branch sweep 3.33 -> 2.81 ms).  Branch sweeps with
-// G = 16 keep the run-time form (the 15 unrolled chain steps spill).
+// scale workload, K = 8: branch sweep 3.33 -> 2.81 ms).  Branch sweeps of up
+// to 4 columns take G = 8 as well (any G with G K <= 64 is a valid layout,
+// and the sums do not depend on it): with G = 16 the 15 unrolled chain
+// steps spill registers.
 template <int MODE, int STORE, int MPL>
 hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
     if constexpr (MPL == 1 && (STORE == TSG_STORE_HOT16 || STORE == TSG_STORE_HOT ||
@@ -2342,7 +2345,7 @@ hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
         if constexpr (MODE == 1) {
             if (a.K >= 1 && a.K <= 4) return launch_gc<MODE, STORE, MPL, 16>(a, shmem, s);
         }
-        if (a.K >= 5 && a.K <= 8) return launch_gc<MODE, STORE, MPL, 8>(a, shmem, s);
+        if (a.K >= 1 && a.K <= 8) return launch_gc<MODE, STORE, MPL, 8>(a, shmem, s);
         if (a.K >= 9 && a.K <= 16) return launch_gc<MODE, STORE, MPL, 4>(a, shmem, s);
         if (a.K >= 17 && a.K <= 32) return launch_gc<MODE, STORE, MPL, 2>(a, shmem, s);
     }
