@@ -321,7 +321,8 @@ class LowLevelTreeSequence:
         opts = self._options(mode, span_normalise, polarised)
         sizes, sets = _parse_sample_sets(sample_set_sizes, sample_sets)
         w = _parse_windows(windows)
-        out = np.zeros(self._result_shape(len(w) - 1, opts, len(sizes)), dtype=np.float64)
+        # every element is written by the library (no need to clear)
+        out = np.empty(self._result_shape(len(w) - 1, opts, len(sizes)), dtype=np.float64)
         fn, h = self._route(fn)
         err = fn(h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
                  len(w) - 1, ptr(w, ctypes.c_double), opts, ptr(out, ctypes.c_double))
@@ -334,7 +335,7 @@ class LowLevelTreeSequence:
         sizes, sets = _parse_sample_sets(sample_set_sizes, sample_sets)
         w = _parse_windows(windows)
         idx = _parse_indexes(indexes, k)
-        out = np.zeros(self._result_shape(len(w) - 1, opts, idx.shape[0]), dtype=np.float64)
+        out = np.empty(self._result_shape(len(w) - 1, opts, idx.shape[0]), dtype=np.float64)
         fn, h = self._route(fn)
         err = fn(h, len(sizes), ptr(sizes, ctypes.c_uint64), ptr(sets, ctypes.c_int32),
                  idx.shape[0], ptr(idx, ctypes.c_int32), len(w) - 1, ptr(w, ctypes.c_double),
