@@ -435,24 +435,26 @@ int check_windows(const tsg_treeseq *ts, uint64_t nw, const double *w) {  // tre
 int parse_sample_sets(const tsg_treeseq *ts, uint64_t K, const uint64_t *sizes, const int32_t *sets,
                       std::vector<std::vector<int32_t>> &out) {
     if (K == 0) return TSG_ERR_INSUFFICIENT_SAMPLE_SETS;
+    // every sample checked (and mapped to its sample index) before any set
+    // is checked for duplicates, as the reference orders its errors
+    out.assign(K, {});
     uint64_t j = 0;
     for (uint64_t k = 0; k < K; k++) {
         if (sizes[k] == 0) return TSG_ERR_EMPTY_SAMPLE_SET;
+        out[k].resize(sizes[k]);
         for (uint64_t l = 0; l < sizes[k]; l++) {
             const int32_t u = sets[j++];
             if (u < 0 || u >= ts->num_nodes) return TSG_ERR_NODE_OUT_OF_BOUNDS;
-            if (ts->sample_index_map[u] == TSG_NULL) return TSG_ERR_BAD_SAMPLES;
+            const int32_t si = ts->sample_index_map[u];
+            if (si == TSG_NULL) return TSG_ERR_BAD_SAMPLES;
+            out[k][l] = si;
         }
     }
-    out.assign(K, {});
     // a sample twice in one set: marks in a per-sample table, cleared after
     // each set (one table per thread, reused: all zeros between calls)
     thread_local std::vector<uint8_t> seen;
     if (seen.size() < (size_t) ts->num_samples) seen.assign((size_t) ts->num_samples, 0);
-    j = 0;
     for (uint64_t k = 0; k < K; k++) {
-        out[k].reserve(sizes[k]);
-        for (uint64_t l = 0; l < sizes[k]; l++) out[k].push_back(ts->sample_index_map[sets[j++]]);
         bool dup = false;
         for (int32_t si : out[k]) {
             dup |= seen[si] != 0;
