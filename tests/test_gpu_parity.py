@@ -1159,6 +1159,30 @@ def test_device_results(require_gpu):
             lib.tsg_batch_free(bs[k])
 
 
+@pytest.mark.parametrize("K", [3, 6, 12, 24, 40])
+def test_column_layouts(small, K):
+    """Every compile-time count-update layout (hop groups per column 8, 4, 2
+    and 1 in branch mode; 16, 8, 4, 2, 1 in site mode): diversity and
+    divergence of K sets against the oracle, and a query on two of the sets
+    gives the same bits alone as fused with the others."""
+    tc, ts, orc = small
+    sets = cases.sample_sets(200, K, 4, 40 + K)
+    w = np.linspace(0, tc.sequence_length, 4)
+    idx = [(0, 1), (K - 2, K - 1)]
+    for mode in ["branch", "site"]:
+        _say("layout", K, mode)
+        _check("diversity", ts.diversity(sets, windows=w, mode=mode),
+               orc.stat("diversity", sets, None, w, mode))
+        _check("divergence", ts.divergence(sets, indexes=idx, windows=w, mode=mode),
+               orc.stat("divergence", sets, idx, w, mode))
+        alone = ts.divergence(sets[:2], windows=w, mode=mode)
+        b = ts.batch()
+        p = b.divergence(sets[:2], windows=w, mode=mode)
+        b.diversity(sets, windows=w, mode=mode)
+        b.run()
+        assert np.array_equal(p.result(), alone)
+
+
 @pytest.mark.parametrize("same_sets", [True, False])
 def test_two_mode_batch_seed_sharing(require_gpu, same_sets):
     """A batch of branch and site queries on the same sample sets seeds the

@@ -2332,7 +2332,7 @@ hipError_t launch_gc(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
 
 // Sweeps of one output per lane on the hot plan run a build with the
 // count-update layout's G (hop groups per column: 16 for up to 4 columns, 8
-// for 5-8, 4 for 9-16, 2 for 17-32) fixed at compile time: the hop chains'
+// for 5-8, 4 for 9-16, 2 for 17-32, 1 for 33-64) fixed at compile time: the hop chains'
 // steps and the lane arithmetic need no run-time tests (measured on the
 // scale workload, K = 8: branch sweep 3.33 -> 2.81 ms).  Branch sweeps of up
 // to 4 columns take G = 8 as well (any G with G K <= 64 is a valid layout,
@@ -2348,6 +2348,7 @@ hipError_t launch_one(const TsgSweepArgs &a, size_t shmem, hipStream_t s) {
         if (a.K >= 1 && a.K <= 8) return launch_gc<MODE, STORE, MPL, 8>(a, shmem, s);
         if (a.K >= 9 && a.K <= 16) return launch_gc<MODE, STORE, MPL, 4>(a, shmem, s);
         if (a.K >= 17 && a.K <= 32) return launch_gc<MODE, STORE, MPL, 2>(a, shmem, s);
+        if (a.K >= 33 && a.K <= 64) return launch_gc<MODE, STORE, MPL, 1>(a, shmem, s);
     }
     return launch_gc<MODE, STORE, MPL, 0>(a, shmem, s);
 }
